@@ -1,0 +1,405 @@
+/*
+ * bfref — CPU oracle for the D3Q19 moving-boundary step.
+ * TEST INFRASTRUCTURE ONLY (see bfref.h for the scope and the reference
+ * files each function restates).  Build without floating-point contraction:
+ * the FP32 functions are operation-for-operation restatements.
+ */
+#include "bfref.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* D3Q19 ordering: 0; +-x; +-y; +-z; (+-1,+-1,0); (+-1,0,+-1); (0,+-1,+-1).
+ * BirdFlow.metal:7-44; D3Q19.swift:8-45. */
+const int bfo_c[19][3] = {
+    {0, 0, 0},
+    {1, 0, 0}, {-1, 0, 0},
+    {0, 1, 0}, {0, -1, 0},
+    {0, 0, 1}, {0, 0, -1},
+    {1, 1, 0}, {-1, -1, 0},
+    {1, -1, 0}, {-1, 1, 0},
+    {1, 0, 1}, {-1, 0, -1},
+    {1, 0, -1}, {-1, 0, 1},
+    {0, 1, 1}, {0, -1, -1},
+    {0, 1, -1}, {0, -1, 1},
+};
+const int bfo_opp[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+const double bfo_w[19] = {
+    1.0 / 3.0,
+    1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+    1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+    1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+};
+
+#define R float
+#define FN(name) name##_f32
+#include "bfref_fluid.inc"
+#undef R
+#undef FN
+
+#define R double
+#define FN(name) name##_f64
+#include "bfref_fluid.inc"
+#undef R
+#undef FN
+
+/* InterpolatedBounceBackReference.swift:28-42 (FP64, for known-answer tests). */
+double bfo_bouzidi_population_f64(double q, double reflected, double farther,
+                                  double previous, double correction)
+{
+    if (q <= 0.5) return 2 * q * reflected + (1 - 2 * q) * farther + correction;
+    return (reflected + correction) / (2 * q) + (2 * q - 1) * previous / (2 * q);
+}
+
+/* ------------------------------------------------------------------------
+ * Geometry (FP32).  Operation order follows the CPU twin in
+ * MetalIndexedBirdSurfaceValidation.swift:18005-18110 and 20302-20359,
+ * whose output the reference reports bit-identical to its GPU kernels.
+ * ---------------------------------------------------------------------- */
+
+static float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static float len3(const float a[3]) { return sqrtf(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+/* BirdFlow.metal:1529-1583 (Voronoi-region closest point and barycentrics). */
+static void closest_point(const float p[3], const float a[3], const float b[3], const float c[3],
+                          float out[3], float bary[3])
+{
+    float ab[3], ac[3], ap[3];
+    for (int i = 0; i < 3; ++i) {
+        ab[i] = b[i] - a[i];
+        ac[i] = c[i] - a[i];
+        ap[i] = p[i] - a[i];
+    }
+    float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        memcpy(out, a, sizeof(float) * 3);
+        bary[0] = 1; bary[1] = 0; bary[2] = 0;
+        return;
+    }
+    float bp[3];
+    for (int i = 0; i < 3; ++i) bp[i] = p[i] - b[i];
+    float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) {
+        memcpy(out, b, sizeof(float) * 3);
+        bary[0] = 0; bary[1] = 1; bary[2] = 0;
+        return;
+    }
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        float v = d1 / (d1 - d3);
+        for (int i = 0; i < 3; ++i) out[i] = a[i] + v * ab[i];
+        bary[0] = 1.0f - v; bary[1] = v; bary[2] = 0;
+        return;
+    }
+    float cp[3];
+    for (int i = 0; i < 3; ++i) cp[i] = p[i] - c[i];
+    float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) {
+        memcpy(out, c, sizeof(float) * 3);
+        bary[0] = 0; bary[1] = 0; bary[2] = 1;
+        return;
+    }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        float w = d2 / (d2 - d6);
+        for (int i = 0; i < 3; ++i) out[i] = a[i] + w * ac[i];
+        bary[0] = 1.0f - w; bary[1] = 0; bary[2] = w;
+        return;
+    }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int i = 0; i < 3; ++i) out[i] = b[i] + w * (c[i] - b[i]);
+        bary[0] = 0; bary[1] = 1.0f - w; bary[2] = w;
+        return;
+    }
+    float den = va + vb + vc;
+    if (den < 1.0e-20f) den = 1.0e-20f;
+    float inv = 1.0f / den;
+    float v = vb * inv, w = vc * inv;
+    for (int i = 0; i < 3; ++i) out[i] = a[i] + ab[i] * v + ac[i] * w;
+    bary[0] = 1.0f - v - w; bary[1] = v; bary[2] = w;
+}
+
+/* MeasuredBirdSurfaceSequence.swift:174-204 */
+int bfo_frame_interval(const float* times, int frame_count, float t, float* blend, float* duration)
+{
+    if (t <= times[0]) {
+        if (blend) *blend = 0.0f;
+        if (duration) *duration = times[1] - times[0];
+        return 0;
+    }
+    int last = frame_count - 1;
+    if (t >= times[last]) {
+        if (blend) *blend = 1.0f;
+        if (duration) *duration = times[last] - times[last - 1];
+        return last - 1;
+    }
+    int lower = 0, upper = last;
+    while (lower + 1 < upper) {
+        int middle = lower + (upper - lower) / 2;
+        if (t < times[middle]) upper = middle;
+        else lower = middle;
+    }
+    float d = times[upper] - times[lower];
+    if (blend) *blend = (t - times[lower]) / d;
+    if (duration) *duration = d;
+    return lower;
+}
+
+/* BirdFlow.metal:1589-1622: position = translation + mix(P_f, P_f+1, blend),
+ * velocity = (P_f+1 - P_f) / duration * velocity_scale. */
+void bfo_prepare_vertices(const float* points, const float* times, int frame_count,
+                          int vertex_count, int first, float query_time,
+                          const float translation[3], float velocity_scale,
+                          float* position3, float* velocity3)
+{
+    if (first > frame_count - 2) first = frame_count - 2;
+    int second = first + 1;
+    float duration = times[second] - times[first];
+    float blend = (query_time - times[first]) / duration;
+    if (blend < 0.0f) blend = 0.0f;
+    if (blend > 1.0f) blend = 1.0f;
+    for (int v = 0; v < vertex_count; ++v) {
+        const float* a = points + ((size_t)first * vertex_count + v) * 3;
+        const float* b = points + ((size_t)second * vertex_count + v) * 3;
+        for (int i = 0; i < 3; ++i) {
+            float d = b[i] - a[i];
+            position3[3 * v + i] = translation[i] + (a[i] + blend * d);
+            velocity3[3 * v + i] = d / duration * velocity_scale;
+        }
+    }
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* BirdFlow.metal:1638-1686; CPU twin MetalIndexedBirdSurfaceValidation.swift:18005-18064. */
+void bfo_raster(const bfo_params* p, const float* position3, const uint32_t* tri,
+                int triangle_count, float half_thickness, int key_bits, void* keys)
+{
+    const int n[3] = {p->nx, p->ny, p->nz};
+    const float dx = p->dx;
+    for (int t = 0; t < triangle_count; ++t) {
+        if (key_bits == 32 && t >= 4096) break;
+        const float* a = position3 + 3 * (size_t)tri[3 * t + 0];
+        const float* b = position3 + 3 * (size_t)tri[3 * t + 1];
+        const float* c = position3 + 3 * (size_t)tri[3 * t + 2];
+        float guard = half_thickness + 2.0f * dx;
+        int lo[3], hi[3];
+        for (int i = 0; i < 3; ++i) {
+            float mn = fminf(a[i], fminf(b[i], c[i]));
+            float mx = fmaxf(a[i], fmaxf(b[i], c[i]));
+            float lw = mn - guard, uw = mx + guard;
+            float ls = (lw - p->origin[i]) / dx - 0.5f;
+            float us = (uw - p->origin[i]) / dx - 0.5f;
+            lo[i] = clampi((int)floorf(ls), 0, n[i] - 1);
+            hi[i] = clampi((int)ceilf(us), 0, n[i] - 1);
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    float w[3] = {
+                        p->origin[0] + ((float)x + 0.5f) * dx,
+                        p->origin[1] + ((float)y + 0.5f) * dx,
+                        p->origin[2] + ((float)z + 0.5f) * dx,
+                    };
+                    float cp[3], bary[3];
+                    closest_point(w, a, b, c, cp, bary);
+                    float d[3] = {w[0] - cp[0], w[1] - cp[1], w[2] - cp[2]};
+                    float dist_cells = len3(d) / dx;
+                    float scaled = roundf(dist_cells * 65536.0f);
+                    uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
+                    size_t gid = (size_t)x + (size_t)p->nx * ((size_t)y + (size_t)p->ny * (size_t)z);
+                    if (key_bits == 32) {
+                        uint32_t key = (bin << 12) | (uint32_t)t;
+                        uint32_t* k = (uint32_t*)keys;
+                        if (key < k[gid]) k[gid] = key;
+                    } else {
+                        uint64_t key = ((uint64_t)bin << 32) | (uint64_t)(uint32_t)t;
+                        uint64_t* k = (uint64_t*)keys;
+                        if (key < k[gid]) k[gid] = key;
+                    }
+                }
+    }
+}
+
+void bfo_clear_geometry(const bfo_params* p, uint8_t* mask, float* wall4)
+{
+    const size_t N = (size_t)p->nx * p->ny * p->nz;
+    memset(mask, 0, N);
+    for (size_t i = 0; i < N; ++i) {
+        wall4[4 * i + 0] = 0.0f;
+        wall4[4 * i + 1] = 0.0f;
+        wall4[4 * i + 2] = 0.0f;
+        wall4[4 * i + 3] = 16.0f;
+    }
+}
+
+/* BirdFlow.metal:1697-1738; CPU twin MetalIndexedBirdSurfaceValidation.swift:18066-18106. */
+void bfo_resolve(const bfo_params* p, const float* position3, const float* velocity3,
+                 const uint32_t* tri, const uint8_t* tri_part, int triangle_count,
+                 float half_thickness, int key_bits, const void* keys,
+                 uint8_t* mask, float* wall4)
+{
+    const size_t N = (size_t)p->nx * p->ny * p->nz;
+    const float dx = p->dx;
+    for (size_t gid = 0; gid < N; ++gid) {
+        uint32_t t;
+        if (key_bits == 32) {
+            uint32_t key = ((const uint32_t*)keys)[gid];
+            if (key == UINT32_MAX) continue;
+            t = key & 0xFFFu;
+        } else {
+            uint64_t key = ((const uint64_t*)keys)[gid];
+            if (key == UINT64_MAX) continue;
+            t = (uint32_t)(key & 0xFFFFFFFFu);
+        }
+        if ((int)t >= triangle_count) continue;
+        int x = (int)(gid % (size_t)p->nx);
+        size_t yz = gid / (size_t)p->nx;
+        int y = (int)(yz % (size_t)p->ny);
+        int z = (int)(yz / (size_t)p->ny);
+        float w[3] = {
+            p->origin[0] + ((float)x + 0.5f) * dx,
+            p->origin[1] + ((float)y + 0.5f) * dx,
+            p->origin[2] + ((float)z + 0.5f) * dx,
+        };
+        uint32_t ia = tri[3 * t + 0], ib = tri[3 * t + 1], ic = tri[3 * t + 2];
+        float cp[3], bary[3];
+        closest_point(w, position3 + 3 * ia, position3 + 3 * ib, position3 + 3 * ic, cp, bary);
+        float vel[3];
+        for (int i = 0; i < 3; ++i)
+            vel[i] = bary[0] * velocity3[3 * ia + i] + bary[1] * velocity3[3 * ib + i]
+                + bary[2] * velocity3[3 * ic + i];
+        float d[3] = {w[0] - cp[0], w[1] - cp[1], w[2] - cp[2]};
+        float sd = len3(d) - half_thickness;
+        mask[gid] = sd <= 0.0f ? tri_part[t] : 0;
+        wall4[4 * gid + 0] = vel[0];
+        wall4[4 * gid + 1] = vel[1];
+        wall4[4 * gid + 2] = vel[2];
+        wall4[4 * gid + 3] = sd / dx;
+    }
+}
+
+/* BirdFlow.metal:183-195 and 2307-2350 (occupancy and wall velocity only). */
+void bfo_translating_sphere(const bfo_params* p, const float center_radius[4],
+                            const float geometry_velocity[3], const float wall_velocity[4],
+                            float t, uint8_t* mask, float* wall4)
+{
+    float center[3];
+    for (int i = 0; i < 3; ++i) center[i] = center_radius[i] + geometry_velocity[i] * t;
+    float r = center_radius[3];
+    int mode = (int)((wall_velocity[3] > 0.0f ? wall_velocity[3] : 0.0f) + 0.5f);
+    for (int z = 0; z < p->nz; ++z)
+        for (int y = 0; y < p->ny; ++y)
+            for (int x = 0; x < p->nx; ++x) {
+                size_t gid = (size_t)x + (size_t)p->nx * ((size_t)y + (size_t)p->ny * (size_t)z);
+                float rel[3] = {(float)x + 0.5f - center[0], (float)y + 0.5f - center[1],
+                                (float)z + 0.5f - center[2]};
+                int inside = dot3(rel, rel) <= r * r;
+                float wv[3] = {wall_velocity[0], wall_velocity[1], wall_velocity[2]};
+                if (mode != 0) {
+                    float l = len3(rel);
+                    if (l < 1.0e-6f) l = 1.0e-6f;
+                    float nrm[3] = {rel[0] / l, rel[1] / l, rel[2] / l};
+                    float dn = dot3(wv, nrm);
+                    float nw[3] = {dn * nrm[0], dn * nrm[1], dn * nrm[2]};
+                    for (int i = 0; i < 3; ++i) wv[i] = mode == 1 ? wv[i] - nw[i] : nw[i];
+                }
+                mask[gid] = inside ? 1 : 0;
+                wall4[4 * gid + 0] = wv[0];
+                wall4[4 * gid + 1] = wv[1];
+                wall4[4 * gid + 2] = wv[2];
+                wall4[4 * gid + 3] = 0.0f;
+            }
+}
+
+void bfo_link_census(const bfo_params* p, const uint8_t* solid_prev, const uint8_t* solid_cur,
+                     uint64_t link_counts[19], uint64_t* covered, uint64_t* uncovered)
+{
+    for (int q = 0; q < 19; ++q) link_counts[q] = 0;
+    uint64_t cov = 0, unc = 0;
+    for (int z = 0; z < p->nz; ++z)
+        for (int y = 0; y < p->ny; ++y)
+            for (int x = 0; x < p->nx; ++x) {
+                size_t gid = (size_t)x + (size_t)p->nx * ((size_t)y + (size_t)p->ny * (size_t)z);
+                int s = solid_cur[gid] != 0, w = solid_prev[gid] != 0;
+                if (s && !w) ++cov;
+                if (!s && w) ++unc;
+                if (s || w) continue;
+                for (int q = 1; q < 19; ++q) {
+                    size_t src;
+                    if (pull_source_f32(p, x, y, z, q, &src)) continue;
+                    if (solid_cur[src] != 0) ++link_counts[q];
+                }
+            }
+    *covered = cov;
+    *uncovered = unc;
+}
+
+/* ------------------------------------------------------------------------
+ * Rigid body (FP64), BirdFlow.metal:6574-6627 / RigidBody.swift:85-127.
+ * ---------------------------------------------------------------------- */
+static void qnormalize(double q[4])
+{
+    double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (n2 > 1.0e-20) {
+        double s = 1.0 / sqrt(n2);
+        for (int i = 0; i < 4; ++i) q[i] *= s;
+    } else {
+        q[0] = q[1] = q[2] = 0.0;
+        q[3] = 1.0;
+    }
+}
+
+static void cross3d(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static void qrotate(const double q[4], const double v[3], double o[3])
+{
+    double t[3], c[3];
+    cross3d(q, v, t);
+    for (int i = 0; i < 3; ++i) t[i] *= 2.0;
+    cross3d(q, t, c);
+    for (int i = 0; i < 3; ++i) o[i] = v[i] + q[3] * t[i] + c[i];
+}
+
+void bfo_body_integrate(bfo_body* b, const double force[3], const double torque_world[3],
+                        double mass, const double inertia[3], const double gravity[3],
+                        double dt_total, int substeps)
+{
+    if (substeps < 1) substeps = 1;
+    double dt = dt_total / (double)substeps;
+    double q[4];
+    memcpy(q, b->orientation, sizeof q);
+    qnormalize(q);
+    for (int s = 0; s < substeps; ++s) {
+        for (int i = 0; i < 3; ++i) {
+            b->linear_velocity[i] += (force[i] / mass + gravity[i]) * dt;
+            b->position[i] += b->linear_velocity[i] * dt;
+        }
+        double qc[4] = {-q[0], -q[1], -q[2], q[3]};
+        double tb[3];
+        qrotate(qc, torque_world, tb);
+        double* w = b->angular_velocity_body;
+        double L[3] = {inertia[0] * w[0], inertia[1] * w[1], inertia[2] * w[2]};
+        double g[3];
+        cross3d(w, L, g);
+        for (int i = 0; i < 3; ++i) w[i] += (tb[i] - g[i]) / inertia[i] * dt;
+        /* derivative = q (x) (omega, 0) */
+        double d[4];
+        double cq[3];
+        cross3d(q, w, cq);
+        for (int i = 0; i < 3; ++i) d[i] = q[3] * w[i] + cq[i];
+        d[3] = -(q[0] * w[0] + q[1] * w[1] + q[2] * w[2]);
+        for (int i = 0; i < 4; ++i) q[i] = q[i] + 0.5 * dt * d[i];
+        qnormalize(q);
+    }
+    memcpy(b->orientation, q, sizeof q);
+}

@@ -1,0 +1,171 @@
+/*
+ * bfref — CPU oracle for the D3Q19 moving-boundary step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in numi-gpu-physics_amd/csrc.  Only tests/, the smoke entry
+ * point and the CPU-baseline leg of bench.py may load it; the product path
+ * never calls it and has no CPU fallback.
+ *
+ * It restates, in plain C, the arithmetic of the reference solver:
+ *   - D3Q19 tables and second-order equilibrium
+ *       Sources/BirdFlowCore/D3Q19.swift:3-58, Sources/BirdFlowMetal/Metal/BirdFlow.metal:4-44,478-484
+ *   - the fused pull-stream / boundary / collide / load step
+ *       Sources/BirdFlowMetal/Metal/BirdFlow.metal:2507-3236 (stepFluidTRT)
+ *   - sponge factor  BirdFlow.metal:2427-2446
+ *   - interpolated bounce-back and cover/uncover impulses
+ *       Sources/BirdFlowMetal/InterpolatedBounceBackReference.swift:12-61
+ *   - link fraction from signed distances  BirdFlow.metal:2044-2076
+ *   - indexed-mesh vertex preparation, raster, resolve
+ *       BirdFlow.metal:1529-1583 (closest point), 1589-1622 (prepare),
+ *       1638-1686 (raster), 1697-1738 (resolve), 1898-1911 (clear);
+ *       CPU twin Sources/BirdFlowMetal/MetalIndexedBirdSurfaceValidation.swift:18005-18110,20302-20359
+ *   - frame interval search  Sources/BirdFlowMetal/MeasuredBirdSurfaceSequence.swift:174-204
+ *   - shear-wave initial state  BirdFlow.metal:2132-2162 and
+ *       Sources/BirdFlowMetal/MetalShearWaveValidation.swift:666-755
+ *   - translating-sphere topology canonical  BirdFlow.metal:183-195,2263-2350
+ *   - six-degree-of-freedom body update  BirdFlow.metal:6574-6627,
+ *       Sources/BirdFlowCore/RigidBody.swift:85-127
+ *
+ * Fluid functions exist in two precisions: *_f32 (same operation order as the
+ * HIP kernels, built without floating-point contraction) and *_f64 (the FP64
+ * oracle used to state the error of the FP32 and reduced-storage forms).
+ * Geometry is FP32 only: its integer flags are defined by FP32 arithmetic.
+ *
+ * Population layout: SoA, direction-major, f[q * N + x + nx * (y + ny * z)].
+ */
+#ifndef BFREF_H
+#define BFREF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Boundary treatment for a domain face (per axis, low and high side). */
+enum {
+    BFO_BC_PERIODIC = 0,   /* wrap to the opposite face */
+    BFO_BC_FARFIELD = 1,   /* equilibrium at (far_rho, far_u) */
+    BFO_BC_OUTLET = 2      /* zero-gradient: source clamped into the domain */
+};
+
+enum {
+    BFO_COLLISION_TRT = 0,
+    BFO_COLLISION_TRT_SYMMETRIC_LIMITER = 1,
+    BFO_COLLISION_REGULARIZED = 2,
+    BFO_COLLISION_RR3 = 3
+};
+
+enum {
+    BFO_WALL_HALFWAY = 0,            /* halfway bounce-back, wall velocity of the solid source */
+    BFO_WALL_INTERPOLATED_SDF = 1,   /* Bouzidi with q from the signed-distance ratio */
+    BFO_WALL_INTERPOLATED_HALF = 2   /* Bouzidi path with q fixed at 1/2 */
+};
+
+typedef struct bfo_params {
+    int32_t nx, ny, nz;
+    int32_t bc[3][2];              /* [axis][0=low,1=high] */
+    int32_t collision;
+    int32_t wall_rule;
+    int32_t force_mode;            /* 0..6 momentum-exchange estimator */
+    int32_t load_component;        /* 0 all, 1 link exchange only, 2 topology impulse only */
+    int32_t selected_part;         /* 0 = every part */
+    int32_t accumulate_loads;
+    int32_t capture_fields;
+    int32_t pad0;
+    float omega_plus, omega_minus;
+    float sponge_strength, sponge_width;
+    float far_u[3], far_rho;
+    float origin[3], dx;
+    float force_scale;             /* lattice -> physical force */
+    float body_position[3];
+} bfo_params;
+
+/* ---- tables ---- */
+extern const int bfo_c[19][3];
+extern const int bfo_opp[19];
+extern const double bfo_w[19];
+
+/* ---- fluid ---- */
+float bfo_equilibrium_f32(int q, float rho, float ux, float uy, float uz);
+double bfo_equilibrium_f64(int q, double rho, double ux, double uy, double uz);
+
+/* One fused step fin -> fout.  loads (may be NULL) receives
+ * {Fx, Fy, Fz, Tx, Ty, Tz, limiter activations, max limiter restriction},
+ * accumulated in FP64 in ascending cell order.  density/velocity4 are written
+ * only when capture_fields != 0 (velocity4 holds ux,uy,uz,0 per cell). */
+void bfo_step_f32(const bfo_params* p, const float* fin, float* fout,
+                  const uint8_t* solid_prev, const uint8_t* solid_cur,
+                  const float* wall4, float* density, float* velocity4,
+                  double* loads);
+void bfo_step_f64(const bfo_params* p, const double* fin, double* fout,
+                  const uint8_t* solid_prev, const uint8_t* solid_cur,
+                  const float* wall4, double* density, double* velocity4,
+                  double* loads);
+
+/* Equilibrium fill: solid cells at their wall velocity, fluid at far_u. */
+void bfo_init_equilibrium_f32(const bfo_params* p, float* f, const uint8_t* solid, const float* wall4);
+void bfo_init_equilibrium_f64(const bfo_params* p, double* f, const uint8_t* solid, const float* wall4);
+/* Shear wave u_x = amplitude * sin(2 pi y / ny), rho = 1. */
+void bfo_init_shear_wave_f32(const bfo_params* p, float amplitude, float* f);
+void bfo_init_shear_wave_f64(const bfo_params* p, double amplitude, double* f);
+/* Macroscopic fields. */
+void bfo_moments_f32(const bfo_params* p, const float* f, float* density, float* velocity3);
+void bfo_moments_f64(const bfo_params* p, const double* f, double* density, double* velocity3);
+
+/* Link fraction from implicit (signed-distance) values, BirdFlow.metal:2065-2071. */
+float bfo_link_fraction_f32(float fluid_distance, float solid_distance);
+double bfo_link_fraction_f64(double fluid_distance, double solid_distance);
+/* Bouzidi two-branch population, InterpolatedBounceBackReference.swift:28-42. */
+double bfo_bouzidi_population_f64(double q, double reflected, double farther,
+                                  double previous, double correction);
+
+/* ---- geometry (FP32) ---- */
+/* Frame interval for a query time: returns first frame index and blend. */
+int bfo_frame_interval(const float* times, int frame_count, float t, float* blend, float* duration);
+/* Per-vertex prepare from frame-major xyz points. */
+void bfo_prepare_vertices(const float* points, const float* times, int frame_count,
+                          int vertex_count, int first, float query_time,
+                          const float translation[3], float velocity_scale,
+                          float* position3, float* velocity3);
+/* Raster: keys must be pre-filled with all-ones.  key_bits = 32 reproduces the
+ * reference (20-bit bin << 12 | 12-bit triangle, triangles >= 4096 skipped);
+ * key_bits = 64 is the widened key (bin << 32 | triangle). */
+void bfo_raster(const bfo_params* p, const float* position3, const uint32_t* tri,
+                int triangle_count, float half_thickness, int key_bits, void* keys);
+/* Resolve keys into part mask and wall (ux,uy,uz,signed distance/dx). Cells
+ * whose key is empty are left untouched: callers pre-clear mask=0 and
+ * wall=(0,0,0,16). */
+void bfo_resolve(const bfo_params* p, const float* position3, const float* velocity3,
+                 const uint32_t* tri, const uint8_t* tri_part, int triangle_count,
+                 float half_thickness, int key_bits, const void* keys,
+                 uint8_t* mask, float* wall4);
+void bfo_clear_geometry(const bfo_params* p, uint8_t* mask, float* wall4);
+
+/* Translating sphere geometry at time t (geometry center = c0 + v_geom * t). */
+void bfo_translating_sphere(const bfo_params* p, const float center_radius[4],
+                            const float geometry_velocity[3], const float wall_velocity[4],
+                            float t, uint8_t* mask, float* wall4);
+
+/* Link census: per-direction count of (fluid cell, q) whose pull source is
+ * solid in solid_cur, and the fresh cover / uncover counts. */
+void bfo_link_census(const bfo_params* p, const uint8_t* solid_prev, const uint8_t* solid_cur,
+                     uint64_t link_counts[19], uint64_t* covered, uint64_t* uncovered);
+
+/* ---- rigid body (FP64) ---- */
+typedef struct bfo_body {
+    double position[3];
+    double orientation[4];        /* x, y, z, w */
+    double linear_velocity[3];
+    double angular_velocity_body[3];
+} bfo_body;
+
+void bfo_body_integrate(bfo_body* b, const double force[3], const double torque_world[3],
+                        double mass, const double inertia[3], const double gravity[3],
+                        double dt, int substeps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

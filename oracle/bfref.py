@@ -1,0 +1,278 @@
+"""ctypes binding of the CPU oracle (oracle/libbfref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the CPU-baseline leg of ``bench.py``, always as the checker or the
+timed CPU baseline, never as the product path.
+
+The functions restate the reference solver's arithmetic; bfref.h lists the
+reference file:line each one follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libbfref.so")
+
+BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
+COLLISION_TRT, COLLISION_TRT_SYMMETRIC_LIMITER, COLLISION_REGULARIZED, COLLISION_RR3 = 0, 1, 2, 3
+WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF = 0, 1, 2
+
+C = np.array(
+    [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+     (1, 1, 0), (-1, -1, 0), (1, -1, 0), (-1, 1, 0), (1, 0, 1), (-1, 0, -1),
+     (1, 0, -1), (-1, 0, 1), (0, 1, 1), (0, -1, -1), (0, 1, -1), (0, -1, 1)], dtype=np.int64)
+OPP = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17])
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("bc", (ctypes.c_int32 * 2) * 3),
+        ("collision", ctypes.c_int32),
+        ("wall_rule", ctypes.c_int32),
+        ("force_mode", ctypes.c_int32),
+        ("load_component", ctypes.c_int32),
+        ("selected_part", ctypes.c_int32),
+        ("accumulate_loads", ctypes.c_int32),
+        ("capture_fields", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("omega_plus", ctypes.c_float), ("omega_minus", ctypes.c_float),
+        ("sponge_strength", ctypes.c_float), ("sponge_width", ctypes.c_float),
+        ("far_u", ctypes.c_float * 3), ("far_rho", ctypes.c_float),
+        ("origin", ctypes.c_float * 3), ("dx", ctypes.c_float),
+        ("force_scale", ctypes.c_float),
+        ("body_position", ctypes.c_float * 3),
+    ]
+
+
+class Body(ctypes.Structure):
+    _fields_ = [
+        ("position", ctypes.c_double * 3),
+        ("orientation", ctypes.c_double * 4),
+        ("linear_velocity", ctypes.c_double * 3),
+        ("angular_velocity_body", ctypes.c_double * 3),
+    ]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (used where a toolchain exists)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _declare(l):
+    vp, P = ctypes.c_void_p, ctypes.POINTER(Params)
+    for sfx in ("f32", "f64"):
+        getattr(l, f"bfo_step_{sfx}").argtypes = [P, vp, vp, vp, vp, vp, vp, vp, vp]
+        getattr(l, f"bfo_init_equilibrium_{sfx}").argtypes = [P, vp, vp, vp]
+        getattr(l, f"bfo_moments_{sfx}").argtypes = [P, vp, vp, vp]
+    l.bfo_init_shear_wave_f32.argtypes = [P, ctypes.c_float, vp]
+    l.bfo_init_shear_wave_f64.argtypes = [P, ctypes.c_double, vp]
+    l.bfo_equilibrium_f32.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    l.bfo_equilibrium_f32.restype = ctypes.c_float
+    l.bfo_equilibrium_f64.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4
+    l.bfo_equilibrium_f64.restype = ctypes.c_double
+    l.bfo_link_fraction_f64.argtypes = [ctypes.c_double, ctypes.c_double]
+    l.bfo_link_fraction_f64.restype = ctypes.c_double
+    l.bfo_link_fraction_f32.argtypes = [ctypes.c_float, ctypes.c_float]
+    l.bfo_link_fraction_f32.restype = ctypes.c_float
+    l.bfo_bouzidi_population_f64.argtypes = [ctypes.c_double] * 5
+    l.bfo_bouzidi_population_f64.restype = ctypes.c_double
+    l.bfo_frame_interval.argtypes = [vp, ctypes.c_int, ctypes.c_float, vp, vp]
+    l.bfo_frame_interval.restype = ctypes.c_int
+    l.bfo_prepare_vertices.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                       vp, ctypes.c_float, vp, vp]
+    l.bfo_raster.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, vp]
+    l.bfo_resolve.argtypes = [P, vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, vp, vp, vp]
+    l.bfo_clear_geometry.argtypes = [P, vp, vp]
+    l.bfo_translating_sphere.argtypes = [P, vp, vp, vp, ctypes.c_float, vp, vp]
+    l.bfo_link_census.argtypes = [P, vp, vp, vp, vp, vp]
+    l.bfo_body_integrate.argtypes = [ctypes.POINTER(Body), vp, vp, ctypes.c_double, vp, vp,
+                                     ctypes.c_double, ctypes.c_int]
+
+
+def make_params(shape, *, omega_plus, omega_minus, bc=None, collision=COLLISION_TRT,
+                wall_rule=WALL_HALFWAY, force_mode=6, load_component=0, selected_part=0,
+                accumulate_loads=False, capture_fields=False, sponge_strength=0.0,
+                sponge_width=4.0, far_u=(0.0, 0.0, 0.0), far_rho=1.0, origin=(0.0, 0.0, 0.0),
+                dx=1.0, force_scale=1.0, body_position=(0.0, 0.0, 0.0)) -> Params:
+    p = Params()
+    p.nx, p.ny, p.nz = (int(v) for v in shape)
+    if bc is None:
+        bc = ((BC_PERIODIC, BC_PERIODIC),) * 3
+    for a in range(3):
+        for s in range(2):
+            p.bc[a][s] = int(bc[a][s])
+    p.collision, p.wall_rule, p.force_mode = collision, wall_rule, force_mode
+    p.load_component, p.selected_part = load_component, selected_part
+    p.accumulate_loads, p.capture_fields = int(bool(accumulate_loads)), int(bool(capture_fields))
+    p.omega_plus, p.omega_minus = omega_plus, omega_minus
+    p.sponge_strength, p.sponge_width = sponge_strength, sponge_width
+    p.far_u[:] = list(far_u)
+    p.far_rho = far_rho
+    p.origin[:] = list(origin)
+    p.dx = dx
+    p.force_scale = force_scale
+    p.body_position[:] = list(body_position)
+    return p
+
+
+def cells(p: Params) -> int:
+    return p.nx * p.ny * p.nz
+
+
+def _dtype_sfx(dtype):
+    return "f32" if np.dtype(dtype) == np.float32 else "f64"
+
+
+def step(p: Params, fin, solid_prev=None, solid_cur=None, wall4=None, capture=False):
+    """One fused step; returns (fout, loads[8], density, velocity4)."""
+    n = cells(p)
+    sfx = _dtype_sfx(fin.dtype)
+    fin = np.ascontiguousarray(fin)
+    fout = np.empty_like(fin)
+    zeros = np.zeros(n, np.uint8)
+    solid_prev = zeros if solid_prev is None else np.ascontiguousarray(solid_prev, np.uint8)
+    solid_cur = zeros if solid_cur is None else np.ascontiguousarray(solid_cur, np.uint8)
+    if wall4 is None:
+        wall4 = np.zeros((n, 4), np.float32)
+        wall4[:, 3] = 16.0
+    wall4 = np.ascontiguousarray(wall4, np.float32)
+    loads = np.zeros(8, np.float64)
+    density = velocity4 = None
+    p.capture_fields = int(bool(capture))
+    if capture:
+        density = np.empty(n, fin.dtype)
+        velocity4 = np.empty((n, 4), fin.dtype)
+    getattr(lib(), f"bfo_step_{sfx}")(ctypes.byref(p), _ptr(fin), _ptr(fout), _ptr(solid_prev),
+                                      _ptr(solid_cur), _ptr(wall4), _ptr(density), _ptr(velocity4),
+                                      _ptr(loads))
+    return fout, loads, density, velocity4
+
+
+def init_shear_wave(p: Params, amplitude: float, dtype=np.float32):
+    f = np.empty(19 * cells(p), dtype)
+    getattr(lib(), f"bfo_init_shear_wave_{_dtype_sfx(dtype)}")(ctypes.byref(p), amplitude, _ptr(f))
+    return f
+
+
+def init_equilibrium(p: Params, solid=None, wall4=None, dtype=np.float32):
+    f = np.empty(19 * cells(p), dtype)
+    solid = None if solid is None else np.ascontiguousarray(solid, np.uint8)
+    wall4 = None if wall4 is None else np.ascontiguousarray(wall4, np.float32)
+    getattr(lib(), f"bfo_init_equilibrium_{_dtype_sfx(dtype)}")(ctypes.byref(p), _ptr(f), _ptr(solid), _ptr(wall4))
+    return f
+
+
+def moments(p: Params, f):
+    n = cells(p)
+    rho = np.empty(n, f.dtype)
+    u = np.empty((n, 3), f.dtype)
+    getattr(lib(), f"bfo_moments_{_dtype_sfx(f.dtype)}")(ctypes.byref(p), _ptr(np.ascontiguousarray(f)), _ptr(rho), _ptr(u))
+    return rho, u
+
+
+def equilibrium_f32(q, rho, u):
+    return lib().bfo_equilibrium_f32(q, rho, *u)
+
+
+def equilibrium_f64(q, rho, u):
+    return lib().bfo_equilibrium_f64(q, rho, *u)
+
+
+def link_fraction_f64(fluid, solid):
+    return lib().bfo_link_fraction_f64(fluid, solid)
+
+
+def bouzidi_population_f64(q, reflected, farther, previous, correction):
+    return lib().bfo_bouzidi_population_f64(q, reflected, farther, previous, correction)
+
+
+def frame_interval(times, t):
+    times = np.ascontiguousarray(times, np.float32)
+    blend = np.zeros(1, np.float32)
+    dur = np.zeros(1, np.float32)
+    first = lib().bfo_frame_interval(_ptr(times), len(times), np.float32(t), _ptr(blend), _ptr(dur))
+    return first, float(blend[0]), float(dur[0])
+
+
+def prepare_vertices(points, times, first, query_time, translation=(0, 0, 0), velocity_scale=1.0):
+    points = np.ascontiguousarray(points, np.float32)
+    times = np.ascontiguousarray(times, np.float32)
+    frames, verts = points.shape[0], points.shape[1]
+    pos = np.empty((verts, 3), np.float32)
+    vel = np.empty((verts, 3), np.float32)
+    tr = np.ascontiguousarray(translation, np.float32)
+    lib().bfo_prepare_vertices(_ptr(points), _ptr(times), frames, verts, int(first), np.float32(query_time),
+                               _ptr(tr), np.float32(velocity_scale), _ptr(pos), _ptr(vel))
+    return pos, vel
+
+
+def raster_resolve(p: Params, pos, vel, tri, tri_part, half_thickness, key_bits=64):
+    """Clear + raster + resolve; returns (mask, wall4, keys)."""
+    n = cells(p)
+    tri = np.ascontiguousarray(tri, np.uint32)
+    tri_part = np.ascontiguousarray(tri_part, np.uint8)
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    keys = np.full(n, np.iinfo(np.uint32 if key_bits == 32 else np.uint64).max,
+                   np.uint32 if key_bits == 32 else np.uint64)
+    mask = np.zeros(n, np.uint8)
+    wall4 = np.zeros((n, 4), np.float32)
+    lib().bfo_clear_geometry(ctypes.byref(p), _ptr(mask), _ptr(wall4))
+    lib().bfo_raster(ctypes.byref(p), _ptr(pos), _ptr(tri), len(tri), np.float32(half_thickness), key_bits, _ptr(keys))
+    lib().bfo_resolve(ctypes.byref(p), _ptr(pos), _ptr(vel), _ptr(tri), _ptr(tri_part), len(tri),
+                      np.float32(half_thickness), key_bits, _ptr(keys), _ptr(mask), _ptr(wall4))
+    return mask, wall4, keys
+
+
+def translating_sphere(p: Params, center_radius, geometry_velocity, wall_velocity, t):
+    n = cells(p)
+    mask = np.zeros(n, np.uint8)
+    wall4 = np.zeros((n, 4), np.float32)
+    cr = np.ascontiguousarray(center_radius, np.float32)
+    gv = np.ascontiguousarray(geometry_velocity, np.float32)
+    wv = np.ascontiguousarray(wall_velocity, np.float32)
+    lib().bfo_translating_sphere(ctypes.byref(p), _ptr(cr), _ptr(gv), _ptr(wv), np.float32(t), _ptr(mask), _ptr(wall4))
+    return mask, wall4
+
+
+def link_census(p: Params, solid_prev, solid_cur):
+    counts = np.zeros(19, np.uint64)
+    cov = np.zeros(1, np.uint64)
+    unc = np.zeros(1, np.uint64)
+    lib().bfo_link_census(ctypes.byref(p), _ptr(np.ascontiguousarray(solid_prev, np.uint8)),
+                          _ptr(np.ascontiguousarray(solid_cur, np.uint8)), _ptr(counts), _ptr(cov), _ptr(unc))
+    return counts, int(cov[0]), int(unc[0])
+
+
+def body_integrate(body: Body, force, torque, mass, inertia, gravity, dt, substeps=1):
+    f = np.ascontiguousarray(force, np.float64)
+    t = np.ascontiguousarray(torque, np.float64)
+    i = np.ascontiguousarray(inertia, np.float64)
+    g = np.ascontiguousarray(gravity, np.float64)
+    lib().bfo_body_integrate(ctypes.byref(body), _ptr(f), _ptr(t), float(mass), _ptr(i), _ptr(g), float(dt), int(substeps))
+    return body

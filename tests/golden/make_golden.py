@@ -1,0 +1,125 @@
+#!/usr/bin/env python3
+"""Writes the committed golden fixtures under tests/golden/.
+
+Run in the build container only (it reads the read-only reference checkout at
+/root/reference); the fixtures it writes are data and travel with the tests.
+
+Sources:
+  * Reference/shear_wave_reference.py — the reference's FP64 NumPy D3Q19 TRT
+    shear-wave program.  Its equilibrium/stream/moments/collide_trt functions
+    produce a full population trajectory at 16^3; run_case produces the
+    64^3, 1,000-step decay summary at tau+ = 0.8, magic parameter 1/4.
+  * ValidationInputs/deetjen-ob-f03-surface-v1/ — the reference's indexed
+    dove mesh (a frame subset is kept; origin and grid are derived from all
+    144 frames here).
+  * ValidationArtifacts/deetjen-dove-indexed-metal-geometry.json — per-frame
+    occupancy SHA-256 and per-part solid-cell counts.
+  * ValidationArtifacts/translating-body-topology-summary.json — cover /
+    uncover events and mean conservative force of the translating sphere.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+REF = "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def shear_wave():
+    sys.path.insert(0, os.path.join(REF, "Reference"))
+    import shear_wave_reference as swr  # noqa: E402  (reference FP64 NumPy program)
+
+    n, steps, amplitude = 16, 8, 0.01
+    viscosity = 0.1
+    magic = 0.25
+    tau_plus = 0.5 + 3.0 * viscosity
+    tau_minus = 0.5 + magic / (tau_plus - 0.5)
+    omega_plus, omega_minus = 1.0 / tau_plus, 1.0 / tau_minus
+    rho = np.ones((n, n, n))
+    velocity = np.zeros((3, n, n, n))
+    y = np.arange(n, dtype=np.float64)
+    velocity[0] = amplitude * np.sin(2.0 * np.pi * y / n)[None, :, None]
+    f = swr.equilibrium(rho, velocity)
+    f0 = f.copy()
+    history = []
+    for _ in range(steps):
+        streamed = swr.stream(f)
+        rho, velocity = swr.moments(streamed)
+        f = swr.collide_trt(streamed, rho, velocity, omega_plus, omega_minus)
+        history.append(f.copy())
+    # Reference arrays are indexed [q, x, y, z]; store as direction-major
+    # SoA with x fastest: f[q, z, y, x].
+    to_soa = lambda a: np.ascontiguousarray(a.transpose(0, 3, 2, 1)).reshape(19, -1)
+    np.savez_compressed(
+        os.path.join(OUT, "shear_wave_16_fp64.npz"),
+        initial=to_soa(f0),
+        step1=to_soa(history[0]),
+        step8=to_soa(history[-1]),
+        omega_plus=omega_plus, omega_minus=omega_minus, amplitude=amplitude, n=n,
+    )
+    r64 = swr.run_case(resolution=64, steps=1000, viscosity=viscosity,
+                       initial_amplitude=amplitude, magic_parameter=magic)
+    r32 = swr.run_case()  # the reference program's own default case
+    with open(os.path.join(OUT, "shear_wave_decay.json"), "w") as fh:
+        json.dump({"n64_tau0.8_magic0.25_1000steps": r64.__dict__,
+                   "reference_default_case": r32.__dict__}, fh, indent=1, sort_keys=True)
+
+
+def deetjen():
+    d = os.path.join(REF, "ValidationInputs", "deetjen-ob-f03-surface-v1")
+    manifest = json.load(open(os.path.join(d, "manifest.json")))
+    art = json.load(open(os.path.join(REF, "ValidationArtifacts", "deetjen-dove-indexed-metal-geometry.json")))
+    frames = manifest["frames"]["count"]
+    verts = manifest["topology"]["vertexCount"]
+    tris = manifest["topology"]["triangleCount"]
+    raw = open(os.path.join(d, "positions.f32le"), "rb").read()
+    assert hashlib.sha256(raw).hexdigest() == manifest["binary"]["positions"]["sha256"]
+    pts = np.frombuffer(raw, "<f4").reshape(frames, verts, 3)
+    traw = open(os.path.join(d, "triangles.u16le"), "rb").read()
+    assert hashlib.sha256(traw).hexdigest() == manifest["binary"]["triangles"]["sha256"]
+    tri = np.frombuffer(traw, "<u2").reshape(tris, 3)
+    part = np.zeros(tris, np.uint8)
+    for c in manifest["topology"]["components"]:
+        part[c["triangleOffset"]:c["triangleOffset"] + c["triangleCount"]] = c["partIdentifier"]
+    times = np.asarray(manifest["frames"]["timesSeconds"], np.float32)
+    # Domain of MetalIndexedBirdSurfaceValidation.swift:17766-17777 in FP32:
+    # origin = min - 4 dx, grid = ceil(extent / dx) + 1 per axis (>= 16).
+    dx = np.float32(art["cellSizeMeters"])
+    pad = np.float32(4) * dx
+    lo = (pts.reshape(-1, 3).min(axis=0) - pad).astype(np.float32)
+    hi = (pts.reshape(-1, 3).max(axis=0) + pad).astype(np.float32)
+    grid = [max(16, int(np.ceil(np.float32(e / dx))) + 1) for e in (hi - lo).astype(np.float32)]
+    assert grid == [art["gridX"], art["gridY"], art["gridZ"]], grid
+    keep = sorted(set(list(range(0, frames, 12)) + [33, 89, 126, 134, 142, 143]))
+    np.savez_compressed(
+        os.path.join(OUT, "deetjen_subset.npz"),
+        frames=np.asarray(keep, np.int32), points=pts[keep], times=times, triangles=tri.astype(np.uint32),
+        triangle_parts=part, origin=lo, grid=np.asarray(grid, np.int32), dx=dx,
+        half_thickness_cells=np.float32(art["halfThicknessCells"]),
+    )
+    audits = [{"frame": a["frameIndex"], "sha256": a["occupancySHA256"],
+               "counts": a["componentSolidCellCounts"], "cpu_compared": a["cpuRasterCompared"]}
+              for a in art["frameAudits"]]
+    with open(os.path.join(OUT, "deetjen_geometry.json"), "w") as fh:
+        json.dump({"grid": grid, "cell_size": float(dx), "half_thickness_cells": art["halfThicknessCells"],
+                   "frames": audits}, fh, indent=1)
+
+
+def translating_body():
+    s = json.load(open(os.path.join(REF, "ValidationArtifacts", "translating-body-topology-summary.json")))
+    keys = ["gridResolution", "sphereRadiusCells", "translationSpeedLattice", "steps", "newlyCoveredCellEvents",
+            "newlyUncoveredCellEvents", "topologyTransitionSteps", "conservativeMeanForceX",
+            "rawBudgetMeanForceX", "legacyMeanForceX"]
+    with open(os.path.join(OUT, "translating_body.json"), "w") as fh:
+        json.dump({k: s[k] for k in keys}, fh, indent=1)
+
+
+if __name__ == "__main__":
+    deetjen()
+    translating_body()
+    shear_wave()
