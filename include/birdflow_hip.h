@@ -1,0 +1,206 @@
+/*
+ * birdflow_hip.h — C ABI of libbirdflow_hip.so, the MI355X (gfx950) compute
+ * backend for the D3Q19 moving-boundary step.
+ *
+ * It replaces the Metal compute backend of the reference:
+ *   Sources/BirdFlowMetal/MetalBackend.swift:7-186        (device, pipelines, buffers, dispatch)
+ *   Sources/BirdFlowMetal/BirdFlowSimulation.swift:1300-1727 (encode* step graph)
+ *   Sources/BirdFlowMetal/GPUData.swift:4-80               (GPUUniforms lane packing)
+ * and the indexed-mesh replay driver
+ *   Sources/BirdFlowMetal/MetalIndexedBirdSurfaceValidation.swift:17697-18602.
+ * Host code keeps the solver API of BirdFlowSimulation (init, advance,
+ * snapshot, field copies, mesh upload, load readback, body state) on top of
+ * these calls.
+ *
+ * Conventions
+ *   - Plain C types only; the library owns every device allocation and copies
+ *     caller arrays in and out.  No callbacks.
+ *   - One context = one HIP stream on the current device.  Calls on one
+ *     context must be serialized by the caller.
+ *   - Populations are exchanged as FP32, direction-major SoA:
+ *     f[q * N + x + nx * (y + ny * z)], N = nx * ny * nz, q in the reference's
+ *     D3Q19 order (BirdFlow.metal:7-44).
+ *   - Status codes map one-to-one onto BirdFlowError
+ *     (Sources/BirdFlowMetal/BirdFlowError.swift:3-19).  After a failed step
+ *     the context is invalidated: every later call except bf_destroy and
+ *     bf_last_error returns BF_ERROR_STATE_INVALIDATED
+ *     (BirdFlowSimulation.swift:2059-2087).
+ *   - There is no CPU fallback: bf_create fails with
+ *     BF_ERROR_DEVICE_UNAVAILABLE when no gfx950 device is present.
+ */
+#ifndef BIRDFLOW_HIP_H
+#define BIRDFLOW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BF_ABI_VERSION 1u
+
+typedef enum bf_status {
+    BF_OK = 0,
+    BF_ERROR_DEVICE_UNAVAILABLE = 1,          /* metalUnavailable / deviceUnavailable */
+    BF_ERROR_INVALID_CONFIGURATION = 2,       /* BirdFlowConfigurationError (grid, tau, Mach) */
+    BF_ERROR_ALLOCATION_FAILED = 3,           /* allocationFailed(bytes:) */
+    BF_ERROR_BUFFER_EXCEEDS_DEVICE_LIMIT = 4, /* bufferExceedsDeviceLimit */
+    BF_ERROR_WORKING_SET_EXCEEDS_MEMORY = 5,  /* workingSetExceedsRecommendation */
+    BF_ERROR_INVALID_ADVANCE_REQUEST = 6,     /* invalidAdvanceRequest(steps:batchSize:) */
+    BF_ERROR_INVALID_ARGUMENT = 7,            /* resourceMissing / malformed caller arrays */
+    BF_ERROR_LAUNCH_FAILED = 8,               /* commandBufferFailed */
+    BF_ERROR_RUNTIME_SAFETY_VIOLATION = 9,    /* runtimeSafetyViolation */
+    BF_ERROR_STATE_INVALIDATED = 10,          /* simulationStateInvalidated */
+    BF_ERROR_NOT_SUPPORTED = 11
+} bf_status;
+
+/* Face treatment (per axis, low and high side).  The reference offers
+ * all-periodic wrapping or all-faces far-field equilibrium
+ * (BirdFlow.metal:2696-2725); OUTLET (zero-gradient: the pull source is
+ * clamped into the domain) is an addition. */
+enum { BF_BC_PERIODIC = 0, BF_BC_FARFIELD = 1, BF_BC_OUTLET = 2 };
+
+/* Collision operator, replacing the caseParameters.w thresholds
+ * (BirdFlow.metal:2906-2912). */
+enum { BF_COLLISION_TRT = 0, BF_COLLISION_TRT_SYMMETRIC_LIMITER = 1,
+       BF_COLLISION_REGULARIZED = 2, BF_COLLISION_RR3 = 3 };
+
+/* Wall rule: halfway bounce-back, or Bouzidi interpolated bounce-back with the
+ * link fraction from the signed-distance ratio (BirdFlow.metal:2044-2076,
+ * 2729-2806), or the interpolated path at q = 1/2 (translating-sphere
+ * canonical, BirdFlow.metal:2340-2345). */
+enum { BF_WALL_HALFWAY = 0, BF_WALL_INTERPOLATED_SDF = 1, BF_WALL_INTERPOLATED_HALF = 2 };
+
+/* Population storage: FP32, or FP16 / BF16 offsets f - w_q with FP32
+ * arithmetic. */
+enum { BF_STORAGE_FP32 = 0, BF_STORAGE_FP16_OFFSET = 1, BF_STORAGE_BF16_OFFSET = 2 };
+
+/* Where each step's geometry comes from. */
+enum { BF_GEOMETRY_STATIC = 0, BF_GEOMETRY_MESH = 1, BF_GEOMETRY_SPHERE = 2 };
+
+/* Solver configuration in lattice units (the host converts physical units
+ * with the reference's LatticeScaling, SimulationConfiguration.swift:137-183).
+ * Replaces GPUUniforms (GPUData.swift:4-80). */
+typedef struct bf_config {
+    uint32_t nx, ny, nz;          /* each >= 1; nx*ny*nz < 2^32 */
+    uint32_t storage;             /* BF_STORAGE_* */
+    uint32_t collision;           /* BF_COLLISION_* */
+    uint32_t wall_rule;           /* BF_WALL_* */
+    uint32_t force_mode;          /* 0..6 momentum-exchange estimator; 6 = conservative moving-domain */
+    uint32_t load_component;      /* 0 all, 1 link exchange only, 2 topology impulse only */
+    uint32_t selected_part;       /* 0 = all parts */
+    uint32_t accumulate_loads;    /* gravity.w */
+    uint32_t boundary[3][2];      /* BF_BC_* per axis, [low, high] */
+    float origin[3];              /* world position of cell (0,0,0)'s corner */
+    float cell_size;              /* dx (world units) */
+    float time_step;              /* dt (world units) used for geometry time */
+    float velocity_to_lattice;    /* dt / dx */
+    float force_to_physical;      /* rho dx^4 / dt^2 */
+    float omega_plus, omega_minus;
+    float sponge_strength;        /* 0..1 */
+    float sponge_width;           /* cells */
+    float far_velocity[3];        /* lattice units */
+    float far_density;
+    float body_position[3];       /* torque reference point (world units) */
+    uint32_t reserved[8];
+} bf_config;
+
+typedef struct bf_context bf_context;
+
+/* MetalBackend.init + BirdFlowSimulation.init buffer plan
+ * (MetalBackend.swift:14-46, 77-106; BirdFlowSimulation.swift:480-625). */
+bf_status bf_create(const bf_config* config, bf_context** out);
+void bf_destroy(bf_context* ctx);
+const char* bf_last_error(const bf_context* ctx);
+const char* bf_status_string(bf_status status);
+uint32_t bf_abi_version(void);
+/* Device name and memory of the current device (MetalBackend.device). */
+bf_status bf_device_info(char* name, size_t name_capacity, uint64_t* total_bytes);
+/* Bytes of device memory the context holds. */
+uint64_t bf_allocated_bytes(const bf_context* ctx);
+
+/* ---- state ---- */
+/* initializePopulations (BirdFlow.metal:2078-2103): solid cells at their wall
+ * velocity, fluid at the far-field velocity, rho = 1; resets the step index. */
+bf_status bf_init_equilibrium(bf_context* ctx);
+/* initializeShearWave (BirdFlow.metal:2132-2162). */
+bf_status bf_init_shear_wave(bf_context* ctx, float amplitude);
+/* Blit copies of the population lattice (MetalShearWaveSimulation.copyPopulations,
+ * BirdFlowSimulation.swift:1214-1298 checkpoint read-back).  count = 19 * N. */
+bf_status bf_write_populations(bf_context* ctx, const float* host, uint64_t count);
+bf_status bf_read_populations(bf_context* ctx, float* host, uint64_t count);
+/* extractMacroscopicFields (BirdFlow.metal:2107-2127) into host arrays:
+ * density[N], velocity[3N] interleaved. */
+bf_status bf_read_fields(bf_context* ctx, float* density, float* velocity3);
+uint64_t bf_step_index(const bf_context* ctx);
+
+/* ---- geometry ---- */
+/* Static geometry written by the caller: part masks (0 fluid, 1..255 part id)
+ * of the previous and current step and wall[N] = (ux, uy, uz, signed
+ * distance / dx).  mask_prev may be NULL (= mask_cur).  Selects
+ * BF_GEOMETRY_STATIC. */
+bf_status bf_set_geometry(bf_context* ctx, const uint8_t* mask_prev, const uint8_t* mask_cur, const float* wall4);
+/* Current geometry read-back (integer flags for bit-exact parity). */
+bf_status bf_read_geometry(bf_context* ctx, uint8_t* mask, float* wall4);
+
+/* Indexed mesh upload (MeasuredBirdSurfaceSequence.swift:120-204,
+ * MetalIndexedBirdSurfaceValidation.swift:17746-17915): frame-major points
+ * [frame_count][vertex_count][3] (world units), strictly increasing
+ * frame_times, triangles [triangle_count][3] vertex indices, triangle part ids
+ * (1..255), shell half thickness (world units), rigid translation added to
+ * every vertex.  No triangle limit (keys are 64-bit).  Builds the geometry at
+ * time_start into the current step and selects BF_GEOMETRY_MESH; each later
+ * step builds the geometry at time_start + step * time_step. */
+bf_status bf_mesh_upload(bf_context* ctx, const float* points, const float* frame_times, uint32_t frame_count,
+                         uint32_t vertex_count, const uint32_t* triangles, const uint8_t* triangle_parts,
+                         uint32_t triangle_count, float half_thickness, const float translation[3],
+                         float time_start);
+/* Rebuild the current step's geometry at an arbitrary time (geometry audit,
+ * MetalIndexedBirdSurfaceReplay.snapshot, :17916-17992). */
+bf_status bf_mesh_build(bf_context* ctx, float time);
+
+/* Translating-sphere canonical (BirdFlow.metal:2263-2350): center(t) =
+ * center_radius.xyz + geometry_velocity * t, wall_velocity.w selects uniform
+ * (0), tangential (1) or normal (2) wall velocity.  Builds the geometry at
+ * t = step_index * time_step and selects BF_GEOMETRY_SPHERE. */
+bf_status bf_set_translating_sphere(bf_context* ctx, const float center_radius[4], const float geometry_velocity[3],
+                                    const float wall_velocity[4]);
+
+/* ---- stepping ---- */
+/* advance(steps:) (BirdFlowSimulation.swift:687-799): per step, geometry at
+ * t_{n+1} into the next mask, fluid step (current -> next lattice, previous /
+ * current masks), fixed-order FP64 load reduction into the history, swap.
+ * Asynchronous: returns after enqueueing; errors of earlier work surface here
+ * or at bf_synchronize. */
+bf_status bf_step(bf_context* ctx, uint32_t steps);
+bf_status bf_synchronize(bf_context* ctx);
+/* Loads of the last step: {Fx, Fy, Fz, Tx, Ty, Tz, limiter activations,
+ * max limiter restriction} (GPUForceTorque, BirdFlow.metal:197-200), FP64. */
+bf_status bf_read_loads(bf_context* ctx, double out[8]);
+/* Per-step load history (storeForceTorqueSample, BirdFlow.metal:5392-5401):
+ * up to max_records most recent records of 8 doubles, oldest first. */
+bf_status bf_read_load_history(bf_context* ctx, double* out, uint32_t max_records, uint32_t* records);
+/* Link census of the current step: counts[q] = fluid cells whose pull source
+ * along q is solid, counts[19] newly covered, counts[20] newly uncovered. */
+bf_status bf_link_census(bf_context* ctx, uint64_t counts[21]);
+
+/* ---- measurement ---- */
+/* When enabled, every step records HIP events on the context's stream around
+ * the geometry, fast-fluid and general-fluid+reduction launches. */
+bf_status bf_set_profiling(bf_context* ctx, int enabled);
+/* {geometry ms, fast fluid ms, general fluid + reduction ms, steps profiled,
+ * tiles on the general path in the last step, total tiles (64 x 4 cells)}
+ * summed over the steps profiled since the last reset; resets the counters. */
+bf_status bf_read_profile(bf_context* ctx, double out[6]);
+
+/* Device copy bandwidth on the current device: a plain 16-byte-per-lane copy
+ * between two fresh buffers of `bytes` each, repeated `iterations` times;
+ * returns (read + write bytes) / time in GB/s.  Calibrates achieved-bandwidth
+ * figures and hardware-counter byte counts. */
+bf_status bf_copy_bandwidth(uint64_t bytes, uint32_t iterations, double* gbs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+"""ctypes declarations of libbirdflow_hip.so (include/birdflow_hip.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libbirdflow_hip.so")
+
+# bf_status (BirdFlowError.swift:3-19)
+OK = 0
+STATUS_NAMES = {
+    0: "ok",
+    1: "deviceUnavailable",
+    2: "invalidConfiguration",
+    3: "allocationFailed",
+    4: "bufferExceedsDeviceLimit",
+    5: "workingSetExceedsMemory",
+    6: "invalidAdvanceRequest",
+    7: "invalidArgument",
+    8: "launchFailed",
+    9: "runtimeSafetyViolation",
+    10: "simulationStateInvalidated",
+    11: "notSupported",
+}
+ERROR_DEVICE_UNAVAILABLE = 1
+ERROR_INVALID_CONFIGURATION = 2
+ERROR_INVALID_ADVANCE_REQUEST = 6
+ERROR_INVALID_ARGUMENT = 7
+ERROR_STATE_INVALIDATED = 10
+
+BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
+COLLISION_TRT, COLLISION_TRT_SYMMETRIC_LIMITER, COLLISION_REGULARIZED, COLLISION_RR3 = 0, 1, 2, 3
+WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF = 0, 1, 2
+STORAGE_FP32, STORAGE_FP16_OFFSET, STORAGE_BF16_OFFSET = 0, 1, 2
+
+EXPORTED_FUNCTIONS = (
+    "bf_create", "bf_destroy", "bf_last_error", "bf_status_string", "bf_abi_version", "bf_device_info",
+    "bf_allocated_bytes", "bf_init_equilibrium", "bf_init_shear_wave", "bf_write_populations",
+    "bf_read_populations", "bf_read_fields", "bf_step_index", "bf_set_geometry", "bf_read_geometry",
+    "bf_mesh_upload", "bf_mesh_build", "bf_set_translating_sphere", "bf_step", "bf_synchronize",
+    "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
+    "bf_copy_bandwidth",
+)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
+        ("storage", ctypes.c_uint32),
+        ("collision", ctypes.c_uint32),
+        ("wall_rule", ctypes.c_uint32),
+        ("force_mode", ctypes.c_uint32),
+        ("load_component", ctypes.c_uint32),
+        ("selected_part", ctypes.c_uint32),
+        ("accumulate_loads", ctypes.c_uint32),
+        ("boundary", (ctypes.c_uint32 * 2) * 3),
+        ("origin", ctypes.c_float * 3),
+        ("cell_size", ctypes.c_float),
+        ("time_step", ctypes.c_float),
+        ("velocity_to_lattice", ctypes.c_float),
+        ("force_to_physical", ctypes.c_float),
+        ("omega_plus", ctypes.c_float), ("omega_minus", ctypes.c_float),
+        ("sponge_strength", ctypes.c_float),
+        ("sponge_width", ctypes.c_float),
+        ("far_velocity", ctypes.c_float * 3),
+        ("far_density", ctypes.c_float),
+        ("body_position", ctypes.c_float * 3),
+        ("reserved", ctypes.c_uint32 * 8),
+    ]
+
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Load the in-tree library.  There is no fallback: a missing library is an error."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise LibraryMissing(
+                f"{LIBRARY_PATH} is not built; run __graft_entry__.build() or make -C numi-gpu-physics_amd/csrc")
+        lib = ctypes.CDLL(LIBRARY_PATH)
+        _declare(lib)
+        _lib = lib
+    return _lib
+
+
+def _declare(l):
+    vp, u32, u64, f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float
+    st = ctypes.c_int
+    sig = {
+        "bf_create": (st, [ctypes.POINTER(Config), ctypes.POINTER(vp)]),
+        "bf_destroy": (None, [vp]),
+        "bf_last_error": (ctypes.c_char_p, [vp]),
+        "bf_status_string": (ctypes.c_char_p, [st]),
+        "bf_abi_version": (u32, []),
+        "bf_device_info": (st, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(u64)]),
+        "bf_allocated_bytes": (u64, [vp]),
+        "bf_init_equilibrium": (st, [vp]),
+        "bf_init_shear_wave": (st, [vp, f32]),
+        "bf_write_populations": (st, [vp, vp, u64]),
+        "bf_read_populations": (st, [vp, vp, u64]),
+        "bf_read_fields": (st, [vp, vp, vp]),
+        "bf_step_index": (u64, [vp]),
+        "bf_set_geometry": (st, [vp, vp, vp, vp]),
+        "bf_read_geometry": (st, [vp, vp, vp]),
+        "bf_mesh_upload": (st, [vp, vp, vp, u32, u32, vp, vp, u32, f32, vp, f32]),
+        "bf_mesh_build": (st, [vp, f32]),
+        "bf_set_translating_sphere": (st, [vp, vp, vp, vp]),
+        "bf_step": (st, [vp, u32]),
+        "bf_synchronize": (st, [vp]),
+        "bf_read_loads": (st, [vp, vp]),
+        "bf_read_load_history": (st, [vp, vp, u32, ctypes.POINTER(u32)]),
+        "bf_link_census": (st, [vp, vp]),
+        "bf_set_profiling": (st, [vp, ctypes.c_int]),
+        "bf_read_profile": (st, [vp, vp]),
+        "bf_copy_bandwidth": (st, [u64, u32, ctypes.POINTER(ctypes.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(l, name)
+        fn.restype = res
+        fn.argtypes = args

@@ -1,0 +1,278 @@
+"""Host-side solver API over the C ABI.
+
+Mirrors the reference's BirdFlowSimulation surface
+(Sources/BirdFlowMetal/BirdFlowSimulation.swift): construction from a
+validated configuration (:480), advance (:687), population and field copies
+(:1109, :1214), the load history, and the indexed-mesh replay entry points of
+MetalIndexedBirdSurfaceReplay (MetalIndexedBirdSurfaceValidation.swift:17697).
+Every call goes through libbirdflow_hip.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+
+
+class BirdFlowError(RuntimeError):
+    """Error carrying a bf_status code (BirdFlowError.swift:3-19)."""
+
+    def __init__(self, status: int, message: str):
+        self.status = status
+        self.name = _abi.STATUS_NAMES.get(status, "unknown")
+        super().__init__(f"{self.name}: {message}")
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+@dataclass(frozen=True)
+class LatticeScaling:
+    """LatticeScaling of SimulationConfiguration.swift:137-183 in FP32."""
+
+    cell_size: float
+    time_step: float
+    lattice_reference_speed: float
+    lattice_viscosity: float
+    tau_plus: float
+    tau_minus: float
+    omega_plus: float
+    omega_minus: float
+    velocity_to_lattice: float
+    force_to_physical: float
+
+    @classmethod
+    def make(cls, characteristic_length: float, characteristic_cells: int, reference_speed: float,
+             reynolds: float, air_density: float, lattice_reference_speed: float = 0.05,
+             magic: float = 3.0 / 16.0, minimum_tau_plus: float = 0.50005) -> "LatticeScaling":
+        f = np.float32
+        if not (characteristic_length > 0 and characteristic_cells >= 8 and reference_speed > 0
+                and reynolds > 0 and air_density > 0 and lattice_reference_speed > 0 and magic > 0):
+            raise BirdFlowError(_abi.ERROR_INVALID_CONFIGURATION, "all physical and lattice scales must be positive")
+        mach = f(lattice_reference_speed) / f(math.sqrt(1.0 / 3.0))
+        if mach > 0.15:
+            raise BirdFlowError(_abi.ERROR_INVALID_CONFIGURATION, f"lattice Mach {mach} above 0.15")
+        dx = f(characteristic_length) / f(characteristic_cells)
+        dt = f(lattice_reference_speed) * dx / f(reference_speed)
+        nu = f(lattice_reference_speed) * f(characteristic_cells) / f(reynolds)
+        tau_p = f(0.5) + f(3) * nu
+        if tau_p < f(minimum_tau_plus):
+            raise BirdFlowError(_abi.ERROR_INVALID_CONFIGURATION, f"tau+ {tau_p} too close to 1/2")
+        tau_m = f(0.5) + f(magic) / (tau_p - f(0.5))
+        force = f(air_density) * f(dx) ** 4 / f(dt) ** 2
+        return cls(float(dx), float(dt), float(lattice_reference_speed), float(nu), float(tau_p), float(tau_m),
+                   float(f(1) / tau_p), float(f(1) / tau_m), float(dt / dx), float(force))
+
+
+@dataclass
+class SolverConfig:
+    """Configuration of one solver context (bf_config), lattice units."""
+
+    shape: Sequence[int]
+    omega_plus: float
+    omega_minus: float
+    storage: int = _abi.STORAGE_FP32
+    collision: int = _abi.COLLISION_TRT
+    wall_rule: int = _abi.WALL_HALFWAY
+    force_mode: int = 6
+    load_component: int = 0
+    selected_part: int = 0
+    accumulate_loads: bool = False
+    boundary: Sequence[Sequence[int]] = ((0, 0), (0, 0), (0, 0))
+    origin: Sequence[float] = (0.0, 0.0, 0.0)
+    cell_size: float = 1.0
+    time_step: float = 1.0
+    velocity_to_lattice: float = 1.0
+    force_to_physical: float = 1.0
+    sponge_strength: float = 0.0
+    sponge_width: float = 4.0
+    far_velocity: Sequence[float] = (0.0, 0.0, 0.0)
+    far_density: float = 1.0
+    body_position: Sequence[float] = (0.0, 0.0, 0.0)
+
+    def to_c(self) -> _abi.Config:
+        c = _abi.Config()
+        c.nx, c.ny, c.nz = (int(v) for v in self.shape)
+        c.storage, c.collision, c.wall_rule = self.storage, self.collision, self.wall_rule
+        c.force_mode, c.load_component, c.selected_part = self.force_mode, self.load_component, self.selected_part
+        c.accumulate_loads = int(bool(self.accumulate_loads))
+        for a in range(3):
+            for s in range(2):
+                c.boundary[a][s] = int(self.boundary[a][s])
+        c.origin[:] = [float(v) for v in self.origin]
+        c.cell_size, c.time_step = self.cell_size, self.time_step
+        c.velocity_to_lattice, c.force_to_physical = self.velocity_to_lattice, self.force_to_physical
+        c.omega_plus, c.omega_minus = self.omega_plus, self.omega_minus
+        c.sponge_strength, c.sponge_width = self.sponge_strength, self.sponge_width
+        c.far_velocity[:] = [float(v) for v in self.far_velocity]
+        c.far_density = self.far_density
+        c.body_position[:] = [float(v) for v in self.body_position]
+        return c
+
+    @property
+    def cells(self) -> int:
+        return int(self.shape[0]) * int(self.shape[1]) * int(self.shape[2])
+
+
+def device_info():
+    lib = _abi.load()
+    name = ctypes.create_string_buffer(256)
+    total = ctypes.c_uint64(0)
+    st = lib.bf_device_info(name, 256, ctypes.byref(total))
+    if st != _abi.OK:
+        raise BirdFlowError(st, lib.bf_status_string(st).decode())
+    return name.value.decode(), int(total.value)
+
+
+def copy_bandwidth(bytes_per_buffer: int = 1 << 30, iterations: int = 20) -> float:
+    """Measured device copy bandwidth in GB/s (read + write)."""
+    lib = _abi.load()
+    out = ctypes.c_double(0.0)
+    st = lib.bf_copy_bandwidth(int(bytes_per_buffer), int(iterations), ctypes.byref(out))
+    if st != _abi.OK:
+        raise BirdFlowError(st, lib.bf_status_string(st).decode())
+    return float(out.value)
+
+
+class BirdFlowSimulation:
+    """One solver context on the current HIP device."""
+
+    def __init__(self, config: SolverConfig):
+        self.config = config
+        self._lib = _abi.load()
+        self._cfg = config.to_c()
+        handle = ctypes.c_void_p()
+        st = self._lib.bf_create(ctypes.byref(self._cfg), ctypes.byref(handle))
+        if st != _abi.OK:
+            raise BirdFlowError(st, self._lib.bf_status_string(st).decode())
+        self._ctx = handle
+        self.n = config.cells
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.bf_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st):
+        if st != _abi.OK:
+            msg = self._lib.bf_last_error(self._ctx)
+            raise BirdFlowError(st, (msg or b"").decode() or self._lib.bf_status_string(st).decode())
+
+    # -- state -----------------------------------------------------------
+    def init_equilibrium(self):
+        self._check(self._lib.bf_init_equilibrium(self._ctx))
+
+    def init_shear_wave(self, amplitude: float):
+        self._check(self._lib.bf_init_shear_wave(self._ctx, float(amplitude)))
+
+    def write_populations(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, np.float32).ravel()
+        self._check(self._lib.bf_write_populations(self._ctx, _ptr(f), f.size))
+
+    def read_populations(self) -> np.ndarray:
+        out = np.empty(19 * self.n, np.float32)
+        self._check(self._lib.bf_read_populations(self._ctx, _ptr(out), out.size))
+        return out
+
+    def copy_macroscopic_fields(self):
+        rho = np.empty(self.n, np.float32)
+        u = np.empty((self.n, 3), np.float32)
+        self._check(self._lib.bf_read_fields(self._ctx, _ptr(rho), _ptr(u)))
+        return rho, u
+
+    @property
+    def step_index(self) -> int:
+        return int(self._lib.bf_step_index(self._ctx))
+
+    @property
+    def allocated_bytes(self) -> int:
+        return int(self._lib.bf_allocated_bytes(self._ctx))
+
+    # -- geometry --------------------------------------------------------
+    def set_geometry(self, mask_cur, wall4, mask_prev=None):
+        mc = np.ascontiguousarray(mask_cur, np.uint8).ravel()
+        w = np.ascontiguousarray(wall4, np.float32).reshape(-1, 4)
+        mp = None if mask_prev is None else np.ascontiguousarray(mask_prev, np.uint8).ravel()
+        self._check(self._lib.bf_set_geometry(self._ctx, _ptr(mp), _ptr(mc), _ptr(w)))
+
+    def read_geometry(self):
+        mask = np.empty(self.n, np.uint8)
+        wall = np.empty((self.n, 4), np.float32)
+        self._check(self._lib.bf_read_geometry(self._ctx, _ptr(mask), _ptr(wall)))
+        return mask, wall
+
+    def upload_mesh(self, points, frame_times, triangles, triangle_parts, half_thickness,
+                    translation=(0.0, 0.0, 0.0), time_start: Optional[float] = None):
+        pts = np.ascontiguousarray(points, np.float32)
+        frames, verts = pts.shape[0], pts.shape[1]
+        times = np.ascontiguousarray(frame_times, np.float32)
+        tri = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+        parts = np.ascontiguousarray(triangle_parts, np.uint8)
+        tr = np.ascontiguousarray(translation, np.float32)
+        t0 = float(times[0]) if time_start is None else float(time_start)
+        self._check(self._lib.bf_mesh_upload(self._ctx, _ptr(pts), _ptr(times), frames, verts, _ptr(tri), _ptr(parts),
+                                             tri.shape[0], float(half_thickness), _ptr(tr), t0))
+
+    def build_mesh_geometry(self, time: float):
+        self._check(self._lib.bf_mesh_build(self._ctx, float(time)))
+
+    def set_translating_sphere(self, center_radius, geometry_velocity, wall_velocity):
+        cr = np.ascontiguousarray(center_radius, np.float32)
+        gv = np.ascontiguousarray(geometry_velocity, np.float32)
+        wv = np.ascontiguousarray(wall_velocity, np.float32)
+        self._check(self._lib.bf_set_translating_sphere(self._ctx, _ptr(cr), _ptr(gv), _ptr(wv)))
+
+    # -- stepping --------------------------------------------------------
+    def advance(self, steps: int, synchronize: bool = True):
+        if steps < 0:
+            raise BirdFlowError(_abi.ERROR_INVALID_ADVANCE_REQUEST, f"steps must be >= 0, got {steps}")
+        self._check(self._lib.bf_step(self._ctx, int(steps)))
+        if synchronize:
+            self.synchronize()
+
+    def synchronize(self):
+        self._check(self._lib.bf_synchronize(self._ctx))
+
+    def read_loads(self) -> np.ndarray:
+        out = np.zeros(8, np.float64)
+        self._check(self._lib.bf_read_loads(self._ctx, _ptr(out)))
+        return out
+
+    def load_history(self, max_records: int = 1 << 16) -> np.ndarray:
+        out = np.zeros((max_records, 8), np.float64)
+        n = ctypes.c_uint32(0)
+        self._check(self._lib.bf_read_load_history(self._ctx, _ptr(out), max_records, ctypes.byref(n)))
+        return out[: n.value].copy()
+
+    def link_census(self):
+        out = np.zeros(21, np.uint64)
+        self._check(self._lib.bf_link_census(self._ctx, _ptr(out)))
+        return out[:19].copy(), int(out[19]), int(out[20])
+
+    def set_profiling(self, enabled: bool):
+        self._check(self._lib.bf_set_profiling(self._ctx, int(bool(enabled))))
+
+    def read_profile(self):
+        out = np.zeros(6, np.float64)
+        self._check(self._lib.bf_read_profile(self._ctx, _ptr(out)))
+        return {"geometry_ms": out[0], "fast_ms": out[1], "general_ms": out[2], "steps": int(out[3]),
+                "general_tiles": int(out[4]), "tiles": int(out[5])}

@@ -1,0 +1,219 @@
+"""Synthetic moving-boundary workloads as indexed-mesh frame sequences.
+
+Both produce what the reference's indexed-surface replay consumes
+(MeasuredBirdSurfaceSequence.swift:36-60): frame-major vertex positions,
+frame times, a fixed triangle topology and per-triangle part ids
+(1 body, 2 left wing, 3 right wing, 4 tail).  One frame per lattice step, so
+the prepare kernel's piecewise-linear interpolation lands exactly on frames
+and the wall velocity is the one-step finite difference of positions.
+
+* flapping_plate — a rigid rectangular plate (span x chord x thickness,
+  12 triangles: 2 per face of the box) heaving h0 sin(2 pi t/T) and pitching
+  a0 sin(2 pi t/T + pi/2) about its mid-chord span axis.
+* synthetic_bird — ellipsoid body, two tapered wings and a fan tail
+  (about 40k triangles); per-wing stroke, deviation, pitch and twist are
+  3-harmonic Fourier series whose coefficients come from a seed.
+Lengths are in cells (world units = cells, dx = 1).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class MeshSequence:
+    points: np.ndarray          # [frames, vertices, 3] float32
+    times: np.ndarray           # [frames] float32
+    triangles: np.ndarray       # [triangles, 3] uint32
+    parts: np.ndarray           # [triangles] uint8
+    half_thickness: float
+    meta: dict = None
+
+    @property
+    def frames(self):
+        return self.points.shape[0]
+
+
+def _box(span, chord, thickness):
+    """Closed box centred at the origin: x chord, y span, z thickness."""
+    hx, hy, hz = chord / 2.0, span / 2.0, thickness / 2.0
+    v = np.array([[sx * hx, sy * hy, sz * hz] for sz in (-1, 1) for sy in (-1, 1) for sx in (-1, 1)], np.float64)
+    # vertex index = (sz>0)*4 + (sy>0)*2 + (sx>0)
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    tris = []
+    for a, b, c, d in quads:
+        tris += [(a, b, c), (a, c, d)]
+    return v, np.array(tris, np.uint32)
+
+
+def _rot_y(angle):
+    c, s = np.cos(angle), np.sin(angle)
+    return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+
+
+def flapping_plate(steps: int, center=(64.0, 64.0, 64.0), span=48.0, chord=16.0, thickness=2.0,
+                   heave=8.0, pitch_deg=30.0, period=800.0, half_thickness=0.5, t0=0) -> MeshSequence:
+    """Config: plate 48 x 16 x 2 at x = 64, heave 8 sin(2 pi t/800), pitch 30 deg sin(2 pi t/800 + pi/2)."""
+    rest, tri = _box(span, chord, thickness)
+    times = np.arange(t0, t0 + steps + 1, dtype=np.float64)
+    pts = np.empty((len(times), rest.shape[0], 3), np.float64)
+    for i, t in enumerate(times):
+        ph = 2 * math.pi * t / period
+        r = _rot_y(math.radians(pitch_deg) * math.sin(ph + math.pi / 2))
+        off = np.array([center[0], center[1], center[2] + heave * math.sin(ph)])
+        pts[i] = rest @ r.T + off
+    return MeshSequence(pts.astype(np.float32), times.astype(np.float32), tri, np.ones(len(tri), np.uint8),
+                        float(half_thickness))
+
+
+def _grid_sheet(nu, nv):
+    """Triangulated (nu x nv)-quad sheet; returns (u, v) in [0,1]^2 and triangles."""
+    u, v = np.meshgrid(np.linspace(0, 1, nu + 1), np.linspace(0, 1, nv + 1), indexing="ij")
+    idx = np.arange((nu + 1) * (nv + 1)).reshape(nu + 1, nv + 1)
+    a, b = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel()
+    c, d = idx[1:, 1:].ravel(), idx[:-1, 1:].ravel()
+    tri = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)])
+    return u.ravel(), v.ravel(), tri.astype(np.uint32)
+
+
+def _rot(axis, angle):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    x, y, z = axis
+    c, s = math.cos(angle), math.sin(angle)
+    C = 1 - c
+    return np.array([[c + x * x * C, x * y * C - z * s, x * z * C + y * s],
+                     [y * x * C + z * s, c + y * y * C, y * z * C - x * s],
+                     [z * x * C - y * s, z * y * C + x * s, c + z * z * C]])
+
+
+@dataclass
+class BirdKinematics:
+    """3-harmonic Fourier coefficients (radians) per wing and angle."""
+
+    period: float
+    stroke: np.ndarray      # [2 wings, 3 harmonics, 2 (sin, cos)]
+    deviation: np.ndarray
+    pitch: np.ndarray
+    twist: np.ndarray
+
+    @classmethod
+    def from_seed(cls, seed: int, period: float = 1200.0):
+        rng = np.random.default_rng(seed)
+
+        def series(max_deg):
+            # amplitudes decay with harmonic; the total stays within max_deg
+            a = rng.uniform(-1.0, 1.0, size=(2, 3, 2)) * np.array([1.0, 0.35, 0.15])[None, :, None]
+            a *= math.radians(max_deg) / max(np.abs(a).sum(axis=(1, 2)).max(), 1e-12)
+            return a
+
+        return cls(period, series(60.0), series(20.0), series(20.0), series(20.0))
+
+    def angle(self, coeff, wing, t):
+        ph = 2 * math.pi * t / self.period
+        k = np.arange(1, 4)
+        return float((coeff[wing, :, 0] * np.sin(k * ph) + coeff[wing, :, 1] * np.cos(k * ph)).sum())
+
+
+SOUND_SPEED = math.sqrt(1.0 / 3.0)
+
+
+class SyntheticBird:
+    """Rest geometry, topology and Fourier kinematics of the synthetic bird."""
+
+    def __init__(self, seed=7, center=(160.0, 128.0, 128.0), body_radii=(40.0, 14.0, 14.0), half_span=100.0,
+                 root_chord=40.0, tip_chord=12.0, tail=(30.0, 20.0), period=1200.0, body_segments=(96, 64),
+                 wing_segments=(160, 40), tail_segments=(48, 24)):
+        self.kin = BirdKinematics.from_seed(seed, period)
+        self.center = np.asarray(center, np.float64)
+        nu, nv = body_segments
+        th = np.linspace(0, math.pi, nv + 1)
+        ph = np.linspace(0, 2 * math.pi, nu, endpoint=False)
+        T, P = np.meshgrid(th, ph, indexing="ij")
+        self.body = np.stack([body_radii[0] * np.cos(T), body_radii[1] * np.sin(T) * np.cos(P),
+                              body_radii[2] * np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
+        idx = np.arange((nv + 1) * nu).reshape(nv + 1, nu)
+        a, b = idx[:-1, :], np.roll(idx[:-1, :], -1, axis=1)
+        c, d = np.roll(idx[1:, :], -1, axis=1), idx[1:, :]
+        body_tri = np.concatenate([np.stack([a.ravel(), b.ravel(), c.ravel()], 1),
+                                   np.stack([a.ravel(), c.ravel(), d.ravel()], 1)]).astype(np.uint32)
+        su, sv, wing_tri = _grid_sheet(*wing_segments)
+        self.span_fraction = su
+        chord_len = root_chord + (tip_chord - root_chord) * su
+        self.wing_local = np.stack([(sv - 0.25) * chord_len, su * half_span, np.zeros_like(su)], -1)
+        self.root = np.array([0.0, body_radii[1] * 0.8, body_radii[2] * 0.3])
+        tu, tv, tail_tri = _grid_sheet(*tail_segments)
+        fan = (tu - 0.5) * math.radians(60.0)
+        rad = tv * tail[0]
+        width_scale = tail[1] / (2 * tail[0] * math.sin(math.radians(30.0)))
+        self.tail_local = np.stack([-body_radii[0] * 0.9 - rad * np.cos(fan), rad * np.sin(fan) * width_scale * 2,
+                                    np.zeros_like(tu)], -1)
+        nb, nw = len(self.body), len(self.wing_local)
+        self.triangles = np.concatenate([body_tri, wing_tri + nb, wing_tri + nb + nw,
+                                         tail_tri + nb + 2 * nw]).astype(np.uint32)
+        self.parts = np.concatenate([np.full(len(body_tri), 1), np.full(len(wing_tri), 2),
+                                     np.full(len(wing_tri), 3), np.full(len(tail_tri), 4)]).astype(np.uint8)
+        self.amplitude_scale = 1.0
+
+    def wing(self, w, side, t):
+        k, s = self.kin, self.amplitude_scale
+        stroke = s * k.angle(k.stroke, w, t)
+        dev = s * k.angle(k.deviation, w, t)
+        pit = s * k.angle(k.pitch, w, t)
+        tw = s * k.angle(k.twist, w, t)
+        loc = self.wing_local.copy()
+        loc[:, 1] *= side
+        # pitch plus a twist growing linearly with span, about the span axis
+        ang = pit + tw * self.span_fraction
+        cth, sth = np.cos(ang), np.sin(ang)
+        x0, z0 = loc[:, 0].copy(), loc[:, 2].copy()
+        loc[:, 0] = cth * x0 + sth * z0
+        loc[:, 2] = -sth * x0 + cth * z0
+        r = _rot((1.0, 0.0, 0.0), side * stroke) @ _rot((0.0, 0.0, 1.0), side * dev)
+        return loc @ r.T + self.root * np.array([1.0, side, 1.0])
+
+    def frame(self, t):
+        parts = [self.body, self.wing(0, 1.0, t), self.wing(1, -1.0, t), self.tail_local]
+        return np.concatenate(parts) + self.center
+
+    def max_wing_speed(self, samples=240):
+        """Largest wing-vertex speed over one period (central differences)."""
+        period = self.kin.period
+        best = 0.0
+        for t in np.linspace(0.0, period, samples, endpoint=False):
+            for w, side in ((0, 1.0), (1, -1.0)):
+                v = self.wing(w, side, t + 0.5) - self.wing(w, side, t - 0.5)
+                best = max(best, float(np.sqrt((v * v).sum(axis=1)).max()))
+        return best
+
+    def limit_mach(self, relative_translation, mach=0.15):
+        """Scale every drawn angle amplitude by one factor so that
+        (relative translation + wing speed) / c_s <= mach, the guard of
+        BirdModel.swift:345-362."""
+        limit = mach * SOUND_SPEED - relative_translation
+        for _ in range(4):
+            speed = self.max_wing_speed()
+            if speed <= limit * (1 - 1e-6):
+                break
+            self.amplitude_scale *= limit / speed * 0.999
+        return self.amplitude_scale
+
+
+def synthetic_bird(steps: int, seed: int = 7, center=(160.0, 128.0, 128.0), inflow: float = 0.04,
+                   mach: float = 0.15, half_thickness: float = 1.0, t0: int = 0, **geometry) -> MeshSequence:
+    """Synthetic articulated bird of about 40k triangles (512 x 256 x 256 domain)
+    with one frame per step.  Amplitudes are drawn from the seed (stroke <= 60
+    deg, deviation / pitch / twist <= 20 deg) and then scaled uniformly to the
+    reference's lattice-Mach guard for a body held in the inflow."""
+    bird = SyntheticBird(seed=seed, center=center, **geometry)
+    scale = bird.limit_mach(abs(inflow), mach)
+    times = np.arange(t0, t0 + steps + 1, dtype=np.float64)
+    pts = np.empty((len(times), bird.frame(0.0).shape[0], 3), np.float32)
+    for i, t in enumerate(times):
+        pts[i] = bird.frame(t).astype(np.float32)
+    return MeshSequence(pts, times.astype(np.float32), bird.triangles, bird.parts, float(half_thickness),
+                        meta={"amplitude_scale": scale, "max_wing_speed": bird.max_wing_speed(),
+                              "triangles": int(len(bird.triangles)), "vertices": int(pts.shape[1])})

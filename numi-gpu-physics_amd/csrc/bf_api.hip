@@ -1,0 +1,830 @@
+// C ABI of libbirdflow_hip.so (declared in include/birdflow_hip.h).
+//
+// Owns the device state of one simulation and sequences each step the way
+// BirdFlowSimulation.encodeBatch does (Sources/BirdFlowMetal/
+// BirdFlowSimulation.swift:1359-1449): geometry at t_{n+1} into the next
+// mask, fluid step current -> next lattice, load reduction, swap.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/birdflow_hip.h"
+#include "bf_launch.h"
+
+
+using namespace bf;
+
+namespace {
+constexpr uint32_t HISTORY_CAPACITY = 1u << 16;
+constexpr int PROFILE_EVENTS = 4;
+constexpr uint32_t PROFILE_CAPACITY = 4096;
+
+struct DeviceAlloc {
+    std::vector<void*> ptrs;
+    uint64_t bytes = 0;
+    void* take(hipError_t& err, size_t n) {
+        void* p = nullptr;
+        if (err != hipSuccess) return nullptr;
+        err = hipMalloc(&p, n == 0 ? 1 : n);
+        if (err == hipSuccess) {
+            ptrs.push_back(p);
+            bytes += n;
+        }
+        return p;
+    }
+    void release() {
+        for (void* p : ptrs) hipFree(p);
+        ptrs.clear();
+        bytes = 0;
+    }
+};
+}  // namespace
+
+struct bf_context {
+    bf_config cfg{};
+    Grid g{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    DeviceAlloc alloc;
+    DeviceAlloc mesh_alloc;
+    size_t pop_elem = 4;
+    void* f[2] = {nullptr, nullptr};
+    int cur = 0;                          // f[cur] holds the current lattice
+    uint8_t* mask[2] = {nullptr, nullptr};
+    float4* wall = nullptr;
+    uint8_t* dyn[2] = {nullptr, nullptr};
+    uint8_t* static_flags = nullptr;
+    uint32_t* general_list = nullptr;
+    uint32_t* counters = nullptr;         // [0] general count, [1],[2] touched counts
+    double* partials = nullptr;
+    double* block_out = nullptr;
+    double* last_load = nullptr;
+    double* history = nullptr;
+    unsigned long long* census = nullptr;
+    float* density = nullptr;
+    float4* velocity = nullptr;
+    float* staging = nullptr;             // FP32 conversion buffer for reduced storage
+    // geometry
+    int source = BF_GEOMETRY_STATIC;
+    bool static_prev_differs = false;
+    float time_start = 0.0f;
+    // mesh
+    float* m_points = nullptr;
+    float* m_times = nullptr;
+    uint32_t* m_tri = nullptr;
+    uint8_t* m_part = nullptr;
+    float4* m_pos = nullptr;
+    float4* m_vel = nullptr;
+    unsigned long long* keys = nullptr;
+    uint32_t* touched[2] = {nullptr, nullptr};
+    std::vector<float> host_times;
+    uint32_t m_frames = 0, m_vertices = 0, m_triangles = 0;
+    float m_half = 0.0f;
+    float m_translation[3] = {0, 0, 0};
+    // sphere
+    float4 sphere_cr{};
+    float3 sphere_gv{};
+    float4 sphere_wv{};
+    // stepping
+    uint64_t step = 0;
+    uint64_t history_count = 0;
+    bool invalidated = false;
+    std::string error;
+    // profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    uint32_t profiled = 0;
+    int general_blocks = 1024;
+};
+
+namespace {
+
+bf_status fail(bf_context* c, bf_status st, const std::string& msg) {
+    if (c) c->error = msg;
+    return st;
+}
+
+bf_status hip_fail(bf_context* c, hipError_t e, const char* what, bool invalidate = true) {
+    if (c) {
+        c->error = std::string(what) + ": " + hipGetErrorString(e);
+        if (invalidate) c->invalidated = true;
+    }
+    return BF_ERROR_LAUNCH_FAILED;
+}
+
+#define BF_CHECK_CTX(c)                                                                            \
+    do {                                                                                           \
+        if (!(c)) return BF_ERROR_INVALID_ARGUMENT;                                                \
+        if ((c)->invalidated) return BF_ERROR_STATE_INVALIDATED;                                   \
+        hipSetDevice((c)->device);                                                                 \
+    } while (0)
+
+bool periodic_axis(const bf_config& c, int a) {
+    return c.boundary[a][0] == BF_BC_PERIODIC && c.boundary[a][1] == BF_BC_PERIODIC;
+}
+
+StepParams make_step_params(const bf_context* c) {
+    StepParams p{};
+    p.g = c->g;
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) p.bc[a][s] = (int)c->cfg.boundary[a][s];
+    p.wall_rule = (int)c->cfg.wall_rule;
+    p.force_mode = (int)c->cfg.force_mode;
+    p.load_component = (int)c->cfg.load_component;
+    p.selected_part = (int)c->cfg.selected_part;
+    p.accumulate_loads = c->cfg.accumulate_loads ? 1 : 0;
+    p.capture_fields = 0;
+    p.omega_plus = c->cfg.omega_plus;
+    p.omega_minus = c->cfg.omega_minus;
+    p.sponge_strength = c->cfg.sponge_strength;
+    p.sponge_width = c->cfg.sponge_width;
+    for (int i = 0; i < 3; ++i) {
+        p.far_u[i] = c->cfg.far_velocity[i];
+        p.origin[i] = c->cfg.origin[i];
+        p.body[i] = c->cfg.body_position[i];
+    }
+    p.far_rho = c->cfg.far_density;
+    p.dx = c->cfg.cell_size;
+    p.force_scale = c->cfg.force_to_physical;
+    for (int q = 0; q < 19; ++q) p.far_eq[q] = equilibrium<float>(q, p.far_rho, p.far_u[0], p.far_u[1], p.far_u[2]);
+    return p;
+}
+
+// Tiles that must always take the general path: partial tiles at the +x/+y
+// edges, and tiles touching the sponge band.
+std::vector<uint8_t> static_tile_flags(const bf_config& cfg, const Grid& g) {
+    std::vector<uint8_t> flags(g.tiles, 0);
+    const int width = cfg.sponge_strength > 0.0f ? (int)std::ceil(cfg.sponge_width) : 0;
+    for (int z = 0; z < g.nz; ++z)
+        for (int ty = 0; ty < g.tiles_y; ++ty)
+            for (int tx = 0; tx < g.tiles_x; ++tx) {
+                const int x0 = tx * TILE_X, y0 = ty * TILE_Y;
+                const int x1 = std::min(x0 + TILE_X, g.nx) - 1, y1 = std::min(y0 + TILE_Y, g.ny) - 1;
+                bool general = x0 + TILE_X > g.nx || y0 + TILE_Y > g.ny;
+                if (width > 0) {
+                    // minimum over the tile's cells of the distance to the nearest face
+                    const int dxm = std::min(x0, g.nx - 1 - x1) < 0 ? 0 : std::min(x0, g.nx - 1 - x1);
+                    const int dym = std::min(y0, g.ny - 1 - y1) < 0 ? 0 : std::min(y0, g.ny - 1 - y1);
+                    const int dzm = std::min(z, g.nz - 1 - z);
+                    if (std::min(dxm, std::min(dym, dzm)) < width) general = true;
+                }
+                flags[((size_t)z * g.tiles_y + ty) * g.tiles_x + tx] = general ? 1 : 0;
+            }
+    return flags;
+}
+
+bf_status enqueue_fluid(bf_context* c, bool capture) {
+    StepParams p = make_step_params(c);
+    p.capture_fields = capture ? 1 : 0;
+    const uint32_t next_slot = (uint32_t)(c->step % HISTORY_CAPACITY);
+    const int s = (int)(c->step & 1);  // c->step already advanced to the new step
+    FluidBuffers b{};
+    b.fin = c->f[c->cur];
+    b.fout = c->f[1 - c->cur];
+    if (c->source == BF_GEOMETRY_STATIC) {
+        b.solid_cur = c->mask[0];
+        b.solid_prev = c->static_prev_differs ? c->mask[1] : c->mask[0];
+        b.dyn_cur = c->dyn[0];
+        b.dyn_prev = c->static_prev_differs ? c->dyn[1] : c->dyn[0];
+    } else {
+        b.solid_cur = c->mask[s];
+        b.solid_prev = c->mask[1 - s];
+        b.dyn_cur = c->dyn[s];
+        b.dyn_prev = c->dyn[1 - s];
+    }
+    b.wall = c->wall;
+    b.static_flags = c->static_flags;
+    b.general_list = c->general_list;
+    b.general_count = &c->counters[0];
+    b.partials = c->partials;
+    b.block_out = c->block_out;
+    b.last_load = c->last_load;
+    b.history = c->history;
+    b.density = c->density;
+    b.velocity = c->velocity;
+    hipError_t e = launch_fill_u32(&c->counters[0], 0u, 1u, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "general-list reset");
+    const bool prof = c->profiling && c->profiled < PROFILE_CAPACITY;
+    b.after_fast = prof ? c->events[(size_t)c->profiled * PROFILE_EVENTS + 2] : nullptr;
+    if (prof) hipEventRecord(c->events[(size_t)c->profiled * PROFILE_EVENTS + 1], c->stream);
+    e = launch_fluid((int)c->cfg.storage, (int)c->cfg.collision, b, p, next_slot, c->general_blocks, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
+    if (prof) hipEventRecord(c->events[(size_t)c->profiled * PROFILE_EVENTS + 3], c->stream);
+    return BF_OK;
+}
+
+MeshParams mesh_params(const bf_context* c, float t) {
+    MeshParams m{};
+    m.g = c->g;
+    for (int i = 0; i < 3; ++i) {
+        m.origin[i] = c->cfg.origin[i];
+        m.translation[i] = c->m_translation[i];
+        m.periodic[i] = periodic_axis(c->cfg, i) ? 1 : 0;
+    }
+    m.dx = c->cfg.cell_size;
+    m.half_thickness = c->m_half;
+    m.velocity_scale = c->cfg.velocity_to_lattice;
+    m.query_time = t;
+    m.frame_count = c->m_frames;
+    m.vertex_count = c->m_vertices;
+    m.triangle_count = c->m_triangles;
+    // frame interval, MeasuredBirdSurfaceSequence.swift:174-204
+    const std::vector<float>& ts = c->host_times;
+    const int last = (int)c->m_frames - 1;
+    uint32_t first;
+    if (t <= ts[0]) first = 0;
+    else if (t >= ts[last]) first = (uint32_t)(last - 1);
+    else {
+        int lo = 0, hi = last;
+        while (lo + 1 < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if (t < ts[mid]) hi = mid;
+            else lo = mid;
+        }
+        first = (uint32_t)lo;
+    }
+    m.first = first;
+    return m;
+}
+
+// Geometry of step s (s >= 0) into mask[s & 1].  rebuild: the wall field was
+// last written for this same step (geometry audit), not the previous one.
+bf_status enqueue_geometry(bf_context* c, uint64_t s, float t, bool rebuild = false) {
+    const int par = (int)(s & 1);
+    hipError_t e = hipSuccess;
+    if (c->source == BF_GEOMETRY_MESH) {
+        MeshBuffers b{};
+        b.points = c->m_points;
+        b.times = c->m_times;
+        b.tri = c->m_tri;
+        b.tri_part = c->m_part;
+        b.pos = c->m_pos;
+        b.vel = c->m_vel;
+        b.keys = c->keys;
+        b.touched_new = c->touched[par];
+        b.touched_new_count = &c->counters[1 + par];
+        b.touched_mask_old = c->touched[par];
+        b.touched_mask_old_count = &c->counters[1 + par];
+        const int wpar = rebuild ? par : 1 - par;
+        b.touched_wall_old = c->touched[wpar];
+        b.touched_wall_old_count = &c->counters[1 + wpar];
+        b.mask = c->mask[par];
+        b.wall = c->wall;
+        b.dyn = c->dyn[par];
+        e = launch_mesh_geometry(b, mesh_params(c, t), c->stream);
+    } else if (c->source == BF_GEOMETRY_SPHERE) {
+        int periodic[3];
+        for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
+        e = launch_sphere_geometry(c->mask[par], c->wall, c->dyn[par], c->g, c->sphere_cr, c->sphere_gv, c->sphere_wv,
+                                   t, periodic, c->stream);
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "geometry launch");
+    return BF_OK;
+}
+
+bf_status check_host_array(bf_context* c, const void* p, const char* name) {
+    if (!p) return fail(c, BF_ERROR_INVALID_ARGUMENT, std::string(name) + " is NULL");
+    return BF_OK;
+}
+
+}  // namespace
+
+__global__ void copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" {
+
+uint32_t bf_abi_version(void) { return BF_ABI_VERSION; }
+
+bf_status bf_copy_bandwidth(uint64_t bytes, uint32_t iterations, double* gbs) {
+    if (!gbs || bytes < 16 || iterations == 0) return BF_ERROR_INVALID_ARGUMENT;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return BF_ERROR_DEVICE_UNAVAILABLE;
+    const size_t n = (size_t)(bytes / 16);
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return BF_ERROR_ALLOCATION_FAILED;
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return BF_ERROR_ALLOCATION_FAILED;
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemset(a, 0, n * 16);
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(copy_probe, dim3(blocks), dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
+    (void)hipEventRecord(e0, 0);
+    for (uint32_t i = 0; i < iterations; ++i)
+        hipLaunchKernelGGL(copy_probe, dim3(blocks), dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return BF_ERROR_LAUNCH_FAILED;
+    *gbs = 2.0 * (double)n * 16.0 * iterations / (ms * 1e-3) / 1e9;
+    return BF_OK;
+}
+
+const char* bf_status_string(bf_status s) {
+    switch (s) {
+        case BF_OK: return "ok";
+        case BF_ERROR_DEVICE_UNAVAILABLE: return "no gfx950 device is available";
+        case BF_ERROR_INVALID_CONFIGURATION: return "invalid configuration";
+        case BF_ERROR_ALLOCATION_FAILED: return "device allocation failed";
+        case BF_ERROR_BUFFER_EXCEEDS_DEVICE_LIMIT: return "a buffer exceeds the device limit";
+        case BF_ERROR_WORKING_SET_EXCEEDS_MEMORY: return "the working set exceeds device memory";
+        case BF_ERROR_INVALID_ADVANCE_REQUEST: return "invalid advance request";
+        case BF_ERROR_INVALID_ARGUMENT: return "invalid argument";
+        case BF_ERROR_LAUNCH_FAILED: return "a device launch failed";
+        case BF_ERROR_RUNTIME_SAFETY_VIOLATION: return "runtime safety violation";
+        case BF_ERROR_STATE_INVALIDATED: return "the simulation state was invalidated by an earlier failure";
+        case BF_ERROR_NOT_SUPPORTED: return "not supported";
+    }
+    return "unknown status";
+}
+
+const char* bf_last_error(const bf_context* c) { return c ? c->error.c_str() : "null context"; }
+
+bf_status bf_device_info(char* name, size_t cap, uint64_t* total) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return BF_ERROR_DEVICE_UNAVAILABLE;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return BF_ERROR_DEVICE_UNAVAILABLE;
+    if (name && cap) {
+        std::snprintf(name, cap, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (total) *total = (uint64_t)prop.totalGlobalMem;
+    return BF_OK;
+}
+
+uint64_t bf_allocated_bytes(const bf_context* c) { return c ? c->alloc.bytes + c->mesh_alloc.bytes : 0; }
+
+bf_status bf_create(const bf_config* cfg, bf_context** out) {
+    if (!cfg || !out) return BF_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    const uint64_t n64 = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || n64 >= (1ull << 32)) return BF_ERROR_INVALID_CONFIGURATION;
+    if (cfg->storage > BF_STORAGE_BF16_OFFSET || cfg->collision > BF_COLLISION_RR3 ||
+        cfg->wall_rule > BF_WALL_INTERPOLATED_HALF || cfg->force_mode > 6 || cfg->load_component > 2)
+        return BF_ERROR_INVALID_CONFIGURATION;
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s)
+            if (cfg->boundary[a][s] > BF_BC_OUTLET) return BF_ERROR_INVALID_CONFIGURATION;
+    for (int a = 0; a < 3; ++a)
+        if ((cfg->boundary[a][0] == BF_BC_PERIODIC) != (cfg->boundary[a][1] == BF_BC_PERIODIC))
+            return BF_ERROR_INVALID_CONFIGURATION;
+    if (!(cfg->omega_plus > 0.0f && cfg->omega_plus < 2.0f && cfg->omega_minus > 0.0f && cfg->omega_minus < 2.0f) ||
+        !(cfg->cell_size > 0.0f) || !(cfg->far_density > 0.0f) || !(cfg->sponge_strength >= 0.0f) ||
+        !(cfg->sponge_strength <= 1.0f))
+        return BF_ERROR_INVALID_CONFIGURATION;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return BF_ERROR_DEVICE_UNAVAILABLE;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return BF_ERROR_DEVICE_UNAVAILABLE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return BF_ERROR_DEVICE_UNAVAILABLE;
+
+    bf_context* c = new (std::nothrow) bf_context();
+    if (!c) return BF_ERROR_ALLOCATION_FAILED;
+    c->cfg = *cfg;
+    c->device = dev;
+    Grid& g = c->g;
+    g.nx = (int)cfg->nx;
+    g.ny = (int)cfg->ny;
+    g.nz = (int)cfg->nz;
+    g.tiles_x = (g.nx + TILE_X - 1) / TILE_X;
+    g.tiles_y = (g.ny + TILE_Y - 1) / TILE_Y;
+    g.n = (uint32_t)n64;
+    g.tiles = (uint32_t)((uint64_t)g.tiles_x * g.tiles_y * g.nz);
+    c->pop_elem = cfg->storage == BF_STORAGE_FP32 ? 4 : 2;
+    c->general_blocks = std::max(1, prop.multiProcessorCount * 4);
+
+    // Working-set plan (MetalBackend.validateAllocationPlan, :77-106).
+    const uint64_t n = g.n;
+    const uint64_t pop_bytes = 19ull * n * c->pop_elem;
+    const uint64_t plan = 2 * pop_bytes + 2 * n + 16 * n + 3ull * g.tiles + 4ull * g.tiles + 64ull * g.tiles +
+                          64ull * HISTORY_CAPACITY + 4 * n + 16 * n + (c->pop_elem != 4 ? 0 : 0);
+    size_t free_b = 0, total_b = 0;
+    hipMemGetInfo(&free_b, &total_b);
+    if (plan > (uint64_t)free_b) {
+        delete c;
+        return BF_ERROR_WORKING_SET_EXCEEDS_MEMORY;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return BF_ERROR_DEVICE_UNAVAILABLE;
+    }
+    hipError_t e = hipSuccess;
+    DeviceAlloc& A = c->alloc;
+    c->f[0] = A.take(e, pop_bytes);
+    c->f[1] = A.take(e, pop_bytes);
+    c->mask[0] = (uint8_t*)A.take(e, n);
+    c->mask[1] = (uint8_t*)A.take(e, n);
+    c->wall = (float4*)A.take(e, 16 * n);
+    c->dyn[0] = (uint8_t*)A.take(e, g.tiles);
+    c->dyn[1] = (uint8_t*)A.take(e, g.tiles);
+    c->static_flags = (uint8_t*)A.take(e, g.tiles);
+    c->general_list = (uint32_t*)A.take(e, 4ull * g.tiles);
+    c->counters = (uint32_t*)A.take(e, 64);
+    c->partials = (double*)A.take(e, 64ull * g.tiles);
+    c->block_out = (double*)A.take(e, 64ull * ((g.tiles + 1023) / 1024 + 1));
+    c->last_load = (double*)A.take(e, 64);
+    c->history = (double*)A.take(e, 64ull * HISTORY_CAPACITY);
+    c->census = (unsigned long long*)A.take(e, 8 * 24);
+    c->density = (float*)A.take(e, 4 * n);
+    c->velocity = (float4*)A.take(e, 16 * n);
+    if (e != hipSuccess) {
+        std::string msg = hipGetErrorString(e);
+        A.release();
+        hipStreamDestroy(c->stream);
+        delete c;
+        (void)msg;
+        return BF_ERROR_ALLOCATION_FAILED;
+    }
+    std::vector<uint8_t> sf = static_tile_flags(*cfg, g);
+    e = hipMemcpyAsync(c->static_flags, sf.data(), sf.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8(c->mask[0], 0, g.n, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8(c->mask[1], 0, g.n, c->stream);
+    if (e == hipSuccess) e = launch_fill_wall(c->wall, g.n, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8(c->dyn[0], 0, g.tiles, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8(c->dyn[1], 0, g.tiles, c->stream);
+    if (e == hipSuccess) e = launch_fill_u32(c->counters, 0u, 16u, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->last_load, 0, 64, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->partials, 0, (uint32_t)(64ull * g.tiles), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        A.release();
+        hipStreamDestroy(c->stream);
+        delete c;
+        return BF_ERROR_LAUNCH_FAILED;
+    }
+    *out = c;
+    return BF_OK;
+}
+
+void bf_destroy(bf_context* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (hipEvent_t ev : c->events) hipEventDestroy(ev);
+    c->mesh_alloc.release();
+    c->alloc.release();
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+uint64_t bf_step_index(const bf_context* c) { return c ? c->step : 0; }
+
+bf_status bf_init_equilibrium(bf_context* c) {
+    BF_CHECK_CTX(c);
+    const uint8_t* solid = c->source == BF_GEOMETRY_STATIC ? c->mask[0] : c->mask[c->step & 1];
+    hipError_t e = launch_init_equilibrium((int)c->cfg.storage, c->f[c->cur], solid, c->wall, c->cfg.far_velocity,
+                                           c->g.n, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "initialize populations");
+    return BF_OK;
+}
+
+bf_status bf_init_shear_wave(bf_context* c, float amplitude) {
+    BF_CHECK_CTX(c);
+    hipError_t e = launch_init_shear_wave((int)c->cfg.storage, c->f[c->cur], amplitude, c->g, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "initialize shear wave");
+    return BF_OK;
+}
+
+static bf_status ensure_staging(bf_context* c) {
+    if (c->staging || c->cfg.storage == BF_STORAGE_FP32) return BF_OK;
+    hipError_t e = hipSuccess;
+    c->staging = (float*)c->alloc.take(e, 19ull * c->g.n * 4);
+    if (e != hipSuccess) return fail(c, BF_ERROR_ALLOCATION_FAILED, "population staging buffer");
+    return BF_OK;
+}
+
+bf_status bf_write_populations(bf_context* c, const float* host, uint64_t count) {
+    BF_CHECK_CTX(c);
+    if (check_host_array(c, host, "populations")) return BF_ERROR_INVALID_ARGUMENT;
+    if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population count must be 19 * N");
+    hipError_t e;
+    if (c->cfg.storage == BF_STORAGE_FP32) {
+        e = hipMemcpyAsync(c->f[c->cur], host, count * 4, hipMemcpyHostToDevice, c->stream);
+    } else {
+        if (bf_status st = ensure_staging(c)) return st;
+        e = hipMemcpyAsync(c->staging, host, count * 4, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = launch_convert_populations((int)c->cfg.storage, true, c->staging, c->f[c->cur], count, c->g.n,
+                                           c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "write populations");
+    return BF_OK;
+}
+
+bf_status bf_read_populations(bf_context* c, float* host, uint64_t count) {
+    BF_CHECK_CTX(c);
+    if (check_host_array(c, host, "populations")) return BF_ERROR_INVALID_ARGUMENT;
+    if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population count must be 19 * N");
+    hipError_t e;
+    if (c->cfg.storage == BF_STORAGE_FP32) {
+        e = hipMemcpyAsync(host, c->f[c->cur], count * 4, hipMemcpyDeviceToHost, c->stream);
+    } else {
+        if (bf_status st = ensure_staging(c)) return st;
+        e = launch_convert_populations((int)c->cfg.storage, false, c->f[c->cur], c->staging, count, c->g.n,
+                                       c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, c->staging, count * 4, hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "read populations");
+    return BF_OK;
+}
+
+bf_status bf_read_fields(bf_context* c, float* density, float* velocity3) {
+    BF_CHECK_CTX(c);
+    if (check_host_array(c, density, "density") || check_host_array(c, velocity3, "velocity"))
+        return BF_ERROR_INVALID_ARGUMENT;
+    // velocity buffer (16 B/cell) holds 3 floats per cell here
+    float* dv = (float*)c->velocity;
+    hipError_t e = launch_extract_fields((int)c->cfg.storage, c->f[c->cur], c->density, dv, c->g.n, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(density, c->density, 4ull * c->g.n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(velocity3, dv, 12ull * c->g.n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "read fields");
+    return BF_OK;
+}
+
+bf_status bf_set_geometry(bf_context* c, const uint8_t* mask_prev, const uint8_t* mask_cur, const float* wall4) {
+    BF_CHECK_CTX(c);
+    if (check_host_array(c, mask_cur, "mask_cur") || check_host_array(c, wall4, "wall"))
+        return BF_ERROR_INVALID_ARGUMENT;
+    const size_t n = c->g.n;
+    c->source = BF_GEOMETRY_STATIC;
+    hipError_t e = hipMemcpyAsync(c->mask[0], mask_cur, n, hipMemcpyHostToDevice, c->stream);
+    c->static_prev_differs = mask_prev && std::memcmp(mask_prev, mask_cur, n) != 0;
+    if (e == hipSuccess && c->static_prev_differs)
+        e = hipMemcpyAsync(c->mask[1], mask_prev, n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->wall, wall4, 16 * n, hipMemcpyHostToDevice, c->stream);
+    int periodic[3];
+    for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
+    if (e == hipSuccess) e = launch_mark_tiles(c->mask[0], c->dyn[0], c->g, periodic, true, c->stream);
+    if (e == hipSuccess && c->static_prev_differs)
+        e = launch_mark_tiles(c->mask[1], c->dyn[1], c->g, periodic, true, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "set geometry");
+    return BF_OK;
+}
+
+bf_status bf_read_geometry(bf_context* c, uint8_t* mask, float* wall4) {
+    BF_CHECK_CTX(c);
+    const size_t n = c->g.n;
+    const uint8_t* m = c->source == BF_GEOMETRY_STATIC ? c->mask[0] : c->mask[c->step & 1];
+    hipError_t e = hipSuccess;
+    if (mask) e = hipMemcpyAsync(mask, m, n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && wall4) e = hipMemcpyAsync(wall4, c->wall, 16 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "read geometry");
+    return BF_OK;
+}
+
+bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_times, uint32_t frame_count,
+                         uint32_t vertex_count, const uint32_t* triangles, const uint8_t* triangle_parts,
+                         uint32_t triangle_count, float half_thickness, const float translation[3], float time_start) {
+    BF_CHECK_CTX(c);
+    if (!points || !frame_times || !triangles || !triangle_parts)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh arrays must not be NULL");
+    if (frame_count < 2 || vertex_count == 0 || triangle_count == 0)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh needs >= 2 frames, >= 1 vertex and >= 1 triangle");
+    for (uint32_t i = 0; i + 1 < frame_count; ++i)
+        if (!(frame_times[i] < frame_times[i + 1]) || !std::isfinite(frame_times[i]) ||
+            !std::isfinite(frame_times[i + 1]))
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "frame times must be finite and strictly increasing");
+    for (uint64_t i = 0; i < 3ull * triangle_count; ++i)
+        if (triangles[i] >= vertex_count) return fail(c, BF_ERROR_INVALID_ARGUMENT, "triangle index out of range");
+    for (uint32_t t = 0; t < triangle_count; ++t)
+        if (triangle_parts[t] == 0) return fail(c, BF_ERROR_INVALID_ARGUMENT, "triangle part ids must be 1..255");
+    if (!(half_thickness > 0.0f)) return fail(c, BF_ERROR_INVALID_ARGUMENT, "half thickness must be positive");
+
+    hipStreamSynchronize(c->stream);
+    c->mesh_alloc.release();
+    DeviceAlloc& A = c->mesh_alloc;
+    hipError_t e = hipSuccess;
+    const uint64_t pts = 3ull * frame_count * vertex_count;
+    c->m_points = (float*)A.take(e, 4 * pts);
+    c->m_times = (float*)A.take(e, 4ull * frame_count);
+    c->m_tri = (uint32_t*)A.take(e, 12ull * triangle_count);
+    c->m_part = (uint8_t*)A.take(e, triangle_count);
+    c->m_pos = (float4*)A.take(e, 16ull * vertex_count);
+    c->m_vel = (float4*)A.take(e, 16ull * vertex_count);
+    c->keys = (unsigned long long*)A.take(e, 8ull * c->g.n);
+    c->touched[0] = (uint32_t*)A.take(e, 4ull * c->g.n);
+    c->touched[1] = (uint32_t*)A.take(e, 4ull * c->g.n);
+    if (e != hipSuccess) {
+        A.release();
+        return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("mesh buffers: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(c->m_points, points, 4 * pts, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->m_times, frame_times, 4ull * frame_count, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->m_tri, triangles, 12ull * triangle_count, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->m_part, triangle_parts, triangle_count, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_fill_keys(c->keys, c->g.n, c->stream);
+    // fresh geometry: both masks and the wall field cleared, lists empty
+    if (e == hipSuccess) e = launch_fill_u8(c->mask[0], 0, c->g.n, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8(c->mask[1], 0, c->g.n, c->stream);
+    if (e == hipSuccess) e = launch_fill_wall(c->wall, c->g.n, c->stream);
+    if (e == hipSuccess) e = launch_fill_u32(&c->counters[1], 0u, 2u, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8(c->dyn[0], 0, c->g.tiles, c->stream);
+    if (e == hipSuccess) e = launch_fill_u8(c->dyn[1], 0, c->g.tiles, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "mesh upload");
+    c->host_times.assign(frame_times, frame_times + frame_count);
+    c->m_frames = frame_count;
+    c->m_vertices = vertex_count;
+    c->m_triangles = triangle_count;
+    c->m_half = half_thickness;
+    for (int i = 0; i < 3; ++i) c->m_translation[i] = translation ? translation[i] : 0.0f;
+    c->source = BF_GEOMETRY_MESH;
+    c->time_start = time_start;
+    c->static_prev_differs = false;
+    // geometry of the current step; the previous mask starts equal to it
+    const uint64_t s = c->step;
+    if (bf_status st = enqueue_geometry(c, s, time_start + (float)0 * c->cfg.time_step)) return st;
+    // The previous-step mask starts equal to the current one; its touched
+    // list is copied too so that the next-but-one step clears it.
+    const int par = (int)(s & 1);
+    e = hipMemcpyAsync(c->mask[1 - par], c->mask[par], c->g.n, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->dyn[1 - par], c->dyn[par], c->g.tiles, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->touched[1 - par], c->touched[par], 4ull * c->g.n, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&c->counters[1 + (1 - par)], &c->counters[1 + par], 4, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "initial mesh geometry");
+    c->time_start = time_start - (float)((double)s * c->cfg.time_step);
+    return BF_OK;
+}
+
+bf_status bf_mesh_build(bf_context* c, float t) {
+    BF_CHECK_CTX(c);
+    if (c->source != BF_GEOMETRY_MESH) return fail(c, BF_ERROR_INVALID_ARGUMENT, "no mesh uploaded");
+    if (bf_status st = enqueue_geometry(c, c->step, t, true)) return st;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "mesh build");
+    return BF_OK;
+}
+
+bf_status bf_set_translating_sphere(bf_context* c, const float cr[4], const float gv[3], const float wv[4]) {
+    BF_CHECK_CTX(c);
+    if (!cr || !gv || !wv) return fail(c, BF_ERROR_INVALID_ARGUMENT, "sphere parameters must not be NULL");
+    c->sphere_cr = make_float4(cr[0], cr[1], cr[2], cr[3]);
+    c->sphere_gv = make_float3(gv[0], gv[1], gv[2]);
+    c->sphere_wv = make_float4(wv[0], wv[1], wv[2], wv[3]);
+    c->source = BF_GEOMETRY_SPHERE;
+    c->static_prev_differs = false;
+    c->time_start = 0.0f;
+    const uint64_t s = c->step;
+    if (bf_status st = enqueue_geometry(c, s, (float)((double)s * c->cfg.time_step))) return st;
+    hipError_t e = hipMemcpyAsync(c->mask[1 - (s & 1)], c->mask[s & 1], c->g.n, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->dyn[1 - (s & 1)], c->dyn[s & 1], c->g.tiles, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "sphere geometry");
+    return BF_OK;
+}
+
+bf_status bf_step(bf_context* c, uint32_t steps) {
+    BF_CHECK_CTX(c);
+    for (uint32_t i = 0; i < steps; ++i) {
+        const bool prof = c->profiling && c->profiled < PROFILE_CAPACITY;
+        if (prof && c->events.size() < (size_t)(c->profiled + 1) * PROFILE_EVENTS) {
+            for (int k = 0; k < PROFILE_EVENTS; ++k) {
+                hipEvent_t ev;
+                if (hipEventCreate(&ev) != hipSuccess) return fail(c, BF_ERROR_ALLOCATION_FAILED, "profile event");
+                c->events.push_back(ev);
+            }
+        }
+        if (prof) hipEventRecord(c->events[(size_t)c->profiled * PROFILE_EVENTS + 0], c->stream);
+        const uint64_t s = c->step + 1;
+        c->step = s;
+        if (c->source != BF_GEOMETRY_STATIC) {
+            const float t = c->time_start + (float)((double)s * c->cfg.time_step);
+            if (bf_status st = enqueue_geometry(c, s, t)) return st;
+        }
+        if (bf_status st = enqueue_fluid(c, false)) return st;
+        if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs) c->static_prev_differs = false;
+        c->cur = 1 - c->cur;
+        ++c->history_count;
+        if (prof) ++c->profiled;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(c, e, "step");
+    }
+    return BF_OK;
+}
+
+bf_status bf_synchronize(bf_context* c) {
+    BF_CHECK_CTX(c);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "synchronize");
+    return BF_OK;
+}
+
+bf_status bf_read_loads(bf_context* c, double out[8]) {
+    BF_CHECK_CTX(c);
+    if (!out) return BF_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipMemcpyAsync(out, c->last_load, 64, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "read loads");
+    return BF_OK;
+}
+
+bf_status bf_read_load_history(bf_context* c, double* out, uint32_t max_records, uint32_t* records) {
+    BF_CHECK_CTX(c);
+    if (!out || !records) return BF_ERROR_INVALID_ARGUMENT;
+    const uint64_t avail = std::min<uint64_t>(c->history_count, HISTORY_CAPACITY);
+    const uint64_t want = std::min<uint64_t>(avail, max_records);
+    // records are stored at slot (step % capacity); step s occupies s % cap
+    std::vector<double> all(8ull * HISTORY_CAPACITY);
+    hipError_t e = hipMemcpyAsync(all.data(), c->history, 64ull * HISTORY_CAPACITY, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "read load history");
+    for (uint64_t i = 0; i < want; ++i) {
+        const uint64_t step = c->step - want + 1 + i;
+        const uint64_t slot = step % HISTORY_CAPACITY;
+        std::memcpy(out + 8 * i, all.data() + 8 * slot, 64);
+    }
+    *records = (uint32_t)want;
+    return BF_OK;
+}
+
+bf_status bf_link_census(bf_context* c, uint64_t counts[21]) {
+    BF_CHECK_CTX(c);
+    if (!counts) return BF_ERROR_INVALID_ARGUMENT;
+    const uint8_t* cur;
+    const uint8_t* prev;
+    if (c->source == BF_GEOMETRY_STATIC) {
+        cur = c->mask[0];
+        prev = c->static_prev_differs ? c->mask[1] : c->mask[0];
+    } else {
+        cur = c->mask[c->step & 1];
+        prev = c->mask[1 - (c->step & 1)];
+    }
+    int bc[3][2];
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) bc[a][s] = (int)c->cfg.boundary[a][s];
+    hipError_t e = launch_fill_u8((uint8_t*)c->census, 0, 8 * 21, c->stream);
+    if (e == hipSuccess) e = launch_link_census(prev, cur, c->g, bc, c->census, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, c->census, 8 * 21, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "link census");
+    return BF_OK;
+}
+
+bf_status bf_set_profiling(bf_context* c, int enabled) {
+    BF_CHECK_CTX(c);
+    c->profiling = enabled != 0;
+    c->profiled = 0;
+    return BF_OK;
+}
+
+bf_status bf_read_profile(bf_context* c, double out[6]) {
+    BF_CHECK_CTX(c);
+    if (!out) return BF_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "read profile");
+    double geo = 0, fast = 0, general = 0;
+    for (uint32_t i = 0; i < c->profiled; ++i) {
+        float a = 0, b = 0, d = 0;
+        const hipEvent_t* ev = &c->events[(size_t)i * PROFILE_EVENTS];
+        hipEventElapsedTime(&a, ev[0], ev[1]);
+        hipEventElapsedTime(&b, ev[1], ev[2]);
+        hipEventElapsedTime(&d, ev[2], ev[3]);
+        geo += a;
+        fast += b;
+        general += d;
+    }
+    out[0] = geo;
+    out[1] = fast;
+    out[2] = general;
+    out[3] = (double)c->profiled;
+    uint32_t general_tiles = 0;
+    e = hipMemcpy(&general_tiles, &c->counters[0], 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read profile");
+    out[4] = (double)general_tiles;
+    out[5] = (double)c->g.tiles;
+    c->profiled = 0;
+    return BF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,388 @@
+// Moving-wall geometry for gfx950: indexed-mesh vertex preparation, raster to
+// nearest-triangle keys, resolve to part flags / wall velocity / signed
+// distance, and the canonical translating sphere.
+//
+// Reference: Sources/BirdFlowMetal/Metal/BirdFlow.metal
+//   closest point :1529-1583   prepare :1589-1622   raster :1638-1686
+//   resolve :1697-1738         clear   :1898-1911   sphere :183-195, 2307-2350
+//
+// The reference clears and resolves the whole grid every step.  Here the
+// raster appends every cell it touches for the first time to a list (the
+// 64-bit atomicMin returns the previous key, so exactly one lane sees the
+// empty key); resolve, the key reset and the next steps' clears walk only
+// that list.  Keys are (distance bin << 32 | triangle), ordering identical to
+// the reference's (bin << 12 | triangle) for meshes of up to 4,096 triangles
+// and without its 4,096-triangle limit.
+#include "bf_launch.h"
+
+namespace bf {
+
+constexpr uint64_t EMPTY_KEY = ~0ull;
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return ax * bx + ay * by + az * bz;
+}
+
+struct Closest {
+    float px, py, pz;
+    float b0, b1, b2;
+};
+
+// Voronoi-region closest point on triangle (a, b, c), BirdFlow.metal:1529-1583.
+__device__ inline Closest closest_point(float px, float py, float pz, float3 a, float3 b, float3 c) {
+    const float abx = b.x - a.x, aby = b.y - a.y, abz = b.z - a.z;
+    const float acx = c.x - a.x, acy = c.y - a.y, acz = c.z - a.z;
+    const float apx = px - a.x, apy = py - a.y, apz = pz - a.z;
+    const float d1 = dot3(abx, aby, abz, apx, apy, apz);
+    const float d2 = dot3(acx, acy, acz, apx, apy, apz);
+    if (d1 <= 0.0f && d2 <= 0.0f) return {a.x, a.y, a.z, 1.0f, 0.0f, 0.0f};
+    const float bpx = px - b.x, bpy = py - b.y, bpz = pz - b.z;
+    const float d3 = dot3(abx, aby, abz, bpx, bpy, bpz);
+    const float d4 = dot3(acx, acy, acz, bpx, bpy, bpz);
+    if (d3 >= 0.0f && d4 <= d3) return {b.x, b.y, b.z, 0.0f, 1.0f, 0.0f};
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        const float v = d1 / (d1 - d3);
+        return {a.x + v * abx, a.y + v * aby, a.z + v * abz, 1.0f - v, v, 0.0f};
+    }
+    const float cpx = px - c.x, cpy = py - c.y, cpz = pz - c.z;
+    const float d5 = dot3(abx, aby, abz, cpx, cpy, cpz);
+    const float d6 = dot3(acx, acy, acz, cpx, cpy, cpz);
+    if (d6 >= 0.0f && d5 <= d6) return {c.x, c.y, c.z, 0.0f, 0.0f, 1.0f};
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        const float w = d2 / (d2 - d6);
+        return {a.x + w * acx, a.y + w * acy, a.z + w * acz, 1.0f - w, 0.0f, w};
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        return {b.x + w * (c.x - b.x), b.y + w * (c.y - b.y), b.z + w * (c.z - b.z), 0.0f, 1.0f - w, w};
+    }
+    const float inv = 1.0f / fmaxf(va + vb + vc, 1.0e-20f);
+    const float v = vb * inv, w = vc * inv;
+    return {a.x + abx * v + acx * w, a.y + aby * v + acy * w, a.z + abz * v + acz * w, 1.0f - v - w, v, w};
+}
+
+
+// BirdFlow.metal:1589-1622.  points: frame-major float3.
+__global__ void mesh_prepare(const float* __restrict__ points, const float* __restrict__ times,
+                             float4* __restrict__ pos, float4* __restrict__ vel, MeshParams m) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m.vertex_count) return;
+    const uint32_t first = min(m.first, m.frame_count - 2u);
+    const uint32_t second = first + 1u;
+    const float duration = times[second] - times[first];
+    const float blend = fminf(fmaxf((m.query_time - times[first]) / duration, 0.0f), 1.0f);
+    const float* a = points + ((size_t)first * m.vertex_count + v) * 3;
+    const float* b = points + ((size_t)second * m.vertex_count + v) * 3;
+    const float dxv = b[0] - a[0], dyv = b[1] - a[1], dzv = b[2] - a[2];
+    pos[v] = make_float4(m.translation[0] + (a[0] + blend * dxv), m.translation[1] + (a[1] + blend * dyv),
+                         m.translation[2] + (a[2] + blend * dzv), 0.0f);
+    vel[v] = make_float4(dxv / duration * m.velocity_scale, dyv / duration * m.velocity_scale,
+                         dzv / duration * m.velocity_scale, 0.0f);
+}
+
+// Reset what an earlier step wrote: mask entries of the buffer about to be
+// rewritten (its cells are in list_mask), wall entries of the previous step
+// (list_wall).  Grid-stride over device-side counts.
+__global__ void clear_touched(const uint32_t* __restrict__ list_mask, const uint32_t* __restrict__ count_mask,
+                              uint8_t* __restrict__ mask, const uint32_t* __restrict__ list_wall,
+                              const uint32_t* __restrict__ count_wall, float4* __restrict__ wall) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t nm = *count_mask, nw = *count_wall;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += stride) mask[list_mask[i]] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += stride)
+        wall[list_wall[i]] = make_float4(0.0f, 0.0f, 0.0f, 16.0f);
+}
+
+__global__ void fill_u8(uint8_t* __restrict__ p, uint8_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_u64(uint64_t* __restrict__ p, uint64_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_wall(float4* __restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 16.0f);
+}
+
+// One wavefront per triangle; its lanes walk the expanded voxel AABB
+// (BirdFlow.metal:1650-1665 bounds).  A lane skips the atomic when its key
+// cannot win (plain load first); the atomic's result tells whether the cell
+// was empty before, in which case the cell joins the touched list.
+__global__ __launch_bounds__(256) void mesh_raster(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
+                                                   unsigned long long* __restrict__ keys,
+                                                   uint32_t* __restrict__ touched, uint32_t* __restrict__ touched_count,
+                                                   MeshParams m) {
+    const uint32_t t = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (t >= m.triangle_count) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const float4 a4 = pos[tri[3 * (size_t)t + 0]];
+    const float4 b4 = pos[tri[3 * (size_t)t + 1]];
+    const float4 c4 = pos[tri[3 * (size_t)t + 2]];
+    const float3 a = make_float3(a4.x, a4.y, a4.z), b = make_float3(b4.x, b4.y, b4.z), c = make_float3(c4.x, c4.y, c4.z);
+    const float dx = m.dx;
+    const float guard = m.half_thickness + 2.0f * dx;
+    const int n[3] = {m.g.nx, m.g.ny, m.g.nz};
+    const float av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z}, cv[3] = {c.x, c.y, c.z};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float mn = fminf(av[i], fminf(bv[i], cv[i]));
+        const float mx = fmaxf(av[i], fmaxf(bv[i], cv[i]));
+        const float ls = ((mn - guard) - m.origin[i]) / dx - 0.5f;
+        const float us = ((mx + guard) - m.origin[i]) / dx - 0.5f;
+        lo[i] = min(max((int)floorf(ls), 0), n[i] - 1);
+        hi[i] = min(max((int)ceilf(us), 0), n[i] - 1);
+    }
+    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    const uint32_t total = (uint32_t)ex * (uint32_t)ey * (uint32_t)ez;
+    for (uint32_t k = (uint32_t)lane; k < total; k += 64) {
+        const int x = lo[0] + (int)(k % (uint32_t)ex);
+        const uint32_t r = k / (uint32_t)ex;
+        const int y = lo[1] + (int)(r % (uint32_t)ey);
+        const int z = lo[2] + (int)(r / (uint32_t)ey);
+        const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
+        const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
+        const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
+        const Closest cp = closest_point(wx, wy, wz, a, b, c);
+        const float ddx = wx - cp.px, ddy = wy - cp.py, ddz = wz - cp.pz;
+        const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
+        const float scaled = roundf(dist_cells * 65536.0f);
+        const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
+        const unsigned long long key = ((unsigned long long)bin << 32) | (unsigned long long)t;
+        const uint32_t gid = (uint32_t)x + (uint32_t)m.g.nx * ((uint32_t)y + (uint32_t)m.g.ny * (uint32_t)z);
+        if (key < __hip_atomic_load(&keys[gid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            const unsigned long long old = atomicMin(&keys[gid], key);
+            if (old == EMPTY_KEY) touched[atomicAdd(touched_count, 1u)] = gid;
+        }
+    }
+}
+
+// Marks the tiles whose cells pull from, or are, solid cell (x, y, z).
+__device__ inline void mark_tiles(const Grid& g, const int periodic[3], uint8_t* __restrict__ dyn, int x, int y,
+                                  int z) {
+    int xs[2] = {x - 1, x + 1}, ys[2] = {y - 1, y + 1};
+    int zs[3] = {z - 1, z, z + 1};
+    for (int i = 0; i < 2; ++i) {
+        if (xs[i] < 0) xs[i] = periodic[0] ? g.nx - 1 : 0;
+        if (xs[i] >= g.nx) xs[i] = periodic[0] ? 0 : g.nx - 1;
+        if (ys[i] < 0) ys[i] = periodic[1] ? g.ny - 1 : 0;
+        if (ys[i] >= g.ny) ys[i] = periodic[1] ? 0 : g.ny - 1;
+    }
+    for (int i = 0; i < 3; ++i) {
+        if (zs[i] < 0) zs[i] = periodic[2] ? g.nz - 1 : 0;
+        if (zs[i] >= g.nz) zs[i] = periodic[2] ? 0 : g.nz - 1;
+    }
+    // x-1, x, x+1 lie in at most two tiles (the wrapped ones included)
+    const int txs[3] = {xs[0] / TILE_X, x / TILE_X, xs[1] / TILE_X};
+    const int tys[3] = {ys[0] / TILE_Y, y / TILE_Y, ys[1] / TILE_Y};
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i)
+                dyn[((uint32_t)zs[k] * (uint32_t)g.tiles_y + (uint32_t)tys[j]) * (uint32_t)g.tiles_x
+                    + (uint32_t)txs[i]] = 1;
+}
+
+// BirdFlow.metal:1697-1738 over the touched list: part flag when the signed
+// distance is <= 0, barycentric wall velocity, signed distance / dx.  Resets
+// the key of every resolved cell for the next step.
+__global__ void mesh_resolve(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                             const uint32_t* __restrict__ tri, const uint8_t* __restrict__ tri_part,
+                             unsigned long long* __restrict__ keys, const uint32_t* __restrict__ touched,
+                             const uint32_t* __restrict__ touched_count, uint8_t* __restrict__ mask,
+                             float4* __restrict__ wall, uint8_t* __restrict__ dyn, MeshParams m) {
+    const uint32_t count = *touched_count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint32_t gid = touched[i];
+        const unsigned long long key = keys[gid];
+        keys[gid] = EMPTY_KEY;
+        const uint32_t t = (uint32_t)(key & 0xFFFFFFFFull);
+        if (t >= m.triangle_count) continue;
+        const int x = (int)(gid % (uint32_t)m.g.nx);
+        const uint32_t yz = gid / (uint32_t)m.g.nx;
+        const int y = (int)(yz % (uint32_t)m.g.ny);
+        const int z = (int)(yz / (uint32_t)m.g.ny);
+        const float wx = m.origin[0] + ((float)x + 0.5f) * m.dx;
+        const float wy = m.origin[1] + ((float)y + 0.5f) * m.dx;
+        const float wz = m.origin[2] + ((float)z + 0.5f) * m.dx;
+        const uint32_t ia = tri[3 * (size_t)t], ib = tri[3 * (size_t)t + 1], ic = tri[3 * (size_t)t + 2];
+        const float4 a4 = pos[ia], b4 = pos[ib], c4 = pos[ic];
+        const Closest cp = closest_point(wx, wy, wz, make_float3(a4.x, a4.y, a4.z), make_float3(b4.x, b4.y, b4.z),
+                                         make_float3(c4.x, c4.y, c4.z));
+        const float4 va = vel[ia], vb = vel[ib], vc = vel[ic];
+        const float ux = cp.b0 * va.x + cp.b1 * vb.x + cp.b2 * vc.x;
+        const float uy = cp.b0 * va.y + cp.b1 * vb.y + cp.b2 * vc.y;
+        const float uz = cp.b0 * va.z + cp.b1 * vb.z + cp.b2 * vc.z;
+        const float ddx = wx - cp.px, ddy = wy - cp.py, ddz = wz - cp.pz;
+        const float sd = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) - m.half_thickness;
+        const bool solid = sd <= 0.0f;
+        mask[gid] = solid ? tri_part[t] : 0;
+        wall[gid] = make_float4(ux, uy, uz, sd / m.dx);
+        if (solid) mark_tiles(m.g, m.periodic, dyn, x, y, z);
+    }
+}
+
+// Whole-grid tile marking for host-written or canonical geometry.
+__global__ void mark_tiles_from_mask(const uint8_t* __restrict__ mask, uint8_t* __restrict__ dyn, Grid g,
+                                     int px, int py, int pz) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= g.n || mask[gid] == 0) return;
+    const int periodic[3] = {px, py, pz};
+    const int x = (int)(gid % (uint32_t)g.nx);
+    const uint32_t yz = gid / (uint32_t)g.nx;
+    mark_tiles(g, periodic, dyn, x, (int)(yz % (uint32_t)g.ny), (int)(yz / (uint32_t)g.ny));
+}
+
+// Translating sphere (BirdFlow.metal:183-195, 2307-2350): occupancy at
+// center(t) = c0 + v_geom t, uniform / tangential / normal wall velocity.
+__global__ void sphere_geometry(uint8_t* __restrict__ mask, float4* __restrict__ wall, uint8_t* __restrict__ dyn,
+                                Grid g, float4 center_radius, float3 geometry_velocity, float4 wall_velocity, float t,
+                                int px, int py, int pz) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= g.n) return;
+    const int x = (int)(gid % (uint32_t)g.nx);
+    const uint32_t yz = gid / (uint32_t)g.nx;
+    const int y = (int)(yz % (uint32_t)g.ny), z = (int)(yz / (uint32_t)g.ny);
+    const float cxv = center_radius.x + geometry_velocity.x * t;
+    const float cyv = center_radius.y + geometry_velocity.y * t;
+    const float czv = center_radius.z + geometry_velocity.z * t;
+    const float rx = (float)x + 0.5f - cxv, ry = (float)y + 0.5f - cyv, rz = (float)z + 0.5f - czv;
+    const float r = center_radius.w;
+    const bool in = dot3(rx, ry, rz, rx, ry, rz) <= r * r;
+    float wx = wall_velocity.x, wy = wall_velocity.y, wz = wall_velocity.z;
+    const int mode = (int)(fmaxf(wall_velocity.w, 0.0f) + 0.5f);
+    if (mode != 0) {
+        const float l = fmaxf(sqrtf(rx * rx + ry * ry + rz * rz), 1.0e-6f);
+        const float nxv = rx / l, nyv = ry / l, nzv = rz / l;
+        const float dn = dot3(wx, wy, wz, nxv, nyv, nzv);
+        const float nwx = dn * nxv, nwy = dn * nyv, nwz = dn * nzv;
+        if (mode == 1) {
+            wx = wx - nwx;
+            wy = wy - nwy;
+            wz = wz - nwz;
+        } else {
+            wx = nwx;
+            wy = nwy;
+            wz = nwz;
+        }
+    }
+    mask[gid] = in ? 1 : 0;
+    wall[gid] = make_float4(wx, wy, wz, 0.0f);
+    if (in) {
+        const int periodic[3] = {px, py, pz};
+        mark_tiles(g, periodic, dyn, x, y, z);
+    }
+}
+
+// Link census: per direction, (fluid, persistent) cells whose pull source is
+// solid; fresh cover and uncover counts.  counts[0..18] links, [19] covered,
+// [20] uncovered.
+__global__ void link_census(const uint8_t* __restrict__ prev, const uint8_t* __restrict__ cur, Grid g, int bcx0,
+                            int bcx1, int bcy0, int bcy1, int bcz0, int bcz1,
+                            unsigned long long* __restrict__ counts) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= g.n) return;
+    const bool s = cur[gid] != 0, w = prev[gid] != 0;
+    if (s && !w) atomicAdd(&counts[19], 1ull);
+    if (!s && w) atomicAdd(&counts[20], 1ull);
+    if (s || w) return;
+    const int x = (int)(gid % (uint32_t)g.nx);
+    const uint32_t yz = gid / (uint32_t)g.nx;
+    const int y = (int)(yz % (uint32_t)g.ny), z = (int)(yz / (uint32_t)g.ny);
+    const int bc[3][2] = {{bcx0, bcx1}, {bcy0, bcy1}, {bcz0, bcz1}};
+    const int n[3] = {g.nx, g.ny, g.nz};
+    for (int q = 1; q < 19; ++q) {
+        int sv[3] = {x - cx(q), y - cy(q), z - cz(q)};
+        bool far = false;
+        for (int a = 0; a < 3; ++a) {
+            if (sv[a] < 0) {
+                if (bc[a][0] == BC_PERIODIC) sv[a] = n[a] - 1;
+                else if (bc[a][0] == BC_OUTLET) sv[a] = 0;
+                else far = true;
+            } else if (sv[a] >= n[a]) {
+                if (bc[a][1] == BC_PERIODIC) sv[a] = 0;
+                else if (bc[a][1] == BC_OUTLET) sv[a] = n[a] - 1;
+                else far = true;
+            }
+        }
+        if (far) continue;
+        const uint32_t src = (uint32_t)sv[0] + (uint32_t)g.nx * ((uint32_t)sv[1] + (uint32_t)g.ny * (uint32_t)sv[2]);
+        if (cur[src] != 0) atomicAdd(&counts[q], 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+
+hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
+    hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
+                       b.vel, m);
+    hipLaunchKernelGGL(clear_touched, dim3(1024), dim3(256), 0, s, b.touched_mask_old, b.touched_mask_old_count,
+                       b.mask, b.touched_wall_old, b.touched_wall_old_count, b.wall);
+    hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.dyn, (uint8_t)0, m.g.tiles);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.touched_new_count, 0u, 1u);
+    if (m.triangle_count > 0)
+        hipLaunchKernelGGL(mesh_raster, dim3((m.triangle_count + 3) / 4), dim3(256), 0, s, b.pos, b.tri, b.keys,
+                           b.touched_new, b.touched_new_count, m);
+    hipLaunchKernelGGL(mesh_resolve, dim3(2048), dim3(256), 0, s, b.pos, b.vel, b.tri, b.tri_part, b.keys,
+                       b.touched_new, b.touched_new_count, b.mask, b.wall, b.dyn, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_sphere_geometry(uint8_t* mask, float4* wall, uint8_t* dyn, const Grid& g, float4 cr, float3 gv,
+                                  float4 wv, float t, const int periodic[3], hipStream_t s) {
+    hipLaunchKernelGGL(fill_u8, dim3((g.tiles + 255) / 256), dim3(256), 0, s, dyn, (uint8_t)0, g.tiles);
+    hipLaunchKernelGGL(sphere_geometry, dim3((g.n + 255) / 256), dim3(256), 0, s, mask, wall, dyn, g, cr, gv, wv, t,
+                       periodic[0], periodic[1], periodic[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_tiles(const uint8_t* mask, uint8_t* dyn, const Grid& g, const int periodic[3], bool clear,
+                             hipStream_t s) {
+    if (clear) hipLaunchKernelGGL(fill_u8, dim3((g.tiles + 255) / 256), dim3(256), 0, s, dyn, (uint8_t)0, g.tiles);
+    hipLaunchKernelGGL(mark_tiles_from_mask, dim3((g.n + 255) / 256), dim3(256), 0, s, mask, dyn, g, periodic[0],
+                       periodic[1], periodic[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fill_u8, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_keys(unsigned long long* p, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fill_u64, dim3((n + 255) / 256), dim3(256), 0, s, (uint64_t*)p, EMPTY_KEY, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_wall(float4* p, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fill_wall, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_link_census(const uint8_t* prev, const uint8_t* cur, const Grid& g, const int bc[3][2],
+                              unsigned long long* counts, hipStream_t s) {
+    hipLaunchKernelGGL(link_census, dim3((g.n + 255) / 256), dim3(256), 0, s, prev, cur, g, bc[0][0], bc[0][1],
+                       bc[1][0], bc[1][1], bc[2][0], bc[2][1], counts);
+    return hipGetLastError();
+}
+
+}  // namespace bf

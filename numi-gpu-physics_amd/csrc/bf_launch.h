@@ -1,0 +1,86 @@
+// Host-side launcher interface between the C ABI (bf_api.hip) and the
+// kernel translation units (bf_fluid.hip, bf_geometry.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bf_device.h"
+
+namespace bf {
+
+struct FluidBuffers {
+    const void* fin;
+    void* fout;
+    const uint8_t* solid_prev;
+    const uint8_t* solid_cur;
+    const float4* wall;
+    const uint8_t* static_flags;
+    const uint8_t* dyn_prev;
+    const uint8_t* dyn_cur;
+    uint32_t* general_list;
+    uint32_t* general_count;
+    double* partials;
+    double* block_out;
+    double* last_load;
+    double* history;
+    float* density;
+    float4* velocity;
+    hipEvent_t after_fast;     // recorded between the fast and general kernels when non-null
+};
+
+struct MeshParams {
+    Grid g;
+    float origin[3];
+    float dx;
+    float half_thickness;
+    float translation[3];
+    float velocity_scale;
+    float query_time;
+    uint32_t first;            // frame interval chosen on the host
+    uint32_t frame_count;
+    uint32_t vertex_count;
+    uint32_t triangle_count;
+    int periodic[3];
+};
+
+struct MeshBuffers {
+    const float* points;
+    const float* times;
+    const uint32_t* tri;
+    const uint8_t* tri_part;
+    float4* pos;
+    float4* vel;
+    unsigned long long* keys;
+    uint32_t* touched_new;
+    uint32_t* touched_new_count;
+    const uint32_t* touched_mask_old;   // cells last written into `mask`
+    const uint32_t* touched_mask_old_count;
+    const uint32_t* touched_wall_old;   // cells last written into `wall`
+    const uint32_t* touched_wall_old_count;
+    uint8_t* mask;
+    float4* wall;
+    uint8_t* dyn;
+};
+
+hipError_t launch_fluid(int storage, int coll, const FluidBuffers& b, const StepParams& p, uint32_t slot,
+                        int general_blocks, hipStream_t s);
+hipError_t launch_init_equilibrium(int storage, void* f, const uint8_t* solid, const float4* wall,
+                                   const float* far_u, uint32_t n, hipStream_t s);
+hipError_t launch_init_shear_wave(int storage, void* f, float amplitude, const Grid& g, hipStream_t s);
+hipError_t launch_extract_fields(int storage, const void* f, float* rho, float* u, uint32_t n, hipStream_t s);
+hipError_t launch_convert_populations(int storage, bool pack, const void* src, void* dst, size_t count,
+                                      uint32_t n, hipStream_t s);
+hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
+hipError_t launch_sphere_geometry(uint8_t* mask, float4* wall, uint8_t* dyn, const Grid& g, float4 cr, float3 gv,
+                                  float4 wv, float t, const int periodic[3], hipStream_t s);
+hipError_t launch_mark_tiles(const uint8_t* mask, uint8_t* dyn, const Grid& g, const int periodic[3], bool clear,
+                             hipStream_t s);
+hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint32_t n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
+hipError_t launch_fill_keys(unsigned long long* p, uint32_t n, hipStream_t s);
+hipError_t launch_fill_wall(float4* p, uint32_t n, hipStream_t s);
+hipError_t launch_link_census(const uint8_t* prev, const uint8_t* cur, const Grid& g, const int bc[3][2],
+                              unsigned long long* counts, hipStream_t s);
+
+}  // namespace bf

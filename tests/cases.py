@@ -94,3 +94,56 @@ def deetjen_frame_points(d, frame, points_by_frame):
         first = frame
         pts = np.stack([points_by_frame[frame], points_by_frame[frame]])
     return pts, first, t
+
+
+# ---------------------------------------------------------------------------
+# GPU-side helpers: a solver configuration equal to an oracle parameter block,
+# and the oracle's indexed-mesh replay loop.
+# ---------------------------------------------------------------------------
+def solver_config(p, storage=0, collision=None, time_step=1.0, velocity_to_lattice=1.0):
+    import birdflow_hip as bh
+    return bh.SolverConfig(
+        shape=(p.nx, p.ny, p.nz), omega_plus=float(p.omega_plus), omega_minus=float(p.omega_minus),
+        storage=storage, collision=p.collision if collision is None else collision, wall_rule=p.wall_rule,
+        force_mode=p.force_mode, load_component=p.load_component, selected_part=p.selected_part,
+        accumulate_loads=bool(p.accumulate_loads), boundary=[[p.bc[a][0], p.bc[a][1]] for a in range(3)],
+        origin=tuple(p.origin), cell_size=float(p.dx), time_step=time_step,
+        velocity_to_lattice=velocity_to_lattice, force_to_physical=float(p.force_scale),
+        sponge_strength=float(p.sponge_strength), sponge_width=float(p.sponge_width),
+        far_velocity=tuple(p.far_u), far_density=float(p.far_rho), body_position=tuple(p.body_position))
+
+
+def mesh_geometry_oracle(p, seq, t, velocity_scale=1.0):
+    first, _, _ = bfref.frame_interval(seq.times, t)
+    pos, vel = bfref.prepare_vertices(seq.points, seq.times, first, t, velocity_scale=velocity_scale)
+    mask, wall, _ = bfref.raster_resolve(p, pos, vel, seq.triangles, seq.parts, seq.half_thickness, key_bits=64)
+    return mask, wall
+
+
+def run_mesh_oracle(p, seq, steps, dtype=np.float32, record_masks=False):
+    """Oracle replay: geometry of step s at t = s (frame s), fluid step with the
+    previous / current masks (MetalIndexedBirdSurfaceValidation.swift:18347-18602)."""
+    mask0, wall0 = mesh_geometry_oracle(p, seq, float(seq.times[0]))
+    f = bfref.init_equilibrium(p, mask0, wall0, dtype)
+    prev = mask0
+    loads, masks = [], []
+    for s in range(1, steps + 1):
+        mask, wall = mesh_geometry_oracle(p, seq, float(seq.times[0]) + s)
+        f, ld, _, _ = bfref.step(p, f, prev, mask, wall)
+        loads.append(ld)
+        if record_masks:
+            masks.append((mask, wall))
+        prev = mask
+    return f, np.array(loads), masks
+
+
+def plate_params(shape=(96, 48, 48), collision=bfref.COLLISION_RR3, omega_plus=1.0 / 0.6, inflow=0.05):
+    """Scaled flapping-plate case: inlet (far-field equilibrium) at -x,
+    zero-gradient outlet at +x, periodic y/z, interpolated moving walls with
+    signed-distance link fractions, conservative loads."""
+    return bfref.make_params(
+        shape, omega_plus=omega_plus, omega_minus=1.0 / (0.5 + (3.0 / 16.0) / (1.0 / omega_plus - 0.5)),
+        bc=((bfref.BC_FARFIELD, bfref.BC_OUTLET), (bfref.BC_PERIODIC, bfref.BC_PERIODIC),
+            (bfref.BC_PERIODIC, bfref.BC_PERIODIC)),
+        collision=collision, wall_rule=bfref.WALL_INTERPOLATED_SDF, force_mode=6, accumulate_loads=True,
+        far_u=(inflow, 0.0, 0.0), body_position=(shape[0] / 4, shape[1] / 2, shape[2] / 2))
