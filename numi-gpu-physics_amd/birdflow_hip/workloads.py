@@ -129,17 +129,25 @@ class SyntheticBird:
                  wing_segments=(160, 40), tail_segments=(48, 24)):
         self.kin = BirdKinematics.from_seed(seed, period)
         self.center = np.asarray(center, np.float64)
+        # body: ellipsoid with single pole vertices (no zero-area triangles)
         nu, nv = body_segments
-        th = np.linspace(0, math.pi, nv + 1)
+        th = np.linspace(0, math.pi, nv + 1)[1:-1]
         ph = np.linspace(0, 2 * math.pi, nu, endpoint=False)
         T, P = np.meshgrid(th, ph, indexing="ij")
-        self.body = np.stack([body_radii[0] * np.cos(T), body_radii[1] * np.sin(T) * np.cos(P),
-                              body_radii[2] * np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
-        idx = np.arange((nv + 1) * nu).reshape(nv + 1, nu)
-        a, b = idx[:-1, :], np.roll(idx[:-1, :], -1, axis=1)
-        c, d = np.roll(idx[1:, :], -1, axis=1), idx[1:, :]
-        body_tri = np.concatenate([np.stack([a.ravel(), b.ravel(), c.ravel()], 1),
-                                   np.stack([a.ravel(), c.ravel(), d.ravel()], 1)]).astype(np.uint32)
+        rings = np.stack([body_radii[0] * np.cos(T), body_radii[1] * np.sin(T) * np.cos(P),
+                          body_radii[2] * np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
+        self.body = np.concatenate([[[body_radii[0], 0.0, 0.0]], rings, [[-body_radii[0], 0.0, 0.0]]])
+        nr = nv - 1
+        ring = lambda i, j: 1 + i * nu + (j % nu)
+        tri = []
+        for j in range(nu):
+            tri.append((0, ring(0, j + 1), ring(0, j)))
+            tri.append((len(self.body) - 1, ring(nr - 1, j), ring(nr - 1, j + 1)))
+        for i in range(nr - 1):
+            for j in range(nu):
+                a, b, c, d = ring(i, j), ring(i, j + 1), ring(i + 1, j + 1), ring(i + 1, j)
+                tri += [(a, b, c), (a, c, d)]
+        body_tri = np.asarray(tri, np.uint32)
         su, sv, wing_tri = _grid_sheet(*wing_segments)
         self.span_fraction = su
         chord_len = root_chord + (tip_chord - root_chord) * su
@@ -147,7 +155,7 @@ class SyntheticBird:
         self.root = np.array([0.0, body_radii[1] * 0.8, body_radii[2] * 0.3])
         tu, tv, tail_tri = _grid_sheet(*tail_segments)
         fan = (tu - 0.5) * math.radians(60.0)
-        rad = tv * tail[0]
+        rad = 2.0 + tv * (tail[0] - 2.0)
         width_scale = tail[1] / (2 * tail[0] * math.sin(math.radians(30.0)))
         self.tail_local = np.stack([-body_radii[0] * 0.9 - rad * np.cos(fan), rad * np.sin(fan) * width_scale * 2,
                                     np.zeros_like(tu)], -1)

@@ -2,8 +2,20 @@
 //
 // Owns the device state of one simulation and sequences each step the way
 // BirdFlowSimulation.encodeBatch does (Sources/BirdFlowMetal/
-// BirdFlowSimulation.swift:1359-1449): geometry at t_{n+1} into the next
-// mask, fluid step current -> next lattice, load reduction, swap.
+// BirdFlowSimulation.swift:1359-1449): geometry of step n+1 into the next
+// mask, fluid step current -> next lattice with the previous / current
+// masks, load reduction, swap.
+//
+// Two HIP streams form a software pipeline, per step s:
+//   stream A  fluid_fast(s)                              (HBM-bound, wall-free tiles)
+//   stream B  geometry(s+1) + tile list(s+1)  [overlaps fast(s)],
+//             then fluid_general(s) + load reduction      [after fast(s)]
+// fast(s) waits for event R[(s-1)&1] (general(s-1) done, which follows
+// geometry(s) on B); general(s) waits for F[s&1] (fast(s) done).  Geometry of
+// step s+1 only writes buffers that steps s-1 and s-2 last read: masks and
+// solid-proximity tile flags rotate over 3 buffers, the wall field and the
+// general-tile list over 2, the touched-tile flags over 4.  Every entry point
+// other than bf_step drains both streams first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,13 +29,12 @@
 #include "../../include/birdflow_hip.h"
 #include "bf_launch.h"
 
-
 using namespace bf;
 
 namespace {
 constexpr uint32_t HISTORY_CAPACITY = 1u << 16;
-constexpr int PROFILE_EVENTS = 4;
 constexpr uint32_t PROFILE_CAPACITY = 4096;
+constexpr int PROFILE_EVENTS = 6;   // A: fast start/end; B: geometry start/end, general start/end
 
 struct DeviceAlloc {
     std::vector<void*> ptrs;
@@ -39,29 +50,39 @@ struct DeviceAlloc {
         return p;
     }
     void release() {
-        for (void* p : ptrs) hipFree(p);
+        for (void* p : ptrs) (void)hipFree(p);
         ptrs.clear();
         bytes = 0;
     }
 };
+
+inline int mod3(int64_t s) { return (int)(((s % 3) + 3) % 3); }
+inline int mod4(int64_t s) { return (int)(((s % 4) + 4) % 4); }
+inline int par2(int64_t s) { return (int)(s & 1); }
 }  // namespace
 
 struct bf_context {
     bf_config cfg{};
     Grid g{};
-    hipStream_t stream = nullptr;
+    hipStream_t sa = nullptr;             // fast fluid
+    hipStream_t sb = nullptr;             // general fluid, reductions, geometry, everything else
+    hipEvent_t ev_fast[2] = {nullptr, nullptr};
+    hipEvent_t ev_ready[2] = {nullptr, nullptr};
     int device = 0;
     DeviceAlloc alloc;
     DeviceAlloc mesh_alloc;
     size_t pop_elem = 4;
     void* f[2] = {nullptr, nullptr};
     int cur = 0;                          // f[cur] holds the current lattice
-    uint8_t* mask[2] = {nullptr, nullptr};
-    float4* wall = nullptr;
-    uint8_t* dyn[2] = {nullptr, nullptr};
+    uint8_t* mask[3] = {nullptr, nullptr, nullptr};
+    float4* wall[2] = {nullptr, nullptr};
+    uint8_t* dyn[3] = {nullptr, nullptr, nullptr};
     uint8_t* static_flags = nullptr;
-    uint32_t* general_list = nullptr;
-    uint32_t* counters = nullptr;         // [0] general count, [1],[2] touched counts
+    uint8_t* tflags[4] = {nullptr, nullptr, nullptr, nullptr};   // tiles keyed at step s -> tflags[s % 4]
+    uint8_t* no_tiles = nullptr;          // all-zero tile flags
+    uint32_t* general_list[2] = {nullptr, nullptr};
+    uint32_t* geo_list = nullptr;
+    uint32_t* counters = nullptr;         // [0],[1] general-list counts by step parity, [2] geometry tile list
     double* partials = nullptr;
     double* block_out = nullptr;
     double* last_load = nullptr;
@@ -73,7 +94,8 @@ struct bf_context {
     // geometry
     int source = BF_GEOMETRY_STATIC;
     bool static_prev_differs = false;
-    float time_start = 0.0f;
+    float time_start = 0.0f;              // time of step 0
+    int64_t geometry_ready = 0;           // highest step whose geometry is enqueued
     // mesh
     float* m_points = nullptr;
     float* m_times = nullptr;
@@ -82,7 +104,6 @@ struct bf_context {
     float4* m_pos = nullptr;
     float4* m_vel = nullptr;
     unsigned long long* keys = nullptr;
-    uint32_t* touched[2] = {nullptr, nullptr};
     std::vector<float> host_times;
     uint32_t m_frames = 0, m_vertices = 0, m_triangles = 0;
     float m_half = 0.0f;
@@ -92,7 +113,7 @@ struct bf_context {
     float3 sphere_gv{};
     float4 sphere_wv{};
     // stepping
-    uint64_t step = 0;
+    int64_t step = 0;
     uint64_t history_count = 0;
     bool invalidated = false;
     std::string error;
@@ -118,16 +139,24 @@ bf_status hip_fail(bf_context* c, hipError_t e, const char* what, bool invalidat
     return BF_ERROR_LAUNCH_FAILED;
 }
 
-#define BF_CHECK_CTX(c)                                                                            \
-    do {                                                                                           \
-        if (!(c)) return BF_ERROR_INVALID_ARGUMENT;                                                \
-        if ((c)->invalidated) return BF_ERROR_STATE_INVALIDATED;                                   \
-        hipSetDevice((c)->device);                                                                 \
-    } while (0)
-
 bool periodic_axis(const bf_config& c, int a) {
     return c.boundary[a][0] == BF_BC_PERIODIC && c.boundary[a][1] == BF_BC_PERIODIC;
 }
+
+hipError_t drain(bf_context* c) {
+    hipError_t e = hipStreamSynchronize(c->sa);
+    hipError_t e2 = hipStreamSynchronize(c->sb);
+    return e != hipSuccess ? e : e2;
+}
+
+#define BF_ENTER(c)                                                                                \
+    do {                                                                                           \
+        if (!(c)) return BF_ERROR_INVALID_ARGUMENT;                                                \
+        if ((c)->invalidated) return BF_ERROR_STATE_INVALIDATED;                                   \
+        (void)hipSetDevice((c)->device);                                                           \
+        hipError_t drain_err_ = drain(c);                                                          \
+        if (drain_err_ != hipSuccess) return hip_fail((c), drain_err_, "earlier device work");     \
+    } while (0)
 
 StepParams make_step_params(const bf_context* c) {
     StepParams p{};
@@ -156,8 +185,8 @@ StepParams make_step_params(const bf_context* c) {
     return p;
 }
 
-// Tiles that must always take the general path: partial tiles at the +x/+y
-// edges, and tiles touching the sponge band.
+// Tiles that always take the general path: partial tiles at the +x / +y
+// edges and tiles touching the sponge band.
 std::vector<uint8_t> static_tile_flags(const bf_config& cfg, const Grid& g) {
     std::vector<uint8_t> flags(g.tiles, 0);
     const int width = cfg.sponge_strength > 0.0f ? (int)std::ceil(cfg.sponge_width) : 0;
@@ -168,9 +197,8 @@ std::vector<uint8_t> static_tile_flags(const bf_config& cfg, const Grid& g) {
                 const int x1 = std::min(x0 + TILE_X, g.nx) - 1, y1 = std::min(y0 + TILE_Y, g.ny) - 1;
                 bool general = x0 + TILE_X > g.nx || y0 + TILE_Y > g.ny;
                 if (width > 0) {
-                    // minimum over the tile's cells of the distance to the nearest face
-                    const int dxm = std::min(x0, g.nx - 1 - x1) < 0 ? 0 : std::min(x0, g.nx - 1 - x1);
-                    const int dym = std::min(y0, g.ny - 1 - y1) < 0 ? 0 : std::min(y0, g.ny - 1 - y1);
+                    const int dxm = std::max(0, std::min(x0, g.nx - 1 - x1));
+                    const int dym = std::max(0, std::min(y0, g.ny - 1 - y1));
                     const int dzm = std::min(z, g.nz - 1 - z);
                     if (std::min(dxm, std::min(dym, dzm)) < width) general = true;
                 }
@@ -179,47 +207,25 @@ std::vector<uint8_t> static_tile_flags(const bf_config& cfg, const Grid& g) {
     return flags;
 }
 
-bf_status enqueue_fluid(bf_context* c, bool capture) {
-    StepParams p = make_step_params(c);
-    p.capture_fields = capture ? 1 : 0;
-    const uint32_t next_slot = (uint32_t)(c->step % HISTORY_CAPACITY);
-    const int s = (int)(c->step & 1);  // c->step already advanced to the new step
-    FluidBuffers b{};
-    b.fin = c->f[c->cur];
-    b.fout = c->f[1 - c->cur];
+// Geometry views of step s.
+struct StepGeometry {
+    const uint8_t* mask_cur;
+    const uint8_t* mask_prev;
+    const float4* wall;
+    const uint8_t* dyn_cur;
+    const uint8_t* dyn_prev;
+};
+
+StepGeometry geometry_of(const bf_context* c, int64_t s) {
     if (c->source == BF_GEOMETRY_STATIC) {
-        b.solid_cur = c->mask[0];
-        b.solid_prev = c->static_prev_differs ? c->mask[1] : c->mask[0];
-        b.dyn_cur = c->dyn[0];
-        b.dyn_prev = c->static_prev_differs ? c->dyn[1] : c->dyn[0];
-    } else {
-        b.solid_cur = c->mask[s];
-        b.solid_prev = c->mask[1 - s];
-        b.dyn_cur = c->dyn[s];
-        b.dyn_prev = c->dyn[1 - s];
+        (void)s;
+        const bool differs = c->static_prev_differs;   // only the first step after bf_set_geometry
+        return {c->mask[0], differs ? c->mask[1] : c->mask[0], c->wall[0], c->dyn[0], differs ? c->dyn[1] : c->dyn[0]};
     }
-    b.wall = c->wall;
-    b.static_flags = c->static_flags;
-    b.general_list = c->general_list;
-    b.general_count = &c->counters[0];
-    b.partials = c->partials;
-    b.block_out = c->block_out;
-    b.last_load = c->last_load;
-    b.history = c->history;
-    b.density = c->density;
-    b.velocity = c->velocity;
-    hipError_t e = launch_fill_u32(&c->counters[0], 0u, 1u, c->stream);
-    if (e != hipSuccess) return hip_fail(c, e, "general-list reset");
-    const bool prof = c->profiling && c->profiled < PROFILE_CAPACITY;
-    b.after_fast = prof ? c->events[(size_t)c->profiled * PROFILE_EVENTS + 2] : nullptr;
-    if (prof) hipEventRecord(c->events[(size_t)c->profiled * PROFILE_EVENTS + 1], c->stream);
-    e = launch_fluid((int)c->cfg.storage, (int)c->cfg.collision, b, p, next_slot, c->general_blocks, c->stream);
-    if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
-    if (prof) hipEventRecord(c->events[(size_t)c->profiled * PROFILE_EVENTS + 3], c->stream);
-    return BF_OK;
+    return {c->mask[mod3(s)], c->mask[mod3(s - 1)], c->wall[par2(s)], c->dyn[mod3(s)], c->dyn[mod3(s - 1)]};
 }
 
-MeshParams mesh_params(const bf_context* c, float t) {
+MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     MeshParams m{};
     m.g = c->g;
     for (int i = 0; i < 3; ++i) {
@@ -234,6 +240,10 @@ MeshParams mesh_params(const bf_context* c, float t) {
     m.frame_count = c->m_frames;
     m.vertex_count = c->m_vertices;
     m.triangle_count = c->m_triangles;
+    // Stepping keys only the band that the fluid step reads: solid cells
+    // (distance <= h), fluid cells one link away from them (<= h + sqrt(2)),
+    // and cells uncovered this step (<= h + one step's wall travel).
+    m.cull_cells = exact ? INFINITY : c->m_half / c->cfg.cell_size + 1.5f;
     // frame interval, MeasuredBirdSurfaceSequence.swift:174-204
     const std::vector<float>& ts = c->host_times;
     const int last = (int)c->m_frames - 1;
@@ -253,10 +263,14 @@ MeshParams mesh_params(const bf_context* c, float t) {
     return m;
 }
 
-// Geometry of step s (s >= 0) into mask[s & 1].  rebuild: the wall field was
-// last written for this same step (geometry audit), not the previous one.
-bf_status enqueue_geometry(bf_context* c, uint64_t s, float t, bool rebuild = false) {
-    const int par = (int)(s & 1);
+float time_of(const bf_context* c, int64_t s) {
+    return c->time_start + (float)((double)s * c->cfg.time_step);
+}
+
+// Enqueue on stream B the geometry of step s (dynamic sources).  fresh: the
+// mask and wall buffers of step s hold no entries from earlier steps (they
+// were cleared), so no stale tiles need clearing.
+bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
     hipError_t e = hipSuccess;
     if (c->source == BF_GEOMETRY_MESH) {
         MeshBuffers b{};
@@ -267,25 +281,56 @@ bf_status enqueue_geometry(bf_context* c, uint64_t s, float t, bool rebuild = fa
         b.pos = c->m_pos;
         b.vel = c->m_vel;
         b.keys = c->keys;
-        b.touched_new = c->touched[par];
-        b.touched_new_count = &c->counters[1 + par];
-        b.touched_mask_old = c->touched[par];
-        b.touched_mask_old_count = &c->counters[1 + par];
-        const int wpar = rebuild ? par : 1 - par;
-        b.touched_wall_old = c->touched[wpar];
-        b.touched_wall_old_count = &c->counters[1 + wpar];
-        b.mask = c->mask[par];
-        b.wall = c->wall;
-        b.dyn = c->dyn[par];
-        e = launch_mesh_geometry(b, mesh_params(c, t), c->stream);
+        b.tiles_new = c->tflags[mod4(s)];
+        b.tiles_mask_old = fresh ? c->no_tiles : c->tflags[mod4(s - 3)];   // mask[s%3] last written at s-3
+        b.tiles_wall_old = fresh ? c->no_tiles : c->tflags[mod4(s - 2)];   // wall[s&1] last written at s-2
+        b.list = c->geo_list;
+        b.list_count = &c->counters[2];
+        b.mask = c->mask[mod3(s)];
+        b.wall = c->wall[par2(s)];
+        b.dyn = c->dyn[mod3(s)];
+        e = launch_mesh_geometry(b, mesh_params(c, time_of(c, s), exact), c->sb);
     } else if (c->source == BF_GEOMETRY_SPHERE) {
         int periodic[3];
         for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
-        e = launch_sphere_geometry(c->mask[par], c->wall, c->dyn[par], c->g, c->sphere_cr, c->sphere_gv, c->sphere_wv,
-                                   t, periodic, c->stream);
+        e = launch_sphere_geometry(c->mask[mod3(s)], c->wall[par2(s)], c->dyn[mod3(s)], c->g, c->sphere_cr,
+                                   c->sphere_gv, c->sphere_wv, time_of(c, s), periodic, c->sb);
     }
     if (e != hipSuccess) return hip_fail(c, e, "geometry launch");
     return BF_OK;
+}
+
+// List of the tiles that take the general path in step s (stream B).
+bf_status enqueue_tile_list(bf_context* c, int64_t s) {
+    const StepGeometry sg = geometry_of(c, s);
+    hipError_t e = launch_fill_u32(&c->counters[par2(s)], 0u, 1u, c->sb);
+    if (e == hipSuccess)
+        e = launch_build_general_list(c->static_flags, sg.dyn_prev, sg.dyn_cur, c->g.tiles, c->general_list[par2(s)],
+                                      &c->counters[par2(s)], c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "tile list launch");
+    return BF_OK;
+}
+
+FluidBuffers fluid_buffers(const bf_context* c, int64_t s) {
+    const StepGeometry sg = geometry_of(c, s);
+    FluidBuffers b{};
+    b.fin = c->f[c->cur];
+    b.fout = c->f[1 - c->cur];
+    b.solid_cur = sg.mask_cur;
+    b.solid_prev = sg.mask_prev;
+    b.wall = sg.wall;
+    b.dyn_cur = sg.dyn_cur;
+    b.dyn_prev = sg.dyn_prev;
+    b.static_flags = c->static_flags;
+    b.general_list = c->general_list[par2(s)];
+    b.general_count = &c->counters[par2(s)];
+    b.partials = c->partials;
+    b.block_out = c->block_out;
+    b.last_load = c->last_load;
+    b.history = c->history;
+    b.density = c->density;
+    b.velocity = c->velocity;
+    return b;
 }
 
 bf_status check_host_array(bf_context* c, const void* p, const char* name) {
@@ -296,8 +341,8 @@ bf_status check_host_array(bf_context* c, const void* p, const char* name) {
 }  // namespace
 
 __global__ void copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 extern "C" {
@@ -309,6 +354,7 @@ bf_status bf_copy_bandwidth(uint64_t bytes, uint32_t iterations, double* gbs) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return BF_ERROR_DEVICE_UNAVAILABLE;
     const size_t n = (size_t)(bytes / 16);
+    if ((n + 255) / 256 > 0x7fffffffull) return BF_ERROR_INVALID_ARGUMENT;
     void *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, n * 16) != hipSuccess) return BF_ERROR_ALLOCATION_FAILED;
     if (hipMalloc(&b, n * 16) != hipSuccess) {
@@ -319,7 +365,7 @@ bf_status bf_copy_bandwidth(uint64_t bytes, uint32_t iterations, double* gbs) {
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipMemset(a, 0, n * 16);
-    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 32);
+    const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(copy_probe, dim3(blocks), dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
     (void)hipEventRecord(e0, 0);
     for (uint32_t i = 0; i < iterations; ++i)
@@ -361,12 +407,10 @@ bf_status bf_device_info(char* name, size_t cap, uint64_t* total) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return BF_ERROR_DEVICE_UNAVAILABLE;
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return BF_ERROR_DEVICE_UNAVAILABLE;
-    if (name && cap) {
-        std::snprintf(name, cap, "%s (%s)", prop.name, prop.gcnArchName);
-    }
+    if (name && cap) std::snprintf(name, cap, "%s (%s)", prop.name, prop.gcnArchName);
     if (total) *total = (uint64_t)prop.totalGlobalMem;
     return BF_OK;
 }
@@ -395,7 +439,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return BF_ERROR_DEVICE_UNAVAILABLE;
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return BF_ERROR_DEVICE_UNAVAILABLE;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return BF_ERROR_DEVICE_UNAVAILABLE;
@@ -418,29 +462,39 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     // Working-set plan (MetalBackend.validateAllocationPlan, :77-106).
     const uint64_t n = g.n;
     const uint64_t pop_bytes = 19ull * n * c->pop_elem;
-    const uint64_t plan = 2 * pop_bytes + 2 * n + 16 * n + 3ull * g.tiles + 4ull * g.tiles + 64ull * g.tiles +
-                          64ull * HISTORY_CAPACITY + 4 * n + 16 * n + (c->pop_elem != 4 ? 0 : 0);
+    const uint64_t plan = 2 * pop_bytes + 3 * n + 32 * n + 3ull * g.tiles + 4ull * g.tiles + 64ull * g.tiles +
+                          64ull * HISTORY_CAPACITY + 4 * n + 16 * n;
     size_t free_b = 0, total_b = 0;
-    hipMemGetInfo(&free_b, &total_b);
+    (void)hipMemGetInfo(&free_b, &total_b);
     if (plan > (uint64_t)free_b) {
         delete c;
         return BF_ERROR_WORKING_SET_EXCEEDS_MEMORY;
     }
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+    // Both streams at the default priority.
+    if (hipStreamCreateWithFlags(&c->sa, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->sb, hipStreamNonBlocking) != hipSuccess) {
+        bf_destroy(c);
         return BF_ERROR_DEVICE_UNAVAILABLE;
+    }
+    for (int i = 0; i < 2; ++i) {
+        if (hipEventCreateWithFlags(&c->ev_fast[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_ready[i], hipEventDisableTiming) != hipSuccess) {
+            bf_destroy(c);
+            return BF_ERROR_DEVICE_UNAVAILABLE;
+        }
     }
     hipError_t e = hipSuccess;
     DeviceAlloc& A = c->alloc;
     c->f[0] = A.take(e, pop_bytes);
     c->f[1] = A.take(e, pop_bytes);
-    c->mask[0] = (uint8_t*)A.take(e, n);
-    c->mask[1] = (uint8_t*)A.take(e, n);
-    c->wall = (float4*)A.take(e, 16 * n);
-    c->dyn[0] = (uint8_t*)A.take(e, g.tiles);
-    c->dyn[1] = (uint8_t*)A.take(e, g.tiles);
+    for (int i = 0; i < 3; ++i) c->mask[i] = (uint8_t*)A.take(e, n);
+    for (int i = 0; i < 2; ++i) c->wall[i] = (float4*)A.take(e, 16 * n);
+    for (int i = 0; i < 3; ++i) c->dyn[i] = (uint8_t*)A.take(e, g.tiles);
     c->static_flags = (uint8_t*)A.take(e, g.tiles);
-    c->general_list = (uint32_t*)A.take(e, 4ull * g.tiles);
+    for (int i = 0; i < 4; ++i) c->tflags[i] = (uint8_t*)A.take(e, g.tiles);
+    c->no_tiles = (uint8_t*)A.take(e, g.tiles);
+    for (int i = 0; i < 2; ++i) c->general_list[i] = (uint32_t*)A.take(e, 4ull * g.tiles);
+    c->geo_list = (uint32_t*)A.take(e, 4ull * g.tiles);
     c->counters = (uint32_t*)A.take(e, 64);
     c->partials = (double*)A.take(e, 64ull * g.tiles);
     c->block_out = (double*)A.take(e, 64ull * ((g.tiles + 1023) / 1024 + 1));
@@ -450,28 +504,23 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     c->density = (float*)A.take(e, 4 * n);
     c->velocity = (float4*)A.take(e, 16 * n);
     if (e != hipSuccess) {
-        std::string msg = hipGetErrorString(e);
-        A.release();
-        hipStreamDestroy(c->stream);
-        delete c;
-        (void)msg;
+        bf_destroy(c);
         return BF_ERROR_ALLOCATION_FAILED;
     }
     std::vector<uint8_t> sf = static_tile_flags(*cfg, g);
-    e = hipMemcpyAsync(c->static_flags, sf.data(), sf.size(), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8(c->mask[0], 0, g.n, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8(c->mask[1], 0, g.n, c->stream);
-    if (e == hipSuccess) e = launch_fill_wall(c->wall, g.n, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8(c->dyn[0], 0, g.tiles, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8(c->dyn[1], 0, g.tiles, c->stream);
-    if (e == hipSuccess) e = launch_fill_u32(c->counters, 0u, 16u, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->last_load, 0, 64, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->partials, 0, (uint32_t)(64ull * g.tiles), c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipStream_t s = c->sb;
+    e = hipMemcpyAsync(c->static_flags, sf.data(), sf.size(), hipMemcpyHostToDevice, s);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->mask[i], 0, g.n, s);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_fill_wall(c->wall[i], g.n, s);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->dyn[i], 0, g.tiles, s);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = launch_fill_u8(c->tflags[i], 0, g.tiles, s);
+    if (e == hipSuccess) e = launch_fill_u8(c->no_tiles, 0, g.tiles, s);
+    if (e == hipSuccess) e = launch_fill_u32(c->counters, 0u, 16u, s);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->last_load, 0, 64, s);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->partials, 0, (uint32_t)(64ull * g.tiles), s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
-        A.release();
-        hipStreamDestroy(c->stream);
-        delete c;
+        bf_destroy(c);
         return BF_ERROR_LAUNCH_FAILED;
     }
     *out = c;
@@ -480,29 +529,37 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
 
 void bf_destroy(bf_context* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (hipEvent_t ev : c->events) hipEventDestroy(ev);
+    (void)hipSetDevice(c->device);
+    if (c->sa) (void)hipStreamSynchronize(c->sa);
+    if (c->sb) (void)hipStreamSynchronize(c->sb);
+    for (hipEvent_t ev : c->events) (void)hipEventDestroy(ev);
+    for (int i = 0; i < 2; ++i) {
+        if (c->ev_fast[i]) (void)hipEventDestroy(c->ev_fast[i]);
+        if (c->ev_ready[i]) (void)hipEventDestroy(c->ev_ready[i]);
+    }
     c->mesh_alloc.release();
     c->alloc.release();
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->sa) (void)hipStreamDestroy(c->sa);
+    if (c->sb) (void)hipStreamDestroy(c->sb);
     delete c;
 }
 
-uint64_t bf_step_index(const bf_context* c) { return c ? c->step : 0; }
+uint64_t bf_step_index(const bf_context* c) { return c ? (uint64_t)c->step : 0; }
 
 bf_status bf_init_equilibrium(bf_context* c) {
-    BF_CHECK_CTX(c);
-    const uint8_t* solid = c->source == BF_GEOMETRY_STATIC ? c->mask[0] : c->mask[c->step & 1];
-    hipError_t e = launch_init_equilibrium((int)c->cfg.storage, c->f[c->cur], solid, c->wall, c->cfg.far_velocity,
-                                           c->g.n, c->stream);
+    BF_ENTER(c);
+    const StepGeometry sg = geometry_of(c, c->step);
+    hipError_t e = launch_init_equilibrium((int)c->cfg.storage, c->f[c->cur], sg.mask_cur, sg.wall,
+                                           c->cfg.far_velocity, c->g.n, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "initialize populations");
     return BF_OK;
 }
 
 bf_status bf_init_shear_wave(bf_context* c, float amplitude) {
-    BF_CHECK_CTX(c);
-    hipError_t e = launch_init_shear_wave((int)c->cfg.storage, c->f[c->cur], amplitude, c->g, c->stream);
+    BF_ENTER(c);
+    hipError_t e = launch_init_shear_wave((int)c->cfg.storage, c->f[c->cur], amplitude, c->g, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "initialize shear wave");
     return BF_OK;
 }
@@ -516,85 +573,82 @@ static bf_status ensure_staging(bf_context* c) {
 }
 
 bf_status bf_write_populations(bf_context* c, const float* host, uint64_t count) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (check_host_array(c, host, "populations")) return BF_ERROR_INVALID_ARGUMENT;
     if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population count must be 19 * N");
     hipError_t e;
     if (c->cfg.storage == BF_STORAGE_FP32) {
-        e = hipMemcpyAsync(c->f[c->cur], host, count * 4, hipMemcpyHostToDevice, c->stream);
+        e = hipMemcpyAsync(c->f[c->cur], host, count * 4, hipMemcpyHostToDevice, c->sb);
     } else {
         if (bf_status st = ensure_staging(c)) return st;
-        e = hipMemcpyAsync(c->staging, host, count * 4, hipMemcpyHostToDevice, c->stream);
+        e = hipMemcpyAsync(c->staging, host, count * 4, hipMemcpyHostToDevice, c->sb);
         if (e == hipSuccess)
-            e = launch_convert_populations((int)c->cfg.storage, true, c->staging, c->f[c->cur], count, c->g.n,
-                                           c->stream);
+            e = launch_convert_populations((int)c->cfg.storage, true, c->staging, c->f[c->cur], count, c->g.n, c->sb);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "write populations");
     return BF_OK;
 }
 
 bf_status bf_read_populations(bf_context* c, float* host, uint64_t count) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (check_host_array(c, host, "populations")) return BF_ERROR_INVALID_ARGUMENT;
     if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population count must be 19 * N");
     hipError_t e;
     if (c->cfg.storage == BF_STORAGE_FP32) {
-        e = hipMemcpyAsync(host, c->f[c->cur], count * 4, hipMemcpyDeviceToHost, c->stream);
+        e = hipMemcpyAsync(host, c->f[c->cur], count * 4, hipMemcpyDeviceToHost, c->sb);
     } else {
         if (bf_status st = ensure_staging(c)) return st;
-        e = launch_convert_populations((int)c->cfg.storage, false, c->f[c->cur], c->staging, count, c->g.n,
-                                       c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(host, c->staging, count * 4, hipMemcpyDeviceToHost, c->stream);
+        e = launch_convert_populations((int)c->cfg.storage, false, c->f[c->cur], c->staging, count, c->g.n, c->sb);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, c->staging, count * 4, hipMemcpyDeviceToHost, c->sb);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "read populations");
     return BF_OK;
 }
 
 bf_status bf_read_fields(bf_context* c, float* density, float* velocity3) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (check_host_array(c, density, "density") || check_host_array(c, velocity3, "velocity"))
         return BF_ERROR_INVALID_ARGUMENT;
-    // velocity buffer (16 B/cell) holds 3 floats per cell here
-    float* dv = (float*)c->velocity;
-    hipError_t e = launch_extract_fields((int)c->cfg.storage, c->f[c->cur], c->density, dv, c->g.n, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(density, c->density, 4ull * c->g.n, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(velocity3, dv, 12ull * c->g.n, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float* dv = (float*)c->velocity;   // 16 B per cell; holds 3 floats per cell here
+    hipError_t e = launch_extract_fields((int)c->cfg.storage, c->f[c->cur], c->density, dv, c->g.n, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(density, c->density, 4ull * c->g.n, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(velocity3, dv, 12ull * c->g.n, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "read fields");
     return BF_OK;
 }
 
 bf_status bf_set_geometry(bf_context* c, const uint8_t* mask_prev, const uint8_t* mask_cur, const float* wall4) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (check_host_array(c, mask_cur, "mask_cur") || check_host_array(c, wall4, "wall"))
         return BF_ERROR_INVALID_ARGUMENT;
     const size_t n = c->g.n;
     c->source = BF_GEOMETRY_STATIC;
-    hipError_t e = hipMemcpyAsync(c->mask[0], mask_cur, n, hipMemcpyHostToDevice, c->stream);
     c->static_prev_differs = mask_prev && std::memcmp(mask_prev, mask_cur, n) != 0;
-    if (e == hipSuccess && c->static_prev_differs)
-        e = hipMemcpyAsync(c->mask[1], mask_prev, n, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->wall, wall4, 16 * n, hipMemcpyHostToDevice, c->stream);
+    hipStream_t s = c->sb;
+    hipError_t e = hipMemcpyAsync(c->mask[0], mask_cur, n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && c->static_prev_differs) e = hipMemcpyAsync(c->mask[1], mask_prev, n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->wall[0], wall4, 16 * n, hipMemcpyHostToDevice, s);
     int periodic[3];
     for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
-    if (e == hipSuccess) e = launch_mark_tiles(c->mask[0], c->dyn[0], c->g, periodic, true, c->stream);
-    if (e == hipSuccess && c->static_prev_differs)
-        e = launch_mark_tiles(c->mask[1], c->dyn[1], c->g, periodic, true, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = launch_mark_tiles(c->mask[0], c->dyn[0], c->g, periodic, true, s);
+    if (e == hipSuccess && c->static_prev_differs) e = launch_mark_tiles(c->mask[1], c->dyn[1], c->g, periodic, true, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "set geometry");
+    c->geometry_ready = c->step;
     return BF_OK;
 }
 
 bf_status bf_read_geometry(bf_context* c, uint8_t* mask, float* wall4) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
+    const StepGeometry sg = geometry_of(c, c->step);
     const size_t n = c->g.n;
-    const uint8_t* m = c->source == BF_GEOMETRY_STATIC ? c->mask[0] : c->mask[c->step & 1];
     hipError_t e = hipSuccess;
-    if (mask) e = hipMemcpyAsync(mask, m, n, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && wall4) e = hipMemcpyAsync(wall4, c->wall, 16 * n, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (mask) e = hipMemcpyAsync(mask, sg.mask_cur, n, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess && wall4) e = hipMemcpyAsync(wall4, sg.wall, 16 * n, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "read geometry");
     return BF_OK;
 }
@@ -602,7 +656,7 @@ bf_status bf_read_geometry(bf_context* c, uint8_t* mask, float* wall4) {
 bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_times, uint32_t frame_count,
                          uint32_t vertex_count, const uint32_t* triangles, const uint8_t* triangle_parts,
                          uint32_t triangle_count, float half_thickness, const float translation[3], float time_start) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (!points || !frame_times || !triangles || !triangle_parts)
         return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh arrays must not be NULL");
     if (frame_count < 2 || vertex_count == 0 || triangle_count == 0)
@@ -611,13 +665,25 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
         if (!(frame_times[i] < frame_times[i + 1]) || !std::isfinite(frame_times[i]) ||
             !std::isfinite(frame_times[i + 1]))
             return fail(c, BF_ERROR_INVALID_ARGUMENT, "frame times must be finite and strictly increasing");
-    for (uint64_t i = 0; i < 3ull * triangle_count; ++i)
-        if (triangles[i] >= vertex_count) return fail(c, BF_ERROR_INVALID_ARGUMENT, "triangle index out of range");
-    for (uint32_t t = 0; t < triangle_count; ++t)
+    for (uint64_t i = 0; i < 3ull * frame_count * vertex_count; ++i)
+        if (!std::isfinite(points[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "vertex positions must be finite");
+    for (uint32_t t = 0; t < triangle_count; ++t) {
+        const uint32_t* v = triangles + 3ull * t;
+        if (v[0] >= vertex_count || v[1] >= vertex_count || v[2] >= vertex_count)
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "triangle index out of range");
         if (triangle_parts[t] == 0) return fail(c, BF_ERROR_INVALID_ARGUMENT, "triangle part ids must be 1..255");
+        // A zero-area triangle makes the closest-point branches divide 0 by 0.
+        const float* a = points + 3ull * v[0];
+        const float* b = points + 3ull * v[1];
+        const float* d = points + 3ull * v[2];
+        const double ux = (double)b[0] - a[0], uy = (double)b[1] - a[1], uz = (double)b[2] - a[2];
+        const double wx = (double)d[0] - a[0], wy = (double)d[1] - a[1], wz = (double)d[2] - a[2];
+        const double nx = uy * wz - uz * wy, ny = uz * wx - ux * wz, nz = ux * wy - uy * wx;
+        if (!(nx * nx + ny * ny + nz * nz > 0.0))
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "degenerate (zero-area) triangle in the first frame");
+    }
     if (!(half_thickness > 0.0f)) return fail(c, BF_ERROR_INVALID_ARGUMENT, "half thickness must be positive");
 
-    hipStreamSynchronize(c->stream);
     c->mesh_alloc.release();
     DeviceAlloc& A = c->mesh_alloc;
     hipError_t e = hipSuccess;
@@ -629,24 +695,22 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     c->m_pos = (float4*)A.take(e, 16ull * vertex_count);
     c->m_vel = (float4*)A.take(e, 16ull * vertex_count);
     c->keys = (unsigned long long*)A.take(e, 8ull * c->g.n);
-    c->touched[0] = (uint32_t*)A.take(e, 4ull * c->g.n);
-    c->touched[1] = (uint32_t*)A.take(e, 4ull * c->g.n);
     if (e != hipSuccess) {
         A.release();
+        c->source = BF_GEOMETRY_STATIC;
         return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("mesh buffers: ") + hipGetErrorString(e));
     }
-    e = hipMemcpyAsync(c->m_points, points, 4 * pts, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->m_times, frame_times, 4ull * frame_count, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->m_tri, triangles, 12ull * triangle_count, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->m_part, triangle_parts, triangle_count, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = launch_fill_keys(c->keys, c->g.n, c->stream);
-    // fresh geometry: both masks and the wall field cleared, lists empty
-    if (e == hipSuccess) e = launch_fill_u8(c->mask[0], 0, c->g.n, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8(c->mask[1], 0, c->g.n, c->stream);
-    if (e == hipSuccess) e = launch_fill_wall(c->wall, c->g.n, c->stream);
-    if (e == hipSuccess) e = launch_fill_u32(&c->counters[1], 0u, 2u, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8(c->dyn[0], 0, c->g.tiles, c->stream);
-    if (e == hipSuccess) e = launch_fill_u8(c->dyn[1], 0, c->g.tiles, c->stream);
+    hipStream_t s = c->sb;
+    e = hipMemcpyAsync(c->m_points, points, 4 * pts, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->m_times, frame_times, 4ull * frame_count, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->m_tri, triangles, 12ull * triangle_count, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->m_part, triangle_parts, triangle_count, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_fill_keys(c->keys, c->g.n, s);
+    // fresh geometry state: masks, walls and flags cleared, lists empty
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->mask[i], 0, c->g.n, s);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_fill_wall(c->wall[i], c->g.n, s);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->dyn[i], 0, c->g.tiles, s);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = launch_fill_u8(c->tflags[i], 0, c->g.tiles, s);
     if (e != hipSuccess) return hip_fail(c, e, "mesh upload");
     c->host_times.assign(frame_times, frame_times + frame_count);
     c->m_frames = frame_count;
@@ -655,38 +719,46 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     c->m_half = half_thickness;
     for (int i = 0; i < 3; ++i) c->m_translation[i] = translation ? translation[i] : 0.0f;
     c->source = BF_GEOMETRY_MESH;
-    c->time_start = time_start;
     c->static_prev_differs = false;
-    // geometry of the current step; the previous mask starts equal to it
-    const uint64_t s = c->step;
-    if (bf_status st = enqueue_geometry(c, s, time_start + (float)0 * c->cfg.time_step)) return st;
-    // The previous-step mask starts equal to the current one; its touched
-    // list is copied too so that the next-but-one step clears it.
-    const int par = (int)(s & 1);
-    e = hipMemcpyAsync(c->mask[1 - par], c->mask[par], c->g.n, hipMemcpyDeviceToDevice, c->stream);
+    const int64_t st = c->step;
+    c->time_start = time_start - (float)((double)st * c->cfg.time_step);
+    // Geometry of the current step with the reference's full key band; the
+    // previous step's mask starts equal to it (its touched-tile flags are
+    // copied so that the step that reuses that buffer clears it).
+    if (bf_status r = enqueue_geometry(c, st, true, true)) return r;
+    const int k = mod3(st), kp = mod3(st - 1);
+    e = hipMemcpyAsync(c->mask[kp], c->mask[k], c->g.n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->dyn[kp], c->dyn[k], c->g.tiles, hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(c->dyn[1 - par], c->dyn[par], c->g.tiles, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(c->touched[1 - par], c->touched[par], 4ull * c->g.n, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(&c->counters[1 + (1 - par)], &c->counters[1 + par], 4, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        e = hipMemcpyAsync(c->tflags[mod4(st - 1)], c->tflags[mod4(st)], c->g.tiles, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "initial mesh geometry");
-    c->time_start = time_start - (float)((double)s * c->cfg.time_step);
+    c->geometry_ready = st;
     return BF_OK;
 }
 
 bf_status bf_mesh_build(bf_context* c, float t) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (c->source != BF_GEOMETRY_MESH) return fail(c, BF_ERROR_INVALID_ARGUMENT, "no mesh uploaded");
-    if (bf_status st = enqueue_geometry(c, c->step, t, true)) return st;
-    hipError_t e = hipStreamSynchronize(c->stream);
+    // Rebuild the current step's geometry at time t with the full key band
+    // (geometry audit).  A prefetched next-step geometry stays valid: it
+    // neither reads nor writes the current step's buffers.
+    const int64_t st = c->step;
+    hipError_t e0 = launch_fill_u8(c->mask[mod3(st)], 0, c->g.n, c->sb);
+    if (e0 == hipSuccess) e0 = launch_fill_wall(c->wall[par2(st)], c->g.n, c->sb);
+    if (e0 != hipSuccess) return hip_fail(c, e0, "mesh build");
+    const float saved = c->time_start;
+    c->time_start = t - (float)((double)st * c->cfg.time_step);
+    bf_status r = enqueue_geometry(c, st, true, true);
+    c->time_start = saved;
+    if (r) return r;
+    hipError_t e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "mesh build");
     return BF_OK;
 }
 
 bf_status bf_set_translating_sphere(bf_context* c, const float cr[4], const float gv[3], const float wv[4]) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (!cr || !gv || !wv) return fail(c, BF_ERROR_INVALID_ARGUMENT, "sphere parameters must not be NULL");
     c->sphere_cr = make_float4(cr[0], cr[1], cr[2], cr[3]);
     c->sphere_gv = make_float3(gv[0], gv[1], gv[2]);
@@ -694,132 +766,164 @@ bf_status bf_set_translating_sphere(bf_context* c, const float cr[4], const floa
     c->source = BF_GEOMETRY_SPHERE;
     c->static_prev_differs = false;
     c->time_start = 0.0f;
-    const uint64_t s = c->step;
-    if (bf_status st = enqueue_geometry(c, s, (float)((double)s * c->cfg.time_step))) return st;
-    hipError_t e = hipMemcpyAsync(c->mask[1 - (s & 1)], c->mask[s & 1], c->g.n, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(c->dyn[1 - (s & 1)], c->dyn[s & 1], c->g.tiles, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    const int64_t st = c->step;
+    if (bf_status r = enqueue_geometry(c, st, true, false)) return r;
+    const int k = mod3(st), kp = mod3(st - 1);
+    hipError_t e = hipMemcpyAsync(c->mask[kp], c->mask[k], c->g.n, hipMemcpyDeviceToDevice, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->dyn[kp], c->dyn[k], c->g.tiles, hipMemcpyDeviceToDevice, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "sphere geometry");
+    c->geometry_ready = st;
     return BF_OK;
 }
 
 bf_status bf_step(bf_context* c, uint32_t steps) {
-    BF_CHECK_CTX(c);
+    if (!c) return BF_ERROR_INVALID_ARGUMENT;
+    if (c->invalidated) return BF_ERROR_STATE_INVALIDATED;
+    (void)hipSetDevice(c->device);
+    if (steps == 0) return BF_OK;
+    const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision;
+    hipError_t e = hipSuccess;
+    // Prologue: geometry (unless already built) and tile list of the first
+    // step of this call; event R of the previous step then releases fast(s).
+    {
+        const int64_t s = c->step + 1;
+        if (c->source != BF_GEOMETRY_STATIC && c->geometry_ready < s) {
+            if (bf_status r = enqueue_geometry(c, s, false, false)) return r;
+        }
+        if (bf_status r = enqueue_tile_list(c, s)) return r;
+        e = hipEventRecord(c->ev_ready[par2(s - 1)], c->sb);
+        if (e != hipSuccess) return hip_fail(c, e, "event record");
+        c->geometry_ready = s;
+    }
     for (uint32_t i = 0; i < steps; ++i) {
+        const int64_t s = c->step + 1;
         const bool prof = c->profiling && c->profiled < PROFILE_CAPACITY;
-        if (prof && c->events.size() < (size_t)(c->profiled + 1) * PROFILE_EVENTS) {
-            for (int k = 0; k < PROFILE_EVENTS; ++k) {
+        hipEvent_t* pe = nullptr;
+        if (prof) {
+            while (c->events.size() < (size_t)(c->profiled + 1) * PROFILE_EVENTS) {
                 hipEvent_t ev;
                 if (hipEventCreate(&ev) != hipSuccess) return fail(c, BF_ERROR_ALLOCATION_FAILED, "profile event");
                 c->events.push_back(ev);
             }
+            pe = &c->events[(size_t)c->profiled * PROFILE_EVENTS];
         }
-        if (prof) hipEventRecord(c->events[(size_t)c->profiled * PROFILE_EVENTS + 0], c->stream);
-        const uint64_t s = c->step + 1;
+        const StepParams p = make_step_params(c);
+        const FluidBuffers fb = fluid_buffers(c, s);
+        if (c->source == BF_GEOMETRY_STATIC) c->static_prev_differs = false;
+        // stream A: wall-free tiles of step s once general(s-1) (and with it
+        // geometry(s)) is done
+        e = hipStreamWaitEvent(c->sa, c->ev_ready[par2(s - 1)], 0);
+        if (prof && e == hipSuccess) e = hipEventRecord(pe[0], c->sa);
+        if (e == hipSuccess) e = launch_fluid_fast(storage, coll, fb, p, c->sa);
+        if (prof && e == hipSuccess) e = hipEventRecord(pe[1], c->sa);
+        if (e == hipSuccess) e = hipEventRecord(c->ev_fast[par2(s)], c->sa);
+        if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
+        // stream B: geometry and tile list of step s+1 while fast(s) runs ...
+        if (prof) e = hipEventRecord(pe[2], c->sb);
+        if (e == hipSuccess && c->source != BF_GEOMETRY_STATIC) {
+            if (bf_status r = enqueue_geometry(c, s + 1, false, false)) return r;
+        }
+        if (bf_status r = enqueue_tile_list(c, s + 1)) return r;
+        if (prof) e = hipEventRecord(pe[3], c->sb);
+        // ... then the wall tiles of step s once fast(s) is done
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, c->ev_fast[par2(s)], 0);
+        if (prof && e == hipSuccess) e = hipEventRecord(pe[4], c->sb);
+        if (e == hipSuccess)
+            e = launch_fluid_general(storage, coll, fb, p, (uint32_t)(s % HISTORY_CAPACITY), c->general_blocks, c->sb);
+        if (prof && e == hipSuccess) e = hipEventRecord(pe[5], c->sb);
+        if (e == hipSuccess) e = hipEventRecord(c->ev_ready[par2(s)], c->sb);
+        if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
         c->step = s;
-        if (c->source != BF_GEOMETRY_STATIC) {
-            const float t = c->time_start + (float)((double)s * c->cfg.time_step);
-            if (bf_status st = enqueue_geometry(c, s, t)) return st;
-        }
-        if (bf_status st = enqueue_fluid(c, false)) return st;
-        if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs) c->static_prev_differs = false;
         c->cur = 1 - c->cur;
         ++c->history_count;
+        c->geometry_ready = s + 1;
         if (prof) ++c->profiled;
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return hip_fail(c, e, "step");
     }
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(c, e, "step");
     return BF_OK;
 }
 
 bf_status bf_synchronize(bf_context* c) {
-    BF_CHECK_CTX(c);
-    hipError_t e = hipStreamSynchronize(c->stream);
+    if (!c) return BF_ERROR_INVALID_ARGUMENT;
+    if (c->invalidated) return BF_ERROR_STATE_INVALIDATED;
+    (void)hipSetDevice(c->device);
+    hipError_t e = drain(c);
     if (e != hipSuccess) return hip_fail(c, e, "synchronize");
     return BF_OK;
 }
 
 bf_status bf_read_loads(bf_context* c, double out[8]) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (!out) return BF_ERROR_INVALID_ARGUMENT;
-    hipError_t e = hipMemcpyAsync(out, c->last_load, 64, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipError_t e = hipMemcpyAsync(out, c->last_load, 64, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "read loads");
     return BF_OK;
 }
 
 bf_status bf_read_load_history(bf_context* c, double* out, uint32_t max_records, uint32_t* records) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (!out || !records) return BF_ERROR_INVALID_ARGUMENT;
     const uint64_t avail = std::min<uint64_t>(c->history_count, HISTORY_CAPACITY);
     const uint64_t want = std::min<uint64_t>(avail, max_records);
-    // records are stored at slot (step % capacity); step s occupies s % cap
     std::vector<double> all(8ull * HISTORY_CAPACITY);
-    hipError_t e = hipMemcpyAsync(all.data(), c->history, 64ull * HISTORY_CAPACITY, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipError_t e = hipMemcpyAsync(all.data(), c->history, 64ull * HISTORY_CAPACITY, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "read load history");
     for (uint64_t i = 0; i < want; ++i) {
-        const uint64_t step = c->step - want + 1 + i;
-        const uint64_t slot = step % HISTORY_CAPACITY;
-        std::memcpy(out + 8 * i, all.data() + 8 * slot, 64);
+        const uint64_t step = (uint64_t)c->step - want + 1 + i;
+        std::memcpy(out + 8 * i, all.data() + 8 * (step % HISTORY_CAPACITY), 64);
     }
     *records = (uint32_t)want;
     return BF_OK;
 }
 
 bf_status bf_link_census(bf_context* c, uint64_t counts[21]) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (!counts) return BF_ERROR_INVALID_ARGUMENT;
-    const uint8_t* cur;
-    const uint8_t* prev;
-    if (c->source == BF_GEOMETRY_STATIC) {
-        cur = c->mask[0];
-        prev = c->static_prev_differs ? c->mask[1] : c->mask[0];
-    } else {
-        cur = c->mask[c->step & 1];
-        prev = c->mask[1 - (c->step & 1)];
-    }
+    const StepGeometry sg = geometry_of(c, c->step);
+    const uint8_t* prev = sg.mask_prev;
+    if (c->source == BF_GEOMETRY_STATIC) prev = c->static_prev_differs ? c->mask[1] : c->mask[0];
     int bc[3][2];
     for (int a = 0; a < 3; ++a)
         for (int s = 0; s < 2; ++s) bc[a][s] = (int)c->cfg.boundary[a][s];
-    hipError_t e = launch_fill_u8((uint8_t*)c->census, 0, 8 * 21, c->stream);
-    if (e == hipSuccess) e = launch_link_census(prev, cur, c->g, bc, c->census, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(counts, c->census, 8 * 21, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipError_t e = launch_fill_u8((uint8_t*)c->census, 0, 8 * 21, c->sb);
+    if (e == hipSuccess) e = launch_link_census(prev, sg.mask_cur, c->g, bc, c->census, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, c->census, 8 * 21, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "link census");
     return BF_OK;
 }
 
 bf_status bf_set_profiling(bf_context* c, int enabled) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     c->profiling = enabled != 0;
     c->profiled = 0;
     return BF_OK;
 }
 
 bf_status bf_read_profile(bf_context* c, double out[6]) {
-    BF_CHECK_CTX(c);
+    BF_ENTER(c);
     if (!out) return BF_ERROR_INVALID_ARGUMENT;
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return hip_fail(c, e, "read profile");
     double geo = 0, fast = 0, general = 0;
     for (uint32_t i = 0; i < c->profiled; ++i) {
         float a = 0, b = 0, d = 0;
         const hipEvent_t* ev = &c->events[(size_t)i * PROFILE_EVENTS];
-        hipEventElapsedTime(&a, ev[0], ev[1]);
-        hipEventElapsedTime(&b, ev[1], ev[2]);
-        hipEventElapsedTime(&d, ev[2], ev[3]);
-        geo += a;
-        fast += b;
-        general += d;
+        (void)hipEventElapsedTime(&a, ev[0], ev[1]);   // fast kernel (stream A)
+        (void)hipEventElapsedTime(&d, ev[2], ev[3]);   // next step's geometry + tile list (stream B)
+        (void)hipEventElapsedTime(&b, ev[4], ev[5]);   // general + reduction (stream B)
+        fast += a;
+        general += b;
+        geo += d;
     }
     out[0] = geo;
     out[1] = fast;
     out[2] = general;
     out[3] = (double)c->profiled;
     uint32_t general_tiles = 0;
-    e = hipMemcpy(&general_tiles, &c->counters[0], 4, hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpy(&general_tiles, &c->counters[par2(c->step)], 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(c, e, "read profile");
     out[4] = (double)general_tiles;
     out[5] = (double)c->g.tiles;

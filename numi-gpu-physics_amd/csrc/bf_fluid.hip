@@ -133,21 +133,28 @@ __device__ __forceinline__ void collide(const float (&f)[19], float rho, float u
             }
         });
     } else {
+        // Regularized / RR3 (BirdFlow.metal:2917-3062).  Terms whose factor is
+        // a zero component of c_q are dropped at compile time: each is an
+        // exact +-0 that leaves the running sum unchanged, so the result is
+        // bit-identical to the full expression while the instruction count
+        // falls by about half.
         constexpr bool RR3 = COLL == COLL_RR3;
         const float c2 = cs2<float>();
+        const float h_one = 1.0f - c2;   // c_a c_a - cs2 for c_a = +-1
+        const float h_zero = 0.0f - c2;  // and for c_a = 0
         float dg0 = 0.0f, dg1 = 0.0f, dg2 = 0.0f, od0 = 0.0f, od1 = 0.0f, od2 = 0.0f;
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            float neq = f[q] - feq[q];
-            // (c_a c_a - cs2) is -cs2 or 1 - cs2; the products keep the
-            // reference's rounding (neq * H, then accumulate).
-            constexpr float ax = (float)(cx(q) * cx(q)), ay = (float)(cy(q) * cy(q)), az = (float)(cz(q) * cz(q));
-            dg0 = dg0 + neq * (ax - c2);
-            dg1 = dg1 + neq * (ay - c2);
-            dg2 = dg2 + neq * (az - c2);
-            od0 = od0 + neq * (float)(cx(q) * cy(q));
-            od1 = od1 + neq * (float)(cx(q) * cz(q));
-            od2 = od2 + neq * (float)(cy(q) * cz(q));
+            const float neq = f[q] - feq[q];
+            dg0 = dg0 + neq * (cx(q) != 0 ? h_one : h_zero);
+            dg1 = dg1 + neq * (cy(q) != 0 ? h_one : h_zero);
+            dg2 = dg2 + neq * (cz(q) != 0 ? h_one : h_zero);
+            if constexpr (cx(q) * cy(q) > 0) od0 = od0 + neq;
+            if constexpr (cx(q) * cy(q) < 0) od0 = od0 - neq;
+            if constexpr (cx(q) * cz(q) > 0) od1 = od1 + neq;
+            if constexpr (cx(q) * cz(q) < 0) od1 = od1 - neq;
+            if constexpr (cy(q) * cz(q) > 0) od2 = od2 + neq;
+            if constexpr (cy(q) * cz(q) < 0) od2 = od2 - neq;
         });
         float aXxy = 2.0f * ux * od0 + uy * dg0;
         float aXxz = 2.0f * ux * od1 + uz * dg0;
@@ -159,20 +166,32 @@ __device__ __forceinline__ void collide(const float (&f)[19], float rho, float u
         float scale = 1.0f;
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const float fx = (float)cx(q), fy = (float)cy(q), fz = (float)cz(q);
-            float hxx = fx * fx - c2, hyy = fy * fy - c2, hzz = fz * fz - c2;
-            float contraction = (hxx * dg0 + hyy * dg1 + hzz * dg2)
-                + 2.0f * ((fx * fy) * od0 + (fx * fz) * od1 + (fy * fz) * od2);
-            float w = weight<float>(q);
+            constexpr int X = cx(q), Y = cy(q), Z = cz(q);
+            const float hxx = X != 0 ? h_one : h_zero, hyy = Y != 0 ? h_one : h_zero, hzz = Z != 0 ? h_one : h_zero;
+            float contraction = hxx * dg0 + hyy * dg1 + hzz * dg2;
+            // at most one off-diagonal product c_a c_b is nonzero
+            if constexpr (X * Y != 0) contraction = contraction + 2.0f * (X * Y > 0 ? od0 : -od0);
+            if constexpr (X * Z != 0) contraction = contraction + 2.0f * (X * Z > 0 ? od1 : -od1);
+            if constexpr (Y * Z != 0) contraction = contraction + 2.0f * (Y * Z > 0 ? od2 : -od2);
+            const float w = weight<float>(q);
             float reg = w * 4.5f * contraction;
-            if constexpr (RR3) {
-                float third = fy * hxx * aXxy + fz * hxx * aXxz + fx * hyy * aXyy
-                    + fx * hzz * aXzz + fz * hyy * aYyz + fy * hzz * aYzz;
+            if constexpr (RR3 && q != 0) {
+                // sum in the reference's order of the terms whose leading
+                // velocity component is nonzero
+                float third = 0.0f;
+                bool any = false;
+                auto add = [&](float t) { third = any ? third + t : t; any = true; };
+                if constexpr (Y != 0) add((Y > 0 ? hxx : -hxx) * aXxy);
+                if constexpr (Z != 0) add((Z > 0 ? hxx : -hxx) * aXxz);
+                if constexpr (X != 0) add((X > 0 ? hyy : -hyy) * aXyy);
+                if constexpr (X != 0) add((X > 0 ? hzz : -hzz) * aXzz);
+                if constexpr (Z != 0) add((Z > 0 ? hyy : -hyy) * aYyz);
+                if constexpr (Y != 0) add((Y > 0 ? hzz : -hzz) * aYzz);
                 reg = reg + w * 13.5f * third;
             }
-            float cand = feq[q] + (1.0f - op) * reg;
+            const float cand = feq[q] + (1.0f - op) * reg;
             unb[q] = cand;
-            float floorv = fmaxf(1.0e-12f, 1.0e-6f * fmaxf(feq[q], 0.0f));
+            const float floorv = fmaxf(1.0e-12f, 1.0e-6f * fmaxf(feq[q], 0.0f));
             if (cand < floorv) {
                 float adm = (feq[q] - floorv) / fmaxf(feq[q] - cand, 1.0e-30f);
                 adm = fminf(fmaxf(adm, 0.0f), 1.0f);
@@ -245,15 +264,10 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
                                                   const uint8_t* __restrict__ static_flags,
                                                   const uint8_t* __restrict__ dyn_prev,
                                                   const uint8_t* __restrict__ dyn_cur,
-                                                  uint32_t* __restrict__ general_list,
-                                                  uint32_t* __restrict__ general_count,
                                                   float* __restrict__ density, float4* __restrict__ velocity,
                                                   StepParams p) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    if ((static_flags[tile] | dyn_prev[tile] | dyn_cur[tile]) != 0) {
-        if (threadIdx.x == 0) general_list[atomicAdd(general_count, 1u)] = tile;
-        return;
-    }
+    if ((static_flags[tile] | dyn_prev[tile] | dyn_cur[tile]) != 0) return;   // fluid_general owns it
     int x0, y0, z;
     tile_origin(p.g, tile, x0, y0, z);
     const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
@@ -293,6 +307,25 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
     float crho, cmx, cmy, cmz;
     write_post<S>(fout, n, gid, post, 0.0f, p, crho, cmx, cmy, cmz);
     if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
+}
+
+// Work list of the tiles that take the general path this step: flagged by
+// the static mask (partial or sponge tiles) or by solid cells of the
+// previous / current geometry.  Appends are aggregated per wavefront.
+__global__ __launch_bounds__(256) void build_general_list(const uint8_t* __restrict__ static_flags,
+                                                          const uint8_t* __restrict__ dyn_prev,
+                                                          const uint8_t* __restrict__ dyn_cur, uint32_t tiles,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool flagged = t < tiles && (static_flags[t] | dyn_prev[t] | dyn_cur[t]) != 0;
+    const unsigned long long b = __ballot(flagged);
+    if (b == 0) return;
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(count, (uint32_t)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (flagged)
+        list[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -723,13 +756,17 @@ __global__ void unpack_populations(const typename Store<S>::T* __restrict__ src,
 // ---------------------------------------------------------------------------
 
 template <int COLL, int S>
-static hipError_t launch_fluid_t(const FluidBuffers& b, const StepParams& p, uint32_t history_slot,
-                                 int general_blocks, hipStream_t s) {
+static hipError_t launch_fast_t(const FluidBuffers& b, const StepParams& p, hipStream_t s) {
     using T = typename Store<S>::T;
     hipLaunchKernelGGL((fluid_fast<COLL, S>), dim3(p.g.tiles), dim3(256), 0, s, (const T*)b.fin, (T*)b.fout,
-                       b.static_flags, b.dyn_prev, b.dyn_cur, b.general_list, b.general_count, b.density,
-                       b.velocity, p);
-    if (b.after_fast) hipEventRecord(b.after_fast, s);
+                       b.static_flags, b.dyn_prev, b.dyn_cur, b.density, b.velocity, p);
+    return hipGetLastError();
+}
+
+template <int COLL, int S>
+static hipError_t launch_general_t(const FluidBuffers& b, const StepParams& p, uint32_t history_slot,
+                                   int general_blocks, hipStream_t s) {
+    using T = typename Store<S>::T;
     hipLaunchKernelGGL((fluid_general<COLL, S>), dim3(general_blocks), dim3(256), 0, s, (const T*)b.fin,
                        (T*)b.fout, b.solid_prev, b.solid_cur, b.wall, b.general_list, b.general_count,
                        b.partials, b.density, b.velocity, p);
@@ -743,26 +780,41 @@ static hipError_t launch_fluid_t(const FluidBuffers& b, const StepParams& p, uin
     return hipGetLastError();
 }
 
-template <int S>
-static hipError_t launch_fluid_s(int coll, const FluidBuffers& b, const StepParams& p, uint32_t slot, int gb,
-                                 hipStream_t s) {
-    switch (coll) {
-        case COLL_TRT: return launch_fluid_t<COLL_TRT, S>(b, p, slot, gb, s);
-        case COLL_TRT_SYMMETRIC_LIMITER: return launch_fluid_t<COLL_TRT_SYMMETRIC_LIMITER, S>(b, p, slot, gb, s);
-        case COLL_REGULARIZED: return launch_fluid_t<COLL_REGULARIZED, S>(b, p, slot, gb, s);
-        case COLL_RR3: return launch_fluid_t<COLL_RR3, S>(b, p, slot, gb, s);
-        default: return hipErrorInvalidValue;
+#define BF_DISPATCH(storage, coll, CALL)                                                              \
+    switch (storage * 4 + coll) {                                                                     \
+        case 0: return CALL(COLL_TRT, STORE_FP32);                                                    \
+        case 1: return CALL(COLL_TRT_SYMMETRIC_LIMITER, STORE_FP32);                                  \
+        case 2: return CALL(COLL_REGULARIZED, STORE_FP32);                                            \
+        case 3: return CALL(COLL_RR3, STORE_FP32);                                                    \
+        case 4: return CALL(COLL_TRT, STORE_FP16_OFFSET);                                             \
+        case 5: return CALL(COLL_TRT_SYMMETRIC_LIMITER, STORE_FP16_OFFSET);                           \
+        case 6: return CALL(COLL_REGULARIZED, STORE_FP16_OFFSET);                                     \
+        case 7: return CALL(COLL_RR3, STORE_FP16_OFFSET);                                             \
+        case 8: return CALL(COLL_TRT, STORE_BF16_OFFSET);                                             \
+        case 9: return CALL(COLL_TRT_SYMMETRIC_LIMITER, STORE_BF16_OFFSET);                           \
+        case 10: return CALL(COLL_REGULARIZED, STORE_BF16_OFFSET);                                    \
+        case 11: return CALL(COLL_RR3, STORE_BF16_OFFSET);                                            \
+        default: return hipErrorInvalidValue;                                                         \
     }
+
+hipError_t launch_fluid_fast(int storage, int coll, const FluidBuffers& b, const StepParams& p, hipStream_t s) {
+#define CALL(C, S) launch_fast_t<C, S>(b, p, s)
+    BF_DISPATCH(storage, coll, CALL)
+#undef CALL
 }
 
-hipError_t launch_fluid(int storage, int coll, const FluidBuffers& b, const StepParams& p, uint32_t slot,
-                        int general_blocks, hipStream_t s) {
-    switch (storage) {
-        case STORE_FP32: return launch_fluid_s<STORE_FP32>(coll, b, p, slot, general_blocks, s);
-        case STORE_FP16_OFFSET: return launch_fluid_s<STORE_FP16_OFFSET>(coll, b, p, slot, general_blocks, s);
-        case STORE_BF16_OFFSET: return launch_fluid_s<STORE_BF16_OFFSET>(coll, b, p, slot, general_blocks, s);
-        default: return hipErrorInvalidValue;
-    }
+hipError_t launch_fluid_general(int storage, int coll, const FluidBuffers& b, const StepParams& p, uint32_t slot,
+                                int general_blocks, hipStream_t s) {
+#define CALL(C, S) launch_general_t<C, S>(b, p, slot, general_blocks, s)
+    BF_DISPATCH(storage, coll, CALL)
+#undef CALL
+}
+
+hipError_t launch_build_general_list(const uint8_t* static_flags, const uint8_t* dyn_prev, const uint8_t* dyn_cur,
+                                     uint32_t tiles, uint32_t* list, uint32_t* count, hipStream_t s) {
+    hipLaunchKernelGGL(build_general_list, dim3((tiles + 255) / 256), dim3(256), 0, s, static_flags, dyn_prev,
+                       dyn_cur, tiles, list, count);
+    return hipGetLastError();
 }
 
 template <int S>

@@ -7,12 +7,12 @@
 //   resolve :1697-1738         clear   :1898-1911   sphere :183-195, 2307-2350
 //
 // The reference clears and resolves the whole grid every step.  Here the
-// raster appends every cell it touches for the first time to a list (the
-// 64-bit atomicMin returns the previous key, so exactly one lane sees the
-// empty key); resolve, the key reset and the next steps' clears walk only
-// that list.  Keys are (distance bin << 32 | triangle), ordering identical to
-// the reference's (bin << 12 | triangle) for meshes of up to 4,096 triangles
-// and without its 4,096-triangle limit.
+// raster marks the 64x4-cell tiles it keys (one byte per tile per step, plain
+// stores); one tile pass then resolves the marked tiles and clears the tiles
+// whose mask / wall entries are stale from earlier steps, so clear + resolve
+// touch only the neighbourhood of the surface.  Keys are (distance bin << 32
+// | triangle), ordering identical to the reference's (bin << 12 | triangle)
+// for meshes of up to 4,096 triangles and without its 4,096-triangle limit.
 #include "bf_launch.h"
 
 namespace bf {
@@ -83,19 +83,6 @@ __global__ void mesh_prepare(const float* __restrict__ points, const float* __re
                          dzv / duration * m.velocity_scale, 0.0f);
 }
 
-// Reset what an earlier step wrote: mask entries of the buffer about to be
-// rewritten (its cells are in list_mask), wall entries of the previous step
-// (list_wall).  Grid-stride over device-side counts.
-__global__ void clear_touched(const uint32_t* __restrict__ list_mask, const uint32_t* __restrict__ count_mask,
-                              uint8_t* __restrict__ mask, const uint32_t* __restrict__ list_wall,
-                              const uint32_t* __restrict__ count_wall, float4* __restrict__ wall) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t nm = *count_mask, nw = *count_wall;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += stride) mask[list_mask[i]] = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += stride)
-        wall[list_wall[i]] = make_float4(0.0f, 0.0f, 0.0f, 16.0f);
-}
-
 __global__ void fill_u8(uint8_t* __restrict__ p, uint8_t v, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -116,17 +103,26 @@ __global__ void fill_wall(float4* __restrict__ p, uint32_t n) {
     if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 16.0f);
 }
 
-// One wavefront per triangle; its lanes walk the expanded voxel AABB
-// (BirdFlow.metal:1650-1665 bounds).  A lane skips the atomic when its key
-// cannot win (plain load first); the atomic's result tells whether the cell
-// was empty before, in which case the cell joins the touched list.
-__global__ __launch_bounds__(256) void mesh_raster(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
-                                                   unsigned long long* __restrict__ keys,
-                                                   uint32_t* __restrict__ touched, uint32_t* __restrict__ touched_count,
-                                                   MeshParams m) {
-    const uint32_t t = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (t >= m.triangle_count) return;
-    const int lane = (int)(threadIdx.x & 63);
+// One single-wavefront workgroup per triangle; its 64 lanes walk the
+// expanded voxel AABB (BirdFlow.metal:1650-1665 bounds).  Distances and keys
+// are computed in registers; cells whose distance is within the candidate
+// band (m.cull_cells, +inf for the reference's full AABB) are compacted into
+// LDS with ballot / mbcnt, and only those reach memory: a plain load first,
+// the 64-bit atomicMin only when the key can still win; every keyed cell
+// marks its tile for this step's tile pass.
+constexpr int RASTER_CAP = 512;
+
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
+                                                  unsigned long long* __restrict__ keys,
+                                                  uint8_t* __restrict__ tiles_new, MeshParams m) {
+    __shared__ uint32_t cand_gid[RASTER_CAP];
+    __shared__ unsigned long long cand_key[RASTER_CAP];
+    const uint32_t t = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
     const float4 a4 = pos[tri[3 * (size_t)t + 0]];
     const float4 b4 = pos[tri[3 * (size_t)t + 1]];
     const float4 c4 = pos[tri[3 * (size_t)t + 2]];
@@ -147,24 +143,50 @@ __global__ __launch_bounds__(256) void mesh_raster(const float4* __restrict__ po
     }
     const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
     const uint32_t total = (uint32_t)ex * (uint32_t)ey * (uint32_t)ez;
-    for (uint32_t k = (uint32_t)lane; k < total; k += 64) {
-        const int x = lo[0] + (int)(k % (uint32_t)ex);
-        const uint32_t r = k / (uint32_t)ex;
-        const int y = lo[1] + (int)(r % (uint32_t)ey);
-        const int z = lo[2] + (int)(r / (uint32_t)ey);
-        const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
-        const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
-        const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
-        const Closest cp = closest_point(wx, wy, wz, a, b, c);
-        const float ddx = wx - cp.px, ddy = wy - cp.py, ddz = wz - cp.pz;
-        const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
-        const float scaled = roundf(dist_cells * 65536.0f);
-        const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
-        const unsigned long long key = ((unsigned long long)bin << 32) | (unsigned long long)t;
-        const uint32_t gid = (uint32_t)x + (uint32_t)m.g.nx * ((uint32_t)y + (uint32_t)m.g.ny * (uint32_t)z);
-        if (key < __hip_atomic_load(&keys[gid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            const unsigned long long old = atomicMin(&keys[gid], key);
-            if (old == EMPTY_KEY) touched[atomicAdd(touched_count, 1u)] = gid;
+    uint32_t count = 0;  // wave-uniform
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t k = base + lane;
+        bool cand = false;
+        uint32_t gid = 0;
+        unsigned long long key = 0;
+        if (k < total) {
+            const int x = lo[0] + (int)(k % (uint32_t)ex);
+            const uint32_t r = k / (uint32_t)ex;
+            const int y = lo[1] + (int)(r % (uint32_t)ey);
+            const int z = lo[2] + (int)(r / (uint32_t)ey);
+            const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
+            const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
+            const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
+            const Closest cp = closest_point(wx, wy, wz, a, b, c);
+            const float ddx = wx - cp.px, ddy = wy - cp.py, ddz = wz - cp.pz;
+            const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
+            const float scaled = roundf(dist_cells * 65536.0f);
+            const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
+            key = ((unsigned long long)bin << 32) | (unsigned long long)t;
+            gid = (uint32_t)x + (uint32_t)m.g.nx * ((uint32_t)y + (uint32_t)m.g.ny * (uint32_t)z);
+            cand = dist_cells <= m.cull_cells;
+        }
+        const unsigned long long ballot = __ballot(cand);
+        if (cand) {
+            const uint32_t slot = count + lane_rank(ballot);
+            cand_gid[slot] = gid;
+            cand_key[slot] = key;
+        }
+        count += (uint32_t)__popcll(ballot);
+        if (count > RASTER_CAP - 64 || base + 64 >= total) {
+            __syncthreads();
+            for (uint32_t j = lane; j < count; j += 64) {
+                const uint32_t g = cand_gid[j];
+                const unsigned long long kk = cand_key[j];
+                if (kk < __hip_atomic_load(&keys[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    (void)atomicMin(&keys[g], kk);
+                    const int x = (int)(g % (uint32_t)m.g.nx);
+                    const uint32_t yz = g / (uint32_t)m.g.nx;
+                    tiles_new[tile_of(m.g, x, (int)(yz % (uint32_t)m.g.ny), (int)(yz / (uint32_t)m.g.ny))] = 1;
+                }
+            }
+            __syncthreads();
+            count = 0;
         }
     }
 }
@@ -194,26 +216,57 @@ __device__ inline void mark_tiles(const Grid& g, const int periodic[3], uint8_t*
                     + (uint32_t)txs[i]] = 1;
 }
 
-// BirdFlow.metal:1697-1738 over the touched list: part flag when the signed
-// distance is <= 0, barycentric wall velocity, signed distance / dx.  Resets
-// the key of every resolved cell for the next step.
-__global__ void mesh_resolve(const float4* __restrict__ pos, const float4* __restrict__ vel,
-                             const uint32_t* __restrict__ tri, const uint8_t* __restrict__ tri_part,
-                             unsigned long long* __restrict__ keys, const uint32_t* __restrict__ touched,
-                             const uint32_t* __restrict__ touched_count, uint8_t* __restrict__ mask,
-                             float4* __restrict__ wall, uint8_t* __restrict__ dyn, MeshParams m) {
-    const uint32_t count = *touched_count;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-        const uint32_t gid = touched[i];
+// Tiles the geometry pass visits this step: keyed now, or holding mask /
+// wall entries written by the steps that last used those buffers.
+__global__ __launch_bounds__(256) void geometry_tile_list(const uint8_t* __restrict__ tiles_new,
+                                                          const uint8_t* __restrict__ tiles_mask_old,
+                                                          const uint8_t* __restrict__ tiles_wall_old, uint32_t tiles,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool flagged = t < tiles && (tiles_new[t] | tiles_mask_old[t] | tiles_wall_old[t]) != 0;
+    const unsigned long long b = __ballot(flagged);
+    if (b == 0) return;
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(count, (uint32_t)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (flagged) list[base + lane_rank(b)] = t;
+}
+
+// BirdFlow.metal:1697-1738 on the listed tiles.  In a tile keyed this step
+// every cell is written: keyed cells get the part flag (signed distance <=
+// 0), barycentric wall velocity and signed distance / dx and their key is
+// reset; the others are cleared (mask 0, wall (0,0,0,16)).  Other listed
+// tiles only clear the stale buffer(s).  Solid cells mark the tiles whose
+// cells pull from them.
+__global__ __launch_bounds__(256) void geometry_tile_pass(
+    const float4* __restrict__ pos, const float4* __restrict__ vel, const uint32_t* __restrict__ tri,
+    const uint8_t* __restrict__ tri_part, unsigned long long* __restrict__ keys, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ count, const uint8_t* __restrict__ tiles_new,
+    const uint8_t* __restrict__ tiles_mask_old, const uint8_t* __restrict__ tiles_wall_old, uint8_t* __restrict__ mask,
+    float4* __restrict__ wall, uint8_t* __restrict__ dyn, MeshParams m) {
+    const uint32_t n_tiles = *count;
+    for (uint32_t i = blockIdx.x; i < n_tiles; i += gridDim.x) {
+        const uint32_t tile = list[i];
+        int x0, y0, z;
+        tile_origin(m.g, tile, x0, y0, z);
+        const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+        const int y = y0 + (int)(threadIdx.x / TILE_X);
+        if (x >= m.g.nx || y >= m.g.ny) continue;
+        const uint32_t gid = (uint32_t)x + (uint32_t)m.g.nx * ((uint32_t)y + (uint32_t)m.g.ny * (uint32_t)z);
+        const float4 cleared = make_float4(0.0f, 0.0f, 0.0f, 16.0f);
+        if (!tiles_new[tile]) {
+            if (tiles_mask_old[tile]) mask[gid] = 0;
+            if (tiles_wall_old[tile]) wall[gid] = cleared;
+            continue;
+        }
         const unsigned long long key = keys[gid];
-        keys[gid] = EMPTY_KEY;
         const uint32_t t = (uint32_t)(key & 0xFFFFFFFFull);
-        if (t >= m.triangle_count) continue;
-        const int x = (int)(gid % (uint32_t)m.g.nx);
-        const uint32_t yz = gid / (uint32_t)m.g.nx;
-        const int y = (int)(yz % (uint32_t)m.g.ny);
-        const int z = (int)(yz / (uint32_t)m.g.ny);
+        if (key == EMPTY_KEY || t >= m.triangle_count) {
+            mask[gid] = 0;
+            wall[gid] = cleared;
+            continue;
+        }
+        keys[gid] = EMPTY_KEY;
         const float wx = m.origin[0] + ((float)x + 0.5f) * m.dx;
         const float wy = m.origin[1] + ((float)y + 0.5f) * m.dx;
         const float wz = m.origin[2] + ((float)z + 0.5f) * m.dx;
@@ -330,15 +383,15 @@ __global__ void link_census(const uint8_t* __restrict__ prev, const uint8_t* __r
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
     hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
                        b.vel, m);
-    hipLaunchKernelGGL(clear_touched, dim3(1024), dim3(256), 0, s, b.touched_mask_old, b.touched_mask_old_count,
-                       b.mask, b.touched_wall_old, b.touched_wall_old_count, b.wall);
+    hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.dyn, (uint8_t)0, m.g.tiles);
-    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.touched_new_count, 0u, 1u);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
     if (m.triangle_count > 0)
-        hipLaunchKernelGGL(mesh_raster, dim3((m.triangle_count + 3) / 4), dim3(256), 0, s, b.pos, b.tri, b.keys,
-                           b.touched_new, b.touched_new_count, m);
-    hipLaunchKernelGGL(mesh_resolve, dim3(2048), dim3(256), 0, s, b.pos, b.vel, b.tri, b.tri_part, b.keys,
-                       b.touched_new, b.touched_new_count, b.mask, b.wall, b.dyn, m);
+        hipLaunchKernelGGL(mesh_raster, dim3(m.triangle_count), dim3(64), 0, s, b.pos, b.tri, b.keys, b.tiles_new, m);
+    hipLaunchKernelGGL(geometry_tile_list, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new,
+                       b.tiles_mask_old, b.tiles_wall_old, m.g.tiles, b.list, b.list_count);
+    hipLaunchKernelGGL(geometry_tile_pass, dim3(2048), dim3(256), 0, s, b.pos, b.vel, b.tri, b.tri_part, b.keys, b.list,
+                       b.list_count, b.tiles_new, b.tiles_mask_old, b.tiles_wall_old, b.mask, b.wall, b.dyn, m);
     return hipGetLastError();
 }
 

@@ -26,7 +26,6 @@ struct FluidBuffers {
     double* history;
     float* density;
     float4* velocity;
-    hipEvent_t after_fast;     // recorded between the fast and general kernels when non-null
 };
 
 struct MeshParams {
@@ -42,6 +41,7 @@ struct MeshParams {
     uint32_t vertex_count;
     uint32_t triangle_count;
     int periodic[3];
+    float cull_cells;          // key only cells at most this far (cells) from a triangle; +inf = whole AABB
 };
 
 struct MeshBuffers {
@@ -52,19 +52,21 @@ struct MeshBuffers {
     float4* pos;
     float4* vel;
     unsigned long long* keys;
-    uint32_t* touched_new;
-    uint32_t* touched_new_count;
-    const uint32_t* touched_mask_old;   // cells last written into `mask`
-    const uint32_t* touched_mask_old_count;
-    const uint32_t* touched_wall_old;   // cells last written into `wall`
-    const uint32_t* touched_wall_old_count;
+    uint8_t* tiles_new;                 // tiles keyed this step (written here)
+    const uint8_t* tiles_mask_old;      // tiles whose `mask` entries an earlier step wrote
+    const uint8_t* tiles_wall_old;      // tiles whose `wall` entries an earlier step wrote
+    uint32_t* list;                     // scratch: tiles visited by the tile pass
+    uint32_t* list_count;
     uint8_t* mask;
     float4* wall;
     uint8_t* dyn;
 };
 
-hipError_t launch_fluid(int storage, int coll, const FluidBuffers& b, const StepParams& p, uint32_t slot,
-                        int general_blocks, hipStream_t s);
+hipError_t launch_fluid_fast(int storage, int coll, const FluidBuffers& b, const StepParams& p, hipStream_t s);
+hipError_t launch_fluid_general(int storage, int coll, const FluidBuffers& b, const StepParams& p, uint32_t slot,
+                                int general_blocks, hipStream_t s);
+hipError_t launch_build_general_list(const uint8_t* static_flags, const uint8_t* dyn_prev, const uint8_t* dyn_cur,
+                                     uint32_t tiles, uint32_t* list, uint32_t* count, hipStream_t s);
 hipError_t launch_init_equilibrium(int storage, void* f, const uint8_t* solid, const float4* wall,
                                    const float* far_u, uint32_t n, hipStream_t s);
 hipError_t launch_init_shear_wave(int storage, void* f, float amplitude, const Grid& g, hipStream_t s);

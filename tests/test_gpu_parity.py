@@ -161,9 +161,19 @@ def test_flapping_plate_matches_oracle():
         for s in range(steps):
             sim.advance(1)
             if s % 10 == 9:
+                # Stepping keys only cells within h + 1.5 cells of the mesh
+                # (all the fluid step reads); the oracle keys the reference's
+                # whole expanded AABB.  Masks agree everywhere, walls inside
+                # the band; outside it the GPU cell is equal or untouched.
                 mask, wall = sim.read_geometry()
-                assert np.array_equal(mask, geo_o[s][0]), s
-                assert np.array_equal(wall.view(np.uint32), geo_o[s][1].view(np.uint32)), s
+                mo, wo = geo_o[s]
+                assert np.array_equal(mask, mo), s
+                band = wo[:, 3] <= 1.5
+                assert np.array_equal(wall[band].view(np.uint32), wo[band].view(np.uint32)), s
+                rest = ~band
+                same = np.all(wall[rest].view(np.uint32) == wo[rest].view(np.uint32), axis=1)
+                cleared = np.all(wall[rest] == np.array([0, 0, 0, 16], np.float32), axis=1)
+                assert np.all(same | cleared), s
         hist = sim.load_history()
         g = sim.read_populations()
     scale = np.abs(loads_o[:, :6]).max(axis=0)
