@@ -9,7 +9,8 @@
 // Two HIP streams form a software pipeline, per step s:
 //   stream A  fluid_fast(s)                              (HBM-bound, wall-free tiles)
 //   stream B  geometry(s+1) + tile list(s+1)  [overlaps fast(s)],
-//             then fluid_general(s) + load reduction      [after fast(s)]
+//             then fluid_general(s)                       [after fast(s)]
+//             and the load reduction of step s            [overlaps fast(s+1)]
 // fast(s) waits for event R[(s-1)&1] (general(s-1) done, which follows
 // geometry(s) on B); general(s) waits for F[s&1] (fast(s) done).  Geometry of
 // step s+1 only writes buffers that steps s-1 and s-2 last read: masks and
@@ -829,10 +830,13 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         // ... then the wall tiles of step s once fast(s) is done
         if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, c->ev_fast[par2(s)], 0);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[4], c->sb);
-        if (e == hipSuccess)
-            e = launch_fluid_general(storage, coll, fb, p, (uint32_t)(s % HISTORY_CAPACITY), c->general_blocks, c->sb);
+        if (e == hipSuccess) e = launch_fluid_general(storage, coll, fb, p, c->general_blocks, c->sb);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[5], c->sb);
+        // fast(s+1) needs only the populations; the load reduction of step s
+        // overlaps it (it reads partials and flags that step s+1 rewrites
+        // only after it, later on this stream)
         if (e == hipSuccess) e = hipEventRecord(c->ev_ready[par2(s)], c->sb);
+        if (e == hipSuccess) e = launch_load_reduction(fb, p, (uint32_t)(s % HISTORY_CAPACITY), c->sb);
         if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
         c->step = s;
         c->cur = 1 - c->cur;

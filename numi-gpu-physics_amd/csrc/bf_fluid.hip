@@ -764,19 +764,21 @@ static hipError_t launch_fast_t(const FluidBuffers& b, const StepParams& p, hipS
 }
 
 template <int COLL, int S>
-static hipError_t launch_general_t(const FluidBuffers& b, const StepParams& p, uint32_t history_slot,
-                                   int general_blocks, hipStream_t s) {
+static hipError_t launch_general_t(const FluidBuffers& b, const StepParams& p, int general_blocks, hipStream_t s) {
     using T = typename Store<S>::T;
     hipLaunchKernelGGL((fluid_general<COLL, S>), dim3(general_blocks), dim3(256), 0, s, (const T*)b.fin,
                        (T*)b.fout, b.solid_prev, b.solid_cur, b.wall, b.general_list, b.general_count,
                        b.partials, b.density, b.velocity, p);
-    if (p.accumulate_loads) {
-        const uint32_t blocks = (p.g.tiles + 1023) / 1024;
-        hipLaunchKernelGGL(reduce_loads_tiles, dim3(blocks), dim3(256), 0, s, b.partials, b.static_flags,
-                           b.dyn_prev, b.dyn_cur, p.g.tiles, b.block_out);
-        hipLaunchKernelGGL(reduce_loads_final, dim3(1), dim3(256), 0, s, b.block_out, blocks, b.last_load,
-                           b.history, history_slot);
-    }
+    return hipGetLastError();
+}
+
+hipError_t launch_load_reduction(const FluidBuffers& b, const StepParams& p, uint32_t history_slot, hipStream_t s) {
+    if (!p.accumulate_loads) return hipSuccess;
+    const uint32_t blocks = (p.g.tiles + 1023) / 1024;
+    hipLaunchKernelGGL(reduce_loads_tiles, dim3(blocks), dim3(256), 0, s, b.partials, b.static_flags, b.dyn_prev,
+                       b.dyn_cur, p.g.tiles, b.block_out);
+    hipLaunchKernelGGL(reduce_loads_final, dim3(1), dim3(256), 0, s, b.block_out, blocks, b.last_load, b.history,
+                       history_slot);
     return hipGetLastError();
 }
 
@@ -803,9 +805,9 @@ hipError_t launch_fluid_fast(int storage, int coll, const FluidBuffers& b, const
 #undef CALL
 }
 
-hipError_t launch_fluid_general(int storage, int coll, const FluidBuffers& b, const StepParams& p, uint32_t slot,
-                                int general_blocks, hipStream_t s) {
-#define CALL(C, S) launch_general_t<C, S>(b, p, slot, general_blocks, s)
+hipError_t launch_fluid_general(int storage, int coll, const FluidBuffers& b, const StepParams& p, int general_blocks,
+                                hipStream_t s) {
+#define CALL(C, S) launch_general_t<C, S>(b, p, general_blocks, s)
     BF_DISPATCH(storage, coll, CALL)
 #undef CALL
 }

@@ -104,13 +104,19 @@ __global__ void fill_wall(float4* __restrict__ p, uint32_t n) {
 }
 
 // One single-wavefront workgroup per triangle; its 64 lanes walk the
-// expanded voxel AABB (BirdFlow.metal:1650-1665 bounds).  Distances and keys
-// are computed in registers; cells whose distance is within the candidate
-// band (m.cull_cells, +inf for the reference's full AABB) are compacted into
-// LDS with ballot / mbcnt, and only those reach memory: a plain load first,
-// the 64-bit atomicMin only when the key can still win; every keyed cell
-// marks its tile for this step's tile pass.
-constexpr int RASTER_CAP = 512;
+// expanded voxel AABB (BirdFlow.metal:1650-1665 bounds) in three phases:
+//  1. a conservative filter: a cell can be within the candidate band
+//     (m.cull_cells) only if its distance to the triangle's plane and to its
+//     bounding sphere are within the band (both are lower bounds of the
+//     distance to the triangle; a margin of 1e-3 cells absorbs rounding);
+//  2. the exact closest point and key (the reference's arithmetic) for the
+//     survivors, compacted into LDS so that every lane has work;
+//  3. for cells within the band: a plain load of the current key and the
+//     64-bit atomicMin only when the key can still win; the cell's tile is
+//     marked for this step's tile pass.
+// With m.cull_cells = +inf (the reference's full AABB) the filter passes
+// every cell.
+constexpr int RASTER_CAP = 1024;
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -119,8 +125,7 @@ __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
 __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
                                                   unsigned long long* __restrict__ keys,
                                                   uint8_t* __restrict__ tiles_new, MeshParams m) {
-    __shared__ uint32_t cand_gid[RASTER_CAP];
-    __shared__ unsigned long long cand_key[RASTER_CAP];
+    __shared__ uint32_t cand[RASTER_CAP];
     const uint32_t t = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     const float4 a4 = pos[tri[3 * (size_t)t + 0]];
@@ -143,46 +148,67 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
     }
     const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
     const uint32_t total = (uint32_t)ex * (uint32_t)ey * (uint32_t)ez;
+    // filter data (world units)
+    const bool filter = isfinite(m.cull_cells);
+    const float reach = (filter ? m.cull_cells : 0.0f) * dx + 1.0e-3f * dx;
+    float nx_ = (b.y - a.y) * (c.z - a.z) - (b.z - a.z) * (c.y - a.y);
+    float ny_ = (b.z - a.z) * (c.x - a.x) - (b.x - a.x) * (c.z - a.z);
+    float nz_ = (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+    const float nl = sqrtf(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+    if (nl > 0.0f) {
+        nx_ /= nl;
+        ny_ /= nl;
+        nz_ /= nl;
+    }
+    const float cxm = (a.x + b.x + c.x) / 3.0f, cym = (a.y + b.y + c.y) / 3.0f, czm = (a.z + b.z + c.z) / 3.0f;
+    auto d2 = [](float x, float y, float z) { return x * x + y * y + z * z; };
+    const float r = sqrtf(fmaxf(d2(a.x - cxm, a.y - cym, a.z - czm),
+                                fmaxf(d2(b.x - cxm, b.y - cym, b.z - czm), d2(c.x - cxm, c.y - cym, c.z - czm))));
+    const float sphere2 = (reach + r * 1.0001f) * (reach + r * 1.0001f);
+
     uint32_t count = 0;  // wave-uniform
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t k = base + lane;
-        bool cand = false;
-        uint32_t gid = 0;
-        unsigned long long key = 0;
+        bool pass = false;
         if (k < total) {
-            const int x = lo[0] + (int)(k % (uint32_t)ex);
-            const uint32_t r = k / (uint32_t)ex;
-            const int y = lo[1] + (int)(r % (uint32_t)ey);
-            const int z = lo[2] + (int)(r / (uint32_t)ey);
-            const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
-            const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
-            const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
-            const Closest cp = closest_point(wx, wy, wz, a, b, c);
-            const float ddx = wx - cp.px, ddy = wy - cp.py, ddz = wz - cp.pz;
-            const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
-            const float scaled = roundf(dist_cells * 65536.0f);
-            const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
-            key = ((unsigned long long)bin << 32) | (unsigned long long)t;
-            gid = (uint32_t)x + (uint32_t)m.g.nx * ((uint32_t)y + (uint32_t)m.g.ny * (uint32_t)z);
-            cand = dist_cells <= m.cull_cells;
+            pass = true;
+            if (filter) {
+                const int x = lo[0] + (int)(k % (uint32_t)ex);
+                const uint32_t rr = k / (uint32_t)ex;
+                const int y = lo[1] + (int)(rr % (uint32_t)ey);
+                const int z = lo[2] + (int)(rr / (uint32_t)ey);
+                const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
+                const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
+                const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
+                const float plane = fabsf(nx_ * (wx - a.x) + ny_ * (wy - a.y) + nz_ * (wz - a.z));
+                pass = (nl == 0.0f || plane <= reach) && d2(wx - cxm, wy - cym, wz - czm) <= sphere2;
+            }
         }
-        const unsigned long long ballot = __ballot(cand);
-        if (cand) {
-            const uint32_t slot = count + lane_rank(ballot);
-            cand_gid[slot] = gid;
-            cand_key[slot] = key;
-        }
+        const unsigned long long ballot = __ballot(pass);
+        if (pass) cand[count + lane_rank(ballot)] = k;
         count += (uint32_t)__popcll(ballot);
         if (count > RASTER_CAP - 64 || base + 64 >= total) {
             __syncthreads();
             for (uint32_t j = lane; j < count; j += 64) {
-                const uint32_t g = cand_gid[j];
-                const unsigned long long kk = cand_key[j];
-                if (kk < __hip_atomic_load(&keys[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    (void)atomicMin(&keys[g], kk);
-                    const int x = (int)(g % (uint32_t)m.g.nx);
-                    const uint32_t yz = g / (uint32_t)m.g.nx;
-                    tiles_new[tile_of(m.g, x, (int)(yz % (uint32_t)m.g.ny), (int)(yz / (uint32_t)m.g.ny))] = 1;
+                const uint32_t kk = cand[j];
+                const int x = lo[0] + (int)(kk % (uint32_t)ex);
+                const uint32_t rr = kk / (uint32_t)ex;
+                const int y = lo[1] + (int)(rr % (uint32_t)ey);
+                const int z = lo[2] + (int)(rr / (uint32_t)ey);
+                const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
+                const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
+                const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
+                const Closest cp = closest_point(wx, wy, wz, a, b, c);
+                const float ddx = wx - cp.px, ddy = wy - cp.py, ddz = wz - cp.pz;
+                const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
+                if (!(dist_cells <= m.cull_cells)) continue;
+                const float scaled = roundf(dist_cells * 65536.0f);
+                const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
+                const unsigned long long key = ((unsigned long long)bin << 32) | (unsigned long long)t;
+                const uint32_t g = (uint32_t)x + (uint32_t)m.g.nx * ((uint32_t)y + (uint32_t)m.g.ny * (uint32_t)z);
+                if (key < __hip_atomic_load(&keys[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    (void)atomicMin(&keys[g], key);
+                    tiles_new[tile_of(m.g, x, y, z)] = 1;
                 }
             }
             __syncthreads();

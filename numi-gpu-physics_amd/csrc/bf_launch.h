@@ -63,8 +63,9 @@ struct MeshBuffers {
 };
 
 hipError_t launch_fluid_fast(int storage, int coll, const FluidBuffers& b, const StepParams& p, hipStream_t s);
-hipError_t launch_fluid_general(int storage, int coll, const FluidBuffers& b, const StepParams& p, uint32_t slot,
-                                int general_blocks, hipStream_t s);
+hipError_t launch_fluid_general(int storage, int coll, const FluidBuffers& b, const StepParams& p, int general_blocks,
+                                hipStream_t s);
+hipError_t launch_load_reduction(const FluidBuffers& b, const StepParams& p, uint32_t history_slot, hipStream_t s);
 hipError_t launch_build_general_list(const uint8_t* static_flags, const uint8_t* dyn_prev, const uint8_t* dyn_cur,
                                      uint32_t tiles, uint32_t* list, uint32_t* count, hipStream_t s);
 hipError_t launch_init_equilibrium(int storage, void* f, const uint8_t* solid, const float4* wall,
