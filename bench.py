@@ -30,13 +30,15 @@ sys.path[:0] = [os.path.join(ROOT, "numi-gpu-physics_amd")]
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS + %HBM roofline, moving-bird 512×256×256 FP32 vs FP16-store; adjoint/fwd"
 STORAGES = {"fp32": 0, "fp16": 1, "bf16": 2}
+ARITHMETIC = {"fused": 0, "strict": 1}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fluid_fast_traffic.json")
 POP_BYTES = {"fp32": 152, "fp16": 76, "bf16": 76}
 
 
-def bird_config(shape, storage):
+def bird_config(shape, storage, arithmetic="fused"):
     import birdflow_hip as bh
     tau = 0.52
-    return bh.SolverConfig(
+    return bh.SolverConfig(arithmetic=ARITHMETIC[arithmetic],
         shape=shape, omega_plus=1.0 / tau, omega_minus=1.0 / (0.5 + (3.0 / 16.0) / (tau - 0.5)),
         storage=STORAGES[storage], collision=bh.COLLISION_RR3, wall_rule=bh.WALL_INTERPOLATED_SDF, force_mode=6,
         accumulate_loads=True, boundary=((bh.BC_FARFIELD, bh.BC_OUTLET), (bh.BC_PERIODIC, bh.BC_PERIODIC),
@@ -50,10 +52,10 @@ def bird_mesh(steps, shape):
     return workloads.synthetic_bird(steps, seed=7, center=(160.0 * s, 128.0 * s, 128.0 * s))
 
 
-def run_gpu(shape, steps, warmup, storage):
+def run_gpu(shape, steps, warmup, storage, arithmetic="fused"):
     import birdflow_hip as bh
     seq = bird_mesh(warmup + steps, shape)
-    cfg = bird_config(shape, storage)
+    cfg = bird_config(shape, storage, arithmetic)
     sim = bh.BirdFlowSimulation(cfg)
     try:
         sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
@@ -124,12 +126,24 @@ def run_cpu_baseline(shape, steps):
     return n * steps / elapsed / 1e6, threads, elapsed
 
 
+def measured_traffic(storage, arithmetic, cells):
+    """HBM bytes per fast-kernel launch from the committed rocprofv3 PMC passes
+    (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel) for this storage
+    and arithmetic at this launch size; None when no such measurement exists."""
+    try:
+        rec = json.load(open(TRAFFIC_FILE))[f"{storage}/{arithmetic}"]
+    except (OSError, KeyError, ValueError):
+        return None
+    return rec["bytes_per_launch"] if rec.get("cells_per_launch") == cells else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--storage", choices=list(STORAGES), default="fp32")
+    ap.add_argument("--arithmetic", choices=list(ARITHMETIC), default="fused")
     ap.add_argument("--shape", type=int, nargs=3, default=(512, 256, 256))
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--also-fp16", action="store_true", help="also run the FP16-offset storage form")
@@ -144,13 +158,13 @@ def main():
     import birdflow_hip as bh
     copy_gbs = bh.copy_bandwidth(1 << 30, 20)
     print(f"# copy kernel: {copy_gbs:.1f} GB/s (1 GiB buffers)", file=sys.stderr, flush=True)
-    r = run_gpu(shape, args.steps, args.warmup, args.storage)
+    r = run_gpu(shape, args.steps, args.warmup, args.storage, args.arithmetic)
     print(f"# gpu: {r}", file=sys.stderr, flush=True)
     extra = {"copy_kernel_gbs": copy_gbs}
     if not r["finite"]:
         extra["error"] = "non-finite loads: the run diverged"
     if args.also_fp16:
-        r16 = run_gpu(shape, args.steps, args.warmup, "fp16")
+        r16 = run_gpu(shape, args.steps, args.warmup, "fp16", args.arithmetic)
         print(f"# gpu fp16: {r16}", file=sys.stderr, flush=True)
         extra["fp16_store"] = {"value": r16["mlups"], "unit": "MLUPS", "ratio_vs_fp32": r16["mlups"] / r["mlups"],
                                "fast_kernel_ms": r16["fast_kernel_ms"], "achieved_gbs": r16["achieved_gbs"]}
@@ -178,9 +192,11 @@ def main():
         "config": {"workload": f"moving bird {shape[0]}x{shape[1]}x{shape[2]}, ~40k-triangle synthetic "
                                "articulated mesh, RR3, interpolated moving bounce-back, FP64 loads, "
                                "link rebuild every step",
-                   "cells": shape[0] * shape[1] * shape[2], "storage": args.storage},
+                   "cells": shape[0] * shape[1] * shape[2], "storage": args.storage,
+                   "arithmetic": args.arithmetic},
         "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": r["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.storage, args.arithmetic, r["fast_cells"]),
                      "kernel": "fluid_fast", "bytes_per_cell": POP_BYTES[args.storage],
                      "cells_per_launch": r["fast_cells"], "avg_ms": r["fast_kernel_ms"]},
         "step_breakdown_ms": {"geometry": r["geometry_ms"], "fluid_fast": r["fast_kernel_ms"],

@@ -76,6 +76,13 @@ enum { BF_WALL_HALFWAY = 0, BF_WALL_INTERPOLATED_SDF = 1, BF_WALL_INTERPOLATED_H
  * arithmetic. */
 enum { BF_STORAGE_FP32 = 0, BF_STORAGE_FP16_OFFSET = 1, BF_STORAGE_BF16_OFFSET = 2 };
 
+/* Collision arithmetic.  FUSED (the default) evaluates the operators over
+ * opposite-direction pairs with fused multiply-adds; STRICT evaluates the
+ * reference's expressions in the reference's order without contraction
+ * (FP32 storage only), which makes the step reproducible operation for
+ * operation on any IEEE FP32 machine.  The two differ by rounding. */
+enum { BF_ARITHMETIC_FUSED = 0, BF_ARITHMETIC_STRICT = 1 };
+
 /* Where each step's geometry comes from. */
 enum { BF_GEOMETRY_STATIC = 0, BF_GEOMETRY_MESH = 1, BF_GEOMETRY_SPHERE = 2 };
 
@@ -103,7 +110,8 @@ typedef struct bf_config {
     float far_velocity[3];        /* lattice units */
     float far_density;
     float body_position[3];       /* torque reference point (world units) */
-    uint32_t reserved[8];
+    uint32_t arithmetic;          /* BF_ARITHMETIC_*; 0 (fused) when zero-initialised */
+    uint32_t reserved[7];
 } bf_config;
 
 typedef struct bf_context bf_context;

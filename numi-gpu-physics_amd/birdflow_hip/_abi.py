@@ -33,6 +33,7 @@ BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
 COLLISION_TRT, COLLISION_TRT_SYMMETRIC_LIMITER, COLLISION_REGULARIZED, COLLISION_RR3 = 0, 1, 2, 3
 WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF = 0, 1, 2
 STORAGE_FP32, STORAGE_FP16_OFFSET, STORAGE_BF16_OFFSET = 0, 1, 2
+ARITHMETIC_FUSED, ARITHMETIC_STRICT = 0, 1
 
 EXPORTED_FUNCTIONS = (
     "bf_create", "bf_destroy", "bf_last_error", "bf_status_string", "bf_abi_version", "bf_device_info",
@@ -66,7 +67,8 @@ class Config(ctypes.Structure):
         ("far_velocity", ctypes.c_float * 3),
         ("far_density", ctypes.c_float),
         ("body_position", ctypes.c_float * 3),
-        ("reserved", ctypes.c_uint32 * 8),
+        ("arithmetic", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 7),
     ]
 
 

@@ -95,6 +95,7 @@ class SolverConfig:
     far_velocity: Sequence[float] = (0.0, 0.0, 0.0)
     far_density: float = 1.0
     body_position: Sequence[float] = (0.0, 0.0, 0.0)
+    arithmetic: int = _abi.ARITHMETIC_FUSED
 
     def to_c(self) -> _abi.Config:
         c = _abi.Config()
@@ -113,6 +114,7 @@ class SolverConfig:
         c.far_velocity[:] = [float(v) for v in self.far_velocity]
         c.far_density = self.far_density
         c.body_position[:] = [float(v) for v in self.body_position]
+        c.arithmetic = self.arithmetic
         return c
 
     @property

@@ -32,6 +32,11 @@
 
 using namespace bf;
 
+static_assert(BF_ARITHMETIC_FUSED == ARITH_FUSED && BF_ARITHMETIC_STRICT == ARITH_STRICT, "arithmetic ids");
+static_assert(BF_STORAGE_FP32 == STORE_FP32 && BF_STORAGE_FP16_OFFSET == STORE_FP16_OFFSET &&
+                  BF_STORAGE_BF16_OFFSET == STORE_BF16_OFFSET, "storage ids");
+static_assert(BF_COLLISION_RR3 == COLL_RR3 && BF_WALL_INTERPOLATED_HALF == WALL_INTERPOLATED_HALF, "operator ids");
+
 namespace {
 constexpr uint32_t HISTORY_CAPACITY = 1u << 16;
 constexpr uint32_t PROFILE_CAPACITY = 4096;
@@ -331,6 +336,7 @@ FluidBuffers fluid_buffers(const bf_context* c, int64_t s) {
     b.history = c->history;
     b.density = c->density;
     b.velocity = c->velocity;
+    b.activation = &c->counters[4 + 2 * par2(s)];
     return b;
 }
 
@@ -424,7 +430,10 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     const uint64_t n64 = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || n64 >= (1ull << 32)) return BF_ERROR_INVALID_CONFIGURATION;
     if (cfg->storage > BF_STORAGE_BF16_OFFSET || cfg->collision > BF_COLLISION_RR3 ||
-        cfg->wall_rule > BF_WALL_INTERPOLATED_HALF || cfg->force_mode > 6 || cfg->load_component > 2)
+        cfg->wall_rule > BF_WALL_INTERPOLATED_HALF || cfg->force_mode > 6 || cfg->load_component > 2 ||
+        cfg->arithmetic > BF_ARITHMETIC_STRICT)
+        return BF_ERROR_INVALID_CONFIGURATION;
+    if (cfg->arithmetic == BF_ARITHMETIC_STRICT && cfg->storage != BF_STORAGE_FP32)
         return BF_ERROR_INVALID_CONFIGURATION;
     for (int a = 0; a < 3; ++a)
         for (int s = 0; s < 2; ++s)
@@ -783,7 +792,7 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
     if (c->invalidated) return BF_ERROR_STATE_INVALIDATED;
     (void)hipSetDevice(c->device);
     if (steps == 0) return BF_OK;
-    const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision;
+    const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision, arith = (int)c->cfg.arithmetic;
     hipError_t e = hipSuccess;
     // Prologue: geometry (unless already built) and tile list of the first
     // step of this call; event R of the previous step then releases fast(s).
@@ -816,7 +825,7 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         // geometry(s)) is done
         e = hipStreamWaitEvent(c->sa, c->ev_ready[par2(s - 1)], 0);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[0], c->sa);
-        if (e == hipSuccess) e = launch_fluid_fast(storage, coll, fb, p, c->sa);
+        if (e == hipSuccess) e = launch_fluid_fast(arith, storage, coll, fb, p, c->sa);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[1], c->sa);
         if (e == hipSuccess) e = hipEventRecord(c->ev_fast[par2(s)], c->sa);
         if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
@@ -830,7 +839,7 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         // ... then the wall tiles of step s once fast(s) is done
         if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, c->ev_fast[par2(s)], 0);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[4], c->sb);
-        if (e == hipSuccess) e = launch_fluid_general(storage, coll, fb, p, c->general_blocks, c->sb);
+        if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, p, c->general_blocks, c->sb);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[5], c->sb);
         // fast(s+1) needs only the populations; the load reduction of step s
         // overlaps it (it reads partials and flags that step s+1 rewrites

@@ -54,6 +54,7 @@ enum : int { BC_PERIODIC = 0, BC_FARFIELD = 1, BC_OUTLET = 2 };
 enum : int { COLL_TRT = 0, COLL_TRT_SYMMETRIC_LIMITER = 1, COLL_REGULARIZED = 2, COLL_RR3 = 3 };
 enum : int { WALL_HALFWAY = 0, WALL_INTERPOLATED_SDF = 1, WALL_INTERPOLATED_HALF = 2 };
 enum : int { STORE_FP32 = 0, STORE_FP16_OFFSET = 1, STORE_BF16_OFFSET = 2 };
+enum : int { ARITH_FUSED = 0, ARITH_STRICT = 1 };
 
 // Tile of cells owned by one 256-lane workgroup: 64 consecutive x (one
 // wavefront, 256 contiguous bytes per population row) by 4 rows of y.

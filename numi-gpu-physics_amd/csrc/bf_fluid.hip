@@ -1,313 +1,11 @@
-// Fused D3Q19 pull-stream + collide for gfx950.
-//
-// Replaces the fluid part of stepFluidTRT (Sources/BirdFlowMetal/Metal/
-// BirdFlow.metal:2507-3236).  The lattice is split into 64x4 tiles, one
-// 256-lane workgroup each:
-//   * fluid_fast    — tiles with no solid cell (current or previous step) in
-//                     their 1-cell halo and outside any sponge: pure pull
-//                     stream with per-face boundary sources, collision, store.
-//                     Tiles that need more append themselves to a work list.
-//   * fluid_general — the listed tiles: solid nodes, refill of uncovered
-//                     nodes, halfway or interpolated moving bounce-back,
-//                     momentum-exchange loads with cover/uncover impulses,
-//                     sponge.  Per-cell FP32 loads are summed in FP64 with
-//                     wavefront shuffles and an LDS tree into one partial per
-//                     tile; reduce_loads_* finishes the fixed-order sum.
-// Both kernels share the collision code; a fluid cell gets bit-identical
-// results from either.
-#include "bf_launch.h"
+// Non-templated fluid kernels and their launchers: the general-tile work
+// list, the fixed-order load reduction, initialisation, field extraction and
+// storage conversion; plus the dispatch of the fluid step over arithmetic,
+// storage and collision (instantiations live in bf_fluid_fused.hip and
+// bf_fluid_strict.hip).
+#include "bf_fluid.h"
 
 namespace bf {
-
-// ---------------------------------------------------------------------------
-// Collision (BirdFlow.metal:2888-3174).  f: post-stream populations; writes
-// post[] (before sponge).  act/restr: positivity-limiter activation and
-// maximum restriction.
-// ---------------------------------------------------------------------------
-template <int q>
-__device__ __forceinline__ float dir_dot(float ux, float uy, float uz) {
-    // c_q . u with the zero terms dropped; identical in value to the full
-    // three-term dot product because c has entries 0/+-1.
-    float r = 0.0f;
-    bool first = true;
-    if constexpr (cx(q) != 0) { r = cx(q) > 0 ? ux : -ux; first = false; }
-    if constexpr (cy(q) != 0) { r = first ? (cy(q) > 0 ? uy : -uy) : (cy(q) > 0 ? r + uy : r - uy); first = false; }
-    if constexpr (cz(q) != 0) { r = first ? (cz(q) > 0 ? uz : -uz) : (cz(q) > 0 ? r + uz : r - uz); first = false; }
-    return r;
-}
-
-template <int q>
-__device__ __forceinline__ float feq_q(float wrho, float u2_15, float ux, float uy, float uz) {
-    float cu = dir_dot<q>(ux, uy, uz);
-    return wrho * (1.0f + 3.0f * cu + 4.5f * cu * cu - u2_15);
-}
-
-template <int... Q>
-struct QSeq {};
-
-template <typename F, int... Q>
-__device__ __forceinline__ void for_q(QSeq<Q...>, F&& fn) {
-    (fn(std::integral_constant<int, Q>{}), ...);
-}
-using AllQ = QSeq<0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18>;
-
-__device__ __forceinline__ void moments(const float (&f)[19], float& rho, float& mx, float& my, float& mz) {
-    // sum in ascending q, the order of the reference loop
-    rho = 0.0f;
-    mx = 0.0f;
-    my = 0.0f;
-    mz = 0.0f;
-    for_q(AllQ{}, [&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        rho = rho + f[q];
-        if constexpr (cx(q) > 0) mx = mx + f[q];
-        if constexpr (cx(q) < 0) mx = mx - f[q];
-        if constexpr (cy(q) > 0) my = my + f[q];
-        if constexpr (cy(q) < 0) my = my - f[q];
-        if constexpr (cz(q) > 0) mz = mz + f[q];
-        if constexpr (cz(q) < 0) mz = mz - f[q];
-    });
-}
-
-template <int COLL>
-__device__ __forceinline__ void collide(const float (&f)[19], float rho, float ux, float uy, float uz,
-                                        float op, float om, float (&post)[19], float& act, float& restr) {
-    float feq[19];
-    float u2_15 = 1.5f * (ux * ux + uy * uy + uz * uz);
-    float w0 = weight<float>(0) * rho, w1 = weight<float>(1) * rho, w2 = weight<float>(7) * rho;
-    for_q(AllQ{}, [&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        feq[q] = feq_q<q>(q == 0 ? w0 : (q <= 6 ? w1 : w2), u2_15, ux, uy, uz);
-    });
-    act = 0.0f;
-    restr = 0.0f;
-    if constexpr (COLL == COLL_TRT) {
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int o = opp(q);
-            float fp = 0.5f * (f[q] + f[o]);
-            float fm = 0.5f * (f[q] - f[o]);
-            float ep = 0.5f * (feq[q] + feq[o]);
-            float em = 0.5f * (feq[q] - feq[o]);
-            post[q] = f[q] - op * (fp - ep) - om * (fm - em);
-        });
-    } else if constexpr (COLL == COLL_TRT_SYMMETRIC_LIMITER) {
-        float sinc[19], ainc[19];
-        float sscale = 1.0f;
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int o = opp(q);
-            float fp = 0.5f * (f[q] + f[o]);
-            float fm = 0.5f * (f[q] - f[o]);
-            float ep = 0.5f * (feq[q] + feq[o]);
-            float em = 0.5f * (feq[q] - feq[o]);
-            float si = -op * (fp - ep);
-            float ai = -om * (fm - em);
-            sinc[q] = si;
-            ainc[q] = ai;
-            if (si < 0.0f) {
-                float base = f[q] + ai;
-                float floorv = fmaxf(1.0e-12f, 1.0e-6f * fmaxf(feq[q], 0.0f));
-                if (base + si < floorv) {
-                    float cand = (base - floorv) / fmaxf(-si, 1.0e-30f);
-                    cand = fminf(fmaxf(cand, 0.0f), 1.0f);
-                    sscale = fminf(sscale, cand);
-                }
-            }
-        });
-        if (sscale < 1.0f) {
-            act = 1.0f;
-            restr = 1.0f - sscale;
-        }
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int o = opp(q);
-            if (sscale < 1.0f) {
-                post[q] = f[q] + sscale * sinc[q] + ainc[q];
-            } else {
-                float fp = 0.5f * (f[q] + f[o]);
-                float fm = 0.5f * (f[q] - f[o]);
-                float ep = 0.5f * (feq[q] + feq[o]);
-                float em = 0.5f * (feq[q] - feq[o]);
-                post[q] = f[q] - op * (fp - ep) - om * (fm - em);
-            }
-        });
-    } else {
-        // Regularized / RR3 (BirdFlow.metal:2917-3062).  Terms whose factor is
-        // a zero component of c_q are dropped at compile time: each is an
-        // exact +-0 that leaves the running sum unchanged, so the result is
-        // bit-identical to the full expression while the instruction count
-        // falls by about half.
-        constexpr bool RR3 = COLL == COLL_RR3;
-        const float c2 = cs2<float>();
-        const float h_one = 1.0f - c2;   // c_a c_a - cs2 for c_a = +-1
-        const float h_zero = 0.0f - c2;  // and for c_a = 0
-        float dg0 = 0.0f, dg1 = 0.0f, dg2 = 0.0f, od0 = 0.0f, od1 = 0.0f, od2 = 0.0f;
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            const float neq = f[q] - feq[q];
-            dg0 = dg0 + neq * (cx(q) != 0 ? h_one : h_zero);
-            dg1 = dg1 + neq * (cy(q) != 0 ? h_one : h_zero);
-            dg2 = dg2 + neq * (cz(q) != 0 ? h_one : h_zero);
-            if constexpr (cx(q) * cy(q) > 0) od0 = od0 + neq;
-            if constexpr (cx(q) * cy(q) < 0) od0 = od0 - neq;
-            if constexpr (cx(q) * cz(q) > 0) od1 = od1 + neq;
-            if constexpr (cx(q) * cz(q) < 0) od1 = od1 - neq;
-            if constexpr (cy(q) * cz(q) > 0) od2 = od2 + neq;
-            if constexpr (cy(q) * cz(q) < 0) od2 = od2 - neq;
-        });
-        float aXxy = 2.0f * ux * od0 + uy * dg0;
-        float aXxz = 2.0f * ux * od1 + uz * dg0;
-        float aXyy = ux * dg1 + 2.0f * uy * od0;
-        float aXzz = ux * dg2 + 2.0f * uz * od1;
-        float aYyz = 2.0f * uy * od2 + uz * dg1;
-        float aYzz = uy * dg2 + 2.0f * uz * od2;
-        float unb[19];
-        float scale = 1.0f;
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int X = cx(q), Y = cy(q), Z = cz(q);
-            const float hxx = X != 0 ? h_one : h_zero, hyy = Y != 0 ? h_one : h_zero, hzz = Z != 0 ? h_one : h_zero;
-            float contraction = hxx * dg0 + hyy * dg1 + hzz * dg2;
-            // at most one off-diagonal product c_a c_b is nonzero
-            if constexpr (X * Y != 0) contraction = contraction + 2.0f * (X * Y > 0 ? od0 : -od0);
-            if constexpr (X * Z != 0) contraction = contraction + 2.0f * (X * Z > 0 ? od1 : -od1);
-            if constexpr (Y * Z != 0) contraction = contraction + 2.0f * (Y * Z > 0 ? od2 : -od2);
-            const float w = weight<float>(q);
-            float reg = w * 4.5f * contraction;
-            if constexpr (RR3 && q != 0) {
-                // sum in the reference's order of the terms whose leading
-                // velocity component is nonzero
-                float third = 0.0f;
-                bool any = false;
-                auto add = [&](float t) { third = any ? third + t : t; any = true; };
-                if constexpr (Y != 0) add((Y > 0 ? hxx : -hxx) * aXxy);
-                if constexpr (Z != 0) add((Z > 0 ? hxx : -hxx) * aXxz);
-                if constexpr (X != 0) add((X > 0 ? hyy : -hyy) * aXyy);
-                if constexpr (X != 0) add((X > 0 ? hzz : -hzz) * aXzz);
-                if constexpr (Z != 0) add((Z > 0 ? hyy : -hyy) * aYyz);
-                if constexpr (Y != 0) add((Y > 0 ? hzz : -hzz) * aYzz);
-                reg = reg + w * 13.5f * third;
-            }
-            const float cand = feq[q] + (1.0f - op) * reg;
-            unb[q] = cand;
-            const float floorv = fmaxf(1.0e-12f, 1.0e-6f * fmaxf(feq[q], 0.0f));
-            if (cand < floorv) {
-                float adm = (feq[q] - floorv) / fmaxf(feq[q] - cand, 1.0e-30f);
-                adm = fminf(fmaxf(adm, 0.0f), 1.0f);
-                scale = fminf(scale, adm);
-            }
-        });
-        if (scale < 1.0f) {
-            act = 1.0f;
-            restr = 1.0f - scale;
-        }
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            post[q] = feq[q] + scale * (unb[q] - feq[q]);
-        });
-    }
-}
-
-// Pull-source coordinate along one axis with the face treatment; sets far
-// when the far-field equilibrium replaces the population.
-__device__ __forceinline__ int shift_axis(int v, int d, int n, int bc_lo, int bc_hi, bool& far) {
-    int s = v - d;
-    if (s < 0) {
-        if (bc_lo == BC_PERIODIC) s = n - 1;
-        else if (bc_lo == BC_OUTLET) s = 0;
-        else far = true;
-    } else if (s >= n) {
-        if (bc_hi == BC_PERIODIC) s = 0;
-        else if (bc_hi == BC_OUTLET) s = n - 1;
-        else far = true;
-    }
-    return s;
-}
-
-template <int S>
-__device__ __forceinline__ void write_post(typename Store<S>::T* __restrict__ fout, size_t n, uint32_t gid,
-                                           const float (&post)[19], float sponge, const StepParams& p,
-                                           float& crho, float& cmx, float& cmy, float& cmz) {
-    crho = 0.0f;
-    cmx = 0.0f;
-    cmy = 0.0f;
-    cmz = 0.0f;
-    for_q(AllQ{}, [&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        float v = post[q];
-        if (sponge > 0.0f) v = v + (p.far_eq[q] - v) * sponge;
-        Store<S>::store(fout, (size_t)q * n + gid, q, v);
-        crho = crho + v;
-        if constexpr (cx(q) > 0) cmx = cmx + v;
-        if constexpr (cx(q) < 0) cmx = cmx - v;
-        if constexpr (cy(q) > 0) cmy = cmy + v;
-        if constexpr (cy(q) < 0) cmy = cmy - v;
-        if constexpr (cz(q) > 0) cmz = cmz + v;
-        if constexpr (cz(q) < 0) cmz = cmz - v;
-    });
-}
-
-__device__ __forceinline__ void capture(float* __restrict__ density, float4* __restrict__ velocity, uint32_t gid,
-                                        float crho, float cmx, float cmy, float cmz) {
-    crho = fmaxf(crho, 1.0e-8f);
-    density[gid] = crho;
-    velocity[gid] = make_float4(cmx / crho, cmy / crho, cmz / crho, 0.0f);
-}
-
-// ---------------------------------------------------------------------------
-// Fast tiles.
-// ---------------------------------------------------------------------------
-template <int COLL, int S>
-__global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __restrict__ fin,
-                                                  typename Store<S>::T* __restrict__ fout,
-                                                  const uint8_t* __restrict__ static_flags,
-                                                  const uint8_t* __restrict__ dyn_prev,
-                                                  const uint8_t* __restrict__ dyn_cur,
-                                                  float* __restrict__ density, float4* __restrict__ velocity,
-                                                  StepParams p) {
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    if ((static_flags[tile] | dyn_prev[tile] | dyn_cur[tile]) != 0) return;   // fluid_general owns it
-    int x0, y0, z;
-    tile_origin(p.g, tile, x0, y0, z);
-    const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
-    const int y = y0 + (int)(threadIdx.x / TILE_X);
-    const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
-    const size_t n = p.g.n;
-
-    bool xm_far = false, xp_far = false, ym_far = false, yp_far = false, zm_far = false, zp_far = false;
-    // source coordinates for c = +1 (s = v - 1) and c = -1 (s = v + 1)
-    const int xs_p = shift_axis(x, 1, nx, p.bc[0][0], p.bc[0][1], xm_far);
-    const int xs_m = shift_axis(x, -1, nx, p.bc[0][0], p.bc[0][1], xp_far);
-    const int ys_p = shift_axis(y, 1, ny, p.bc[1][0], p.bc[1][1], ym_far);
-    const int ys_m = shift_axis(y, -1, ny, p.bc[1][0], p.bc[1][1], yp_far);
-    const int zs_p = shift_axis(z, 1, nz, p.bc[2][0], p.bc[2][1], zm_far);
-    const int zs_m = shift_axis(z, -1, nz, p.bc[2][0], p.bc[2][1], zp_far);
-
-    float f[19];
-    for_q(AllQ{}, [&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const int xs = cx(q) > 0 ? xs_p : (cx(q) < 0 ? xs_m : x);
-        const int ys = cy(q) > 0 ? ys_p : (cy(q) < 0 ? ys_m : y);
-        const int zs = cz(q) > 0 ? zs_p : (cz(q) < 0 ? zs_m : z);
-        const bool far = (cx(q) > 0 && xm_far) || (cx(q) < 0 && xp_far) || (cy(q) > 0 && ym_far)
-            || (cy(q) < 0 && yp_far) || (cz(q) > 0 && zm_far) || (cz(q) < 0 && zp_far);
-        const uint32_t src = (uint32_t)xs + (uint32_t)nx * ((uint32_t)ys + (uint32_t)ny * (uint32_t)zs);
-        f[q] = far ? p.far_eq[q] : Store<S>::load(fin, (size_t)q * n + src, q);
-    });
-
-    float rho, mx, my, mz;
-    moments(f, rho, mx, my, mz);
-    rho = fmaxf(rho, 1.0e-8f);
-    const float ux = mx / rho, uy = my / rho, uz = mz / rho;
-    float post[19];
-    float act, restr;
-    collide<COLL>(f, rho, ux, uy, uz, p.omega_plus, p.omega_minus, post, act, restr);
-    const uint32_t gid = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
-    float crho, cmx, cmy, cmz;
-    write_post<S>(fout, n, gid, post, 0.0f, p, crho, cmx, cmy, cmz);
-    if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
-}
 
 // Work list of the tiles that take the general path this step: flagged by
 // the static mask (partial or sponge tiles) or by solid cells of the
@@ -328,319 +26,7 @@ __global__ __launch_bounds__(256) void build_general_list(const uint8_t* __restr
         list[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u))] = t;
 }
 
-// ---------------------------------------------------------------------------
-// General tiles: full moving-boundary semantics of stepFluidTRT.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void cross3(float ax, float ay, float az, float bx, float by, float bz,
-                                       float& ox, float& oy, float& oz) {
-    ox = ay * bz - az * by;
-    oy = az * bx - ax * bz;
-    oz = ax * by - ay * bx;
-}
 
-__device__ __forceinline__ bool inside(const Grid& g, int x, int y, int z) {
-    return x >= 0 && y >= 0 && z >= 0 && x < g.nx && y < g.ny && z < g.nz;
-}
-
-__device__ __forceinline__ uint32_t flat(const Grid& g, int x, int y, int z) {
-    return (uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
-}
-
-struct CellLoad {
-    float fx, fy, fz, tx, ty, tz, act, restr;
-};
-
-template <int COLL, int S>
-__device__ void general_cell(const typename Store<S>::T* __restrict__ fin, typename Store<S>::T* __restrict__ fout,
-                             const uint8_t* __restrict__ solid_prev, const uint8_t* __restrict__ solid_cur,
-                             const float4* __restrict__ wall, float* __restrict__ density,
-                             float4* __restrict__ velocity, const StepParams& p, int x, int y, int z,
-                             CellLoad& L) {
-    const Grid& g = p.g;
-    const size_t n = g.n;
-    const uint32_t gid = flat(g, x, y, z);
-    const float wx = p.origin[0] + ((float)x + 0.5f) * p.dx;
-    const float wy = p.origin[1] + ((float)y + 0.5f) * p.dx;
-    const float wz = p.origin[2] + ((float)z + 0.5f) * p.dx;
-    const float ax = wx - p.body[0], ay = wy - p.body[1], az = wz - p.body[2];
-    const bool was_solid = solid_prev[gid] != 0;
-    const bool is_solid = solid_cur[gid] != 0;
-    const int mode = p.force_mode;
-    const bool include_link = p.load_component != 2;
-    const bool include_topology = p.load_component != 1;
-    const float fscale = p.force_scale;
-    L = CellLoad{0, 0, 0, 0, 0, 0, 0, 0};
-
-    if (is_solid) {
-        const float4 wv = wall[gid];
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            Store<S>::store(fout, (size_t)q * n + gid, q, equilibrium<float>(q, 1.0f, wv.x, wv.y, wv.z));
-        });
-        if (p.accumulate_loads && !was_solid && include_topology) {
-            float prho = 0.0f, pmx = 0.0f, pmy = 0.0f, pmz = 0.0f;
-            for_q(AllQ{}, [&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                float v = Store<S>::load(fin, (size_t)q * n + gid, q);
-                prho = prho + v;
-                pmx = pmx + v * (float)cx(q);
-                pmy = pmy + v * (float)cy(q);
-                pmz = pmz + v * (float)cz(q);
-            });
-            float cvx = mode == 6 ? pmx : pmx - prho * wv.x;
-            float cvy = mode == 6 ? pmy : pmy - prho * wv.y;
-            float cvz = mode == 6 ? pmz : pmz - prho * wv.z;
-            L.fx = cvx * fscale;
-            L.fy = cvy * fscale;
-            L.fz = cvz * fscale;
-            cross3(ax, ay, az, L.fx, L.fy, L.fz, L.tx, L.ty, L.tz);
-        }
-        if (p.capture_fields) {
-            density[gid] = 1.0f;
-            velocity[gid] = make_float4(wv.x, wv.y, wv.z, 0.0f);
-        }
-        return;
-    }
-
-    float f[19];
-    float rho = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
-    float flx = 0.0f, fly = 0.0f, flz = 0.0f;
-    float tqx = 0.0f, tqy = 0.0f, tqz = 0.0f;
-
-    if (was_solid) {
-        const float4 rv = wall[gid];
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            float v = equilibrium<float>(q, 1.0f, rv.x, rv.y, rv.z);
-            f[q] = v;
-            rho = rho + v;
-            mx = mx + v * (float)cx(q);
-            my = my + v * (float)cy(q);
-            mz = mz + v * (float)cz(q);
-        });
-        if (p.accumulate_loads && include_topology && mode == 6) {
-            float tx = -mx, ty = -my, tz = -mz;
-            for_q(AllQ{}, [&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                if constexpr (q != 0) {
-                    const int bx = x + cx(q), by = y + cy(q), bz = z + cz(q);
-                    if (inside(g, bx, by, bz)) {
-                        const uint32_t nb = flat(g, bx, by, bz);
-                        if (solid_prev[nb] == 0 && solid_cur[nb] == 0) {
-                            float s = Store<S>::load(fin, (size_t)q * n + gid, q)
-                                + Store<S>::load(fin, (size_t)opp(q) * n + nb, opp(q));
-                            tx = tx - s * (float)cx(q);
-                            ty = ty - s * (float)cy(q);
-                            tz = tz - s * (float)cz(q);
-                        }
-                    }
-                }
-            });
-            flx = flx + tx;
-            fly = fly + ty;
-            flz = flz + tz;
-            float ox, oy, oz;
-            cross3(ax, ay, az, tx * fscale, ty * fscale, tz * fscale, ox, oy, oz);
-            tqx = tqx + ox;
-            tqy = tqy + oy;
-            tqz = tqz + oz;
-        }
-    } else {
-        const float mwd = p.far_rho;
-        const bool interpolated_rule = p.wall_rule != WALL_HALFWAY;
-        const float4 wg = wall[gid];
-        for_q(AllQ{}, [&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr float dxq = (float)cx(q), dyq = (float)cy(q), dzq = (float)cz(q);
-            bool far = false;
-            const int sx = shift_axis(x, cx(q), g.nx, p.bc[0][0], p.bc[0][1], far);
-            const int sy = shift_axis(y, cy(q), g.ny, p.bc[1][0], p.bc[1][1], far);
-            const int sz = shift_axis(z, cz(q), g.nz, p.bc[2][0], p.bc[2][1], far);
-            float value;
-            if (far) {
-                value = p.far_eq[q];
-            } else {
-                const uint32_t src = flat(g, sx, sy, sz);
-                const uint8_t part = solid_cur[src];
-                if (part != 0) {
-                    const float reflected = Store<S>::load(fin, (size_t)opp(q) * n + gid, opp(q));
-                    float lf = 0.5f;
-                    const float4 ws = wall[src];
-                    float wvx = ws.x, wvy = ws.y, wvz = ws.z;
-                    uint32_t farther = 0;
-                    bool interp = interpolated_rule;
-                    if (interp) {
-                        if (p.wall_rule == WALL_INTERPOLATED_SDF) {
-                            // BirdFlow.metal:2062-2071: q from the fluid and
-                            // solid signed distances of this link.
-                            float df = fmaxf(wg.w, 0.0f);
-                            float dsv = fminf(ws.w, 0.0f);
-                            lf = fminf(fmaxf(df / fmaxf(df - dsv, 1.0e-6f), 1.0e-4f), 1.0f);
-                        }
-                        wvx = wg.x + (ws.x - wg.x) * lf;
-                        wvy = wg.y + (ws.y - wg.y) * lf;
-                        wvz = wg.z + (ws.z - wg.z) * lf;
-                        if (lf <= 0.5f) {
-                            const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
-                            const bool ok = inside(g, fx, fy, fz);
-                            if (ok) farther = flat(g, fx, fy, fz);
-                            if (!ok || solid_cur[farther] != 0) {
-                                interp = false;
-                                lf = 0.5f;
-                                wvx = ws.x;
-                                wvy = ws.y;
-                                wvz = ws.z;
-                            }
-                        }
-                    }
-                    const float dotdw = dxq * wvx + dyq * wvy + dzq * wvz;
-                    const float corr = 2.0f * weight<float>(q) * mwd * dotdw / cs2<float>();
-                    value = reflected + corr;
-                    if (interp) {
-                        if (lf <= 0.5f) {
-                            const float fo = Store<S>::load(fin, (size_t)opp(q) * n + farther, opp(q));
-                            value = 2.0f * lf * reflected + (1.0f - 2.0f * lf) * fo + corr;
-                        } else {
-                            const float pin = Store<S>::load(fin, (size_t)q * n + gid, q);
-                            value = (reflected + corr) / (2.0f * lf) + (2.0f * lf - 1.0f) * pin / (2.0f * lf);
-                        }
-                    }
-                    const int sel = p.selected_part;
-                    if (p.accumulate_loads && include_link && (sel == 0 || sel == (int)part)) {
-                        float lx = 0.0f, ly = 0.0f, lz = 0.0f;
-                        if (mode == 0) {
-                            lx = -(value * (dxq - wvx) - reflected * (-dxq - wvx));
-                            ly = -(value * (dyq - wvy) - reflected * (-dyq - wvy));
-                            lz = -(value * (dzq - wvz) - reflected * (-dzq - wvz));
-                        } else if (mode == 1 || mode == 6) {
-                            float s = -(value + reflected);
-                            lx = s * dxq;
-                            ly = s * dyq;
-                            lz = s * dzq;
-                        } else if (mode == 2) {
-                            float s = -2.0f * reflected;
-                            lx = s * dxq;
-                            ly = s * dyq;
-                            lz = s * dzq;
-                        } else if (mode == 3) {
-                            float s = -corr;
-                            lx = s * dxq;
-                            ly = s * dyq;
-                            lz = s * dzq;
-                        } else if (mode == 4) {
-                            float s = -(value - reflected - corr);
-                            lx = s * dxq;
-                            ly = s * dyq;
-                            lz = s * dzq;
-                        } else if (mode == 5) {
-                            float s = value - reflected;
-                            lx = s * wvx;
-                            ly = s * wvy;
-                            lz = s * wvz;
-                        }
-                        flx = flx + lx;
-                        fly = fly + ly;
-                        flz = flz + lz;
-                        const float bpx = wx - lf * dxq * p.dx - p.body[0];
-                        const float bpy = wy - lf * dyq * p.dx - p.body[1];
-                        const float bpz = wz - lf * dzq * p.dx - p.body[2];
-                        float ox, oy, oz;
-                        cross3(bpx, bpy, bpz, lx * fscale, ly * fscale, lz * fscale, ox, oy, oz);
-                        tqx = tqx + ox;
-                        tqy = tqy + oy;
-                        tqz = tqz + oz;
-                    }
-                } else {
-                    value = Store<S>::load(fin, (size_t)q * n + src, q);
-                }
-            }
-            f[q] = value;
-            rho = rho + value;
-            mx = mx + value * dxq;
-            my = my + value * dyq;
-            mz = mz + value * dzq;
-        });
-    }
-
-    rho = fmaxf(rho, 1.0e-8f);
-    const float ux = mx / rho, uy = my / rho, uz = mz / rho;
-    const float sponge = p.sponge_strength > 0.0f ? sponge_factor(p, x, y, z) : 0.0f;
-    float post[19];
-    float act, restr;
-    collide<COLL>(f, rho, ux, uy, uz, p.omega_plus, p.omega_minus, post, act, restr);
-    float crho, cmx, cmy, cmz;
-    write_post<S>(fout, n, gid, post, sponge, p, crho, cmx, cmy, cmz);
-    if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
-    if (p.accumulate_loads) {
-        L.fx = flx * fscale;
-        L.fy = fly * fscale;
-        L.fz = flz * fscale;
-        L.tx = tqx;
-        L.ty = tqy;
-        L.tz = tqz;
-        L.act = act;
-        L.restr = restr;
-    }
-}
-
-// Block sum of 8 FP64 lanes (sums for 0..6, max for 7) in a fixed tree:
-// wavefront shuffles, then 4 wavefront results combined by lane 0.
-__device__ __forceinline__ void block_reduce8(double (&v)[8], double* __restrict__ out) {
-    __shared__ double wave_part[4][8];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            double o = __shfl_down(v[i], off, 64);
-            v[i] = i == 7 ? fmax(v[i], o) : v[i] + o;
-        }
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) wave_part[wave][i] = v[i];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            double s = wave_part[0][i];
-            for (int w = 1; w < (int)(blockDim.x / 64); ++w)
-                s = i == 7 ? fmax(s, wave_part[w][i]) : s + wave_part[w][i];
-            out[i] = s;
-        }
-    }
-    __syncthreads();
-}
-
-template <int COLL, int S>
-__global__ __launch_bounds__(256) void fluid_general(const typename Store<S>::T* __restrict__ fin,
-                                                     typename Store<S>::T* __restrict__ fout,
-                                                     const uint8_t* __restrict__ solid_prev,
-                                                     const uint8_t* __restrict__ solid_cur,
-                                                     const float4* __restrict__ wall,
-                                                     const uint32_t* __restrict__ general_list,
-                                                     const uint32_t* __restrict__ general_count,
-                                                     double* __restrict__ partials,
-                                                     float* __restrict__ density, float4* __restrict__ velocity,
-                                                     StepParams p) {
-    const uint32_t count = *general_count;
-    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
-        const uint32_t tile = general_list[i];
-        int x0, y0, z;
-        tile_origin(p.g, tile, x0, y0, z);
-        const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
-        const int y = y0 + (int)(threadIdx.x / TILE_X);
-        CellLoad L{0, 0, 0, 0, 0, 0, 0, 0};
-        if (x < p.g.nx && y < p.g.ny)
-            general_cell<COLL, S>(fin, fout, solid_prev, solid_cur, wall, density, velocity, p, x, y, z, L);
-        if (p.accumulate_loads) {
-            double v[8] = {(double)L.fx, (double)L.fy, (double)L.fz, (double)L.tx,
-                           (double)L.ty, (double)L.tz, (double)L.act, (double)L.restr};
-            block_reduce8(v, partials + (size_t)tile * 8);
-        }
-    }
-}
 
 // Fixed-order sum of the per-tile partials of the tiles the general kernel
 // processed (flags identify them; other tiles contribute exactly zero).
@@ -663,7 +49,10 @@ __global__ __launch_bounds__(256) void reduce_loads_tiles(const double* __restri
     block_reduce8(v, block_out + (size_t)blockIdx.x * 8);
 }
 
+// Adds the fast tiles' limiter activations (an integer count and a maximum,
+// exact in any order) and clears them for the step that reuses the slot.
 __global__ __launch_bounds__(256) void reduce_loads_final(const double* __restrict__ block_out, uint32_t blocks,
+                                                          uint32_t* __restrict__ activation,
                                                           double* __restrict__ last, double* __restrict__ history,
                                                           uint32_t history_slot) {
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -675,9 +64,14 @@ __global__ __launch_bounds__(256) void reduce_loads_final(const double* __restri
     __shared__ double total[8];
     block_reduce8(v, total);
     if (threadIdx.x < 8) {
-        last[threadIdx.x] = total[threadIdx.x];
-        if (history) history[(size_t)history_slot * 8 + threadIdx.x] = total[threadIdx.x];
+        double t = total[threadIdx.x];
+        if (threadIdx.x == 6) t += (double)activation[0];
+        if (threadIdx.x == 7) t = fmax(t, (double)__uint_as_float(activation[1]));
+        last[threadIdx.x] = t;
+        if (history) history[(size_t)history_slot * 8 + threadIdx.x] = t;
     }
+    __syncthreads();
+    if (threadIdx.x < 2) activation[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -751,65 +145,33 @@ __global__ void unpack_populations(const typename Store<S>::T* __restrict__ src,
     dst[i] = Store<S>::load(src, i, (int)(i / n));
 }
 
-// ---------------------------------------------------------------------------
-// Host-side launchers (called from bf_api.hip).
-// ---------------------------------------------------------------------------
-
-template <int COLL, int S>
-static hipError_t launch_fast_t(const FluidBuffers& b, const StepParams& p, hipStream_t s) {
-    using T = typename Store<S>::T;
-    hipLaunchKernelGGL((fluid_fast<COLL, S>), dim3(p.g.tiles), dim3(256), 0, s, (const T*)b.fin, (T*)b.fout,
-                       b.static_flags, b.dyn_prev, b.dyn_cur, b.density, b.velocity, p);
-    return hipGetLastError();
-}
-
-template <int COLL, int S>
-static hipError_t launch_general_t(const FluidBuffers& b, const StepParams& p, int general_blocks, hipStream_t s) {
-    using T = typename Store<S>::T;
-    hipLaunchKernelGGL((fluid_general<COLL, S>), dim3(general_blocks), dim3(256), 0, s, (const T*)b.fin,
-                       (T*)b.fout, b.solid_prev, b.solid_cur, b.wall, b.general_list, b.general_count,
-                       b.partials, b.density, b.velocity, p);
-    return hipGetLastError();
-}
 
 hipError_t launch_load_reduction(const FluidBuffers& b, const StepParams& p, uint32_t history_slot, hipStream_t s) {
     if (!p.accumulate_loads) return hipSuccess;
     const uint32_t blocks = (p.g.tiles + 1023) / 1024;
     hipLaunchKernelGGL(reduce_loads_tiles, dim3(blocks), dim3(256), 0, s, b.partials, b.static_flags, b.dyn_prev,
                        b.dyn_cur, p.g.tiles, b.block_out);
-    hipLaunchKernelGGL(reduce_loads_final, dim3(1), dim3(256), 0, s, b.block_out, blocks, b.last_load, b.history,
-                       history_slot);
+    hipLaunchKernelGGL(reduce_loads_final, dim3(1), dim3(256), 0, s, b.block_out, blocks, b.activation, b.last_load,
+                       b.history, history_slot);
     return hipGetLastError();
 }
 
-#define BF_DISPATCH(storage, coll, CALL)                                                              \
-    switch (storage * 4 + coll) {                                                                     \
-        case 0: return CALL(COLL_TRT, STORE_FP32);                                                    \
-        case 1: return CALL(COLL_TRT_SYMMETRIC_LIMITER, STORE_FP32);                                  \
-        case 2: return CALL(COLL_REGULARIZED, STORE_FP32);                                            \
-        case 3: return CALL(COLL_RR3, STORE_FP32);                                                    \
-        case 4: return CALL(COLL_TRT, STORE_FP16_OFFSET);                                             \
-        case 5: return CALL(COLL_TRT_SYMMETRIC_LIMITER, STORE_FP16_OFFSET);                           \
-        case 6: return CALL(COLL_REGULARIZED, STORE_FP16_OFFSET);                                     \
-        case 7: return CALL(COLL_RR3, STORE_FP16_OFFSET);                                             \
-        case 8: return CALL(COLL_TRT, STORE_BF16_OFFSET);                                             \
-        case 9: return CALL(COLL_TRT_SYMMETRIC_LIMITER, STORE_BF16_OFFSET);                           \
-        case 10: return CALL(COLL_REGULARIZED, STORE_BF16_OFFSET);                                    \
-        case 11: return CALL(COLL_RR3, STORE_BF16_OFFSET);                                            \
-        default: return hipErrorInvalidValue;                                                         \
+hipError_t launch_fluid_fast(int arith, int storage, int coll, const FluidBuffers& b, const StepParams& p,
+                             hipStream_t s) {
+    if (arith == ARITH_STRICT) {
+        if (storage != STORE_FP32) return hipErrorInvalidValue;
+        return launch_fluid_fast_strict(coll, b, p, s);
     }
-
-hipError_t launch_fluid_fast(int storage, int coll, const FluidBuffers& b, const StepParams& p, hipStream_t s) {
-#define CALL(C, S) launch_fast_t<C, S>(b, p, s)
-    BF_DISPATCH(storage, coll, CALL)
-#undef CALL
+    return launch_fluid_fast_fused(storage, coll, b, p, s);
 }
 
-hipError_t launch_fluid_general(int storage, int coll, const FluidBuffers& b, const StepParams& p, int general_blocks,
-                                hipStream_t s) {
-#define CALL(C, S) launch_general_t<C, S>(b, p, general_blocks, s)
-    BF_DISPATCH(storage, coll, CALL)
-#undef CALL
+hipError_t launch_fluid_general(int arith, int storage, int coll, const FluidBuffers& b, const StepParams& p,
+                                int general_blocks, hipStream_t s) {
+    if (arith == ARITH_STRICT) {
+        if (storage != STORE_FP32) return hipErrorInvalidValue;
+        return launch_fluid_general_strict(coll, b, p, general_blocks, s);
+    }
+    return launch_fluid_general_fused(storage, coll, b, p, general_blocks, s);
 }
 
 hipError_t launch_build_general_list(const uint8_t* static_flags, const uint8_t* dyn_prev, const uint8_t* dyn_cur,

@@ -26,6 +26,7 @@ struct FluidBuffers {
     double* history;
     float* density;
     float4* velocity;
+    uint32_t* activation;   // [count, max restriction bits] of this step's fast tiles
 };
 
 struct MeshParams {
@@ -62,9 +63,19 @@ struct MeshBuffers {
     uint8_t* dyn;
 };
 
-hipError_t launch_fluid_fast(int storage, int coll, const FluidBuffers& b, const StepParams& p, hipStream_t s);
-hipError_t launch_fluid_general(int storage, int coll, const FluidBuffers& b, const StepParams& p, int general_blocks,
-                                hipStream_t s);
+hipError_t launch_fluid_fast(int arith, int storage, int coll, const FluidBuffers& b, const StepParams& p,
+                             hipStream_t s);
+hipError_t launch_fluid_general(int arith, int storage, int coll, const FluidBuffers& b, const StepParams& p,
+                                int general_blocks, hipStream_t s);
+// Per translation unit: fused arithmetic for every storage, strict
+// arithmetic for FP32 storage.
+hipError_t launch_fluid_fast_fused(int storage, int coll, const FluidBuffers& b, const StepParams& p,
+                                   hipStream_t s);
+hipError_t launch_fluid_general_fused(int storage, int coll, const FluidBuffers& b, const StepParams& p,
+                                      int general_blocks, hipStream_t s);
+hipError_t launch_fluid_fast_strict(int coll, const FluidBuffers& b, const StepParams& p, hipStream_t s);
+hipError_t launch_fluid_general_strict(int coll, const FluidBuffers& b, const StepParams& p, int general_blocks,
+                                       hipStream_t s);
 hipError_t launch_load_reduction(const FluidBuffers& b, const StepParams& p, uint32_t history_slot, hipStream_t s);
 hipError_t launch_build_general_list(const uint8_t* static_flags, const uint8_t* dyn_prev, const uint8_t* dyn_cur,
                                      uint32_t tiles, uint32_t* list, uint32_t* count, hipStream_t s);

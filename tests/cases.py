@@ -100,9 +100,13 @@ def deetjen_frame_points(d, frame, points_by_frame):
 # GPU-side helpers: a solver configuration equal to an oracle parameter block,
 # and the oracle's indexed-mesh replay loop.
 # ---------------------------------------------------------------------------
-def solver_config(p, storage=0, collision=None, time_step=1.0, velocity_to_lattice=1.0):
+def solver_config(p, storage=0, collision=None, time_step=1.0, velocity_to_lattice=1.0, arithmetic=None):
+    """SolverConfig of oracle parameters p.  FP32 storage defaults to strict
+    arithmetic (bit-comparable with the oracle), reduced storage to fused."""
     import birdflow_hip as bh
-    return bh.SolverConfig(
+    if arithmetic is None:
+        arithmetic = bh.ARITHMETIC_STRICT if storage == bh.STORAGE_FP32 else bh.ARITHMETIC_FUSED
+    return bh.SolverConfig(arithmetic=arithmetic,
         shape=(p.nx, p.ny, p.nz), omega_plus=float(p.omega_plus), omega_minus=float(p.omega_minus),
         storage=storage, collision=p.collision if collision is None else collision, wall_rule=p.wall_rule,
         force_mode=p.force_mode, load_component=p.load_component, selected_part=p.selected_part,

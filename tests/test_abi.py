@@ -32,11 +32,13 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_config_struct_size_matches_header():
-    # 10 uint32 + 6 uint32 boundary + 3+1+1+1+1 + 2 + 2 + 3 + 1 + 3 floats + 8 reserved
-    assert ctypes.sizeof(_abi.Config) == 4 * (10 + 6 + 3 + 1 + 1 + 1 + 1 + 2 + 2 + 3 + 1 + 3 + 8)
+    # 10 uint32 + 6 uint32 boundary + 3+1+1+1+1 + 2 + 2 + 3 + 1 + 3 floats + arithmetic + 7 reserved
+    assert ctypes.sizeof(_abi.Config) == 4 * (10 + 6 + 3 + 1 + 1 + 1 + 1 + 2 + 2 + 3 + 1 + 3 + 1 + 7)
+    assert _abi.Config.arithmetic.offset == 4 * (10 + 6 + 3 + 1 + 1 + 1 + 1 + 2 + 2 + 3 + 1 + 3)
 
 
-@pytest.mark.parametrize("field,value", [("nx", 0), ("omega_plus", 2.5), ("storage", 7), ("force_mode", 9)])
+@pytest.mark.parametrize("field,value", [("nx", 0), ("omega_plus", 2.5), ("storage", 7), ("force_mode", 9),
+                                         ("arithmetic", 2)])
 def test_invalid_configuration_is_rejected_before_device_use(field, value):
     cfg = bh.SolverConfig(shape=(16, 16, 16), omega_plus=1.25, omega_minus=0.75)
     c = cfg.to_c()
@@ -45,6 +47,13 @@ def test_invalid_configuration_is_rejected_before_device_use(field, value):
     st = _abi.load().bf_create(ctypes.byref(c), ctypes.byref(handle))
     assert st == _abi.ERROR_INVALID_CONFIGURATION
     assert not handle.value
+
+
+def test_strict_arithmetic_requires_fp32_storage():
+    cfg = bh.SolverConfig(shape=(16, 16, 16), omega_plus=1.25, omega_minus=0.75, storage=bh.STORAGE_FP16_OFFSET,
+                          arithmetic=bh.ARITHMETIC_STRICT)
+    handle = ctypes.c_void_p()
+    assert _abi.load().bf_create(ctypes.byref(cfg.to_c()), ctypes.byref(handle)) == _abi.ERROR_INVALID_CONFIGURATION
 
 
 def test_mixed_periodic_axis_is_rejected():
