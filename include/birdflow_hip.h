@@ -208,6 +208,51 @@ bf_status bf_read_profile(bf_context* ctx, double out[6]);
  * figures and hardware-counter byte counts. */
 bf_status bf_copy_bandwidth(uint64_t bytes, uint32_t iterations, double* gbs);
 
+/* ---- Rigid body (six degrees of freedom, FP64 on the device) ----
+ * Replaces integrateBirdBody / monitorBirdRuntimeSafety (BirdFlow.metal:
+ * 6574-6707), RigidBodyIntegrator.integrate (RigidBody.swift:85-126) and the
+ * host side of freeFlight stepping (BirdFlowSimulation.swift:1412-1424).
+ * Units are world units (those of origin, cell_size and time_step).  The
+ * fluid kernels take the torque reference point from the device body
+ * position; bf_create places the body at config.body_position, at rest. */
+typedef struct bf_body_state {
+    double position[3];
+    double orientation[4];            /* body to world, quaternion x, y, z, w */
+    double linear_velocity[3];
+    double angular_velocity_body[3];  /* body frame */
+} bf_body_state;
+
+typedef struct bf_body_params {
+    double mass;                      /* bodyRadiiAndMass.w */
+    double inertia[3];                /* principal moments, body frame */
+    double gravity[3];                /* world acceleration (uniforms.gravity.xyz) */
+    double external_force[3];         /* added to the fluid load every step (world): */
+    double external_torque[3];        /*   e.g. wing hinge reactions (BirdFlow.metal:6586-6591) */
+    double safety_extent[3];          /* body-frame half extents (safetyGeometry.xyz) */
+    double safety_radius;             /* rotation radius (safetyGeometry.w) */
+    double wing_speed;                /* added surface speed (safetyLimits.x) */
+    double mach_limit;                /* safetyLimits.y; <= 0 turns the monitor off */
+    uint32_t substeps;                /* integration.x, >= 1 */
+    uint32_t free_flight;             /* flags.x: integrate after every step (needs accumulate_loads) */
+    uint32_t move_mesh;               /* mesh frames are body-frame; the body pose places them each step */
+    uint32_t reserved;
+} bf_body_params;
+
+/* Set the body state and/or parameters (either may be NULL).  Resets the
+ * safety record and the body history.  With move_mesh, call it before
+ * bf_mesh_upload / bf_mesh_build so that the first geometry uses the pose. */
+bf_status bf_body_set(bf_context* ctx, const bf_body_state* state, const bf_body_params* params);
+bf_status bf_body_get(bf_context* ctx, bf_body_state* state);
+/* Body state after each of the last min(max_records, steps) free-flight
+ * steps, oldest first. */
+bf_status bf_read_body_history(bf_context* ctx, bf_body_state* out, uint32_t max_records, uint32_t* records);
+/* Safety record: metrics = max Mach, min clearance, Mach and clearance at
+ * the first event; flags = 1 Mach above the limit, 2 clearance below zero,
+ * 4 non-finite state (GPURuntimeSafetyRecord, GPUData.swift:234-245).
+ * bf_synchronize returns BF_ERROR_RUNTIME_SAFETY_VIOLATION once when a new
+ * event is recorded; the context stays usable. */
+bf_status bf_read_safety(bf_context* ctx, double metrics[4], uint64_t* event_step, uint32_t* flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,7 @@ ERROR_DEVICE_UNAVAILABLE = 1
 ERROR_INVALID_CONFIGURATION = 2
 ERROR_INVALID_ADVANCE_REQUEST = 6
 ERROR_INVALID_ARGUMENT = 7
+ERROR_RUNTIME_SAFETY_VIOLATION = 9
 ERROR_STATE_INVALIDATED = 10
 
 BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
@@ -41,7 +42,7 @@ EXPORTED_FUNCTIONS = (
     "bf_read_populations", "bf_read_fields", "bf_step_index", "bf_set_geometry", "bf_read_geometry",
     "bf_mesh_upload", "bf_mesh_build", "bf_set_translating_sphere", "bf_step", "bf_synchronize",
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
-    "bf_copy_bandwidth",
+    "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
 )
 
 
@@ -69,6 +70,35 @@ class Config(ctypes.Structure):
         ("body_position", ctypes.c_float * 3),
         ("arithmetic", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32 * 7),
+    ]
+
+
+class BodyState(ctypes.Structure):
+    """bf_body_state: FP64 rigid-body state (world units)."""
+    _fields_ = [
+        ("position", ctypes.c_double * 3),
+        ("orientation", ctypes.c_double * 4),
+        ("linear_velocity", ctypes.c_double * 3),
+        ("angular_velocity_body", ctypes.c_double * 3),
+    ]
+
+
+class BodyParams(ctypes.Structure):
+    """bf_body_params."""
+    _fields_ = [
+        ("mass", ctypes.c_double),
+        ("inertia", ctypes.c_double * 3),
+        ("gravity", ctypes.c_double * 3),
+        ("external_force", ctypes.c_double * 3),
+        ("external_torque", ctypes.c_double * 3),
+        ("safety_extent", ctypes.c_double * 3),
+        ("safety_radius", ctypes.c_double),
+        ("wing_speed", ctypes.c_double),
+        ("mach_limit", ctypes.c_double),
+        ("substeps", ctypes.c_uint32),
+        ("free_flight", ctypes.c_uint32),
+        ("move_mesh", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -122,6 +152,10 @@ def _declare(l):
         "bf_set_profiling": (st, [vp, ctypes.c_int]),
         "bf_read_profile": (st, [vp, vp]),
         "bf_copy_bandwidth": (st, [u64, u32, ctypes.POINTER(ctypes.c_double)]),
+        "bf_body_set": (st, [vp, ctypes.POINTER(BodyState), ctypes.POINTER(BodyParams)]),
+        "bf_body_get": (st, [vp, ctypes.POINTER(BodyState)]),
+        "bf_read_body_history": (st, [vp, vp, u32, ctypes.POINTER(u32)]),
+        "bf_read_safety": (st, [vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(l, name)

@@ -142,6 +142,52 @@ def copy_bandwidth(bytes_per_buffer: int = 1 << 30, iterations: int = 20) -> flo
     return float(out.value)
 
 
+@dataclass
+class BodyParameters:
+    """Rigid-body parameters (bf_body_params), world units."""
+
+    mass: float
+    inertia: Sequence[float]
+    gravity: Sequence[float] = (0.0, 0.0, 0.0)
+    external_force: Sequence[float] = (0.0, 0.0, 0.0)
+    external_torque: Sequence[float] = (0.0, 0.0, 0.0)
+    safety_extent: Sequence[float] = (0.0, 0.0, 0.0)
+    safety_radius: float = 0.0
+    wing_speed: float = 0.0
+    mach_limit: float = 0.0          # <= 0: no safety monitor
+    substeps: int = 1
+    free_flight: bool = True
+    move_mesh: bool = False
+
+    def to_c(self) -> _abi.BodyParams:
+        b = _abi.BodyParams()
+        b.mass = float(self.mass)
+        for name in ("inertia", "gravity", "external_force", "external_torque", "safety_extent"):
+            getattr(b, name)[:] = [float(v) for v in getattr(self, name)]
+        b.safety_radius, b.wing_speed, b.mach_limit = self.safety_radius, self.wing_speed, self.mach_limit
+        b.substeps = int(self.substeps)
+        b.free_flight, b.move_mesh = int(bool(self.free_flight)), int(bool(self.move_mesh))
+        return b
+
+
+def _body_to_c(state) -> _abi.BodyState:
+    """13-vector (position, orientation x y z w, linear velocity, body angular velocity)."""
+    v = np.asarray(state, np.float64).ravel()
+    if v.size != 13:
+        raise ValueError("a body state has 13 components")
+    b = _abi.BodyState()
+    b.position[:] = v[0:3].tolist()
+    b.orientation[:] = v[3:7].tolist()
+    b.linear_velocity[:] = v[7:10].tolist()
+    b.angular_velocity_body[:] = v[10:13].tolist()
+    return b
+
+
+def _body_from_c(b) -> np.ndarray:
+    return np.array(list(b.position) + list(b.orientation) + list(b.linear_velocity)
+                    + list(b.angular_velocity_body), np.float64)
+
+
 class BirdFlowSimulation:
     """One solver context on the current HIP device."""
 
@@ -264,6 +310,34 @@ class BirdFlowSimulation:
         n = ctypes.c_uint32(0)
         self._check(self._lib.bf_read_load_history(self._ctx, _ptr(out), max_records, ctypes.byref(n)))
         return out[: n.value].copy()
+
+    # -- rigid body ------------------------------------------------------
+    def set_body(self, state=None, params: Optional[BodyParameters] = None):
+        """Body state as a 13-vector (position, orientation x y z w, linear
+        velocity, body angular velocity) and/or parameters."""
+        cs = _body_to_c(state) if state is not None else None
+        cp = params.to_c() if params is not None else None
+        self._check(self._lib.bf_body_set(self._ctx, ctypes.byref(cs) if cs is not None else None,
+                                          ctypes.byref(cp) if cp is not None else None))
+
+    def body_state(self) -> np.ndarray:
+        b = _abi.BodyState()
+        self._check(self._lib.bf_body_get(self._ctx, ctypes.byref(b)))
+        return _body_from_c(b)
+
+    def body_history(self, max_records: int = 1 << 16) -> np.ndarray:
+        arr = (_abi.BodyState * max_records)()
+        n = ctypes.c_uint32(0)
+        self._check(self._lib.bf_read_body_history(self._ctx, arr, max_records, ctypes.byref(n)))
+        return np.array([_body_from_c(arr[i]) for i in range(n.value)], np.float64).reshape(-1, 13)
+
+    def safety(self):
+        """(metrics [max Mach, min clearance, Mach, clearance at the first event], event step, flags)."""
+        m = np.zeros(4, np.float64)
+        step = ctypes.c_uint64(0)
+        flags = ctypes.c_uint32(0)
+        self._check(self._lib.bf_read_safety(self._ctx, _ptr(m), ctypes.byref(step), ctypes.byref(flags)))
+        return m, int(step.value), int(flags.value)
 
     def link_census(self):
         out = np.zeros(21, np.uint64)

@@ -1,0 +1,632 @@
+// Hand-written adjoint of the moving-boundary step for gfx950.
+//
+// For J = sum_s c_s . L_s (L_s: force and torque of step s), the reverse
+// sweep carries lambda_s = dJ/df_s back through each step with the masks of
+// the step fixed, and accumulates dJ/dtheta through the wall field of every
+// step (link fractions from the signed distances and wall velocities), the
+// wall field's tangents d wall / d theta coming from the mesh tangents.
+//
+// The forward step pulls: g_q(x) = f_q(x - c_q), or the bounce-back value of
+// a wall link, or the far-field equilibrium; then collides.  Its adjoint is
+// split so that every write has exactly one writer (no atomics, results are
+// deterministic):
+//   adjoint_fast / adjoint_general_p1  each cell x computes
+//        mu = (d post / d g)^T lambda_s(x)  and writes lambda_{s-1,q}(x - c_q),
+//        whose only puller is x (zero where that pull is a wall link or x is
+//        solid or refilled; entries nobody pulls are zeroed by their owner);
+//   adjoint_general_p2a  adds what a cell's own populations feed besides the
+//        regular pull: reflected and x-side populations of its links, the
+//        cover / uncover impulses; and the wall-field gradient, contracted
+//        with the tangents into per-tile FP64 partials;
+//   adjoint_general_p2b  adds the far-side populations x + c_q of
+//        interpolated links and of uncover impulses.
+// Targets of p2a are the cell itself; a target of p2b has one contributor.
+//
+// Supported: FP32 storage, TRT / regularized / RR3, periodic and far-field
+// faces, sponge, every wall rule, force mode 6.
+#include "bf_adjoint_math.h"
+#include "bf_fluid.h"
+
+namespace bf {
+
+constexpr uint32_t NO_TRIANGLE = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void cross3f(const float a[3], const float b[3], float o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// c_q . (cF + cT x arm): d J / d (link or impulse momentum along c_q) per
+// unit of force scale
+template <int q>
+__device__ __forceinline__ float load_weight(const AdjointParams& w, const float arm[3]) {
+    float t[3];
+    cross3f(w.cT, arm, t);
+    return (float)cx(q) * (w.cF[0] + t[0]) + (float)cy(q) * (w.cF[1] + t[1]) + (float)cz(q) * (w.cF[2] + t[2]);
+}
+
+// Sources of the six axis directions with their far flags, as fluid_fast.
+struct Sources {
+    int xs_p, xs_m, ys_p, ys_m, zs_p, zs_m;
+    bool xm_far, xp_far, ym_far, yp_far, zm_far, zp_far;
+};
+
+__device__ __forceinline__ Sources sources(const StepParams& p, int x, int y, int z) {
+    Sources s{};
+    s.xm_far = s.xp_far = s.ym_far = s.yp_far = s.zm_far = s.zp_far = false;
+    s.xs_p = shift_axis(x, 1, p.g.nx, p.bc[0][0], p.bc[0][1], s.xm_far);
+    s.xs_m = shift_axis(x, -1, p.g.nx, p.bc[0][0], p.bc[0][1], s.xp_far);
+    s.ys_p = shift_axis(y, 1, p.g.ny, p.bc[1][0], p.bc[1][1], s.ym_far);
+    s.ys_m = shift_axis(y, -1, p.g.ny, p.bc[1][0], p.bc[1][1], s.yp_far);
+    s.zs_p = shift_axis(z, 1, p.g.nz, p.bc[2][0], p.bc[2][1], s.zm_far);
+    s.zs_m = shift_axis(z, -1, p.g.nz, p.bc[2][0], p.bc[2][1], s.zp_far);
+    return s;
+}
+
+template <int q>
+__device__ __forceinline__ uint32_t source_of(const Sources& s, const Grid& g, int x, int y, int z, bool& far) {
+    const int xs = cx(q) > 0 ? s.xs_p : (cx(q) < 0 ? s.xs_m : x);
+    const int ys = cy(q) > 0 ? s.ys_p : (cy(q) < 0 ? s.ys_m : y);
+    const int zs = cz(q) > 0 ? s.zs_p : (cz(q) < 0 ? s.zs_m : z);
+    far = (cx(q) > 0 && s.xm_far) || (cx(q) < 0 && s.xp_far) || (cy(q) > 0 && s.ym_far) || (cy(q) < 0 && s.yp_far) ||
+          (cz(q) > 0 && s.zm_far) || (cz(q) < 0 && s.zp_far);
+    return (uint32_t)xs + (uint32_t)g.nx * ((uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs);
+}
+
+// (x, q) is pulled by nobody when x + c_q leaves the domain through a
+// far-field face
+template <int q>
+__device__ __forceinline__ bool orphan(const Sources& s) {
+    return (cx(q) > 0 && s.xp_far) || (cx(q) < 0 && s.xm_far) || (cy(q) > 0 && s.yp_far) || (cy(q) < 0 && s.ym_far) ||
+           (cz(q) > 0 && s.zp_far) || (cz(q) < 0 && s.zm_far);
+}
+
+template <int COLL>
+__device__ __forceinline__ void collision_adjoint(const float (&g)[19], float op, float om, const float (&lp)[19],
+                                                  float (&mu)[19]) {
+    adj::reverse<COLL, float>(g, op, om, lp, mu);
+}
+
+// ---------------------------------------------------------------------------
+// Wall-free tiles.
+// ---------------------------------------------------------------------------
+template <int COLL>
+__global__ __launch_bounds__(256) void adjoint_fast(AdjointBuffers b, StepParams p) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    if ((b.static_flags[tile] | b.dyn_prev[tile] | b.dyn_cur[tile]) != 0) return;
+    int x0, y0, z;
+    tile_origin(p.g, tile, x0, y0, z);
+    const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+    const int y = y0 + (int)(threadIdx.x / TILE_X);
+    const size_t n = p.g.n;
+    const uint32_t gid = flat(p.g, x, y, z);
+    const Sources s = sources(p, x, y, z);
+    float g[19], lp[19], mu[19];
+    uint32_t src[19];
+    bool far[19];
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        src[q] = source_of<q>(s, p.g, x, y, z, far[q]);
+        const float v = b.f_prev[(size_t)q * n + (far[q] ? gid : src[q])];   // unconditional load
+        g[q] = far[q] ? p.far_eq[q] : v;
+        lp[q] = b.lam_in[(size_t)q * n + gid];
+    });
+    collision_adjoint<COLL>(g, p.omega_plus, p.omega_minus, lp, mu);
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if (!far[q]) b.lam_out[(size_t)q * n + src[q]] = mu[q];
+        if (orphan<q>(s)) b.lam_out[(size_t)q * n + gid] = 0.0f;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Wall tiles, pass 1: collision adjoint and the single-writer pull targets.
+// nu (per listed tile, direction-major, 256 lanes): for a fluid cell the
+// sensitivity of J to each wall-link value (collision part plus the link's
+// own load), for a refilled cell mu.
+// ---------------------------------------------------------------------------
+template <int COLL>
+__device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const StepParams& p,
+                                                const AdjointParams& w, int x, int y, int z, float bx, float by,
+                                                float bz, float* __restrict__ nu_cell) {
+    const Grid& g = p.g;
+    const size_t n = g.n;
+    const uint32_t gid = flat(g, x, y, z);
+    const bool is_solid = b.solid_cur[gid] != 0;
+    const bool was_solid = b.solid_prev[gid] != 0;
+    const Sources s = sources(p, x, y, z);
+    uint32_t src[19];
+    bool far[19];
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        src[q] = source_of<q>(s, g, x, y, z, far[q]);
+        if (orphan<q>(s)) b.lam_out[(size_t)q * n + gid] = 0.0f;
+    });
+    if (is_solid) {
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (!far[q]) b.lam_out[(size_t)q * n + src[q]] = 0.0f;
+            nu_cell[q * 256] = 0.0f;
+        });
+        return;
+    }
+    const float sponge = p.sponge_strength > 0.0f ? sponge_factor(p, x, y, z) : 0.0f;
+    float lp[19], gq[19], mu[19];
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        lp[q] = b.lam_in[(size_t)q * n + gid] * (1.0f - sponge);
+    });
+    if (was_solid) {
+        // refilled with feq(1, u_wall): no dependence on the pulled populations
+        const float4 rv = b.wall[gid];
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            gq[q] = equilibrium<float>(q, 1.0f, rv.x, rv.y, rv.z);
+        });
+        collision_adjoint<COLL>(gq, p.omega_plus, p.omega_minus, lp, mu);
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (!far[q]) b.lam_out[(size_t)q * n + src[q]] = 0.0f;
+            nu_cell[q * 256] = mu[q];
+        });
+        return;
+    }
+    const float4 wg4 = b.wall[gid];
+    const float wgv[4] = {wg4.x, wg4.y, wg4.z, wg4.w};
+    const float wx = p.origin[0] + ((float)x + 0.5f) * p.dx;
+    const float wy = p.origin[1] + ((float)y + 0.5f) * p.dx;
+    const float wz = p.origin[2] + ((float)z + 0.5f) * p.dx;
+    bool link[19];
+    float load_dv[19];
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        link[q] = false;
+        load_dv[q] = 0.0f;
+        if (far[q]) {
+            gq[q] = p.far_eq[q];
+            return;
+        }
+        const uint8_t part = b.solid_cur[src[q]];
+        if (part == 0) {
+            gq[q] = b.f_prev[(size_t)q * n + src[q]];
+            return;
+        }
+        link[q] = true;
+        const float4 ws4 = b.wall[src[q]];
+        const float wsv[4] = {ws4.x, ws4.y, ws4.z, ws4.w};
+        const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+        const bool fin_ok = inside(g, fx, fy, fz);
+        const uint32_t fid = fin_ok ? flat(g, fx, fy, fz) : gid;
+        const bool farther_ok = fin_ok && b.solid_cur[fid] == 0;
+        const float refl = b.f_prev[(size_t)opp(q) * n + gid];
+        const float farther = farther_ok ? b.f_prev[(size_t)opp(q) * n + fid] : 0.0f;
+        const float pin = b.f_prev[(size_t)q * n + gid];
+        const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
+                                                           p.wall_rule, farther_ok, refl, farther, pin, wgv, wsv);
+        gq[q] = d.value;
+        const int sel = p.selected_part;
+        if (p.accumulate_loads && p.load_component != 2 && (sel == 0 || sel == (int)part)) {
+            const float arm[3] = {wx - d.lf * (float)cx(q) * p.dx - bx, wy - d.lf * (float)cy(q) * p.dx - by,
+                                  wz - d.lf * (float)cz(q) * p.dx - bz};
+            // J_link = -fscale (value + reflected) c_q . e(arm)
+            load_dv[q] = -p.force_scale * load_weight<q>(w, arm);
+        }
+    });
+    collision_adjoint<COLL>(gq, p.omega_plus, p.omega_minus, lp, mu);
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if (!far[q]) b.lam_out[(size_t)q * n + src[q]] = link[q] ? 0.0f : mu[q];
+        nu_cell[q * 256] = link[q] ? mu[q] + load_dv[q] : 0.0f;
+    });
+}
+
+template <int COLL>
+__global__ __launch_bounds__(256) void adjoint_general_p1(AdjointBuffers b, StepParams p, AdjointParams w) {
+    const uint32_t count = *b.general_count;
+    const float bx = (float)b.body[0], by = (float)b.body[1], bz = (float)b.body[2];
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        int x0, y0, z;
+        tile_origin(p.g, b.general_list[i], x0, y0, z);
+        const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+        const int y = y0 + (int)(threadIdx.x / TILE_X);
+        if (x < p.g.nx && y < p.g.ny)
+            general_p1_cell<COLL>(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x);
+    }
+}
+
+// Sum of 8 FP64 lanes over the workgroup (fixed tree).
+__device__ __forceinline__ void block_sum8(double (&v)[8], double* __restrict__ out) {
+    __shared__ double part[4][8];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] += __shfl_down(v[i], off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) part[wave][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            double s = part[0][i];
+            for (int k = 1; k < (int)(blockDim.x / 64); ++k) s += part[k][i];
+            out[i] = s;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void contract(double (&acc)[8], int K, const float4* __restrict__ dW, size_t n,
+                                         uint32_t cell, float gx, float gy, float gz, float gw) {
+    for (int k = 0; k < K; ++k) {
+        const float4 t = dW[(size_t)k * n + cell];
+        acc[k] += (double)gx * t.x + (double)gy * t.y + (double)gz * t.z + (double)gw * t.w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wall tiles, pass 2a: the cell's own populations and the wall gradient.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const StepParams& p,
+                                                 const AdjointParams& w, int x, int y, int z, float bx, float by,
+                                                 float bz, const float* __restrict__ nu_cell, double (&acc)[8]) {
+    const Grid& g = p.g;
+    const size_t n = g.n;
+    const uint32_t gid = flat(g, x, y, z);
+    const bool is_solid = b.solid_cur[gid] != 0;
+    const bool was_solid = b.solid_prev[gid] != 0;
+    const bool topology = p.accumulate_loads && p.load_component != 1 && p.force_mode == 6;
+    const float wx = p.origin[0] + ((float)x + 0.5f) * p.dx;
+    const float wy = p.origin[1] + ((float)y + 0.5f) * p.dx;
+    const float wz = p.origin[2] + ((float)z + 0.5f) * p.dx;
+    const float arm[3] = {wx - bx, wy - by, wz - bz};
+    const float fs = p.force_scale;
+    if (is_solid) {
+        // f_s(x) = feq(1, u_wall(x))
+        const float4 wv = b.wall[gid];
+        float l[19];
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            l[q] = b.lam_in[(size_t)q * n + gid];
+        });
+        float gx, gy, gz;
+        adj::feq_velocity_adjoint<float>(l, wv.x, wv.y, wv.z, gx, gy, gz);
+        contract(acc, w.K, b.dW, n, gid, gx, gy, gz, 0.0f);
+        if (topology && !was_solid) {
+            // fresh cover: F += sum_q f_q c_q (the preserved momentum)
+            for_q(AllQ{}, [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (q != 0) b.lam_out[(size_t)q * n + gid] += fs * load_weight<q>(w, arm);
+            });
+        }
+        return;
+    }
+    if (was_solid) {
+        const float4 rv = b.wall[gid];
+        float l[19];
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            l[q] = nu_cell[q * 256];
+        });
+        float gx, gy, gz;
+        adj::feq_velocity_adjoint<float>(l, rv.x, rv.y, rv.z, gx, gy, gz);
+        if (topology) {
+            // fresh uncover: F -= P_refill (= u_wall) + sum over persistent
+            // fluid neighbours of (f_q(x) + f_opp(x + c_q)) c_q
+            float e[3];
+            cross3f(w.cT, arm, e);
+            gx -= fs * (w.cF[0] + e[0]);
+            gy -= fs * (w.cF[1] + e[1]);
+            gz -= fs * (w.cF[2] + e[2]);
+            for_q(AllQ{}, [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (q != 0) {
+                    const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+                    if (inside(g, fx, fy, fz)) {
+                        const uint32_t nb = flat(g, fx, fy, fz);
+                        if (b.solid_prev[nb] == 0 && b.solid_cur[nb] == 0)
+                            b.lam_out[(size_t)q * n + gid] += -fs * load_weight<q>(w, arm);
+                    }
+                }
+            });
+        }
+        contract(acc, w.K, b.dW, n, gid, gx, gy, gz, 0.0f);
+        return;
+    }
+    const Sources s = sources(p, x, y, z);
+    const float4 wg4 = b.wall[gid];
+    const float wgv[4] = {wg4.x, wg4.y, wg4.z, wg4.w};
+    float gxw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        bool far;
+        const uint32_t src = source_of<q>(s, g, x, y, z, far);
+        if (far) return;
+        const uint8_t part = b.solid_cur[src];
+        if (part == 0) return;
+        const float4 ws4 = b.wall[src];
+        const float wsv[4] = {ws4.x, ws4.y, ws4.z, ws4.w};
+        const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+        const bool fin_ok = inside(g, fx, fy, fz);
+        const uint32_t fid = fin_ok ? flat(g, fx, fy, fz) : gid;
+        const bool farther_ok = fin_ok && b.solid_cur[fid] == 0;
+        const float refl = b.f_prev[(size_t)opp(q) * n + gid];
+        const float farther = farther_ok ? b.f_prev[(size_t)opp(q) * n + fid] : 0.0f;
+        const float pin = b.f_prev[(size_t)q * n + gid];
+        const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
+                                                           p.wall_rule, farther_ok, refl, farther, pin, wgv, wsv);
+        const float nu = nu_cell[q * 256];
+        float refl_load = 0.0f;
+        const int sel = p.selected_part;
+        if (p.accumulate_loads && p.load_component != 2 && (sel == 0 || sel == (int)part)) {
+            const float larm[3] = {wx - d.lf * (float)cx(q) * p.dx - bx, wy - d.lf * (float)cy(q) * p.dx - by,
+                                   wz - d.lf * (float)cz(q) * p.dx - bz};
+            refl_load = -fs * load_weight<q>(w, larm);
+        }
+        b.lam_out[(size_t)opp(q) * n + gid] += nu * d.d_refl + refl_load;
+        if (d.uses_pin) b.lam_out[(size_t)q * n + gid] += nu * d.d_pin;
+        for (int i = 0; i < 4; ++i) gxw[i] += nu * d.d_wg[i];
+        contract(acc, w.K, b.dW, n, src, nu * d.d_ws[0], nu * d.d_ws[1], nu * d.d_ws[2], nu * d.d_ws[3]);
+    });
+    contract(acc, w.K, b.dW, n, gid, gxw[0], gxw[1], gxw[2], gxw[3]);
+}
+
+__global__ __launch_bounds__(256) void adjoint_general_p2a(AdjointBuffers b, StepParams p, AdjointParams w) {
+    const uint32_t count = *b.general_count;
+    const float bx = (float)b.body[0], by = (float)b.body[1], bz = (float)b.body[2];
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        int x0, y0, z;
+        tile_origin(p.g, b.general_list[i], x0, y0, z);
+        const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+        const int y = y0 + (int)(threadIdx.x / TILE_X);
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (x < p.g.nx && y < p.g.ny)
+            general_p2a_cell(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x, acc);
+        block_sum8(acc, b.grad_partials + (size_t)i * 8);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wall tiles, pass 2b: populations one link beyond the cell.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, StepParams p, AdjointParams w) {
+    const uint32_t count = *b.general_count;
+    const float bx = (float)b.body[0], by = (float)b.body[1], bz = (float)b.body[2];
+    const Grid& g = p.g;
+    const size_t n = g.n;
+    for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        int x0, y0, z;
+        tile_origin(g, b.general_list[i], x0, y0, z);
+        const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+        const int y = y0 + (int)(threadIdx.x / TILE_X);
+        if (x >= g.nx || y >= g.ny) continue;
+        const uint32_t gid = flat(g, x, y, z);
+        const float* nu_cell = b.nu + (size_t)i * 19 * 256 + threadIdx.x;
+        if (b.solid_cur[gid] != 0) continue;
+        if (b.solid_prev[gid] != 0) {
+            if (!(p.accumulate_loads && p.load_component != 1 && p.force_mode == 6)) continue;
+            const float arm[3] = {p.origin[0] + ((float)x + 0.5f) * p.dx - bx,
+                                  p.origin[1] + ((float)y + 0.5f) * p.dx - by,
+                                  p.origin[2] + ((float)z + 0.5f) * p.dx - bz};
+            for_q(AllQ{}, [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (q != 0) {
+                    const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+                    if (inside(g, fx, fy, fz)) {
+                        const uint32_t nb = flat(g, fx, fy, fz);
+                        if (b.solid_prev[nb] == 0 && b.solid_cur[nb] == 0)
+                            b.lam_out[(size_t)opp(q) * n + nb] += -p.force_scale * load_weight<q>(w, arm);
+                    }
+                }
+            });
+            continue;
+        }
+        if (p.wall_rule == WALL_HALFWAY) continue;
+        const Sources s = sources(p, x, y, z);
+        const float4 wg4 = b.wall[gid];
+        const float wgv[4] = {wg4.x, wg4.y, wg4.z, wg4.w};
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            bool far;
+            const uint32_t src = source_of<q>(s, g, x, y, z, far);
+            if (far || b.solid_cur[src] == 0) return;
+            const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+            if (!inside(g, fx, fy, fz)) return;
+            const uint32_t fid = flat(g, fx, fy, fz);
+            if (b.solid_cur[fid] != 0) return;
+            const float4 ws4 = b.wall[src];
+            const float wsv[4] = {ws4.x, ws4.y, ws4.z, ws4.w};
+            const adj::LinkD<float> d = adj::link_value<float>(
+                cx(q), cy(q), cz(q), weight<float>(q), p.far_rho, p.wall_rule, true, b.f_prev[(size_t)opp(q) * n + gid],
+                b.f_prev[(size_t)opp(q) * n + fid], b.f_prev[(size_t)q * n + gid], wgv, wsv);
+            if (d.uses_farther) b.lam_out[(size_t)opp(q) * n + fid] += nu_cell[q * 256] * d.d_farther;
+        });
+    }
+}
+
+// Fixed-order sum of the listed tiles' gradient partials into the running
+// FP64 gradient.
+__global__ __launch_bounds__(256) void reduce_gradient(const double* __restrict__ partials,
+                                                       const uint32_t* __restrict__ count,
+                                                       double* __restrict__ total) {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t c = *count;
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] += partials[(size_t)i * 8 + k];
+    }
+    __shared__ double out[8];
+    block_sum8(v, out);
+    if (threadIdx.x < 8) total[threadIdx.x] += out[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// Wall-field tangents.  Mesh frames carry per-parameter vertex tangents;
+// vertex positions and velocities at the query time are the frame
+// interpolation (prepare), so their tangents interpolate the same way.
+// ---------------------------------------------------------------------------
+__global__ void tangent_vertices(const float* __restrict__ dpoints, const float* __restrict__ times,
+                                 float4* __restrict__ dpos, float4* __restrict__ dvel, MeshParams m, int K) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m.vertex_count) return;
+    const uint32_t first = min(m.first, m.frame_count - 2u);
+    const uint32_t second = first + 1u;
+    const float duration = times[second] - times[first];
+    const float blend = fminf(fmaxf((m.query_time - times[first]) / duration, 0.0f), 1.0f);
+    for (int k = 0; k < K; ++k) {
+        const float* a = dpoints + (((size_t)k * m.frame_count + first) * m.vertex_count + v) * 3;
+        const float* b = dpoints + (((size_t)k * m.frame_count + second) * m.vertex_count + v) * 3;
+        const float d0 = b[0] - a[0], d1 = b[1] - a[1], d2 = b[2] - a[2];
+        dpos[(size_t)k * m.vertex_count + v] = make_float4(a[0] + blend * d0, a[1] + blend * d1, a[2] + blend * d2, 0.0f);
+        dvel[(size_t)k * m.vertex_count + v] =
+            make_float4(d0 / duration * m.velocity_scale, d1 / duration * m.velocity_scale,
+                        d2 / duration * m.velocity_scale, 0.0f);
+    }
+}
+
+// Dual number (value, derivative along one parameter).
+struct D {
+    float v, d;
+};
+__device__ __forceinline__ D operator+(D a, D b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ D operator-(D a, D b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ D operator*(D a, D b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ D operator/(D a, D b) {
+    const float q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ D dconst(float v) { return {v, 0.0f}; }
+__device__ __forceinline__ D ddot(const D a[3], const D b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// Closest point on triangle (a, b, c) to the fixed point P with the Voronoi
+// region chosen on the values (the region is held fixed, like the link set),
+// derivatives along one parameter.  Same regions and formulas as
+// closest_point in bf_geometry.hip.
+__device__ void closest_point_dual(const float P[3], const D a[3], const D b[3], const D c[3], D cp[3], D bc[3]) {
+    D ab[3], ac[3], ap[3], bp[3], cpv[3];
+    for (int i = 0; i < 3; ++i) {
+        ab[i] = b[i] - a[i];
+        ac[i] = c[i] - a[i];
+        ap[i] = dconst(P[i]) - a[i];
+        bp[i] = dconst(P[i]) - b[i];
+        cpv[i] = dconst(P[i]) - c[i];
+    }
+    const D d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+    auto set = [&](const D p[3], D b0, D b1, D b2) {
+        for (int i = 0; i < 3; ++i) cp[i] = p[i];
+        bc[0] = b0;
+        bc[1] = b1;
+        bc[2] = b2;
+    };
+    if (d1.v <= 0.0f && d2.v <= 0.0f) return set(a, dconst(1), dconst(0), dconst(0));
+    const D d3 = ddot(ab, bp), d4 = ddot(ac, bp);
+    if (d3.v >= 0.0f && d4.v <= d3.v) return set(b, dconst(0), dconst(1), dconst(0));
+    const D vc = d1 * d4 - d3 * d2;
+    if (vc.v <= 0.0f && d1.v >= 0.0f && d3.v <= 0.0f) {
+        const D v = d1 / (d1 - d3);
+        D p[3];
+        for (int i = 0; i < 3; ++i) p[i] = a[i] + v * ab[i];
+        return set(p, dconst(1) - v, v, dconst(0));
+    }
+    const D d5 = ddot(ab, cpv), d6 = ddot(ac, cpv);
+    if (d6.v >= 0.0f && d5.v <= d6.v) return set(c, dconst(0), dconst(0), dconst(1));
+    const D vb = d5 * d2 - d1 * d6;
+    if (vb.v <= 0.0f && d2.v >= 0.0f && d6.v <= 0.0f) {
+        const D w = d2 / (d2 - d6);
+        D p[3];
+        for (int i = 0; i < 3; ++i) p[i] = a[i] + w * ac[i];
+        return set(p, dconst(1) - w, dconst(0), w);
+    }
+    const D va = d3 * d6 - d5 * d4;
+    if (va.v <= 0.0f && (d4.v - d3.v) >= 0.0f && (d5.v - d6.v) >= 0.0f) {
+        const D w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        D p[3];
+        for (int i = 0; i < 3; ++i) p[i] = b[i] + w * (c[i] - b[i]);
+        return set(p, dconst(0), dconst(1) - w, w);
+    }
+    D den = va + vb + vc;
+    if (den.v < 1.0e-20f) den = dconst(1.0e-20f);
+    const D v = vb / den, w = vc / den;
+    D p[3];
+    for (int i = 0; i < 3; ++i) p[i] = a[i] + ab[i] * v + ac[i] * w;
+    return set(p, dconst(1) - v - w, v, w);
+}
+
+// d wall / d theta_k of every keyed cell: wall = (sum_i b_i v_i, (|P - cp| - h) / dx)
+__global__ void wall_tangent(const uint32_t* __restrict__ tri_of_cell, const uint32_t* __restrict__ tri,
+                             const float4* __restrict__ pos, const float4* __restrict__ vel,
+                             const float4* __restrict__ dpos, const float4* __restrict__ dvel, float4* __restrict__ dW,
+                             MeshParams m, int K) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= m.g.n) return;
+    const uint32_t t = tri_of_cell[gid];
+    if (t == NO_TRIANGLE || t >= m.triangle_count) return;
+    const int x = (int)(gid % (uint32_t)m.g.nx);
+    const uint32_t yz = gid / (uint32_t)m.g.nx;
+    const int y = (int)(yz % (uint32_t)m.g.ny), z = (int)(yz / (uint32_t)m.g.ny);
+    const float P[3] = {m.origin[0] + ((float)x + 0.5f) * m.dx, m.origin[1] + ((float)y + 0.5f) * m.dx,
+                        m.origin[2] + ((float)z + 0.5f) * m.dx};
+    const uint32_t iv[3] = {tri[3 * (size_t)t], tri[3 * (size_t)t + 1], tri[3 * (size_t)t + 2]};
+    const size_t nv = m.vertex_count;
+    for (int k = 0; k < K; ++k) {
+        D vtx[3][3], vv[3][3];
+        for (int j = 0; j < 3; ++j) {
+            const float4 p4 = pos[iv[j]], dp4 = dpos[(size_t)k * nv + iv[j]];
+            const float4 v4 = vel[iv[j]], dv4 = dvel[(size_t)k * nv + iv[j]];
+            vtx[j][0] = {p4.x, dp4.x};
+            vtx[j][1] = {p4.y, dp4.y};
+            vtx[j][2] = {p4.z, dp4.z};
+            vv[j][0] = {v4.x, dv4.x};
+            vv[j][1] = {v4.y, dv4.y};
+            vv[j][2] = {v4.z, dv4.z};
+        }
+        D cp[3], bc[3];
+        closest_point_dual(P, vtx[0], vtx[1], vtx[2], cp, bc);
+        D u[3];
+        for (int i = 0; i < 3; ++i) u[i] = bc[0] * vv[0][i] + bc[1] * vv[1][i] + bc[2] * vv[2][i];
+        const float r[3] = {P[0] - cp[0].v, P[1] - cp[1].v, P[2] - cp[2].v};
+        const float len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const float dlen = len > 0.0f ? -(r[0] * cp[0].d + r[1] * cp[1].d + r[2] * cp[2].d) / len : 0.0f;
+        dW[(size_t)k * m.g.n + gid] = make_float4(u[0].d, u[1].d, u[2].d, dlen / m.dx);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+template <int COLL>
+static hipError_t adjoint_step_t(const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
+                                 int general_blocks, hipStream_t s) {
+    hipLaunchKernelGGL(adjoint_fast<COLL>, dim3(p.g.tiles), dim3(256), 0, s, b, p);
+    hipLaunchKernelGGL(adjoint_general_p1<COLL>, dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    hipLaunchKernelGGL(adjoint_general_p2a, dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    hipLaunchKernelGGL(adjoint_general_p2b, dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    hipLaunchKernelGGL(reduce_gradient, dim3(1), dim3(256), 0, s, b.grad_partials, b.general_count, b.grad_total);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_step(int coll, const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
+                               int general_blocks, hipStream_t s) {
+    switch (coll) {
+        case COLL_TRT: return adjoint_step_t<COLL_TRT>(b, p, w, general_blocks, s);
+        case COLL_REGULARIZED: return adjoint_step_t<COLL_REGULARIZED>(b, p, w, general_blocks, s);
+        case COLL_RR3: return adjoint_step_t<COLL_RR3>(b, p, w, general_blocks, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_wall_tangent(const uint32_t* tri_of_cell, const uint32_t* tri, const float4* pos,
+                               const float4* vel, const float* dpoints, const float* times, float4* dpos,
+                               float4* dvel, float4* dW, const MeshParams& m, int K, hipStream_t s) {
+    hipLaunchKernelGGL(tangent_vertices, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, dpoints, times, dpos,
+                       dvel, m, K);
+    hipLaunchKernelGGL(wall_tangent, dim3((m.g.n + 255) / 256), dim3(256), 0, s, tri_of_cell, tri, pos, vel, dpos,
+                       dvel, dW, m, K);
+    return hipGetLastError();
+}
+
+}  // namespace bf

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -118,6 +119,18 @@ struct bf_context {
     float4 sphere_cr{};
     float3 sphere_gv{};
     float4 sphere_wv{};
+    // rigid body: FP64 device state (position[3], orientation[4] x y z w,
+    // linear velocity[3], body angular velocity[3]), per-step history,
+    // safety record
+    double* body = nullptr;
+    double* body_history = nullptr;
+    double* safety_metrics = nullptr;     // max Mach, min clearance, Mach / clearance at the first event
+    uint32_t* safety_event = nullptr;     // step low, step high, flags
+    BodyParams body_params{};
+    bool free_flight = false;
+    bool move_mesh = false;
+    uint32_t safety_reported = 0;
+    uint64_t body_history_count = 0;
     // stepping
     int64_t step = 0;
     uint64_t history_count = 0;
@@ -182,7 +195,6 @@ StepParams make_step_params(const bf_context* c) {
     for (int i = 0; i < 3; ++i) {
         p.far_u[i] = c->cfg.far_velocity[i];
         p.origin[i] = c->cfg.origin[i];
-        p.body[i] = c->cfg.body_position[i];
     }
     p.far_rho = c->cfg.far_density;
     p.dx = c->cfg.cell_size;
@@ -250,6 +262,7 @@ MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     // (distance <= h), fluid cells one link away from them (<= h + sqrt(2)),
     // and cells uncovered this step (<= h + one step's wall travel).
     m.cull_cells = exact ? INFINITY : c->m_half / c->cfg.cell_size + 1.5f;
+    m.move_with_body = c->move_mesh ? 1 : 0;
     // frame interval, MeasuredBirdSurfaceSequence.swift:174-204
     const std::vector<float>& ts = c->host_times;
     const int last = (int)c->m_frames - 1;
@@ -280,6 +293,7 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
     hipError_t e = hipSuccess;
     if (c->source == BF_GEOMETRY_MESH) {
         MeshBuffers b{};
+        b.body = c->body;
         b.points = c->m_points;
         b.times = c->m_times;
         b.tri = c->m_tri;
@@ -337,6 +351,7 @@ FluidBuffers fluid_buffers(const bf_context* c, int64_t s) {
     b.density = c->density;
     b.velocity = c->velocity;
     b.activation = &c->counters[4 + 2 * par2(s)];
+    b.body = c->body;
     return b;
 }
 
@@ -473,7 +488,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     const uint64_t n = g.n;
     const uint64_t pop_bytes = 19ull * n * c->pop_elem;
     const uint64_t plan = 2 * pop_bytes + 3 * n + 32 * n + 3ull * g.tiles + 4ull * g.tiles + 64ull * g.tiles +
-                          64ull * HISTORY_CAPACITY + 4 * n + 16 * n;
+                          64ull * HISTORY_CAPACITY + 128ull * HISTORY_CAPACITY + 4 * n + 16 * n;
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     if (plan > (uint64_t)free_b) {
@@ -513,6 +528,10 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     c->census = (unsigned long long*)A.take(e, 8 * 24);
     c->density = (float*)A.take(e, 4 * n);
     c->velocity = (float4*)A.take(e, 16 * n);
+    c->body = (double*)A.take(e, 16 * sizeof(double));
+    c->body_history = (double*)A.take(e, 16 * sizeof(double) * HISTORY_CAPACITY);
+    c->safety_metrics = (double*)A.take(e, 4 * sizeof(double));
+    c->safety_event = (uint32_t*)A.take(e, 4 * sizeof(uint32_t));
     if (e != hipSuccess) {
         bf_destroy(c);
         return BF_ERROR_ALLOCATION_FAILED;
@@ -528,6 +547,14 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     if (e == hipSuccess) e = launch_fill_u32(c->counters, 0u, 16u, s);
     if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->last_load, 0, 64, s);
     if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->partials, 0, (uint32_t)(64ull * g.tiles), s);
+    // body at rest at the torque reference point, identity orientation
+    const double body0[16] = {cfg->body_position[0], cfg->body_position[1], cfg->body_position[2], 0.0, 0.0, 0.0,
+                              1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double safety0[4] = {0.0, DBL_MAX, 0.0, 0.0};
+    const uint32_t event0[4] = {0u, 0u, 0u, 0u};
+    if (e == hipSuccess) e = hipMemcpyAsync(c->body, body0, sizeof body0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->safety_metrics, safety0, sizeof safety0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->safety_event, event0, sizeof event0, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         bf_destroy(c);
@@ -829,27 +856,45 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         if (prof && e == hipSuccess) e = hipEventRecord(pe[1], c->sa);
         if (e == hipSuccess) e = hipEventRecord(c->ev_fast[par2(s)], c->sa);
         if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
-        // stream B: geometry and tile list of step s+1 while fast(s) runs ...
-        if (prof) e = hipEventRecord(pe[2], c->sb);
-        if (e == hipSuccess && c->source != BF_GEOMETRY_STATIC) {
-            if (bf_status r = enqueue_geometry(c, s + 1, false, false)) return r;
+        // stream B: geometry and tile list of step s+1 (while fast(s) runs,
+        // unless the mesh rides on a free-flying body: then they need the
+        // pose integrated from the loads of step s and follow it)
+        const bool coupled = c->free_flight && c->move_mesh && c->source == BF_GEOMETRY_MESH;
+        auto next_geometry = [&]() -> bf_status {
+            if (prof) e = hipEventRecord(pe[2], c->sb);
+            if (e == hipSuccess && c->source != BF_GEOMETRY_STATIC) {
+                if (bf_status r = enqueue_geometry(c, s + 1, false, false)) return r;
+            }
+            if (bf_status r = enqueue_tile_list(c, s + 1)) return r;
+            if (prof) e = hipEventRecord(pe[3], c->sb);
+            return e == hipSuccess ? BF_OK : hip_fail(c, e, "geometry launch");
+        };
+        if (!coupled) {
+            if (bf_status r = next_geometry()) return r;
         }
-        if (bf_status r = enqueue_tile_list(c, s + 1)) return r;
-        if (prof) e = hipEventRecord(pe[3], c->sb);
         // ... then the wall tiles of step s once fast(s) is done
         if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, c->ev_fast[par2(s)], 0);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[4], c->sb);
         if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, p, c->general_blocks, c->sb);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[5], c->sb);
-        // fast(s+1) needs only the populations; the load reduction of step s
-        // overlaps it (it reads partials and flags that step s+1 rewrites
-        // only after it, later on this stream)
-        if (e == hipSuccess) e = hipEventRecord(c->ev_ready[par2(s)], c->sb);
+        // fast(s+1) needs only the populations; the load reduction and body
+        // update of step s overlap it (they read partials and flags that
+        // step s+1 rewrites only after them, later on this stream)
+        if (e == hipSuccess && !coupled) e = hipEventRecord(c->ev_ready[par2(s)], c->sb);
         if (e == hipSuccess) e = launch_load_reduction(fb, p, (uint32_t)(s % HISTORY_CAPACITY), c->sb);
+        if (e == hipSuccess && c->free_flight)
+            e = launch_body_step(c->body, c->last_load, c->body_history, (uint32_t)(s % HISTORY_CAPACITY),
+                                 c->safety_metrics, c->safety_event, (uint64_t)s, c->body_params, c->sb);
         if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
+        if (coupled) {
+            if (bf_status r = next_geometry()) return r;
+            e = hipEventRecord(c->ev_ready[par2(s)], c->sb);
+            if (e != hipSuccess) return hip_fail(c, e, "event record");
+        }
         c->step = s;
         c->cur = 1 - c->cur;
         ++c->history_count;
+        if (c->free_flight) ++c->body_history_count;
         c->geometry_ready = s + 1;
         if (prof) ++c->profiled;
     }
@@ -864,6 +909,129 @@ bf_status bf_synchronize(bf_context* c) {
     (void)hipSetDevice(c->device);
     hipError_t e = drain(c);
     if (e != hipSuccess) return hip_fail(c, e, "synchronize");
+    if (c->free_flight && c->body_params.monitor) {
+        // a new runtime-safety event is reported once (the reference throws
+        // runtimeSafetyViolation after the batch, BirdFlowSimulation.swift:735-741)
+        uint32_t ev[4];
+        e = hipMemcpy(ev, c->safety_event, sizeof ev, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(c, e, "read safety record");
+        if (ev[2] != 0u && c->safety_reported == 0u) {
+            c->safety_reported = ev[2];
+            char msg[160];
+            std::snprintf(msg, sizeof msg, "runtime safety violation at step %llu (flags %u: 1 Mach, 2 clearance, "
+                          "4 non-finite)", (unsigned long long)ev[0] | ((unsigned long long)ev[1] << 32), ev[2]);
+            c->error = msg;
+            return BF_ERROR_RUNTIME_SAFETY_VIOLATION;
+        }
+    }
+    return BF_OK;
+}
+
+bf_status bf_body_set(bf_context* c, const bf_body_state* state, const bf_body_params* params) {
+    BF_ENTER(c);
+    if (params) {
+        if (!(params->mass > 0.0) || !(params->inertia[0] > 0.0) || !(params->inertia[1] > 0.0) ||
+            !(params->inertia[2] > 0.0) || params->substeps < 1 || params->free_flight > 1u || params->move_mesh > 1u)
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "body parameters: mass and inertia > 0, substeps >= 1");
+        if (params->free_flight && !c->cfg.accumulate_loads)
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "free flight integrates the fluid loads: accumulate_loads must be on");
+        BodyParams& b = c->body_params;
+        b.mass = params->mass;
+        for (int i = 0; i < 3; ++i) {
+            b.inertia[i] = params->inertia[i];
+            b.gravity[i] = params->gravity[i];
+            b.external_force[i] = params->external_force[i];
+            b.external_torque[i] = params->external_torque[i];
+            b.extent[i] = params->safety_extent[i];
+            const double dx = c->cfg.cell_size;
+            const double margin = ((double)c->cfg.sponge_width + 3.0) * dx;   // BirdFlow.metal:6650-6657
+            const double n = i == 0 ? c->cfg.nx : (i == 1 ? c->cfg.ny : c->cfg.nz);
+            b.domain_min[i] = (double)c->cfg.origin[i] + margin;
+            b.domain_max[i] = (double)c->cfg.origin[i] + n * dx - margin;
+            b.far_velocity_world[i] = (double)c->cfg.far_velocity[i] / (double)c->cfg.velocity_to_lattice;
+        }
+        b.dt = c->cfg.time_step;
+        b.substeps = (int)params->substeps;
+        b.monitor = params->mach_limit > 0.0 ? 1 : 0;
+        b.rotation_radius = params->safety_radius;
+        b.wing_speed = params->wing_speed;
+        b.mach_limit = params->mach_limit;
+        b.velocity_to_lattice = c->cfg.velocity_to_lattice;
+        c->free_flight = params->free_flight != 0u;
+        c->move_mesh = params->move_mesh != 0u;
+    }
+    if (state) {
+        double v[16] = {0};
+        for (int i = 0; i < 3; ++i) {
+            v[i] = state->position[i];
+            v[7 + i] = state->linear_velocity[i];
+            v[10 + i] = state->angular_velocity_body[i];
+        }
+        for (int i = 0; i < 4; ++i) v[3 + i] = state->orientation[i];
+        hipError_t e = hipMemcpy(c->body, v, sizeof v, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(c, e, "body upload");
+        // a mesh carried by the body is placed again for the next step
+        if (c->move_mesh && c->source == BF_GEOMETRY_MESH) c->geometry_ready = c->step;
+    }
+    if (state || params) {
+        // a new body (or new body parameters) starts a new safety record
+        const double safety0[4] = {0.0, DBL_MAX, 0.0, 0.0};
+        const uint32_t event0[4] = {0u, 0u, 0u, 0u};
+        hipError_t e = hipMemcpy(c->safety_metrics, safety0, sizeof safety0, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->safety_event, event0, sizeof event0, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(c, e, "safety reset");
+        c->safety_reported = 0u;
+        c->body_history_count = 0;
+    }
+    return BF_OK;
+}
+
+bf_status bf_body_get(bf_context* c, bf_body_state* state) {
+    BF_ENTER(c);
+    if (!state) return BF_ERROR_INVALID_ARGUMENT;
+    double v[16];
+    hipError_t e = hipMemcpy(v, c->body, sizeof v, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "body read");
+    for (int i = 0; i < 3; ++i) {
+        state->position[i] = v[i];
+        state->linear_velocity[i] = v[7 + i];
+        state->angular_velocity_body[i] = v[10 + i];
+    }
+    for (int i = 0; i < 4; ++i) state->orientation[i] = v[3 + i];
+    return BF_OK;
+}
+
+bf_status bf_read_body_history(bf_context* c, bf_body_state* out, uint32_t max_records, uint32_t* records) {
+    BF_ENTER(c);
+    if (!out || !records) return BF_ERROR_INVALID_ARGUMENT;
+    const uint64_t avail = std::min<uint64_t>(c->body_history_count, HISTORY_CAPACITY);
+    const uint64_t want = std::min<uint64_t>(avail, max_records);
+    std::vector<double> all(16ull * HISTORY_CAPACITY);
+    hipError_t e = hipMemcpy(all.data(), c->body_history, 128ull * HISTORY_CAPACITY, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read body history");
+    for (uint64_t i = 0; i < want; ++i) {
+        const uint64_t step = (uint64_t)c->step - want + 1 + i;
+        const double* v = all.data() + 16 * (step % HISTORY_CAPACITY);
+        for (int k = 0; k < 3; ++k) {
+            out[i].position[k] = v[k];
+            out[i].linear_velocity[k] = v[7 + k];
+            out[i].angular_velocity_body[k] = v[10 + k];
+        }
+        for (int k = 0; k < 4; ++k) out[i].orientation[k] = v[3 + k];
+    }
+    *records = (uint32_t)want;
+    return BF_OK;
+}
+
+bf_status bf_read_safety(bf_context* c, double metrics[4], uint64_t* event_step, uint32_t* flags) {
+    BF_ENTER(c);
+    if (!metrics || !event_step || !flags) return BF_ERROR_INVALID_ARGUMENT;
+    uint32_t ev[4];
+    hipError_t e = hipMemcpy(metrics, c->safety_metrics, 4 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ev, c->safety_event, sizeof ev, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read safety record");
+    *event_step = (uint64_t)ev[0] | ((uint64_t)ev[1] << 32);
+    *flags = ev[2];
     return BF_OK;
 }
 

@@ -84,7 +84,6 @@ struct StepParams {
     float far_u[3], far_rho;
     float origin[3], dx;
     float force_scale;
-    float body[3];
     float far_eq[19];          // equilibrium(q, far_rho, far_u), FP32
 };
 

@@ -492,7 +492,11 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
                                                   float* __restrict__ density, float4* __restrict__ velocity,
                                                   uint32_t* __restrict__ activation, StepParams p) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    if ((static_flags[tile] | dyn_prev[tile] | dyn_cur[tile]) != 0) return;   // fluid_general owns it
+    // Tiles with walls or sponge belong to fluid_general.  The flags are
+    // loaded together with the populations and tested only before the
+    // stores: a test up front would put a dependent memory round trip ahead
+    // of every workgroup's population loads.
+    const bool general_tile = (static_flags[tile] | dyn_prev[tile] | dyn_cur[tile]) != 0;
     int x0, y0, z;
     tile_origin(p.g, tile, x0, y0, z);
     const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
@@ -509,7 +513,9 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
     const int zs_p = shift_axis(z, 1, nz, p.bc[2][0], p.bc[2][1], zm_far);
     const int zs_m = shift_axis(z, -1, nz, p.bc[2][0], p.bc[2][1], zp_far);
 
+    const uint32_t gid_self = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
     float f[19];
+    bool far_q[19];
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const int xs = cx(q) > 0 ? xs_p : (cx(q) < 0 ? xs_m : x);
@@ -518,7 +524,23 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
         const bool far = (cx(q) > 0 && xm_far) || (cx(q) < 0 && xp_far) || (cy(q) > 0 && ym_far)
             || (cy(q) < 0 && yp_far) || (cz(q) > 0 && zm_far) || (cz(q) < 0 && zp_far);
         const uint32_t src = (uint32_t)xs + (uint32_t)nx * ((uint32_t)ys + (uint32_t)ny * (uint32_t)zs);
-        f[q] = far ? p.far_eq[q] : Store<S>::load(fin, (size_t)q * n + src, q);
+        // unconditional load: a far-field source reads an in-domain cell and
+        // discards it
+        f[q] = Store<S>::load(fin, (size_t)q * n + (far ? gid_self : src), q);
+        far_q[q] = far;
+    });
+    // One empty asm over all 19 loaded values: every load is issued, without
+    // a branch, before any of them is consumed (otherwise the far-field
+    // select turns each load into a conditional block, and the scheduler
+    // starts the moment sums after a few loads and drains the queue), and
+    // none sinks below the tile test.
+    asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]), "+v"(f[3]), "+v"(f[4]), "+v"(f[5]), "+v"(f[6]),
+                 "+v"(f[7]), "+v"(f[8]), "+v"(f[9]), "+v"(f[10]), "+v"(f[11]), "+v"(f[12]), "+v"(f[13]),
+                 "+v"(f[14]), "+v"(f[15]), "+v"(f[16]), "+v"(f[17]), "+v"(f[18]));
+    if (general_tile) return;
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if (far_q[q]) f[q] = p.far_eq[q];
     });
 
     float post[19];
@@ -583,14 +605,14 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                              const uint8_t* __restrict__ solid_prev, const uint8_t* __restrict__ solid_cur,
                              const float4* __restrict__ wall, float* __restrict__ density,
                              float4* __restrict__ velocity, const StepParams& p, int x, int y, int z,
-                             CellLoad& L) {
+                             float bx, float by, float bz, CellLoad& L) {
     const Grid& g = p.g;
     const size_t n = g.n;
     const uint32_t gid = flat(g, x, y, z);
     const float wx = p.origin[0] + ((float)x + 0.5f) * p.dx;
     const float wy = p.origin[1] + ((float)y + 0.5f) * p.dx;
     const float wz = p.origin[2] + ((float)z + 0.5f) * p.dx;
-    const float ax = wx - p.body[0], ay = wy - p.body[1], az = wz - p.body[2];
+    const float ax = wx - bx, ay = wy - by, az = wz - bz;
     const bool was_solid = solid_prev[gid] != 0;
     const bool is_solid = solid_cur[gid] != 0;
     const int mode = p.force_mode;
@@ -769,9 +791,9 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                         flx = flx + lx;
                         fly = fly + ly;
                         flz = flz + lz;
-                        const float bpx = wx - lf * dxq * p.dx - p.body[0];
-                        const float bpy = wy - lf * dyq * p.dx - p.body[1];
-                        const float bpz = wz - lf * dzq * p.dx - p.body[2];
+                        const float bpx = wx - lf * dxq * p.dx - bx;
+                        const float bpy = wy - lf * dyq * p.dx - by;
+                        const float bpz = wz - lf * dzq * p.dx - bz;
                         float ox, oy, oz;
                         cross3(bpx, bpy, bpz, lx * fscale, ly * fscale, lz * fscale, ox, oy, oz);
                         tqx = tqx + ox;
@@ -855,8 +877,10 @@ __global__ __launch_bounds__(256) void fluid_general(const typename Store<S>::T*
                                                      const uint32_t* __restrict__ general_count,
                                                      double* __restrict__ partials,
                                                      float* __restrict__ density, float4* __restrict__ velocity,
-                                                     StepParams p) {
+                                                     const double* __restrict__ body, StepParams p) {
     const uint32_t count = *general_count;
+    // torque reference point: the body position, read once per workgroup
+    const float bx = (float)body[0], by = (float)body[1], bz = (float)body[2];
     for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
         const uint32_t tile = general_list[i];
         int x0, y0, z;
@@ -865,7 +889,8 @@ __global__ __launch_bounds__(256) void fluid_general(const typename Store<S>::T*
         const int y = y0 + (int)(threadIdx.x / TILE_X);
         CellLoad L{0, 0, 0, 0, 0, 0, 0, 0};
         if (x < p.g.nx && y < p.g.ny)
-            general_cell<ARITH, COLL, S>(fin, fout, solid_prev, solid_cur, wall, density, velocity, p, x, y, z, L);
+            general_cell<ARITH, COLL, S>(fin, fout, solid_prev, solid_cur, wall, density, velocity, p, x, y, z, bx, by,
+                                             bz, L);
         if (p.accumulate_loads) {
             double v[8] = {(double)L.fx, (double)L.fy, (double)L.fz, (double)L.tx,
                            (double)L.ty, (double)L.tz, (double)L.act, (double)L.restr};
@@ -891,7 +916,7 @@ static hipError_t launch_general_t(const FluidBuffers& b, const StepParams& p, i
     using T = typename Store<S>::T;
     hipLaunchKernelGGL((fluid_general<ARITH, COLL, S>), dim3(general_blocks), dim3(256), 0, s, (const T*)b.fin,
                        (T*)b.fout, b.solid_prev, b.solid_cur, b.wall, b.general_list, b.general_count,
-                       b.partials, b.density, b.velocity, p);
+                       b.partials, b.density, b.velocity, b.body, p);
     return hipGetLastError();
 }
 

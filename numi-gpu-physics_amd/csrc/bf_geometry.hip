@@ -66,8 +66,21 @@ __device__ inline Closest closest_point(float px, float py, float pz, float3 a, 
 
 
 // BirdFlow.metal:1589-1622.  points: frame-major float3.
+// v' = v + w t + q x t with t = 2 q x v (q = x, y, z, w), FP32
+__device__ __forceinline__ void qrotate_f(const float q[4], const float v[3], float o[3]) {
+    const float t[3] = {2.0f * (q[1] * v[2] - q[2] * v[1]), 2.0f * (q[2] * v[0] - q[0] * v[2]),
+                        2.0f * (q[0] * v[1] - q[1] * v[0])};
+    const float c[3] = {q[1] * t[2] - q[2] * t[1], q[2] * t[0] - q[0] * t[2], q[0] * t[1] - q[1] * t[0]};
+    for (int i = 0; i < 3; ++i) o[i] = v[i] + q[3] * t[i] + c[i];
+}
+
+// Vertex positions and lattice velocities at the query time
+// (BirdFlow.metal:1589-1622).  With move_with_body the frames are in the
+// body frame and the device body pose places them:
+// p = x_b + R p_local, v = s (v_b + R (w_b x p_local) + R v_local).
 __global__ void mesh_prepare(const float* __restrict__ points, const float* __restrict__ times,
-                             float4* __restrict__ pos, float4* __restrict__ vel, MeshParams m) {
+                             float4* __restrict__ pos, float4* __restrict__ vel, const double* __restrict__ body,
+                             MeshParams m) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= m.vertex_count) return;
     const uint32_t first = min(m.first, m.frame_count - 2u);
@@ -77,10 +90,39 @@ __global__ void mesh_prepare(const float* __restrict__ points, const float* __re
     const float* a = points + ((size_t)first * m.vertex_count + v) * 3;
     const float* b = points + ((size_t)second * m.vertex_count + v) * 3;
     const float dxv = b[0] - a[0], dyv = b[1] - a[1], dzv = b[2] - a[2];
-    pos[v] = make_float4(m.translation[0] + (a[0] + blend * dxv), m.translation[1] + (a[1] + blend * dyv),
-                         m.translation[2] + (a[2] + blend * dzv), 0.0f);
-    vel[v] = make_float4(dxv / duration * m.velocity_scale, dyv / duration * m.velocity_scale,
-                         dzv / duration * m.velocity_scale, 0.0f);
+    const float lp[3] = {m.translation[0] + (a[0] + blend * dxv), m.translation[1] + (a[1] + blend * dyv),
+                         m.translation[2] + (a[2] + blend * dzv)};
+    if (!m.move_with_body) {
+        pos[v] = make_float4(lp[0], lp[1], lp[2], 0.0f);
+        vel[v] = make_float4(dxv / duration * m.velocity_scale, dyv / duration * m.velocity_scale,
+                             dzv / duration * m.velocity_scale, 0.0f);
+        return;
+    }
+    const float lv[3] = {dxv / duration, dyv / duration, dzv / duration};
+    // normalised pose in FP64, then rounded once to FP32
+    double qd[4] = {body[3], body[4], body[5], body[6]};
+    const double n2 = qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2] + qd[3] * qd[3];
+    if (n2 > 1.0e-20) {
+        const double sc = 1.0 / sqrt(n2);
+        for (int i = 0; i < 4; ++i) qd[i] *= sc;
+    } else {
+        qd[0] = qd[1] = qd[2] = 0.0;
+        qd[3] = 1.0;
+    }
+    const float q[4] = {(float)qd[0], (float)qd[1], (float)qd[2], (float)qd[3]};
+    const float w[3] = {(float)body[10], (float)body[11], (float)body[12]};
+    const float wxp[3] = {w[1] * lp[2] - w[2] * lp[1], w[2] * lp[0] - w[0] * lp[2], w[0] * lp[1] - w[1] * lp[0]};
+    float r[3], rw[3], ru[3];
+    qrotate_f(q, lp, r);
+    qrotate_f(q, wxp, rw);
+    qrotate_f(q, lv, ru);
+    float po[3], ve[3];
+    for (int i = 0; i < 3; ++i) {
+        po[i] = (float)body[i] + r[i];
+        ve[i] = ((float)body[7 + i] + (rw[i] + ru[i])) * m.velocity_scale;
+    }
+    pos[v] = make_float4(po[0], po[1], po[2], 0.0f);
+    vel[v] = make_float4(ve[0], ve[1], ve[2], 0.0f);
 }
 
 __global__ void fill_u8(uint8_t* __restrict__ p, uint8_t v, uint32_t n) {
@@ -269,7 +311,7 @@ __global__ __launch_bounds__(256) void geometry_tile_pass(
     const uint8_t* __restrict__ tri_part, unsigned long long* __restrict__ keys, const uint32_t* __restrict__ list,
     const uint32_t* __restrict__ count, const uint8_t* __restrict__ tiles_new,
     const uint8_t* __restrict__ tiles_mask_old, const uint8_t* __restrict__ tiles_wall_old, uint8_t* __restrict__ mask,
-    float4* __restrict__ wall, uint8_t* __restrict__ dyn, MeshParams m) {
+    float4* __restrict__ wall, uint8_t* __restrict__ dyn, uint32_t* __restrict__ tri_out, MeshParams m) {
     const uint32_t n_tiles = *count;
     for (uint32_t i = blockIdx.x; i < n_tiles; i += gridDim.x) {
         const uint32_t tile = list[i];
@@ -293,6 +335,7 @@ __global__ __launch_bounds__(256) void geometry_tile_pass(
             continue;
         }
         keys[gid] = EMPTY_KEY;
+        if (tri_out) tri_out[gid] = t;   // nearest triangle, for wall tangents
         const float wx = m.origin[0] + ((float)x + 0.5f) * m.dx;
         const float wy = m.origin[1] + ((float)y + 0.5f) * m.dx;
         const float wz = m.origin[2] + ((float)z + 0.5f) * m.dx;
@@ -408,7 +451,7 @@ __global__ void link_census(const uint8_t* __restrict__ prev, const uint8_t* __r
 
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
     hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
-                       b.vel, m);
+                       b.vel, b.body, m);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.dyn, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
@@ -417,7 +460,8 @@ hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipSt
     hipLaunchKernelGGL(geometry_tile_list, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new,
                        b.tiles_mask_old, b.tiles_wall_old, m.g.tiles, b.list, b.list_count);
     hipLaunchKernelGGL(geometry_tile_pass, dim3(2048), dim3(256), 0, s, b.pos, b.vel, b.tri, b.tri_part, b.keys, b.list,
-                       b.list_count, b.tiles_new, b.tiles_mask_old, b.tiles_wall_old, b.mask, b.wall, b.dyn, m);
+                       b.list_count, b.tiles_new, b.tiles_mask_old, b.tiles_wall_old, b.mask, b.wall, b.dyn, b.tri_out,
+                       m);
     return hipGetLastError();
 }
 

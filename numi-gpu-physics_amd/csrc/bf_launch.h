@@ -27,6 +27,7 @@ struct FluidBuffers {
     float* density;
     float4* velocity;
     uint32_t* activation;   // [count, max restriction bits] of this step's fast tiles
+    const double* body;     // body state (position first): torque reference point
 };
 
 struct MeshParams {
@@ -43,9 +44,11 @@ struct MeshParams {
     uint32_t triangle_count;
     int periodic[3];
     float cull_cells;          // key only cells at most this far (cells) from a triangle; +inf = whole AABB
+    int move_with_body;        // frames are body-frame; place them with the device body pose
 };
 
 struct MeshBuffers {
+    const double* body;                 // body state, read when MeshParams::move_with_body
     const float* points;
     const float* times;
     const uint32_t* tri;
@@ -61,7 +64,60 @@ struct MeshBuffers {
     uint8_t* mask;
     float4* wall;
     uint8_t* dyn;
+    uint32_t* tri_out;                  // optional: nearest triangle of each resolved cell
 };
+
+// Adjoint of one step (bf_adjoint.hip).  lam_in = dJ/df_s, lam_out =
+// dJ/df_{s-1}; f_prev = f_{s-1}; masks and wall of step s-1 / s; nu:
+// per listed tile 19 x 256 floats; dW: [K][N] wall tangents of step s.
+struct AdjointBuffers {
+    const float* f_prev;
+    const float* lam_in;
+    float* lam_out;
+    const uint8_t* solid_prev;
+    const uint8_t* solid_cur;
+    const float4* wall;
+    const uint8_t* static_flags;
+    const uint8_t* dyn_prev;
+    const uint8_t* dyn_cur;
+    const uint32_t* general_list;
+    const uint32_t* general_count;
+    float* nu;
+    const float4* dW;
+    double* grad_partials;        // [listed tile][8]
+    double* grad_total;           // [8], accumulated over steps
+    const double* body;
+};
+struct AdjointParams {
+    float cF[3], cT[3];           // load weights of the step: J_s = cF . F + cT . T
+    int K;                        // parameters (<= 8)
+};
+hipError_t launch_adjoint_step(int coll, const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
+                               int general_blocks, hipStream_t s);
+hipError_t launch_wall_tangent(const uint32_t* tri_of_cell, const uint32_t* tri, const float4* pos,
+                               const float4* vel, const float* dpoints, const float* times, float4* dpos,
+                               float4* dvel, float4* dW, const MeshParams& m, int K, hipStream_t s);
+
+// Rigid body (FP64): integration after a step's loads and the safety monitor.
+struct BodyParams {
+    double mass;
+    double inertia[3];
+    double gravity[3];
+    double external_force[3];
+    double external_torque[3];
+    double dt;                       // world time per fluid step
+    int substeps;
+    int monitor;
+    double extent[3];                // body-frame half extents for the clearance check
+    double rotation_radius;
+    double wing_speed;
+    double mach_limit;
+    double domain_min[3], domain_max[3];
+    double far_velocity_world[3];
+    double velocity_to_lattice;
+};
+hipError_t launch_body_step(double* state, const double* load, double* history, uint32_t history_slot,
+                            double* metrics, uint32_t* event, uint64_t step, const BodyParams& bp, hipStream_t s);
 
 hipError_t launch_fluid_fast(int arith, int storage, int coll, const FluidBuffers& b, const StepParams& p,
                              hipStream_t s);

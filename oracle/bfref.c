@@ -403,3 +403,86 @@ void bfo_body_integrate(bfo_body* b, const double force[3], const double torque_
     }
     memcpy(b->orientation, q, sizeof q);
 }
+
+/* BirdFlow.metal:6629-6707, evaluated in FP64. */
+uint32_t bfo_body_safety(const bfo_body* b, const double extent[3], double rotation_radius,
+                         double wing_speed, double mach_limit, const double domain_min[3],
+                         const double domain_max[3], const double far_velocity_world[3],
+                         double velocity_to_lattice, uint64_t step, double metrics[4], uint32_t event[3])
+{
+    double q[4];
+    memcpy(q, b->orientation, sizeof q);
+    qnormalize(q);
+    double world_extent[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < 3; ++a) {
+        double e[3] = {0.0, 0.0, 0.0}, r[3];
+        e[a] = extent[a];
+        qrotate(q, e, r);
+        for (int i = 0; i < 3; ++i) world_extent[i] += fabs(r[i]);
+    }
+    double clearance = INFINITY;
+    for (int i = 0; i < 3; ++i) {
+        double lo = b->position[i] - world_extent[i] - domain_min[i];
+        double hi = domain_max[i] - b->position[i] - world_extent[i];
+        clearance = fmin(clearance, fmin(lo, hi));
+    }
+    double rel[3];
+    for (int i = 0; i < 3; ++i) rel[i] = b->linear_velocity[i] - far_velocity_world[i];
+    const double* w = b->angular_velocity_body;
+    double translation = sqrt(rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2]);
+    double rotation = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]) * rotation_radius;
+    double mach = (translation + rotation + wing_speed) * velocity_to_lattice / sqrt(1.0 / 3.0);
+    int finite = isfinite(mach) && isfinite(clearance);
+    for (int i = 0; i < 3; ++i)
+        finite = finite && isfinite(b->position[i]) && isfinite(b->linear_velocity[i]) && isfinite(w[i]);
+    for (int i = 0; i < 4; ++i) finite = finite && isfinite(b->orientation[i]);
+    if (isfinite(mach)) metrics[0] = fmax(metrics[0], mach);
+    if (isfinite(clearance)) metrics[1] = fmin(metrics[1], clearance);
+    uint32_t flags = 0;
+    if (!finite) flags |= 4u;
+    if (isfinite(mach) && mach > mach_limit) flags |= 1u;
+    if (isfinite(clearance) && clearance < 0.0) flags |= 2u;
+    if (flags != 0u && event[2] == 0u) {
+        event[0] = (uint32_t)step;
+        event[1] = (uint32_t)(step >> 32);
+        event[2] = flags;
+        metrics[2] = mach;
+        metrics[3] = clearance;
+    }
+    return flags;
+}
+
+static void qrotatef(const float q[4], const float v[3], float o[3])
+{
+    float t[3] = {2.0f * (q[1] * v[2] - q[2] * v[1]), 2.0f * (q[2] * v[0] - q[0] * v[2]),
+                  2.0f * (q[0] * v[1] - q[1] * v[0])};
+    float c[3] = {q[1] * t[2] - q[2] * t[1], q[2] * t[0] - q[0] * t[2], q[0] * t[1] - q[1] * t[0]};
+    for (int i = 0; i < 3; ++i) o[i] = v[i] + q[3] * t[i] + c[i];
+}
+
+void bfo_place_vertices(const bfo_body* b, int vertex_count, float velocity_scale,
+                        const float* local_position3, const float* local_velocity3,
+                        float* position3, float* velocity3)
+{
+    double qd[4];
+    memcpy(qd, b->orientation, sizeof qd);
+    qnormalize(qd);
+    const float q[4] = {(float)qd[0], (float)qd[1], (float)qd[2], (float)qd[3]};
+    const float x[3] = {(float)b->position[0], (float)b->position[1], (float)b->position[2]};
+    const float vb[3] = {(float)b->linear_velocity[0], (float)b->linear_velocity[1], (float)b->linear_velocity[2]};
+    const float w[3] = {(float)b->angular_velocity_body[0], (float)b->angular_velocity_body[1],
+                        (float)b->angular_velocity_body[2]};
+    for (int v = 0; v < vertex_count; ++v) {
+        const float* p = local_position3 + 3 * v;
+        const float* u = local_velocity3 + 3 * v;
+        float wxp[3] = {w[1] * p[2] - w[2] * p[1], w[2] * p[0] - w[0] * p[2], w[0] * p[1] - w[1] * p[0]};
+        float r[3], rw[3], ru[3];
+        qrotatef(q, p, r);
+        qrotatef(q, wxp, rw);
+        qrotatef(q, u, ru);
+        for (int i = 0; i < 3; ++i) {
+            position3[3 * v + i] = x[i] + r[i];
+            velocity3[3 * v + i] = (vb[i] + (rw[i] + ru[i])) * velocity_scale;
+        }
+    }
+}

@@ -24,6 +24,7 @@
  *       Sources/BirdFlowMetal/MetalShearWaveValidation.swift:666-755
  *   - translating-sphere topology canonical  BirdFlow.metal:183-195,2263-2350
  *   - six-degree-of-freedom body update  BirdFlow.metal:6574-6627,
+ *   - runtime safety monitor             BirdFlow.metal:6629-6707,
  *       Sources/BirdFlowCore/RigidBody.swift:85-127
  *
  * Fluid functions exist in two precisions: *_f32 (same operation order as the
@@ -164,6 +165,23 @@ typedef struct bfo_body {
 void bfo_body_integrate(bfo_body* b, const double force[3], const double torque_world[3],
                         double mass, const double inertia[3], const double gravity[3],
                         double dt, int substeps);
+
+/* Runtime safety monitor (BirdFlow.metal:6629-6707) in FP64.  metrics:
+ * [0] max Mach, [1] min clearance, [2] Mach and [3] clearance at the first
+ * event; event[0..1] = step low/high words, event[2] = flags (1 Mach,
+ * 2 clearance, 4 non-finite state), kept from the first event on.
+ * Returns this call's flags. */
+uint32_t bfo_body_safety(const bfo_body* b, const double extent[3], double rotation_radius,
+                         double wing_speed, double mach_limit, const double domain_min[3],
+                         const double domain_max[3], const double far_velocity_world[3],
+                         double velocity_to_lattice, uint64_t step, double metrics[4], uint32_t event[3]);
+
+/* Mesh vertices carried by the body pose (FP32, the order the device
+ * uses): p = x_b + R(q) p_local, v = s (v_b + R(w_b x p_local) + R v_local),
+ * with p_local / v_local the body-frame frame interpolation. */
+void bfo_place_vertices(const bfo_body* b, int vertex_count, float velocity_scale,
+                        const float* local_position3, const float* local_velocity3,
+                        float* position3, float* velocity3);
 
 #ifdef __cplusplus
 }

@@ -112,6 +112,10 @@ def _declare(l):
     l.bfo_link_census.argtypes = [P, vp, vp, vp, vp, vp]
     l.bfo_body_integrate.argtypes = [ctypes.POINTER(Body), vp, vp, ctypes.c_double, vp, vp,
                                      ctypes.c_double, ctypes.c_int]
+    l.bfo_body_safety.argtypes = [ctypes.POINTER(Body), vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  vp, vp, vp, ctypes.c_double, ctypes.c_uint64, vp, vp]
+    l.bfo_body_safety.restype = ctypes.c_uint32
+    l.bfo_place_vertices.argtypes = [ctypes.POINTER(Body), ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
 
 
 def make_params(shape, *, omega_plus, omega_minus, bc=None, collision=COLLISION_TRT,
@@ -267,6 +271,52 @@ def link_census(p: Params, solid_prev, solid_cur):
     lib().bfo_link_census(ctypes.byref(p), _ptr(np.ascontiguousarray(solid_prev, np.uint8)),
                           _ptr(np.ascontiguousarray(solid_cur, np.uint8)), _ptr(counts), _ptr(cov), _ptr(unc))
     return counts, int(cov[0]), int(unc[0])
+
+
+def make_body(position=(0, 0, 0), orientation=(0, 0, 0, 1), linear_velocity=(0, 0, 0),
+              angular_velocity_body=(0, 0, 0)) -> Body:
+    b = Body()
+    b.position[:] = [float(v) for v in position]
+    b.orientation[:] = [float(v) for v in orientation]
+    b.linear_velocity[:] = [float(v) for v in linear_velocity]
+    b.angular_velocity_body[:] = [float(v) for v in angular_velocity_body]
+    return b
+
+
+def body_array(body: Body) -> np.ndarray:
+    """position, orientation (x y z w), linear velocity, body angular velocity."""
+    return np.array(list(body.position) + list(body.orientation) + list(body.linear_velocity)
+                    + list(body.angular_velocity_body), np.float64)
+
+
+class Safety:
+    """Running safety record: metrics [max Mach, min clearance, Mach and
+    clearance at the first event], event [step lo, step hi, flags]."""
+
+    def __init__(self):
+        self.metrics = np.array([0.0, np.finfo(np.float64).max, 0.0, 0.0], np.float64)
+        self.event = np.zeros(3, np.uint32)
+
+
+def body_safety(body: Body, record: Safety, step, extent, rotation_radius, wing_speed, mach_limit, domain_min,
+                domain_max, far_velocity_world, velocity_to_lattice):
+    e = np.ascontiguousarray(extent, np.float64)
+    lo = np.ascontiguousarray(domain_min, np.float64)
+    hi = np.ascontiguousarray(domain_max, np.float64)
+    fv = np.ascontiguousarray(far_velocity_world, np.float64)
+    return int(lib().bfo_body_safety(ctypes.byref(body), _ptr(e), float(rotation_radius), float(wing_speed),
+                                     float(mach_limit), _ptr(lo), _ptr(hi), _ptr(fv), float(velocity_to_lattice),
+                                     int(step), _ptr(record.metrics), _ptr(record.event)))
+
+
+def place_vertices(body: Body, local_pos, local_vel, velocity_scale=1.0):
+    lp = np.ascontiguousarray(local_pos, np.float32)
+    lv = np.ascontiguousarray(local_vel, np.float32)
+    pos = np.empty_like(lp)
+    vel = np.empty_like(lv)
+    lib().bfo_place_vertices(ctypes.byref(body), lp.shape[0], np.float32(velocity_scale), _ptr(lp), _ptr(lv),
+                             _ptr(pos), _ptr(vel))
+    return pos, vel
 
 
 def body_integrate(body: Body, force, torque, mass, inertia, gravity, dt, substeps=1):

@@ -151,3 +151,41 @@ def plate_params(shape=(96, 48, 48), collision=bfref.COLLISION_RR3, omega_plus=1
             (bfref.BC_PERIODIC, bfref.BC_PERIODIC)),
         collision=collision, wall_rule=bfref.WALL_INTERPOLATED_SDF, force_mode=6, accumulate_loads=True,
         far_u=(inflow, 0.0, 0.0), body_position=(shape[0] / 4, shape[1] / 2, shape[2] / 2))
+
+
+def integrate_body_oracle(body, loads, params, dt):
+    """One free-flight update of an oracle Body from a step's loads (FP64)."""
+    force = np.asarray(loads[:3], np.float64) + np.asarray(params["external_force"], np.float64)
+    torque = np.asarray(loads[3:6], np.float64) + np.asarray(params["external_torque"], np.float64)
+    return bfref.body_integrate(body, force, torque, params["mass"], params["inertia"], params["gravity"], dt,
+                                params["substeps"])
+
+
+def run_coupled_body_oracle(p, seq, steps, body, params, dtype=np.float32):
+    """Free flight with the mesh carried by the body (frames in the body frame):
+    geometry of step s from the pose after step s-1 at time s, fluid step with
+    the torque reference at that pose, loads, then the body update.  Returns
+    the final populations, the load and body histories and the final mask."""
+    def geometry(t):
+        first, _, _ = bfref.frame_interval(seq.times, t)
+        lp, lv = bfref.prepare_vertices(seq.points, seq.times, first, t, velocity_scale=1.0)
+        pos, vel = bfref.place_vertices(body, lp, lv, velocity_scale=1.0)
+        mask, wall, _ = bfref.raster_resolve(p, pos, vel, seq.triangles, seq.parts, seq.half_thickness, 64)
+        return mask, wall
+
+    for i in range(3):
+        p.body_position[i] = body.position[i]
+    mask0, wall0 = geometry(float(seq.times[0]))
+    f = bfref.init_equilibrium(p, mask0, wall0, dtype)
+    prev = mask0
+    loads, states = [], []
+    for s in range(1, steps + 1):
+        mask, wall = geometry(float(seq.times[0]) + s)
+        for i in range(3):
+            p.body_position[i] = body.position[i]
+        f, ld, _, _ = bfref.step(p, f, prev, mask, wall)
+        integrate_body_oracle(body, ld, params, 1.0)
+        loads.append(ld)
+        states.append(bfref.body_array(body))
+        prev = mask
+    return f, np.array(loads), np.array(states), prev

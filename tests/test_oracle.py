@@ -235,3 +235,55 @@ def test_rigid_body_constant_force_and_free_spin():
     assert list(s.angular_velocity_body) == [0, 0, 0.5]
     angle = 2 * math.atan2(s.orientation[2], s.orientation[3])
     assert abs(angle - 0.5) < 1e-6
+
+
+def test_rigid_body_spin_follows_discrete_closed_form():
+    # q <- normalize(q + dt/2 q (x) (w, 0)) with constant body w multiplies q
+    # by normalize(w dt/2, 1): a rotation by 2 atan(|w| dt / 2) per substep.
+    s = bfref.make_body(angular_velocity_body=(0.0, 0.0, 0.5))
+    for _ in range(250):
+        bfref.body_integrate(s, (0, 0, 0), (0, 0, 0), 1.0, (1.0, 2.0, 3.0), (0, 0, 0), 4e-3, substeps=4)
+    angle = 2 * math.atan2(s.orientation[2], s.orientation[3])
+    assert abs(angle - 1000 * 2 * math.atan(0.5 * 1e-3 / 2)) < 1e-12
+    # a body-frame torque about the spin axis accelerates it linearly
+    t = bfref.make_body()
+    for _ in range(10):
+        bfref.body_integrate(t, (0, 0, 0), (0, 0, 0.3), 1.0, (1.0, 2.0, 3.0), (0, 0, 0), 0.1)
+    assert abs(t.angular_velocity_body[2] - 10 * 0.3 / 3.0 * 0.1) < 1e-15
+
+
+def test_safety_monitor_known_answers():
+    # Mach = (|v - u_far| + |w| r + wing speed) * dt/dx / c_s; clearance of
+    # the rotated extent box to the domain shrunk by (sponge width + 3) dx
+    b = bfref.make_body(position=(5, 5, 5), linear_velocity=(0.1, 0, 0), angular_velocity_body=(0, 0, 2))
+    rec = bfref.Safety()
+    args = dict(extent=(1, 1, 1), rotation_radius=2.0, wing_speed=0.0, mach_limit=1.0, domain_min=(0, 0, 0),
+                domain_max=(10, 10, 10), far_velocity_world=(0, 0, 0), velocity_to_lattice=0.1)
+    assert bfref.body_safety(b, rec, 3, **args) == 0
+    mach = (0.1 + 4.0) * 0.1 / math.sqrt(1.0 / 3.0)
+    assert abs(rec.metrics[0] - mach) < 1e-15 and rec.metrics[1] == 4.0 and rec.event[2] == 0
+    # a 45-degree yaw widens the x/y extent to sqrt(2); moving the body to
+    # x = 8.9 leaves 10 - 8.9 - sqrt(2) < 0
+    c = math.cos(math.pi / 8)
+    b2 = bfref.make_body(position=(8.9, 5, 5), orientation=(0, 0, math.sin(math.pi / 8), c))
+    assert bfref.body_safety(b2, rec, 7, **args) == 2
+    assert rec.event[0] == 7 and rec.event[2] == 2
+    assert abs(rec.metrics[3] - (10 - 8.9 - math.sqrt(2))) < 1e-12
+    # later events keep the first; non-finite states raise flag 4
+    b3 = bfref.make_body(position=(float("nan"), 5, 5))
+    assert bfref.body_safety(b3, rec, 9, **args) & 4
+    assert rec.event[0] == 7 and rec.event[2] == 2
+
+
+def test_place_vertices_rigid_motion():
+    # 90-degree rotation about z, translation, spin about z: p = x + R p_l,
+    # v = s (v_b + R (w x p_l) + R v_l)
+    h = math.sqrt(0.5)
+    b = bfref.make_body(position=(10, 20, 30), orientation=(0, 0, h, h), linear_velocity=(0.5, 0, 0),
+                        angular_velocity_body=(0, 0, 0.25))
+    lp = np.array([[1.0, 0.0, 0.0], [0.0, 2.0, 1.0]], np.float32)
+    lv = np.array([[0.0, 0.0, 0.1], [0.0, 0.0, 0.0]], np.float32)
+    pos, vel = bfref.place_vertices(b, lp, lv, velocity_scale=2.0)
+    assert np.allclose(pos, [[10, 21, 30], [8, 20, 31]], atol=1e-5)
+    # w x p_l = (0, 0.25, 0) and (-0.5, 0, 0), rotated by R: (-0.25, 0, 0) and (0, -0.5, 0)
+    assert np.allclose(vel, 2.0 * np.array([[0.25, 0.0, 0.1], [0.5, -0.5, 0.0]]), atol=1e-6)
