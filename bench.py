@@ -30,12 +30,12 @@ sys.path[:0] = [os.path.join(ROOT, "numi-gpu-physics_amd")]
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS + %HBM roofline, moving-bird 512×256×256 FP32 vs FP16-store; adjoint/fwd"
 STORAGES = {"fp32": 0, "fp16": 1, "bf16": 2}
-ARITHMETIC = {"fused": 0, "strict": 1}
+ARITHMETIC = {"default": 0, "strict": 1, "fused": 2}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fluid_fast_traffic.json")
 POP_BYTES = {"fp32": 152, "fp16": 76, "bf16": 76}
 
 
-def bird_config(shape, storage, arithmetic="fused"):
+def bird_config(shape, storage, arithmetic="default"):
     import birdflow_hip as bh
     tau = 0.52
     return bh.SolverConfig(arithmetic=ARITHMETIC[arithmetic],
@@ -52,7 +52,7 @@ def bird_mesh(steps, shape):
     return workloads.synthetic_bird(steps, seed=7, center=(160.0 * s, 128.0 * s, 128.0 * s))
 
 
-def run_gpu(shape, steps, warmup, storage, arithmetic="fused"):
+def run_gpu(shape, steps, warmup, storage, arithmetic="default"):
     import birdflow_hip as bh
     seq = bird_mesh(warmup + steps, shape)
     cfg = bird_config(shape, storage, arithmetic)
@@ -126,6 +126,39 @@ def run_cpu_baseline(shape, steps):
     return n * steps / elapsed / 1e6, threads, elapsed
 
 
+def run_adjoint(shape, steps, interval):
+    """Adjoint cost on the differentiation workload: the two-harmonic flapping
+    plate (8 kinematic parameters), RR3, interpolated walls, far field in x,
+    periodic y / z, u_x = 0.05; J = time-averaged lift over the last two
+    thirds of the steps.  Returns forward and reverse times per step."""
+    import birdflow_hip as bh
+    from birdflow_hip import workloads
+    s = shape[0] / 256.0
+    tau = 0.6
+    cfg = bh.SolverConfig(
+        shape=shape, omega_plus=1.0 / tau, omega_minus=1.0 / (0.5 + (3.0 / 16.0) / (tau - 0.5)),
+        collision=bh.COLLISION_RR3, wall_rule=bh.WALL_INTERPOLATED_SDF, force_mode=6, accumulate_loads=True,
+        boundary=((bh.BC_FARFIELD, bh.BC_FARFIELD), (0, 0), (0, 0)), far_velocity=(0.05, 0.0, 0.0),
+        body_position=(64.0 * s, 64.0 * s, 64.0 * s))
+    theta = np.array([8.0 * s, 0.0, 2.0 * s, 0.3, 0.5, np.pi / 2, 0.1, 0.2])
+    seq, _, tan = workloads.parametric_plate(theta, steps, center=(64.0 * s, 64.0 * s, 64.0 * s), span=48.0 * s,
+                                             chord=16.0 * s, thickness=2.0, period=800.0, half_thickness=0.5)
+    weights = np.zeros((steps, 6))
+    weights[steps // 3:, 2] = 1.0 / (steps - steps // 3)
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.adjoint_set_tangents(tan)
+        sim.init_equilibrium()
+        J, grad, t = sim.adjoint_gradient(steps, weights, interval)
+    return {"forward_ms_per_step": t["forward_ms"] / steps, "reverse_ms_per_step": t["reverse_ms"] / steps,
+            "ratio_reverse_vs_forward": t["reverse_ms"] / t["forward_ms"],
+            "ratio_gradient_vs_forward": (t["forward_ms"] + t["reverse_ms"]) / t["forward_ms"],
+            "objective": J, "gradient": grad.tolist(), "finite": bool(np.isfinite(grad).all()),
+            "config": {"workload": f"parametric flapping plate {shape[0]}x{shape[1]}x{shape[2]}, RR3, "
+                                   "interpolated walls, 8 kinematic parameters, mean lift",
+                       "steps": steps, "checkpoint_interval": interval}}
+
+
 def measured_traffic(storage, arithmetic, cells):
     """HBM bytes per fast-kernel launch from the committed rocprofv3 PMC passes
     (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel) for this storage
@@ -143,10 +176,15 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--storage", choices=list(STORAGES), default="fp32")
-    ap.add_argument("--arithmetic", choices=list(ARITHMETIC), default="fused")
+    ap.add_argument("--arithmetic", choices=list(ARITHMETIC), default="default",
+                    help="collision arithmetic; default: strict for fp32, fused for fp16/bf16")
     ap.add_argument("--shape", type=int, nargs=3, default=(512, 256, 256))
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
-    ap.add_argument("--also-fp16", action="store_true", help="also run the FP16-offset storage form")
+    ap.add_argument("--skip-fp16", action="store_true",
+                    help="skip the FP16-offset storage form (reported beside FP32 by default)")
+    ap.add_argument("--adjoint-steps", type=int, default=240,
+                    help="steps of the adjoint cost measurement on the 256x128x128 plate (0: skip)")
+    ap.add_argument("--adjoint-interval", type=int, default=60, help="checkpoint interval of that sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -163,11 +201,15 @@ def main():
     extra = {"copy_kernel_gbs": copy_gbs}
     if not r["finite"]:
         extra["error"] = "non-finite loads: the run diverged"
-    if args.also_fp16:
+    if args.storage == "fp32" and not args.skip_fp16:
         r16 = run_gpu(shape, args.steps, args.warmup, "fp16", args.arithmetic)
         print(f"# gpu fp16: {r16}", file=sys.stderr, flush=True)
         extra["fp16_store"] = {"value": r16["mlups"], "unit": "MLUPS", "ratio_vs_fp32": r16["mlups"] / r["mlups"],
                                "fast_kernel_ms": r16["fast_kernel_ms"], "achieved_gbs": r16["achieved_gbs"]}
+    if args.adjoint_steps > 0:
+        adj = run_adjoint((256, 128, 128), args.adjoint_steps, args.adjoint_interval)
+        print(f"# adjoint: {adj}", file=sys.stderr, flush=True)
+        extra["adjoint"] = adj
     cpu = None
     if args.cpu_steps > 0:
         mlups, threads, elapsed = run_cpu_baseline(shape, args.cpu_steps)

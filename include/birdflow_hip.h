@@ -76,12 +76,15 @@ enum { BF_WALL_HALFWAY = 0, BF_WALL_INTERPOLATED_SDF = 1, BF_WALL_INTERPOLATED_H
  * arithmetic. */
 enum { BF_STORAGE_FP32 = 0, BF_STORAGE_FP16_OFFSET = 1, BF_STORAGE_BF16_OFFSET = 2 };
 
-/* Collision arithmetic.  FUSED (the default) evaluates the operators over
- * opposite-direction pairs with fused multiply-adds; STRICT evaluates the
- * reference's expressions in the reference's order without contraction
- * (FP32 storage only), which makes the step reproducible operation for
- * operation on any IEEE FP32 machine.  The two differ by rounding. */
-enum { BF_ARITHMETIC_FUSED = 0, BF_ARITHMETIC_STRICT = 1 };
+/* Collision arithmetic.  STRICT evaluates the reference's expressions in
+ * the reference's order without contraction (FP32 storage only): the step is
+ * reproducible operation for operation on any IEEE FP32 machine.  FUSED
+ * evaluates the same operators over opposite-direction pairs with fused
+ * multiply-adds, about a third of the instructions; the two differ by
+ * rounding.  DEFAULT (0, a zero-initialised config) is STRICT for FP32
+ * storage, where the step is bandwidth-bound either way, and FUSED for
+ * FP16 / BF16 storage, where the arithmetic would otherwise bound it. */
+enum { BF_ARITHMETIC_DEFAULT = 0, BF_ARITHMETIC_STRICT = 1, BF_ARITHMETIC_FUSED = 2 };
 
 /* Where each step's geometry comes from. */
 enum { BF_GEOMETRY_STATIC = 0, BF_GEOMETRY_MESH = 1, BF_GEOMETRY_SPHERE = 2 };
@@ -110,7 +113,7 @@ typedef struct bf_config {
     float far_velocity[3];        /* lattice units */
     float far_density;
     float body_position[3];       /* torque reference point (world units) */
-    uint32_t arithmetic;          /* BF_ARITHMETIC_*; 0 (fused) when zero-initialised */
+    uint32_t arithmetic;          /* BF_ARITHMETIC_*; 0 = default for the storage */
     uint32_t reserved[7];
 } bf_config;
 
@@ -252,6 +255,33 @@ bf_status bf_read_body_history(bf_context* ctx, bf_body_state* out, uint32_t max
  * bf_synchronize returns BF_ERROR_RUNTIME_SAFETY_VIOLATION once when a new
  * event is recorded; the context stays usable. */
 bf_status bf_read_safety(bf_context* ctx, double metrics[4], uint64_t* event_step, uint32_t* flags);
+
+/* ---- Adjoint (reverse-mode gradient of a load functional) ----
+ * J = sum over the next `steps` steps s of sum_i w[s][i] L_s[i], with L_s the
+ * step's force (i = 0..2) and torque (3..5) as bf_read_loads reports them.
+ * dJ/dtheta for up to 8 parameters theta_k that move the mesh: the caller
+ * supplies d(frame vertex positions)/d theta_k; wall velocities and signed
+ * distances then follow from the same frame interpolation and closest-point
+ * geometry.  Each step's link set (masks) is held fixed, as it is
+ * discontinuous in theta; link fractions and wall velocities are
+ * differentiated.  No counterpart exists in the reference.  Supported: FP32
+ * storage, TRT / regularized / RR3, periodic and far-field faces, force
+ * mode 6, mesh or static geometry, no free flight.
+ * Populations are snapshotted every checkpoint_interval steps in the forward
+ * pass and recomputed segment by segment in the reverse sweep (memory:
+ * about (segments + interval) x 76 B and interval x 21 B per cell). */
+/* point_tangents: [parameter_count][frame_count][vertex_count][3]; 0 clears.
+ * A new bf_mesh_upload clears the tangents. */
+bf_status bf_adjoint_set_tangents(bf_context* ctx, uint32_t parameter_count, const float* point_tangents);
+/* Runs `steps` forward steps from the current state (the context advances
+ * as with bf_step), then the reverse sweep.  load_weights: [steps][6].
+ * objective <- J; gradient[parameter_count] <- dJ/dtheta (FP64 sums);
+ * timings_ms (may be NULL) <- {forward pass, reverse sweep}. */
+bf_status bf_adjoint_gradient(bf_context* ctx, uint32_t steps, const double* load_weights,
+                              uint32_t checkpoint_interval, double* objective, double* gradient,
+                              double timings_ms[2]);
+/* dJ/df at the state the last sweep started from (19 x cells, FP32). */
+bf_status bf_adjoint_read_sensitivity(bf_context* ctx, float* out, uint64_t count);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,7 @@ BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
 COLLISION_TRT, COLLISION_TRT_SYMMETRIC_LIMITER, COLLISION_REGULARIZED, COLLISION_RR3 = 0, 1, 2, 3
 WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF = 0, 1, 2
 STORAGE_FP32, STORAGE_FP16_OFFSET, STORAGE_BF16_OFFSET = 0, 1, 2
-ARITHMETIC_FUSED, ARITHMETIC_STRICT = 0, 1
+ARITHMETIC_DEFAULT, ARITHMETIC_STRICT, ARITHMETIC_FUSED = 0, 1, 2
 
 EXPORTED_FUNCTIONS = (
     "bf_create", "bf_destroy", "bf_last_error", "bf_status_string", "bf_abi_version", "bf_device_info",
@@ -43,6 +43,7 @@ EXPORTED_FUNCTIONS = (
     "bf_mesh_upload", "bf_mesh_build", "bf_set_translating_sphere", "bf_step", "bf_synchronize",
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
+    "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity",
 )
 
 
@@ -156,6 +157,9 @@ def _declare(l):
         "bf_body_get": (st, [vp, ctypes.POINTER(BodyState)]),
         "bf_read_body_history": (st, [vp, vp, u32, ctypes.POINTER(u32)]),
         "bf_read_safety": (st, [vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
+        "bf_adjoint_set_tangents": (st, [vp, u32, vp]),
+        "bf_adjoint_gradient": (st, [vp, u32, vp, u32, ctypes.POINTER(ctypes.c_double), vp, vp]),
+        "bf_adjoint_read_sensitivity": (st, [vp, vp, u64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(l, name)

@@ -95,7 +95,7 @@ class SolverConfig:
     far_velocity: Sequence[float] = (0.0, 0.0, 0.0)
     far_density: float = 1.0
     body_position: Sequence[float] = (0.0, 0.0, 0.0)
-    arithmetic: int = _abi.ARITHMETIC_FUSED
+    arithmetic: int = _abi.ARITHMETIC_DEFAULT
 
     def to_c(self) -> _abi.Config:
         c = _abi.Config()
@@ -338,6 +338,33 @@ class BirdFlowSimulation:
         flags = ctypes.c_uint32(0)
         self._check(self._lib.bf_read_safety(self._ctx, _ptr(m), ctypes.byref(step), ctypes.byref(flags)))
         return m, int(step.value), int(flags.value)
+
+    # -- adjoint ---------------------------------------------------------
+    def adjoint_set_tangents(self, tangents):
+        """d(mesh frame vertex positions)/d theta, [K, frames, vertices, 3] (K <= 8)."""
+        t = np.ascontiguousarray(tangents, np.float32)
+        k = 0 if t.size == 0 else t.shape[0]
+        self._check(self._lib.bf_adjoint_set_tangents(self._ctx, k, _ptr(t) if k else None))
+        self._parameters = k
+
+    def adjoint_gradient(self, steps: int, load_weights, checkpoint_interval: int = 100):
+        """Runs `steps` steps and the reverse sweep for J = sum_s w[s] . L_s
+        (load_weights [steps, 6]: force xyz, torque xyz).  Returns
+        (J, dJ/dtheta [K], {forward_ms, reverse_ms})."""
+        w = np.ascontiguousarray(load_weights, np.float64).reshape(int(steps), 6)
+        k = getattr(self, "_parameters", 0)
+        grad = np.zeros(max(k, 1), np.float64)
+        timings = np.zeros(2, np.float64)
+        J = ctypes.c_double(0.0)
+        self._check(self._lib.bf_adjoint_gradient(self._ctx, int(steps), _ptr(w), int(checkpoint_interval),
+                                                  ctypes.byref(J), _ptr(grad), _ptr(timings)))
+        return J.value, grad[:k].copy(), {"forward_ms": float(timings[0]), "reverse_ms": float(timings[1])}
+
+    def adjoint_sensitivity(self) -> np.ndarray:
+        """dJ/df at the state the last adjoint sweep started from (19 x cells)."""
+        out = np.empty(19 * self.config.cells, np.float32)
+        self._check(self._lib.bf_adjoint_read_sensitivity(self._ctx, _ptr(out), out.size))
+        return out
 
     def link_census(self):
         out = np.zeros(21, np.uint64)

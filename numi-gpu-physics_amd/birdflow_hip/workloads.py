@@ -69,6 +69,50 @@ def flapping_plate(steps: int, center=(64.0, 64.0, 64.0), span=48.0, chord=16.0,
                         float(half_thickness))
 
 
+PLATE_PARAMETERS = ("heave_1", "heave_phase_1", "heave_2", "heave_phase_2",
+                    "pitch_1", "pitch_phase_1", "pitch_2", "pitch_phase_2")
+
+
+def parametric_plate(theta, steps: int, center=(64.0, 64.0, 64.0), span=48.0, chord=16.0, thickness=2.0,
+                     period=800.0, half_thickness=0.5, t0=0):
+    """Flapping plate with two-harmonic kinematics and its parameter tangents.
+
+    theta = (H1, phi1, H2, phi2, P1, psi1, P2, psi2) (cells, radians):
+    heave(t) = H1 sin(w t + phi1) + H2 sin(2 w t + phi2),
+    pitch(t) = P1 sin(w t + psi1) + P2 sin(2 w t + psi2), w = 2 pi / period;
+    vertex p(t) = R_y(pitch(t)) p_rest + center + (0, 0, heave(t)).
+    Returns (MeshSequence with FP32 frames, FP64 frames [F, V, 3], tangents
+    d frames / d theta [8, F, V, 3] in FP64)."""
+    th = np.asarray(theta, np.float64)
+    rest, tri = _box(span, chord, thickness)
+    times = np.arange(t0, t0 + steps + 1, dtype=np.float64)
+    F, V = len(times), rest.shape[0]
+    pts = np.empty((F, V, 3))
+    tan = np.zeros((8, F, V, 3))
+    w = 2 * math.pi / period
+    for i, t in enumerate(times):
+        a1, a2 = w * t + th[1], 2 * w * t + th[3]
+        b1, b2 = w * t + th[5], 2 * w * t + th[7]
+        heave = th[0] * math.sin(a1) + th[2] * math.sin(a2)
+        pitch = th[4] * math.sin(b1) + th[6] * math.sin(b2)
+        c, s_ = math.cos(pitch), math.sin(pitch)
+        r = np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]])
+        dr = np.array([[-s_, 0, c], [0, 0, 0], [-c, 0, -s_]])      # d R_y / d pitch
+        pts[i] = rest @ r.T + np.array([center[0], center[1], center[2] + heave])
+        dpitch = rest @ dr.T
+        tan[0, i, :, 2] = math.sin(a1)
+        tan[1, i, :, 2] = th[0] * math.cos(a1)
+        tan[2, i, :, 2] = math.sin(a2)
+        tan[3, i, :, 2] = th[2] * math.cos(a2)
+        tan[4, i] = dpitch * math.sin(b1)
+        tan[5, i] = dpitch * th[4] * math.cos(b1)
+        tan[6, i] = dpitch * math.sin(b2)
+        tan[7, i] = dpitch * th[6] * math.cos(b2)
+    seq = MeshSequence(pts.astype(np.float32), times.astype(np.float32), tri, np.ones(len(tri), np.uint8),
+                       float(half_thickness), meta={"theta": th.tolist(), "parameters": PLATE_PARAMETERS})
+    return seq, pts, tan
+
+
 def _grid_sheet(nu, nv):
     """Triangulated (nu x nv)-quad sheet; returns (u, v) in [0,1]^2 and triangles."""
     u, v = np.meshgrid(np.linspace(0, 1, nu + 1), np.linspace(0, 1, nv + 1), indexing="ij")

@@ -46,42 +46,6 @@ __device__ __forceinline__ float load_weight(const AdjointParams& w, const float
     return (float)cx(q) * (w.cF[0] + t[0]) + (float)cy(q) * (w.cF[1] + t[1]) + (float)cz(q) * (w.cF[2] + t[2]);
 }
 
-// Sources of the six axis directions with their far flags, as fluid_fast.
-struct Sources {
-    int xs_p, xs_m, ys_p, ys_m, zs_p, zs_m;
-    bool xm_far, xp_far, ym_far, yp_far, zm_far, zp_far;
-};
-
-__device__ __forceinline__ Sources sources(const StepParams& p, int x, int y, int z) {
-    Sources s{};
-    s.xm_far = s.xp_far = s.ym_far = s.yp_far = s.zm_far = s.zp_far = false;
-    s.xs_p = shift_axis(x, 1, p.g.nx, p.bc[0][0], p.bc[0][1], s.xm_far);
-    s.xs_m = shift_axis(x, -1, p.g.nx, p.bc[0][0], p.bc[0][1], s.xp_far);
-    s.ys_p = shift_axis(y, 1, p.g.ny, p.bc[1][0], p.bc[1][1], s.ym_far);
-    s.ys_m = shift_axis(y, -1, p.g.ny, p.bc[1][0], p.bc[1][1], s.yp_far);
-    s.zs_p = shift_axis(z, 1, p.g.nz, p.bc[2][0], p.bc[2][1], s.zm_far);
-    s.zs_m = shift_axis(z, -1, p.g.nz, p.bc[2][0], p.bc[2][1], s.zp_far);
-    return s;
-}
-
-template <int q>
-__device__ __forceinline__ uint32_t source_of(const Sources& s, const Grid& g, int x, int y, int z, bool& far) {
-    const int xs = cx(q) > 0 ? s.xs_p : (cx(q) < 0 ? s.xs_m : x);
-    const int ys = cy(q) > 0 ? s.ys_p : (cy(q) < 0 ? s.ys_m : y);
-    const int zs = cz(q) > 0 ? s.zs_p : (cz(q) < 0 ? s.zs_m : z);
-    far = (cx(q) > 0 && s.xm_far) || (cx(q) < 0 && s.xp_far) || (cy(q) > 0 && s.ym_far) || (cy(q) < 0 && s.yp_far) ||
-          (cz(q) > 0 && s.zm_far) || (cz(q) < 0 && s.zp_far);
-    return (uint32_t)xs + (uint32_t)g.nx * ((uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs);
-}
-
-// (x, q) is pulled by nobody when x + c_q leaves the domain through a
-// far-field face
-template <int q>
-__device__ __forceinline__ bool orphan(const Sources& s) {
-    return (cx(q) > 0 && s.xp_far) || (cx(q) < 0 && s.xm_far) || (cy(q) > 0 && s.yp_far) || (cy(q) < 0 && s.ym_far) ||
-           (cz(q) > 0 && s.zp_far) || (cz(q) < 0 && s.zm_far);
-}
-
 template <int COLL>
 __device__ __forceinline__ void collision_adjoint(const float (&g)[19], float op, float om, const float (&lp)[19],
                                                   float (&mu)[19]) {

@@ -23,6 +23,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -33,7 +34,6 @@
 
 using namespace bf;
 
-static_assert(BF_ARITHMETIC_FUSED == ARITH_FUSED && BF_ARITHMETIC_STRICT == ARITH_STRICT, "arithmetic ids");
 static_assert(BF_STORAGE_FP32 == STORE_FP32 && BF_STORAGE_FP16_OFFSET == STORE_FP16_OFFSET &&
                   BF_STORAGE_BF16_OFFSET == STORE_BF16_OFFSET, "storage ids");
 static_assert(BF_COLLISION_RR3 == COLL_RR3 && BF_WALL_INTERPOLATED_HALF == WALL_INTERPOLATED_HALF, "operator ids");
@@ -62,6 +62,13 @@ struct DeviceAlloc {
         bytes = 0;
     }
 };
+
+// Arithmetic of the collision kernels for a configuration.
+inline int arithmetic_of(const bf_config& cfg) {
+    if (cfg.arithmetic == BF_ARITHMETIC_STRICT) return ARITH_STRICT;
+    if (cfg.arithmetic == BF_ARITHMETIC_FUSED) return ARITH_FUSED;
+    return cfg.storage == BF_STORAGE_FP32 ? ARITH_STRICT : ARITH_FUSED;
+}
 
 inline int mod3(int64_t s) { return (int)(((s % 3) + 3) % 3); }
 inline int mod4(int64_t s) { return (int)(((s % 4) + 4) % 4); }
@@ -131,6 +138,32 @@ struct bf_context {
     bool move_mesh = false;
     uint32_t safety_reported = 0;
     uint64_t body_history_count = 0;
+    // adjoint: parameter tangents of the mesh frames and sweep buffers
+    struct {
+        DeviceAlloc alloc;                // sweep buffers (sized by the checkpoint interval)
+        DeviceAlloc tangent_alloc;
+        uint32_t K = 0;
+        float* dpoints = nullptr;         // [K][frames][vertices][3]
+        uint32_t interval = 0, segments = 0;
+        float* checkpoints = nullptr;     // [segments][19N]
+        float* seg_f = nullptr;           // [interval][19N]: f_a .. f_{b-1}
+        uint8_t* seg_mask = nullptr;      // [interval + 1][N]: geometry of steps a .. b
+        float4* seg_wall = nullptr;
+        uint32_t* seg_tri = nullptr;
+        uint8_t* seg_dyn = nullptr;       // [interval + 1][tiles]
+        uint8_t* tiles_new = nullptr;
+        float* lam[2] = {nullptr, nullptr};
+        int lam_cur = 0;
+        bool have_sensitivity = false;
+        float* nu = nullptr;              // [tiles][19][256]
+        float4* dW = nullptr;             // [8][N]
+        float4* dpos = nullptr;           // [8][vertices]
+        float4* dvel = nullptr;
+        double* grad_partials = nullptr;  // [tiles][8]
+        double* grad_total = nullptr;     // [8]
+        uint32_t* list = nullptr;         // general list of the step being reversed
+        uint32_t* count = nullptr;
+    } adj;
     // stepping
     int64_t step = 0;
     uint64_t history_count = 0;
@@ -141,6 +174,10 @@ struct bf_context {
     std::vector<hipEvent_t> events;
     uint32_t profiled = 0;
     int general_blocks = 1024;
+    // geometry grids: bounded so that the geometry of step s+1 leaves the
+    // fluid kernel of step s most of the machine
+    uint32_t raster_blocks = 1024;
+    uint32_t tile_pass_blocks = 512;
 };
 
 namespace {
@@ -263,6 +300,8 @@ MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     // and cells uncovered this step (<= h + one step's wall travel).
     m.cull_cells = exact ? INFINITY : c->m_half / c->cfg.cell_size + 1.5f;
     m.move_with_body = c->move_mesh ? 1 : 0;
+    m.raster_blocks = c->raster_blocks;
+    m.tile_pass_blocks = c->tile_pass_blocks;
     // frame interval, MeasuredBirdSurfaceSequence.swift:174-204
     const std::vector<float>& ts = c->host_times;
     const int last = (int)c->m_frames - 1;
@@ -446,7 +485,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || n64 >= (1ull << 32)) return BF_ERROR_INVALID_CONFIGURATION;
     if (cfg->storage > BF_STORAGE_BF16_OFFSET || cfg->collision > BF_COLLISION_RR3 ||
         cfg->wall_rule > BF_WALL_INTERPOLATED_HALF || cfg->force_mode > 6 || cfg->load_component > 2 ||
-        cfg->arithmetic > BF_ARITHMETIC_STRICT)
+        cfg->arithmetic > BF_ARITHMETIC_FUSED)
         return BF_ERROR_INVALID_CONFIGURATION;
     if (cfg->arithmetic == BF_ARITHMETIC_STRICT && cfg->storage != BF_STORAGE_FP32)
         return BF_ERROR_INVALID_CONFIGURATION;
@@ -483,6 +522,12 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     g.tiles = (uint32_t)((uint64_t)g.tiles_x * g.tiles_y * g.nz);
     c->pop_elem = cfg->storage == BF_STORAGE_FP32 ? 4 : 2;
     c->general_blocks = std::max(1, prop.multiProcessorCount * 4);
+    c->raster_blocks = (uint32_t)std::max(1, prop.multiProcessorCount * 4);
+    c->tile_pass_blocks = (uint32_t)std::max(1, prop.multiProcessorCount * 2);
+    // tuning overrides (workgroup counts)
+    if (const char* v = std::getenv("BIRDFLOW_RASTER_BLOCKS")) c->raster_blocks = (uint32_t)std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("BIRDFLOW_TILE_PASS_BLOCKS"))
+        c->tile_pass_blocks = (uint32_t)std::max(1, std::atoi(v));
 
     // Working-set plan (MetalBackend.validateAllocationPlan, :77-106).
     const uint64_t n = g.n;
@@ -575,6 +620,8 @@ void bf_destroy(bf_context* c) {
         if (c->ev_ready[i]) (void)hipEventDestroy(c->ev_ready[i]);
     }
     c->mesh_alloc.release();
+    c->adj.alloc.release();
+    c->adj.tangent_alloc.release();
     c->alloc.release();
     if (c->sa) (void)hipStreamDestroy(c->sa);
     if (c->sb) (void)hipStreamDestroy(c->sb);
@@ -722,6 +769,9 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     if (!(half_thickness > 0.0f)) return fail(c, BF_ERROR_INVALID_ARGUMENT, "half thickness must be positive");
 
     c->mesh_alloc.release();
+    c->adj.tangent_alloc.release();   // tangents belong to the previous mesh
+    c->adj.K = 0;
+    c->adj.dpoints = nullptr;
     DeviceAlloc& A = c->mesh_alloc;
     hipError_t e = hipSuccess;
     const uint64_t pts = 3ull * frame_count * vertex_count;
@@ -819,7 +869,7 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
     if (c->invalidated) return BF_ERROR_STATE_INVALIDATED;
     (void)hipSetDevice(c->device);
     if (steps == 0) return BF_OK;
-    const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision, arith = (int)c->cfg.arithmetic;
+    const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision, arith = arithmetic_of(c->cfg);
     hipError_t e = hipSuccess;
     // Prologue: geometry (unless already built) and tile list of the first
     // step of this call; event R of the previous step then releases fast(s).
@@ -1032,6 +1082,312 @@ bf_status bf_read_safety(bf_context* c, double metrics[4], uint64_t* event_step,
     if (e != hipSuccess) return hip_fail(c, e, "read safety record");
     *event_step = (uint64_t)ev[0] | ((uint64_t)ev[1] << 32);
     *flags = ev[2];
+    return BF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint (reverse sweep with checkpoint / recompute).
+// ---------------------------------------------------------------------------
+namespace {
+
+bf_status adjoint_supported(bf_context* c) {
+    const bf_config& g = c->cfg;
+    if (g.storage != BF_STORAGE_FP32)
+        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint runs on FP32 population storage");
+    if (g.collision == BF_COLLISION_TRT_SYMMETRIC_LIMITER)
+        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers TRT, regularized and RR3 collisions");
+    if (g.force_mode != 6) return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates force mode 6");
+    if (!g.accumulate_loads) return fail(c, BF_ERROR_INVALID_ARGUMENT, "the objective needs accumulate_loads");
+    for (int a = 0; a < 3; ++a)
+        for (int sd = 0; sd < 2; ++sd)
+            if (g.boundary[a][sd] == BF_BC_OUTLET)
+                return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers periodic and far-field faces");
+    if (c->free_flight) return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint holds the body fixed (no free flight)");
+    if (c->source == BF_GEOMETRY_SPHERE)
+        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers mesh and static geometry");
+    if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs)
+        return fail(c, BF_ERROR_NOT_SUPPORTED, "step once after bf_set_geometry with a different previous mask");
+    return BF_OK;
+}
+
+bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
+    auto& a = c->adj;
+    if (a.interval == interval && a.segments >= segments && a.checkpoints) return BF_OK;
+    a.alloc.release();
+    const uint64_t n = c->g.n, tiles = c->g.tiles, pop = 19ull * n * 4;
+    const uint64_t verts = std::max<uint32_t>(c->m_vertices, 1u);
+    const uint64_t need = pop * segments + pop * interval + (uint64_t)(interval + 1) * (21 * n + tiles) + 2 * pop +
+                          tiles * 19 * 256 * 4 + 8 * 16 * n + 2 * 8 * 16 * verts + tiles * 64 + 4 * tiles + 1024;
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    if (need > (uint64_t)free_b) {
+        char msg[200];
+        std::snprintf(msg, sizeof msg, "adjoint buffers need %.2f GB, %.2f GB free: lower the checkpoint interval",
+                      need / 1e9, free_b / 1e9);
+        return fail(c, BF_ERROR_WORKING_SET_EXCEEDS_MEMORY, msg);
+    }
+    hipError_t e = hipSuccess;
+    DeviceAlloc& A = a.alloc;
+    a.checkpoints = (float*)A.take(e, pop * segments);
+    a.seg_f = (float*)A.take(e, pop * interval);
+    a.seg_mask = (uint8_t*)A.take(e, (interval + 1) * n);
+    a.seg_wall = (float4*)A.take(e, (interval + 1) * 16 * n);
+    a.seg_tri = (uint32_t*)A.take(e, (interval + 1) * 4 * n);
+    a.seg_dyn = (uint8_t*)A.take(e, (interval + 1) * tiles);
+    a.tiles_new = (uint8_t*)A.take(e, tiles);
+    a.lam[0] = (float*)A.take(e, pop);
+    a.lam[1] = (float*)A.take(e, pop);
+    a.nu = (float*)A.take(e, tiles * 19 * 256 * 4);
+    a.dW = (float4*)A.take(e, 8 * 16 * n);
+    a.dpos = (float4*)A.take(e, 8 * 16 * verts);
+    a.dvel = (float4*)A.take(e, 8 * 16 * verts);
+    a.grad_partials = (double*)A.take(e, tiles * 64);
+    a.grad_total = (double*)A.take(e, 64);
+    a.list = (uint32_t*)A.take(e, 4 * tiles);
+    a.count = (uint32_t*)A.take(e, 64);
+    if (e != hipSuccess) {
+        A.release();
+        a.checkpoints = nullptr;
+        return fail(c, BF_ERROR_ALLOCATION_FAILED, "adjoint buffers");
+    }
+    a.interval = interval;
+    a.segments = segments;
+    return BF_OK;
+}
+
+struct SlotGeometry {
+    const uint8_t* mask;
+    const float4* wall;
+    const uint8_t* dyn;
+    const uint32_t* tri;
+};
+
+// Geometry of step s into segment slot k: the same raster / resolve as the
+// forward pass (masks identical, walls identical wherever the step reads
+// them), plus each keyed cell's nearest triangle.
+bf_status adjoint_geometry(bf_context* c, int64_t s, uint32_t k) {
+    auto& a = c->adj;
+    if (c->source == BF_GEOMETRY_STATIC) return BF_OK;
+    const uint64_t n = c->g.n;
+    uint8_t* mask = a.seg_mask + k * n;
+    float4* wall = a.seg_wall + k * n;
+    uint32_t* tri = a.seg_tri + k * n;
+    uint8_t* dyn = a.seg_dyn + (uint64_t)k * c->g.tiles;
+    hipError_t e = launch_fill_u8(mask, 0, (uint32_t)n, c->sb);
+    if (e == hipSuccess) e = launch_fill_wall(wall, (uint32_t)n, c->sb);
+    if (e == hipSuccess) e = launch_fill_u32(tri, 0xFFFFFFFFu, (uint32_t)n, c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "adjoint geometry clear");
+    MeshBuffers b{};
+    b.body = c->body;
+    b.points = c->m_points;
+    b.times = c->m_times;
+    b.tri = c->m_tri;
+    b.tri_part = c->m_part;
+    b.pos = c->m_pos;
+    b.vel = c->m_vel;
+    b.keys = c->keys;
+    b.tiles_new = a.tiles_new;
+    b.tiles_mask_old = c->no_tiles;
+    b.tiles_wall_old = c->no_tiles;
+    b.list = c->geo_list;
+    b.list_count = &c->counters[2];
+    b.mask = mask;
+    b.wall = wall;
+    b.dyn = dyn;
+    b.tri_out = tri;
+    e = launch_mesh_geometry(b, mesh_params(c, time_of(c, s), false), c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "adjoint geometry");
+    return BF_OK;
+}
+
+SlotGeometry slot_geometry(const bf_context* c, uint32_t k) {
+    const auto& a = c->adj;
+    if (c->source == BF_GEOMETRY_STATIC) return {c->mask[0], c->wall[0], c->dyn[0], nullptr};
+    const uint64_t n = c->g.n;
+    return {a.seg_mask + k * n, a.seg_wall + k * n, a.seg_dyn + (uint64_t)k * c->g.tiles, a.seg_tri + k * n};
+}
+
+hipError_t adjoint_list(bf_context* c, const SlotGeometry& prev, const SlotGeometry& cur) {
+    hipError_t e = launch_fill_u32(c->adj.count, 0u, 1u, c->sb);
+    if (e == hipSuccess)
+        e = launch_build_general_list(c->static_flags, prev.dyn, cur.dyn, c->g.tiles, c->adj.list, c->adj.count, c->sb);
+    return e;
+}
+
+}  // namespace
+
+bf_status bf_adjoint_set_tangents(bf_context* c, uint32_t parameter_count, const float* point_tangents) {
+    BF_ENTER(c);
+    if (parameter_count > 8) return fail(c, BF_ERROR_INVALID_ARGUMENT, "at most 8 parameters");
+    if (parameter_count > 0 && !point_tangents) return fail(c, BF_ERROR_INVALID_ARGUMENT, "tangents are NULL");
+    if (parameter_count > 0 && c->source != BF_GEOMETRY_MESH)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "tangents describe mesh frames: upload a mesh first");
+    c->adj.tangent_alloc.release();
+    c->adj.dpoints = nullptr;
+    c->adj.K = 0;
+    if (parameter_count == 0) return BF_OK;
+    const uint64_t count = 3ull * parameter_count * c->m_frames * c->m_vertices;
+    for (uint64_t i = 0; i < count; ++i)
+        if (!std::isfinite(point_tangents[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "non-finite tangent");
+    hipError_t e = hipSuccess;
+    c->adj.dpoints = (float*)c->adj.tangent_alloc.take(e, 4 * count);
+    if (e == hipSuccess) e = hipMemcpy(c->adj.dpoints, point_tangents, 4 * count, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(c, e, "tangent upload", false);
+    c->adj.K = parameter_count;
+    return BF_OK;
+}
+
+bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_weights, uint32_t checkpoint_interval,
+                              double* objective, double* gradient, double timings_ms[2]) {
+    BF_ENTER(c);
+    if (steps == 0 || !load_weights || !objective || checkpoint_interval == 0)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "steps, load weights, objective and interval are required");
+    if (c->adj.K > 0 && !gradient) return fail(c, BF_ERROR_INVALID_ARGUMENT, "gradient output is NULL");
+    if (steps > HISTORY_CAPACITY) return fail(c, BF_ERROR_INVALID_ADVANCE_REQUEST, "at most 65536 steps per sweep");
+    if (bf_status r = adjoint_supported(c)) return r;
+    auto& a = c->adj;
+    const uint32_t interval = std::min(checkpoint_interval, steps);
+    const uint32_t segments = (steps + interval - 1) / interval;
+    if (bf_status r = adjoint_alloc(c, interval, segments)) return r;
+    const uint64_t n = c->g.n, pop = 19ull * n;
+    const int64_t s0 = c->step;
+    hipEvent_t t0, t1, t2;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    (void)hipEventCreate(&t2);
+    a.have_sensitivity = false;
+
+    // forward pass: the ordinary pipelined steps, a snapshot every interval
+    hipError_t e = hipEventRecord(t0, c->sb);
+    for (uint32_t seg = 0; seg < segments && e == hipSuccess; ++seg) {
+        e = hipMemcpyAsync(a.checkpoints + seg * pop, c->f[c->cur], 4 * pop, hipMemcpyDeviceToDevice, c->sb);
+        if (e != hipSuccess) break;
+        const uint32_t todo = std::min<uint32_t>(interval, steps - seg * interval);
+        if (bf_status r = bf_step(c, todo)) return r;
+        e = drain(c);
+    }
+    if (e == hipSuccess) e = hipEventRecord(t1, c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "adjoint forward pass");
+    std::vector<double> hist(8ull * steps);
+    uint32_t got = 0;
+    if (bf_status r = bf_read_load_history(c, hist.data(), steps, &got)) return r;
+    double J = 0.0;
+    for (uint32_t i = 0; i < got; ++i)
+        for (int k = 0; k < 6; ++k) J += load_weights[6 * i + k] * hist[8 * i + k];
+    *objective = J;
+
+    // reverse sweep
+    const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision, arith = arithmetic_of(c->cfg);
+    StepParams pf = make_step_params(c);
+    pf.accumulate_loads = 0;     // the recompute needs populations only
+    const StepParams pa = make_step_params(c);
+    a.lam_cur = 0;
+    e = launch_fill_u32((uint32_t*)a.lam[0], 0u, (uint32_t)pop, c->sb);    // dJ/df at the last step: 0
+    if (e == hipSuccess) e = launch_fill_u32((uint32_t*)a.grad_total, 0u, 16u, c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "adjoint init");
+    for (int64_t seg = (int64_t)segments - 1; seg >= 0; --seg) {
+        const int64_t sa_ = s0 + seg * interval;
+        const int64_t sb_ = std::min<int64_t>(sa_ + interval, s0 + steps);
+        for (int64_t s = sa_; s <= sb_; ++s)
+            if (bf_status r = adjoint_geometry(c, s, (uint32_t)(s - sa_))) return r;
+        e = hipMemcpyAsync(a.seg_f, a.checkpoints + seg * pop, 4 * pop, hipMemcpyDeviceToDevice, c->sb);
+        for (int64_t s = sa_ + 1; s < sb_ && e == hipSuccess; ++s) {
+            const SlotGeometry gp = slot_geometry(c, (uint32_t)(s - 1 - sa_)), gc = slot_geometry(c, (uint32_t)(s - sa_));
+            FluidBuffers fb{};
+            fb.fin = a.seg_f + (s - 1 - sa_) * pop;
+            fb.fout = a.seg_f + (s - sa_) * pop;
+            fb.solid_prev = gp.mask;
+            fb.solid_cur = gc.mask;
+            fb.wall = gc.wall;
+            fb.static_flags = c->static_flags;
+            fb.dyn_prev = gp.dyn;
+            fb.dyn_cur = gc.dyn;
+            fb.general_list = a.list;
+            fb.general_count = a.count;
+            fb.partials = c->partials;
+            fb.density = c->density;
+            fb.velocity = c->velocity;
+            fb.activation = &c->counters[8];
+            fb.body = c->body;
+            e = adjoint_list(c, gp, gc);
+            if (e == hipSuccess) e = launch_fluid_fast(arith, storage, coll, fb, pf, c->sb);
+            if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, pf, c->general_blocks, c->sb);
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "adjoint recompute");
+        for (int64_t s = sb_; s > sa_; --s) {
+            const uint32_t k = (uint32_t)(s - sa_);
+            const SlotGeometry gp = slot_geometry(c, k - 1), gc = slot_geometry(c, k);
+            if (a.K > 0) {
+                // vertex positions and velocities of step s, then the tangents
+                MeshBuffers mb{};
+                mb.body = c->body;
+                mb.points = c->m_points;
+                mb.times = c->m_times;
+                mb.pos = c->m_pos;
+                mb.vel = c->m_vel;
+                const MeshParams mp = mesh_params(c, time_of(c, s), false);
+                e = launch_mesh_prepare(mb, mp, c->sb);
+                if (e == hipSuccess)
+                    e = launch_wall_tangent(gc.tri, c->m_tri, c->m_pos, c->m_vel, a.dpoints, c->m_times, a.dpos,
+                                            a.dvel, a.dW, mp, (int)a.K, c->sb);
+                if (e != hipSuccess) return hip_fail(c, e, "wall tangents");
+            }
+            AdjointBuffers ab{};
+            ab.f_prev = a.seg_f + (k - 1) * pop;
+            ab.lam_in = a.lam[a.lam_cur];
+            ab.lam_out = a.lam[1 - a.lam_cur];
+            ab.solid_prev = gp.mask;
+            ab.solid_cur = gc.mask;
+            ab.wall = gc.wall;
+            ab.static_flags = c->static_flags;
+            ab.dyn_prev = gp.dyn;
+            ab.dyn_cur = gc.dyn;
+            ab.general_list = a.list;
+            ab.general_count = a.count;
+            ab.nu = a.nu;
+            ab.dW = a.dW;
+            ab.grad_partials = a.grad_partials;
+            ab.grad_total = a.grad_total;
+            ab.body = c->body;
+            AdjointParams w{};
+            const double* lw = load_weights + 6 * (size_t)(s - s0 - 1);
+            for (int i = 0; i < 3; ++i) {
+                w.cF[i] = (float)lw[i];
+                w.cT[i] = (float)lw[3 + i];
+            }
+            w.K = (int)a.K;
+            e = adjoint_list(c, gp, gc);
+            if (e == hipSuccess) e = launch_adjoint_step(coll, ab, pa, w, c->general_blocks, c->sb);
+            if (e != hipSuccess) return hip_fail(c, e, "adjoint step");
+            a.lam_cur = 1 - a.lam_cur;
+        }
+    }
+    e = hipEventRecord(t2, c->sb);
+    double g8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(g8, a.grad_total, 64, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "adjoint result");
+    for (uint32_t k = 0; k < a.K; ++k) gradient[k] = g8[k];
+    if (timings_ms) {
+        float f_ms = 0.0f, r_ms = 0.0f;
+        (void)hipEventElapsedTime(&f_ms, t0, t1);
+        (void)hipEventElapsedTime(&r_ms, t1, t2);
+        timings_ms[0] = f_ms;
+        timings_ms[1] = r_ms;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipEventDestroy(t2);
+    a.have_sensitivity = true;
+    return BF_OK;
+}
+
+bf_status bf_adjoint_read_sensitivity(bf_context* c, float* out, uint64_t count) {
+    BF_ENTER(c);
+    if (!out || count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "count must be 19 * cells");
+    if (!c->adj.have_sensitivity) return fail(c, BF_ERROR_INVALID_ARGUMENT, "no adjoint sweep has run");
+    hipError_t e = hipMemcpy(out, c->adj.lam[c->adj.lam_cur], 4 * count, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read sensitivity");
     return BF_OK;
 }
 

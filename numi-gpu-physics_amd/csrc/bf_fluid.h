@@ -25,6 +25,13 @@
 
 #include "bf_launch.h"
 
+// Operand list of an empty asm that pins 19 values in registers: every load
+// feeding them is issued before any is consumed.
+#define BF_ASM19(a)                                                                                           \
+    "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7]),          \
+        "+v"(a[8]), "+v"(a[9]), "+v"(a[10]), "+v"(a[11]), "+v"(a[12]), "+v"(a[13]), "+v"(a[14]), "+v"(a[15]), \
+        "+v"(a[16]), "+v"(a[17]), "+v"(a[18])
+
 namespace bf {
 
 // ---------------------------------------------------------------------------
@@ -450,6 +457,43 @@ __device__ __forceinline__ int shift_axis(int v, int d, int n, int bc_lo, int bc
     return s;
 }
 
+// Pull sources of the six axis directions with their far flags; the
+// source of a direction combines them per axis.
+struct Sources {
+    int xs_p, xs_m, ys_p, ys_m, zs_p, zs_m;
+    bool xm_far, xp_far, ym_far, yp_far, zm_far, zp_far;
+};
+
+__device__ __forceinline__ Sources sources(const StepParams& p, int x, int y, int z) {
+    Sources s{};
+    s.xm_far = s.xp_far = s.ym_far = s.yp_far = s.zm_far = s.zp_far = false;
+    s.xs_p = shift_axis(x, 1, p.g.nx, p.bc[0][0], p.bc[0][1], s.xm_far);
+    s.xs_m = shift_axis(x, -1, p.g.nx, p.bc[0][0], p.bc[0][1], s.xp_far);
+    s.ys_p = shift_axis(y, 1, p.g.ny, p.bc[1][0], p.bc[1][1], s.ym_far);
+    s.ys_m = shift_axis(y, -1, p.g.ny, p.bc[1][0], p.bc[1][1], s.yp_far);
+    s.zs_p = shift_axis(z, 1, p.g.nz, p.bc[2][0], p.bc[2][1], s.zm_far);
+    s.zs_m = shift_axis(z, -1, p.g.nz, p.bc[2][0], p.bc[2][1], s.zp_far);
+    return s;
+}
+
+template <int q>
+__device__ __forceinline__ uint32_t source_of(const Sources& s, const Grid& g, int x, int y, int z, bool& far) {
+    const int xs = cx(q) > 0 ? s.xs_p : (cx(q) < 0 ? s.xs_m : x);
+    const int ys = cy(q) > 0 ? s.ys_p : (cy(q) < 0 ? s.ys_m : y);
+    const int zs = cz(q) > 0 ? s.zs_p : (cz(q) < 0 ? s.zs_m : z);
+    far = (cx(q) > 0 && s.xm_far) || (cx(q) < 0 && s.xp_far) || (cy(q) > 0 && s.ym_far) || (cy(q) < 0 && s.yp_far) ||
+          (cz(q) > 0 && s.zm_far) || (cz(q) < 0 && s.zp_far);
+    return (uint32_t)xs + (uint32_t)g.nx * ((uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs);
+}
+
+// (x, q) is pulled by nobody when x + c_q leaves the domain through a
+// far-field face
+template <int q>
+__device__ __forceinline__ bool orphan(const Sources& s) {
+    return (cx(q) > 0 && s.xp_far) || (cx(q) < 0 && s.xm_far) || (cy(q) > 0 && s.yp_far) || (cy(q) < 0 && s.ym_far) ||
+           (cz(q) > 0 && s.zp_far) || (cz(q) < 0 && s.zm_far);
+}
+
 template <int S>
 __device__ __forceinline__ void write_post(typename Store<S>::T* __restrict__ fout, size_t n, uint32_t gid,
                                            const float (&post)[19], float sponge, const StepParams& p,
@@ -534,9 +578,7 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
     // select turns each load into a conditional block, and the scheduler
     // starts the moment sums after a few loads and drains the queue), and
     // none sinks below the tile test.
-    asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]), "+v"(f[3]), "+v"(f[4]), "+v"(f[5]), "+v"(f[6]),
-                 "+v"(f[7]), "+v"(f[8]), "+v"(f[9]), "+v"(f[10]), "+v"(f[11]), "+v"(f[12]), "+v"(f[13]),
-                 "+v"(f[14]), "+v"(f[15]), "+v"(f[16]), "+v"(f[17]), "+v"(f[18]));
+    asm volatile("" : BF_ASM19(f));
     if (general_tile) return;
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -699,25 +741,58 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
         const float mwd = p.far_rho;
         const bool interpolated_rule = p.wall_rule != WALL_HALFWAY;
         const float4 wg = wall[gid];
+        // Loads in two rounds.  First, unconditionally and without a branch
+        // between them (a far-field source, or a far side outside the
+        // domain, reads this cell and is ignored): every source's part id and
+        // population, this cell's own populations (reflected and x-side
+        // values of its links) and the part id of each link's far side.
+        // Conditional loads here would cost one memory round trip per
+        // direction.  Then, per link, the source's wall entry and the
+        // far-side population.
+        const Sources sq = sources(p, x, y, z);
+        uint32_t srcq[19], fidq[19], partq[19], fpart[19];
+        float fsrc[19], fown[19];
+        uint32_t farbits = 0, okbits = 0;
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            bool far;
+            const uint32_t src = source_of<q>(sq, g, x, y, z, far);
+            farbits |= (uint32_t)far << q;
+            srcq[q] = far ? gid : src;
+            const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+            const bool ok = inside(g, fx, fy, fz);
+            okbits |= (uint32_t)ok << q;
+            fidq[q] = ok ? flat(g, fx, fy, fz) : gid;
+            partq[q] = solid_cur[srcq[q]];
+            fpart[q] = solid_cur[fidq[q]];
+            fsrc[q] = Store<S>::load(fin, (size_t)q * n + srcq[q], q);
+            fown[q] = Store<S>::load(fin, (size_t)q * n + gid, q);
+        });
+        asm volatile("" : BF_ASM19(fsrc), BF_ASM19(fown), BF_ASM19(partq), BF_ASM19(fpart));
+        // links; far sides that force the halfway fallback; links whose part
+        // counts toward the loads
+        uint32_t linkbits = 0, fallbits = 0, selbits = 0;
+        const int sel = p.selected_part;
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            linkbits |= (uint32_t)(partq[q] != 0 && !((farbits >> q) & 1u)) << q;
+            fallbits |= (uint32_t)(!((okbits >> q) & 1u) || fpart[q] != 0) << q;
+            selbits |= (uint32_t)(sel == 0 || sel == (int)partq[q]) << q;
+        });
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr float dxq = (float)cx(q), dyq = (float)cy(q), dzq = (float)cz(q);
-            bool far = false;
-            const int sx = shift_axis(x, cx(q), g.nx, p.bc[0][0], p.bc[0][1], far);
-            const int sy = shift_axis(y, cy(q), g.ny, p.bc[1][0], p.bc[1][1], far);
-            const int sz = shift_axis(z, cz(q), g.nz, p.bc[2][0], p.bc[2][1], far);
             float value;
-            if (far) {
+            if ((farbits >> q) & 1u) {
                 value = p.far_eq[q];
             } else {
-                const uint32_t src = flat(g, sx, sy, sz);
-                const uint8_t part = solid_cur[src];
-                if (part != 0) {
-                    const float reflected = Store<S>::load(fin, (size_t)opp(q) * n + gid, opp(q));
+                const uint32_t src = srcq[q];
+                if ((linkbits >> q) & 1u) {
+                    const float reflected = fown[opp(q)];
                     float lf = 0.5f;
                     const float4 ws = wall[src];
+                    const float fo_far = Store<S>::load(fin, (size_t)opp(q) * n + fidq[q], opp(q));
                     float wvx = ws.x, wvy = ws.y, wvz = ws.z;
-                    uint32_t farther = 0;
                     bool interp = interpolated_rule;
                     if (interp) {
                         if (p.wall_rule == WALL_INTERPOLATED_SDF) {
@@ -731,10 +806,7 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                         wvy = wg.y + (ws.y - wg.y) * lf;
                         wvz = wg.z + (ws.z - wg.z) * lf;
                         if (lf <= 0.5f) {
-                            const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
-                            const bool ok = inside(g, fx, fy, fz);
-                            if (ok) farther = flat(g, fx, fy, fz);
-                            if (!ok || solid_cur[farther] != 0) {
+                            if ((fallbits >> q) & 1u) {
                                 interp = false;
                                 lf = 0.5f;
                                 wvx = ws.x;
@@ -748,15 +820,14 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                     value = reflected + corr;
                     if (interp) {
                         if (lf <= 0.5f) {
-                            const float fo = Store<S>::load(fin, (size_t)opp(q) * n + farther, opp(q));
+                            const float fo = fo_far;
                             value = 2.0f * lf * reflected + (1.0f - 2.0f * lf) * fo + corr;
                         } else {
-                            const float pin = Store<S>::load(fin, (size_t)q * n + gid, q);
+                            const float pin = fown[q];
                             value = (reflected + corr) / (2.0f * lf) + (2.0f * lf - 1.0f) * pin / (2.0f * lf);
                         }
                     }
-                    const int sel = p.selected_part;
-                    if (p.accumulate_loads && include_link && (sel == 0 || sel == (int)part)) {
+                    if (p.accumulate_loads && include_link && ((selbits >> q) & 1u)) {
                         float lx = 0.0f, ly = 0.0f, lz = 0.0f;
                         if (mode == 0) {
                             lx = -(value * (dxq - wvx) - reflected * (-dxq - wvx));
@@ -801,7 +872,7 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                         tqz = tqz + oz;
                     }
                 } else {
-                    value = Store<S>::load(fin, (size_t)q * n + src, q);
+                    value = fsrc[q];
                 }
             }
             f[q] = value;

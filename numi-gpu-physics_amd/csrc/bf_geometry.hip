@@ -15,6 +15,8 @@
 // for meshes of up to 4,096 triangles and without its 4,096-triangle limit.
 #include "bf_launch.h"
 
+#include <algorithm>
+
 namespace bf {
 
 constexpr uint64_t EMPTY_KEY = ~0ull;
@@ -168,8 +170,10 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
                                                   unsigned long long* __restrict__ keys,
                                                   uint8_t* __restrict__ tiles_new, MeshParams m) {
     __shared__ uint32_t cand[RASTER_CAP];
-    const uint32_t t = blockIdx.x;
     const uint32_t lane = threadIdx.x;
+    // grid-stride over triangles: a bounded grid keeps the raster's
+    // footprint small while the fluid kernel streams on the other stream
+    for (uint32_t t = blockIdx.x; t < m.triangle_count; t += gridDim.x) {
     const float4 a4 = pos[tri[3 * (size_t)t + 0]];
     const float4 b4 = pos[tri[3 * (size_t)t + 1]];
     const float4 c4 = pos[tri[3 * (size_t)t + 2]];
@@ -256,6 +260,7 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
             __syncthreads();
             count = 0;
         }
+    }
     }
 }
 
@@ -449,6 +454,12 @@ __global__ void link_census(const uint8_t* __restrict__ prev, const uint8_t* __r
 // Host launchers.
 // ---------------------------------------------------------------------------
 
+hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
+    hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
+                       b.vel, b.body, m);
+    return hipGetLastError();
+}
+
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
     hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
                        b.vel, b.body, m);
@@ -456,10 +467,11 @@ hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipSt
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.dyn, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
     if (m.triangle_count > 0)
-        hipLaunchKernelGGL(mesh_raster, dim3(m.triangle_count), dim3(64), 0, s, b.pos, b.tri, b.keys, b.tiles_new, m);
+        hipLaunchKernelGGL(mesh_raster, dim3(std::min(m.triangle_count, m.raster_blocks)), dim3(64), 0, s, b.pos,
+                           b.tri, b.keys, b.tiles_new, m);
     hipLaunchKernelGGL(geometry_tile_list, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new,
                        b.tiles_mask_old, b.tiles_wall_old, m.g.tiles, b.list, b.list_count);
-    hipLaunchKernelGGL(geometry_tile_pass, dim3(2048), dim3(256), 0, s, b.pos, b.vel, b.tri, b.tri_part, b.keys, b.list,
+    hipLaunchKernelGGL(geometry_tile_pass, dim3(m.tile_pass_blocks), dim3(256), 0, s, b.pos, b.vel, b.tri, b.tri_part, b.keys, b.list,
                        b.list_count, b.tiles_new, b.tiles_mask_old, b.tiles_wall_old, b.mask, b.wall, b.dyn, b.tri_out,
                        m);
     return hipGetLastError();

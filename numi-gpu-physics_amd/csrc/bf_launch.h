@@ -45,6 +45,8 @@ struct MeshParams {
     int periodic[3];
     float cull_cells;          // key only cells at most this far (cells) from a triangle; +inf = whole AABB
     int move_with_body;        // frames are body-frame; place them with the device body pose
+    uint32_t raster_blocks;    // raster grid (grid-stride over triangles)
+    uint32_t tile_pass_blocks; // tile-pass grid (grid-stride over listed tiles)
 };
 
 struct MeshBuffers {
@@ -142,6 +144,7 @@ hipError_t launch_extract_fields(int storage, const void* f, float* rho, float* 
 hipError_t launch_convert_populations(int storage, bool pack, const void* src, void* dst, size_t count,
                                       uint32_t n, hipStream_t s);
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
+hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
 hipError_t launch_sphere_geometry(uint8_t* mask, float4* wall, uint8_t* dyn, const Grid& g, float4 cr, float3 gv,
                                   float4 wv, float t, const int periodic[3], hipStream_t s);
 hipError_t launch_mark_tiles(const uint8_t* mask, uint8_t* dyn, const Grid& g, const int periodic[3], bool clear,

@@ -34,15 +34,29 @@ const double bfo_w[19] = {
 };
 
 #define R float
+#define WT float
 #define FN(name) name##_f32
 #include "bfref_fluid.inc"
 #undef R
+#undef WT
 #undef FN
 
 #define R double
+#define WT float
 #define FN(name) name##_f64
 #include "bfref_fluid.inc"
 #undef R
+#undef WT
+#undef FN
+
+/* FP64 twin with an FP64 wall field: the reference gradient of the adjoint
+ * tests differentiates through wall values, which FP32 would round. */
+#define R double
+#define WT double
+#define FN(name) name##_f64w
+#include "bfref_fluid.inc"
+#undef R
+#undef WT
 #undef FN
 
 /* InterpolatedBounceBackReference.swift:28-42 (FP64, for known-answer tests). */

@@ -104,6 +104,11 @@ void bfo_step_f64(const bfo_params* p, const double* fin, double* fout,
                   const uint8_t* solid_prev, const uint8_t* solid_cur,
                   const float* wall4, double* density, double* velocity4,
                   double* loads);
+/* The same FP64 step with an FP64 wall field (test reference gradients). */
+void bfo_step_f64w(const bfo_params* p, const double* fin, double* fout,
+                   const uint8_t* solid_prev, const uint8_t* solid_cur,
+                   const double* wall4, double* density, double* velocity4,
+                   double* loads);
 
 /* Equilibrium fill: solid cells at their wall velocity, fluid at far_u. */
 void bfo_init_equilibrium_f32(const bfo_params* p, float* f, const uint8_t* solid, const float* wall4);

@@ -89,6 +89,7 @@ def _declare(l):
         getattr(l, f"bfo_step_{sfx}").argtypes = [P, vp, vp, vp, vp, vp, vp, vp, vp]
         getattr(l, f"bfo_init_equilibrium_{sfx}").argtypes = [P, vp, vp, vp]
         getattr(l, f"bfo_moments_{sfx}").argtypes = [P, vp, vp, vp]
+    l.bfo_step_f64w.argtypes = [P, vp, vp, vp, vp, vp, vp, vp, vp]
     l.bfo_init_shear_wave_f32.argtypes = [P, ctypes.c_float, vp]
     l.bfo_init_shear_wave_f64.argtypes = [P, ctypes.c_double, vp]
     l.bfo_equilibrium_f32.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
@@ -164,7 +165,11 @@ def step(p: Params, fin, solid_prev=None, solid_cur=None, wall4=None, capture=Fa
     if wall4 is None:
         wall4 = np.zeros((n, 4), np.float32)
         wall4[:, 3] = 16.0
-    wall4 = np.ascontiguousarray(wall4, np.float32)
+    if sfx == "f64" and np.asarray(wall4).dtype == np.float64:
+        sfx = "f64w"            # FP64 wall field (reference gradients)
+        wall4 = np.ascontiguousarray(wall4, np.float64)
+    else:
+        wall4 = np.ascontiguousarray(wall4, np.float32)
     loads = np.zeros(8, np.float64)
     density = velocity4 = None
     p.capture_fields = int(bool(capture))
@@ -326,3 +331,77 @@ def body_integrate(body: Body, force, torque, mass, inertia, gravity, dt, subste
     g = np.ascontiguousarray(gravity, np.float64)
     lib().bfo_body_integrate(ctypes.byref(body), _ptr(f), _ptr(t), float(mass), _ptr(i), _ptr(g), float(dt), int(substeps))
     return body
+
+
+def wall_field_f64(p: Params, pos, vel, tri, tri_of_cell, half_thickness):
+    """FP64 wall field (velocity xyz, signed distance / dx) of the cells whose
+    nearest triangle is given (tri_of_cell >= 0), by the closest-point
+    regions of BirdFlow.metal:1529-1583 and the resolve of :1697-1738
+    evaluated in FP64; other cells get (0, 0, 0, 16).  With the triangle
+    assignment frozen this is the smooth wall field the adjoint
+    differentiates."""
+    n = cells(p)
+    wall = np.zeros((n, 4), np.float64)
+    wall[:, 3] = 16.0
+    cellsel = np.nonzero(tri_of_cell >= 0)[0]
+    if cellsel.size == 0:
+        return wall
+    t = tri_of_cell[cellsel]
+    x = cellsel % p.nx
+    y = (cellsel // p.nx) % p.ny
+    z = cellsel // (p.nx * p.ny)
+    P = np.stack([p.origin[0] + (x + 0.5) * p.dx, p.origin[1] + (y + 0.5) * p.dx,
+                  p.origin[2] + (z + 0.5) * p.dx], 1).astype(np.float64)
+    tri = np.asarray(tri, np.int64)
+    pos = np.asarray(pos, np.float64)
+    vel = np.asarray(vel, np.float64)
+    A, B, C = pos[tri[t, 0]], pos[tri[t, 1]], pos[tri[t, 2]]
+    dot = lambda u, v: np.einsum("ij,ij->i", u, v)
+    ab, ac, ap = B - A, C - A, P - A
+    d1, d2 = dot(ab, ap), dot(ac, ap)
+    bp = P - B
+    d3, d4 = dot(ab, bp), dot(ac, bp)
+    cp_ = P - C
+    d5, d6 = dot(ab, cp_), dot(ac, cp_)
+    vc = d1 * d4 - d3 * d2
+    vb = d5 * d2 - d1 * d6
+    va = d3 * d6 - d5 * d4
+    m = len(t)
+    bar = np.zeros((m, 3))
+    done = np.zeros(m, bool)
+
+    def take(cond, b0, b1, b2):
+        sel = cond & ~done
+        bar[sel, 0], bar[sel, 1], bar[sel, 2] = b0[sel], b1[sel], b2[sel]
+        done[sel] = True
+
+    one, zero = np.ones(m), np.zeros(m)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        take((d1 <= 0) & (d2 <= 0), one, zero, zero)
+        take((d3 >= 0) & (d4 <= d3), zero, one, zero)
+        v = d1 / (d1 - d3)
+        take((vc <= 0) & (d1 >= 0) & (d3 <= 0), 1 - v, v, zero)
+        take((d6 >= 0) & (d5 <= d6), zero, zero, one)
+        w = d2 / (d2 - d6)
+        take((vb <= 0) & (d2 >= 0) & (d6 <= 0), 1 - w, zero, w)
+        w = (d4 - d3) / ((d4 - d3) + (d5 - d6))
+        take((va <= 0) & ((d4 - d3) >= 0) & ((d5 - d6) >= 0), zero, 1 - w, w)
+        inv = 1.0 / np.maximum(va + vb + vc, 1e-20)
+        take(np.ones(m, bool), 1 - vb * inv - vc * inv, vb * inv, vc * inv)
+    cp = bar[:, :1] * A + bar[:, 1:2] * B + bar[:, 2:] * C
+    u = bar[:, :1] * vel[tri[t, 0]] + bar[:, 1:2] * vel[tri[t, 1]] + bar[:, 2:] * vel[tri[t, 2]]
+    sd = np.linalg.norm(P - cp, axis=1) - half_thickness
+    wall[cellsel, :3] = u
+    wall[cellsel, 3] = sd / p.dx
+    return wall
+
+
+def interpolate_frames_f64(points, times, t, velocity_scale=1.0):
+    """FP64 frame interpolation at time t (the prepare step,
+    BirdFlow.metal:1589-1622): positions and lattice velocities."""
+    first, _, _ = frame_interval(np.asarray(times, np.float32), t)
+    times = np.asarray(times, np.float64)
+    a, b = np.asarray(points[first], np.float64), np.asarray(points[first + 1], np.float64)
+    duration = times[first + 1] - times[first]
+    blend = min(max((t - times[first]) / duration, 0.0), 1.0)
+    return a + blend * (b - a), (b - a) / duration * velocity_scale

@@ -101,11 +101,11 @@ def deetjen_frame_points(d, frame, points_by_frame):
 # and the oracle's indexed-mesh replay loop.
 # ---------------------------------------------------------------------------
 def solver_config(p, storage=0, collision=None, time_step=1.0, velocity_to_lattice=1.0, arithmetic=None):
-    """SolverConfig of oracle parameters p.  FP32 storage defaults to strict
-    arithmetic (bit-comparable with the oracle), reduced storage to fused."""
+    """SolverConfig of oracle parameters p (default arithmetic: strict for
+    FP32 storage, bit-comparable with the oracle; fused for reduced storage)."""
     import birdflow_hip as bh
     if arithmetic is None:
-        arithmetic = bh.ARITHMETIC_STRICT if storage == bh.STORAGE_FP32 else bh.ARITHMETIC_FUSED
+        arithmetic = bh.ARITHMETIC_DEFAULT
     return bh.SolverConfig(arithmetic=arithmetic,
         shape=(p.nx, p.ny, p.nz), omega_plus=float(p.omega_plus), omega_minus=float(p.omega_minus),
         storage=storage, collision=p.collision if collision is None else collision, wall_rule=p.wall_rule,

@@ -38,7 +38,7 @@ def test_config_struct_size_matches_header():
 
 
 @pytest.mark.parametrize("field,value", [("nx", 0), ("omega_plus", 2.5), ("storage", 7), ("force_mode", 9),
-                                         ("arithmetic", 2)])
+                                         ("arithmetic", 3)])
 def test_invalid_configuration_is_rejected_before_device_use(field, value):
     cfg = bh.SolverConfig(shape=(16, 16, 16), omega_plus=1.25, omega_minus=0.75)
     c = cfg.to_c()
