@@ -126,11 +126,22 @@ def run_cpu_baseline(shape, steps):
     return n * steps / elapsed / 1e6, threads, elapsed
 
 
+def plate_theta(seed=7):
+    """Kinematic parameters of the differentiation workload: the flapping
+    plate's heave 8 sin(2 pi t / 800) and pitch 30 deg sin(2 pi t / 800 +
+    pi / 2) as first harmonics, second harmonics of a quarter of that
+    amplitude, each of the 8 values drawn +-20% with the seed."""
+    base = np.array([8.0, 0.0, 2.0, 0.0, np.radians(30.0), np.pi / 2, np.radians(7.5), np.pi / 2])
+    rng = np.random.default_rng(seed)
+    return base * (1.0 + 0.2 * rng.uniform(-1.0, 1.0, 8))
+
+
 def run_adjoint(shape, steps, interval):
     """Adjoint cost on the differentiation workload: the two-harmonic flapping
-    plate (8 kinematic parameters), RR3, interpolated walls, far field in x,
-    periodic y / z, u_x = 0.05; J = time-averaged lift over the last two
-    thirds of the steps.  Returns forward and reverse times per step."""
+    plate (span 48, chord 16, thickness 2 cells at x = 64; 8 kinematic
+    parameters), RR3, interpolated walls, far field in x, periodic y / z,
+    u_x = 0.05; J = lift averaged over the second half of the steps.  Returns
+    forward and reverse times per step."""
     import birdflow_hip as bh
     from birdflow_hip import workloads
     s = shape[0] / 256.0
@@ -140,11 +151,11 @@ def run_adjoint(shape, steps, interval):
         collision=bh.COLLISION_RR3, wall_rule=bh.WALL_INTERPOLATED_SDF, force_mode=6, accumulate_loads=True,
         boundary=((bh.BC_FARFIELD, bh.BC_FARFIELD), (0, 0), (0, 0)), far_velocity=(0.05, 0.0, 0.0),
         body_position=(64.0 * s, 64.0 * s, 64.0 * s))
-    theta = np.array([8.0 * s, 0.0, 2.0 * s, 0.3, 0.5, np.pi / 2, 0.1, 0.2])
+    theta = plate_theta()
     seq, _, tan = workloads.parametric_plate(theta, steps, center=(64.0 * s, 64.0 * s, 64.0 * s), span=48.0 * s,
                                              chord=16.0 * s, thickness=2.0, period=800.0, half_thickness=0.5)
     weights = np.zeros((steps, 6))
-    weights[steps // 3:, 2] = 1.0 / (steps - steps // 3)
+    weights[steps // 2:, 2] = 1.0 / (steps - steps // 2)
     with bh.BirdFlowSimulation(cfg) as sim:
         sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
         sim.adjoint_set_tangents(tan)
@@ -155,8 +166,8 @@ def run_adjoint(shape, steps, interval):
             "ratio_gradient_vs_forward": (t["forward_ms"] + t["reverse_ms"]) / t["forward_ms"],
             "objective": J, "gradient": grad.tolist(), "finite": bool(np.isfinite(grad).all()),
             "config": {"workload": f"parametric flapping plate {shape[0]}x{shape[1]}x{shape[2]}, RR3, "
-                                   "interpolated walls, 8 kinematic parameters, mean lift",
-                       "steps": steps, "checkpoint_interval": interval}}
+                                   "interpolated walls, 8 kinematic parameters (seed 7), lift averaged over "
+                                   "the second half", "steps": steps, "checkpoint_interval": interval}}
 
 
 def measured_traffic(storage, arithmetic, cells):
@@ -182,9 +193,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--skip-fp16", action="store_true",
                     help="skip the FP16-offset storage form (reported beside FP32 by default)")
-    ap.add_argument("--adjoint-steps", type=int, default=240,
+    ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="steps of the adjoint cost measurement on the 256x128x128 plate (0: skip)")
-    ap.add_argument("--adjoint-interval", type=int, default=60, help="checkpoint interval of that sweep")
+    ap.add_argument("--adjoint-interval", type=int, default=100, help="checkpoint interval of that sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",

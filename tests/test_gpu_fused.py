@@ -207,3 +207,25 @@ def test_limiter_activations_of_wall_free_tiles(collision):
         # the counters start from zero again on the next step
         fo2, lo2, *_ = bfref.step(p, fo)
         assert abs(lg2[6] - lo2[6]) <= (0 if arith == STRICT else max(2.0, 0.002 * lo2[6]))
+
+
+@pytest.mark.parametrize("storage,bar", [(bh.STORAGE_FP16_OFFSET, 5e-3), (bh.STORAGE_BF16_OFFSET, 5e-2)])
+def test_reduced_storage_plate_force_history_vs_fp64(storage, bar):
+    # FP16 offsets: force history within 0.5% of an FP64 run on the same
+    # inputs; BF16 offsets carry ~3 significant digits and are held to 5%.
+    steps = 60
+    p = cases.plate_params()
+    seq = workloads.flapping_plate(steps, center=(24.0, 24.0, 24.0), span=20.0, chord=8.0, thickness=2.0,
+                                   heave=2.0, period=200.0, half_thickness=0.5)
+    say(f"case: flapping plate, storage {storage}, {steps} steps, force history vs FP64 oracle")
+    _, loads64, _ = cases.run_mesh_oracle(p, seq, steps, dtype=np.float64)
+    with bh.BirdFlowSimulation(cases.solver_config(p, storage=storage)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        sim.advance(steps)
+        hist = sim.load_history()
+    scale = np.abs(loads64[:, :3]).max()
+    err = np.abs(hist[:, :3] - loads64[:, :3]).max() / scale
+    mean_err = np.abs(hist[:, 2].mean() - loads64[:, 2].mean()) / abs(loads64[:, 2].mean())
+    say(f"  force history max |diff| / max |F| = {err:.3e}; mean lift rel diff {mean_err:.3e}")
+    assert err <= bar
