@@ -253,7 +253,7 @@ def main():
                      "kernel": "fluid_fast", "bytes_per_cell": POP_BYTES[args.storage],
                      "cells_per_launch": r["fast_cells"], "avg_ms": r["fast_kernel_ms"]},
         "step_breakdown_ms": {"geometry": r["geometry_ms"], "fluid_fast": r["fast_kernel_ms"],
-                              "fluid_general_and_reduce": r["general_ms"]},
+                              "fluid_general": r["general_ms"]},
         "step_roofline_frac": r["step_gbs"] / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
     }

@@ -282,6 +282,12 @@ bf_status bf_adjoint_gradient(bf_context* ctx, uint32_t steps, const double* loa
                               double timings_ms[2]);
 /* dJ/df at the state the last sweep started from (19 x cells, FP32). */
 bf_status bf_adjoint_read_sensitivity(bf_context* ctx, float* out, uint64_t count);
+/* Replays recorded part masks: masks[k] (cells bytes) replaces the part mask
+ * of step (current step + k) for k < count, the current step's at once; wall
+ * fields still follow the mesh.  Finite differences of a load functional with
+ * the link set held fixed (the derivative the adjoint computes) re-step with
+ * the masks of the unperturbed run.  count = 0 ends the replay. */
+bf_status bf_mesh_freeze_masks(bf_context* ctx, const uint8_t* masks, uint32_t count);
 
 #ifdef __cplusplus
 }

@@ -43,7 +43,7 @@ EXPORTED_FUNCTIONS = (
     "bf_mesh_upload", "bf_mesh_build", "bf_set_translating_sphere", "bf_step", "bf_synchronize",
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
-    "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity",
+    "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity", "bf_mesh_freeze_masks",
 )
 
 
@@ -160,6 +160,7 @@ def _declare(l):
         "bf_adjoint_set_tangents": (st, [vp, u32, vp]),
         "bf_adjoint_gradient": (st, [vp, u32, vp, u32, ctypes.POINTER(ctypes.c_double), vp, vp]),
         "bf_adjoint_read_sensitivity": (st, [vp, vp, u64]),
+        "bf_mesh_freeze_masks": (st, [vp, vp, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(l, name)

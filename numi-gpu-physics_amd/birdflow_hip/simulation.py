@@ -366,6 +366,15 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_adjoint_read_sensitivity(self._ctx, _ptr(out), out.size))
         return out
 
+    def freeze_masks(self, masks):
+        """Replay recorded part masks ([steps, cells] uint8) from the current
+        step on; None ends the replay."""
+        if masks is None:
+            self._check(self._lib.bf_mesh_freeze_masks(self._ctx, None, 0))
+            return
+        m = np.ascontiguousarray(masks, np.uint8).reshape(-1, self.config.cells)
+        self._check(self._lib.bf_mesh_freeze_masks(self._ctx, _ptr(m), m.shape[0]))
+
     def link_census(self):
         out = np.zeros(21, np.uint64)
         self._check(self._lib.bf_link_census(self._ctx, _ptr(out)))

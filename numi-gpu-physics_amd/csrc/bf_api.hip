@@ -174,9 +174,15 @@ struct bf_context {
     std::vector<hipEvent_t> events;
     uint32_t profiled = 0;
     int general_blocks = 1024;
+    // replayed part masks (finite-difference checks with the link set fixed)
+    DeviceAlloc frozen_alloc;
+    uint8_t* frozen = nullptr;
+    uint32_t frozen_count = 0;
+    int64_t frozen_first = 0;
     // geometry grids: bounded so that the geometry of step s+1 leaves the
     // fluid kernel of step s most of the machine
-    uint32_t raster_blocks = 1024;
+    uint32_t raster_blocks = 1u << 30;
+    uint32_t raster_batch = 64;
     uint32_t tile_pass_blocks = 512;
 };
 
@@ -301,6 +307,7 @@ MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     m.cull_cells = exact ? INFINITY : c->m_half / c->cfg.cell_size + 1.5f;
     m.move_with_body = c->move_mesh ? 1 : 0;
     m.raster_blocks = c->raster_blocks;
+    m.raster_batch = c->raster_batch;
     m.tile_pass_blocks = c->tile_pass_blocks;
     // frame interval, MeasuredBirdSurfaceSequence.swift:174-204
     const std::vector<float>& ts = c->host_times;
@@ -328,6 +335,19 @@ float time_of(const bf_context* c, int64_t s) {
 // Enqueue on stream B the geometry of step s (dynamic sources).  fresh: the
 // mask and wall buffers of step s hold no entries from earlier steps (they
 // were cleared), so no stale tiles need clearing.
+// Replaces the part mask in slot `slot` (step s) by replayed mask k and
+// rebuilds its solid-proximity flags; marks every tile as written so that
+// later incremental steps clear the whole slot.
+hipError_t replay_mask(bf_context* c, int64_t s, uint32_t k, int slot) {
+    int periodic[3];
+    for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
+    hipError_t e = hipMemcpyAsync(c->mask[slot], c->frozen + (uint64_t)k * c->g.n, c->g.n, hipMemcpyDeviceToDevice,
+                                  c->sb);
+    if (e == hipSuccess) e = launch_mark_tiles(c->mask[slot], c->dyn[slot], c->g, periodic, true, c->sb);
+    if (e == hipSuccess) e = launch_fill_u8(c->tflags[mod4(s)], 1, c->g.tiles, c->sb);
+    return e;
+}
+
 bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
     hipError_t e = hipSuccess;
     if (c->source == BF_GEOMETRY_MESH) {
@@ -349,6 +369,8 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         b.wall = c->wall[par2(s)];
         b.dyn = c->dyn[mod3(s)];
         e = launch_mesh_geometry(b, mesh_params(c, time_of(c, s), exact), c->sb);
+        if (e == hipSuccess && c->frozen_count && s >= c->frozen_first && s < c->frozen_first + c->frozen_count)
+            e = replay_mask(c, s, (uint32_t)(s - c->frozen_first), mod3(s));
     } else if (c->source == BF_GEOMETRY_SPHERE) {
         int periodic[3];
         for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
@@ -383,7 +405,7 @@ FluidBuffers fluid_buffers(const bf_context* c, int64_t s) {
     b.static_flags = c->static_flags;
     b.general_list = c->general_list[par2(s)];
     b.general_count = &c->counters[par2(s)];
-    b.partials = c->partials;
+    b.partials = c->partials + 8ull * c->g.tiles * (uint64_t)par2(s);
     b.block_out = c->block_out;
     b.last_load = c->last_load;
     b.history = c->history;
@@ -522,17 +544,21 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     g.tiles = (uint32_t)((uint64_t)g.tiles_x * g.tiles_y * g.nz);
     c->pop_elem = cfg->storage == BF_STORAGE_FP32 ? 4 : 2;
     c->general_blocks = std::max(1, prop.multiProcessorCount * 4);
-    c->raster_blocks = (uint32_t)std::max(1, prop.multiProcessorCount * 4);
-    c->tile_pass_blocks = (uint32_t)std::max(1, prop.multiProcessorCount * 2);
+    // one tile-pass workgroup per CU: the fastest of a sweep (CU x 1/2/4/8)
+    // on the 512x256x256 bird, where the passes overlap the fluid kernel;
+    // the raster launches every triangle batch at once
+    c->tile_pass_blocks = (uint32_t)std::max(1, prop.multiProcessorCount);
     // tuning overrides (workgroup counts)
     if (const char* v = std::getenv("BIRDFLOW_RASTER_BLOCKS")) c->raster_blocks = (uint32_t)std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("BIRDFLOW_RASTER_BATCH"))
+        c->raster_batch = (uint32_t)std::min(64, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("BIRDFLOW_TILE_PASS_BLOCKS"))
         c->tile_pass_blocks = (uint32_t)std::max(1, std::atoi(v));
 
     // Working-set plan (MetalBackend.validateAllocationPlan, :77-106).
     const uint64_t n = g.n;
     const uint64_t pop_bytes = 19ull * n * c->pop_elem;
-    const uint64_t plan = 2 * pop_bytes + 3 * n + 32 * n + 3ull * g.tiles + 4ull * g.tiles + 64ull * g.tiles +
+    const uint64_t plan = 2 * pop_bytes + 3 * n + 32 * n + 3ull * g.tiles + 4ull * g.tiles + 128ull * g.tiles +
                           64ull * HISTORY_CAPACITY + 128ull * HISTORY_CAPACITY + 4 * n + 16 * n;
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
@@ -566,7 +592,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     for (int i = 0; i < 2; ++i) c->general_list[i] = (uint32_t*)A.take(e, 4ull * g.tiles);
     c->geo_list = (uint32_t*)A.take(e, 4ull * g.tiles);
     c->counters = (uint32_t*)A.take(e, 64);
-    c->partials = (double*)A.take(e, 64ull * g.tiles);
+    c->partials = (double*)A.take(e, 128ull * g.tiles);   // two steps' per-tile load partials
     c->block_out = (double*)A.take(e, 64ull * ((g.tiles + 1023) / 1024 + 1));
     c->last_load = (double*)A.take(e, 64);
     c->history = (double*)A.take(e, 64ull * HISTORY_CAPACITY);
@@ -591,7 +617,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     if (e == hipSuccess) e = launch_fill_u8(c->no_tiles, 0, g.tiles, s);
     if (e == hipSuccess) e = launch_fill_u32(c->counters, 0u, 16u, s);
     if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->last_load, 0, 64, s);
-    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->partials, 0, (uint32_t)(64ull * g.tiles), s);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->partials, 0, (uint32_t)(128ull * g.tiles), s);
     // body at rest at the torque reference point, identity orientation
     const double body0[16] = {cfg->body_position[0], cfg->body_position[1], cfg->body_position[2], 0.0, 0.0, 0.0,
                               1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -622,6 +648,7 @@ void bf_destroy(bf_context* c) {
     c->mesh_alloc.release();
     c->adj.alloc.release();
     c->adj.tangent_alloc.release();
+    c->frozen_alloc.release();
     c->alloc.release();
     if (c->sa) (void)hipStreamDestroy(c->sa);
     if (c->sb) (void)hipStreamDestroy(c->sb);
@@ -879,7 +906,7 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
             if (bf_status r = enqueue_geometry(c, s, false, false)) return r;
         }
         if (bf_status r = enqueue_tile_list(c, s)) return r;
-        e = hipEventRecord(c->ev_ready[par2(s - 1)], c->sb);
+        e = hipEventRecord(c->ev_ready[par2(s)], c->sb);
         if (e != hipSuccess) return hip_fail(c, e, "event record");
         c->geometry_ready = s;
     }
@@ -898,17 +925,26 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         const StepParams p = make_step_params(c);
         const FluidBuffers fb = fluid_buffers(c, s);
         if (c->source == BF_GEOMETRY_STATIC) c->static_prev_differs = false;
-        // stream A: wall-free tiles of step s once general(s-1) (and with it
-        // geometry(s)) is done
-        e = hipStreamWaitEvent(c->sa, c->ev_ready[par2(s - 1)], 0);
+        // stream A: the wall tiles of step s, then the wall-free tiles, once
+        // the tile list of step s (event L[s], stream B) is built.  Both read
+        // f_in, which the previous step finished writing on this stream.
+        e = hipStreamWaitEvent(c->sa, c->ev_ready[par2(s)], 0);
+        if (prof && e == hipSuccess) e = hipEventRecord(pe[4], c->sa);
+        if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, p, c->general_blocks, c->sa);
+        if (prof && e == hipSuccess) e = hipEventRecord(pe[5], c->sa);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[0], c->sa);
         if (e == hipSuccess) e = launch_fluid_fast(arith, storage, coll, fb, p, c->sa);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[1], c->sa);
         if (e == hipSuccess) e = hipEventRecord(c->ev_fast[par2(s)], c->sa);
         if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
-        // stream B: geometry and tile list of step s+1 (while fast(s) runs,
-        // unless the mesh rides on a free-flying body: then they need the
-        // pose integrated from the loads of step s and follow it)
+        // stream B: geometry and tile list of step s+1 while step s streams
+        // (they write only buffers that step s-1 last read; this stream
+        // waited for step s-1 before its load reduction), then event L[s+1];
+        // then the load reduction of step s (partials double-buffered by
+        // step parity) and the body update.  general(s+1) reads the torque
+        // reference point from the body state, so in free flight L[s+1]
+        // follows the body update; a mesh riding on a free-flying body also
+        // needs the pose after step s, so its geometry follows it too.
         const bool coupled = c->free_flight && c->move_mesh && c->source == BF_GEOMETRY_MESH;
         auto next_geometry = [&]() -> bf_status {
             if (prof) e = hipEventRecord(pe[2], c->sb);
@@ -922,15 +958,8 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         if (!coupled) {
             if (bf_status r = next_geometry()) return r;
         }
-        // ... then the wall tiles of step s once fast(s) is done
+        if (!c->free_flight) e = hipEventRecord(c->ev_ready[par2(s + 1)], c->sb);
         if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, c->ev_fast[par2(s)], 0);
-        if (prof && e == hipSuccess) e = hipEventRecord(pe[4], c->sb);
-        if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, p, c->general_blocks, c->sb);
-        if (prof && e == hipSuccess) e = hipEventRecord(pe[5], c->sb);
-        // fast(s+1) needs only the populations; the load reduction and body
-        // update of step s overlap it (they read partials and flags that
-        // step s+1 rewrites only after them, later on this stream)
-        if (e == hipSuccess && !coupled) e = hipEventRecord(c->ev_ready[par2(s)], c->sb);
         if (e == hipSuccess) e = launch_load_reduction(fb, p, (uint32_t)(s % HISTORY_CAPACITY), c->sb);
         if (e == hipSuccess && c->free_flight)
             e = launch_body_step(c->body, c->last_load, c->body_history, (uint32_t)(s % HISTORY_CAPACITY),
@@ -938,7 +967,9 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
         if (coupled) {
             if (bf_status r = next_geometry()) return r;
-            e = hipEventRecord(c->ev_ready[par2(s)], c->sb);
+        }
+        if (c->free_flight) {
+            e = hipEventRecord(c->ev_ready[par2(s + 1)], c->sb);
             if (e != hipSuccess) return hip_fail(c, e, "event record");
         }
         c->step = s;
@@ -1382,6 +1413,30 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
     return BF_OK;
 }
 
+bf_status bf_mesh_freeze_masks(bf_context* c, const uint8_t* masks, uint32_t count) {
+    BF_ENTER(c);
+    c->frozen_alloc.release();
+    c->frozen = nullptr;
+    c->frozen_count = 0;
+    if (count == 0) return BF_OK;
+    if (!masks) return fail(c, BF_ERROR_INVALID_ARGUMENT, "masks are NULL");
+    if (c->source != BF_GEOMETRY_MESH) return fail(c, BF_ERROR_INVALID_ARGUMENT, "mask replay needs a mesh");
+    hipError_t e = hipSuccess;
+    c->frozen = (uint8_t*)c->frozen_alloc.take(e, (uint64_t)count * c->g.n);
+    if (e == hipSuccess) e = hipMemcpy(c->frozen, masks, (uint64_t)count * c->g.n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(c, e, "mask replay upload", false);
+    c->frozen_count = count;
+    c->frozen_first = c->step;
+    // the current step's geometry (and its copy as the previous step's)
+    e = replay_mask(c, c->step, 0, mod3(c->step));
+    if (e == hipSuccess) e = replay_mask(c, c->step - 1, 0, mod3(c->step - 1));
+    if (e == hipSuccess && c->geometry_ready > c->step && count > 1)
+        e = replay_mask(c, c->step + 1, 1, mod3(c->step + 1));
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "mask replay");
+    return BF_OK;
+}
+
 bf_status bf_adjoint_read_sensitivity(bf_context* c, float* out, uint64_t count) {
     BF_ENTER(c);
     if (!out || count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "count must be 19 * cells");
@@ -1450,7 +1505,7 @@ bf_status bf_read_profile(bf_context* c, double out[6]) {
         const hipEvent_t* ev = &c->events[(size_t)i * PROFILE_EVENTS];
         (void)hipEventElapsedTime(&a, ev[0], ev[1]);   // fast kernel (stream A)
         (void)hipEventElapsedTime(&d, ev[2], ev[3]);   // next step's geometry + tile list (stream B)
-        (void)hipEventElapsedTime(&b, ev[4], ev[5]);   // general + reduction (stream B)
+        (void)hipEventElapsedTime(&b, ev[4], ev[5]);   // general kernel (stream A, before fast)
         fast += a;
         general += b;
         geo += d;

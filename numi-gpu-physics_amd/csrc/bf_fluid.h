@@ -750,22 +750,32 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
         // direction.  Then, per link, the source's wall entry and the
         // far-side population.
         const Sources sq = sources(p, x, y, z);
-        uint32_t srcq[19], fidq[19], partq[19], fpart[19];
+        // source and far-side cell of direction q (far-field / outside: this
+        // cell); recomputed where needed rather than kept in registers
+        auto src_of = [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            bool far;
+            const uint32_t src = source_of<q>(sq, g, x, y, z, far);
+            return far ? gid : src;
+        };
+        auto far_side_of = [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+            return inside(g, fx, fy, fz) ? flat(g, fx, fy, fz) : gid;
+        };
+        uint32_t partq[19], fpart[19];
         float fsrc[19], fown[19];
         uint32_t farbits = 0, okbits = 0;
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
             bool far;
-            const uint32_t src = source_of<q>(sq, g, x, y, z, far);
+            (void)source_of<q>(sq, g, x, y, z, far);
             farbits |= (uint32_t)far << q;
-            srcq[q] = far ? gid : src;
-            const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
-            const bool ok = inside(g, fx, fy, fz);
-            okbits |= (uint32_t)ok << q;
-            fidq[q] = ok ? flat(g, fx, fy, fz) : gid;
-            partq[q] = solid_cur[srcq[q]];
-            fpart[q] = solid_cur[fidq[q]];
-            fsrc[q] = Store<S>::load(fin, (size_t)q * n + srcq[q], q);
+            okbits |= (uint32_t)inside(g, x + cx(q), y + cy(q), z + cz(q)) << q;
+            const uint32_t src = src_of(qc), fid = far_side_of(qc);
+            partq[q] = solid_cur[src];
+            fpart[q] = solid_cur[fid];
+            fsrc[q] = Store<S>::load(fin, (size_t)q * n + src, q);
             fown[q] = Store<S>::load(fin, (size_t)q * n + gid, q);
         });
         asm volatile("" : BF_ASM19(fsrc), BF_ASM19(fown), BF_ASM19(partq), BF_ASM19(fpart));
@@ -786,12 +796,12 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
             if ((farbits >> q) & 1u) {
                 value = p.far_eq[q];
             } else {
-                const uint32_t src = srcq[q];
                 if ((linkbits >> q) & 1u) {
+                    const uint32_t src = src_of(qc);
                     const float reflected = fown[opp(q)];
                     float lf = 0.5f;
                     const float4 ws = wall[src];
-                    const float fo_far = Store<S>::load(fin, (size_t)opp(q) * n + fidq[q], opp(q));
+                    const float fo_far = Store<S>::load(fin, (size_t)opp(q) * n + far_side_of(qc), opp(q));
                     float wvx = ws.x, wvy = ws.y, wvz = ws.z;
                     bool interp = interpolated_rule;
                     if (interp) {

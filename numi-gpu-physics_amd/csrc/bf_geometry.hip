@@ -147,120 +147,162 @@ __global__ void fill_wall(float4* __restrict__ p, uint32_t n) {
     if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 16.0f);
 }
 
-// One single-wavefront workgroup per triangle; its 64 lanes walk the
-// expanded voxel AABB (BirdFlow.metal:1650-1665 bounds) in three phases:
-//  1. a conservative filter: a cell can be within the candidate band
-//     (m.cull_cells) only if its distance to the triangle's plane and to its
+// Raster to nearest-triangle keys (BirdFlow.metal:1638-1686 bounds and key).
+// One single-wavefront workgroup per batch of up to 64 triangles
+// (m.raster_batch; grid-stride over batches):
+//  0. lane j loads triangle (batch + j) and sets up its expanded voxel AABB,
+//     plane and bounding sphere, so one gather round trip serves the batch;
+//     the wave then walks the batch one triangle at a time with the
+//     triangle's values broadcast from lane j (readlane, scalar registers);
+//  1. a conservative filter over the voxels of the AABB (cut to the bounding
+//     sphere's box when the band is finite): lanes take the (x, y) columns of
+//     the box and step through z; a cell can lie in the candidate band
+//     (m.cull_cells) only if its distance to the triangle's plane and to the
 //     bounding sphere are within the band (both are lower bounds of the
 //     distance to the triangle; a margin of 1e-3 cells absorbs rounding);
 //  2. the exact closest point and key (the reference's arithmetic) for the
 //     survivors, compacted into LDS so that every lane has work;
-//  3. for cells within the band: a plain load of the current key and the
-//     64-bit atomicMin only when the key can still win; the cell's tile is
-//     marked for this step's tile pass.
-// With m.cull_cells = +inf (the reference's full AABB) the filter passes
-// every cell.
+//  3. for cells within the band: a 64-bit atomicMin whose result is unused
+//     (the wave does not wait for it) and a byte store marking the cell's
+//     64x4 tile for this step's tile pass, skipped when the lane below marked
+//     the same tile.  A cell within the band always ends with a key, so
+//     marking on every attempt marks the same tiles as marking on a win.
+// Keys are a minimum over (cell, triangle) pairs: the result does not depend
+// on the order of the atomics.  With m.cull_cells = +inf (the reference's
+// full AABB) the filter passes every cell of the AABB.
 constexpr int RASTER_CAP = 1024;
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+__device__ __forceinline__ float bcast(float v, uint32_t j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)j));
+}
+
+__device__ __forceinline__ int bcast(int v, uint32_t j) { return __builtin_amdgcn_readlane(v, (int)j); }
+
 __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
                                                   unsigned long long* __restrict__ keys,
                                                   uint8_t* __restrict__ tiles_new, MeshParams m) {
-    __shared__ uint32_t cand[RASTER_CAP];
+    __shared__ uint2 cand[RASTER_CAP];
     const uint32_t lane = threadIdx.x;
-    // grid-stride over triangles: a bounded grid keeps the raster's
-    // footprint small while the fluid kernel streams on the other stream
-    for (uint32_t t = blockIdx.x; t < m.triangle_count; t += gridDim.x) {
-    const float4 a4 = pos[tri[3 * (size_t)t + 0]];
-    const float4 b4 = pos[tri[3 * (size_t)t + 1]];
-    const float4 c4 = pos[tri[3 * (size_t)t + 2]];
-    const float3 a = make_float3(a4.x, a4.y, a4.z), b = make_float3(b4.x, b4.y, b4.z), c = make_float3(c4.x, c4.y, c4.z);
     const float dx = m.dx;
     const float guard = m.half_thickness + 2.0f * dx;
     const int n[3] = {m.g.nx, m.g.ny, m.g.nz};
-    const float av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z}, cv[3] = {c.x, c.y, c.z};
-    int lo[3], hi[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const float mn = fminf(av[i], fminf(bv[i], cv[i]));
-        const float mx = fmaxf(av[i], fmaxf(bv[i], cv[i]));
-        const float ls = ((mn - guard) - m.origin[i]) / dx - 0.5f;
-        const float us = ((mx + guard) - m.origin[i]) / dx - 0.5f;
-        lo[i] = min(max((int)floorf(ls), 0), n[i] - 1);
-        hi[i] = min(max((int)ceilf(us), 0), n[i] - 1);
-    }
-    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
-    const uint32_t total = (uint32_t)ex * (uint32_t)ey * (uint32_t)ez;
-    // filter data (world units)
     const bool filter = isfinite(m.cull_cells);
     const float reach = (filter ? m.cull_cells : 0.0f) * dx + 1.0e-3f * dx;
-    float nx_ = (b.y - a.y) * (c.z - a.z) - (b.z - a.z) * (c.y - a.y);
-    float ny_ = (b.z - a.z) * (c.x - a.x) - (b.x - a.x) * (c.z - a.z);
-    float nz_ = (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
-    const float nl = sqrtf(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
-    if (nl > 0.0f) {
-        nx_ /= nl;
-        ny_ /= nl;
-        nz_ /= nl;
-    }
-    const float cxm = (a.x + b.x + c.x) / 3.0f, cym = (a.y + b.y + c.y) / 3.0f, czm = (a.z + b.z + c.z) / 3.0f;
-    auto d2 = [](float x, float y, float z) { return x * x + y * y + z * z; };
-    const float r = sqrtf(fmaxf(d2(a.x - cxm, a.y - cym, a.z - czm),
-                                fmaxf(d2(b.x - cxm, b.y - cym, b.z - czm), d2(c.x - cxm, c.y - cym, c.z - czm))));
-    const float sphere2 = (reach + r * 1.0001f) * (reach + r * 1.0001f);
-
-    uint32_t count = 0;  // wave-uniform
-    for (uint32_t base = 0; base < total; base += 64) {
-        const uint32_t k = base + lane;
-        bool pass = false;
-        if (k < total) {
-            pass = true;
-            if (filter) {
-                const int x = lo[0] + (int)(k % (uint32_t)ex);
-                const uint32_t rr = k / (uint32_t)ex;
-                const int y = lo[1] + (int)(rr % (uint32_t)ey);
-                const int z = lo[2] + (int)(rr / (uint32_t)ey);
-                const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
-                const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
-                const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
-                const float plane = fabsf(nx_ * (wx - a.x) + ny_ * (wy - a.y) + nz_ * (wz - a.z));
-                pass = (nl == 0.0f || plane <= reach) && d2(wx - cxm, wy - cym, wz - czm) <= sphere2;
-            }
+    const uint32_t bsz = m.raster_batch;  // triangles per batch, 1..64
+    for (uint32_t batch = blockIdx.x * bsz; batch < m.triangle_count; batch += gridDim.x * bsz) {
+        // 0. per-lane set-up of triangle batch + lane
+        const uint32_t tl = batch + lane;
+        const bool mine = lane < bsz && tl < m.triangle_count;
+        float3 a = make_float3(0.0f, 0.0f, 0.0f), b = a, c = a;
+        if (mine) {
+            const float4 a4 = pos[tri[3 * (size_t)tl + 0]];
+            const float4 b4 = pos[tri[3 * (size_t)tl + 1]];
+            const float4 c4 = pos[tri[3 * (size_t)tl + 2]];
+            a = make_float3(a4.x, a4.y, a4.z);
+            b = make_float3(b4.x, b4.y, b4.z);
+            c = make_float3(c4.x, c4.y, c4.z);
         }
-        const unsigned long long ballot = __ballot(pass);
-        if (pass) cand[count + lane_rank(ballot)] = k;
-        count += (uint32_t)__popcll(ballot);
-        if (count > RASTER_CAP - 64 || base + 64 >= total) {
-            __syncthreads();
-            for (uint32_t j = lane; j < count; j += 64) {
-                const uint32_t kk = cand[j];
-                const int x = lo[0] + (int)(kk % (uint32_t)ex);
-                const uint32_t rr = kk / (uint32_t)ex;
-                const int y = lo[1] + (int)(rr % (uint32_t)ey);
-                const int z = lo[2] + (int)(rr / (uint32_t)ey);
+        float nx_ = (b.y - a.y) * (c.z - a.z) - (b.z - a.z) * (c.y - a.y);
+        float ny_ = (b.z - a.z) * (c.x - a.x) - (b.x - a.x) * (c.z - a.z);
+        float nz_ = (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+        const float nl = sqrtf(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+        if (nl > 0.0f) {
+            nx_ /= nl;
+            ny_ /= nl;
+            nz_ /= nl;
+        }
+        const float cm[3] = {(a.x + b.x + c.x) / 3.0f, (a.y + b.y + c.y) / 3.0f, (a.z + b.z + c.z) / 3.0f};
+        auto d2 = [](float x, float y, float z) { return x * x + y * y + z * z; };
+        const float r = sqrtf(fmaxf(d2(a.x - cm[0], a.y - cm[1], a.z - cm[2]),
+                                    fmaxf(d2(b.x - cm[0], b.y - cm[1], b.z - cm[2]),
+                                          d2(c.x - cm[0], c.y - cm[1], c.z - cm[2]))));
+        const float R = reach + r * 1.0001f;
+        const float sphere2 = R * R;
+        const float av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z}, cv[3] = {c.x, c.y, c.z};
+        int lo[3], ext[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float mn = fminf(av[i], fminf(bv[i], cv[i]));
+            const float mx = fmaxf(av[i], fmaxf(bv[i], cv[i]));
+            const float ls = ((mn - guard) - m.origin[i]) / dx - 0.5f;
+            const float us = ((mx + guard) - m.origin[i]) / dx - 0.5f;
+            int l = min(max((int)floorf(ls), 0), n[i] - 1);
+            int h = min(max((int)ceilf(us), 0), n[i] - 1);
+            if (filter) {
+                // cells of the bounding sphere, one cell of margin for rounding
+                const float top = (float)n[i] + 2.0f;
+                const float sl = fminf(fmaxf(((cm[i] - R) - m.origin[i]) / dx - 0.5f, -2.0f), top);
+                const float su = fminf(fmaxf(((cm[i] + R) - m.origin[i]) / dx - 0.5f, -2.0f), top);
+                l = max(l, (int)floorf(sl) - 1);
+                h = min(h, (int)ceilf(su) + 1);
+            }
+            lo[i] = l;
+            ext[i] = mine ? max(h - l + 1, 0) : 0;
+        }
+        const uint32_t nb = min(bsz, m.triangle_count - batch);
+        for (uint32_t j = 0; j < nb; ++j) {
+            const uint32_t t = batch + j;
+            const float3 ta = make_float3(bcast(a.x, j), bcast(a.y, j), bcast(a.z, j));
+            const float3 tb = make_float3(bcast(b.x, j), bcast(b.y, j), bcast(b.z, j));
+            const float3 tc = make_float3(bcast(c.x, j), bcast(c.y, j), bcast(c.z, j));
+            const float tnx = bcast(nx_, j), tny = bcast(ny_, j), tnz = bcast(nz_, j), tnl = bcast(nl, j);
+            const float tcx = bcast(cm[0], j), tcy = bcast(cm[1], j), tcz = bcast(cm[2], j);
+            const float ts2 = bcast(sphere2, j);
+            const int lx = bcast(lo[0], j), ly = bcast(lo[1], j), lz = bcast(lo[2], j);
+            const int ex = bcast(ext[0], j), ey = bcast(ext[1], j), ez = bcast(ext[2], j);
+            const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
+            uint32_t count = 0;  // wave-uniform
+            for (uint32_t c0 = 0; c0 < cols; c0 += 64) {
+                const uint32_t col = c0 + lane;
+                const int y = ly + (int)(col / (uint32_t)max(ex, 1));
+                const int x = lx + (int)(col - (uint32_t)(y - ly) * (uint32_t)ex);
                 const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
                 const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
-                const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
-                const Closest cp = closest_point(wx, wy, wz, a, b, c);
-                const float ddx = wx - cp.px, ddy = wy - cp.py, ddz = wz - cp.pz;
-                const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
-                if (!(dist_cells <= m.cull_cells)) continue;
-                const float scaled = roundf(dist_cells * 65536.0f);
-                const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
-                const unsigned long long key = ((unsigned long long)bin << 32) | (unsigned long long)t;
-                const uint32_t g = (uint32_t)x + (uint32_t)m.g.nx * ((uint32_t)y + (uint32_t)m.g.ny * (uint32_t)z);
-                if (key < __hip_atomic_load(&keys[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    (void)atomicMin(&keys[g], key);
-                    tiles_new[tile_of(m.g, x, y, z)] = 1;
+                for (int zi = 0; zi < ez; ++zi) {
+                    const int z = lz + zi;
+                    bool pass = col < cols;
+                    if (pass && filter) {
+                        const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
+                        const float plane = fabsf(tnx * (wx - ta.x) + tny * (wy - ta.y) + tnz * (wz - ta.z));
+                        pass = (tnl == 0.0f || plane <= reach) && d2(wx - tcx, wy - tcy, wz - tcz) <= ts2;
+                    }
+                    const unsigned long long ballot = __ballot(pass);
+                    if (pass) cand[count + lane_rank(ballot)] = make_uint2((uint32_t)x | ((uint32_t)y << 16), (uint32_t)z);
+                    count += (uint32_t)__popcll(ballot);
+                    const bool last = zi + 1 == ez && c0 + 64 >= cols;
+                    if (count > RASTER_CAP - 64 || (last && count > 0)) {
+                        __syncthreads();
+                        for (uint32_t k = lane; k < count; k += 64) {
+                            const uint2 e = cand[k];
+                            const int cx = (int)(e.x & 0xFFFFu), cy = (int)(e.x >> 16), cz = (int)e.y;
+                            const float px = m.origin[0] + ((float)cx + 0.5f) * dx;
+                            const float py = m.origin[1] + ((float)cy + 0.5f) * dx;
+                            const float pz = m.origin[2] + ((float)cz + 0.5f) * dx;
+                            const Closest cp = closest_point(px, py, pz, ta, tb, tc);
+                            const float ddx = px - cp.px, ddy = py - cp.py, ddz = pz - cp.pz;
+                            const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
+                            const bool in_band = dist_cells <= m.cull_cells;
+                            const uint32_t tile = in_band ? tile_of(m.g, cx, cy, cz) : 0xFFFFFFFFu;
+                            const uint32_t below = (uint32_t)__shfl_up((int)tile, 1);
+                            if (in_band) {
+                                const float scaled = roundf(dist_cells * 65536.0f);
+                                const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
+                                const unsigned long long key = ((unsigned long long)bin << 32) | (unsigned long long)t;
+                                const uint32_t g = (uint32_t)cx + (uint32_t)m.g.nx * ((uint32_t)cy + (uint32_t)m.g.ny * (uint32_t)cz);
+                                (void)__hip_atomic_fetch_min(&keys[g], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                                if (lane == 0 || below != tile) tiles_new[tile] = 1;
+                            }
+                        }
+                        __syncthreads();
+                        count = 0;
+                    }
                 }
             }
-            __syncthreads();
-            count = 0;
         }
-    }
     }
 }
 
@@ -461,13 +503,15 @@ hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStr
 }
 
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
+    if (m.g.nx > 65535 || m.g.ny > 65535) return hipErrorInvalidValue;   // raster packs x | y << 16
+    if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
                        b.vel, b.body, m);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.dyn, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
     if (m.triangle_count > 0)
-        hipLaunchKernelGGL(mesh_raster, dim3(std::min(m.triangle_count, m.raster_blocks)), dim3(64), 0, s, b.pos,
+        hipLaunchKernelGGL(mesh_raster, dim3(std::min((m.triangle_count + m.raster_batch - 1) / m.raster_batch, m.raster_blocks)), dim3(64), 0, s, b.pos,
                            b.tri, b.keys, b.tiles_new, m);
     hipLaunchKernelGGL(geometry_tile_list, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new,
                        b.tiles_mask_old, b.tiles_wall_old, m.g.tiles, b.list, b.list_count);

@@ -45,7 +45,8 @@ struct MeshParams {
     int periodic[3];
     float cull_cells;          // key only cells at most this far (cells) from a triangle; +inf = whole AABB
     int move_with_body;        // frames are body-frame; place them with the device body pose
-    uint32_t raster_blocks;    // raster grid (grid-stride over triangles)
+    uint32_t raster_blocks;    // raster grid cap (grid-stride over triangle batches)
+    uint32_t raster_batch;     // triangles per raster wave batch, 1..64
     uint32_t tile_pass_blocks; // tile-pass grid (grid-stride over listed tiles)
 };
 

@@ -8,8 +8,10 @@
   differences of the FP64 step with FP64 wall fields on the same frozen link
   sets (masks and nearest triangles of each step), the quantity the adjoint
   differentiates.  Bar: 1e-3 relative.
-* the same gradient against the solver's own FP32 central differences
-  (h = 1e-3), which also re-raster the masks.
+* the same gradient against the solver's own FP32 central differences at
+  two step sizes with the unperturbed run's masks replayed
+  (bf_mesh_freeze_masks); differences with re-rastered masks are printed
+  beside.
 * the torch.autograd wrapper returns the adjoint gradient.
 """
 import sys
@@ -142,37 +144,66 @@ def test_adjoint_gradient_of_plate_lift_matches_fp64_reference():
 
 
 def test_adjoint_gradient_against_solver_finite_differences():
-    # the solver's own FP32 central differences re-raster the masks, so they
-    # include the (rare, small) link-set changes that the adjoint holds fixed
+    # The solver's own FP32 central differences from the same start state,
+    # with the masks of the unperturbed run replayed (the adjoint holds each
+    # step's link set fixed), at two step sizes h and h/4.  With fixed masks
+    # the objective is still only piecewise smooth (interpolation fractions of
+    # the replayed links clamp as the wall moves), and FP32 rounding limits
+    # the small step: the two differences bracket the derivative to about
+    # 1 %.  Bar: the adjoint lies within the interval the two differences span,
+    # widened by half its width plus 1e-3 of the largest component, and within
+    # 1e-2 relative L2 of the h/4 differences.  Differences with re-rastered
+    # masks are printed beside (cover / uncover events dominate them).
     steps = 30
     p, weights = plate_case(steps)
     cfg = cases.solver_config(p)
+    base_masks, f0 = [], None
+    seq, _, tan = plate_frames(THETA0, steps)
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        f0 = sim.read_populations()
+        base_masks.append(sim.read_geometry()[0])
+        for _ in range(steps):
+            sim.advance(1)
+            base_masks.append(sim.read_geometry()[0])
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.adjoint_set_tangents(tan)
+        sim.write_populations(f0)
+        J, grad, timings = sim.adjoint_gradient(steps, weights, checkpoint_interval=15)
 
-    def J_gpu(theta, want_grad=False):
-        seq, _, tan = plate_frames(theta, steps)
+    def J_gpu(theta, frozen):
+        sq, _, _ = plate_frames(theta, steps)
         with bh.BirdFlowSimulation(cfg) as sim:
-            sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
-            if want_grad:
-                sim.adjoint_set_tangents(tan)
-                sim.init_equilibrium()
-                return sim.adjoint_gradient(steps, weights, checkpoint_interval=15)
-            sim.init_equilibrium()
+            sim.upload_mesh(sq.points, sq.times, sq.triangles, sq.parts, sq.half_thickness)
+            if frozen:
+                sim.freeze_masks(np.array(base_masks))
+            sim.write_populations(f0)
             sim.advance(steps)
             hist = sim.load_history()
         return float(np.sum(weights * hist[:, :6]))
 
-    J, grad, timings = J_gpu(THETA0, True)
-    fd = np.zeros(8)
+    fd, fd_small, fd_free = np.zeros(8), np.zeros(8), np.zeros(8)
     for k in range(8):
-        h = 1e-3 * max(1.0, abs(THETA0[k]))
+        h = 4e-3 * max(1.0, abs(THETA0[k]))
         e = np.zeros(8)
-        e[k] = h
-        fd[k] = (J_gpu(THETA0 + e) - J_gpu(THETA0 - e)) / (2 * h)
-    err = np.abs(grad - fd) / np.abs(fd).max()
+        e[k] = 1.0
+        fd[k] = (J_gpu(THETA0 + h * e, True) - J_gpu(THETA0 - h * e, True)) / (2 * h)
+        fd_small[k] = (J_gpu(THETA0 + h / 4 * e, True) - J_gpu(THETA0 - h / 4 * e, True)) / (h / 2)
+        fd_free[k] = (J_gpu(THETA0 + h / 4 * e, False) - J_gpu(THETA0 - h / 4 * e, False)) / (h / 2)
+    lo, hi = np.minimum(fd, fd_small), np.maximum(fd, fd_small)
+    outside = np.maximum(0.0, np.maximum(lo - grad, grad - hi))
+    allowed = 1e-3 * np.abs(fd).max() + 0.5 * (hi - lo)
     for k, name in enumerate(workloads.PLATE_PARAMETERS):
-        say(f"  {name:14s} adjoint {grad[k]: .6e}  solver FP32 central difference {fd[k]: .6e}  rel {err[k]:.2e}")
-    say(f"  adjoint sweep: forward {timings['forward_ms']:.1f} ms, reverse {timings['reverse_ms']:.1f} ms")
-    assert err.max() <= 2e-2
+        say(f"  {name:14s} adjoint {grad[k]: .6e}  FP32 central differences (fixed links) h {fd[k]: .6e}, "
+            f"h/4 {fd_small[k]: .6e}  outside {outside[k]:.2e} (allowed {allowed[k]:.2e});  "
+            f"re-rastered masks, h/4: {fd_free[k]: .6e}")
+    l2 = np.linalg.norm(grad - fd_small) / np.linalg.norm(fd_small)
+    say(f"  relative L2 against h/4 differences {l2:.2e}; adjoint sweep: forward {timings['forward_ms']:.1f} ms, "
+        f"reverse {timings['reverse_ms']:.1f} ms")
+    assert np.all(outside <= allowed)
+    assert l2 <= 1e-2
 
 
 def test_torch_autograd_returns_adjoint_gradient():
