@@ -305,36 +305,119 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
     const float4 wg4 = b.wall[gid];
     const float wgv[4] = {wg4.x, wg4.y, wg4.z, wg4.w};
     float gxw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for_q(AllQ{}, [&](auto qc) {
+    // Round 1, unconditional (a far-field source or a far side outside the
+    // domain reads this cell and is ignored): the part ids of every source
+    // and far side, the cell's previous populations, its adjoint slots (kept
+    // in registers and stored once at the end: only this cell writes them
+    // in this pass) and nu.
+    auto src_of = [&](auto qc) {
         constexpr int q = decltype(qc)::value;
         bool far;
         const uint32_t src = source_of<q>(s, g, x, y, z, far);
-        if (far) return;
-        const uint8_t part = b.solid_cur[src];
-        if (part == 0) return;
-        const float4 ws4 = b.wall[src];
-        const float wsv[4] = {ws4.x, ws4.y, ws4.z, ws4.w};
+        return far ? gid : src;
+    };
+    auto fid_of = [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
         const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
-        const bool fin_ok = inside(g, fx, fy, fz);
-        const uint32_t fid = fin_ok ? flat(g, fx, fy, fz) : gid;
-        const bool farther_ok = fin_ok && b.solid_cur[fid] == 0;
-        const float refl = b.f_prev[(size_t)opp(q) * n + gid];
-        const float farther = farther_ok ? b.f_prev[(size_t)opp(q) * n + fid] : 0.0f;
-        const float pin = b.f_prev[(size_t)q * n + gid];
-        const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
-                                                           p.wall_rule, farther_ok, refl, farther, pin, wgv, wsv);
-        const float nu = nu_cell[q * 256];
-        float refl_load = 0.0f;
-        const int sel = p.selected_part;
-        if (p.accumulate_loads && p.load_component != 2 && (sel == 0 || sel == (int)part)) {
-            const float larm[3] = {wx - d.lf * (float)cx(q) * p.dx - bx, wy - d.lf * (float)cy(q) * p.dx - by,
-                                   wz - d.lf * (float)cz(q) * p.dx - bz};
-            refl_load = -fs * load_weight<q>(w, larm);
+        return inside(g, fx, fy, fz) ? flat(g, fx, fy, fz) : gid;
+    };
+    uint32_t part[19], fpart[19];
+    float fown[19], lam[19], nuv[19];
+    uint32_t farbits = 0, okbits = 0;
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        bool far;
+        (void)source_of<q>(s, g, x, y, z, far);
+        farbits |= (uint32_t)far << q;
+        okbits |= (uint32_t)inside(g, x + cx(q), y + cy(q), z + cz(q)) << q;
+        part[q] = b.solid_cur[src_of(qc)];
+        fpart[q] = b.solid_cur[fid_of(qc)];
+        fown[q] = b.f_prev[(size_t)q * n + gid];
+        lam[q] = b.lam_out[(size_t)q * n + gid];
+        nuv[q] = nu_cell[q * 256];
+    });
+    asm volatile("" : BF_ASM19(part), BF_ASM19(fpart), BF_ASM19(fown));
+    asm volatile("" : BF_ASM19(lam), BF_ASM19(nuv));
+    // links, links whose far side is an in-domain fluid cell, links whose
+    // part counts toward the loads
+    uint32_t linkbits = 0, fartherbits = 0, selbits = 0;
+    const int sel = p.selected_part;
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        linkbits |= (uint32_t)(part[q] != 0 && !((farbits >> q) & 1u)) << q;
+        fartherbits |= (uint32_t)(((okbits >> q) & 1u) && fpart[q] == 0) << q;
+        selbits |= (uint32_t)(sel == 0 || sel == (int)part[q]) << q;
+    });
+    // Round 2, three directions at a time: each link's source wall entry
+    // and far-side population loaded together; then the wall-field tangents
+    // of the group's link sources.
+    auto link_group = [&](auto seq, auto first_c, uint32_t gmask) {
+        constexpr int q0 = decltype(first_c)::value;
+        float4 ws[3];
+        float fo[3];
+        uint32_t srcs[3];
+        float gws[3][4];
+        for (int j = 0; j < 3; ++j) {
+            ws[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            fo[j] = 0.0f;
+            srcs[j] = gid;
+            for (int i = 0; i < 4; ++i) gws[j][i] = 0.0f;
         }
-        b.lam_out[(size_t)opp(q) * n + gid] += nu * d.d_refl + refl_load;
-        if (d.uses_pin) b.lam_out[(size_t)q * n + gid] += nu * d.d_pin;
-        for (int i = 0; i < 4; ++i) gxw[i] += nu * d.d_wg[i];
-        contract(acc, w.K, b.dW, n, src, nu * d.d_ws[0], nu * d.d_ws[1], nu * d.d_ws[2], nu * d.d_ws[3]);
+        if (!(linkbits & gmask)) return;
+        for_q(seq, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const bool lk = (linkbits >> q) & 1u;
+            const uint32_t src = lk ? src_of(qc) : gid;
+            srcs[q - q0] = src;
+            ws[q - q0] = b.wall[src];
+            fo[q - q0] = b.f_prev[(size_t)opp(q) * n + (lk ? fid_of(qc) : gid)];
+        });
+        asm volatile("" : "+v"(ws[0].x), "+v"(ws[0].y), "+v"(ws[0].z), "+v"(ws[0].w), "+v"(ws[1].x),
+                     "+v"(ws[1].y), "+v"(ws[1].z), "+v"(ws[1].w), "+v"(ws[2].x), "+v"(ws[2].y), "+v"(ws[2].z),
+                     "+v"(ws[2].w), "+v"(fo[0]), "+v"(fo[1]), "+v"(fo[2]));
+        for_q(seq, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if (!((linkbits >> q) & 1u)) return;
+            const float4 ws4 = ws[q - q0];
+            const float wsv[4] = {ws4.x, ws4.y, ws4.z, ws4.w};
+            const bool farther_ok = (fartherbits >> q) & 1u;
+            const float refl = fown[opp(q)];
+            const float farther = farther_ok ? fo[q - q0] : 0.0f;
+            const float pin = fown[q];
+            const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
+                                                               p.wall_rule, farther_ok, refl, farther, pin, wgv, wsv);
+            const float nu = nuv[q];
+            float refl_load = 0.0f;
+            if (p.accumulate_loads && p.load_component != 2 && ((selbits >> q) & 1u)) {
+                const float larm[3] = {wx - d.lf * (float)cx(q) * p.dx - bx, wy - d.lf * (float)cy(q) * p.dx - by,
+                                       wz - d.lf * (float)cz(q) * p.dx - bz};
+                refl_load = -fs * load_weight<q>(w, larm);
+            }
+            lam[opp(q)] += nu * d.d_refl + refl_load;
+            if (d.uses_pin) lam[q] += nu * d.d_pin;
+            for (int i = 0; i < 4; ++i) gxw[i] += nu * d.d_wg[i];
+            for (int i = 0; i < 4; ++i) gws[q - q0][i] = nu * d.d_ws[i];
+        });
+        // contract in ascending q per parameter (a non-link direction adds
+        // zero times this cell's tangent)
+        for (int k = 0; k < w.K; ++k) {
+            float4 t[3];
+            for (int j = 0; j < 3; ++j) t[j] = b.dW[(size_t)k * n + srcs[j]];
+            for (int j = 0; j < 3; ++j)
+                if ((linkbits >> (q0 + j)) & 1u)
+                    acc[k] += (double)gws[j][0] * t[j].x + (double)gws[j][1] * t[j].y + (double)gws[j][2] * t[j].z +
+                              (double)gws[j][3] * t[j].w;
+        }
+    };
+    link_group(QSeq<1, 2, 3>{}, std::integral_constant<int, 1>{}, 0xEu);
+    link_group(QSeq<4, 5, 6>{}, std::integral_constant<int, 4>{}, 0x70u);
+    link_group(QSeq<7, 8, 9>{}, std::integral_constant<int, 7>{}, 0x380u);
+    link_group(QSeq<10, 11, 12>{}, std::integral_constant<int, 10>{}, 0x1C00u);
+    link_group(QSeq<13, 14, 15>{}, std::integral_constant<int, 13>{}, 0xE000u);
+    link_group(QSeq<16, 17, 18>{}, std::integral_constant<int, 16>{}, 0x70000u);
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        b.lam_out[(size_t)q * n + gid] = lam[q];
     });
     contract(acc, w.K, b.dW, n, gid, gxw[0], gxw[1], gxw[2], gxw[3]);
 }
@@ -521,17 +604,26 @@ __device__ void closest_point_dual(const float P[3], const D a[3], const D b[3],
 }
 
 // d wall / d theta_k of every keyed cell: wall = (sum_i b_i v_i, (|P - cp| - h) / dx)
-__global__ void wall_tangent(const uint32_t* __restrict__ tri_of_cell, const uint32_t* __restrict__ tri,
-                             const float4* __restrict__ pos, const float4* __restrict__ vel,
-                             const float4* __restrict__ dpos, const float4* __restrict__ dvel, float4* __restrict__ dW,
-                             MeshParams m, int K) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= m.g.n) return;
+// Cells of the step's listed tiles only: the reverse step reads dW at its
+// listed cells and at the solid sources of their links, whose tiles are
+// listed too (a solid cell flags its own tile).
+__global__ __launch_bounds__(256) void wall_tangent(const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ count,
+                                                   const uint32_t* __restrict__ tri_of_cell,
+                                                   const uint32_t* __restrict__ tri, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const float4* __restrict__ dpos,
+                                                   const float4* __restrict__ dvel, float4* __restrict__ dW,
+                                                   MeshParams m, int K) {
+    const uint32_t tiles = *count;
+    for (uint32_t i = blockIdx.x; i < tiles; i += gridDim.x) {
+    int x0, y0, z;
+    tile_origin(m.g, list[i], x0, y0, z);
+    const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+    const int y = y0 + (int)(threadIdx.x / TILE_X);
+    if (x >= m.g.nx || y >= m.g.ny) continue;
+    const uint32_t gid = (uint32_t)x + (uint32_t)m.g.nx * ((uint32_t)y + (uint32_t)m.g.ny * (uint32_t)z);
     const uint32_t t = tri_of_cell[gid];
-    if (t == NO_TRIANGLE || t >= m.triangle_count) return;
-    const int x = (int)(gid % (uint32_t)m.g.nx);
-    const uint32_t yz = gid / (uint32_t)m.g.nx;
-    const int y = (int)(yz % (uint32_t)m.g.ny), z = (int)(yz / (uint32_t)m.g.ny);
+    if (t == NO_TRIANGLE || t >= m.triangle_count) continue;
     const float P[3] = {m.origin[0] + ((float)x + 0.5f) * m.dx, m.origin[1] + ((float)y + 0.5f) * m.dx,
                         m.origin[2] + ((float)z + 0.5f) * m.dx};
     const uint32_t iv[3] = {tri[3 * (size_t)t], tri[3 * (size_t)t + 1], tri[3 * (size_t)t + 2]};
@@ -556,6 +648,7 @@ __global__ void wall_tangent(const uint32_t* __restrict__ tri_of_cell, const uin
         const float len = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
         const float dlen = len > 0.0f ? -(r[0] * cp[0].d + r[1] * cp[1].d + r[2] * cp[2].d) / len : 0.0f;
         dW[(size_t)k * m.g.n + gid] = make_float4(u[0].d, u[1].d, u[2].d, dlen / m.dx);
+    }
     }
 }
 
@@ -583,12 +676,13 @@ hipError_t launch_adjoint_step(int coll, const AdjointBuffers& b, const StepPara
     }
 }
 
-hipError_t launch_wall_tangent(const uint32_t* tri_of_cell, const uint32_t* tri, const float4* pos,
-                               const float4* vel, const float* dpoints, const float* times, float4* dpos,
-                               float4* dvel, float4* dW, const MeshParams& m, int K, hipStream_t s) {
+hipError_t launch_wall_tangent(const uint32_t* list, const uint32_t* count, int blocks, const uint32_t* tri_of_cell,
+                               const uint32_t* tri, const float4* pos, const float4* vel, const float* dpoints,
+                               const float* times, float4* dpos, float4* dvel, float4* dW, const MeshParams& m, int K,
+                               hipStream_t s) {
     hipLaunchKernelGGL(tangent_vertices, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, dpoints, times, dpos,
                        dvel, m, K);
-    hipLaunchKernelGGL(wall_tangent, dim3((m.g.n + 255) / 256), dim3(256), 0, s, tri_of_cell, tri, pos, vel, dpos,
+    hipLaunchKernelGGL(wall_tangent, dim3(blocks), dim3(256), 0, s, list, count, tri_of_cell, tri, pos, vel, dpos,
                        dvel, dW, m, K);
     return hipGetLastError();
 }

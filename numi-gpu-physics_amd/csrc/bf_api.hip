@@ -147,22 +147,33 @@ struct bf_context {
         uint32_t interval = 0, segments = 0;
         float* checkpoints = nullptr;     // [segments][19N]
         float* seg_f = nullptr;           // [interval][19N]: f_a .. f_{b-1}
-        uint8_t* seg_mask = nullptr;      // [interval + 1][N]: geometry of steps a .. b
-        float4* seg_wall = nullptr;
-        uint32_t* seg_tri = nullptr;
-        uint8_t* seg_dyn = nullptr;       // [interval + 1][tiles]
-        uint8_t* tiles_new = nullptr;
+        // geometry of steps a .. b of a segment, two sets: the reverse sweep
+        // of one segment (stream B) reads one while the geometry of the next
+        // segment is built into the other (stream A)
+        uint8_t* seg_mask[2] = {nullptr, nullptr};    // [interval + 1][N]
+        float4* seg_wall[2] = {nullptr, nullptr};
+        uint32_t* seg_tri[2] = {nullptr, nullptr};
+        uint8_t* seg_dyn[2] = {nullptr, nullptr};     // [interval + 1][tiles]
+        uint8_t* slot_tiles[2] = {nullptr, nullptr};  // [2][interval + 1][tiles]: touched tiles per build
+        int gen[2] = {0, 0};                          // which half of slot_tiles the next build writes
+        uint32_t* slot_list[2] = {nullptr, nullptr};  // [interval + 1][tiles]: general list of steps (k-1, k)
+        uint32_t* slot_count[2] = {nullptr, nullptr}; // [interval + 1]
+        float4* gpos = nullptr;                       // vertex positions / velocities of that build
+        float4* gvel = nullptr;
+        hipEvent_t ev_geo[2] = {nullptr, nullptr};    // set built (stream A)
+        hipEvent_t ev_done[2] = {nullptr, nullptr};   // set released (stream B)
         float* lam[2] = {nullptr, nullptr};
         int lam_cur = 0;
         bool have_sensitivity = false;
         float* nu = nullptr;              // [tiles][19][256]
-        float4* dW = nullptr;             // [8][N]
+        float4* dW[2] = {nullptr, nullptr};           // [8][N], by step parity
+        hipStream_t sc = nullptr;                     // wall tangents (third stream, default priority)
+        hipEvent_t ev_tan[2] = {nullptr, nullptr};    // dW[j] written (stream C)
+        hipEvent_t ev_used[2] = {nullptr, nullptr};   // dW[j] read (stream B)
         float4* dpos = nullptr;           // [8][vertices]
         float4* dvel = nullptr;
         double* grad_partials = nullptr;  // [tiles][8]
         double* grad_total = nullptr;     // [8]
-        uint32_t* list = nullptr;         // general list of the step being reversed
-        uint32_t* count = nullptr;
     } adj;
     // stepping
     int64_t step = 0;
@@ -182,7 +193,8 @@ struct bf_context {
     // geometry grids: bounded so that the geometry of step s+1 leaves the
     // fluid kernel of step s most of the machine
     uint32_t raster_blocks = 1u << 30;
-    uint32_t raster_batch = 64;
+    uint32_t raster_batch = 0;            // triangles per raster wave; 0: from the mesh size
+    uint32_t compute_units = 1;
     uint32_t tile_pass_blocks = 512;
 };
 
@@ -208,7 +220,8 @@ bool periodic_axis(const bf_config& c, int a) {
 hipError_t drain(bf_context* c) {
     hipError_t e = hipStreamSynchronize(c->sa);
     hipError_t e2 = hipStreamSynchronize(c->sb);
-    return e != hipSuccess ? e : e2;
+    hipError_t e3 = c->adj.sc ? hipStreamSynchronize(c->adj.sc) : hipSuccess;
+    return e != hipSuccess ? e : (e2 != hipSuccess ? e2 : e3);
 }
 
 #define BF_ENTER(c)                                                                                \
@@ -307,7 +320,17 @@ MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     m.cull_cells = exact ? INFINITY : c->m_half / c->cfg.cell_size + 1.5f;
     m.move_with_body = c->move_mesh ? 1 : 0;
     m.raster_blocks = c->raster_blocks;
-    m.raster_batch = c->raster_batch;
+    // enough batches for two waves per CU, at most 64 triangles each: one
+    // gather round trip then serves many triangles of a large mesh, while a
+    // small mesh of large triangles spreads one triangle per wave
+    m.raster_batch = c->raster_batch ? c->raster_batch
+                                     : std::min(64u, std::max(1u, c->m_triangles / (2u * c->compute_units)));
+    // a mesh of fewer triangles than that splits each triangle's box among
+    // several wavefronts
+    {
+        const uint32_t batches = (c->m_triangles + m.raster_batch - 1) / std::max(1u, m.raster_batch);
+        m.raster_split = std::min(64u, std::max(1u, (2u * c->compute_units) / std::max(1u, batches)));
+    }
     m.tile_pass_blocks = c->tile_pass_blocks;
     // frame interval, MeasuredBirdSurfaceSequence.swift:174-204
     const std::vector<float>& ts = c->host_times;
@@ -544,6 +567,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     g.tiles = (uint32_t)((uint64_t)g.tiles_x * g.tiles_y * g.nz);
     c->pop_elem = cfg->storage == BF_STORAGE_FP32 ? 4 : 2;
     c->general_blocks = std::max(1, prop.multiProcessorCount * 4);
+    c->compute_units = (uint32_t)std::max(1, prop.multiProcessorCount);
     // one tile-pass workgroup per CU: the fastest of a sweep (CU x 1/2/4/8)
     // on the 512x256x256 bird, where the passes overlap the fluid kernel;
     // the raster launches every triangle batch at once
@@ -640,10 +664,15 @@ void bf_destroy(bf_context* c) {
     (void)hipSetDevice(c->device);
     if (c->sa) (void)hipStreamSynchronize(c->sa);
     if (c->sb) (void)hipStreamSynchronize(c->sb);
+    if (c->adj.sc) (void)hipStreamSynchronize(c->adj.sc);
     for (hipEvent_t ev : c->events) (void)hipEventDestroy(ev);
     for (int i = 0; i < 2; ++i) {
         if (c->ev_fast[i]) (void)hipEventDestroy(c->ev_fast[i]);
         if (c->ev_ready[i]) (void)hipEventDestroy(c->ev_ready[i]);
+        if (c->adj.ev_geo[i]) (void)hipEventDestroy(c->adj.ev_geo[i]);
+        if (c->adj.ev_done[i]) (void)hipEventDestroy(c->adj.ev_done[i]);
+        if (c->adj.ev_tan[i]) (void)hipEventDestroy(c->adj.ev_tan[i]);
+        if (c->adj.ev_used[i]) (void)hipEventDestroy(c->adj.ev_used[i]);
     }
     c->mesh_alloc.release();
     c->adj.alloc.release();
@@ -652,6 +681,7 @@ void bf_destroy(bf_context* c) {
     c->alloc.release();
     if (c->sa) (void)hipStreamDestroy(c->sa);
     if (c->sb) (void)hipStreamDestroy(c->sb);
+    if (c->adj.sc) (void)hipStreamDestroy(c->adj.sc);
     delete c;
 }
 
@@ -1147,8 +1177,9 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     a.alloc.release();
     const uint64_t n = c->g.n, tiles = c->g.tiles, pop = 19ull * n * 4;
     const uint64_t verts = std::max<uint32_t>(c->m_vertices, 1u);
-    const uint64_t need = pop * segments + pop * interval + (uint64_t)(interval + 1) * (21 * n + tiles) + 2 * pop +
-                          tiles * 19 * 256 * 4 + 8 * 16 * n + 2 * 8 * 16 * verts + tiles * 64 + 4 * tiles + 1024;
+    const uint64_t need = pop * segments + pop * interval + 2 * (uint64_t)(interval + 1) * (21 * n + 7 * tiles + 4) +
+                          2 * pop + tiles * 19 * 256 * 4 + 2 * 8 * 16 * n + 2 * 8 * 16 * verts + 2 * 16 * verts +
+                          tiles * 64 + 4 * tiles + 1024;
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     if (need > (uint64_t)free_b) {
@@ -1161,25 +1192,57 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     DeviceAlloc& A = a.alloc;
     a.checkpoints = (float*)A.take(e, pop * segments);
     a.seg_f = (float*)A.take(e, pop * interval);
-    a.seg_mask = (uint8_t*)A.take(e, (interval + 1) * n);
-    a.seg_wall = (float4*)A.take(e, (interval + 1) * 16 * n);
-    a.seg_tri = (uint32_t*)A.take(e, (interval + 1) * 4 * n);
-    a.seg_dyn = (uint8_t*)A.take(e, (interval + 1) * tiles);
-    a.tiles_new = (uint8_t*)A.take(e, tiles);
+    for (int i = 0; i < 2; ++i) {
+        a.seg_mask[i] = (uint8_t*)A.take(e, (interval + 1) * n);
+        a.seg_wall[i] = (float4*)A.take(e, (interval + 1) * 16 * n);
+        a.seg_tri[i] = (uint32_t*)A.take(e, (interval + 1) * 4 * n);
+        a.seg_dyn[i] = (uint8_t*)A.take(e, (interval + 1) * tiles);
+    }
+    for (int i = 0; i < 2; ++i) {
+        a.slot_tiles[i] = (uint8_t*)A.take(e, 2 * (interval + 1) * tiles);
+        a.slot_list[i] = (uint32_t*)A.take(e, (interval + 1) * 4 * tiles);
+        a.slot_count[i] = (uint32_t*)A.take(e, 4 * (interval + 1));
+    }
+    a.gpos = (float4*)A.take(e, 16 * verts);
+    a.gvel = (float4*)A.take(e, 16 * verts);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        if (!a.ev_geo[i]) e = hipEventCreateWithFlags(&a.ev_geo[i], hipEventDisableTiming);
+        if (e == hipSuccess && !a.ev_done[i]) e = hipEventCreateWithFlags(&a.ev_done[i], hipEventDisableTiming);
+    }
     a.lam[0] = (float*)A.take(e, pop);
     a.lam[1] = (float*)A.take(e, pop);
     a.nu = (float*)A.take(e, tiles * 19 * 256 * 4);
-    a.dW = (float4*)A.take(e, 8 * 16 * n);
+    a.dW[0] = (float4*)A.take(e, 8 * 16 * n);
+    a.dW[1] = (float4*)A.take(e, 8 * 16 * n);
+    if (e == hipSuccess && !a.sc) e = hipStreamCreateWithFlags(&a.sc, hipStreamNonBlocking);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        if (!a.ev_tan[i]) e = hipEventCreateWithFlags(&a.ev_tan[i], hipEventDisableTiming);
+        if (e == hipSuccess && !a.ev_used[i]) e = hipEventCreateWithFlags(&a.ev_used[i], hipEventDisableTiming);
+    }
     a.dpos = (float4*)A.take(e, 8 * 16 * verts);
     a.dvel = (float4*)A.take(e, 8 * 16 * verts);
     a.grad_partials = (double*)A.take(e, tiles * 64);
     a.grad_total = (double*)A.take(e, 64);
-    a.list = (uint32_t*)A.take(e, 4 * tiles);
-    a.count = (uint32_t*)A.take(e, 64);
     if (e != hipSuccess) {
         A.release();
         a.checkpoints = nullptr;
         return fail(c, BF_ERROR_ALLOCATION_FAILED, "adjoint buffers");
+    }
+    // geometry slots start cleared, with no touched tiles
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+        for (uint64_t k = 0; k <= interval && e == hipSuccess; ++k) {
+            e = launch_fill_u8(a.seg_mask[i] + k * n, 0, (uint32_t)n, c->sb);
+            if (e == hipSuccess) e = launch_fill_wall(a.seg_wall[i] + k * n, (uint32_t)n, c->sb);
+            if (e == hipSuccess) e = launch_fill_u32(a.seg_tri[i] + k * n, 0xFFFFFFFFu, (uint32_t)n, c->sb);
+        }
+        if (e == hipSuccess) e = launch_fill_u8(a.slot_tiles[i], 0, (uint32_t)(2 * (interval + 1) * tiles), c->sb);
+        a.gen[i] = 0;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) {
+        A.release();
+        a.checkpoints = nullptr;
+        return hip_fail(c, e, "adjoint buffers");
     }
     a.interval = interval;
     a.segments = segments;
@@ -1196,54 +1259,88 @@ struct SlotGeometry {
 // Geometry of step s into segment slot k: the same raster / resolve as the
 // forward pass (masks identical, walls identical wherever the step reads
 // them), plus each keyed cell's nearest triangle.
-bf_status adjoint_geometry(bf_context* c, int64_t s, uint32_t k) {
+SlotGeometry slot_geometry(const bf_context* c, uint32_t k, int set) {
+    const auto& a = c->adj;
+    if (c->source == BF_GEOMETRY_STATIC) return {c->mask[0], c->wall[0], c->dyn[0], nullptr};
+    const uint64_t n = c->g.n;
+    return {a.seg_mask[set] + k * n, a.seg_wall[set] + k * n, a.seg_dyn[set] + (uint64_t)k * c->g.tiles,
+            a.seg_tri[set] + k * n};
+}
+
+// Geometry of step s into slot k of set `set`, on stream `st` (its own
+// vertex buffers; keys, geometry list and touched-tile flags are used by no
+// other work during a sweep).
+bf_status adjoint_geometry(bf_context* c, int64_t s, uint32_t k, int set, hipStream_t st) {
     auto& a = c->adj;
     if (c->source == BF_GEOMETRY_STATIC) return BF_OK;
     const uint64_t n = c->g.n;
-    uint8_t* mask = a.seg_mask + k * n;
-    float4* wall = a.seg_wall + k * n;
-    uint32_t* tri = a.seg_tri + k * n;
-    uint8_t* dyn = a.seg_dyn + (uint64_t)k * c->g.tiles;
-    hipError_t e = launch_fill_u8(mask, 0, (uint32_t)n, c->sb);
-    if (e == hipSuccess) e = launch_fill_wall(wall, (uint32_t)n, c->sb);
-    if (e == hipSuccess) e = launch_fill_u32(tri, 0xFFFFFFFFu, (uint32_t)n, c->sb);
-    if (e != hipSuccess) return hip_fail(c, e, "adjoint geometry clear");
+    uint8_t* mask = a.seg_mask[set] + k * n;
+    float4* wall = a.seg_wall[set] + k * n;
+    uint32_t* tri = a.seg_tri[set] + k * n;
+    uint8_t* dyn = a.seg_dyn[set] + (uint64_t)k * c->g.tiles;
+    // The slot holds the geometry of its previous build: the tile pass
+    // clears the tiles that build touched (old flags) and resolves the new
+    // ones, as in stepping.
+    const uint64_t tiles = c->g.tiles, slots = a.interval + 1;
+    uint8_t* touched_new = a.slot_tiles[set] + ((uint64_t)a.gen[set] * slots + k) * tiles;
+    uint8_t* touched_old = a.slot_tiles[set] + ((uint64_t)(1 - a.gen[set]) * slots + k) * tiles;
+    hipError_t e = hipSuccess;
     MeshBuffers b{};
     b.body = c->body;
     b.points = c->m_points;
     b.times = c->m_times;
     b.tri = c->m_tri;
     b.tri_part = c->m_part;
-    b.pos = c->m_pos;
-    b.vel = c->m_vel;
+    b.pos = a.gpos;
+    b.vel = a.gvel;
     b.keys = c->keys;
-    b.tiles_new = a.tiles_new;
-    b.tiles_mask_old = c->no_tiles;
-    b.tiles_wall_old = c->no_tiles;
+    b.tiles_new = touched_new;
+    b.tiles_mask_old = touched_old;
+    b.tiles_wall_old = touched_old;
     b.list = c->geo_list;
     b.list_count = &c->counters[2];
     b.mask = mask;
     b.wall = wall;
     b.dyn = dyn;
     b.tri_out = tri;
-    e = launch_mesh_geometry(b, mesh_params(c, time_of(c, s), false), c->sb);
+    e = launch_mesh_geometry(b, mesh_params(c, time_of(c, s), false), st);
     if (e != hipSuccess) return hip_fail(c, e, "adjoint geometry");
     return BF_OK;
 }
 
-SlotGeometry slot_geometry(const bf_context* c, uint32_t k) {
-    const auto& a = c->adj;
-    if (c->source == BF_GEOMETRY_STATIC) return {c->mask[0], c->wall[0], c->dyn[0], nullptr};
-    const uint64_t n = c->g.n;
-    return {a.seg_mask + k * n, a.seg_wall + k * n, a.seg_dyn + (uint64_t)k * c->g.tiles, a.seg_tri + k * n};
+// Geometry of every step of segment `seg` into set seg & 1, on stream A.
+bf_status segment_geometry(bf_context* c, int64_t s0, uint32_t steps, uint32_t interval, int64_t seg) {
+    const int64_t sa_ = s0 + seg * interval;
+    const int64_t sb_ = std::min<int64_t>(sa_ + interval, s0 + steps);
+    const int set = (int)(seg & 1);
+    auto& a = c->adj;
+    a.gen[set] ^= 1;
+    for (int64_t s = sa_; s <= sb_; ++s)
+        if (bf_status r = adjoint_geometry(c, s, (uint32_t)(s - sa_), set, c->sa)) return r;
+    // slots a short (last) segment leaves alone keep their contents, and so
+    // their touched-tile flags
+    const uint64_t tiles = c->g.tiles, slots = a.interval + 1;
+    for (uint64_t k = (uint64_t)(sb_ - sa_) + 1; k < slots; ++k) {
+        hipError_t e = hipMemcpyAsync(a.slot_tiles[set] + ((uint64_t)a.gen[set] * slots + k) * tiles,
+                                      a.slot_tiles[set] + ((uint64_t)(1 - a.gen[set]) * slots + k) * tiles, tiles,
+                                      hipMemcpyDeviceToDevice, c->sa);
+        if (e != hipSuccess) return hip_fail(c, e, "adjoint geometry");
+    }
+    // the general list of each step (k - 1, k), used by its recompute and its
+    // reverse step
+    hipError_t e = launch_fill_u32(a.slot_count[set], 0u, (uint32_t)(sb_ - sa_ + 1), c->sa);
+    for (int64_t s = sa_ + 1; s <= sb_ && e == hipSuccess; ++s) {
+        const uint32_t k = (uint32_t)(s - sa_);
+        const SlotGeometry gp = slot_geometry(c, k - 1, set), gc = slot_geometry(c, k, set);
+        e = launch_build_general_list(c->static_flags, gp.dyn, gc.dyn, c->g.tiles,
+                                      a.slot_list[set] + (uint64_t)k * c->g.tiles, a.slot_count[set] + k, c->sa);
+    }
+    if (e == hipSuccess) e = hipEventRecord(a.ev_geo[set], c->sa);
+    if (e != hipSuccess) return hip_fail(c, e, "adjoint geometry");
+    return BF_OK;
 }
 
-hipError_t adjoint_list(bf_context* c, const SlotGeometry& prev, const SlotGeometry& cur) {
-    hipError_t e = launch_fill_u32(c->adj.count, 0u, 1u, c->sb);
-    if (e == hipSuccess)
-        e = launch_build_general_list(c->static_flags, prev.dyn, cur.dyn, c->g.tiles, c->adj.list, c->adj.count, c->sb);
-    return e;
-}
+
 
 }  // namespace
 
@@ -1316,14 +1413,26 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
     e = launch_fill_u32((uint32_t*)a.lam[0], 0u, (uint32_t)pop, c->sb);    // dJ/df at the last step: 0
     if (e == hipSuccess) e = launch_fill_u32((uint32_t*)a.grad_total, 0u, 16u, c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "adjoint init");
+    // Segments in reverse order on stream B; the geometry of the segment
+    // after the current one is built on stream A meanwhile, into the other
+    // set (released by stream B once the segment that last used it is done).
+    if (bf_status r = segment_geometry(c, s0, steps, interval, (int64_t)segments - 1)) return r;
+    uint64_t tangent_steps = 0;
     for (int64_t seg = (int64_t)segments - 1; seg >= 0; --seg) {
         const int64_t sa_ = s0 + seg * interval;
         const int64_t sb_ = std::min<int64_t>(sa_ + interval, s0 + steps);
-        for (int64_t s = sa_; s <= sb_; ++s)
-            if (bf_status r = adjoint_geometry(c, s, (uint32_t)(s - sa_))) return r;
-        e = hipMemcpyAsync(a.seg_f, a.checkpoints + seg * pop, 4 * pop, hipMemcpyDeviceToDevice, c->sb);
+        const int set = (int)(seg & 1);
+        e = hipStreamWaitEvent(c->sb, a.ev_geo[set], 0);
+        if (e == hipSuccess && seg > 0) {
+            if (seg + 1 < (int64_t)segments) e = hipStreamWaitEvent(c->sa, a.ev_done[1 - set], 0);
+            if (e != hipSuccess) return hip_fail(c, e, "adjoint schedule");
+            if (bf_status r = segment_geometry(c, s0, steps, interval, seg - 1)) return r;
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(a.seg_f, a.checkpoints + seg * pop, 4 * pop, hipMemcpyDeviceToDevice, c->sb);
         for (int64_t s = sa_ + 1; s < sb_ && e == hipSuccess; ++s) {
-            const SlotGeometry gp = slot_geometry(c, (uint32_t)(s - 1 - sa_)), gc = slot_geometry(c, (uint32_t)(s - sa_));
+            const SlotGeometry gp = slot_geometry(c, (uint32_t)(s - 1 - sa_), set),
+                               gc = slot_geometry(c, (uint32_t)(s - sa_), set);
             FluidBuffers fb{};
             fb.fin = a.seg_f + (s - 1 - sa_) * pop;
             fb.fout = a.seg_f + (s - sa_) * pop;
@@ -1333,23 +1442,27 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
             fb.static_flags = c->static_flags;
             fb.dyn_prev = gp.dyn;
             fb.dyn_cur = gc.dyn;
-            fb.general_list = a.list;
-            fb.general_count = a.count;
+            fb.general_list = a.slot_list[set] + (uint64_t)(s - sa_) * c->g.tiles;
+            fb.general_count = a.slot_count[set] + (s - sa_);
             fb.partials = c->partials;
             fb.density = c->density;
             fb.velocity = c->velocity;
             fb.activation = &c->counters[8];
             fb.body = c->body;
-            e = adjoint_list(c, gp, gc);
-            if (e == hipSuccess) e = launch_fluid_fast(arith, storage, coll, fb, pf, c->sb);
+            e = launch_fluid_fast(arith, storage, coll, fb, pf, c->sb);
             if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, pf, c->general_blocks, c->sb);
         }
         if (e != hipSuccess) return hip_fail(c, e, "adjoint recompute");
         for (int64_t s = sb_; s > sa_; --s) {
             const uint32_t k = (uint32_t)(s - sa_);
-            const SlotGeometry gp = slot_geometry(c, k - 1), gc = slot_geometry(c, k);
+            const SlotGeometry gp = slot_geometry(c, k - 1, set), gc = slot_geometry(c, k, set);
+            const int j = (int)(s & 1);
             if (a.K > 0) {
-                // vertex positions and velocities of step s, then the tangents
+                // Stream C, up to two steps ahead: vertex positions and
+                // velocities of step s, then the wall tangents into dW[s & 1]
+                // once the step that last read it (s + 2) is done.
+                if (s == sb_) e = hipStreamWaitEvent(a.sc, a.ev_geo[set], 0);
+                if (e == hipSuccess && tangent_steps >= 2) e = hipStreamWaitEvent(a.sc, a.ev_used[j], 0);
                 MeshBuffers mb{};
                 mb.body = c->body;
                 mb.points = c->m_points;
@@ -1357,11 +1470,15 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
                 mb.pos = c->m_pos;
                 mb.vel = c->m_vel;
                 const MeshParams mp = mesh_params(c, time_of(c, s), false);
-                e = launch_mesh_prepare(mb, mp, c->sb);
+                if (e == hipSuccess) e = launch_mesh_prepare(mb, mp, a.sc);
                 if (e == hipSuccess)
-                    e = launch_wall_tangent(gc.tri, c->m_tri, c->m_pos, c->m_vel, a.dpoints, c->m_times, a.dpos,
-                                            a.dvel, a.dW, mp, (int)a.K, c->sb);
+                    e = launch_wall_tangent(a.slot_list[set] + (uint64_t)k * c->g.tiles, a.slot_count[set] + k,
+                                            c->general_blocks, gc.tri, c->m_tri, c->m_pos, c->m_vel, a.dpoints,
+                                            c->m_times, a.dpos, a.dvel, a.dW[j], mp, (int)a.K, a.sc);
+                if (e == hipSuccess) e = hipEventRecord(a.ev_tan[j], a.sc);
+                if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, a.ev_tan[j], 0);
                 if (e != hipSuccess) return hip_fail(c, e, "wall tangents");
+                ++tangent_steps;
             }
             AdjointBuffers ab{};
             ab.f_prev = a.seg_f + (k - 1) * pop;
@@ -1373,10 +1490,10 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
             ab.static_flags = c->static_flags;
             ab.dyn_prev = gp.dyn;
             ab.dyn_cur = gc.dyn;
-            ab.general_list = a.list;
-            ab.general_count = a.count;
+            ab.general_list = a.slot_list[set] + (uint64_t)k * c->g.tiles;
+            ab.general_count = a.slot_count[set] + k;
             ab.nu = a.nu;
-            ab.dW = a.dW;
+            ab.dW = a.dW[j];
             ab.grad_partials = a.grad_partials;
             ab.grad_total = a.grad_total;
             ab.body = c->body;
@@ -1387,11 +1504,13 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
                 w.cT[i] = (float)lw[3 + i];
             }
             w.K = (int)a.K;
-            e = adjoint_list(c, gp, gc);
-            if (e == hipSuccess) e = launch_adjoint_step(coll, ab, pa, w, c->general_blocks, c->sb);
+            e = launch_adjoint_step(coll, ab, pa, w, c->general_blocks, c->sb);
+            if (e == hipSuccess && a.K > 0) e = hipEventRecord(a.ev_used[j], c->sb);
             if (e != hipSuccess) return hip_fail(c, e, "adjoint step");
             a.lam_cur = 1 - a.lam_cur;
         }
+        e = hipEventRecord(a.ev_done[set], c->sb);
+        if (e != hipSuccess) return hip_fail(c, e, "adjoint schedule");
     }
     e = hipEventRecord(t2, c->sb);
     double g8[8] = {0, 0, 0, 0, 0, 0, 0, 0};

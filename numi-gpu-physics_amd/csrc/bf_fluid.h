@@ -712,19 +712,38 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
         });
         if (p.accumulate_loads && include_topology && mode == 6) {
             float tx = -mx, ty = -my, tz = -mz;
+            // two rounds of unconditional loads (a neighbour outside the
+            // domain reads this cell and is ignored): the neighbours' part
+            // ids, then the populations of the open directions
+            auto nb_of = [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const int bx = x + cx(q), by = y + cy(q), bz = z + cz(q);
+                return inside(g, bx, by, bz) ? flat(g, bx, by, bz) : gid;
+            };
+            uint32_t mk[19];
+            uint32_t inbits = 0;
+            for_q(AllQ{}, [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const uint32_t nb = nb_of(qc);
+                mk[q] = (uint32_t)solid_prev[nb] | (uint32_t)solid_cur[nb];
+                inbits |= (uint32_t)inside(g, x + cx(q), y + cy(q), z + cz(q)) << q;
+            });
+            asm volatile("" : BF_ASM19(mk));
+            float own[19], far[19];
+            for_q(AllQ{}, [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                own[q] = Store<S>::load(fin, (size_t)q * n + gid, q);
+                far[q] = Store<S>::load(fin, (size_t)opp(q) * n + nb_of(qc), opp(q));
+            });
+            asm volatile("" : BF_ASM19(own), BF_ASM19(far));
             for_q(AllQ{}, [&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 if constexpr (q != 0) {
-                    const int bx = x + cx(q), by = y + cy(q), bz = z + cz(q);
-                    if (inside(g, bx, by, bz)) {
-                        const uint32_t nb = flat(g, bx, by, bz);
-                        if (solid_prev[nb] == 0 && solid_cur[nb] == 0) {
-                            float s = Store<S>::load(fin, (size_t)q * n + gid, q)
-                                + Store<S>::load(fin, (size_t)opp(q) * n + nb, opp(q));
-                            tx = tx - s * (float)cx(q);
-                            ty = ty - s * (float)cy(q);
-                            tz = tz - s * (float)cz(q);
-                        }
+                    if (((inbits >> q) & 1u) && mk[q] == 0) {
+                        float s = own[q] + far[q];
+                        tx = tx - s * (float)cx(q);
+                        ty = ty - s * (float)cy(q);
+                        tz = tz - s * (float)cz(q);
                     }
                 }
             });
@@ -789,7 +808,40 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
             fallbits |= (uint32_t)(!((okbits >> q) & 1u) || fpart[q] != 0) << q;
             selbits |= (uint32_t)(sel == 0 || sel == (int)partq[q]) << q;
         });
-        for_q(AllQ{}, [&](auto qc) {
+        // Round 3, six directions at a time: the wall entry of each link's
+        // source and the population on its far side, all loaded before any is
+        // used (a direction without a link reads this cell's entries and
+        // ignores them); then the values of the group's directions in
+        // ascending q.  A load per link inside the link branch would cost one
+        // memory round trip per link direction of the wavefront.
+        auto link_group = [&](auto seq, auto first_c, uint32_t gmask) {
+            constexpr int q0 = decltype(first_c)::value;
+            float4 wsg[7];
+            float fog[7];
+            for (int k = 0; k < 7; ++k) {
+                wsg[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                fog[k] = 0.0f;
+            }
+            if (linkbits & gmask) {
+                for_q(seq, [&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    if constexpr (q != 0) {
+                        const bool lk = (linkbits >> q) & 1u;
+                        wsg[q - q0] = wall[lk ? src_of(qc) : gid];
+                        fog[q - q0] = Store<S>::load(fin, (size_t)opp(q) * n + (lk ? far_side_of(qc) : gid), opp(q));
+                    }
+                });
+                asm volatile("" : "+v"(wsg[0].x), "+v"(wsg[0].y), "+v"(wsg[0].z), "+v"(wsg[0].w),
+                             "+v"(wsg[1].x), "+v"(wsg[1].y), "+v"(wsg[1].z), "+v"(wsg[1].w),
+                             "+v"(wsg[2].x), "+v"(wsg[2].y), "+v"(wsg[2].z), "+v"(wsg[2].w),
+                             "+v"(wsg[3].x), "+v"(wsg[3].y), "+v"(wsg[3].z), "+v"(wsg[3].w),
+                             "+v"(wsg[4].x), "+v"(wsg[4].y), "+v"(wsg[4].z), "+v"(wsg[4].w),
+                             "+v"(wsg[5].x), "+v"(wsg[5].y), "+v"(wsg[5].z), "+v"(wsg[5].w),
+                             "+v"(wsg[6].x), "+v"(wsg[6].y), "+v"(wsg[6].z), "+v"(wsg[6].w),
+                             "+v"(fog[0]), "+v"(fog[1]), "+v"(fog[2]), "+v"(fog[3]), "+v"(fog[4]), "+v"(fog[5]),
+                             "+v"(fog[6]));
+            }
+            for_q(seq, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
             constexpr float dxq = (float)cx(q), dyq = (float)cy(q), dzq = (float)cz(q);
             float value;
@@ -797,11 +849,10 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                 value = p.far_eq[q];
             } else {
                 if ((linkbits >> q) & 1u) {
-                    const uint32_t src = src_of(qc);
                     const float reflected = fown[opp(q)];
                     float lf = 0.5f;
-                    const float4 ws = wall[src];
-                    const float fo_far = Store<S>::load(fin, (size_t)opp(q) * n + far_side_of(qc), opp(q));
+                    const float4 ws = wsg[q - q0];
+                    const float fo_far = fog[q - q0];
                     float wvx = ws.x, wvy = ws.y, wvz = ws.z;
                     bool interp = interpolated_rule;
                     if (interp) {
@@ -890,7 +941,11 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
             mx = mx + value * dxq;
             my = my + value * dyq;
             mz = mz + value * dzq;
-        });
+            });
+        };
+        link_group(QSeq<0, 1, 2, 3, 4, 5, 6>{}, std::integral_constant<int, 0>{}, 0x7Eu);
+        link_group(QSeq<7, 8, 9, 10, 11, 12>{}, std::integral_constant<int, 7>{}, 0x1F80u);
+        link_group(QSeq<13, 14, 15, 16, 17, 18>{}, std::integral_constant<int, 13>{}, 0x7E000u);
     }
 
     const float sponge = p.sponge_strength > 0.0f ? sponge_factor(p, x, y, z) : 0.0f;

@@ -149,7 +149,9 @@ __global__ void fill_wall(float4* __restrict__ p, uint32_t n) {
 
 // Raster to nearest-triangle keys (BirdFlow.metal:1638-1686 bounds and key).
 // One single-wavefront workgroup per batch of up to 64 triangles
-// (m.raster_batch; grid-stride over batches):
+// (m.raster_batch; grid-stride over batches), or, for a mesh of few large
+// triangles, m.raster_split wavefronts per triangle (grid y), each taking
+// every raster_split-th 64-column chunk of the box:
 //  0. lane j loads triangle (batch + j) and sets up its expanded voxel AABB,
 //     plane and bounding sphere, so one gather round trip serves the batch;
 //     the wave then walks the batch one triangle at a time with the
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
             const int ex = bcast(ext[0], j), ey = bcast(ext[1], j), ez = bcast(ext[2], j);
             const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
             uint32_t count = 0;  // wave-uniform
-            for (uint32_t c0 = 0; c0 < cols; c0 += 64) {
+            for (uint32_t c0 = blockIdx.y * 64u; c0 < cols; c0 += gridDim.y * 64u) {
                 const uint32_t col = c0 + lane;
                 const int y = ly + (int)(col / (uint32_t)max(ex, 1));
                 const int x = lx + (int)(col - (uint32_t)(y - ly) * (uint32_t)ex);
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
                     const unsigned long long ballot = __ballot(pass);
                     if (pass) cand[count + lane_rank(ballot)] = make_uint2((uint32_t)x | ((uint32_t)y << 16), (uint32_t)z);
                     count += (uint32_t)__popcll(ballot);
-                    const bool last = zi + 1 == ez && c0 + 64 >= cols;
+                    const bool last = zi + 1 == ez && c0 + gridDim.y * 64u >= cols;
                     if (count > RASTER_CAP - 64 || (last && count > 0)) {
                         __syncthreads();
                         for (uint32_t k = lane; k < count; k += 64) {
@@ -371,7 +373,10 @@ __global__ __launch_bounds__(256) void geometry_tile_pass(
         const float4 cleared = make_float4(0.0f, 0.0f, 0.0f, 16.0f);
         if (!tiles_new[tile]) {
             if (tiles_mask_old[tile]) mask[gid] = 0;
-            if (tiles_wall_old[tile]) wall[gid] = cleared;
+            if (tiles_wall_old[tile]) {
+                wall[gid] = cleared;
+                if (tri_out) tri_out[gid] = 0xFFFFFFFFu;
+            }
             continue;
         }
         const unsigned long long key = keys[gid];
@@ -379,6 +384,7 @@ __global__ __launch_bounds__(256) void geometry_tile_pass(
         if (key == EMPTY_KEY || t >= m.triangle_count) {
             mask[gid] = 0;
             wall[gid] = cleared;
+            if (tri_out) tri_out[gid] = 0xFFFFFFFFu;
             continue;
         }
         keys[gid] = EMPTY_KEY;
@@ -504,14 +510,15 @@ hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStr
 
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
     if (m.g.nx > 65535 || m.g.ny > 65535) return hipErrorInvalidValue;   // raster packs x | y << 16
-    if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1) return hipErrorInvalidValue;
+    if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1 || m.raster_split < 1 || m.raster_split > 64)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
                        b.vel, b.body, m);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.dyn, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
     if (m.triangle_count > 0)
-        hipLaunchKernelGGL(mesh_raster, dim3(std::min((m.triangle_count + m.raster_batch - 1) / m.raster_batch, m.raster_blocks)), dim3(64), 0, s, b.pos,
+        hipLaunchKernelGGL(mesh_raster, dim3(std::min((m.triangle_count + m.raster_batch - 1) / m.raster_batch, m.raster_blocks), m.raster_split), dim3(64), 0, s, b.pos,
                            b.tri, b.keys, b.tiles_new, m);
     hipLaunchKernelGGL(geometry_tile_list, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new,
                        b.tiles_mask_old, b.tiles_wall_old, m.g.tiles, b.list, b.list_count);

@@ -47,6 +47,7 @@ struct MeshParams {
     int move_with_body;        // frames are body-frame; place them with the device body pose
     uint32_t raster_blocks;    // raster grid cap (grid-stride over triangle batches)
     uint32_t raster_batch;     // triangles per raster wave batch, 1..64
+    uint32_t raster_split;     // wavefronts per batch (column chunks split among them), 1..64
     uint32_t tile_pass_blocks; // tile-pass grid (grid-stride over listed tiles)
 };
 
@@ -97,9 +98,10 @@ struct AdjointParams {
 };
 hipError_t launch_adjoint_step(int coll, const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
                                int general_blocks, hipStream_t s);
-hipError_t launch_wall_tangent(const uint32_t* tri_of_cell, const uint32_t* tri, const float4* pos,
-                               const float4* vel, const float* dpoints, const float* times, float4* dpos,
-                               float4* dvel, float4* dW, const MeshParams& m, int K, hipStream_t s);
+hipError_t launch_wall_tangent(const uint32_t* list, const uint32_t* count, int blocks, const uint32_t* tri_of_cell,
+                               const uint32_t* tri, const float4* pos, const float4* vel, const float* dpoints,
+                               const float* times, float4* dpos, float4* dvel, float4* dW, const MeshParams& m, int K,
+                               hipStream_t s);
 
 // Rigid body (FP64): integration after a step's loads and the safety monitor.
 struct BodyParams {
