@@ -197,12 +197,14 @@ bf_status bf_read_load_history(bf_context* ctx, double* out, uint32_t max_record
 bf_status bf_link_census(bf_context* ctx, uint64_t counts[21]);
 
 /* ---- measurement ---- */
-/* When enabled, every step records HIP events on the context's stream around
- * the geometry, fast-fluid and general-fluid+reduction launches. */
+/* enabled = 1: every step records HIP events around the fast-fluid launch on
+ * its stream; 2: also around the general-fluid launch and the next step's
+ * geometry (each event pair adds latency to the step); 0: off. */
 bf_status bf_set_profiling(bf_context* ctx, int enabled);
-/* {geometry ms, fast fluid ms, general fluid + reduction ms, steps profiled,
- * tiles on the general path in the last step, total tiles (64 x 4 cells)}
- * summed over the steps profiled since the last reset; resets the counters. */
+/* {geometry ms, fast fluid ms, general fluid ms (0 unless level 2), steps
+ * profiled, tiles on the general path in the last step, total tiles (64 x 4
+ * cells)} summed over the steps profiled since the last reset; resets the
+ * counters. */
 bf_status bf_read_profile(bf_context* ctx, double out[6]);
 
 /* Device copy bandwidth on the current device: a plain 16-byte-per-lane copy

@@ -380,8 +380,12 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_link_census(self._ctx, _ptr(out)))
         return out[:19].copy(), int(out[19]), int(out[20])
 
-    def set_profiling(self, enabled: bool):
-        self._check(self._lib.bf_set_profiling(self._ctx, int(bool(enabled))))
+    def set_profiling(self, enabled=True, level=None):
+        """Per-step HIP events: level 1 around the fast-fluid kernel only, 2
+        also around the general kernel and the geometry (default 2 when
+        enabled)."""
+        lvl = 0 if not enabled else (2 if level is None else int(level))
+        self._check(self._lib.bf_set_profiling(self._ctx, lvl))
 
     def read_profile(self):
         out = np.zeros(6, np.float64)

@@ -64,15 +64,20 @@ __global__ __launch_bounds__(256) void adjoint_fast(AdjointBuffers b, StepParams
     const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
     const int y = y0 + (int)(threadIdx.x / TILE_X);
     const size_t n = p.g.n;
-    const uint32_t gid = flat(p.g, x, y, z);
+    const uint32_t row = (uint32_t)y + (uint32_t)p.g.ny * (uint32_t)z;
+    const uint32_t gid = (uint32_t)x + (uint32_t)p.g.nx * row;
     const Sources s = sources(p, x, y, z);
     float g[19], lp[19], mu[19];
     uint32_t src[19];
     bool far[19];
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        src[q] = source_of<q>(s, p.g, x, y, z, far[q]);
-        const float v = b.f_prev[(size_t)q * n + (far[q] ? gid : src[q])];   // unconditional load
+        int xs;
+        uint32_t rows;
+        source_xr<q>(s, p.g, x, y, z, xs, rows, far[q]);
+        src[q] = (uint32_t)xs + (uint32_t)p.g.nx * rows;
+        // unconditional load (a far-field source reads this cell)
+        const float v = b.f_prev[far[q] ? pop_index<STORE_FP32>(p.g, q, x, row) : pop_index<STORE_FP32>(p.g, q, xs, rows)];
         g[q] = far[q] ? p.far_eq[q] : v;
         lp[q] = b.lam_in[(size_t)q * n + gid];
     });
@@ -96,7 +101,8 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
                                                 float bz, float* __restrict__ nu_cell) {
     const Grid& g = p.g;
     const size_t n = g.n;
-    const uint32_t gid = flat(g, x, y, z);
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
+    const uint32_t gid = (uint32_t)x + (uint32_t)g.nx * row;
     const bool is_solid = b.solid_cur[gid] != 0;
     const bool was_solid = b.solid_prev[gid] != 0;
     const Sources s = sources(p, x, y, z);
@@ -153,7 +159,11 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
         }
         const uint8_t part = b.solid_cur[src[q]];
         if (part == 0) {
-            gq[q] = b.f_prev[(size_t)q * n + src[q]];
+            int xs;
+            uint32_t rows;
+            bool fq;
+            source_xr<q>(s, g, x, y, z, xs, rows, fq);
+            gq[q] = b.f_prev[pop_index<STORE_FP32>(g, q, xs, rows)];
             return;
         }
         link[q] = true;
@@ -163,9 +173,11 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
         const bool fin_ok = inside(g, fx, fy, fz);
         const uint32_t fid = fin_ok ? flat(g, fx, fy, fz) : gid;
         const bool farther_ok = fin_ok && b.solid_cur[fid] == 0;
-        const float refl = b.f_prev[(size_t)opp(q) * n + gid];
-        const float farther = farther_ok ? b.f_prev[(size_t)opp(q) * n + fid] : 0.0f;
-        const float pin = b.f_prev[(size_t)q * n + gid];
+        const float refl = b.f_prev[pop_index<STORE_FP32>(g, opp(q), x, row)];
+        const float farther =
+            farther_ok ? b.f_prev[pop_index<STORE_FP32>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)]
+                       : 0.0f;
+        const float pin = b.f_prev[pop_index<STORE_FP32>(g, q, x, row)];
         const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
                                                            p.wall_rule, farther_ok, refl, farther, pin, wgv, wsv);
         gq[q] = d.value;
@@ -240,7 +252,8 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
                                                  float bz, const float* __restrict__ nu_cell, double (&acc)[8]) {
     const Grid& g = p.g;
     const size_t n = g.n;
-    const uint32_t gid = flat(g, x, y, z);
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
+    const uint32_t gid = (uint32_t)x + (uint32_t)g.nx * row;
     const bool is_solid = b.solid_cur[gid] != 0;
     const bool was_solid = b.solid_prev[gid] != 0;
     const bool topology = p.accumulate_loads && p.load_component != 1 && p.force_mode == 6;
@@ -332,7 +345,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
         okbits |= (uint32_t)inside(g, x + cx(q), y + cy(q), z + cz(q)) << q;
         part[q] = b.solid_cur[src_of(qc)];
         fpart[q] = b.solid_cur[fid_of(qc)];
-        fown[q] = b.f_prev[(size_t)q * n + gid];
+        fown[q] = b.f_prev[pop_index<STORE_FP32>(g, q, x, row)];
         lam[q] = b.lam_out[(size_t)q * n + gid];
         nuv[q] = nu_cell[q * 256];
     });
@@ -370,7 +383,10 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
             const uint32_t src = lk ? src_of(qc) : gid;
             srcs[q - q0] = src;
             ws[q - q0] = b.wall[src];
-            fo[q - q0] = b.f_prev[(size_t)opp(q) * n + (lk ? fid_of(qc) : gid)];
+            const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+            fo[q - q0] = b.f_prev[lk && inside(g, fx, fy, fz)
+                                      ? pop_index<STORE_FP32>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)
+                                      : pop_index<STORE_FP32>(g, opp(q), x, row)];
         });
         asm volatile("" : "+v"(ws[0].x), "+v"(ws[0].y), "+v"(ws[0].z), "+v"(ws[0].w), "+v"(ws[1].x),
                      "+v"(ws[1].y), "+v"(ws[1].z), "+v"(ws[1].w), "+v"(ws[2].x), "+v"(ws[2].y), "+v"(ws[2].z),
@@ -451,7 +467,8 @@ __global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, Ste
         const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
         const int y = y0 + (int)(threadIdx.x / TILE_X);
         if (x >= g.nx || y >= g.ny) continue;
-        const uint32_t gid = flat(g, x, y, z);
+        const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
+        const uint32_t gid = (uint32_t)x + (uint32_t)g.nx * row;
         const float* nu_cell = b.nu + (size_t)i * 19 * 256 + threadIdx.x;
         if (b.solid_cur[gid] != 0) continue;
         if (b.solid_prev[gid] != 0) {
@@ -488,8 +505,10 @@ __global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, Ste
             const float4 ws4 = b.wall[src];
             const float wsv[4] = {ws4.x, ws4.y, ws4.z, ws4.w};
             const adj::LinkD<float> d = adj::link_value<float>(
-                cx(q), cy(q), cz(q), weight<float>(q), p.far_rho, p.wall_rule, true, b.f_prev[(size_t)opp(q) * n + gid],
-                b.f_prev[(size_t)opp(q) * n + fid], b.f_prev[(size_t)q * n + gid], wgv, wsv);
+                cx(q), cy(q), cz(q), weight<float>(q), p.far_rho, p.wall_rule, true,
+                b.f_prev[pop_index<STORE_FP32>(g, opp(q), x, row)],
+                b.f_prev[pop_index<STORE_FP32>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)],
+                b.f_prev[pop_index<STORE_FP32>(g, q, x, row)], wgv, wsv);
             if (d.uses_farther) b.lam_out[(size_t)opp(q) * n + fid] += nu_cell[q * 256] * d.d_farther;
         });
     }

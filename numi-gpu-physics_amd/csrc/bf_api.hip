@@ -82,6 +82,7 @@ struct bf_context {
     hipStream_t sb = nullptr;             // general fluid, reductions, geometry, everything else
     hipEvent_t ev_fast[2] = {nullptr, nullptr};
     hipEvent_t ev_ready[2] = {nullptr, nullptr};
+    hipEvent_t ev_general[2] = {nullptr, nullptr};   // general(s) done (stream A)
     int device = 0;
     DeviceAlloc alloc;
     DeviceAlloc mesh_alloc;
@@ -104,7 +105,7 @@ struct bf_context {
     unsigned long long* census = nullptr;
     float* density = nullptr;
     float4* velocity = nullptr;
-    float* staging = nullptr;             // FP32 conversion buffer for reduced storage
+    float* staging = nullptr;             // host-layout FP32 buffer for population transfers
     // geometry
     int source = BF_GEOMETRY_STATIC;
     bool static_prev_differs = false;
@@ -182,6 +183,7 @@ struct bf_context {
     std::string error;
     // profiling
     bool profiling = false;
+    int profiling_level = 0;              // 1: fast kernel only; 2: also general and geometry
     std::vector<hipEvent_t> events;
     uint32_t profiled = 0;
     int general_blocks = 1024;
@@ -320,11 +322,12 @@ MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     m.cull_cells = exact ? INFINITY : c->m_half / c->cfg.cell_size + 1.5f;
     m.move_with_body = c->move_mesh ? 1 : 0;
     m.raster_blocks = c->raster_blocks;
-    // enough batches for two waves per CU, at most 64 triangles each: one
-    // gather round trip then serves many triangles of a large mesh, while a
-    // small mesh of large triangles spreads one triangle per wave
+    // enough batches for two waves per CU, at most 32 triangles each (the
+    // fastest of 8 / 16 / 32 / 64 on the 40k-triangle bird): one gather round
+    // trip serves many triangles of a large mesh, while a small mesh of large
+    // triangles spreads one triangle per wave
     m.raster_batch = c->raster_batch ? c->raster_batch
-                                     : std::min(64u, std::max(1u, c->m_triangles / (2u * c->compute_units)));
+                                     : std::min(32u, std::max(1u, c->m_triangles / (2u * c->compute_units)));
     // a mesh of fewer triangles than that splits each triangle's box among
     // several wavefronts
     {
@@ -598,7 +601,8 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     }
     for (int i = 0; i < 2; ++i) {
         if (hipEventCreateWithFlags(&c->ev_fast[i], hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&c->ev_ready[i], hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&c->ev_ready[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_general[i], hipEventDisableTiming) != hipSuccess) {
             bf_destroy(c);
             return BF_ERROR_DEVICE_UNAVAILABLE;
         }
@@ -669,6 +673,7 @@ void bf_destroy(bf_context* c) {
     for (int i = 0; i < 2; ++i) {
         if (c->ev_fast[i]) (void)hipEventDestroy(c->ev_fast[i]);
         if (c->ev_ready[i]) (void)hipEventDestroy(c->ev_ready[i]);
+        if (c->ev_general[i]) (void)hipEventDestroy(c->ev_general[i]);
         if (c->adj.ev_geo[i]) (void)hipEventDestroy(c->adj.ev_geo[i]);
         if (c->adj.ev_done[i]) (void)hipEventDestroy(c->adj.ev_done[i]);
         if (c->adj.ev_tan[i]) (void)hipEventDestroy(c->adj.ev_tan[i]);
@@ -691,7 +696,7 @@ bf_status bf_init_equilibrium(bf_context* c) {
     BF_ENTER(c);
     const StepGeometry sg = geometry_of(c, c->step);
     hipError_t e = launch_init_equilibrium((int)c->cfg.storage, c->f[c->cur], sg.mask_cur, sg.wall,
-                                           c->cfg.far_velocity, c->g.n, c->sb);
+                                           c->cfg.far_velocity, c->g, c->sb);
     if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "initialize populations");
     return BF_OK;
@@ -705,11 +710,16 @@ bf_status bf_init_shear_wave(bf_context* c, float amplitude) {
     return BF_OK;
 }
 
+// Host-layout FP32 staging for population transfers (every storage: the
+// device layout is not the host's direction-major one).
 static bf_status ensure_staging(bf_context* c) {
-    if (c->staging || c->cfg.storage == BF_STORAGE_FP32) return BF_OK;
+    if (c->staging) return BF_OK;
     hipError_t e = hipSuccess;
     c->staging = (float*)c->alloc.take(e, 19ull * c->g.n * 4);
-    if (e != hipSuccess) return fail(c, BF_ERROR_ALLOCATION_FAILED, "population staging buffer");
+    if (e != hipSuccess || !c->staging) {
+        c->staging = nullptr;
+        return fail(c, BF_ERROR_ALLOCATION_FAILED, "population staging buffer");
+    }
     return BF_OK;
 }
 
@@ -717,15 +727,12 @@ bf_status bf_write_populations(bf_context* c, const float* host, uint64_t count)
     BF_ENTER(c);
     if (check_host_array(c, host, "populations")) return BF_ERROR_INVALID_ARGUMENT;
     if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population count must be 19 * N");
-    hipError_t e;
-    if (c->cfg.storage == BF_STORAGE_FP32) {
-        e = hipMemcpyAsync(c->f[c->cur], host, count * 4, hipMemcpyHostToDevice, c->sb);
-    } else {
-        if (bf_status st = ensure_staging(c)) return st;
-        e = hipMemcpyAsync(c->staging, host, count * 4, hipMemcpyHostToDevice, c->sb);
-        if (e == hipSuccess)
-            e = launch_convert_populations((int)c->cfg.storage, true, c->staging, c->f[c->cur], count, c->g.n, c->sb);
-    }
+    // through the staging buffer: the device layout differs from the host's
+    // direction-major one for every storage
+    if (bf_status st = ensure_staging(c)) return st;
+    hipError_t e = hipMemcpyAsync(c->staging, host, count * 4, hipMemcpyHostToDevice, c->sb);
+    if (e == hipSuccess)
+        e = launch_convert_populations((int)c->cfg.storage, true, c->staging, c->f[c->cur], count, c->g, c->sb);
     if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "write populations");
     return BF_OK;
@@ -735,14 +742,9 @@ bf_status bf_read_populations(bf_context* c, float* host, uint64_t count) {
     BF_ENTER(c);
     if (check_host_array(c, host, "populations")) return BF_ERROR_INVALID_ARGUMENT;
     if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population count must be 19 * N");
-    hipError_t e;
-    if (c->cfg.storage == BF_STORAGE_FP32) {
-        e = hipMemcpyAsync(host, c->f[c->cur], count * 4, hipMemcpyDeviceToHost, c->sb);
-    } else {
-        if (bf_status st = ensure_staging(c)) return st;
-        e = launch_convert_populations((int)c->cfg.storage, false, c->f[c->cur], c->staging, count, c->g.n, c->sb);
-        if (e == hipSuccess) e = hipMemcpyAsync(host, c->staging, count * 4, hipMemcpyDeviceToHost, c->sb);
-    }
+    if (bf_status st = ensure_staging(c)) return st;
+    hipError_t e = launch_convert_populations((int)c->cfg.storage, false, c->f[c->cur], c->staging, count, c->g, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, c->staging, count * 4, hipMemcpyDeviceToHost, c->sb);
     if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "read populations");
     return BF_OK;
@@ -753,7 +755,7 @@ bf_status bf_read_fields(bf_context* c, float* density, float* velocity3) {
     if (check_host_array(c, density, "density") || check_host_array(c, velocity3, "velocity"))
         return BF_ERROR_INVALID_ARGUMENT;
     float* dv = (float*)c->velocity;   // 16 B per cell; holds 3 floats per cell here
-    hipError_t e = launch_extract_fields((int)c->cfg.storage, c->f[c->cur], c->density, dv, c->g.n, c->sb);
+    hipError_t e = launch_extract_fields((int)c->cfg.storage, c->f[c->cur], c->density, dv, c->g, c->sb);
     if (e == hipSuccess) e = hipMemcpyAsync(density, c->density, 4ull * c->g.n, hipMemcpyDeviceToHost, c->sb);
     if (e == hipSuccess) e = hipMemcpyAsync(velocity3, dv, 12ull * c->g.n, hipMemcpyDeviceToHost, c->sb);
     if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
@@ -943,6 +945,7 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
     for (uint32_t i = 0; i < steps; ++i) {
         const int64_t s = c->step + 1;
         const bool prof = c->profiling && c->profiled < PROFILE_CAPACITY;
+        const bool prof_all = prof && c->profiling_level >= 2;
         hipEvent_t* pe = nullptr;
         if (prof) {
             while (c->events.size() < (size_t)(c->profiled + 1) * PROFILE_EVENTS) {
@@ -959,9 +962,10 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         // the tile list of step s (event L[s], stream B) is built.  Both read
         // f_in, which the previous step finished writing on this stream.
         e = hipStreamWaitEvent(c->sa, c->ev_ready[par2(s)], 0);
-        if (prof && e == hipSuccess) e = hipEventRecord(pe[4], c->sa);
+        if (prof_all && e == hipSuccess) e = hipEventRecord(pe[4], c->sa);
         if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, p, c->general_blocks, c->sa);
-        if (prof && e == hipSuccess) e = hipEventRecord(pe[5], c->sa);
+        if (prof_all && e == hipSuccess) e = hipEventRecord(pe[5], c->sa);
+        if (e == hipSuccess) e = hipEventRecord(c->ev_general[par2(s)], c->sa);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[0], c->sa);
         if (e == hipSuccess) e = launch_fluid_fast(arith, storage, coll, fb, p, c->sa);
         if (prof && e == hipSuccess) e = hipEventRecord(pe[1], c->sa);
@@ -977,15 +981,20 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         // needs the pose after step s, so its geometry follows it too.
         const bool coupled = c->free_flight && c->move_mesh && c->source == BF_GEOMETRY_MESH;
         auto next_geometry = [&]() -> bf_status {
-            if (prof) e = hipEventRecord(pe[2], c->sb);
+            if (prof_all) e = hipEventRecord(pe[2], c->sb);
             if (e == hipSuccess && c->source != BF_GEOMETRY_STATIC) {
                 if (bf_status r = enqueue_geometry(c, s + 1, false, false)) return r;
             }
             if (bf_status r = enqueue_tile_list(c, s + 1)) return r;
-            if (prof) e = hipEventRecord(pe[3], c->sb);
+            if (prof_all) e = hipEventRecord(pe[3], c->sb);
             return e == hipSuccess ? BF_OK : hip_fail(c, e, "geometry launch");
         };
         if (!coupled) {
+            // after general(s): its latency-bound wavefronts would otherwise
+            // share the machine with the raster; the geometry then overlaps
+            // fast(s) only
+            e = hipStreamWaitEvent(c->sb, c->ev_general[par2(s)], 0);
+            if (e != hipSuccess) return hip_fail(c, e, "event wait");
             if (bf_status r = next_geometry()) return r;
         }
         if (!c->free_flight) e = hipEventRecord(c->ev_ready[par2(s + 1)], c->sb);
@@ -1610,7 +1619,9 @@ bf_status bf_link_census(bf_context* c, uint64_t counts[21]) {
 
 bf_status bf_set_profiling(bf_context* c, int enabled) {
     BF_ENTER(c);
+    if (enabled < 0 || enabled > 2) return fail(c, BF_ERROR_INVALID_ARGUMENT, "profiling level is 0, 1 or 2");
     c->profiling = enabled != 0;
+    c->profiling_level = enabled;
     c->profiled = 0;
     return BF_OK;
 }
@@ -1623,8 +1634,10 @@ bf_status bf_read_profile(bf_context* c, double out[6]) {
         float a = 0, b = 0, d = 0;
         const hipEvent_t* ev = &c->events[(size_t)i * PROFILE_EVENTS];
         (void)hipEventElapsedTime(&a, ev[0], ev[1]);   // fast kernel (stream A)
-        (void)hipEventElapsedTime(&d, ev[2], ev[3]);   // next step's geometry + tile list (stream B)
-        (void)hipEventElapsedTime(&b, ev[4], ev[5]);   // general kernel (stream A, before fast)
+        if (c->profiling_level >= 2) {
+            (void)hipEventElapsedTime(&d, ev[2], ev[3]);   // next step's geometry + tile list (stream B)
+            (void)hipEventElapsedTime(&b, ev[4], ev[5]);   // general kernel (stream A, before fast)
+        }
         fast += a;
         general += b;
         geo += d;

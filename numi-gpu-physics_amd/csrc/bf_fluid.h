@@ -476,14 +476,25 @@ __device__ __forceinline__ Sources sources(const StepParams& p, int x, int y, in
     return s;
 }
 
+// Source cell of direction q: its x and row (y + ny z), and whether it lies
+// beyond a far-field face.
 template <int q>
-__device__ __forceinline__ uint32_t source_of(const Sources& s, const Grid& g, int x, int y, int z, bool& far) {
-    const int xs = cx(q) > 0 ? s.xs_p : (cx(q) < 0 ? s.xs_m : x);
+__device__ __forceinline__ void source_xr(const Sources& s, const Grid& g, int x, int y, int z, int& xs,
+                                          uint32_t& rows, bool& far) {
+    xs = cx(q) > 0 ? s.xs_p : (cx(q) < 0 ? s.xs_m : x);
     const int ys = cy(q) > 0 ? s.ys_p : (cy(q) < 0 ? s.ys_m : y);
     const int zs = cz(q) > 0 ? s.zs_p : (cz(q) < 0 ? s.zs_m : z);
     far = (cx(q) > 0 && s.xm_far) || (cx(q) < 0 && s.xp_far) || (cy(q) > 0 && s.ym_far) || (cy(q) < 0 && s.yp_far) ||
           (cz(q) > 0 && s.zm_far) || (cz(q) < 0 && s.zp_far);
-    return (uint32_t)xs + (uint32_t)g.nx * ((uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs);
+    rows = (uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs;
+}
+
+template <int q>
+__device__ __forceinline__ uint32_t source_of(const Sources& s, const Grid& g, int x, int y, int z, bool& far) {
+    int xs;
+    uint32_t rows;
+    source_xr<q>(s, g, x, y, z, xs, rows, far);
+    return (uint32_t)xs + (uint32_t)g.nx * rows;
 }
 
 // (x, q) is pulled by nobody when x + c_q leaves the domain through a
@@ -495,7 +506,7 @@ __device__ __forceinline__ bool orphan(const Sources& s) {
 }
 
 template <int S>
-__device__ __forceinline__ void write_post(typename Store<S>::T* __restrict__ fout, size_t n, uint32_t gid,
+__device__ __forceinline__ void write_post(typename Store<S>::T* __restrict__ fout, int x, uint32_t row,
                                            const float (&post)[19], float sponge, const StepParams& p,
                                            float& crho, float& cmx, float& cmy, float& cmz) {
     crho = 0.0f;
@@ -506,7 +517,7 @@ __device__ __forceinline__ void write_post(typename Store<S>::T* __restrict__ fo
         constexpr int q = decltype(qc)::value;
         float v = post[q];
         if (sponge > 0.0f) v = v + (p.far_eq[q] - v) * sponge;
-        Store<S>::store(fout, (size_t)q * n + gid, q, v);
+        Store<S>::store(fout, pop_index<S>(p.g, q, x, row), q, v);
         crho = crho + v;
         if constexpr (cx(q) > 0) cmx = cmx + v;
         if constexpr (cx(q) < 0) cmx = cmx - v;
@@ -546,7 +557,6 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
     const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
     const int y = y0 + (int)(threadIdx.x / TILE_X);
     const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
-    const size_t n = p.g.n;
 
     bool xm_far = false, xp_far = false, ym_far = false, yp_far = false, zm_far = false, zp_far = false;
     // source coordinates for c = +1 (s = v - 1) and c = -1 (s = v + 1)
@@ -557,7 +567,7 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
     const int zs_p = shift_axis(z, 1, nz, p.bc[2][0], p.bc[2][1], zm_far);
     const int zs_m = shift_axis(z, -1, nz, p.bc[2][0], p.bc[2][1], zp_far);
 
-    const uint32_t gid_self = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+    const uint32_t row_self = (uint32_t)y + (uint32_t)ny * (uint32_t)z;
     float f[19];
     bool far_q[19];
     for_q(AllQ{}, [&](auto qc) {
@@ -567,10 +577,10 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
         const int zs = cz(q) > 0 ? zs_p : (cz(q) < 0 ? zs_m : z);
         const bool far = (cx(q) > 0 && xm_far) || (cx(q) < 0 && xp_far) || (cy(q) > 0 && ym_far)
             || (cy(q) < 0 && yp_far) || (cz(q) > 0 && zm_far) || (cz(q) < 0 && zp_far);
-        const uint32_t src = (uint32_t)xs + (uint32_t)nx * ((uint32_t)ys + (uint32_t)ny * (uint32_t)zs);
+        const uint32_t row_src = (uint32_t)ys + (uint32_t)ny * (uint32_t)zs;
         // unconditional load: a far-field source reads an in-domain cell and
         // discards it
-        f[q] = Store<S>::load(fin, (size_t)q * n + (far ? gid_self : src), q);
+        f[q] = Store<S>::load(fin, far ? pop_index<S>(p.g, q, x, row_self) : pop_index<S>(p.g, q, xs, row_src), q);
         far_q[q] = far;
     });
     // One empty asm over all 19 loaded values: every load is issued, without
@@ -596,9 +606,9 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
         const float ux = mx / rho, uy = my / rho, uz = mz / rho;
         collide<COLL>(f, rho, ux, uy, uz, p.omega_plus, p.omega_minus, post, act, restr);
     }
-    const uint32_t gid = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+    const uint32_t gid = (uint32_t)x + (uint32_t)nx * row_self;
     float crho, cmx, cmy, cmz;
-    write_post<S>(fout, n, gid, post, 0.0f, p, crho, cmx, cmy, cmz);
+    write_post<S>(fout, x, row_self, post, 0.0f, p, crho, cmx, cmy, cmz);
     if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
     if constexpr (COLL != COLL_TRT) {
         // Limiter activations of fast cells: an exact count and the maximum
@@ -649,8 +659,8 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                              float4* __restrict__ velocity, const StepParams& p, int x, int y, int z,
                              float bx, float by, float bz, CellLoad& L) {
     const Grid& g = p.g;
-    const size_t n = g.n;
-    const uint32_t gid = flat(g, x, y, z);
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
+    const uint32_t gid = (uint32_t)x + (uint32_t)g.nx * row;
     const float wx = p.origin[0] + ((float)x + 0.5f) * p.dx;
     const float wy = p.origin[1] + ((float)y + 0.5f) * p.dx;
     const float wz = p.origin[2] + ((float)z + 0.5f) * p.dx;
@@ -667,13 +677,13 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
         const float4 wv = wall[gid];
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            Store<S>::store(fout, (size_t)q * n + gid, q, equilibrium<float>(q, 1.0f, wv.x, wv.y, wv.z));
+            Store<S>::store(fout, pop_index<S>(g, q, x, row), q, equilibrium<float>(q, 1.0f, wv.x, wv.y, wv.z));
         });
         if (p.accumulate_loads && !was_solid && include_topology) {
             float prho = 0.0f, pmx = 0.0f, pmy = 0.0f, pmz = 0.0f;
             for_q(AllQ{}, [&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                float v = Store<S>::load(fin, (size_t)q * n + gid, q);
+                float v = Store<S>::load(fin, pop_index<S>(g, q, x, row), q);
                 prho = prho + v;
                 pmx = pmx + v * (float)cx(q);
                 pmy = pmy + v * (float)cy(q);
@@ -720,6 +730,13 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                 const int bx = x + cx(q), by = y + cy(q), bz = z + cz(q);
                 return inside(g, bx, by, bz) ? flat(g, bx, by, bz) : gid;
             };
+            // population opp(q) of that neighbour (of this cell outside the domain)
+            auto nb_pop = [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const int bx = x + cx(q), by = y + cy(q), bz = z + cz(q);
+                return inside(g, bx, by, bz) ? pop_index<S>(g, opp(q), bx, (uint32_t)by + (uint32_t)g.ny * (uint32_t)bz)
+                                             : pop_index<S>(g, opp(q), x, row);
+            };
             uint32_t mk[19];
             uint32_t inbits = 0;
             for_q(AllQ{}, [&](auto qc) {
@@ -732,8 +749,8 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
             float own[19], far[19];
             for_q(AllQ{}, [&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                own[q] = Store<S>::load(fin, (size_t)q * n + gid, q);
-                far[q] = Store<S>::load(fin, (size_t)opp(q) * n + nb_of(qc), opp(q));
+                own[q] = Store<S>::load(fin, pop_index<S>(g, q, x, row), q);
+                far[q] = Store<S>::load(fin, nb_pop(qc), opp(q));
             });
             asm volatile("" : BF_ASM19(own), BF_ASM19(far));
             for_q(AllQ{}, [&](auto qc) {
@@ -782,6 +799,13 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
             const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
             return inside(g, fx, fy, fz) ? flat(g, fx, fy, fz) : gid;
         };
+        // population opp(q) on the far side (of this cell outside the domain)
+        auto far_side_pop = [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
+            return inside(g, fx, fy, fz) ? pop_index<S>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)
+                                         : pop_index<S>(g, opp(q), x, row);
+        };
         uint32_t partq[19], fpart[19];
         float fsrc[19], fown[19];
         uint32_t farbits = 0, okbits = 0;
@@ -794,8 +818,12 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
             const uint32_t src = src_of(qc), fid = far_side_of(qc);
             partq[q] = solid_cur[src];
             fpart[q] = solid_cur[fid];
-            fsrc[q] = Store<S>::load(fin, (size_t)q * n + src, q);
-            fown[q] = Store<S>::load(fin, (size_t)q * n + gid, q);
+            int xs;
+            uint32_t rows;
+            bool farq;
+            source_xr<q>(sq, g, x, y, z, xs, rows, farq);
+            fsrc[q] = Store<S>::load(fin, farq ? pop_index<S>(g, q, x, row) : pop_index<S>(g, q, xs, rows), q);
+            fown[q] = Store<S>::load(fin, pop_index<S>(g, q, x, row), q);
         });
         asm volatile("" : BF_ASM19(fsrc), BF_ASM19(fown), BF_ASM19(partq), BF_ASM19(fpart));
         // links; far sides that force the halfway fallback; links whose part
@@ -828,7 +856,7 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                     if constexpr (q != 0) {
                         const bool lk = (linkbits >> q) & 1u;
                         wsg[q - q0] = wall[lk ? src_of(qc) : gid];
-                        fog[q - q0] = Store<S>::load(fin, (size_t)opp(q) * n + (lk ? far_side_of(qc) : gid), opp(q));
+                        fog[q - q0] = Store<S>::load(fin, lk ? far_side_pop(qc) : pop_index<S>(g, opp(q), x, row), opp(q));
                     }
                 });
                 asm volatile("" : "+v"(wsg[0].x), "+v"(wsg[0].y), "+v"(wsg[0].z), "+v"(wsg[0].w),
@@ -959,7 +987,7 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
         collide<COLL>(f, rho, ux, uy, uz, p.omega_plus, p.omega_minus, post, act, restr);
     }
     float crho, cmx, cmy, cmz;
-    write_post<S>(fout, n, gid, post, sponge, p, crho, cmx, cmy, cmz);
+    write_post<S>(fout, x, row, post, sponge, p, crho, cmx, cmy, cmz);
     if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
     if (p.accumulate_loads) {
         L.fx = flx * fscale;

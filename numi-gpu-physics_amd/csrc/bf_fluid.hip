@@ -79,9 +79,11 @@ __global__ __launch_bounds__(256) void reduce_loads_final(const double* __restri
 // ---------------------------------------------------------------------------
 template <int S>
 __global__ void init_equilibrium(typename Store<S>::T* __restrict__ f, const uint8_t* __restrict__ solid,
-                                 const float4* __restrict__ wall, float fux, float fuy, float fuz, uint32_t n) {
+                                 const float4* __restrict__ wall, float fux, float fuy, float fuz, Grid g) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n) return;
+    if (gid >= g.n) return;
+    const uint32_t row = gid / (uint32_t)g.nx;
+    const int x = (int)(gid - row * (uint32_t)g.nx);
     float ux = fux, uy = fuy, uz = fuz;
     if (solid[gid] != 0) {
         const float4 w = wall[gid];
@@ -91,7 +93,7 @@ __global__ void init_equilibrium(typename Store<S>::T* __restrict__ f, const uin
     }
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        Store<S>::store(f, (size_t)q * n + gid, q, equilibrium<float>(q, 1.0f, ux, uy, uz));
+        Store<S>::store(f, pop_index<S>(g, q, x, row), q, equilibrium<float>(q, 1.0f, ux, uy, uz));
     });
 }
 
@@ -101,24 +103,28 @@ template <int S>
 __global__ void init_shear_wave(typename Store<S>::T* __restrict__ f, float amplitude, Grid g) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= g.n) return;
-    const uint32_t y = (gid / (uint32_t)g.nx) % (uint32_t)g.ny;
+    const uint32_t row = gid / (uint32_t)g.nx;
+    const int x = (int)(gid - row * (uint32_t)g.nx);
+    const uint32_t y = row % (uint32_t)g.ny;
     const float phase = 2.0f * 3.14159265358979323846f * (float)y / (float)g.ny;
     const float ux = amplitude * (float)sin((double)phase);
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        Store<S>::store(f, (size_t)q * g.n + gid, q, equilibrium<float>(q, 1.0f, ux, 0.0f, 0.0f));
+        Store<S>::store(f, pop_index<S>(g, q, x, row), q, equilibrium<float>(q, 1.0f, ux, 0.0f, 0.0f));
     });
 }
 
 template <int S>
 __global__ void extract_fields(const typename Store<S>::T* __restrict__ f, float* __restrict__ density,
-                               float* __restrict__ velocity3, uint32_t n) {
+                               float* __restrict__ velocity3, Grid g) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n) return;
+    if (gid >= g.n) return;
+    const uint32_t row = gid / (uint32_t)g.nx;
+    const int x = (int)(gid - row * (uint32_t)g.nx);
     float v[19];
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        v[q] = Store<S>::load(f, (size_t)q * n + gid, q);
+        v[q] = Store<S>::load(f, pop_index<S>(g, q, x, row), q);
     });
     float rho, mx, my, mz;
     moments(v, rho, mx, my, mz);
@@ -129,20 +135,27 @@ __global__ void extract_fields(const typename Store<S>::T* __restrict__ f, float
     velocity3[3 * (size_t)gid + 2] = mz / rho;
 }
 
+// Host layout (direction-major, float) <-> device layout and storage.
 template <int S>
 __global__ void pack_populations(const float* __restrict__ src, typename Store<S>::T* __restrict__ dst, size_t count,
-                                 uint32_t n) {
+                                 Grid g) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    Store<S>::store(dst, i, (int)(i / n), src[i]);
+    const int q = (int)(i / g.n);
+    const uint32_t gid = (uint32_t)(i - (size_t)q * g.n);
+    const uint32_t row = gid / (uint32_t)g.nx;
+    Store<S>::store(dst, pop_index<S>(g, q, (int)(gid - row * (uint32_t)g.nx), row), q, src[i]);
 }
 
 template <int S>
 __global__ void unpack_populations(const typename Store<S>::T* __restrict__ src, float* __restrict__ dst,
-                                   size_t count, uint32_t n) {
+                                   size_t count, Grid g) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    dst[i] = Store<S>::load(src, i, (int)(i / n));
+    const int q = (int)(i / g.n);
+    const uint32_t gid = (uint32_t)(i - (size_t)q * g.n);
+    const uint32_t row = gid / (uint32_t)g.nx;
+    dst[i] = Store<S>::load(src, pop_index<S>(g, q, (int)(gid - row * (uint32_t)g.nx), row), q);
 }
 
 
@@ -182,19 +195,19 @@ hipError_t launch_build_general_list(const uint8_t* static_flags, const uint8_t*
 }
 
 template <int S>
-static hipError_t init_eq_s(void* f, const uint8_t* solid, const float4* wall, const float* fu, uint32_t n,
+static hipError_t init_eq_s(void* f, const uint8_t* solid, const float4* wall, const float* fu, const Grid& g,
                             hipStream_t s) {
-    hipLaunchKernelGGL(init_equilibrium<S>, dim3((n + 255) / 256), dim3(256), 0, s, (typename Store<S>::T*)f, solid,
-                       wall, fu[0], fu[1], fu[2], n);
+    hipLaunchKernelGGL(init_equilibrium<S>, dim3((g.n + 255) / 256), dim3(256), 0, s, (typename Store<S>::T*)f, solid,
+                       wall, fu[0], fu[1], fu[2], g);
     return hipGetLastError();
 }
 
 hipError_t launch_init_equilibrium(int storage, void* f, const uint8_t* solid, const float4* wall,
-                                   const float* far_u, uint32_t n, hipStream_t s) {
+                                   const float* far_u, const Grid& g, hipStream_t s) {
     switch (storage) {
-        case STORE_FP32: return init_eq_s<STORE_FP32>(f, solid, wall, far_u, n, s);
-        case STORE_FP16_OFFSET: return init_eq_s<STORE_FP16_OFFSET>(f, solid, wall, far_u, n, s);
-        case STORE_BF16_OFFSET: return init_eq_s<STORE_BF16_OFFSET>(f, solid, wall, far_u, n, s);
+        case STORE_FP32: return init_eq_s<STORE_FP32>(f, solid, wall, far_u, g, s);
+        case STORE_FP16_OFFSET: return init_eq_s<STORE_FP16_OFFSET>(f, solid, wall, far_u, g, s);
+        case STORE_BF16_OFFSET: return init_eq_s<STORE_BF16_OFFSET>(f, solid, wall, far_u, g, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -215,39 +228,39 @@ hipError_t launch_init_shear_wave(int storage, void* f, float amplitude, const G
 }
 
 template <int S>
-static hipError_t fields_s(const void* f, float* rho, float* u, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(extract_fields<S>, dim3((n + 255) / 256), dim3(256), 0, s, (const typename Store<S>::T*)f, rho,
-                       u, n);
+static hipError_t fields_s(const void* f, float* rho, float* u, const Grid& g, hipStream_t s) {
+    hipLaunchKernelGGL(extract_fields<S>, dim3((g.n + 255) / 256), dim3(256), 0, s, (const typename Store<S>::T*)f,
+                       rho, u, g);
     return hipGetLastError();
 }
 
-hipError_t launch_extract_fields(int storage, const void* f, float* rho, float* u, uint32_t n, hipStream_t s) {
+hipError_t launch_extract_fields(int storage, const void* f, float* rho, float* u, const Grid& g, hipStream_t s) {
     switch (storage) {
-        case STORE_FP32: return fields_s<STORE_FP32>(f, rho, u, n, s);
-        case STORE_FP16_OFFSET: return fields_s<STORE_FP16_OFFSET>(f, rho, u, n, s);
-        case STORE_BF16_OFFSET: return fields_s<STORE_BF16_OFFSET>(f, rho, u, n, s);
+        case STORE_FP32: return fields_s<STORE_FP32>(f, rho, u, g, s);
+        case STORE_FP16_OFFSET: return fields_s<STORE_FP16_OFFSET>(f, rho, u, g, s);
+        case STORE_BF16_OFFSET: return fields_s<STORE_BF16_OFFSET>(f, rho, u, g, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <int S>
-static hipError_t conv_s(bool pack, const void* src, void* dst, size_t count, uint32_t n, hipStream_t s) {
+static hipError_t conv_s(bool pack, const void* src, void* dst, size_t count, const Grid& g, hipStream_t s) {
     const unsigned blocks = (unsigned)((count + 255) / 256);
     if (pack)
         hipLaunchKernelGGL(pack_populations<S>, dim3(blocks), dim3(256), 0, s, (const float*)src,
-                           (typename Store<S>::T*)dst, count, n);
+                           (typename Store<S>::T*)dst, count, g);
     else
         hipLaunchKernelGGL(unpack_populations<S>, dim3(blocks), dim3(256), 0, s, (const typename Store<S>::T*)src,
-                           (float*)dst, count, n);
+                           (float*)dst, count, g);
     return hipGetLastError();
 }
 
-hipError_t launch_convert_populations(int storage, bool pack, const void* src, void* dst, size_t count, uint32_t n,
+hipError_t launch_convert_populations(int storage, bool pack, const void* src, void* dst, size_t count, const Grid& g,
                                       hipStream_t s) {
     switch (storage) {
-        case STORE_FP32: return conv_s<STORE_FP32>(pack, src, dst, count, n, s);
-        case STORE_FP16_OFFSET: return conv_s<STORE_FP16_OFFSET>(pack, src, dst, count, n, s);
-        case STORE_BF16_OFFSET: return conv_s<STORE_BF16_OFFSET>(pack, src, dst, count, n, s);
+        case STORE_FP32: return conv_s<STORE_FP32>(pack, src, dst, count, g, s);
+        case STORE_FP16_OFFSET: return conv_s<STORE_FP16_OFFSET>(pack, src, dst, count, g, s);
+        case STORE_BF16_OFFSET: return conv_s<STORE_BF16_OFFSET>(pack, src, dst, count, g, s);
         default: return hipErrorInvalidValue;
     }
 }

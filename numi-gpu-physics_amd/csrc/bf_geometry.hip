@@ -158,21 +158,28 @@ __global__ void fill_wall(float4* __restrict__ p, uint32_t n) {
 //     triangle's values broadcast from lane j (readlane, scalar registers);
 //  1. a conservative filter over the voxels of the AABB (cut to the bounding
 //     sphere's box when the band is finite): lanes take the (x, y) columns of
-//     the box and step through z; a cell can lie in the candidate band
+//     the box and step through z (through the sphere's chord of the column
+//     when the band is finite); a cell can lie in the candidate band
 //     (m.cull_cells) only if its distance to the triangle's plane and to the
 //     bounding sphere are within the band (both are lower bounds of the
 //     distance to the triangle; a margin of 1e-3 cells absorbs rounding);
 //  2. the exact closest point and key (the reference's arithmetic) for the
 //     survivors, compacted into LDS so that every lane has work;
-//  3. for cells within the band: a 64-bit atomicMin whose result is unused
-//     (the wave does not wait for it) and a byte store marking the cell's
-//     64x4 tile for this step's tile pass, skipped when the lane below marked
-//     the same tile.  A cell within the band always ends with a key, so
-//     marking on every attempt marks the same tiles as marking on a win.
+//  3. for cells within the band: the key is merged into a per-batch LDS
+//     table (cell -> minimum key; a batch's triangles are neighbours on the
+//     mesh and key largely the same cells); at the end of the batch each
+//     table entry goes to the grid as one 64-bit atomicMin whose result is
+//     unused (the wave does not wait for it), and marks the cell's 64x4 tile
+//     for this step's tile pass (a crowded table sends a key straight to the
+//     grid).  A cell within the band always ends with a key, so marking on
+//     every attempt marks the same tiles as marking on a win.
 // Keys are a minimum over (cell, triangle) pairs: the result does not depend
 // on the order of the atomics.  With m.cull_cells = +inf (the reference's
 // full AABB) the filter passes every cell of the AABB.
 constexpr int RASTER_CAP = 1024;
+constexpr int RASTER_TABLE_BITS = 10;
+constexpr uint32_t RASTER_TABLE = 1u << RASTER_TABLE_BITS;
+constexpr uint32_t RASTER_PROBES = 16;
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -188,7 +195,17 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
                                                   unsigned long long* __restrict__ keys,
                                                   uint8_t* __restrict__ tiles_new, MeshParams m) {
     __shared__ uint2 cand[RASTER_CAP];
+    // per-batch minimum key of each cell the batch's triangles key (open
+    // addressing, linear probing); flushed to the grid at the end of the batch
+    __shared__ unsigned long long tkey[RASTER_TABLE];
+    __shared__ uint32_t tcell[RASTER_TABLE];
+    __shared__ uint32_t ttile[RASTER_TABLE];
     const uint32_t lane = threadIdx.x;
+    for (uint32_t j = lane; j < RASTER_TABLE; j += 64) {
+        tkey[j] = EMPTY_KEY;
+        tcell[j] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
     const float dx = m.dx;
     const float guard = m.half_thickness + 2.0f * dx;
     const int n[3] = {m.g.nx, m.g.ny, m.g.nz};
@@ -235,12 +252,15 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
             int l = min(max((int)floorf(ls), 0), n[i] - 1);
             int h = min(max((int)ceilf(us), 0), n[i] - 1);
             if (filter) {
-                // cells of the bounding sphere, one cell of margin for rounding
+                // cells whose centre lies within R of the sphere's centre along
+                // this axis (the sphere test fails for every other cell: its
+                // squared distance is at least this axis' term); 0.01 cell of
+                // margin absorbs rounding
                 const float top = (float)n[i] + 2.0f;
                 const float sl = fminf(fmaxf(((cm[i] - R) - m.origin[i]) / dx - 0.5f, -2.0f), top);
                 const float su = fminf(fmaxf(((cm[i] + R) - m.origin[i]) / dx - 0.5f, -2.0f), top);
-                l = max(l, (int)floorf(sl) - 1);
-                h = min(h, (int)ceilf(su) + 1);
+                l = max(l, (int)floorf(sl - 0.01f));
+                h = min(h, (int)ceilf(su + 0.01f));
             }
             lo[i] = l;
             ext[i] = mine ? max(h - l + 1, 0) : 0;
@@ -258,15 +278,85 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
             const int ex = bcast(ext[0], j), ey = bcast(ext[1], j), ez = bcast(ext[2], j);
             const uint32_t cols = (uint32_t)ex * (uint32_t)ey;
             uint32_t count = 0;  // wave-uniform
+            // 2./3. the exact key for the compacted candidates
+            auto flush = [&]() {
+                __syncthreads();
+                for (uint32_t k = lane; k < count; k += 64) {
+                    const uint2 e = cand[k];
+                    const int cx = (int)(e.x & 0xFFFFu), cy = (int)(e.x >> 16), cz = (int)e.y;
+                    const float px = m.origin[0] + ((float)cx + 0.5f) * dx;
+                    const float py = m.origin[1] + ((float)cy + 0.5f) * dx;
+                    const float pz = m.origin[2] + ((float)cz + 0.5f) * dx;
+                    const Closest cp = closest_point(px, py, pz, ta, tb, tc);
+                    const float ddx = px - cp.px, ddy = py - cp.py, ddz = pz - cp.pz;
+                    const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
+                    if (dist_cells <= m.cull_cells) {
+                        const float scaled = roundf(dist_cells * 65536.0f);
+                        const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
+                        const unsigned long long key = ((unsigned long long)bin << 32) | (unsigned long long)t;
+                        const uint32_t g = (uint32_t)cx + (uint32_t)m.g.nx * ((uint32_t)cy + (uint32_t)m.g.ny * (uint32_t)cz);
+                        const uint32_t tile = tile_of(m.g, cx, cy, cz);
+                        const uint32_t h = (g * 2654435761u) >> (32 - RASTER_TABLE_BITS);
+                        bool done = false;
+#pragma unroll 1
+                        for (uint32_t probe = 0; probe < RASTER_PROBES && !done; ++probe) {
+                            const uint32_t slot = (h + probe) & (RASTER_TABLE - 1);
+                            const uint32_t prev = atomicCAS(&tcell[slot], 0xFFFFFFFFu, g);
+                            if (prev == 0xFFFFFFFFu) ttile[slot] = tile;
+                            if (prev == 0xFFFFFFFFu || prev == g) {
+                                (void)atomicMin(&tkey[slot], key);
+                                done = true;
+                            }
+                        }
+                        if (!done) {
+                            // table crowded: straight to the grid
+                            (void)__hip_atomic_fetch_min(&keys[g], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            tiles_new[tile] = 1;
+                        }
+                    }
+                }
+                __syncthreads();
+                count = 0;
+            };
             for (uint32_t c0 = blockIdx.y * 64u; c0 < cols; c0 += gridDim.y * 64u) {
                 const uint32_t col = c0 + lane;
                 const int y = ly + (int)(col / (uint32_t)max(ex, 1));
                 const int x = lx + (int)(col - (uint32_t)(y - ly) * (uint32_t)ex);
                 const float wx = m.origin[0] + ((float)x + 0.5f) * dx;
                 const float wy = m.origin[1] + ((float)y + 0.5f) * dx;
-                for (int zi = 0; zi < ez; ++zi) {
+                // this lane's z range: the whole box, or with a finite band the
+                // chord of the bounding sphere through the column (a cell whose
+                // (x, y) distance alone exceeds the sphere fails the sphere test
+                // for every z; the chord is widened by 1e-5 R^2 under the root
+                // and 0.01 cell for rounding)
+                int zlo = 0, zhi = ez - 1;
+                if (col >= cols) {
+                    zlo = 1 << 30;
+                    zhi = -1;
+                } else if (filter) {
+                    const float ddx = wx - tcx, ddy = wy - tcy;
+                    const float dxy2 = ddx * ddx + ddy * ddy;
+                    if (dxy2 > ts2) {
+                        zlo = 1 << 30;
+                        zhi = -1;
+                    } else {
+                        const float half = sqrtf((ts2 - dxy2) + 1.0e-5f * ts2);
+                        const float top = (float)n[2] + 2.0f;
+                        const float kl = fminf(fmaxf(((tcz - half) - m.origin[2]) / dx - 0.5f, -2.0f), top);
+                        const float kh = fminf(fmaxf(((tcz + half) - m.origin[2]) / dx - 0.5f, -2.0f), top);
+                        zlo = max(0, (int)floorf(kl - 0.01f) - lz);
+                        zhi = min(ez - 1, (int)ceilf(kh + 0.01f) - lz);
+                    }
+                }
+                int wlo = zlo, whi = zhi;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    wlo = min(wlo, __shfl_xor(wlo, off, 64));
+                    whi = max(whi, __shfl_xor(whi, off, 64));
+                }
+                for (int zi = wlo; zi <= whi; ++zi) {
                     const int z = lz + zi;
-                    bool pass = col < cols;
+                    bool pass = zi >= zlo && zi <= zhi;
                     if (pass && filter) {
                         const float wz = m.origin[2] + ((float)z + 0.5f) * dx;
                         const float plane = fabsf(tnx * (wx - ta.x) + tny * (wy - ta.y) + tnz * (wz - ta.z));
@@ -275,36 +365,24 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
                     const unsigned long long ballot = __ballot(pass);
                     if (pass) cand[count + lane_rank(ballot)] = make_uint2((uint32_t)x | ((uint32_t)y << 16), (uint32_t)z);
                     count += (uint32_t)__popcll(ballot);
-                    const bool last = zi + 1 == ez && c0 + gridDim.y * 64u >= cols;
-                    if (count > RASTER_CAP - 64 || (last && count > 0)) {
-                        __syncthreads();
-                        for (uint32_t k = lane; k < count; k += 64) {
-                            const uint2 e = cand[k];
-                            const int cx = (int)(e.x & 0xFFFFu), cy = (int)(e.x >> 16), cz = (int)e.y;
-                            const float px = m.origin[0] + ((float)cx + 0.5f) * dx;
-                            const float py = m.origin[1] + ((float)cy + 0.5f) * dx;
-                            const float pz = m.origin[2] + ((float)cz + 0.5f) * dx;
-                            const Closest cp = closest_point(px, py, pz, ta, tb, tc);
-                            const float ddx = px - cp.px, ddy = py - cp.py, ddz = pz - cp.pz;
-                            const float dist_cells = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) / dx;
-                            const bool in_band = dist_cells <= m.cull_cells;
-                            const uint32_t tile = in_band ? tile_of(m.g, cx, cy, cz) : 0xFFFFFFFFu;
-                            const uint32_t below = (uint32_t)__shfl_up((int)tile, 1);
-                            if (in_band) {
-                                const float scaled = roundf(dist_cells * 65536.0f);
-                                const uint32_t bin = scaled >= (float)0xFFFFF ? 0xFFFFFu : (uint32_t)scaled;
-                                const unsigned long long key = ((unsigned long long)bin << 32) | (unsigned long long)t;
-                                const uint32_t g = (uint32_t)cx + (uint32_t)m.g.nx * ((uint32_t)cy + (uint32_t)m.g.ny * (uint32_t)cz);
-                                (void)__hip_atomic_fetch_min(&keys[g], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                                if (lane == 0 || below != tile) tiles_new[tile] = 1;
-                            }
-                        }
-                        __syncthreads();
-                        count = 0;
-                    }
+                    if (count > RASTER_CAP - 64) flush();
                 }
             }
+            if (count > 0) flush();
         }
+        // the batch's minimum keys to the grid (64-bit atomicMin whose result
+        // is unused) and their tiles marked; the table is emptied
+        __syncthreads();
+        for (uint32_t j = lane; j < RASTER_TABLE; j += 64) {
+            const uint32_t g = tcell[j];
+            if (g != 0xFFFFFFFFu) {
+                (void)__hip_atomic_fetch_min(&keys[g], tkey[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                tiles_new[ttile[j]] = 1;
+                tkey[j] = EMPTY_KEY;
+                tcell[j] = 0xFFFFFFFFu;
+            }
+        }
+        __syncthreads();
     }
 }
 

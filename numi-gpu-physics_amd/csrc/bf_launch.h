@@ -141,11 +141,12 @@ hipError_t launch_load_reduction(const FluidBuffers& b, const StepParams& p, uin
 hipError_t launch_build_general_list(const uint8_t* static_flags, const uint8_t* dyn_prev, const uint8_t* dyn_cur,
                                      uint32_t tiles, uint32_t* list, uint32_t* count, hipStream_t s);
 hipError_t launch_init_equilibrium(int storage, void* f, const uint8_t* solid, const float4* wall,
-                                   const float* far_u, uint32_t n, hipStream_t s);
+                                   const float* far_u, const Grid& g, hipStream_t s);
 hipError_t launch_init_shear_wave(int storage, void* f, float amplitude, const Grid& g, hipStream_t s);
-hipError_t launch_extract_fields(int storage, const void* f, float* rho, float* u, uint32_t n, hipStream_t s);
+hipError_t launch_extract_fields(int storage, const void* f, float* rho, float* u, const Grid& g, hipStream_t s);
+// host layout (direction-major FP32) <-> device population buffer
 hipError_t launch_convert_populations(int storage, bool pack, const void* src, void* dst, size_t count,
-                                      uint32_t n, hipStream_t s);
+                                      const Grid& g, hipStream_t s);
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
 hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
 hipError_t launch_sphere_geometry(uint8_t* mask, float4* wall, uint8_t* dyn, const Grid& g, float4 cr, float3 gv,
