@@ -79,6 +79,8 @@ struct StepParams {
     int selected_part;
     int accumulate_loads;
     int capture_fields;
+    int fast_priority;         // s_setprio level of the fast kernel's wavefronts (0..3)
+    int general_priority;      // and of the general kernel's
     float omega_plus, omega_minus;
     float sponge_strength, sponge_width;
     float far_u[3], far_rho;
@@ -130,16 +132,14 @@ template <int S>
 struct Store;
 
 // Position of direction q of cell (x, row), row = y + ny z, in a population
-// buffer.  FP32 interleaves the 19 directions row by row ([z][y][q][x]): the
-// 19 loads and stores of a wavefront then fall in one 38 KB stretch per row
-// instead of 19 planes a lattice apart, which streams about 5 % faster on
-// MI355X.  FP16 / BF16 stay direction-major ([q][z][y][x]), which streams
-// faster for 2-byte values.  The host-facing layout is direction-major for
-// every storage (the pack / unpack kernels convert).
+// buffer: the 19 directions interleaved row by row ([z][y][q][x]).  The 19
+// loads and stores of a wavefront then fall in one stretch of 19 rows
+// instead of 19 planes a lattice apart; on MI355X the fast kernel streams 7 %
+// (FP32) and 9 % (FP16) faster than with direction-major planes.  The
+// host-facing layout is direction-major (the pack / unpack kernels convert).
 template <int S>
 __device__ __forceinline__ size_t pop_index(const Grid& g, int q, int x, uint32_t row) {
-    if constexpr (S == STORE_FP32) return ((size_t)row * 19u + (size_t)q) * (size_t)g.nx + (size_t)x;
-    else return (size_t)q * (size_t)g.n + (size_t)row * (size_t)g.nx + (size_t)x;
+    return ((size_t)row * 19u + (size_t)q) * (size_t)g.nx + (size_t)x;
 }
 
 template <>

@@ -57,6 +57,16 @@ __device__ __forceinline__ float feq_q(float wrho, float u2_15, float ux, float 
     return wrho * (1.0f + 3.0f * cu + 4.5f * cu * cu - u2_15);
 }
 
+// Wavefront issue priority (s_setprio takes an immediate).
+__device__ __forceinline__ void set_priority(int level) {
+    switch (level) {
+        case 1: __builtin_amdgcn_s_setprio(1); break;
+        case 2: __builtin_amdgcn_s_setprio(2); break;
+        case 3: __builtin_amdgcn_s_setprio(3); break;
+        default: break;
+    }
+}
+
 template <int... Q>
 struct QSeq {};
 
@@ -546,6 +556,7 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
                                                   const uint8_t* __restrict__ dyn_cur,
                                                   float* __restrict__ density, float4* __restrict__ velocity,
                                                   uint32_t* __restrict__ activation, StepParams p) {
+    set_priority(p.fast_priority);
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     // Tiles with walls or sponge belong to fluid_general.  The flags are
     // loaded together with the populations and tested only before the
@@ -1042,6 +1053,7 @@ __global__ __launch_bounds__(256) void fluid_general(const typename Store<S>::T*
                                                      double* __restrict__ partials,
                                                      float* __restrict__ density, float4* __restrict__ velocity,
                                                      const double* __restrict__ body, StepParams p) {
+    set_priority(p.general_priority);
     const uint32_t count = *general_count;
     // torque reference point: the body position, read once per workgroup
     const float bx = (float)body[0], by = (float)body[1], bz = (float)body[2];

@@ -176,8 +176,12 @@ __global__ void fill_wall(float4* __restrict__ p, uint32_t n) {
 // Keys are a minimum over (cell, triangle) pairs: the result does not depend
 // on the order of the atomics.  With m.cull_cells = +inf (the reference's
 // full AABB) the filter passes every cell of the AABB.
-constexpr int RASTER_CAP = 1024;
-constexpr int RASTER_TABLE_BITS = 10;
+// LDS per single-wavefront workgroup: candidates 2 KB + key table 8 KB.  At
+// 10 KB the compiler targets 4 waves per SIMD and keeps the raster under 100
+// VGPRs; at 24 KB it targeted 2 and reserved 169, which took register room
+// from the concurrent fluid kernel.
+constexpr int RASTER_CAP = 256;
+constexpr int RASTER_TABLE_BITS = 9;
 constexpr uint32_t RASTER_TABLE = 1u << RASTER_TABLE_BITS;
 constexpr uint32_t RASTER_PROBES = 16;
 
