@@ -44,13 +44,22 @@ def bird_config(shape, storage, arithmetic="default"):
         storage=STORAGES[storage], collision=bh.COLLISION_RR3, wall_rule=bh.WALL_INTERPOLATED_SDF, force_mode=6,
         accumulate_loads=True, boundary=((bh.BC_FARFIELD, bh.BC_OUTLET), (bh.BC_PERIODIC, bh.BC_PERIODIC),
                                          (bh.BC_PERIODIC, bh.BC_PERIODIC)),
-        far_velocity=(0.04, 0.0, 0.0), body_position=(160.0, 128.0, 128.0))
+        far_velocity=(0.04, 0.0, 0.0), body_position=(160.0 * shape[0] / 512.0, 128.0 * shape[0] / 512.0,
+                                                     128.0 * shape[0] / 512.0))
 
 
 def bird_mesh(steps, shape):
+    """The synthetic bird placed in `shape`; for other than 512 cells along x
+    the bird (and its stroke period) scale with the domain, so that
+    1024x512x512 is the 512x256x256 case scaled 2x per axis."""
     from birdflow_hip import workloads
     s = shape[0] / 512.0
-    return workloads.synthetic_bird(steps, seed=7, center=(160.0 * s, 128.0 * s, 128.0 * s))
+    if s == 1.0:
+        return workloads.synthetic_bird(steps, seed=7, center=(160.0, 128.0, 128.0))
+    return workloads.synthetic_bird(steps, seed=7, center=(160.0 * s, 128.0 * s, 128.0 * s),
+                                    body_radii=(40.0 * s, 14.0 * s, 14.0 * s), half_span=100.0 * s,
+                                    root_chord=40.0 * s, tip_chord=12.0 * s, tail=(30.0 * s, 20.0 * s),
+                                    period=1200.0 * s)
 
 
 def run_gpu(shape, steps, warmup, storage, arithmetic="default"):
@@ -102,10 +111,19 @@ def run_gpu(shape, steps, warmup, storage, arithmetic="default"):
     }
 
 
-def run_cpu_baseline(shape, steps):
-    """The oracle (C, OpenMP over z) on a bounded sample of the same workload."""
+def omp_threads(n):
+    """Sets the oracle's OpenMP thread count (libgomp, loaded with it)."""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def run_cpu_baseline(shape, steps, threads=None):
+    """The oracle (C, OpenMP over z) on a bounded sample of the same workload,
+    on `threads` OpenMP threads (default: OMP_NUM_THREADS or all host cores)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import bfref
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    omp_threads(threads or all_threads)
     seq = bird_mesh(steps, shape)
     cfg = bird_config(shape, "fp32")
     p = bfref.make_params(shape, omega_plus=cfg.omega_plus, omega_minus=cfg.omega_minus,
@@ -128,9 +146,9 @@ def run_cpu_baseline(shape, steps):
         f, _, _, _ = bfref.step(p, f, prev, mask, wall)
         prev = mask
     elapsed = time.perf_counter() - t0
+    omp_threads(all_threads)
     n = shape[0] * shape[1] * shape[2]
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return n * steps / elapsed / 1e6, threads, elapsed
+    return n * steps / elapsed / 1e6, threads or all_threads, elapsed
 
 
 def plate_theta(seed=7):
@@ -198,6 +216,8 @@ def main():
                     help="collision arithmetic; default: strict for fp32, fused for fp16/bf16")
     ap.add_argument("--shape", type=int, nargs=3, default=(512, 256, 256))
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-single-steps", type=int, default=1,
+                    help="oracle steps timed on one thread beside it (0: skip)")
     ap.add_argument("--skip-fp16", action="store_true",
                     help="skip the FP16-offset storage form (reported beside FP32 by default)")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
@@ -236,6 +256,12 @@ def main():
         cpu = {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": "port",
                "sample": f"oracle (C, OpenMP over z) on the same bird workload, {args.cpu_steps} full steps "
                          f"(geometry + fluid) at {shape[0]}x{shape[1]}x{shape[2]}"}
+        if args.cpu_single_steps > 0:
+            m1, _, e1 = run_cpu_baseline(shape, args.cpu_single_steps, threads=1)
+            print(f"# cpu, 1 thread: {m1:.2f} MLUPS over {args.cpu_single_steps} steps ({e1:.1f} s)",
+                  file=sys.stderr, flush=True)
+            cpu["single_thread"] = {"value": m1, "unit": "MLUPS", "cores": 1,
+                                    "sample": f"the same, {args.cpu_single_steps} full step(s), one thread"}
     line = {
         "metric": METRIC,
         "value": r["mlups"],

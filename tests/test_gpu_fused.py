@@ -229,3 +229,67 @@ def test_reduced_storage_plate_force_history_vs_fp64(storage, bar):
     mean_err = np.abs(hist[:, 2].mean() - loads64[:, 2].mean()) / abs(loads64[:, 2].mean())
     say(f"  force history max |diff| / max |F| = {err:.3e}; mean lift rel diff {mean_err:.3e}")
     assert err <= bar
+
+
+BIRD_SHAPE = (256, 128, 128)
+BIRD_STEPS = 80
+
+
+def half_bird(steps):
+    """The 512x256x256 synthetic bird scaled by 1/2 (body, wings, tail and
+    stroke period) into 256x128x128."""
+    return workloads.synthetic_bird(steps, seed=7, center=(80.0, 64.0, 64.0), body_radii=(20.0, 7.0, 7.0),
+                                    half_span=50.0, root_chord=20.0, tip_chord=6.0, tail=(15.0, 10.0),
+                                    period=600.0)
+
+
+def bird_params():
+    return bfref.make_params(
+        BIRD_SHAPE, omega_plus=1.0 / 0.52, omega_minus=1.0 / (0.5 + (3.0 / 16.0) / 0.02),
+        bc=((bfref.BC_FARFIELD, bfref.BC_OUTLET), (bfref.BC_PERIODIC, bfref.BC_PERIODIC),
+            (bfref.BC_PERIODIC, bfref.BC_PERIODIC)),
+        collision=bfref.COLLISION_RR3, wall_rule=bfref.WALL_INTERPOLATED_SDF, force_mode=6, accumulate_loads=True,
+        far_u=(0.04, 0.0, 0.0), body_position=(80.0, 64.0, 64.0))
+
+
+@pytest.fixture(scope="module")
+def bird_fp64_loads():
+    """FP64 oracle force history of the half-scale bird on the same inputs:
+    each step's masks and wall field are the GPU's (bit-identical to the
+    oracle's FP32 raster, checked elsewhere), read back step by step."""
+    p = bird_params()
+    seq = half_bird(BIRD_STEPS)
+    loads = np.zeros((BIRD_STEPS, 8))
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        prev, wall = sim.read_geometry()
+        f = bfref.init_equilibrium(p, prev, wall, np.float64)
+        for s in range(BIRD_STEPS):
+            sim.advance(1)
+            mask, wall = sim.read_geometry()
+            f, l, _, _ = bfref.step(p, f, prev, mask, wall)
+            loads[s] = l
+            prev = mask
+    return loads
+
+
+@pytest.mark.parametrize("storage,bar", [(bh.STORAGE_FP16_OFFSET, 5e-3), (bh.STORAGE_BF16_OFFSET, 5e-2)])
+def test_reduced_storage_bird_force_history_vs_fp64(bird_fp64_loads, storage, bar):
+    # The precision case on the articulated bird (40k triangles, RR3, moving
+    # walls with cover / uncover) at half scale: FP16 offsets within 0.5 % of
+    # an FP64 run on the same inputs, BF16 within 5 %.
+    p = bird_params()
+    seq = half_bird(BIRD_STEPS)
+    say(f"case: half-scale bird {BIRD_SHAPE}, storage {storage}, {BIRD_STEPS} steps, force history vs FP64 oracle")
+    with bh.BirdFlowSimulation(cases.solver_config(p, storage=storage)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        sim.advance(BIRD_STEPS)
+        hist = sim.load_history()
+    ref = bird_fp64_loads
+    scale = np.abs(ref[:, :3]).max()
+    err = np.abs(hist[:, :3] - ref[:, :3]).max() / scale
+    tail = slice(BIRD_STEPS // 2, None)
+    mean_err = np.abs(hist[tail, :3].mean(0) - ref[tail, :3].mean(0)).max() / np.abs(ref[tail, :3].mean(0)).max()
+    say(f"  force history max |diff| / max |F| = {err:.3e}; mean force (second half) rel diff {mean_err:.3e}")
+    assert err <= bar
