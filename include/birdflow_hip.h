@@ -141,10 +141,34 @@ bf_status bf_init_shear_wave(bf_context* ctx, float amplitude);
  * BirdFlowSimulation.swift:1214-1298 checkpoint read-back).  count = 19 * N. */
 bf_status bf_write_populations(bf_context* ctx, const float* host, uint64_t count);
 bf_status bf_read_populations(bf_context* ctx, float* host, uint64_t count);
+/* The stored form itself, unchanged (the raw buffer copies of the
+ * checkpoint, BirdFlowSimulation.swift:1214-1298 and 1920-1978): 19 * N
+ * elements in the direction-major order above, FP32 populations or the FP16 /
+ * BF16 bits of the offsets f_q - w_q; bytes = 19 * N * (4 or 2).  A
+ * write / read round trip through this form is bit-exact for every storage
+ * (through FP32 values it is not for FP16 / BF16). */
+bf_status bf_write_population_storage(bf_context* ctx, const void* host, uint64_t bytes);
+bf_status bf_read_population_storage(bf_context* ctx, void* host, uint64_t bytes);
 /* extractMacroscopicFields (BirdFlow.metal:2107-2127) into host arrays:
  * density[N], velocity[3N] interleaved. */
 bf_status bf_read_fields(bf_context* ctx, float* density, float* velocity3);
 uint64_t bf_step_index(const bf_context* ctx);
+/* Restart state beside the populations (no single counterpart in the
+ * reference, whose restore sets the step, clock, last load and solid mask
+ * inside restoreCheckpointState, BirdFlowSimulation.swift:1920-1990, behind
+ * init(checkpointURL:), BirdFlowSimulation.swift:661-680).
+ * bf_read_restart_state: the completed step index and the mesh clock origin
+ * (mesh time of step s = origin + s * time_step).
+ * bf_set_restart_state: renumbers the completed step, sets the clock origin,
+ * optionally the last reduced load (8 doubles as bf_read_loads reports) and
+ * the solid mask the next step starts from (N bytes, x fastest), and rebuilds
+ * the current step's mesh or sphere geometry at the new time.  Call it after
+ * the geometry source and body are set and before bf_write_populations.
+ * Load and body histories restart empty.  A mask that differs from a static
+ * geometry is BF_ERROR_INVALID_ARGUMENT. */
+bf_status bf_read_restart_state(bf_context* ctx, uint64_t* step, float* mesh_time_origin);
+bf_status bf_set_restart_state(bf_context* ctx, uint64_t step, float mesh_time_origin, const double last_load[8],
+                               const uint8_t* solid_mask);
 
 /* ---- geometry ---- */
 /* Static geometry written by the caller: part masks (0 fluid, 1..255 part id)

@@ -39,7 +39,9 @@ ARITHMETIC_DEFAULT, ARITHMETIC_STRICT, ARITHMETIC_FUSED = 0, 1, 2
 EXPORTED_FUNCTIONS = (
     "bf_create", "bf_destroy", "bf_last_error", "bf_status_string", "bf_abi_version", "bf_device_info",
     "bf_allocated_bytes", "bf_init_equilibrium", "bf_init_shear_wave", "bf_write_populations",
-    "bf_read_populations", "bf_read_fields", "bf_step_index", "bf_set_geometry", "bf_read_geometry",
+    "bf_read_populations", "bf_write_population_storage", "bf_read_population_storage", "bf_read_fields",
+    "bf_step_index", "bf_read_restart_state", "bf_set_restart_state",
+    "bf_set_geometry", "bf_read_geometry",
     "bf_mesh_upload", "bf_mesh_build", "bf_set_translating_sphere", "bf_step", "bf_synchronize",
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
@@ -138,8 +140,12 @@ def _declare(l):
         "bf_init_shear_wave": (st, [vp, f32]),
         "bf_write_populations": (st, [vp, vp, u64]),
         "bf_read_populations": (st, [vp, vp, u64]),
+        "bf_write_population_storage": (st, [vp, vp, u64]),
+        "bf_read_population_storage": (st, [vp, vp, u64]),
         "bf_read_fields": (st, [vp, vp, vp]),
         "bf_step_index": (u64, [vp]),
+        "bf_read_restart_state": (st, [vp, vp, vp]),
+        "bf_set_restart_state": (st, [vp, u64, f32, vp, vp]),
         "bf_set_geometry": (st, [vp, vp, vp, vp]),
         "bf_read_geometry": (st, [vp, vp, vp]),
         "bf_mesh_upload": (st, [vp, vp, vp, u32, u32, vp, vp, u32, f32, vp, f32]),

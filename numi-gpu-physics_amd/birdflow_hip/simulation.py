@@ -201,6 +201,9 @@ class BirdFlowSimulation:
             raise BirdFlowError(st, self._lib.bf_status_string(st).decode())
         self._ctx = handle
         self.n = config.cells
+        # inputs of the geometry source and body parameters, kept for checkpoints
+        self.geometry_source = None
+        self.body_parameters = None
 
     # -- lifetime --------------------------------------------------------
     def close(self):
@@ -241,6 +244,44 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_read_populations(self._ctx, _ptr(out), out.size))
         return out
 
+    @property
+    def storage_dtype(self):
+        """Element type of the stored form: float32 populations, or the
+        float16 / bfloat16 bits (uint16) of the offsets f_q - w_q."""
+        return np.float32 if self.config.storage == _abi.STORAGE_FP32 else np.uint16
+
+    def write_population_storage(self, raw: np.ndarray):
+        """The stored form itself, direction-major (bit-exact for every storage)."""
+        r = np.ascontiguousarray(raw, self.storage_dtype).ravel()
+        self._check(self._lib.bf_write_population_storage(self._ctx, _ptr(r), r.nbytes))
+
+    def read_population_storage(self) -> np.ndarray:
+        out = np.empty(19 * self.n, self.storage_dtype)
+        self._check(self._lib.bf_read_population_storage(self._ctx, _ptr(out), out.nbytes))
+        return out
+
+    def restart_state(self):
+        """(completed step, mesh clock origin): mesh time of step s is
+        origin + s * time_step."""
+        step = ctypes.c_uint64(0)
+        origin = ctypes.c_float(0.0)
+        self._check(self._lib.bf_read_restart_state(self._ctx, ctypes.byref(step), ctypes.byref(origin)))
+        return int(step.value), float(origin.value)
+
+    def set_restart_state(self, step: int, mesh_time_origin: float, last_load=None, solid_mask=None):
+        """Renumbers the completed step, sets the mesh clock origin, the last
+        reduced load (8 values) and the solid mask the next step starts from,
+        and rebuilds the current step's geometry.  Call after the geometry
+        source and body are set, before the populations are written."""
+        load = None if last_load is None else np.ascontiguousarray(last_load, np.float64).ravel()
+        if load is not None and load.size != 8:
+            raise ValueError("last_load has 8 components")
+        mask = None if solid_mask is None else np.ascontiguousarray(solid_mask, np.uint8).ravel()
+        if mask is not None and mask.size != self.n:
+            raise ValueError("solid_mask has one byte per cell")
+        self._check(self._lib.bf_set_restart_state(self._ctx, int(step), float(mesh_time_origin), _ptr(load),
+                                                   _ptr(mask)))
+
     def copy_macroscopic_fields(self):
         rho = np.empty(self.n, np.float32)
         u = np.empty((self.n, 3), np.float32)
@@ -261,6 +302,10 @@ class BirdFlowSimulation:
         w = np.ascontiguousarray(wall4, np.float32).reshape(-1, 4)
         mp = None if mask_prev is None else np.ascontiguousarray(mask_prev, np.uint8).ravel()
         self._check(self._lib.bf_set_geometry(self._ctx, _ptr(mp), _ptr(mc), _ptr(w)))
+        # the current mask and wall are read back from the device when saved;
+        # a distinct previous mask matters only until the next step
+        self.geometry_source = {"kind": "static", "mask_prev": None if mp is None else mp.copy(),
+                                "set_at_step": self.step_index}
 
     def read_geometry(self):
         mask = np.empty(self.n, np.uint8)
@@ -279,6 +324,9 @@ class BirdFlowSimulation:
         t0 = float(times[0]) if time_start is None else float(time_start)
         self._check(self._lib.bf_mesh_upload(self._ctx, _ptr(pts), _ptr(times), frames, verts, _ptr(tri), _ptr(parts),
                                              tri.shape[0], float(half_thickness), _ptr(tr), t0))
+        self.geometry_source = {"kind": "mesh", "points": pts.copy(), "frame_times": times.copy(),
+                                "triangles": tri.copy(), "triangle_parts": parts.copy(),
+                                "half_thickness": float(half_thickness), "translation": tr.copy()}
 
     def build_mesh_geometry(self, time: float):
         self._check(self._lib.bf_mesh_build(self._ctx, float(time)))
@@ -288,6 +336,8 @@ class BirdFlowSimulation:
         gv = np.ascontiguousarray(geometry_velocity, np.float32)
         wv = np.ascontiguousarray(wall_velocity, np.float32)
         self._check(self._lib.bf_set_translating_sphere(self._ctx, _ptr(cr), _ptr(gv), _ptr(wv)))
+        self.geometry_source = {"kind": "sphere", "center_radius": cr.tolist(), "geometry_velocity": gv.tolist(),
+                                "wall_velocity": wv.tolist()}
 
     # -- stepping --------------------------------------------------------
     def advance(self, steps: int, synchronize: bool = True):
@@ -319,6 +369,8 @@ class BirdFlowSimulation:
         cp = params.to_c() if params is not None else None
         self._check(self._lib.bf_body_set(self._ctx, ctypes.byref(cs) if cs is not None else None,
                                           ctypes.byref(cp) if cp is not None else None))
+        if params is not None:
+            self.body_parameters = params
 
     def body_state(self) -> np.ndarray:
         b = _abi.BodyState()

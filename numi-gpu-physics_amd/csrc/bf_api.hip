@@ -714,6 +714,58 @@ void bf_destroy(bf_context* c) {
 
 uint64_t bf_step_index(const bf_context* c) { return c ? (uint64_t)c->step : 0; }
 
+static bf_status rebuild_mesh_geometry(bf_context* c);
+static bf_status rebuild_sphere_geometry(bf_context* c);
+
+bf_status bf_read_restart_state(bf_context* c, uint64_t* step, float* mesh_time_origin) {
+    BF_ENTER(c);
+    if (!step || !mesh_time_origin) return fail(c, BF_ERROR_INVALID_ARGUMENT, "output pointers must not be NULL");
+    *step = (uint64_t)c->step;
+    *mesh_time_origin = c->time_start;
+    return BF_OK;
+}
+
+bf_status bf_set_restart_state(bf_context* c, uint64_t step, float mesh_time_origin, const double last_load[8],
+                               const uint8_t* solid_mask) {
+    BF_ENTER(c);
+    if (step > (uint64_t)(INT64_MAX / 4) || !std::isfinite(mesh_time_origin))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "restart step or mesh time origin out of range");
+    if (last_load) {
+        hipError_t e = hipMemcpy(c->last_load, last_load, 64, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(c, e, "restart load");
+    }
+    c->step = (int64_t)step;
+    c->time_start = mesh_time_origin;
+    c->history_count = 0;
+    c->body_history_count = 0;
+    c->geometry_ready = c->step;
+    // the clock moved: the current step's geometry is built again
+    bf_status r = BF_OK;
+    if (c->source == BF_GEOMETRY_MESH) r = rebuild_mesh_geometry(c);
+    if (c->source == BF_GEOMETRY_SPHERE) r = rebuild_sphere_geometry(c);
+    if (r != BF_OK || !solid_mask) return r;
+    // The saved mask is the previous-step mask of the next step.  It equals
+    // the rebuilt one unless the geometry followed a free-flying body whose
+    // pose moved on after it was built; then the saved mask replaces it, its
+    // tiles are marked, and the whole slot is cleared when it is reused.
+    const size_t n = c->g.n;
+    std::vector<uint8_t> cur(n);
+    hipError_t e = hipMemcpy(cur.data(), geometry_of(c, c->step).mask_cur, n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "restart mask");
+    if (std::memcmp(cur.data(), solid_mask, n) == 0) return BF_OK;
+    if (c->source == BF_GEOMETRY_STATIC)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "restart solid mask differs from the static geometry");
+    int periodic[3];
+    for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
+    const int k = mod3(c->step);
+    e = hipMemcpy(c->mask[k], solid_mask, n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_mark_tiles(c->mask[k], c->dyn[k], c->g, periodic, false, c->sb);
+    if (e == hipSuccess) e = launch_fill_u8(c->tflags[mod4(c->step)], 1, c->g.tiles, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "restart mask");
+    return BF_OK;
+}
+
 bf_status bf_init_equilibrium(bf_context* c) {
     BF_ENTER(c);
     const StepGeometry sg = geometry_of(c, c->step);
@@ -772,6 +824,39 @@ bf_status bf_read_populations(bf_context* c, float* host, uint64_t count) {
     return BF_OK;
 }
 
+static uint64_t stored_bytes(const bf_context* c) {
+    return 19ull * c->g.n * (c->cfg.storage == BF_STORAGE_FP32 ? 4u : 2u);
+}
+
+bf_status bf_write_population_storage(bf_context* c, const void* host, uint64_t bytes) {
+    BF_ENTER(c);
+    if (check_host_array(c, host, "population storage")) return BF_ERROR_INVALID_ARGUMENT;
+    if (bytes != stored_bytes(c))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "population storage size must be 19 * N * element size");
+    if (bf_status st = ensure_staging(c)) return st;
+    hipError_t e = hipMemcpyAsync(c->staging, host, bytes, hipMemcpyHostToDevice, c->sb);
+    if (e == hipSuccess)
+        e = launch_permute_populations((int)c->cfg.storage, true, c->staging, c->f[c->cur], 19ull * c->g.n, c->g,
+                                       c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "write population storage");
+    return BF_OK;
+}
+
+bf_status bf_read_population_storage(bf_context* c, void* host, uint64_t bytes) {
+    BF_ENTER(c);
+    if (check_host_array(c, host, "population storage")) return BF_ERROR_INVALID_ARGUMENT;
+    if (bytes != stored_bytes(c))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "population storage size must be 19 * N * element size");
+    if (bf_status st = ensure_staging(c)) return st;
+    hipError_t e = launch_permute_populations((int)c->cfg.storage, false, c->f[c->cur], c->staging, 19ull * c->g.n,
+                                              c->g, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, c->staging, bytes, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "read population storage");
+    return BF_OK;
+}
+
 bf_status bf_read_fields(bf_context* c, float* density, float* velocity3) {
     BF_ENTER(c);
     if (check_host_array(c, density, "density") || check_host_array(c, velocity3, "velocity"))
@@ -815,6 +900,31 @@ bf_status bf_read_geometry(bf_context* c, uint8_t* mask, float* wall4) {
     if (e == hipSuccess && wall4) e = hipMemcpyAsync(wall4, sg.wall, 16 * n, hipMemcpyDeviceToHost, c->sb);
     if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "read geometry");
+    return BF_OK;
+}
+
+// Geometry of the current step from cleared buffers (masks, walls, flags,
+// keys), with the reference's full key band; the previous step's mask starts
+// equal to it (its touched-tile flags are copied so that the step that reuses
+// that buffer clears it).
+static bf_status rebuild_mesh_geometry(bf_context* c) {
+    hipStream_t s = c->sb;
+    hipError_t e = launch_fill_keys(c->keys, c->g.n, s);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->mask[i], 0, c->g.n, s);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_fill_wall(c->wall[i], c->g.n, s);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->dyn[i], 0, c->g.tiles, s);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = launch_fill_u8(c->tflags[i], 0, c->g.tiles, s);
+    if (e != hipSuccess) return hip_fail(c, e, "mesh geometry clear");
+    const int64_t st = c->step;
+    if (bf_status r = enqueue_geometry(c, st, true, true)) return r;
+    const int k = mod3(st), kp = mod3(st - 1);
+    e = hipMemcpyAsync(c->mask[kp], c->mask[k], c->g.n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->dyn[kp], c->dyn[k], c->g.tiles, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->tflags[mod4(st - 1)], c->tflags[mod4(st)], c->g.tiles, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e, "initial mesh geometry");
+    c->geometry_ready = st;
     return BF_OK;
 }
 
@@ -873,12 +983,6 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     if (e == hipSuccess) e = hipMemcpyAsync(c->m_times, frame_times, 4ull * frame_count, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->m_tri, triangles, 12ull * triangle_count, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->m_part, triangle_parts, triangle_count, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = launch_fill_keys(c->keys, c->g.n, s);
-    // fresh geometry state: masks, walls and flags cleared, lists empty
-    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->mask[i], 0, c->g.n, s);
-    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_fill_wall(c->wall[i], c->g.n, s);
-    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->dyn[i], 0, c->g.tiles, s);
-    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = launch_fill_u8(c->tflags[i], 0, c->g.tiles, s);
     if (e != hipSuccess) return hip_fail(c, e, "mesh upload");
     c->host_times.assign(frame_times, frame_times + frame_count);
     c->m_frames = frame_count;
@@ -888,21 +992,8 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     for (int i = 0; i < 3; ++i) c->m_translation[i] = translation ? translation[i] : 0.0f;
     c->source = BF_GEOMETRY_MESH;
     c->static_prev_differs = false;
-    const int64_t st = c->step;
-    c->time_start = time_start - (float)((double)st * c->cfg.time_step);
-    // Geometry of the current step with the reference's full key band; the
-    // previous step's mask starts equal to it (its touched-tile flags are
-    // copied so that the step that reuses that buffer clears it).
-    if (bf_status r = enqueue_geometry(c, st, true, true)) return r;
-    const int k = mod3(st), kp = mod3(st - 1);
-    e = hipMemcpyAsync(c->mask[kp], c->mask[k], c->g.n, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->dyn[kp], c->dyn[k], c->g.tiles, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(c->tflags[mod4(st - 1)], c->tflags[mod4(st)], c->g.tiles, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(c, e, "initial mesh geometry");
-    c->geometry_ready = st;
-    return BF_OK;
+    c->time_start = time_start - (float)((double)c->step * c->cfg.time_step);
+    return rebuild_mesh_geometry(c);
 }
 
 bf_status bf_mesh_build(bf_context* c, float t) {
@@ -925,15 +1016,9 @@ bf_status bf_mesh_build(bf_context* c, float t) {
     return BF_OK;
 }
 
-bf_status bf_set_translating_sphere(bf_context* c, const float cr[4], const float gv[3], const float wv[4]) {
-    BF_ENTER(c);
-    if (!cr || !gv || !wv) return fail(c, BF_ERROR_INVALID_ARGUMENT, "sphere parameters must not be NULL");
-    c->sphere_cr = make_float4(cr[0], cr[1], cr[2], cr[3]);
-    c->sphere_gv = make_float3(gv[0], gv[1], gv[2]);
-    c->sphere_wv = make_float4(wv[0], wv[1], wv[2], wv[3]);
-    c->source = BF_GEOMETRY_SPHERE;
-    c->static_prev_differs = false;
-    c->time_start = 0.0f;
+// Sphere geometry of the current step; the previous step's mask starts
+// equal to it.
+static bf_status rebuild_sphere_geometry(bf_context* c) {
     const int64_t st = c->step;
     if (bf_status r = enqueue_geometry(c, st, true, false)) return r;
     const int k = mod3(st), kp = mod3(st - 1);
@@ -943,6 +1028,18 @@ bf_status bf_set_translating_sphere(bf_context* c, const float cr[4], const floa
     if (e != hipSuccess) return hip_fail(c, e, "sphere geometry");
     c->geometry_ready = st;
     return BF_OK;
+}
+
+bf_status bf_set_translating_sphere(bf_context* c, const float cr[4], const float gv[3], const float wv[4]) {
+    BF_ENTER(c);
+    if (!cr || !gv || !wv) return fail(c, BF_ERROR_INVALID_ARGUMENT, "sphere parameters must not be NULL");
+    c->sphere_cr = make_float4(cr[0], cr[1], cr[2], cr[3]);
+    c->sphere_gv = make_float3(gv[0], gv[1], gv[2]);
+    c->sphere_wv = make_float4(wv[0], wv[1], wv[2], wv[3]);
+    c->source = BF_GEOMETRY_SPHERE;
+    c->static_prev_differs = false;
+    c->time_start = 0.0f;
+    return rebuild_sphere_geometry(c);
 }
 
 bf_status bf_step(bf_context* c, uint32_t steps) {

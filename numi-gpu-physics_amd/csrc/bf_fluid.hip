@@ -158,6 +158,23 @@ __global__ void unpack_populations(const typename Store<S>::T* __restrict__ src,
     dst[i] = Store<S>::load(src, pop_index<S>(g, q, (int)(gid - row * (uint32_t)g.nx), row), q);
 }
 
+// Stored values moved unchanged between the host's direction-major order
+// ([q][z][y][x]) and the device's row-interleaved one (checkpoint images of
+// the storage form, bit-exact for every storage).
+template <typename T>
+__global__ void permute_populations(const T* __restrict__ src, T* __restrict__ dst, size_t count, Grid g,
+                                    int to_device) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int q = (int)(i / g.n);
+    const uint32_t gid = (uint32_t)(i - (size_t)q * g.n);
+    const uint32_t row = gid / (uint32_t)g.nx;
+    const size_t d = pop_index<STORE_FP32>(g, q, (int)(gid - row * (uint32_t)g.nx), row);
+    if (to_device)
+        dst[d] = src[i];
+    else
+        dst[i] = src[d];
+}
 
 hipError_t launch_load_reduction(const FluidBuffers& b, const StepParams& p, uint32_t history_slot, hipStream_t s) {
     if (!p.accumulate_loads) return hipSuccess;
@@ -252,6 +269,20 @@ static hipError_t conv_s(bool pack, const void* src, void* dst, size_t count, co
     else
         hipLaunchKernelGGL(unpack_populations<S>, dim3(blocks), dim3(256), 0, s, (const typename Store<S>::T*)src,
                            (float*)dst, count, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_permute_populations(int storage, bool to_device, const void* src, void* dst, size_t count,
+                                      const Grid& g, hipStream_t s) {
+    const unsigned blocks = (unsigned)((count + 255) / 256);
+    if (storage == STORE_FP32)
+        hipLaunchKernelGGL(permute_populations<uint32_t>, dim3(blocks), dim3(256), 0, s, (const uint32_t*)src,
+                           (uint32_t*)dst, count, g, to_device ? 1 : 0);
+    else if (storage == STORE_FP16_OFFSET || storage == STORE_BF16_OFFSET)
+        hipLaunchKernelGGL(permute_populations<uint16_t>, dim3(blocks), dim3(256), 0, s, (const uint16_t*)src,
+                           (uint16_t*)dst, count, g, to_device ? 1 : 0);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -145,6 +145,10 @@ hipError_t launch_init_equilibrium(int storage, void* f, const uint8_t* solid, c
 hipError_t launch_init_shear_wave(int storage, void* f, float amplitude, const Grid& g, hipStream_t s);
 hipError_t launch_extract_fields(int storage, const void* f, float* rho, float* u, const Grid& g, hipStream_t s);
 // host layout (direction-major FP32) <-> device population buffer
+// Stored values (4 or 2 bytes by storage) between host direction-major and
+// device row-interleaved order, unchanged.
+hipError_t launch_permute_populations(int storage, bool to_device, const void* src, void* dst, size_t count,
+                                      const Grid& g, hipStream_t s);
 hipError_t launch_convert_populations(int storage, bool pack, const void* src, void* dst, size_t count,
                                       const Grid& g, hipStream_t s);
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s);

@@ -1,0 +1,137 @@
+"""GPU checks of restart archives, after the reference's checkpoint test
+(Tests/BirdFlowMetalTests/ObservationAndCheckpointTests.swift:115-178):
+
+* a context restored from an archive and saved again writes byte-identical
+  files;
+* the restored context continues exactly as the saved one: stored
+  populations, loads, masks and body state bit-identical after more steps
+  (the reference allows 1e-6 / 1e-5; here the restart is exact, for FP32,
+  FP16 and BF16 storage, a flapping-plate mesh, a translating sphere and a
+  free-flying body carrying its mesh);
+* a corrupted populations file is refused with CheckpointError.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import birdflow_hip as bh
+import cases
+from birdflow_hip import checkpoint, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def plate_sequence(steps):
+    return workloads.flapping_plate(steps, center=(32.0, 24.0, 24.0), span=20.0, chord=8.0, thickness=2.0,
+                                    heave=2.0, period=40.0, half_thickness=0.5)
+
+
+def files_of(directory):
+    out = {}
+    for name in sorted(os.listdir(directory)):
+        with open(os.path.join(directory, name), "rb") as fh:
+            out[name] = fh.read()
+    return out
+
+
+def check_restart(sim, tmp_path, before, after):
+    """Advance `before` steps, save, restore, save again, advance both
+    `after` steps, and compare."""
+    sim.advance(before)
+    a = str(tmp_path / "a")
+    manifest = checkpoint.save_checkpoint(sim, a)
+    restored = checkpoint.load_checkpoint(a)
+    with restored:
+        b = str(tmp_path / "b")
+        checkpoint.save_checkpoint(restored, b)
+        fa, fb = files_of(a), files_of(b)
+        say(f"  saved at step {manifest['step']}: files {sorted(fa)}; "
+            f"{sum(len(v) for v in fa.values())} bytes")
+        assert sorted(fa) == sorted(fb)
+        for name in fa:
+            assert fa[name] == fb[name], f"re-saved {name} differs"
+        assert restored.step_index == sim.step_index == before
+        sim.advance(after)
+        restored.advance(after)
+        pa, pb = sim.read_population_storage(), restored.read_population_storage()
+        la, lb = sim.load_history()[-after:], restored.load_history()
+        ma, mb = sim.read_geometry()[0], restored.read_geometry()[0]
+        ba, bb = sim.body_state(), restored.body_state()
+        say(f"  after {after} more steps: populations differing {int((pa.view(np.uint8) != pb.view(np.uint8)).sum())} B, "
+            f"load records equal {np.array_equal(la, lb)}, masks equal {np.array_equal(ma, mb)}, "
+            f"body max |diff| {np.abs(ba - bb).max():.3e}")
+        assert lb.shape[0] == after
+        assert np.all(np.isfinite(la)) and np.all(np.isfinite(ba))
+        assert np.array_equal(pa.view(np.uint8), pb.view(np.uint8))
+        assert np.array_equal(la, lb)
+        assert np.array_equal(ma, mb)
+        assert np.array_equal(ba, bb)
+        assert restored.restart_state() == sim.restart_state()
+
+
+@pytest.mark.parametrize("storage", [bh.STORAGE_FP32, bh.STORAGE_FP16_OFFSET, bh.STORAGE_BF16_OFFSET])
+def test_restart_of_flapping_plate_is_exact(storage, tmp_path):
+    say(f"case: checkpoint of a flapping plate 96x48x48, storage {storage}, 5 + 4 steps")
+    p = cases.plate_params()
+    seq = plate_sequence(12)
+    with bh.BirdFlowSimulation(cases.solver_config(p, storage=storage)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        check_restart(sim, tmp_path, 5, 4)
+
+
+def test_restart_of_translating_sphere_is_exact(tmp_path):
+    say("case: checkpoint of the translating sphere 24^3, 6 + 5 steps")
+    p = cases.translating_sphere_params()
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.set_translating_sphere(cases.SPHERE_CENTER_RADIUS, cases.SPHERE_GEOMETRY_VELOCITY,
+                                   cases.SPHERE_WALL_VELOCITY)
+        sim.init_equilibrium()
+        check_restart(sim, tmp_path, 6, 5)
+
+
+def test_restart_of_free_flying_body_carrying_its_mesh_is_exact(tmp_path):
+    # The mesh follows the body pose: the mask the next step starts from was
+    # built at the pose before the last body update, so the archive's mask,
+    # not a rebuilt one, is the restart's previous mask.
+    say("case: checkpoint of a free-flying plate carried by its body, 96x48x48, 6 + 4 steps")
+    p = cases.plate_params()
+    seq = workloads.flapping_plate(12, center=(0.0, 0.0, 0.0), span=20.0, chord=8.0, thickness=2.0,
+                                   heave=2.0, period=40.0, half_thickness=0.5)
+    state0 = np.array([24.0, 24.0, 24.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.set_body(state0, bh.BodyParameters(mass=2.0e4, inertia=(2.0e5, 1.5e5, 2.5e5), gravity=(0.0, 0.0, -2e-4),
+                                               substeps=2, move_mesh=True))
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        check_restart(sim, tmp_path, 6, 4)
+        moved = np.abs(sim.body_state()[:3] - state0[:3]).max()
+        say(f"  body moved {moved:.3e} cells")
+        assert moved > 0.0
+
+
+def test_corrupted_checkpoint_is_refused(tmp_path):
+    say("case: corrupted checkpoint populations file")
+    p = cases.plate_params(shape=(64, 32, 32))
+    seq = plate_sequence(4)
+    a = str(tmp_path / "a")
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        sim.advance(2)
+        checkpoint.save_checkpoint(sim, a, compression="none")
+    path = os.path.join(a, checkpoint.POPULATIONS_NAME)
+    with open(path, "r+b") as fh:
+        fh.seek(1000)
+        b = fh.read(1)
+        fh.seek(1000)
+        fh.write(bytes([b[0] ^ 0x10]))
+    with pytest.raises(checkpoint.CheckpointError, match="checksum"):
+        checkpoint.load_checkpoint(a)
