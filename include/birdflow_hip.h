@@ -87,7 +87,7 @@ enum { BF_STORAGE_FP32 = 0, BF_STORAGE_FP16_OFFSET = 1, BF_STORAGE_BF16_OFFSET =
 enum { BF_ARITHMETIC_DEFAULT = 0, BF_ARITHMETIC_STRICT = 1, BF_ARITHMETIC_FUSED = 2 };
 
 /* Where each step's geometry comes from. */
-enum { BF_GEOMETRY_STATIC = 0, BF_GEOMETRY_MESH = 1, BF_GEOMETRY_SPHERE = 2 };
+enum { BF_GEOMETRY_STATIC = 0, BF_GEOMETRY_MESH = 1, BF_GEOMETRY_SPHERE = 2, BF_GEOMETRY_BIRD = 3 };
 
 /* Solver configuration in lattice units (the host converts physical units
  * with the reference's LatticeScaling, SimulationConfiguration.swift:137-183).
@@ -201,6 +201,32 @@ bf_status bf_mesh_build(bf_context* ctx, float time);
  * t = step_index * time_step and selects BF_GEOMETRY_SPHERE. */
 bf_status bf_set_translating_sphere(bf_context* ctx, const float center_radius[4], const float geometry_velocity[3],
                                     const float wall_velocity[4]);
+
+/* Analytic signed-distance bird: BirdParameters geometry and analytic wing
+ * kinematics as GPUBirdParameters packs them (GPUData.swift:82-140, from
+ * BirdModel.swift:129-217): metres, radians, Hz. */
+typedef struct bf_bird_geometry {
+    float body_radii[3];
+    float wing_span, wing_root_chord, wing_tip_chord, wing_thickness, wing_sweep;
+    float wing_root_offset[3];
+    float tail_length, tail_half_width, tail_thickness;
+    float frequency, stroke_amplitude, stroke_bias, pitch_mean, pitch_amplitude, pitch_phase;
+} bf_bird_geometry;
+
+/* The reference's product geometry (prepareBirdGeometry + buildBirdGeometry,
+ * BirdFlow.metal:686-938; encodeGeometryPreparation / encodeGeometry,
+ * BirdFlowSimulation.swift:1451-1510): each step's frame from the device body
+ * pose at t = step * time_step, then part masks (1 body, 2 left wing, 3 right
+ * wing, 4 tail) and wall = (velocity * velocity_to_lattice, 0) in world
+ * units (origin, cell_size).  Builds the current step (no previous geometry,
+ * as the reference's initialization) and selects BF_GEOMETRY_BIRD; in free
+ * flight every step's geometry follows the integrated pose. */
+bf_status bf_set_bird(bf_context* ctx, const bf_bird_geometry* bird);
+/* Prepared frame of the current step (GPUPreparedBirdGeometry, 14 float4:
+ * body position, orientation x y z w, linear velocity, world angular
+ * velocity + geometry radius, then left and right wing root, chord (w: tip
+ * twist), span, normal, angular velocity (w: tip twist rate)). */
+bf_status bf_read_bird_frame(bf_context* ctx, float frame[56]);
 
 /* ---- stepping ---- */
 /* advance(steps:) (BirdFlowSimulation.swift:687-799): per step, geometry at

@@ -34,6 +34,7 @@ BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
 COLLISION_TRT, COLLISION_TRT_SYMMETRIC_LIMITER, COLLISION_REGULARIZED, COLLISION_RR3 = 0, 1, 2, 3
 WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF = 0, 1, 2
 STORAGE_FP32, STORAGE_FP16_OFFSET, STORAGE_BF16_OFFSET = 0, 1, 2
+GEOMETRY_STATIC, GEOMETRY_MESH, GEOMETRY_SPHERE, GEOMETRY_BIRD = 0, 1, 2, 3
 ARITHMETIC_DEFAULT, ARITHMETIC_STRICT, ARITHMETIC_FUSED = 0, 1, 2
 
 EXPORTED_FUNCTIONS = (
@@ -46,6 +47,7 @@ EXPORTED_FUNCTIONS = (
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
     "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity", "bf_mesh_freeze_masks",
+    "bf_set_bird", "bf_read_bird_frame",
 )
 
 
@@ -102,6 +104,19 @@ class BodyParams(ctypes.Structure):
         ("free_flight", ctypes.c_uint32),
         ("move_mesh", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+    ]
+
+
+class BirdGeometry(ctypes.Structure):
+    """bf_bird_geometry (metres, radians, Hz)."""
+    _fields_ = [
+        ("body_radii", ctypes.c_float * 3),
+        ("wing_span", ctypes.c_float), ("wing_root_chord", ctypes.c_float), ("wing_tip_chord", ctypes.c_float),
+        ("wing_thickness", ctypes.c_float), ("wing_sweep", ctypes.c_float),
+        ("wing_root_offset", ctypes.c_float * 3),
+        ("tail_length", ctypes.c_float), ("tail_half_width", ctypes.c_float), ("tail_thickness", ctypes.c_float),
+        ("frequency", ctypes.c_float), ("stroke_amplitude", ctypes.c_float), ("stroke_bias", ctypes.c_float),
+        ("pitch_mean", ctypes.c_float), ("pitch_amplitude", ctypes.c_float), ("pitch_phase", ctypes.c_float),
     ]
 
 
@@ -167,6 +182,8 @@ def _declare(l):
         "bf_adjoint_gradient": (st, [vp, u32, vp, u32, ctypes.POINTER(ctypes.c_double), vp, vp]),
         "bf_adjoint_read_sensitivity": (st, [vp, vp, u64]),
         "bf_mesh_freeze_masks": (st, [vp, vp, u32]),
+        "bf_set_bird": (st, [vp, ctypes.POINTER(BirdGeometry)]),
+        "bf_read_bird_frame": (st, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(l, name)

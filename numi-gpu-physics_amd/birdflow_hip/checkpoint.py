@@ -18,9 +18,11 @@ Where this archive differs from the reference's, and why:
   BF16 bits of the offsets f_q - w_q) in direction-major order, so a restart
   is bit-exact for every storage; the reference copies its buffer bytes in
   its own layout.
-* The geometry source travels in the archive (mesh arrays, sphere
-  parameters, or the static wall field): the reference regenerates its bird
-  surface from BirdParameters, while here the surface is caller data.
+* The geometry source travels in the archive (mesh arrays, sphere or
+  analytic-bird parameters, or the static wall field); the reference's
+  archive holds the bird parameters and its prepared frame, which a restore
+  here prepares again from the body pose, and a mesh surface is caller
+  data.
 * The completed step index and the mesh clock origin are restored exactly
   (bf_set_restart_state), so the continued run sees the same mesh times.
 
@@ -44,7 +46,7 @@ from typing import Dict, Optional
 import numpy as np
 
 from . import _abi
-from .simulation import BirdFlowSimulation, BodyParameters, SolverConfig
+from .simulation import BirdFlowSimulation, BirdGeometry, BodyParameters, SolverConfig
 
 SCHEMA_VERSION = 1
 MANIFEST_NAME = "manifest.json"
@@ -58,6 +60,7 @@ _GEOMETRY_ARRAYS = {
              ("triangle_parts", np.uint8), ("translation", np.float32)),
     "static": (("wall", np.float32), ("mask_prev", np.uint8)),
     "sphere": (),
+    "bird": (),
 }
 
 
@@ -203,6 +206,10 @@ def save_checkpoint(sim: BirdFlowSimulation, destination: str, compression: str 
     elif kind == "sphere":
         for name in ("center_radius", "geometry_velocity", "wall_velocity"):
             geometry[name] = _plain(src[name])
+    elif kind == "bird":
+        # the shape; a restore prepares the frame again from the body pose
+        # and keeps the archive's mask as the next step's previous mask
+        geometry["bird"] = {k: _plain(v) for k, v in src["bird"].items()}
     else:
         arrays["wall"] = wall
         if src["mask_prev"] is not None and step == src["set_at_step"]:
@@ -288,6 +295,8 @@ def load_checkpoint(source: str) -> BirdFlowSimulation:
                             translation=a["translation"])
         elif g["kind"] == "sphere":
             sim.set_translating_sphere(g["center_radius"], g["geometry_velocity"], g["wall_velocity"])
+        elif g["kind"] == "bird":
+            sim.set_bird(BirdGeometry(**g["bird"]))
         else:
             sim.set_geometry(ck.solid_mask, ck.geometry["wall"], mask_prev=ck.geometry.get("mask_prev"))
         params = None if m["body_parameters"] is None else BodyParameters(**m["body_parameters"])

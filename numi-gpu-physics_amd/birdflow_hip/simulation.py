@@ -10,6 +10,7 @@ Every call goes through libbirdflow_hip.so; there is no CPU path.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import math
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
@@ -167,6 +168,48 @@ class BodyParameters:
         b.safety_radius, b.wing_speed, b.mach_limit = self.safety_radius, self.wing_speed, self.mach_limit
         b.substeps = int(self.substeps)
         b.free_flight, b.move_mesh = int(bool(self.free_flight)), int(bool(self.move_mesh))
+        return b
+
+
+@dataclass
+class BirdGeometry:
+    """BirdParameters geometry and analytic wing kinematics
+    (BirdModel.swift:129-217): metres, radians, Hz."""
+
+    body_radii: Sequence[float]
+    wing_span: float
+    wing_root_chord: float
+    wing_tip_chord: float
+    wing_thickness: float
+    wing_sweep: float
+    wing_root_offset: Sequence[float]
+    tail_length: float
+    tail_half_width: float
+    tail_thickness: float
+    frequency: float
+    stroke_amplitude: float
+    stroke_bias: float = 0.0
+    pitch_mean: float = 0.0
+    pitch_amplitude: float = 0.0
+    pitch_phase: float = 0.0
+
+    @classmethod
+    def demonstration(cls) -> "BirdGeometry":
+        """BirdParameters.demonstration (BirdModel.swift:198-217)."""
+        r = math.radians
+        return cls(body_radii=(0.18, 0.065, 0.075), wing_span=0.34, wing_root_chord=0.14, wing_tip_chord=0.055,
+                   wing_thickness=0.018, wing_sweep=0.055, wing_root_offset=(0.02, 0.055, 0.02), tail_length=0.17,
+                   tail_half_width=0.11, tail_thickness=0.018, frequency=4.5, stroke_amplitude=r(42),
+                   stroke_bias=r(2), pitch_mean=r(8), pitch_amplitude=r(24), pitch_phase=r(78))
+
+    def to_c(self) -> _abi.BirdGeometry:
+        b = _abi.BirdGeometry()
+        for name, _ in _abi.BirdGeometry._fields_:
+            v = getattr(self, name)
+            if name in ("body_radii", "wing_root_offset"):
+                getattr(b, name)[:] = [float(x) for x in v]
+            else:
+                setattr(b, name, float(v))
         return b
 
 
@@ -338,6 +381,19 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_set_translating_sphere(self._ctx, _ptr(cr), _ptr(gv), _ptr(wv)))
         self.geometry_source = {"kind": "sphere", "center_radius": cr.tolist(), "geometry_velocity": gv.tolist(),
                                 "wall_velocity": wv.tolist()}
+
+    def set_bird(self, bird: "BirdGeometry"):
+        """Analytic signed-distance bird (BirdFlow.metal:686-938) flying on the
+        body pose; builds the current step."""
+        cb = bird.to_c()
+        self._check(self._lib.bf_set_bird(self._ctx, ctypes.byref(cb)))
+        self.geometry_source = {"kind": "bird", "bird": dataclasses.asdict(bird)}
+
+    def bird_frame(self) -> np.ndarray:
+        """Prepared frame of the current step, [14, 4] (GPUPreparedBirdGeometry rows)."""
+        out = np.zeros((14, 4), np.float32)
+        self._check(self._lib.bf_read_bird_frame(self._ctx, _ptr(out)))
+        return out
 
     # -- stepping --------------------------------------------------------
     def advance(self, steps: int, synchronize: bool = True):

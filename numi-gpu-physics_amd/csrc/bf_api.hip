@@ -123,6 +123,9 @@ struct bf_context {
     uint32_t m_frames = 0, m_vertices = 0, m_triangles = 0;
     float m_half = 0.0f;
     float m_translation[3] = {0, 0, 0};
+    // analytic bird: shape and the prepared frame of each step parity
+    BirdShape bird{};
+    float4* bird_frame = nullptr;         // [2][BIRD_FRAME_ROWS]
     // sphere
     float4 sphere_cr{};
     float3 sphere_gv{};
@@ -402,6 +405,33 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         e = launch_mesh_geometry(b, mesh_params(c, time_of(c, s), exact), c->sb);
         if (e == hipSuccess && c->frozen_count && s >= c->frozen_first && s < c->frozen_first + c->frozen_count)
             e = replay_mask(c, s, (uint32_t)(s - c->frozen_first), mod3(s));
+    } else if (c->source == BF_GEOMETRY_BIRD) {
+        // fresh: built from cleared buffers with no previous geometry
+        // (the reference's initialization build)
+        BirdBuffers b{};
+        b.body = c->body;
+        b.frame = c->bird_frame + BIRD_FRAME_ROWS * par2(s);
+        b.mask_prev = fresh ? nullptr : c->mask[mod3(s - 1)];
+        b.dyn_prev = fresh ? nullptr : c->dyn[mod3(s - 1)];
+        b.tiles_new = c->tflags[mod4(s)];
+        b.tiles_mask_old = fresh ? c->no_tiles : c->tflags[mod4(s - 3)];
+        b.tiles_wall_old = fresh ? c->no_tiles : c->tflags[mod4(s - 2)];
+        b.list = c->geo_list;
+        b.list_count = &c->counters[2];
+        b.mask = c->mask[mod3(s)];
+        b.wall = c->wall[par2(s)];
+        b.dyn = c->dyn[mod3(s)];
+        BirdParams bp{};
+        bp.g = c->g;
+        for (int a = 0; a < 3; ++a) {
+            bp.origin[a] = c->cfg.origin[a];
+            bp.periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
+        }
+        bp.dx = c->cfg.cell_size;
+        bp.time = time_of(c, s);
+        bp.velocity_scale = c->cfg.velocity_to_lattice;
+        bp.tile_pass_blocks = 4u * c->tile_pass_blocks;
+        e = launch_bird_geometry(c->bird, b, bp, c->sb);
     } else if (c->source == BF_GEOMETRY_SPHERE) {
         int periodic[3];
         for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
@@ -641,6 +671,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     c->no_tiles = (uint8_t*)A.take(e, g.tiles);
     for (int i = 0; i < 2; ++i) c->general_list[i] = (uint32_t*)A.take(e, 4ull * g.tiles);
     c->geo_list = (uint32_t*)A.take(e, 4ull * g.tiles);
+    c->bird_frame = (float4*)A.take(e, 2ull * BIRD_FRAME_ROWS * sizeof(float4));
     c->counters = (uint32_t*)A.take(e, 64);
     c->partials = (double*)A.take(e, 128ull * g.tiles);   // two steps' per-tile load partials
     c->block_out = (double*)A.take(e, 64ull * ((g.tiles + 1023) / 1024 + 1));
@@ -716,6 +747,7 @@ uint64_t bf_step_index(const bf_context* c) { return c ? (uint64_t)c->step : 0; 
 
 static bf_status rebuild_mesh_geometry(bf_context* c);
 static bf_status rebuild_sphere_geometry(bf_context* c);
+static bf_status rebuild_bird_geometry(bf_context* c);
 
 bf_status bf_read_restart_state(bf_context* c, uint64_t* step, float* mesh_time_origin) {
     BF_ENTER(c);
@@ -743,6 +775,7 @@ bf_status bf_set_restart_state(bf_context* c, uint64_t step, float mesh_time_ori
     bf_status r = BF_OK;
     if (c->source == BF_GEOMETRY_MESH) r = rebuild_mesh_geometry(c);
     if (c->source == BF_GEOMETRY_SPHERE) r = rebuild_sphere_geometry(c);
+    if (c->source == BF_GEOMETRY_BIRD) r = rebuild_bird_geometry(c);
     if (r != BF_OK || !solid_mask) return r;
     // The saved mask is the previous-step mask of the next step.  It equals
     // the rebuilt one unless the geometry followed a free-flying body whose
@@ -1042,6 +1075,61 @@ bf_status bf_set_translating_sphere(bf_context* c, const float cr[4], const floa
     return rebuild_sphere_geometry(c);
 }
 
+// Bird geometry of the current step from cleared buffers (walls zero, as
+// the reference's far cells), with no previous geometry; the previous
+// step's mask starts equal to it.
+static bf_status rebuild_bird_geometry(bf_context* c) {
+    hipStream_t s = c->sb;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->mask[i], 0, c->g.n, s);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_fill_u32((uint32_t*)c->wall[i], 0u, 4u * c->g.n, s);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->dyn[i], 0, c->g.tiles, s);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = launch_fill_u8(c->tflags[i], 0, c->g.tiles, s);
+    if (e != hipSuccess) return hip_fail(c, e, "bird geometry clear");
+    const int64_t st = c->step;
+    if (bf_status r = enqueue_geometry(c, st, true, true)) return r;
+    const int k = mod3(st), kp = mod3(st - 1);
+    e = hipMemcpyAsync(c->mask[kp], c->mask[k], c->g.n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->dyn[kp], c->dyn[k], c->g.tiles, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->tflags[mod4(st - 1)], c->tflags[mod4(st)], c->g.tiles, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e, "initial bird geometry");
+    c->geometry_ready = st;
+    return BF_OK;
+}
+
+bf_status bf_set_bird(bf_context* c, const bf_bird_geometry* bird) {
+    BF_ENTER(c);
+    if (!bird) return fail(c, BF_ERROR_INVALID_ARGUMENT, "bird geometry must not be NULL");
+    static_assert(sizeof(bf_bird_geometry) == sizeof(BirdShape), "bird layouts differ");
+    const float* v = (const float*)bird;
+    for (size_t i = 0; i < sizeof(bf_bird_geometry) / sizeof(float); ++i)
+        if (!std::isfinite(v[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "bird parameters must be finite");
+    const bf_bird_geometry& b = *bird;
+    if (!(b.body_radii[0] > 0.0f && b.body_radii[1] > 0.0f && b.body_radii[2] > 0.0f))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "body radii must be positive");
+    if (!(b.wing_span > 0.0f && b.wing_root_chord > 0.0f && b.wing_tip_chord > 0.0f && b.wing_thickness > 0.0f))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "wing span, chords and thickness must be positive");
+    if (!(b.tail_length >= 0.0f && b.tail_half_width >= 0.0f && b.tail_thickness >= 0.0f && b.frequency >= 0.0f))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "tail sizes and flapping frequency must not be negative");
+    std::memcpy(&c->bird, bird, sizeof(BirdShape));
+    c->source = BF_GEOMETRY_BIRD;
+    c->static_prev_differs = false;
+    c->time_start = 0.0f;
+    return rebuild_bird_geometry(c);
+}
+
+bf_status bf_read_bird_frame(bf_context* c, float frame[56]) {
+    BF_ENTER(c);
+    if (!frame) return fail(c, BF_ERROR_INVALID_ARGUMENT, "frame must not be NULL");
+    if (c->source != BF_GEOMETRY_BIRD) return fail(c, BF_ERROR_INVALID_ARGUMENT, "no bird geometry set");
+    hipError_t e = hipMemcpy(frame, c->bird_frame + BIRD_FRAME_ROWS * par2(c->step),
+                             BIRD_FRAME_ROWS * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read bird frame");
+    return BF_OK;
+}
+
 bf_status bf_step(bf_context* c, uint32_t steps) {
     if (!c) return BF_ERROR_INVALID_ARGUMENT;
     if (c->invalidated) return BF_ERROR_STATE_INVALIDATED;
@@ -1098,7 +1186,8 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         // reference point from the body state, so in free flight L[s+1]
         // follows the body update; a mesh riding on a free-flying body also
         // needs the pose after step s, so its geometry follows it too.
-        const bool coupled = c->free_flight && c->move_mesh && c->source == BF_GEOMETRY_MESH;
+        const bool coupled = c->free_flight && ((c->move_mesh && c->source == BF_GEOMETRY_MESH) ||
+                                                c->source == BF_GEOMETRY_BIRD);
         auto next_geometry = [&]() -> bf_status {
             if (prof_all) e = hipEventRecord(pe[2], c->sb);
             if (e == hipSuccess && c->source != BF_GEOMETRY_STATIC) {
@@ -1210,7 +1299,8 @@ bf_status bf_body_set(bf_context* c, const bf_body_state* state, const bf_body_p
         hipError_t e = hipMemcpy(c->body, v, sizeof v, hipMemcpyHostToDevice);
         if (e != hipSuccess) return hip_fail(c, e, "body upload");
         // a mesh carried by the body is placed again for the next step
-        if (c->move_mesh && c->source == BF_GEOMETRY_MESH) c->geometry_ready = c->step;
+        if ((c->move_mesh && c->source == BF_GEOMETRY_MESH) || c->source == BF_GEOMETRY_BIRD)
+            c->geometry_ready = c->step;
     }
     if (state || params) {
         // a new body (or new body parameters) starts a new safety record
@@ -1292,7 +1382,7 @@ bf_status adjoint_supported(bf_context* c) {
             if (g.boundary[a][sd] == BF_BC_OUTLET)
                 return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers periodic and far-field faces");
     if (c->free_flight) return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint holds the body fixed (no free flight)");
-    if (c->source == BF_GEOMETRY_SPHERE)
+    if (c->source == BF_GEOMETRY_SPHERE || c->source == BF_GEOMETRY_BIRD)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers mesh and static geometry");
     if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "step once after bf_set_geometry with a different previous mask");

@@ -590,6 +590,291 @@ hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStr
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Analytic signed-distance bird.  prepareBirdGeometry (BirdFlow.metal:686-814,
+// analytic kinematics) runs in one lane; buildBirdGeometry (:816-938) runs on
+// the tiles that hold cells within the geometry radius, cells that were
+// solid last step, or stale entries.  Conventions shared with the oracle:
+// rsqrt(x) = 1 / sqrt(x), normalize(v) = v * (1 / |v|), mix(a, b, t) =
+// a + (b - a) t, and bird_sincos below for sin / cos; no contraction, so the
+// flags and wall velocities equal the oracle's bit for bit.
+namespace {
+
+struct V3 {
+    float x, y, z;
+};
+__device__ inline V3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ inline V3 vadd(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline V3 vsub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline V3 vmul(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ inline float vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline float vlen(V3 a) { return sqrtf(vdot(a, a)); }
+__device__ inline V3 vcross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+__device__ inline V3 vnormalize(V3 a) { return vmul(a, 1.0f / sqrtf(vdot(a, a))); }
+__device__ inline float smin(float a, float b) { return a < b ? a : b; }
+__device__ inline float smax(float a, float b) { return a > b ? a : b; }
+__device__ inline float clamp01(float v) { return smin(smax(v, 0.0f), 1.0f); }
+__device__ inline float mixf(float a, float b, float t) { return a + (b - a) * t; }
+__device__ inline V3 row3(const float4& r) { return v3(r.x, r.y, r.z); }
+__device__ inline float4 row4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+// x = k pi/2 + r, k = rint(x 2/pi), pi/2 in three floats (fused
+// multiply-add reduction); Taylor series of sin / cos on |r| <= pi/4 through
+// r^11 / r^12 by Horner with fused multiply-adds; quadrant k mod 4.
+__device__ inline void bird_sincos(float x, float& s, float& c) {
+    const float k = __builtin_rintf(x * 0x1.45f306p-1f);
+    float r = __builtin_fmaf(-k, 0x1.921fb6p+0f, x);
+    r = __builtin_fmaf(-k, -0x1.777a5cp-25f, r);
+    r = __builtin_fmaf(-k, -0x1.ee59dap-50f, r);
+    const float r2 = r * r;
+    float ps = __builtin_fmaf(-1.0f / 39916800.0f, r2, 1.0f / 362880.0f);
+    ps = __builtin_fmaf(ps, r2, -1.0f / 5040.0f);
+    ps = __builtin_fmaf(ps, r2, 1.0f / 120.0f);
+    ps = __builtin_fmaf(ps, r2, -1.0f / 6.0f);
+    const float sr = __builtin_fmaf(r * r2, ps, r);
+    float pc = __builtin_fmaf(1.0f / 479001600.0f, r2, -1.0f / 3628800.0f);
+    pc = __builtin_fmaf(pc, r2, 1.0f / 40320.0f);
+    pc = __builtin_fmaf(pc, r2, -1.0f / 720.0f);
+    pc = __builtin_fmaf(pc, r2, 1.0f / 24.0f);
+    pc = __builtin_fmaf(pc, r2, -0.5f);
+    const float cr = __builtin_fmaf(r2, pc, 1.0f);
+    switch ((int)(long long)k & 3) {
+        case 0: s = sr; c = cr; break;
+        case 1: s = cr; c = -sr; break;
+        case 2: s = -sr; c = -cr; break;
+        default: s = -cr; c = sr; break;
+    }
+}
+
+struct Quat {
+    float x, y, z, w;
+};
+
+__device__ inline V3 qrotate(Quat q, V3 v) {
+    const V3 u = v3(q.x, q.y, q.z);
+    const V3 t = vmul(vcross(u, v), 2.0f);
+    return vadd(vadd(v, vmul(t, q.w)), vcross(u, t));
+}
+
+__device__ inline V3 rotate_axis(V3 value, V3 axis, float angle) {
+    float s, c;
+    bird_sincos(angle, s, c);
+    return vadd(vadd(vmul(value, c), vmul(vcross(axis, value), s)), vmul(vmul(axis, vdot(axis, value)), 1.0f - c));
+}
+
+__device__ inline float sd_box(float dx, float dy, float dz) {
+    return vlen(v3(smax(dx, 0.0f), smax(dy, 0.0f), smax(dz, 0.0f))) + smin(smax(dx, smax(dy, dz)), 0.0f);
+}
+
+__device__ inline float sd_ellipsoid(V3 p, const BirdShape& b) {
+    const float rx = b.body_radii[0], ry = b.body_radii[1], rz = b.body_radii[2];
+    const float k0 = vlen(v3(p.x / rx, p.y / ry, p.z / rz));
+    const float k1 = vlen(v3(p.x / (rx * rx), p.y / (ry * ry), p.z / (rz * rz)));
+    return k1 > 1.0e-12f ? k0 * (k0 - 1.0f) / k1 : -smin(rx, smin(ry, rz));
+}
+
+__device__ inline float sd_wing(V3 l, const BirdShape& b) {
+    const float frac = clamp01(l.y / smax(b.wing_span, 1.0e-6f));
+    const float chord = mixf(b.wing_root_chord, b.wing_tip_chord, frac);
+    const float center = -b.wing_sweep * frac;
+    return sd_box(fabsf(l.x - center) - 0.5f * chord, smax(-l.y, l.y - b.wing_span),
+                  fabsf(l.z) - 0.5f * b.wing_thickness);
+}
+
+__device__ inline float sd_tail(V3 l, const BirdShape& b) {
+    const float from_root = -(l.x + b.body_radii[0]);
+    const float frac = clamp01(from_root / smax(b.tail_length, 1.0e-6f));
+    const float half_width = mixf(0.35f * b.tail_half_width, b.tail_half_width, frac);
+    return sd_box(smax(-from_root, from_root - b.tail_length), fabsf(l.y) - half_width,
+                  fabsf(l.z + 0.15f * b.body_radii[2]) - 0.5f * b.tail_thickness);
+}
+
+// makeWingFrame (BirdFlow.metal:611-684) with zero deviation and tip twist;
+// writes root, chord (w: tip twist), span, normal, angular velocity (w: tip
+// twist rate).
+__device__ inline void wing_frame(float side, float stroke, float stroke_rate, float pitch, float pitch_rate,
+                                  const BirdShape& b, Quat q, V3 body_pos, float4* out) {
+    const V3 ex = qrotate(q, v3(1.0f, 0.0f, 0.0f));
+    const V3 ey = qrotate(q, v3(0.0f, 1.0f, 0.0f));
+    const V3 ez = qrotate(q, v3(0.0f, 0.0f, 1.0f));
+    const V3 root = vadd(body_pos, qrotate(q, v3(b.wing_root_offset[0], side * b.wing_root_offset[1],
+                                                 b.wing_root_offset[2])));
+    const float signed_stroke = side * stroke;
+    const V3 span_stroke = rotate_axis(vmul(ey, side), ex, signed_stroke);
+    const V3 normal_stroke = rotate_axis(vmul(ez, side), ex, signed_stroke);
+    const float deviation = 0.0f, deviation_rate = 0.0f;
+    const V3 sp = rotate_axis(span_stroke, normal_stroke, deviation);
+    const V3 chord_pre = rotate_axis(ex, normal_stroke, deviation);
+    const float signed_pitch = side * pitch;
+    const V3 ch = rotate_axis(chord_pre, sp, signed_pitch);
+    const V3 nm = rotate_axis(normal_stroke, sp, signed_pitch);
+    const V3 spn = vnormalize(sp);
+    const V3 w = vadd(vadd(vmul(ex, side * stroke_rate), vmul(normal_stroke, deviation_rate)),
+                      vmul(spn, side * pitch_rate));
+    const float tip_twist = 0.0f, tip_twist_rate = 0.0f;
+    out[0] = row4(root, 0.0f);
+    out[1] = row4(vnormalize(ch), side * tip_twist);
+    out[2] = row4(spn, 0.0f);
+    out[3] = row4(vnormalize(nm), 0.0f);
+    out[4] = row4(w, side * tip_twist_rate);
+}
+
+}  // namespace
+
+// One lane: the frame rows body position, orientation, linear velocity,
+// (world angular velocity, geometry radius), then left and right wing
+// (root, chord, span, normal, angular velocity).
+__global__ void bird_prepare(BirdShape b, const double* __restrict__ body, float time, float dx,
+                             float4* __restrict__ frame) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const V3 pos = v3((float)body[0], (float)body[1], (float)body[2]);
+    Quat q = {(float)body[3], (float)body[4], (float)body[5], (float)body[6]};
+    const float n2 = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+    if (n2 > 1.0e-20f) {
+        const float inv = 1.0f / sqrtf(n2);
+        q = {q.x * inv, q.y * inv, q.z * inv, q.w * inv};
+    } else {
+        q = {0.0f, 0.0f, 0.0f, 1.0f};
+    }
+    const V3 lin = v3((float)body[7], (float)body[8], (float)body[9]);
+    const V3 wb = v3((float)body[10], (float)body[11], (float)body[12]);
+    const float omega = 2.0f * 3.14159265358979323846f * b.frequency;
+    const float phase = omega * time;
+    float s0, c0, s1, c1;
+    bird_sincos(phase, s0, c0);
+    bird_sincos(phase + b.pitch_phase, s1, c1);
+    const float stroke = b.stroke_bias + b.stroke_amplitude * s0;
+    const float stroke_rate = omega * b.stroke_amplitude * c0;
+    const float pitch = b.pitch_mean + b.pitch_amplitude * s1;
+    const float pitch_rate = omega * b.pitch_amplitude * c1;
+    float4 rows[BIRD_FRAME_ROWS];
+    wing_frame(1.0f, stroke, stroke_rate, pitch, pitch_rate, b, q, pos, rows + 4);
+    wing_frame(-1.0f, stroke, stroke_rate, pitch, pitch_rate, b, q, pos, rows + 9);
+    rows[0] = row4(pos, 0.0f);
+    rows[1] = make_float4(q.x, q.y, q.z, q.w);
+    rows[2] = row4(lin, 0.0f);
+    const float body_radius = vlen(v3(b.body_radii[0], b.body_radii[1], b.body_radii[2]));
+    const float tail_radius = vlen(v3(b.body_radii[0] + b.tail_length, smax(b.body_radii[1], b.tail_half_width),
+                                      b.body_radii[2] + 0.5f * b.tail_thickness));
+    const float root_radius = vlen(v3(b.wing_root_offset[0], b.wing_root_offset[1], b.wing_root_offset[2]));
+    const float wing_radius = root_radius + b.wing_span + fabsf(b.wing_sweep)
+                              + 0.5f * smax(b.wing_root_chord, b.wing_tip_chord) + 0.5f * b.wing_thickness;
+    const float radius = smax(body_radius, smax(tail_radius, wing_radius)) + 2.0f * dx;
+    rows[3] = row4(qrotate(q, wb), radius);
+    for (int i = 0; i < BIRD_FRAME_ROWS; ++i) frame[i] = rows[i];
+}
+
+// Tiles written this step: those with a cell centre within one cell of the
+// geometry sphere's bounding box, and (with a previous geometry) those near
+// last step's solid cells, whose cells the build evaluates even beyond the
+// radius.  Every tile's flag is written.
+__global__ __launch_bounds__(256) void bird_mark_tiles(const float4* __restrict__ frame,
+                                                       const uint8_t* __restrict__ dyn_prev,
+                                                       uint8_t* __restrict__ tiles_new, BirdParams p) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= p.g.tiles) return;
+    int x0, y0, z;
+    tile_origin(p.g, t, x0, y0, z);
+    const int x1 = min(x0 + TILE_X, p.g.nx) - 1, y1 = min(y0 + TILE_Y, p.g.ny) - 1;
+    const float4 c = frame[0];
+    const float reach = frame[3].w + p.dx;
+    const bool hit = p.origin[0] + ((float)x1 + 0.5f) * p.dx >= c.x - reach
+                     && p.origin[0] + ((float)x0 + 0.5f) * p.dx <= c.x + reach
+                     && p.origin[1] + ((float)y1 + 0.5f) * p.dx >= c.y - reach
+                     && p.origin[1] + ((float)y0 + 0.5f) * p.dx <= c.y + reach
+                     && fabsf(p.origin[2] + ((float)z + 0.5f) * p.dx - c.z) <= reach;
+    tiles_new[t] = (hit || (dyn_prev && dyn_prev[t])) ? 1 : 0;
+}
+
+// buildBirdGeometry on the listed tiles; every cell of a listed tile gets
+// the reference's value (mask 0 and wall 0 beyond the radius unless solid
+// last step).  Solid cells mark the tiles whose cells pull from them.
+__global__ __launch_bounds__(256) void bird_tile_pass(BirdShape b, const float4* __restrict__ frame,
+                                                      const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count,
+                                                      const uint8_t* __restrict__ mask_prev,
+                                                      uint8_t* __restrict__ mask, float4* __restrict__ wall,
+                                                      uint8_t* __restrict__ dyn, BirdParams p) {
+    __shared__ float4 fr[BIRD_FRAME_ROWS];
+    if (threadIdx.x < BIRD_FRAME_ROWS) fr[threadIdx.x] = frame[threadIdx.x];
+    __syncthreads();
+    const V3 body_pos = row3(fr[0]);
+    const Quat q = {fr[1].x, fr[1].y, fr[1].z, fr[1].w};
+    const Quat qc = {-q.x, -q.y, -q.z, q.w};
+    const V3 lin = row3(fr[2]);
+    const V3 omega = row3(fr[3]);
+    const float radius = fr[3].w;
+    const uint32_t n_tiles = *count;
+    for (uint32_t i = blockIdx.x; i < n_tiles; i += gridDim.x) {
+        const uint32_t tile = list[i];
+        int x0, y0, z;
+        tile_origin(p.g, tile, x0, y0, z);
+        const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+        const int y = y0 + (int)(threadIdx.x / TILE_X);
+        if (x >= p.g.nx || y >= p.g.ny) continue;
+        const uint32_t gid = (uint32_t)x + (uint32_t)p.g.nx * ((uint32_t)y + (uint32_t)p.g.ny * (uint32_t)z);
+        const V3 world = v3(p.origin[0] + ((float)x + 0.5f) * p.dx, p.origin[1] + ((float)y + 0.5f) * p.dx,
+                            p.origin[2] + ((float)z + 0.5f) * p.dx);
+        const V3 rel = vsub(world, body_pos);
+        const bool was_solid = mask_prev && mask_prev[gid] != 0;
+        if (!was_solid && vdot(rel, rel) > radius * radius) {
+            mask[gid] = 0;
+            wall[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            continue;
+        }
+        const V3 local = qrotate(qc, rel);
+        const V3 base = vadd(lin, vcross(omega, rel));
+        float best = sd_ellipsoid(local, b);
+        V3 velocity = base;
+        uint8_t part = 1;
+#pragma unroll
+        for (int wing = 0; wing < 2; ++wing) {
+            const float4* f = fr + 4 + 5 * wing;
+            const V3 root = row3(f[0]), chord = row3(f[1]), span = row3(f[2]), normal = row3(f[3]);
+            const V3 ang = row3(f[4]);
+            const float tip_twist = f[1].w, tip_twist_rate = f[4].w;
+            const V3 r = vsub(world, root);
+            const V3 lu = v3(vdot(r, chord), vdot(r, span), vdot(r, normal));
+            const float frac = clamp01(lu.y / smax(b.wing_span, 1.0e-6f));
+            float ts, tc;
+            bird_sincos(tip_twist * frac, ts, tc);
+            const V3 l = v3(tc * lu.x + ts * lu.z, lu.y, -ts * lu.x + tc * lu.z);
+            const float d = sd_wing(l, b);
+            if (d < best) {
+                best = d;
+                const V3 twist_axis = vmul(span, frac * tip_twist_rate);
+                velocity = vadd(vadd(base, vcross(ang, r)), vcross(twist_axis, vsub(world, vadd(root, vmul(span, lu.y)))));
+                part = wing == 0 ? 2 : 3;
+            }
+        }
+        const float tail = sd_tail(local, b);
+        if (tail < best) {
+            best = tail;
+            velocity = base;
+            part = 4;
+        }
+        const bool solid = best <= 0.0f;
+        mask[gid] = solid ? part : 0;
+        wall[gid] = make_float4(velocity.x * p.velocity_scale, velocity.y * p.velocity_scale,
+                                velocity.z * p.velocity_scale, 0.0f);
+        if (solid) mark_tiles(p.g, p.periodic, dyn, x, y, z);
+    }
+}
+
+hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s) {
+    const unsigned tile_blocks = (p.g.tiles + 255) / 256;
+    hipLaunchKernelGGL(bird_prepare, dim3(1), dim3(64), 0, s, shape, b.body, p.time, p.dx, b.frame);
+    hipLaunchKernelGGL(bird_mark_tiles, dim3(tile_blocks), dim3(256), 0, s, (const float4*)b.frame, b.dyn_prev,
+                       b.tiles_new, p);
+    hipLaunchKernelGGL(fill_u8, dim3(tile_blocks), dim3(256), 0, s, b.dyn, (uint8_t)0, p.g.tiles);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
+    hipLaunchKernelGGL(geometry_tile_list, dim3(tile_blocks), dim3(256), 0, s, b.tiles_new, b.tiles_mask_old,
+                       b.tiles_wall_old, p.g.tiles, b.list, b.list_count);
+    hipLaunchKernelGGL(bird_tile_pass, dim3(p.tile_pass_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
+                       b.list, b.list_count, b.mask_prev, b.mask, b.wall, b.dyn, p);
+    return hipGetLastError();
+}
+
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
     if (m.g.nx > 65535 || m.g.ny > 65535) return hipErrorInvalidValue;   // raster packs x | y << 16
     if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1 || m.raster_split < 1 || m.raster_split > 64)

@@ -51,6 +51,43 @@ struct MeshParams {
     uint32_t tile_pass_blocks; // tile-pass grid (grid-stride over listed tiles)
 };
 
+// Analytic signed-distance bird (BirdFlow.metal:486-533, 611-938): shape
+// and analytic wing kinematics as GPUBirdParameters holds them (metres,
+// radians, Hz), same field order as bf_bird_geometry.
+struct BirdShape {
+    float body_radii[3];
+    float wing_span, wing_root_chord, wing_tip_chord, wing_thickness, wing_sweep;
+    float wing_root_offset[3];
+    float tail_length, tail_half_width, tail_thickness;
+    float frequency, stroke_amplitude, stroke_bias, pitch_mean, pitch_amplitude, pitch_phase;
+};
+constexpr int BIRD_FRAME_ROWS = 14;   // GPUPreparedBirdGeometry: 14 float4
+
+struct BirdParams {
+    Grid g;
+    float origin[3];
+    float dx;
+    float time;                // geometry time of the step (s)
+    float velocity_scale;      // metres / second -> lattice
+    int periodic[3];
+    uint32_t tile_pass_blocks;
+};
+
+struct BirdBuffers {
+    const double* body;               // device body state (FP64)
+    float4* frame;                    // prepared frame of this step (written)
+    const uint8_t* mask_prev;         // previous step's mask, or null: no previous geometry
+    const uint8_t* dyn_prev;          // tiles near the previous step's solid cells, or null
+    uint8_t* tiles_new;               // tiles written this step (written here)
+    const uint8_t* tiles_mask_old;    // tiles whose `mask` entries an earlier step wrote
+    const uint8_t* tiles_wall_old;    // tiles whose `wall` entries an earlier step wrote
+    uint32_t* list;                   // scratch: tiles visited by the tile pass
+    uint32_t* list_count;
+    uint8_t* mask;
+    float4* wall;
+    uint8_t* dyn;
+};
+
 struct MeshBuffers {
     const double* body;                 // body state, read when MeshParams::move_with_body
     const float* points;
@@ -153,6 +190,7 @@ hipError_t launch_convert_populations(int storage, bool pack, const void* src, v
                                       const Grid& g, hipStream_t s);
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
 hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
+hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s);
 hipError_t launch_sphere_geometry(uint8_t* mask, float4* wall, uint8_t* dyn, const Grid& g, float4 cr, float3 gv,
                                   float4 wv, float t, const int periodic[3], hipStream_t s);
 hipError_t launch_mark_tiles(const uint8_t* mask, uint8_t* dyn, const Grid& g, const int periodic[3], bool clear,

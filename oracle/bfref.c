@@ -500,3 +500,5 @@ void bfo_place_vertices(const bfo_body* b, int vertex_count, float velocity_scal
         }
     }
 }
+
+#include "bfref_bird.inc"

@@ -25,6 +25,7 @@
  *   - translating-sphere topology canonical  BirdFlow.metal:183-195,2263-2350
  *   - six-degree-of-freedom body update  BirdFlow.metal:6574-6627,
  *   - runtime safety monitor             BirdFlow.metal:6629-6707,
+ *   - analytic signed-distance bird      BirdFlow.metal:486-533,611-938
  *       Sources/BirdFlowCore/RigidBody.swift:85-127
  *
  * Fluid functions exist in two precisions: *_f32 (same operation order as the
@@ -187,6 +188,37 @@ uint32_t bfo_body_safety(const bfo_body* b, const double extent[3], double rotat
 void bfo_place_vertices(const bfo_body* b, int vertex_count, float velocity_scale,
                         const float* local_position3, const float* local_velocity3,
                         float* position3, float* velocity3);
+
+/* ---- analytic signed-distance bird (FP32) ---- */
+/* GPUBirdParameters geometry and analytic kinematics (GPUData.swift:82-140):
+ * lengths in metres, angles in radians, frequency in Hz. */
+typedef struct bfo_bird {
+    float body_radii[3];
+    float wing_span, wing_root_chord, wing_tip_chord, wing_thickness, wing_sweep;
+    float wing_root_offset[3];
+    float tail_length, tail_half_width, tail_thickness;
+    float frequency, stroke_amplitude, stroke_bias, pitch_mean, pitch_amplitude, pitch_phase;
+} bfo_bird;
+
+/* GPUPreparedBirdGeometry (GPUData.swift:325-340): 14 float4. */
+typedef struct bfo_bird_frame {
+    float body_position[4], orientation[4], linear_velocity[4], omega_world_radius[4];
+    float left_root[4], left_chord[4], left_span[4], left_normal[4], left_angular[4];
+    float right_root[4], right_chord[4], right_span[4], right_normal[4], right_angular[4];
+} bfo_bird_frame;
+
+/* sin / cos of the geometry (quadrant reduction + Taylor polynomials). */
+void bfo_sincos(float x, float* s, float* c);
+void bfo_sincos_n(const float* x, int n, float* s, float* c);
+/* prepareBirdGeometry (BirdFlow.metal:686-814) at `time` for a body pose. */
+void bfo_bird_prepare(const bfo_bird* b, const float body_position[3], const float orientation[4],
+                      const float linear_velocity[3], const float angular_velocity_body[3], float time, float dx,
+                      bfo_bird_frame* out);
+/* buildBirdGeometry (BirdFlow.metal:816-938) over the grid of p (origin, dx):
+ * part mask 1 body, 2 left wing, 3 right wing, 4 tail; wall = (velocity *
+ * velocity_to_lattice, 0).  solid_prev NULL: no previous geometry. */
+void bfo_bird_build(const bfo_params* p, const bfo_bird* b, const bfo_bird_frame* f, const uint8_t* solid_prev,
+                    float velocity_to_lattice, uint8_t* mask, float* wall4);
 
 #ifdef __cplusplus
 }

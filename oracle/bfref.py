@@ -60,6 +60,24 @@ class Body(ctypes.Structure):
     ]
 
 
+class Bird(ctypes.Structure):
+    """bfo_bird: GPUBirdParameters geometry and analytic kinematics."""
+    _fields_ = [
+        ("body_radii", ctypes.c_float * 3),
+        ("wing_span", ctypes.c_float), ("wing_root_chord", ctypes.c_float), ("wing_tip_chord", ctypes.c_float),
+        ("wing_thickness", ctypes.c_float), ("wing_sweep", ctypes.c_float),
+        ("wing_root_offset", ctypes.c_float * 3),
+        ("tail_length", ctypes.c_float), ("tail_half_width", ctypes.c_float), ("tail_thickness", ctypes.c_float),
+        ("frequency", ctypes.c_float), ("stroke_amplitude", ctypes.c_float), ("stroke_bias", ctypes.c_float),
+        ("pitch_mean", ctypes.c_float), ("pitch_amplitude", ctypes.c_float), ("pitch_phase", ctypes.c_float),
+    ]
+
+
+BIRD_FRAME_FIELDS = ("body_position", "orientation", "linear_velocity", "omega_world_radius",
+                     "left_root", "left_chord", "left_span", "left_normal", "left_angular",
+                     "right_root", "right_chord", "right_span", "right_normal", "right_angular")
+
+
 def build() -> str:
     """Compile the oracle with its Makefile (used where a toolchain exists)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -117,6 +135,9 @@ def _declare(l):
                                   vp, vp, vp, ctypes.c_double, ctypes.c_uint64, vp, vp]
     l.bfo_body_safety.restype = ctypes.c_uint32
     l.bfo_place_vertices.argtypes = [ctypes.POINTER(Body), ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
+    l.bfo_sincos_n.argtypes = [vp, ctypes.c_int, vp, vp]
+    l.bfo_bird_prepare.argtypes = [ctypes.POINTER(Bird), vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp]
+    l.bfo_bird_build.argtypes = [P, ctypes.POINTER(Bird), vp, vp, ctypes.c_float, vp, vp]
 
 
 def make_params(shape, *, omega_plus, omega_minus, bc=None, collision=COLLISION_TRT,
@@ -405,3 +426,46 @@ def interpolate_frames_f64(points, times, t, velocity_scale=1.0):
     duration = times[first + 1] - times[first]
     blend = min(max((t - times[first]) / duration, 0.0), 1.0)
     return a + blend * (b - a), (b - a) / duration * velocity_scale
+
+
+# ---------------------------------------------------------------------------
+# analytic signed-distance bird
+# ---------------------------------------------------------------------------
+def make_bird(**kw) -> Bird:
+    """bfo_bird from keyword fields (sequences for the 3-vectors)."""
+    b = Bird()
+    for k, v in kw.items():
+        if k in ("body_radii", "wing_root_offset"):
+            getattr(b, k)[:] = [float(x) for x in v]
+        else:
+            setattr(b, k, float(v))
+    return b
+
+
+def sincos(x):
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib().bfo_sincos_n(_ptr(x), x.size, _ptr(s), _ptr(c))
+    return s, c
+
+
+def bird_prepare(bird: Bird, position, orientation=(0, 0, 0, 1), linear_velocity=(0, 0, 0),
+                 angular_velocity_body=(0, 0, 0), time=0.0, dx=1.0) -> np.ndarray:
+    """Prepared frame [14, 4] (rows in BIRD_FRAME_FIELDS order)."""
+    out = np.zeros((14, 4), np.float32)
+    arrs = [np.ascontiguousarray(v, np.float32) for v in (position, orientation, linear_velocity,
+                                                          angular_velocity_body)]
+    lib().bfo_bird_prepare(ctypes.byref(bird), *[_ptr(a) for a in arrs], np.float32(time), np.float32(dx), _ptr(out))
+    return out
+
+
+def bird_build(p: Params, bird: Bird, frame, solid_prev=None, velocity_to_lattice=1.0):
+    n = cells(p)
+    mask = np.zeros(n, np.uint8)
+    wall4 = np.zeros((n, 4), np.float32)
+    fr = np.ascontiguousarray(frame, np.float32)
+    prev = None if solid_prev is None else np.ascontiguousarray(solid_prev, np.uint8)
+    lib().bfo_bird_build(ctypes.byref(p), ctypes.byref(bird), _ptr(fr), _ptr(prev), np.float32(velocity_to_lattice),
+                         _ptr(mask), _ptr(wall4))
+    return mask, wall4

@@ -189,3 +189,40 @@ def run_coupled_body_oracle(p, seq, steps, body, params, dtype=np.float32):
         states.append(bfref.body_array(body))
         prev = mask
     return f, np.array(loads), np.array(states), prev
+
+
+# ---------------------------------------------------------------------------
+# Analytic signed-distance bird: the reference's compact execution case
+# (Tests/BirdFlowMetalTests/MetalExecutionTests.swift:411-470).
+# ---------------------------------------------------------------------------
+COMPACT_BIRD = dict(body_radii=(0.015, 0.011, 0.012), wing_span=0.025, wing_root_chord=0.020, wing_tip_chord=0.012,
+                    wing_thickness=0.008, wing_sweep=0.004, wing_root_offset=(0.0, 0.010, 0.003), tail_length=0.020,
+                    tail_half_width=0.014, tail_thickness=0.008, frequency=3.0, stroke_amplitude=0.25,
+                    stroke_bias=0.0, pitch_mean=0.0, pitch_amplitude=0.15, pitch_phase=0.4)
+
+
+def compact_bird_case(free_flight=False, shape=(48, 48, 48)):
+    """(oracle params, SolverConfig, BirdGeometry, oracle Bird, scaling): the
+    compact bird at the domain centre, LatticeScaling(chord 0.02 m over 8
+    cells, 1 m/s, Re 100, air 1.225, lattice speed 0.03), far field -1 m/s
+    in x (0 in free flight), sponge 0.04 over 4 cells, far-field faces,
+    TRT, halfway walls, force mode 6."""
+    import birdflow_hip as bh
+    sc = bh.LatticeScaling.make(0.020, 8, 1.0, 100.0, 1.225, lattice_reference_speed=0.03)
+    dx = np.float32(sc.cell_size)
+    center = tuple(float(np.float32(0.5 * n * dx)) for n in shape)
+    far = (0.0, 0.0, 0.0) if free_flight else (float(np.float32(-1.0) * np.float32(sc.velocity_to_lattice)), 0.0, 0.0)
+    p = bfref.make_params(shape, omega_plus=sc.omega_plus, omega_minus=sc.omega_minus,
+                          bc=((bfref.BC_FARFIELD, bfref.BC_FARFIELD),) * 3, collision=bfref.COLLISION_TRT,
+                          wall_rule=bfref.WALL_HALFWAY, force_mode=6, accumulate_loads=True, sponge_strength=0.04,
+                          sponge_width=4.0, far_u=far, origin=(0.0, 0.0, 0.0), dx=float(dx),
+                          force_scale=sc.force_to_physical, body_position=center)
+    cfg = solver_config(p, time_step=sc.time_step, velocity_to_lattice=sc.velocity_to_lattice)
+    return p, cfg, bh.BirdGeometry(**COMPACT_BIRD), bfref.make_bird(**COMPACT_BIRD), sc
+
+
+def bird_frame_oracle(bird, body, time, dx):
+    """Oracle frame from an FP64 body 13-vector, cast to FP32 as the device does."""
+    b = np.asarray(body, np.float64)
+    return bfref.bird_prepare(bird, b[0:3].astype(np.float32), b[3:7].astype(np.float32),
+                              b[7:10].astype(np.float32), b[10:13].astype(np.float32), time=time, dx=dx)

@@ -1,0 +1,231 @@
+"""Pins the oracle's analytic signed-distance bird (BirdFlow.metal:486-533,
+611-938) with closed forms and invariants, none of which retypes the
+oracle's expressions:
+
+  * sin / cos against FP64 numpy (<= 2 ulp where |value| > 1e-3, 1e-7
+    absolute everywhere) and exact at 0;
+  * wing frames at rest are the body axes (span +-y, chord x, normal +-z),
+    the roots sit at the body position plus the mirrored root offset;
+  * the stroke rotates the left span to (0, cos theta, sin theta) and the
+    right one to (0, -cos theta, sin theta), theta = bias + A sin(2 pi f t);
+  * chord, span, normal are orthonormal and right-handed for any pose;
+  * the wing angular velocity reproduces central differences of the frames;
+  * the grid radius is the closed-form bound plus two cells;
+  * occupancy of an isolated ellipsoid body equals the implicit inside
+    test away from the surface;
+  * body-cell wall velocities are the rigid-body field v + w x r (scaled),
+    wing cells add the wing's relative rotation about its root;
+  * a symmetric pose gives a mask mirrored across the body x-z plane with
+    the wing part ids swapped;
+  * a cell solid in the previous mask is evaluated even beyond the radius.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import bfref
+
+DEMO = dict(body_radii=(0.18, 0.065, 0.075), wing_span=0.34, wing_root_chord=0.14, wing_tip_chord=0.055,
+            wing_thickness=0.018, wing_sweep=0.055, wing_root_offset=(0.02, 0.055, 0.02), tail_length=0.17,
+            tail_half_width=0.11, tail_thickness=0.018, frequency=4.5, stroke_amplitude=math.radians(42),
+            stroke_bias=math.radians(2), pitch_mean=math.radians(8), pitch_amplitude=math.radians(24),
+            pitch_phase=math.radians(78))
+COMPACT = dict(body_radii=(0.015, 0.011, 0.012), wing_span=0.025, wing_root_chord=0.020, wing_tip_chord=0.012,
+               wing_thickness=0.008, wing_sweep=0.004, wing_root_offset=(0.0, 0.010, 0.003), tail_length=0.020,
+               tail_half_width=0.014, tail_thickness=0.008, frequency=3.0, stroke_amplitude=0.25, stroke_bias=0.0,
+               pitch_mean=0.0, pitch_amplitude=0.15, pitch_phase=0.4)
+FIELDS = {name: i for i, name in enumerate(bfref.BIRD_FRAME_FIELDS)}
+
+
+def row(frame, name):
+    return frame[FIELDS[name]].astype(np.float64)
+
+
+def test_sincos_against_fp64():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([np.linspace(-400, 400, 400001), rng.uniform(-1e3, 1e3, 100000),
+                        np.arange(-64, 65) * (math.pi / 4)]).astype(np.float32)
+    s, c = bfref.sincos(x)
+    xd = x.astype(np.float64)
+    for got, want in ((s, np.sin(xd)), (c, np.cos(xd))):
+        err = np.abs(got.astype(np.float64) - want)
+        assert err.max() <= 1e-7
+        big = np.abs(want) > 1e-3
+        ulp = np.spacing(np.abs(want[big]).astype(np.float32)).astype(np.float64)
+        assert (err[big] / ulp).max() <= 2.0
+    s0, c0 = bfref.sincos([0.0])
+    assert s0[0] == 0.0 and c0[0] == 1.0
+
+
+def test_wing_frames_at_rest_are_the_body_axes():
+    b = bfref.make_bird(**dict(COMPACT, stroke_amplitude=0.0, pitch_amplitude=0.0))
+    pos = np.array([0.3, 0.2, 0.1])
+    f = bfref.bird_prepare(b, pos, time=0.123, dx=0.002)
+    off = np.array(COMPACT["wing_root_offset"])
+    for side, name in ((1, "left"), (-1, "right")):
+        assert np.allclose(row(f, f"{name}_span")[:3], (0, side, 0), atol=1e-7)
+        assert np.allclose(row(f, f"{name}_chord")[:3], (1, 0, 0), atol=1e-7)
+        assert np.allclose(row(f, f"{name}_normal")[:3], (0, 0, side), atol=1e-7)
+        assert np.allclose(row(f, f"{name}_root")[:3], pos + off * (1, side, 1), atol=1e-7)
+        assert np.allclose(row(f, f"{name}_angular"), 0.0, atol=0)
+    assert np.allclose(row(f, "body_position")[:3], pos, atol=1e-7)
+    assert np.array_equal(row(f, "orientation"), [0, 0, 0, 1])
+
+
+@pytest.mark.parametrize("t", [0.0, 0.01, 0.037, 0.1, 0.29])
+def test_stroke_rotates_the_spans_about_the_body_x_axis(t):
+    b = bfref.make_bird(**dict(DEMO, pitch_mean=0.0, pitch_amplitude=0.0))
+    f = bfref.bird_prepare(b, (0, 0, 0), time=t, dx=0.005)
+    theta = DEMO["stroke_bias"] + DEMO["stroke_amplitude"] * math.sin(2 * math.pi * DEMO["frequency"] * t)
+    rate = 2 * math.pi * DEMO["frequency"] * DEMO["stroke_amplitude"] * math.cos(2 * math.pi * DEMO["frequency"] * t)
+    assert np.allclose(row(f, "left_span")[:3], (0, math.cos(theta), math.sin(theta)), atol=2e-6)
+    assert np.allclose(row(f, "right_span")[:3], (0, -math.cos(theta), math.sin(theta)), atol=2e-6)
+    assert np.allclose(row(f, "left_normal")[:3], (0, -math.sin(theta), math.cos(theta)), atol=2e-6)
+    assert np.allclose(row(f, "left_angular")[:3], (rate, 0, 0), rtol=2e-6, atol=2e-6)
+    assert np.allclose(row(f, "right_angular")[:3], (-rate, 0, 0), rtol=2e-6, atol=2e-6)
+
+
+def random_pose(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    return rng.uniform(-1, 1, 3), q, rng.normal(size=3), rng.normal(size=3)
+
+
+def test_frames_are_orthonormal_and_right_handed():
+    rng = np.random.default_rng(7)
+    b = bfref.make_bird(**DEMO)
+    for _ in range(50):
+        pos, q, v, w = random_pose(rng)
+        f = bfref.bird_prepare(b, pos, q * rng.uniform(0.5, 2.0), v, w, time=rng.uniform(0, 2), dx=0.004)
+        assert abs(np.linalg.norm(row(f, "orientation")) - 1) < 1e-6
+        for name in ("left", "right"):
+            c, s, n = (row(f, f"{name}_{k}")[:3] for k in ("chord", "span", "normal"))
+            for a in (c, s, n):
+                assert abs(np.linalg.norm(a) - 1) < 2e-6
+            assert abs(c @ s) < 2e-6 and abs(c @ n) < 2e-6 and abs(s @ n) < 2e-6
+            assert np.allclose(np.cross(c, s), n, atol=4e-6)
+
+
+def test_wing_angular_velocity_matches_frame_differences():
+    rng = np.random.default_rng(3)
+    b = bfref.make_bird(**DEMO)
+    pos, q, _, _ = random_pose(rng)
+    h = 2e-4
+    for t in (0.013, 0.071, 0.18):
+        fp = bfref.bird_prepare(b, pos, q, time=t + h, dx=0.004)
+        fm = bfref.bird_prepare(b, pos, q, time=t - h, dx=0.004)
+        f = bfref.bird_prepare(b, pos, q, time=t, dx=0.004)
+        for name in ("left", "right"):
+            w = row(f, f"{name}_angular")[:3]
+            for k in ("chord", "span", "normal"):
+                d = (row(fp, f"{name}_{k}")[:3] - row(fm, f"{name}_{k}")[:3]) / (2 * h)
+                want = np.cross(w, row(f, f"{name}_{k}")[:3])
+                assert np.linalg.norm(d - want) <= 2e-3 * np.linalg.norm(w) + 1e-3
+
+
+def test_geometry_radius_is_the_closed_form_bound():
+    b = bfref.make_bird(**DEMO)
+    dx = 0.0037
+    f = bfref.bird_prepare(b, (0, 0, 0), time=0.0, dx=dx)
+    r = np.array(DEMO["body_radii"])
+    body = np.linalg.norm(r)
+    tail = np.linalg.norm([r[0] + DEMO["tail_length"], max(r[1], DEMO["tail_half_width"]),
+                           r[2] + 0.5 * DEMO["tail_thickness"]])
+    wing = (np.linalg.norm(DEMO["wing_root_offset"]) + DEMO["wing_span"] + abs(DEMO["wing_sweep"])
+            + 0.5 * max(DEMO["wing_root_chord"], DEMO["wing_tip_chord"]) + 0.5 * DEMO["wing_thickness"])
+    assert abs(row(f, "omega_world_radius")[3] - (max(body, tail, wing) + 2 * dx)) < 1e-6
+
+
+def grid(shape, dx, origin=(0.0, 0.0, 0.0)):
+    return bfref.make_params(shape, omega_plus=1.0, omega_minus=1.0, origin=origin, dx=dx)
+
+
+def cell_centers(p):
+    z, y, x = np.meshgrid(np.arange(p.nz), np.arange(p.ny), np.arange(p.nx), indexing="ij")
+    o = np.array(p.origin, np.float64)
+    return np.stack([o[0] + (x.ravel() + 0.5) * p.dx, o[1] + (y.ravel() + 0.5) * p.dx,
+                     o[2] + (z.ravel() + 0.5) * p.dx], axis=1)
+
+
+def test_isolated_ellipsoid_occupancy_is_the_implicit_inside_test():
+    tiny = dict(COMPACT, wing_span=1e-4, wing_root_chord=1e-4, wing_tip_chord=1e-4, wing_thickness=1e-4,
+                wing_sweep=0.0, wing_root_offset=(0.0, 0.0, 0.0), tail_length=1e-4, tail_half_width=1e-4,
+                tail_thickness=1e-4)
+    b = bfref.make_bird(**tiny)
+    p = grid((40, 32, 32), 0.001)
+    pos = np.array([0.0201, 0.0163, 0.0158])
+    f = bfref.bird_prepare(b, pos, time=0.0, dx=p.dx)
+    mask, _ = bfref.bird_build(p, b, f)
+    rel = cell_centers(p) - pos
+    F = ((rel / np.array(tiny["body_radii"])) ** 2).sum(axis=1)
+    assert (mask[F < 0.999] == 1).all()
+    assert (mask[F > 1.001] == 0).all()
+    assert (mask != 0).sum() > 1000
+
+
+def test_wall_velocity_is_the_rigid_field_plus_wing_rotation():
+    rng = np.random.default_rng(11)
+    b = bfref.make_bird(**COMPACT)
+    p = grid((48, 48, 48), 0.002)
+    pos = np.array([0.048, 0.048, 0.048])
+    q = np.array([0.05, -0.08, 0.1, 1.0])
+    v, w = np.array([1.0, -0.3, 0.2]), np.array([2.0, -1.0, 3.0])
+    scale = 0.7
+    f = bfref.bird_prepare(b, pos, q, v, w, time=0.05, dx=p.dx)
+    mask, wall = bfref.bird_build(p, b, f, velocity_to_lattice=scale)
+    centers = cell_centers(p)
+    rel = centers - row(f, "body_position")[:3]
+    omega = row(f, "omega_world_radius")[:3]
+    rigid = v + np.cross(omega, rel)
+    body = (mask == 1) | (mask == 4)
+    assert body.sum() > 500
+    assert np.allclose(wall[body, :3], rigid[body] * scale, rtol=1e-5, atol=1e-6)
+    for part, name in ((2, "left"), (3, "right")):
+        cells = mask == part
+        assert cells.sum() > 100
+        rel_root = centers[cells] - row(f, f"{name}_root")[:3]
+        want = rigid[cells] + np.cross(row(f, f"{name}_angular")[:3], rel_root)
+        assert np.allclose(wall[cells, :3], want * scale, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(wall[:, 3], np.zeros(len(wall), np.float32))
+    # beyond the radius: fluid and zero velocity
+    far = (np.linalg.norm(rel, axis=1) > row(f, "omega_world_radius")[3] * 1.001)
+    assert (mask[far] == 0).all() and (wall[far] == 0).all()
+
+
+def test_symmetric_pose_gives_a_mirrored_mask():
+    b = bfref.make_bird(**COMPACT)
+    p = grid((48, 48, 48), 0.002)
+    pos = np.array([0.048, 0.048, 0.048])       # the y mid-plane between cells 23 and 24
+    for t in (0.0, 0.04, 0.1):
+        f = bfref.bird_prepare(b, pos, time=t, dx=p.dx)
+        mask, wall = bfref.bird_build(p, b, f)
+        m = mask.reshape(48, 48, 48)
+        mirrored = m[:, ::-1, :].copy()
+        swap = mirrored.copy()
+        swap[mirrored == 2] = 3
+        swap[mirrored == 3] = 2
+        differ = int((swap != m).sum())
+        solid = int((m != 0).sum())
+        assert (m == 2).sum() > 100 and (m == 3).sum() > 100
+        assert differ <= max(2, solid // 1000)
+        wy = wall[:, 1].reshape(48, 48, 48)
+        same = swap == m
+        assert np.allclose(wy[same & (m != 0)], -wy[:, ::-1, :][same & (m != 0)], atol=1e-6)
+
+
+def test_previously_solid_cell_beyond_the_radius_is_evaluated():
+    b = bfref.make_bird(**COMPACT)
+    p = grid((48, 48, 48), 0.002)
+    pos = np.array([0.048, 0.048, 0.048])
+    v = np.array([1.5, 0.0, 0.0])
+    f = bfref.bird_prepare(b, pos, linear_velocity=v, time=0.0, dx=p.dx)
+    prev = np.zeros(48 ** 3, np.uint8)
+    gid = 1 + 48 * (1 + 48 * 1)           # a corner cell, far beyond the radius
+    prev[gid] = 1
+    m0, w0 = bfref.bird_build(p, b, f)
+    m1, w1 = bfref.bird_build(p, b, f, solid_prev=prev)
+    assert m1[gid] == 0 and w0[gid, 0] == 0.0 and w1[gid, 0] == np.float32(1.5)
+    other = np.ones(48 ** 3, bool)
+    other[gid] = False
+    assert np.array_equal(m0[other], m1[other]) and np.array_equal(w0[other], w1[other])

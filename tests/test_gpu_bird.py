@@ -1,0 +1,200 @@
+"""GPU parity of the analytic signed-distance bird (BirdFlow.metal:486-533,
+611-938) and of the reference's product step on it (halfway walls, force
+mode 6, sponge, far field; BirdFlowSimulation.swift:1359-1449) against the
+FP32 oracle, on the reference's compact case
+(MetalExecutionTests.swift:411-470):
+
+* the prepared frame and the part masks / wall velocities of every step are
+  bit-identical to the oracle's (a rotated, translating, spinning pose; the
+  wings flapping);
+* fluid steps with the bird: populations bit-identical, loads to 1e-12
+  relative (FP64 sums of the same FP32 terms in another order);
+* free flight: the body integrates its loads and the bird follows the pose
+  (masks and populations bit-identical; loads and the FP64 body history to
+  1e-12, the order of the FP64 load sums differing);
+* stepping in batches of different sizes gives identical results
+  (MetalExecutionTests.swift:472-591).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import bfref
+import birdflow_hip as bh
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def geometry_time(cfg, s):
+    return np.float32(np.float64(s) * np.float64(np.float32(cfg.time_step)))
+
+
+POSE = np.array([0.061, 0.058, 0.0595, 0.08, -0.05, 0.12, 0.985, 0.4, -0.2, 0.1, 3.0, -2.0, 4.0])
+
+
+def test_bird_geometry_matches_oracle_every_step():
+    p, cfg, bird, obird, _ = cases.compact_bird_case()
+    cfg.time_step = 0.004          # large enough for the wings to sweep several cells per step
+    body = POSE.copy()
+    body[3:7] /= np.linalg.norm(body[3:7])
+    say(f"case: compact bird {p.nx}x{p.ny}x{p.nz}, rotated spinning pose, dt {cfg.time_step}, 6 steps")
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.set_body(body)
+        sim.set_bird(bird)
+        frame = sim.bird_frame()
+        mask, wall = sim.read_geometry()
+        f_o = cases.bird_frame_oracle(obird, body, geometry_time(cfg, 0), p.dx)
+        m_o, w_o = bfref.bird_build(p, obird, f_o, None, cfg.velocity_to_lattice)
+        say(f"  step 0: solid {int((m_o != 0).sum())} cells, parts {np.bincount(m_o, minlength=5)[1:]}, "
+            f"frame equal {np.array_equal(frame, f_o)}, mask differing {int((mask != m_o).sum())}, "
+            f"wall differing {int((wall != w_o).any(axis=1).sum())}")
+        assert np.array_equal(frame.view(np.uint32), f_o.view(np.uint32))
+        assert np.array_equal(mask, m_o) and np.array_equal(wall.view(np.uint32), w_o.view(np.uint32))
+        assert all((m_o == k).sum() > 50 for k in (1, 2, 3, 4))
+        sim.init_equilibrium()
+        prev = m_o
+        for s in range(1, 7):
+            sim.advance(1)
+            frame = sim.bird_frame()
+            mask, wall = sim.read_geometry()
+            f_o = cases.bird_frame_oracle(obird, body, geometry_time(cfg, s), p.dx)
+            m_o, w_o = bfref.bird_build(p, obird, f_o, prev, cfg.velocity_to_lattice)
+            moved = int((m_o != prev).sum())
+            say(f"  step {s}: cells changing part {moved}, mask differing {int((mask != m_o).sum())}, "
+                f"wall differing {int((wall != w_o).any(axis=1).sum())}")
+            assert np.array_equal(frame.view(np.uint32), f_o.view(np.uint32))
+            assert np.array_equal(mask, m_o)
+            assert np.array_equal(wall.view(np.uint32), w_o.view(np.uint32))
+            assert moved > 0
+            prev = m_o
+
+
+def load_error(a, b):
+    """Largest difference of the force (torque) history relative to the
+    largest force (torque) component: the device and the oracle add the same
+    FP32 link terms in FP64 in different orders.  (A component that the
+    bird's symmetry keeps near zero is compared on the scale of the others.)"""
+    f = np.abs(a[:, :3] - b[:, :3]).max() / max(np.abs(b[:, :3]).max(), 1e-30)
+    t = np.abs(a[:, 3:6] - b[:, 3:6]).max() / max(np.abs(b[:, 3:6]).max(), 1e-30)
+    return float(max(f, t))
+
+
+def run_bird_oracle(p, cfg, obird, body, steps):
+    """Oracle loop of the prescribed (non-free-flight) bird."""
+    f_o = cases.bird_frame_oracle(obird, body, geometry_time(cfg, 0), p.dx)
+    m, w = bfref.bird_build(p, obird, f_o, None, cfg.velocity_to_lattice)
+    f = bfref.init_equilibrium(p, m, w)
+    prev = m
+    loads = []
+    for s in range(1, steps + 1):
+        fr = cases.bird_frame_oracle(obird, body, geometry_time(cfg, s), p.dx)
+        m, w = bfref.bird_build(p, obird, fr, prev, cfg.velocity_to_lattice)
+        f, ld, _, _ = bfref.step(p, f, prev, m, w)
+        loads.append(ld)
+        prev = m
+    return f, np.array(loads), prev
+
+
+def test_bird_fluid_steps_match_oracle_bit_for_bit():
+    p, cfg, bird, obird, _ = cases.compact_bird_case()
+    cfg.time_step = 0.002
+    steps = 10
+    body = np.zeros(13)
+    body[:3] = p.body_position[:]
+    body[6] = 1.0
+    say(f"case: compact bird fluid steps, {p.nx}^3, TRT, halfway walls, mode 6, strict FP32, {steps} steps")
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.set_bird(bird)
+        sim.init_equilibrium()
+        sim.advance(steps)
+        g = sim.read_populations()
+        loads = sim.load_history()
+        mask, _ = sim.read_geometry()
+    f_o, loads_o, mask_o = run_bird_oracle(p, cfg, obird, body, steps)
+    err = load_error(loads, loads_o)
+    say(f"  populations differing {int((g != f_o).sum())}; load history max rel err {err:.2e}; "
+        f"last force {loads[-1][:3]}")
+    assert np.array_equal(mask, mask_o)
+    assert np.array_equal(g.view(np.uint32), f_o.view(np.uint32))
+    assert err <= 1e-12 and np.array_equal(loads[:, 6:], loads_o[:, 6:])
+    assert np.abs(loads_o[:, :3]).max() > 0
+
+
+def test_bird_free_flight_matches_oracle():
+    p, cfg, bird, obird, sc = cases.compact_bird_case(free_flight=True)
+    cfg.time_step = 0.002
+    steps = 12
+    params = dict(mass=0.025, inertia=(4e-6, 7e-6, 8e-6), gravity=(0.0, 0.0, -9.81), external_force=(0.0, 0.0, 0.0),
+                  external_torque=(0.0, 0.0, 0.0), substeps=2)
+    state0 = np.array(list(p.body_position) + [0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    say(f"case: free-flying compact bird, {steps} steps, strict FP32, FP64 body")
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.set_body(state0, bh.BodyParameters(mass=params["mass"], inertia=params["inertia"],
+                                               gravity=params["gravity"], substeps=2))
+        sim.set_bird(bird)
+        sim.init_equilibrium()
+        sim.advance(steps)
+        hist = sim.body_history()
+        loads = sim.load_history()
+        mask, _ = sim.read_geometry()
+        g = sim.read_populations()
+    body = bfref.make_body(position=state0[:3], linear_velocity=state0[7:10])
+
+    def pose():
+        return bfref.body_array(body)
+
+    f_o = cases.bird_frame_oracle(obird, pose(), geometry_time(cfg, 0), p.dx)
+    m, w = bfref.bird_build(p, obird, f_o, None, cfg.velocity_to_lattice)
+    f = bfref.init_equilibrium(p, m, w)
+    prev = m
+    loads_o, states_o = [], []
+    for s in range(1, steps + 1):
+        fr = cases.bird_frame_oracle(obird, pose(), geometry_time(cfg, s), p.dx)
+        m, w = bfref.bird_build(p, obird, fr, prev, cfg.velocity_to_lattice)
+        for i in range(3):
+            p.body_position[i] = body.position[i]
+        f, ld, _, _ = bfref.step(p, f, prev, m, w)
+        cases.integrate_body_oracle(body, ld, params, float(np.float32(cfg.time_step)))
+        loads_o.append(ld)
+        states_o.append(pose())
+        prev = m
+    loads_o, states_o = np.array(loads_o), np.array(states_o)
+    moved = np.abs(states_o[-1, :3] - state0[:3]).max()
+    err = load_error(loads, loads_o)
+    berr = np.abs(hist - states_o).max()
+    say(f"  body moved {moved:.3e} m; body history max |diff| {berr:.3e}; load history max rel err {err:.2e}; "
+        f"mask differing {int((mask != prev).sum())}; populations differing {int((g != f).sum())}")
+    assert moved > 1e-4
+    assert np.array_equal(mask, prev)
+    assert err <= 1e-12
+    assert berr <= 1e-12
+    assert np.array_equal(g.view(np.uint32), f.view(np.uint32))
+
+
+def test_bird_batches_are_invariant():
+    # MetalExecutionTests.swift:472-591: one call of 16 steps against 16
+    # calls of one step (loads, fields, body state identical).
+    p, cfg, bird, _, _ = cases.compact_bird_case(free_flight=True)
+    cfg.time_step = 0.002
+    state0 = np.array(list(p.body_position) + [0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    out = []
+    for chunks in ([16], [1] * 16, [3, 5, 8]):
+        with bh.BirdFlowSimulation(cfg) as sim:
+            sim.set_body(state0, bh.BodyParameters(mass=0.025, inertia=(4e-6, 7e-6, 8e-6), substeps=2))
+            sim.set_bird(bird)
+            sim.init_equilibrium()
+            for k in chunks:
+                sim.advance(k)
+            out.append((sim.load_history(), sim.body_history(), sim.copy_macroscopic_fields()))
+    say(f"case: bird batch partitions {[16]}, 16 x [1], [3, 5, 8]")
+    for other in out[1:]:
+        assert np.array_equal(out[0][0], other[0])
+        assert np.array_equal(out[0][1], other[1])
+        assert np.array_equal(out[0][2][0], other[2][0]) and np.array_equal(out[0][2][1], other[2][1])

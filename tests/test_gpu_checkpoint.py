@@ -6,8 +6,8 @@
 * the restored context continues exactly as the saved one: stored
   populations, loads, masks and body state bit-identical after more steps
   (the reference allows 1e-6 / 1e-5; here the restart is exact, for FP32,
-  FP16 and BF16 storage, a flapping-plate mesh, a translating sphere and a
-  free-flying body carrying its mesh);
+  FP16 and BF16 storage, a flapping-plate mesh, a translating sphere, a
+  free-flying body carrying its mesh and a free-flying analytic bird);
 * a corrupted populations file is refused with CheckpointError.
 """
 import os
@@ -115,6 +115,19 @@ def test_restart_of_free_flying_body_carrying_its_mesh_is_exact(tmp_path):
         moved = np.abs(sim.body_state()[:3] - state0[:3]).max()
         say(f"  body moved {moved:.3e} cells")
         assert moved > 0.0
+
+
+def test_restart_of_free_flying_analytic_bird_is_exact(tmp_path):
+    say("case: checkpoint of the free-flying compact analytic bird 48^3, 5 + 4 steps")
+    p, cfg, bird, _, _ = cases.compact_bird_case(free_flight=True)
+    cfg.time_step = 0.002
+    state0 = np.array(list(p.body_position) + [0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.set_body(state0, bh.BodyParameters(mass=0.025, inertia=(4e-6, 7e-6, 8e-6), gravity=(0.0, 0.0, -9.81),
+                                               substeps=2))
+        sim.set_bird(bird)
+        sim.init_equilibrium()
+        check_restart(sim, tmp_path, 5, 4)
 
 
 def test_corrupted_checkpoint_is_refused(tmp_path):
