@@ -228,6 +228,22 @@ bf_status bf_set_bird(bf_context* ctx, const bf_bird_geometry* bird);
  * twist), span, normal, angular velocity (w: tip twist rate)). */
 bf_status bf_read_bird_frame(bf_context* ctx, float frame[56]);
 
+/* ---- momentum ledger (FP64 on the device) ----
+ * The device sums of BirdFlowSimulation.advanceWithCoupledMomentumLedger
+ * (BirdFlowSimulation.swift:805-1085; kernels BirdFlow.metal:4272-4526),
+ * formed per cell in FP64 and summed in a fixed order.  Lattice units.
+ * bf_ledger_fluid: mass and momentum (x, y, z) of the current populations
+ * over the fluid cells of the current mask (measureFluidMassMomentum).
+ * bf_ledger_step_sources: of the step just completed (call it after bf_step
+ * and before any other population write): sums[0..3] fluid change (after
+ * over the current mask minus before over the previous mask), [4..7]
+ * open-boundary sources (far-field and outlet links), [8..11] sponge
+ * sources, [12..15] persistent-wall-link exchange (halfway form, wall density
+ * far_rho); counts: open-boundary links, sponge cells, persistent links,
+ * cells whose occupancy changed.  Otherwise BF_ERROR_INVALID_ARGUMENT. */
+bf_status bf_ledger_fluid(bf_context* ctx, double out[4]);
+bf_status bf_ledger_step_sources(bf_context* ctx, double sums[16], uint64_t counts[4]);
+
 /* ---- stepping ---- */
 /* advance(steps:) (BirdFlowSimulation.swift:687-799): per step, geometry at
  * t_{n+1} into the next mask, fluid step (current -> next lattice, previous /

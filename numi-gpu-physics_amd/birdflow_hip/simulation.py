@@ -395,6 +395,25 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_read_bird_frame(self._ctx, _ptr(out)))
         return out
 
+    # -- momentum ledger -------------------------------------------------
+    def fluid_momentum(self) -> np.ndarray:
+        """Mass and momentum (x, y, z) of the fluid populations, lattice units, FP64."""
+        out = np.zeros(4, np.float64)
+        self._check(self._lib.bf_ledger_fluid(self._ctx, _ptr(out)))
+        return out
+
+    def step_momentum_sources(self) -> dict:
+        """Momentum sums of the step just completed (lattice units, FP64):
+        fluid change, open-boundary, sponge and persistent-link sources as
+        (mass, px, py, pz), and the link / cell counts."""
+        sums = np.zeros(16, np.float64)
+        counts = np.zeros(4, np.uint64)
+        self._check(self._lib.bf_ledger_step_sources(self._ctx, _ptr(sums), _ptr(counts)))
+        return {"change": sums[0:4].copy(), "open_boundary": sums[4:8].copy(), "sponge": sums[8:12].copy(),
+                "persistent_link": sums[12:16].copy(), "open_boundary_links": int(counts[0]),
+                "sponge_cells": int(counts[1]), "persistent_links": int(counts[2]),
+                "topology_transitions": int(counts[3])}
+
     # -- stepping --------------------------------------------------------
     def advance(self, steps: int, synchronize: bool = True):
         if steps < 0:

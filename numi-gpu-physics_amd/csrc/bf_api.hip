@@ -123,6 +123,18 @@ struct bf_context {
     uint32_t m_frames = 0, m_vertices = 0, m_triangles = 0;
     float m_half = 0.0f;
     float m_translation[3] = {0, 0, 0};
+    // momentum ledger: block partials and results (allocated on first use);
+    // the last step's inputs are intact while no population writer ran
+    DeviceAlloc ledger_alloc;
+    double* ledger_pv = nullptr;
+    unsigned long long* ledger_pn = nullptr;
+    double* ledger_out = nullptr;
+    unsigned long long* ledger_out_n = nullptr;
+    uint32_t ledger_blocks = 0;
+    const uint8_t* ledger_prev = nullptr;   // previous-step mask of the last step
+    uint64_t f_writes = 0;                  // population writes other than steps
+    uint64_t ledger_epoch = ~0ull;          // f_writes when the last step ran
+    int64_t ledger_step = -1;               // and its index
     // analytic bird: shape and the prepared frame of each step parity
     BirdShape bird{};
     float4* bird_frame = nullptr;         // [2][BIRD_FRAME_ROWS]
@@ -733,6 +745,7 @@ void bf_destroy(bf_context* c) {
         if (c->adj.ev_used[i]) (void)hipEventDestroy(c->adj.ev_used[i]);
     }
     c->mesh_alloc.release();
+    c->ledger_alloc.release();
     c->adj.alloc.release();
     c->adj.tangent_alloc.release();
     c->frozen_alloc.release();
@@ -760,6 +773,7 @@ bf_status bf_read_restart_state(bf_context* c, uint64_t* step, float* mesh_time_
 bf_status bf_set_restart_state(bf_context* c, uint64_t step, float mesh_time_origin, const double last_load[8],
                                const uint8_t* solid_mask) {
     BF_ENTER(c);
+    ++c->f_writes;
     if (step > (uint64_t)(INT64_MAX / 4) || !std::isfinite(mesh_time_origin))
         return fail(c, BF_ERROR_INVALID_ARGUMENT, "restart step or mesh time origin out of range");
     if (last_load) {
@@ -801,6 +815,7 @@ bf_status bf_set_restart_state(bf_context* c, uint64_t step, float mesh_time_ori
 
 bf_status bf_init_equilibrium(bf_context* c) {
     BF_ENTER(c);
+    ++c->f_writes;
     const StepGeometry sg = geometry_of(c, c->step);
     hipError_t e = launch_init_equilibrium((int)c->cfg.storage, c->f[c->cur], sg.mask_cur, sg.wall,
                                            c->cfg.far_velocity, c->g, c->sb);
@@ -811,6 +826,7 @@ bf_status bf_init_equilibrium(bf_context* c) {
 
 bf_status bf_init_shear_wave(bf_context* c, float amplitude) {
     BF_ENTER(c);
+    ++c->f_writes;
     hipError_t e = launch_init_shear_wave((int)c->cfg.storage, c->f[c->cur], amplitude, c->g, c->sb);
     if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "initialize shear wave");
@@ -832,6 +848,7 @@ static bf_status ensure_staging(bf_context* c) {
 
 bf_status bf_write_populations(bf_context* c, const float* host, uint64_t count) {
     BF_ENTER(c);
+    ++c->f_writes;
     if (check_host_array(c, host, "populations")) return BF_ERROR_INVALID_ARGUMENT;
     if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population count must be 19 * N");
     // through the staging buffer: the device layout differs from the host's
@@ -863,6 +880,7 @@ static uint64_t stored_bytes(const bf_context* c) {
 
 bf_status bf_write_population_storage(bf_context* c, const void* host, uint64_t bytes) {
     BF_ENTER(c);
+    ++c->f_writes;
     if (check_host_array(c, host, "population storage")) return BF_ERROR_INVALID_ARGUMENT;
     if (bytes != stored_bytes(c))
         return fail(c, BF_ERROR_INVALID_ARGUMENT, "population storage size must be 19 * N * element size");
@@ -1130,6 +1148,73 @@ bf_status bf_read_bird_frame(bf_context* c, float frame[56]) {
     return BF_OK;
 }
 
+static bf_status ensure_ledger(bf_context* c) {
+    if (c->ledger_pv) return BF_OK;
+    hipError_t e = hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>(4096u, (c->g.n + 255u) / 256u);
+    DeviceAlloc& A = c->ledger_alloc;
+    double* pv = (double*)A.take(e, 16ull * 8 * blocks);
+    unsigned long long* pn = (unsigned long long*)A.take(e, 4ull * 8 * blocks);
+    double* out = (double*)A.take(e, 16 * 8);
+    unsigned long long* out_n = (unsigned long long*)A.take(e, 4 * 8);
+    if (e != hipSuccess) {
+        A.release();
+        return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("ledger buffers: ") + hipGetErrorString(e));
+    }
+    c->ledger_pv = pv;
+    c->ledger_pn = pn;
+    c->ledger_out = out;
+    c->ledger_out_n = out_n;
+    c->ledger_blocks = blocks;
+    return BF_OK;
+}
+
+bf_status bf_ledger_fluid(bf_context* c, double out[4]) {
+    BF_ENTER(c);
+    if (!out) return fail(c, BF_ERROR_INVALID_ARGUMENT, "output must not be NULL");
+    if (bf_status r = ensure_ledger(c)) return r;
+    LedgerBuffers b{};
+    b.f_out = c->f[c->cur];
+    b.solid_cur = geometry_of(c, c->step).mask_cur;
+    b.blocks = c->ledger_blocks;
+    b.partial_v = c->ledger_pv;
+    b.partial_n = c->ledger_pn;
+    b.out_v = c->ledger_out;
+    b.out_n = c->ledger_out_n;
+    hipError_t e = launch_ledger((int)c->cfg.storage, b, make_step_params(c), false, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, c->ledger_out, 4 * 8, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "fluid momentum");
+    return BF_OK;
+}
+
+bf_status bf_ledger_step_sources(bf_context* c, double sums[16], uint64_t counts[4]) {
+    BF_ENTER(c);
+    if (!sums || !counts) return fail(c, BF_ERROR_INVALID_ARGUMENT, "outputs must not be NULL");
+    if (c->ledger_step != c->step || c->ledger_epoch != c->f_writes || c->step < 1)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT,
+                    "the ledger needs the inputs of the last step: call it right after bf_step");
+    if (bf_status r = ensure_ledger(c)) return r;
+    const StepGeometry sg = geometry_of(c, c->step);
+    LedgerBuffers b{};
+    b.f_in = c->f[1 - c->cur];
+    b.f_out = c->f[c->cur];
+    b.solid_prev = c->ledger_prev;
+    b.solid_cur = sg.mask_cur;
+    b.wall = sg.wall;
+    b.blocks = c->ledger_blocks;
+    b.partial_v = c->ledger_pv;
+    b.partial_n = c->ledger_pn;
+    b.out_v = c->ledger_out;
+    b.out_n = c->ledger_out_n;
+    hipError_t e = launch_ledger((int)c->cfg.storage, b, make_step_params(c), true, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(sums, c->ledger_out, 16 * 8, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, c->ledger_out_n, 4 * 8, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "step momentum sources");
+    return BF_OK;
+}
+
 bf_status bf_step(bf_context* c, uint32_t steps) {
     if (!c) return BF_ERROR_INVALID_ARGUMENT;
     if (c->invalidated) return BF_ERROR_STATE_INVALIDATED;
@@ -1164,6 +1249,7 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         }
         const StepParams p = make_step_params(c);
         const FluidBuffers fb = fluid_buffers(c, s);
+        c->ledger_prev = fb.solid_prev;
         if (c->source == BF_GEOMETRY_STATIC) c->static_prev_differs = false;
         // stream A: the wall tiles of step s, then the wall-free tiles, once
         // the tile list of step s (event L[s], stream B) is built.  Both read
@@ -1228,6 +1314,8 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
     }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(c, e, "step");
+    c->ledger_epoch = c->f_writes;
+    c->ledger_step = c->step;
     return BF_OK;
 }
 
@@ -1586,6 +1674,7 @@ bf_status bf_adjoint_set_tangents(bf_context* c, uint32_t parameter_count, const
 bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_weights, uint32_t checkpoint_interval,
                               double* objective, double* gradient, double timings_ms[2]) {
     BF_ENTER(c);
+    ++c->f_writes;
     if (steps == 0 || !load_weights || !objective || checkpoint_interval == 0)
         return fail(c, BF_ERROR_INVALID_ARGUMENT, "steps, load weights, objective and interval are required");
     if (c->adj.K > 0 && !gradient) return fail(c, BF_ERROR_INVALID_ARGUMENT, "gradient output is NULL");
@@ -1747,6 +1836,7 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
     (void)hipEventDestroy(t1);
     (void)hipEventDestroy(t2);
     a.have_sensitivity = true;
+    ++c->f_writes;   // the reverse sweep leaves no step's inputs in place
     return BF_OK;
 }
 

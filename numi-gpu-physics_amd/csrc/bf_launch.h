@@ -190,6 +190,24 @@ hipError_t launch_convert_populations(int storage, bool pack, const void* src, v
                                       const Grid& g, hipStream_t s);
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
 hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
+
+// Momentum ledger (bf_ledger.hip).  sources: the completed step's change,
+// open-boundary, sponge and persistent-link sums (16 doubles) and counts (4);
+// otherwise the mass and momentum of f_out over the fluid of solid_cur (4
+// doubles).  partial_v / partial_n hold blocks x 16 / blocks x 4 entries.
+struct LedgerBuffers {
+    const void* f_in;
+    const void* f_out;
+    const uint8_t* solid_prev;
+    const uint8_t* solid_cur;
+    const float4* wall;
+    uint32_t blocks;
+    double* partial_v;
+    unsigned long long* partial_n;
+    double* out_v;
+    unsigned long long* out_n;
+};
+hipError_t launch_ledger(int storage, const LedgerBuffers& b, const StepParams& p, bool sources, hipStream_t s);
 hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s);
 hipError_t launch_sphere_geometry(uint8_t* mask, float4* wall, uint8_t* dyn, const Grid& g, float4 cr, float3 gv,
                                   float4 wv, float t, const int periodic[3], hipStream_t s);

@@ -469,3 +469,104 @@ def bird_build(p: Params, bird: Bird, frame, solid_prev=None, velocity_to_lattic
     lib().bfo_bird_build(ctypes.byref(p), ctypes.byref(bird), _ptr(fr), _ptr(prev), np.float32(velocity_to_lattice),
                          _ptr(mask), _ptr(wall4))
     return mask, wall4
+
+
+# ---------------------------------------------------------------------------
+# momentum ledger (FP64 restatement, numpy)
+#   measureFluidMassMomentum / measureFluidMassMomentumChange /
+#   captureExternalFluidSources, BirdFlow.metal:4272-4526, with per-face
+#   boundaries (far-field and outlet faces are open boundaries) and every
+#   term formed in FP64 from the FP32 populations.
+# ---------------------------------------------------------------------------
+def _grid_coords(p: Params):
+    z, y, x = np.meshgrid(np.arange(p.nz), np.arange(p.ny), np.arange(p.nx), indexing="ij")
+    return x.ravel(), y.ravel(), z.ravel()
+
+
+def ledger_fluid(p: Params, f, solid):
+    """(mass, px, py, pz) of the fluid cells (mask 0)."""
+    n = cells(p)
+    f = np.asarray(f, np.float64).reshape(19, n)
+    fluid = np.asarray(solid).ravel() == 0
+    m = f[:, fluid]
+    return np.array([m.sum()] + [(m * C[:, a][:, None]).sum() for a in range(3)])
+
+
+def _sponge_factor(p: Params, x, y, z):
+    """Sponge factor of each cell, FP32 as the device (BirdFlow.metal:2427-2446)."""
+    f32 = np.float32
+    if not p.sponge_strength > 0:
+        return np.zeros(x.shape, f32)
+    d = np.minimum.reduce([x, y, z, p.nx - 1 - x, p.ny - 1 - y, p.nz - 1 - z]).astype(f32)
+    width = f32(p.sponge_width)
+    nrm = np.clip((width - d) / np.maximum(width, f32(1.0)), f32(0), f32(1)).astype(f32)
+    return (f32(p.sponge_strength) * nrm * nrm).astype(f32)
+
+
+def ledger_sources(p: Params, f_in, f_out, solid_prev, solid_cur, wall4):
+    """Sums (change, open boundary, sponge, persistent link) as 4 x (mass,
+    px, py, pz) and counts (open-boundary links, sponge cells, persistent
+    links, occupancy changes) of one step."""
+    n = cells(p)
+    fi = np.asarray(f_in, np.float32).reshape(19, n).astype(np.float64)
+    fo = np.asarray(f_out, np.float32).reshape(19, n).astype(np.float64)
+    was = np.asarray(solid_prev).ravel() != 0
+    is_ = np.asarray(solid_cur).ravel() != 0
+    wall = np.asarray(wall4, np.float32).reshape(n, 4)
+    x, y, z = _grid_coords(p)
+    sums = np.zeros((4, 4))
+    counts = np.zeros(4, np.int64)
+    d = np.where(~is_, fo, 0.0) - np.where(~was, fi, 0.0)
+    sums[0] = [d.sum()] + [(d * C[:, a][:, None]).sum() for a in range(3)]
+    counts[3] = int((was != is_).sum())
+    fluid = ~is_
+    far_eq = np.array([bfref_far_eq(p, q) for q in range(19)], np.float64)
+    w32 = W.astype(np.float32).astype(np.float64)
+    dims = (p.nx, p.ny, p.nz)
+    coords = (x, y, z)
+    for q in range(1, 19):
+        far = np.zeros(n, bool)
+        opn = np.zeros(n, bool)
+        src = []
+        for a in range(3):
+            u = coords[a] - C[q, a]
+            lo, hi = p.bc[a][0], p.bc[a][1]
+            s = u.copy()
+            below, above = u < 0, u >= dims[a]
+            far |= (below & (lo == BC_FARFIELD)) | (above & (hi == BC_FARFIELD))
+            opn |= (below & (lo == BC_OUTLET)) | (above & (hi == BC_OUTLET))
+            s[below] = dims[a] - 1 if lo == BC_PERIODIC else 0
+            s[above] = 0 if hi == BC_PERIODIC else dims[a] - 1
+            src.append(s)
+        sid = src[0] + p.nx * (src[1] + p.ny * src[2])
+        out = fi[OPP[q]]
+        cq = C[q].astype(np.float64)
+        sel = fluid & far
+        inc = far_eq[q]
+        sums[1, 0] += (inc - out[sel]).sum()
+        sums[1, 1:] += (inc + out[sel]).sum() * cq
+        sel2 = fluid & ~far & opn
+        inc2 = fi[q][sid[sel2]]
+        sums[1, 0] += (inc2 - out[sel2]).sum()
+        sums[1, 1:] += (inc2 + out[sel2]).sum() * cq
+        counts[0] += int(sel.sum() + sel2.sum())
+        sel3 = fluid & ~far & ~opn & ~was
+        sel3 &= is_[np.where(sel3, sid, 0)] & was[np.where(sel3, sid, 0)]
+        u = wall[sid[sel3], :3].astype(np.float64)
+        corr = 2.0 * w32[q] * float(np.float32(p.far_rho)) * (u @ cq) * 3.0
+        sums[3, 0] += corr.sum()
+        sums[3, 1:] += (2.0 * out[sel3] + corr).sum() * cq
+        counts[2] += int(sel3.sum())
+    s = _sponge_factor(p, x, y, z).astype(np.float64)
+    sp = fluid & (s > 0)
+    if sp.any():
+        inv = 1.0 / np.maximum(1.0 - s[sp], 1e-7)
+        dlt = s[sp][None, :] * (far_eq[:, None] - fo[:, sp]) * inv[None, :]
+        sums[2] = [dlt.sum()] + [(dlt * C[:, a][:, None]).sum() for a in range(3)]
+        counts[1] = int(sp.sum())
+    return sums, counts
+
+
+def bfref_far_eq(p: Params, q):
+    """FP32 far-field equilibrium of direction q (the value the step uses)."""
+    return float(equilibrium_f32(q, float(np.float32(p.far_rho)), tuple(float(v) for v in p.far_u)))
