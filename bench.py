@@ -111,6 +111,55 @@ def run_gpu(shape, steps, warmup, storage, arithmetic="default"):
     }
 
 
+def run_sdf_bird(shape, steps, warmup):
+    """The reference's product path on its demonstration bird
+    (BirdParameters.demonstration, analytic signed-distance geometry every
+    step): free flight under gravity, far field -10 m/s in x, TRT, halfway
+    walls, force mode 6, sponge 0.04 over 8 cells, FP32; dx = 4 mm (root chord
+    35 cells), Re 5000 on the chord.  MLUPS of whole steps (geometry, fluid,
+    loads, body update)."""
+    import birdflow_hip as bh
+    sc = bh.LatticeScaling.make(0.14, 35, 10.0, 5000.0, 1.225, lattice_reference_speed=0.05)
+    dx = sc.cell_size
+    cfg = bh.SolverConfig(shape=shape, omega_plus=sc.omega_plus, omega_minus=sc.omega_minus,
+                          storage=bh.STORAGE_FP32, collision=bh.COLLISION_TRT, wall_rule=bh.WALL_HALFWAY,
+                          force_mode=6, accumulate_loads=True, boundary=((bh.BC_FARFIELD, bh.BC_FARFIELD),) * 3,
+                          cell_size=dx, time_step=sc.time_step, velocity_to_lattice=sc.velocity_to_lattice,
+                          force_to_physical=sc.force_to_physical, sponge_strength=0.04, sponge_width=8.0,
+                          far_velocity=(-10.0 * sc.velocity_to_lattice, 0.0, 0.0),
+                          body_position=(0.3 * shape[0] * dx, 0.5 * shape[1] * dx, 0.5 * shape[2] * dx))
+    sim = bh.BirdFlowSimulation(cfg)
+    try:
+        state0 = list(cfg.body_position) + [0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+        sim.set_body(state0, bh.BodyParameters(mass=0.42, inertia=(0.0024, 0.0068, 0.0074), gravity=(0.0, 0.0, -9.81)))
+        sim.set_bird(bh.BirdGeometry.demonstration())
+        sim.init_equilibrium()
+        mask, _ = sim.read_geometry()
+        solid = int((mask != 0).sum())
+        sim.advance(warmup)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.advance(steps, synchronize=False)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        loads = sim.load_history(steps)
+        sim.set_profiling(True, level=2)
+        sim.advance(BREAKDOWN_STEPS)
+        brk = sim.read_profile()
+        sim.set_profiling(False)
+    finally:
+        sim.close()
+    n = cfg.cells
+    k = max(brk["steps"], 1)
+    return {"value": n * steps / (t1 - t0) / 1e6, "unit": "MLUPS", "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "steps": steps, "solid_cells": solid, "general_tiles": int(brk["general_tiles"]), "tiles": int(brk["tiles"]),
+            "step_breakdown_ms": {"geometry": brk["geometry_ms"] / k, "fluid_fast": brk["fast_ms"] / k,
+                                  "fluid_general": brk["general_ms"] / k},
+            "finite": bool(np.isfinite(loads).all()),
+            "workload": f"analytic demonstration bird {shape[0]}x{shape[1]}x{shape[2]}, free flight, TRT, "
+                        "halfway walls, dx 4 mm, FP32"}
+
+
 def omp_threads(n):
     """Sets the oracle's OpenMP thread count (libgomp, loaded with it)."""
     import ctypes
@@ -223,6 +272,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="steps of the adjoint cost measurement on the 256x128x128 plate (0: skip)")
     ap.add_argument("--adjoint-interval", type=int, default=100, help="checkpoint interval of that sweep")
+    ap.add_argument("--sdf-bird-steps", type=int, default=100,
+                    help="timed steps of the analytic demonstration bird on the same grid (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -244,6 +295,10 @@ def main():
         print(f"# gpu fp16: {r16}", file=sys.stderr, flush=True)
         extra["fp16_store"] = {"value": r16["mlups"], "unit": "MLUPS", "ratio_vs_fp32": r16["mlups"] / r["mlups"],
                                "fast_kernel_ms": r16["fast_kernel_ms"], "achieved_gbs": r16["achieved_gbs"]}
+    if args.sdf_bird_steps > 0:
+        sdf = run_sdf_bird(shape, args.sdf_bird_steps, args.warmup)
+        print(f"# analytic bird: {sdf}", file=sys.stderr, flush=True)
+        extra["analytic_bird"] = sdf
     if args.adjoint_steps > 0:
         adj = run_adjoint((256, 128, 128), args.adjoint_steps, args.adjoint_interval)
         print(f"# adjoint: {adj}", file=sys.stderr, flush=True)

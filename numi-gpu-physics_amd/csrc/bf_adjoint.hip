@@ -67,6 +67,8 @@ __global__ __launch_bounds__(256) void adjoint_fast(AdjointBuffers b, StepParams
     const uint32_t row = (uint32_t)y + (uint32_t)p.g.ny * (uint32_t)z;
     const uint32_t gid = (uint32_t)x + (uint32_t)p.g.nx * row;
     const Sources s = sources(p, x, y, z);
+    // the sponge blend f_out = post + (f_far - post) s scales the adjoint by 1 - s
+    const float keep = 1.0f - (p.sponge_strength > 0.0f ? sponge_factor(p, x, y, z) : 0.0f);
     float g[19], lp[19], mu[19];
     uint32_t src[19];
     bool far[19];
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(256) void adjoint_fast(AdjointBuffers b, StepParams
         // unconditional load (a far-field source reads this cell)
         const float v = b.f_prev[far[q] ? pop_index<STORE_FP32>(p.g, q, x, row) : pop_index<STORE_FP32>(p.g, q, xs, rows)];
         g[q] = far[q] ? p.far_eq[q] : v;
-        lp[q] = b.lam_in[(size_t)q * n + gid];
+        lp[q] = b.lam_in[(size_t)q * n + gid] * keep;
     });
     collision_adjoint<COLL>(g, p.omega_plus, p.omega_minus, lp, mu);
     for_q(AllQ{}, [&](auto qc) {

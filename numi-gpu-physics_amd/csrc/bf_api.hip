@@ -282,22 +282,16 @@ StepParams make_step_params(const bf_context* c) {
 }
 
 // Tiles that always take the general path: partial tiles at the +x / +y
-// edges and tiles touching the sponge band.
+// edges (the fast kernel streams whole tiles; the sponge band streams there
+// too).
 std::vector<uint8_t> static_tile_flags(const bf_config& cfg, const Grid& g) {
+    (void)cfg;
     std::vector<uint8_t> flags(g.tiles, 0);
-    const int width = cfg.sponge_strength > 0.0f ? (int)std::ceil(cfg.sponge_width) : 0;
     for (int z = 0; z < g.nz; ++z)
         for (int ty = 0; ty < g.tiles_y; ++ty)
             for (int tx = 0; tx < g.tiles_x; ++tx) {
                 const int x0 = tx * TILE_X, y0 = ty * TILE_Y;
-                const int x1 = std::min(x0 + TILE_X, g.nx) - 1, y1 = std::min(y0 + TILE_Y, g.ny) - 1;
-                bool general = x0 + TILE_X > g.nx || y0 + TILE_Y > g.ny;
-                if (width > 0) {
-                    const int dxm = std::max(0, std::min(x0, g.nx - 1 - x1));
-                    const int dym = std::max(0, std::min(y0, g.ny - 1 - y1));
-                    const int dzm = std::min(z, g.nz - 1 - z);
-                    if (std::min(dxm, std::min(dym, dzm)) < width) general = true;
-                }
+                const bool general = x0 + TILE_X > g.nx || y0 + TILE_Y > g.ny;
                 flags[((size_t)z * g.tiles_y + ty) * g.tiles_x + tx] = general ? 1 : 0;
             }
     return flags;

@@ -5,8 +5,8 @@
 // BirdFlow.metal:2507-3236).  The lattice is split into 64x4 tiles, one
 // 256-lane workgroup each:
 //   * fluid_fast    — tiles with no solid cell (current or previous step) in
-//                     their 1-cell halo and outside any sponge: pure pull
-//                     stream with per-face boundary sources, collision, store.
+//                     their 1-cell halo: pure pull stream with per-face
+//                     boundary sources, collision, sponge blend, store.
 //   * fluid_general — the listed tiles: solid nodes, refill of uncovered
 //                     nodes, halfway or interpolated moving bounce-back,
 //                     momentum-exchange loads with cover/uncover impulses,
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
                                                   uint32_t* __restrict__ activation, StepParams p) {
     set_priority(p.fast_priority);
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    // Tiles with walls or sponge belong to fluid_general.  The flags are
+    // Tiles with walls belong to fluid_general.  The flags are
     // loaded together with the populations and tested only before the
     // stores: a test up front would put a dependent memory round trip ahead
     // of every workgroup's population loads.
@@ -618,8 +618,11 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
         collide<COLL>(f, rho, ux, uy, uz, p.omega_plus, p.omega_minus, post, act, restr);
     }
     const uint32_t gid = (uint32_t)x + (uint32_t)nx * row_self;
+    // wall-free sponge tiles stream here too (the sponge is a per-cell blend
+    // toward the far-field equilibrium after the collision)
+    const float sponge = p.sponge_strength > 0.0f ? sponge_factor(p, x, y, z) : 0.0f;
     float crho, cmx, cmy, cmz;
-    write_post<S>(fout, x, row_self, post, 0.0f, p, crho, cmx, cmy, cmz);
+    write_post<S>(fout, x, row_self, post, sponge, p, crho, cmx, cmy, cmz);
     if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
     if constexpr (COLL != COLL_TRT) {
         // Limiter activations of fast cells: an exact count and the maximum

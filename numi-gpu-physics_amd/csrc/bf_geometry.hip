@@ -764,8 +764,8 @@ __global__ void bird_prepare(BirdShape b, const double* __restrict__ body, float
     for (int i = 0; i < BIRD_FRAME_ROWS; ++i) frame[i] = rows[i];
 }
 
-// Tiles written this step: those with a cell centre within one cell of the
-// geometry sphere's bounding box, and (with a previous geometry) those near
+// Tiles written this step: those whose box of cell centres comes within one
+// cell of the geometry sphere, and (with a previous geometry) those near
 // last step's solid cells, whose cells the build evaluates even beyond the
 // radius.  Every tile's flag is written.
 __global__ __launch_bounds__(256) void bird_mark_tiles(const float4* __restrict__ frame,
@@ -776,13 +776,15 @@ __global__ __launch_bounds__(256) void bird_mark_tiles(const float4* __restrict_
     int x0, y0, z;
     tile_origin(p.g, t, x0, y0, z);
     const int x1 = min(x0 + TILE_X, p.g.nx) - 1, y1 = min(y0 + TILE_Y, p.g.ny) - 1;
+    // distance from the sphere centre to the box of the tile's cell centres
     const float4 c = frame[0];
     const float reach = frame[3].w + p.dx;
-    const bool hit = p.origin[0] + ((float)x1 + 0.5f) * p.dx >= c.x - reach
-                     && p.origin[0] + ((float)x0 + 0.5f) * p.dx <= c.x + reach
-                     && p.origin[1] + ((float)y1 + 0.5f) * p.dx >= c.y - reach
-                     && p.origin[1] + ((float)y0 + 0.5f) * p.dx <= c.y + reach
-                     && fabsf(p.origin[2] + ((float)z + 0.5f) * p.dx - c.z) <= reach;
+    const float lx = p.origin[0] + ((float)x0 + 0.5f) * p.dx, hx = p.origin[0] + ((float)x1 + 0.5f) * p.dx;
+    const float ly = p.origin[1] + ((float)y0 + 0.5f) * p.dx, hy = p.origin[1] + ((float)y1 + 0.5f) * p.dx;
+    const float dx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.0f);
+    const float dy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.0f);
+    const float dz = fabsf(p.origin[2] + ((float)z + 0.5f) * p.dx - c.z);
+    const bool hit = dx * dx + dy * dy + dz * dz <= reach * reach;
     tiles_new[t] = (hit || (dyn_prev && dyn_prev[t])) ? 1 : 0;
 }
 
