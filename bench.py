@@ -274,12 +274,19 @@ def main():
     ap.add_argument("--adjoint-interval", type=int, default=100, help="checkpoint interval of that sweep")
     ap.add_argument("--sdf-bird-steps", type=int, default=100,
                     help="timed steps of the analytic demonstration bird on the same grid (0: skip)")
+    ap.add_argument("--sdf-bird-only", action="store_true",
+                    help="run only the analytic demonstration bird and print its result line (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
                           "requested_gpus": args.gpus}), flush=True)
         sys.exit(2)
     shape = tuple(args.shape)
+    if args.sdf_bird_only:
+        sdf = run_sdf_bird(shape, args.sdf_bird_steps, args.warmup)
+        print(json.dumps(dict(sdf, metric="analytic demonstration bird step", n_gpus=1, warmup=args.warmup,
+                              dtype="f32", data="synthetic")), flush=True)
+        return
     print(f"# bench: bird {shape}, storage {args.storage}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     import birdflow_hip as bh

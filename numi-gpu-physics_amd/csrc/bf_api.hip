@@ -1286,7 +1286,18 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
             if (bf_status r = next_geometry()) return r;
         }
         if (!c->free_flight) e = hipEventRecord(c->ev_ready[par2(s + 1)], c->sb);
-        if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, c->ev_fast[par2(s)], 0);
+        // Coupled free flight puts the reduction, body update and next
+        // geometry on the critical path.  With TRT the loads come from the
+        // wall tiles alone (the fast kernel adds limiter activations only for
+        // the other collisions), so they start when general(s) ends and
+        // overlap fast(s): fast(s) reads none of the buffers they write
+        // (geometry s+1 writes the dyn flags of s-2, which fast(s-1) read and
+        // general(s) followed), and general(s) itself follows fast(s-1).
+        // Uncoupled, the geometry of s+1 may precede general(s) on this
+        // stream, so the reduction keeps waiting for fast(s).
+        const bool early_loads = coupled && c->cfg.collision == BF_COLLISION_TRT;
+        if (e == hipSuccess)
+            e = hipStreamWaitEvent(c->sb, early_loads ? c->ev_general[par2(s)] : c->ev_fast[par2(s)], 0);
         if (e == hipSuccess) e = launch_load_reduction(fb, p, (uint32_t)(s % HISTORY_CAPACITY), c->sb);
         if (e == hipSuccess && c->free_flight)
             e = launch_body_step(c->body, c->last_load, c->body_history, (uint32_t)(s % HISTORY_CAPACITY),

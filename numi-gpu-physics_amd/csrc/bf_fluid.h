@@ -619,10 +619,13 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
     }
     const uint32_t gid = (uint32_t)x + (uint32_t)nx * row_self;
     // wall-free sponge tiles stream here too (the sponge is a per-cell blend
-    // toward the far-field equilibrium after the collision)
-    const float sponge = p.sponge_strength > 0.0f ? sponge_factor(p, x, y, z) : 0.0f;
+    // toward the far-field equilibrium after the collision); a uniform branch,
+    // so that without a sponge the store path carries no blend
     float crho, cmx, cmy, cmz;
-    write_post<S>(fout, x, row_self, post, sponge, p, crho, cmx, cmy, cmz);
+    if (p.sponge_strength > 0.0f)
+        write_post<S>(fout, x, row_self, post, sponge_factor(p, x, y, z), p, crho, cmx, cmy, cmz);
+    else
+        write_post<S>(fout, x, row_self, post, 0.0f, p, crho, cmx, cmy, cmz);
     if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
     if constexpr (COLL != COLL_TRT) {
         // Limiter activations of fast cells: an exact count and the maximum

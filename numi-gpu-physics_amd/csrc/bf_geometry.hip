@@ -764,11 +764,23 @@ __global__ void bird_prepare(BirdShape b, const double* __restrict__ body, float
     for (int i = 0; i < BIRD_FRAME_ROWS; ++i) frame[i] = rows[i];
 }
 
-// Tiles written this step: those whose box of cell centres comes within one
-// cell of the geometry sphere, and (with a previous geometry) those near
-// last step's solid cells, whose cells the build evaluates even beyond the
-// radius.  Every tile's flag is written.
-__global__ __launch_bounds__(256) void bird_mark_tiles(const float4* __restrict__ frame,
+// Squared distance from point c to the box of a tile's cell centres.
+__device__ inline float tile_box_dist2(float lx, float hx, float ly, float hy, float zc, float cx_, float cy_,
+                                       float cz_) {
+    const float dx = fmaxf(fmaxf(lx - cx_, cx_ - hx), 0.0f);
+    const float dy = fmaxf(fmaxf(ly - cy_, cy_ - hy), 0.0f);
+    const float dz = zc - cz_;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Tiles evaluated this step: those whose box of cell centres comes within
+// two cells of a bounding sphere of the body ellipsoid (largest radius), of
+// the tail box or of either wing box (centre at mid-span, radius half the
+// span plus sweep, half the largest chord and half the thickness), so every
+// cell that can be solid lies in one; and, with a previous geometry, the
+// tiles near last step's solid cells (evaluated even beyond the radius, as
+// the reference does).  Every tile's flag is written.
+__global__ __launch_bounds__(256) void bird_mark_tiles(BirdShape b, const float4* __restrict__ frame,
                                                        const uint8_t* __restrict__ dyn_prev,
                                                        uint8_t* __restrict__ tiles_new, BirdParams p) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -776,24 +788,44 @@ __global__ __launch_bounds__(256) void bird_mark_tiles(const float4* __restrict_
     int x0, y0, z;
     tile_origin(p.g, t, x0, y0, z);
     const int x1 = min(x0 + TILE_X, p.g.nx) - 1, y1 = min(y0 + TILE_Y, p.g.ny) - 1;
-    // distance from the sphere centre to the box of the tile's cell centres
-    const float4 c = frame[0];
-    const float reach = frame[3].w + p.dx;
     const float lx = p.origin[0] + ((float)x0 + 0.5f) * p.dx, hx = p.origin[0] + ((float)x1 + 0.5f) * p.dx;
     const float ly = p.origin[1] + ((float)y0 + 0.5f) * p.dx, hy = p.origin[1] + ((float)y1 + 0.5f) * p.dx;
-    const float dx = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.0f);
-    const float dy = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.0f);
-    const float dz = fabsf(p.origin[2] + ((float)z + 0.5f) * p.dx - c.z);
-    const bool hit = dx * dx + dy * dy + dz * dz <= reach * reach;
+    const float zc = p.origin[2] + ((float)z + 0.5f) * p.dx;
+    const float guard = 2.0f * p.dx;
+    const float4 c = frame[0];
+    const Quat q = {frame[1].x, frame[1].y, frame[1].z, frame[1].w};
+    bool hit = false;
+    {
+        const float r = smax(b.body_radii[0], smax(b.body_radii[1], b.body_radii[2])) + guard;
+        hit = hit || tile_box_dist2(lx, hx, ly, hy, zc, c.x, c.y, c.z) <= r * r;
+    }
+    {
+        const V3 lc = v3(-b.body_radii[0] - 0.5f * b.tail_length, 0.0f, -0.15f * b.body_radii[2]);
+        const V3 wc = vadd(v3(c.x, c.y, c.z), qrotate(q, lc));
+        const float r = vlen(v3(0.5f * b.tail_length, b.tail_half_width, 0.5f * b.tail_thickness)) + guard;
+        hit = hit || tile_box_dist2(lx, hx, ly, hy, zc, wc.x, wc.y, wc.z) <= r * r;
+    }
+    const float rw = 0.5f * b.wing_span + fabsf(b.wing_sweep) + 0.5f * smax(b.wing_root_chord, b.wing_tip_chord)
+                     + 0.5f * b.wing_thickness + guard;
+#pragma unroll
+    for (int wing = 0; wing < 2; ++wing) {
+        const V3 wc = vadd(row3(frame[4 + 5 * wing]), vmul(row3(frame[6 + 5 * wing]), 0.5f * b.wing_span));
+        hit = hit || tile_box_dist2(lx, hx, ly, hy, zc, wc.x, wc.y, wc.z) <= rw * rw;
+    }
     tiles_new[t] = (hit || (dyn_prev && dyn_prev[t])) ? 1 : 0;
 }
 
-// buildBirdGeometry on the listed tiles; every cell of a listed tile gets
-// the reference's value (mask 0 and wall 0 beyond the radius unless solid
-// last step).  Solid cells mark the tiles whose cells pull from them.
+// buildBirdGeometry on the tiles marked this step: every cell gets the
+// reference's value (mask 0 and wall 0 beyond the radius unless solid last
+// step).  Other listed tiles (stale entries of earlier steps) are cleared to
+// mask 0, wall 0: their cells lie outside every component sphere and were
+// not solid last step, so the flags equal the reference's everywhere and the
+// wall velocity wherever a step reads it (cells solid this step or the
+// previous one).  Solid cells mark the tiles whose cells pull from them.
 __global__ __launch_bounds__(256) void bird_tile_pass(BirdShape b, const float4* __restrict__ frame,
                                                       const uint32_t* __restrict__ list,
                                                       const uint32_t* __restrict__ count,
+                                                      const uint8_t* __restrict__ tiles_new,
                                                       const uint8_t* __restrict__ mask_prev,
                                                       uint8_t* __restrict__ mask, float4* __restrict__ wall,
                                                       uint8_t* __restrict__ dyn, BirdParams p) {
@@ -819,7 +851,7 @@ __global__ __launch_bounds__(256) void bird_tile_pass(BirdShape b, const float4*
                             p.origin[2] + ((float)z + 0.5f) * p.dx);
         const V3 rel = vsub(world, body_pos);
         const bool was_solid = mask_prev && mask_prev[gid] != 0;
-        if (!was_solid && vdot(rel, rel) > radius * radius) {
+        if (!tiles_new[tile] || (!was_solid && vdot(rel, rel) > radius * radius)) {
             mask[gid] = 0;
             wall[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             continue;
@@ -866,14 +898,14 @@ __global__ __launch_bounds__(256) void bird_tile_pass(BirdShape b, const float4*
 hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s) {
     const unsigned tile_blocks = (p.g.tiles + 255) / 256;
     hipLaunchKernelGGL(bird_prepare, dim3(1), dim3(64), 0, s, shape, b.body, p.time, p.dx, b.frame);
-    hipLaunchKernelGGL(bird_mark_tiles, dim3(tile_blocks), dim3(256), 0, s, (const float4*)b.frame, b.dyn_prev,
-                       b.tiles_new, p);
+    hipLaunchKernelGGL(bird_mark_tiles, dim3(tile_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
+                       b.dyn_prev, b.tiles_new, p);
     hipLaunchKernelGGL(fill_u8, dim3(tile_blocks), dim3(256), 0, s, b.dyn, (uint8_t)0, p.g.tiles);
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
     hipLaunchKernelGGL(geometry_tile_list, dim3(tile_blocks), dim3(256), 0, s, b.tiles_new, b.tiles_mask_old,
                        b.tiles_wall_old, p.g.tiles, b.list, b.list_count);
     hipLaunchKernelGGL(bird_tile_pass, dim3(p.tile_pass_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
-                       b.list, b.list_count, b.mask_prev, b.mask, b.wall, b.dyn, p);
+                       b.list, b.list_count, (const uint8_t*)b.tiles_new, b.mask_prev, b.mask, b.wall, b.dyn, p);
     return hipGetLastError();
 }
 

@@ -4,9 +4,11 @@ mode 6, sponge, far field; BirdFlowSimulation.swift:1359-1449) against the
 FP32 oracle, on the reference's compact case
 (MetalExecutionTests.swift:411-470):
 
-* the prepared frame and the part masks / wall velocities of every step are
-  bit-identical to the oracle's (a rotated, translating, spinning pose; the
-  wings flapping);
+* the prepared frame and the part masks of every step are bit-identical to
+  the oracle's, and so are the wall velocities wherever a step reads them
+  (cells solid in this step or the previous one; the device evaluates only
+  tiles near the bird's components), for a rotated, translating, spinning
+  pose with the wings flapping;
 * fluid steps with the bird: populations bit-identical, loads to 1e-12
   relative (FP64 sums of the same FP32 terms in another order);
 * free flight: the body integrates its loads and the bird follows the pose
@@ -54,9 +56,11 @@ def test_bird_geometry_matches_oracle_every_step():
         m_o, w_o = bfref.bird_build(p, obird, f_o, None, cfg.velocity_to_lattice)
         say(f"  step 0: solid {int((m_o != 0).sum())} cells, parts {np.bincount(m_o, minlength=5)[1:]}, "
             f"frame equal {np.array_equal(frame, f_o)}, mask differing {int((mask != m_o).sum())}, "
-            f"wall differing {int((wall != w_o).any(axis=1).sum())}")
+            f"wall differing where read {int((wall[m_o != 0] != w_o[m_o != 0]).any(axis=1).sum())}")
+        read = m_o != 0
         assert np.array_equal(frame.view(np.uint32), f_o.view(np.uint32))
-        assert np.array_equal(mask, m_o) and np.array_equal(wall.view(np.uint32), w_o.view(np.uint32))
+        assert np.array_equal(mask, m_o)
+        assert np.array_equal(wall[read].view(np.uint32), w_o[read].view(np.uint32))
         assert all((m_o == k).sum() > 50 for k in (1, 2, 3, 4))
         sim.init_equilibrium()
         prev = m_o
@@ -67,11 +71,12 @@ def test_bird_geometry_matches_oracle_every_step():
             f_o = cases.bird_frame_oracle(obird, body, geometry_time(cfg, s), p.dx)
             m_o, w_o = bfref.bird_build(p, obird, f_o, prev, cfg.velocity_to_lattice)
             moved = int((m_o != prev).sum())
+            read = (m_o != 0) | (prev != 0)
             say(f"  step {s}: cells changing part {moved}, mask differing {int((mask != m_o).sum())}, "
-                f"wall differing {int((wall != w_o).any(axis=1).sum())}")
+                f"wall differing where read {int((wall[read] != w_o[read]).any(axis=1).sum())} of {int(read.sum())}")
             assert np.array_equal(frame.view(np.uint32), f_o.view(np.uint32))
             assert np.array_equal(mask, m_o)
-            assert np.array_equal(wall.view(np.uint32), w_o.view(np.uint32))
+            assert np.array_equal(wall[read].view(np.uint32), w_o[read].view(np.uint32))
             assert moved > 0
             prev = m_o
 
