@@ -222,6 +222,15 @@ typedef struct bf_bird_geometry {
  * as the reference's initialization) and selects BF_GEOMETRY_BIRD; in free
  * flight every step's geometry follows the integrated pose. */
 bf_status bf_set_bird(bf_context* ctx, const bf_bird_geometry* bird);
+/* Measured wing kinematics (BirdParameters.measuredWingKinematics,
+ * MeasuredBirdDataset.swift:105-160; sampled by measuredWingKinematics,
+ * BirdFlow.metal:550-609, cubic Hermite over the cycle phase time *
+ * frequency): `count` keyframes of 17 floats (phase, left stroke / deviation /
+ * pitch / tip twist angles, their rates per second, right angles, right
+ * rates); 4...4096 keyframes, phases strictly increasing in [0, 1) from 0.
+ * count 0 returns to the analytic sinusoid.  With the bird set, the current
+ * step's geometry is rebuilt. */
+bf_status bf_set_bird_keyframes(bf_context* ctx, const float* keyframes, uint32_t count);
 /* Prepared frame of the current step (GPUPreparedBirdGeometry, 14 float4:
  * body position, orientation x y z w, linear velocity, world angular
  * velocity + geometry radius, then left and right wing root, chord (w: tip

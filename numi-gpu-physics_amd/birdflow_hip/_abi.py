@@ -47,7 +47,7 @@ EXPORTED_FUNCTIONS = (
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
     "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity", "bf_mesh_freeze_masks",
-    "bf_set_bird", "bf_read_bird_frame", "bf_ledger_fluid", "bf_ledger_step_sources",
+    "bf_set_bird", "bf_set_bird_keyframes", "bf_read_bird_frame", "bf_ledger_fluid", "bf_ledger_step_sources",
 )
 
 
@@ -184,6 +184,7 @@ def _declare(l):
         "bf_mesh_freeze_masks": (st, [vp, vp, u32]),
         "bf_set_bird": (st, [vp, ctypes.POINTER(BirdGeometry)]),
         "bf_read_bird_frame": (st, [vp, vp]),
+        "bf_set_bird_keyframes": (st, [vp, vp, u32]),
         "bf_ledger_fluid": (st, [vp, vp]),
         "bf_ledger_step_sources": (st, [vp, vp, vp]),
     }

@@ -60,7 +60,7 @@ _GEOMETRY_ARRAYS = {
              ("triangle_parts", np.uint8), ("translation", np.float32)),
     "static": (("wall", np.float32), ("mask_prev", np.uint8)),
     "sphere": (),
-    "bird": (),
+    "bird": (("keyframes", np.float32),),
 }
 
 
@@ -210,6 +210,9 @@ def save_checkpoint(sim: BirdFlowSimulation, destination: str, compression: str 
         # the shape; a restore prepares the frame again from the body pose
         # and keeps the archive's mask as the next step's previous mask
         geometry["bird"] = {k: _plain(v) for k, v in src["bird"].items()}
+        if src.get("keyframes") is not None:
+            arrays["keyframes"] = src["keyframes"]
+            geometry["shapes"] = {"keyframes": list(src["keyframes"].shape)}
     else:
         arrays["wall"] = wall
         if src["mask_prev"] is not None and step == src["set_at_step"]:
@@ -267,7 +270,7 @@ def read_checkpoint(source: str) -> Checkpoint:
     for name, dtype in _GEOMETRY_ARRAYS[kind]:
         key = f"geometry-{name}.bin"
         if key not in files:
-            if kind == "static" and name == "mask_prev":
+            if (kind == "static" and name == "mask_prev") or (kind == "bird" and name == "keyframes"):
                 continue
             raise CheckpointError(f"invalid archive: {key} missing")
         a = np.frombuffer(files[key], dtype)
@@ -296,7 +299,7 @@ def load_checkpoint(source: str) -> BirdFlowSimulation:
         elif g["kind"] == "sphere":
             sim.set_translating_sphere(g["center_radius"], g["geometry_velocity"], g["wall_velocity"])
         elif g["kind"] == "bird":
-            sim.set_bird(BirdGeometry(**g["bird"]))
+            sim.set_bird(BirdGeometry(**g["bird"]), keyframes=ck.geometry.get("keyframes"))
         else:
             sim.set_geometry(ck.solid_mask, ck.geometry["wall"], mask_prev=ck.geometry.get("mask_prev"))
         params = None if m["body_parameters"] is None else BodyParameters(**m["body_parameters"])

@@ -382,12 +382,18 @@ class BirdFlowSimulation:
         self.geometry_source = {"kind": "sphere", "center_radius": cr.tolist(), "geometry_velocity": gv.tolist(),
                                 "wall_velocity": wv.tolist()}
 
-    def set_bird(self, bird: "BirdGeometry"):
+    def set_bird(self, bird: "BirdGeometry", keyframes=None):
         """Analytic signed-distance bird (BirdFlow.metal:686-938) flying on the
-        body pose; builds the current step."""
+        body pose; builds the current step.  keyframes: measured wing
+        kinematics, [count, 17] (phase, left angles stroke / deviation / pitch
+        / tip twist, left rates, right angles, right rates), or None for the
+        analytic sinusoid."""
+        kf = None if keyframes is None else np.ascontiguousarray(keyframes, np.float32).reshape(-1, 17)
+        self._check(self._lib.bf_set_bird_keyframes(self._ctx, _ptr(kf), 0 if kf is None else kf.shape[0]))
         cb = bird.to_c()
         self._check(self._lib.bf_set_bird(self._ctx, ctypes.byref(cb)))
-        self.geometry_source = {"kind": "bird", "bird": dataclasses.asdict(bird)}
+        self.geometry_source = {"kind": "bird", "bird": dataclasses.asdict(bird),
+                                "keyframes": None if kf is None else kf.copy()}
 
     def bird_frame(self) -> np.ndarray:
         """Prepared frame of the current step, [14, 4] (GPUPreparedBirdGeometry rows)."""

@@ -138,6 +138,9 @@ struct bf_context {
     // analytic bird: shape and the prepared frame of each step parity
     BirdShape bird{};
     float4* bird_frame = nullptr;         // [2][BIRD_FRAME_ROWS]
+    DeviceAlloc keyframe_alloc;           // measured wing keyframes, 17 floats each
+    float* bird_keyframes = nullptr;
+    uint32_t bird_keyframe_count = 0;
     // sphere
     float4 sphere_cr{};
     float3 sphere_gv{};
@@ -416,6 +419,8 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         // (the reference's initialization build)
         BirdBuffers b{};
         b.body = c->body;
+        b.keyframes = c->bird_keyframes;
+        b.keyframe_count = c->bird_keyframe_count;
         b.frame = c->bird_frame + BIRD_FRAME_ROWS * par2(s);
         b.mask_prev = fresh ? nullptr : c->mask[mod3(s - 1)];
         b.dyn_prev = fresh ? nullptr : c->dyn[mod3(s - 1)];
@@ -740,6 +745,7 @@ void bf_destroy(bf_context* c) {
     }
     c->mesh_alloc.release();
     c->ledger_alloc.release();
+    c->keyframe_alloc.release();
     c->adj.alloc.release();
     c->adj.tangent_alloc.release();
     c->frozen_alloc.release();
@@ -1125,11 +1131,52 @@ bf_status bf_set_bird(bf_context* c, const bf_bird_geometry* bird) {
         return fail(c, BF_ERROR_INVALID_ARGUMENT, "wing span, chords and thickness must be positive");
     if (!(b.tail_length >= 0.0f && b.tail_half_width >= 0.0f && b.tail_thickness >= 0.0f && b.frequency >= 0.0f))
         return fail(c, BF_ERROR_INVALID_ARGUMENT, "tail sizes and flapping frequency must not be negative");
+    if (c->bird_keyframe_count > 0 && !(b.frequency > 0.0f))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "measured keyframes need a positive flapping frequency");
     std::memcpy(&c->bird, bird, sizeof(BirdShape));
     c->source = BF_GEOMETRY_BIRD;
     c->static_prev_differs = false;
     c->time_start = 0.0f;
     return rebuild_bird_geometry(c);
+}
+
+bf_status bf_set_bird_keyframes(bf_context* c, const float* keyframes, uint32_t count) {
+    BF_ENTER(c);
+    if (count == 0) {
+        c->keyframe_alloc.release();
+        c->bird_keyframes = nullptr;
+        c->bird_keyframe_count = 0;
+    } else {
+        if (!keyframes) return fail(c, BF_ERROR_INVALID_ARGUMENT, "keyframes must not be NULL");
+        if (count < 4 || count > 4096) return fail(c, BF_ERROR_INVALID_ARGUMENT, "keyframes must number 4...4096");
+        if (c->source == BF_GEOMETRY_BIRD && !(c->bird.frequency > 0.0f))
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "measured keyframes need a positive flapping frequency");
+        if (keyframes[0] != 0.0f) return fail(c, BF_ERROR_INVALID_ARGUMENT, "the first keyframe phase must be zero");
+        float previous = -1.0f;
+        for (uint32_t k = 0; k < count; ++k) {
+            const float* f = keyframes + 17ull * k;
+            if (!(std::isfinite(f[0]) && f[0] >= 0.0f && f[0] < 1.0f && f[0] > previous))
+                return fail(c, BF_ERROR_INVALID_ARGUMENT,
+                            "keyframe phases must be finite, strictly increasing and in [0, 1)");
+            previous = f[0];
+            for (int i = 1; i < 17; ++i)
+                if (!std::isfinite(f[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "keyframe values must be finite");
+        }
+        c->keyframe_alloc.release();
+        hipError_t e = hipSuccess;
+        float* d = (float*)c->keyframe_alloc.take(e, 17ull * 4 * count);
+        if (e == hipSuccess) e = hipMemcpy(d, keyframes, 17ull * 4 * count, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            c->keyframe_alloc.release();
+            c->bird_keyframes = nullptr;
+            c->bird_keyframe_count = 0;
+            return hip_fail(c, e, "keyframe upload");
+        }
+        c->bird_keyframes = d;
+        c->bird_keyframe_count = count;
+    }
+    // the current step's geometry follows the new kinematics
+    return c->source == BF_GEOMETRY_BIRD ? rebuild_bird_geometry(c) : BF_OK;
 }
 
 bf_status bf_read_bird_frame(bf_context* c, float frame[56]) {

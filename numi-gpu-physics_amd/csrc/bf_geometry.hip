@@ -689,11 +689,12 @@ __device__ inline float sd_tail(V3 l, const BirdShape& b) {
                   fabsf(l.z + 0.15f * b.body_radii[2]) - 0.5f * b.tail_thickness);
 }
 
-// makeWingFrame (BirdFlow.metal:611-684) with zero deviation and tip twist;
-// writes root, chord (w: tip twist), span, normal, angular velocity (w: tip
-// twist rate).
-__device__ inline void wing_frame(float side, float stroke, float stroke_rate, float pitch, float pitch_rate,
-                                  const BirdShape& b, Quat q, V3 body_pos, float4* out) {
+// makeWingFrame (BirdFlow.metal:611-684); writes root, chord (w: tip
+// twist), span, normal, angular velocity (w: tip twist rate).
+__device__ inline void wing_frame(float side, const float (&a)[4], const float (&r)[4], const BirdShape& b, Quat q,
+                                  V3 body_pos, float4* out) {
+    const float stroke = a[0], deviation = a[1], pitch = a[2], tip_twist = a[3];
+    const float stroke_rate = r[0], deviation_rate = r[1], pitch_rate = r[2], tip_twist_rate = r[3];
     const V3 ex = qrotate(q, v3(1.0f, 0.0f, 0.0f));
     const V3 ey = qrotate(q, v3(0.0f, 1.0f, 0.0f));
     const V3 ez = qrotate(q, v3(0.0f, 0.0f, 1.0f));
@@ -702,7 +703,6 @@ __device__ inline void wing_frame(float side, float stroke, float stroke_rate, f
     const float signed_stroke = side * stroke;
     const V3 span_stroke = rotate_axis(vmul(ey, side), ex, signed_stroke);
     const V3 normal_stroke = rotate_axis(vmul(ez, side), ex, signed_stroke);
-    const float deviation = 0.0f, deviation_rate = 0.0f;
     const V3 sp = rotate_axis(span_stroke, normal_stroke, deviation);
     const V3 chord_pre = rotate_axis(ex, normal_stroke, deviation);
     const float signed_pitch = side * pitch;
@@ -711,7 +711,6 @@ __device__ inline void wing_frame(float side, float stroke, float stroke_rate, f
     const V3 spn = vnormalize(sp);
     const V3 w = vadd(vadd(vmul(ex, side * stroke_rate), vmul(normal_stroke, deviation_rate)),
                       vmul(spn, side * pitch_rate));
-    const float tip_twist = 0.0f, tip_twist_rate = 0.0f;
     out[0] = row4(root, 0.0f);
     out[1] = row4(vnormalize(ch), side * tip_twist);
     out[2] = row4(spn, 0.0f);
@@ -719,13 +718,50 @@ __device__ inline void wing_frame(float side, float stroke, float stroke_rate, f
     out[4] = row4(w, side * tip_twist_rate);
 }
 
+// measuredWingKinematics (BirdFlow.metal:550-609): cubic Hermite between the
+// keyframes around the cycle phase.  A keyframe is 17 floats: phase, left
+// angles (stroke, deviation, pitch, tip twist), left rates (per second),
+// right angles, right rates.
+__device__ inline void measured_angles(const float* __restrict__ kf, int count, float frequency, float phase,
+                                       int off, float (&ang)[4], float (&rate)[4]) {
+    int low = 0, high = count;
+    while (low < high) {
+        const int middle = low + (high - low) / 2;
+        if (kf[17 * middle] <= phase) low = middle + 1;
+        else high = middle;
+    }
+    const int upper = low == count ? 0 : low;
+    const int lower = upper == 0 ? count - 1 : upper - 1;
+    const float phase0 = kf[17 * lower];
+    const float phase1 = upper == 0 ? 1.0f : kf[17 * upper];
+    const float adjusted = upper == 0 && phase < phase0 ? phase + 1.0f : phase;
+    const float interval = phase1 - phase0;
+    const float f = (adjusted - phase0) / interval;
+    const float seconds = interval / frequency;
+    const float t2 = f * f, t3 = t2 * f;
+    const float h00 = 2.0f * t3 - 3.0f * t2 + 1.0f;
+    const float h10 = t3 - 2.0f * t2 + f;
+    const float h01 = -2.0f * t3 + 3.0f * t2;
+    const float h11 = t3 - t2;
+    const float d00 = 6.0f * t2 - 6.0f * f;
+    const float d10 = 3.0f * t2 - 4.0f * f + 1.0f;
+    const float d01 = -6.0f * t2 + 6.0f * f;
+    const float d11 = 3.0f * t2 - 2.0f * f;
+    const float* a0 = kf + 17 * lower + off;
+    const float* a1 = kf + 17 * upper + off;
+    for (int i = 0; i < 4; ++i) {
+        ang[i] = h00 * a0[i] + h10 * seconds * a0[4 + i] + h01 * a1[i] + h11 * seconds * a1[4 + i];
+        rate[i] = (d00 * a0[i] + d10 * seconds * a0[4 + i] + d01 * a1[i] + d11 * seconds * a1[4 + i]) / seconds;
+    }
+}
+
 }  // namespace
 
 // One lane: the frame rows body position, orientation, linear velocity,
 // (world angular velocity, geometry radius), then left and right wing
 // (root, chord, span, normal, angular velocity).
-__global__ void bird_prepare(BirdShape b, const double* __restrict__ body, float time, float dx,
-                             float4* __restrict__ frame) {
+__global__ void bird_prepare(BirdShape b, const double* __restrict__ body, const float* __restrict__ keyframes,
+                             int keyframe_count, float time, float dx, float4* __restrict__ frame) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const V3 pos = v3((float)body[0], (float)body[1], (float)body[2]);
     Quat q = {(float)body[3], (float)body[4], (float)body[5], (float)body[6]};
@@ -738,18 +774,32 @@ __global__ void bird_prepare(BirdShape b, const double* __restrict__ body, float
     }
     const V3 lin = v3((float)body[7], (float)body[8], (float)body[9]);
     const V3 wb = v3((float)body[10], (float)body[11], (float)body[12]);
-    const float omega = 2.0f * 3.14159265358979323846f * b.frequency;
-    const float phase = omega * time;
-    float s0, c0, s1, c1;
-    bird_sincos(phase, s0, c0);
-    bird_sincos(phase + b.pitch_phase, s1, c1);
-    const float stroke = b.stroke_bias + b.stroke_amplitude * s0;
-    const float stroke_rate = omega * b.stroke_amplitude * c0;
-    const float pitch = b.pitch_mean + b.pitch_amplitude * s1;
-    const float pitch_rate = omega * b.pitch_amplitude * c1;
+    float la[4], lr[4], ra[4], rr[4];
+    if (keyframe_count > 0) {
+        const float cycles = time * b.frequency;
+        float phase = cycles - truncf(cycles);
+        phase = phase < 0.0f ? phase + 1.0f : phase;
+        measured_angles(keyframes, keyframe_count, b.frequency, phase, 1, la, lr);
+        measured_angles(keyframes, keyframe_count, b.frequency, phase, 9, ra, rr);
+    } else {
+        const float omega = 2.0f * 3.14159265358979323846f * b.frequency;
+        const float phase = omega * time;
+        float s0, c0, s1, c1;
+        bird_sincos(phase, s0, c0);
+        bird_sincos(phase + b.pitch_phase, s1, c1);
+        la[0] = b.stroke_bias + b.stroke_amplitude * s0;
+        lr[0] = omega * b.stroke_amplitude * c0;
+        la[2] = b.pitch_mean + b.pitch_amplitude * s1;
+        lr[2] = omega * b.pitch_amplitude * c1;
+        la[1] = lr[1] = la[3] = lr[3] = 0.0f;
+        for (int i = 0; i < 4; ++i) {
+            ra[i] = la[i];
+            rr[i] = lr[i];
+        }
+    }
     float4 rows[BIRD_FRAME_ROWS];
-    wing_frame(1.0f, stroke, stroke_rate, pitch, pitch_rate, b, q, pos, rows + 4);
-    wing_frame(-1.0f, stroke, stroke_rate, pitch, pitch_rate, b, q, pos, rows + 9);
+    wing_frame(1.0f, la, lr, b, q, pos, rows + 4);
+    wing_frame(-1.0f, ra, rr, b, q, pos, rows + 9);
     rows[0] = row4(pos, 0.0f);
     rows[1] = make_float4(q.x, q.y, q.z, q.w);
     rows[2] = row4(lin, 0.0f);
@@ -897,7 +947,8 @@ __global__ __launch_bounds__(256) void bird_tile_pass(BirdShape b, const float4*
 
 hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s) {
     const unsigned tile_blocks = (p.g.tiles + 255) / 256;
-    hipLaunchKernelGGL(bird_prepare, dim3(1), dim3(64), 0, s, shape, b.body, p.time, p.dx, b.frame);
+    hipLaunchKernelGGL(bird_prepare, dim3(1), dim3(64), 0, s, shape, b.body, b.keyframes, (int)b.keyframe_count,
+                       p.time, p.dx, b.frame);
     hipLaunchKernelGGL(bird_mark_tiles, dim3(tile_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
                        b.dyn_prev, b.tiles_new, p);
     hipLaunchKernelGGL(fill_u8, dim3(tile_blocks), dim3(256), 0, s, b.dyn, (uint8_t)0, p.g.tiles);

@@ -75,6 +75,8 @@ struct BirdParams {
 
 struct BirdBuffers {
     const double* body;               // device body state (FP64)
+    const float* keyframes;           // measured wing keyframes (17 floats each), or null
+    uint32_t keyframe_count;          // 0: the analytic sinusoid
     float4* frame;                    // prepared frame of this step (written)
     const uint8_t* mask_prev;         // previous step's mask, or null: no previous geometry
     const uint8_t* dyn_prev;          // tiles near the previous step's solid cells, or null

@@ -210,10 +210,13 @@ typedef struct bfo_bird_frame {
 /* sin / cos of the geometry (quadrant reduction + Taylor polynomials). */
 void bfo_sincos(float x, float* s, float* c);
 void bfo_sincos_n(const float* x, int n, float* s, float* c);
-/* prepareBirdGeometry (BirdFlow.metal:686-814) at `time` for a body pose. */
+/* prepareBirdGeometry (BirdFlow.metal:686-814) at `time` for a body pose;
+ * keyframe_count > 0: measured keyframes (17 floats each: phase, left angles
+ * stroke / deviation / pitch / tip twist, left rates, right angles, right
+ * rates) instead of the analytic sinusoid. */
 void bfo_bird_prepare(const bfo_bird* b, const float body_position[3], const float orientation[4],
                       const float linear_velocity[3], const float angular_velocity_body[3], float time, float dx,
-                      bfo_bird_frame* out);
+                      const float* keyframes, int keyframe_count, bfo_bird_frame* out);
 /* buildBirdGeometry (BirdFlow.metal:816-938) over the grid of p (origin, dx):
  * part mask 1 body, 2 left wing, 3 right wing, 4 tail; wall = (velocity *
  * velocity_to_lattice, 0).  solid_prev NULL: no previous geometry. */

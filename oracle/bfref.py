@@ -136,7 +136,8 @@ def _declare(l):
     l.bfo_body_safety.restype = ctypes.c_uint32
     l.bfo_place_vertices.argtypes = [ctypes.POINTER(Body), ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
     l.bfo_sincos_n.argtypes = [vp, ctypes.c_int, vp, vp]
-    l.bfo_bird_prepare.argtypes = [ctypes.POINTER(Bird), vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp]
+    l.bfo_bird_prepare.argtypes = [ctypes.POINTER(Bird), vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp,
+                                   ctypes.c_int, vp]
     l.bfo_bird_build.argtypes = [P, ctypes.POINTER(Bird), vp, vp, ctypes.c_float, vp, vp]
 
 
@@ -451,12 +452,15 @@ def sincos(x):
 
 
 def bird_prepare(bird: Bird, position, orientation=(0, 0, 0, 1), linear_velocity=(0, 0, 0),
-                 angular_velocity_body=(0, 0, 0), time=0.0, dx=1.0) -> np.ndarray:
-    """Prepared frame [14, 4] (rows in BIRD_FRAME_FIELDS order)."""
+                 angular_velocity_body=(0, 0, 0), time=0.0, dx=1.0, keyframes=None) -> np.ndarray:
+    """Prepared frame [14, 4] (rows in BIRD_FRAME_FIELDS order); keyframes:
+    [count, 17] measured wing keyframes or None (analytic sinusoid)."""
     out = np.zeros((14, 4), np.float32)
     arrs = [np.ascontiguousarray(v, np.float32) for v in (position, orientation, linear_velocity,
                                                           angular_velocity_body)]
-    lib().bfo_bird_prepare(ctypes.byref(bird), *[_ptr(a) for a in arrs], np.float32(time), np.float32(dx), _ptr(out))
+    kf = None if keyframes is None else np.ascontiguousarray(keyframes, np.float32).reshape(-1, 17)
+    lib().bfo_bird_prepare(ctypes.byref(bird), *[_ptr(a) for a in arrs], np.float32(time), np.float32(dx),
+                           _ptr(kf), 0 if kf is None else kf.shape[0], _ptr(out))
     return out
 
 

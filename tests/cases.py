@@ -221,8 +221,32 @@ def compact_bird_case(free_flight=False, shape=(48, 48, 48)):
     return p, cfg, bh.BirdGeometry(**COMPACT_BIRD), bfref.make_bird(**COMPACT_BIRD), sc
 
 
-def bird_frame_oracle(bird, body, time, dx):
+def bird_frame_oracle(bird, body, time, dx, keyframes=None):
     """Oracle frame from an FP64 body 13-vector, cast to FP32 as the device does."""
     b = np.asarray(body, np.float64)
     return bfref.bird_prepare(bird, b[0:3].astype(np.float32), b[3:7].astype(np.float32),
-                              b[7:10].astype(np.float32), b[10:13].astype(np.float32), time=time, dx=dx)
+                              b[7:10].astype(np.float32), b[10:13].astype(np.float32), time=time, dx=dx,
+                              keyframes=keyframes)
+
+
+def compact_bird_keyframes(count=24):
+    """Measured-style keyframes for the compact bird: stroke and pitch of its
+    sinusoids, deviation 0.2 sin(.+0.5) and tip twist 0.35 sin(.+1.0) on the
+    left, the right side lagging by a tenth of a cycle (asymmetric)."""
+    c = COMPACT_BIRD
+    f = c["frequency"]
+    w = 2 * math.pi * f
+    kf = np.zeros((count, 17))
+    for k in range(count):
+        ph = k / count
+        row = [ph]
+        for lag in (0.0, 0.1):
+            t = (ph - lag) / f
+            a = [c["stroke_bias"] + c["stroke_amplitude"] * math.sin(w * t), 0.2 * math.sin(w * t + 0.5),
+                 c["pitch_mean"] + c["pitch_amplitude"] * math.sin(w * t + c["pitch_phase"]),
+                 0.35 * math.sin(w * t + 1.0)]
+            r = [w * c["stroke_amplitude"] * math.cos(w * t), w * 0.2 * math.cos(w * t + 0.5),
+                 w * c["pitch_amplitude"] * math.cos(w * t + c["pitch_phase"]), w * 0.35 * math.cos(w * t + 1.0)]
+            row += a + r
+        kf[k] = row
+    return kf.astype(np.float32)

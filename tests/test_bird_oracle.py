@@ -17,7 +17,12 @@ oracle's expressions:
     wing cells add the wing's relative rotation about its root;
   * a symmetric pose gives a mask mirrored across the body x-z plane with
     the wing part ids swapped;
-  * a cell solid in the previous mask is evaluated even beyond the radius.
+  * a cell solid in the previous mask is evaluated even beyond the radius;
+  * measured keyframes (BirdFlow.metal:550-609): keyframes sampled from the
+    sinusoid reproduce the analytic frame, keyframe values are hit at their
+    phases, a deviation rotates span and chord about the stroke normal in
+    closed form, and the angular velocities (deviation and tip-twist rates
+    included) match frame differences.
 """
 import math
 
@@ -229,3 +234,92 @@ def test_previously_solid_cell_beyond_the_radius_is_evaluated():
     other = np.ones(48 ** 3, bool)
     other[gid] = False
     assert np.array_equal(m0[other], m1[other]) and np.array_equal(w0[other], w1[other])
+
+
+# ---------------------------------------------------------------------------
+# measured-keyframe kinematics (BirdFlow.metal:550-609)
+# ---------------------------------------------------------------------------
+def sinusoid_keyframes(cfg, count, deviation=(0.0, 0.0), twist=(0.0, 0.0)):
+    """Keyframes sampling the analytic stroke / pitch sinusoids (exact rates),
+    plus optional deviation and tip-twist sinusoids (amplitude, phase)."""
+    f = cfg["frequency"]
+    w = 2 * math.pi * f
+    kf = np.zeros((count, 17))
+    for k in range(count):
+        ph = k / count
+        t = ph / f
+        stroke = cfg["stroke_bias"] + cfg["stroke_amplitude"] * math.sin(w * t)
+        srate = w * cfg["stroke_amplitude"] * math.cos(w * t)
+        pitch = cfg["pitch_mean"] + cfg["pitch_amplitude"] * math.sin(w * t + cfg["pitch_phase"])
+        prate = w * cfg["pitch_amplitude"] * math.cos(w * t + cfg["pitch_phase"])
+        dev = deviation[0] * math.sin(w * t + deviation[1])
+        drate = w * deviation[0] * math.cos(w * t + deviation[1])
+        tw = twist[0] * math.sin(w * t + twist[1])
+        trate = w * twist[0] * math.cos(w * t + twist[1])
+        side = [stroke, dev, pitch, tw, srate, drate, prate, trate]
+        kf[k] = [ph] + side + side
+    return kf.astype(np.float32)
+
+
+def test_measured_keyframes_of_the_sinusoid_reproduce_the_analytic_frame():
+    b = bfref.make_bird(**DEMO)
+    kf = sinusoid_keyframes(DEMO, 96)
+    rng = np.random.default_rng(5)
+    pos, q, _, _ = random_pose(rng)
+    worst = 0.0
+    for t in rng.uniform(0.0, 2.0, 40):
+        fa = bfref.bird_prepare(b, pos, q, time=t, dx=0.004)
+        fm = bfref.bird_prepare(b, pos, q, time=t, dx=0.004, keyframes=kf)
+        for name in ("left", "right"):
+            for k in ("root", "chord", "span", "normal"):
+                worst = max(worst, np.abs(row(fa, f"{name}_{k}")[:3] - row(fm, f"{name}_{k}")[:3]).max())
+            wa, wm = row(fa, f"{name}_angular")[:3], row(fm, f"{name}_angular")[:3]
+            assert np.linalg.norm(wa - wm) <= 2e-4 * max(1.0, np.linalg.norm(wa))
+    assert worst < 2e-5
+
+
+def test_measured_keyframe_values_are_hit_exactly_at_their_phases():
+    b = bfref.make_bird(**dict(DEMO, frequency=4.0))
+    kf = sinusoid_keyframes(dict(DEMO, frequency=4.0), 8, deviation=(0.2, 0.3), twist=(0.4, 1.1))
+    for k in range(8):
+        t = float(np.float32(kf[k, 0])) / 4.0
+        f = bfref.bird_prepare(b, (0, 0, 0), time=t, dx=0.004, keyframes=kf)
+        assert row(f, "left_chord")[3] == pytest.approx(float(kf[k, 4]), abs=2e-6)
+        assert row(f, "right_chord")[3] == pytest.approx(-float(kf[k, 12]), abs=2e-6)
+        assert row(f, "left_angular")[3] == pytest.approx(float(kf[k, 8]), rel=1e-4, abs=1e-4)
+
+
+def test_deviation_rotates_span_and_chord_about_the_stroke_normal():
+    b = bfref.make_bird(**DEMO)
+    for dev in (-0.4, 0.15, 0.7):
+        kf = np.zeros((4, 17), np.float32)
+        kf[:, 0] = [0.0, 0.25, 0.5, 0.75]
+        kf[:, 2] = dev
+        kf[:, 10] = dev
+        f = bfref.bird_prepare(b, (0, 0, 0), time=0.3, dx=0.004, keyframes=kf)
+        c, s = math.cos(dev), math.sin(dev)
+        assert np.allclose(row(f, "left_span")[:3], (-s, c, 0), atol=2e-6)
+        assert np.allclose(row(f, "left_chord")[:3], (c, s, 0), atol=2e-6)
+        # right side: rotation about its own stroke normal (-z)
+        assert np.allclose(row(f, "right_span")[:3], (-s, -c, 0), atol=2e-6)
+        assert np.allclose(row(f, "right_chord")[:3], (c, -s, 0), atol=2e-6)
+
+
+def test_measured_wing_angular_velocity_matches_frame_differences():
+    b = bfref.make_bird(**DEMO)
+    kf = sinusoid_keyframes(DEMO, 64, deviation=(0.25, 0.7), twist=(0.3, 0.2))
+    rng = np.random.default_rng(9)
+    pos, q, _, _ = random_pose(rng)
+    h = 2e-4
+    for t in (0.021, 0.09, 0.16):
+        fp = bfref.bird_prepare(b, pos, q, time=t + h, dx=0.004, keyframes=kf)
+        fm = bfref.bird_prepare(b, pos, q, time=t - h, dx=0.004, keyframes=kf)
+        f = bfref.bird_prepare(b, pos, q, time=t, dx=0.004, keyframes=kf)
+        for name in ("left", "right"):
+            w = row(f, f"{name}_angular")[:3]
+            for k in ("chord", "span", "normal"):
+                d = (row(fp, f"{name}_{k}")[:3] - row(fm, f"{name}_{k}")[:3]) / (2 * h)
+                assert np.linalg.norm(d - np.cross(w, row(f, f"{name}_{k}")[:3])) <= 2e-3 * np.linalg.norm(w) + 1e-3
+            # tip twist rate against differences of the tip twist
+            dtw = (row(fp, f"{name}_chord")[3] - row(fm, f"{name}_chord")[3]) / (2 * h)
+            assert abs(dtw - row(f, f"{name}_angular")[3]) <= 2e-3 * abs(dtw) + 1e-3

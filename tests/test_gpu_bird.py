@@ -41,18 +41,21 @@ def geometry_time(cfg, s):
 POSE = np.array([0.061, 0.058, 0.0595, 0.08, -0.05, 0.12, 0.985, 0.4, -0.2, 0.1, 3.0, -2.0, 4.0])
 
 
-def test_bird_geometry_matches_oracle_every_step():
+@pytest.mark.parametrize("measured", [False, True])
+def test_bird_geometry_matches_oracle_every_step(measured):
     p, cfg, bird, obird, _ = cases.compact_bird_case()
     cfg.time_step = 0.004          # large enough for the wings to sweep several cells per step
     body = POSE.copy()
     body[3:7] /= np.linalg.norm(body[3:7])
-    say(f"case: compact bird {p.nx}x{p.ny}x{p.nz}, rotated spinning pose, dt {cfg.time_step}, 6 steps")
+    kf = cases.compact_bird_keyframes() if measured else None
+    say(f"case: compact bird {p.nx}x{p.ny}x{p.nz}, rotated spinning pose, dt {cfg.time_step}, 6 steps, "
+        f"{'measured keyframes with deviation and tip twist' if measured else 'analytic kinematics'}")
     with bh.BirdFlowSimulation(cfg) as sim:
         sim.set_body(body)
-        sim.set_bird(bird)
+        sim.set_bird(bird, keyframes=kf)
         frame = sim.bird_frame()
         mask, wall = sim.read_geometry()
-        f_o = cases.bird_frame_oracle(obird, body, geometry_time(cfg, 0), p.dx)
+        f_o = cases.bird_frame_oracle(obird, body, geometry_time(cfg, 0), p.dx, kf)
         m_o, w_o = bfref.bird_build(p, obird, f_o, None, cfg.velocity_to_lattice)
         say(f"  step 0: solid {int((m_o != 0).sum())} cells, parts {np.bincount(m_o, minlength=5)[1:]}, "
             f"frame equal {np.array_equal(frame, f_o)}, mask differing {int((mask != m_o).sum())}, "
@@ -68,7 +71,7 @@ def test_bird_geometry_matches_oracle_every_step():
             sim.advance(1)
             frame = sim.bird_frame()
             mask, wall = sim.read_geometry()
-            f_o = cases.bird_frame_oracle(obird, body, geometry_time(cfg, s), p.dx)
+            f_o = cases.bird_frame_oracle(obird, body, geometry_time(cfg, s), p.dx, kf)
             m_o, w_o = bfref.bird_build(p, obird, f_o, prev, cfg.velocity_to_lattice)
             moved = int((m_o != prev).sum())
             read = (m_o != 0) | (prev != 0)

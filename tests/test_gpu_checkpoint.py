@@ -118,14 +118,14 @@ def test_restart_of_free_flying_body_carrying_its_mesh_is_exact(tmp_path):
 
 
 def test_restart_of_free_flying_analytic_bird_is_exact(tmp_path):
-    say("case: checkpoint of the free-flying compact analytic bird 48^3, 5 + 4 steps")
+    say("case: checkpoint of the free-flying compact analytic bird 48^3 on measured keyframes, 5 + 4 steps")
     p, cfg, bird, _, _ = cases.compact_bird_case(free_flight=True)
     cfg.time_step = 0.002
     state0 = np.array(list(p.body_position) + [0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
     with bh.BirdFlowSimulation(cfg) as sim:
         sim.set_body(state0, bh.BodyParameters(mass=0.025, inertia=(4e-6, 7e-6, 8e-6), gravity=(0.0, 0.0, -9.81),
                                                substeps=2))
-        sim.set_bird(bird)
+        sim.set_bird(bird, keyframes=cases.compact_bird_keyframes())
         sim.init_equilibrium()
         check_restart(sim, tmp_path, 5, 4)
 
