@@ -70,7 +70,8 @@ enum { BF_COLLISION_TRT = 0, BF_COLLISION_TRT_SYMMETRIC_LIMITER = 1,
  * link fraction from the signed-distance ratio (BirdFlow.metal:2044-2076,
  * 2729-2806), or the interpolated path at q = 1/2 (translating-sphere
  * canonical, BirdFlow.metal:2340-2345). */
-enum { BF_WALL_HALFWAY = 0, BF_WALL_INTERPOLATED_SDF = 1, BF_WALL_INTERPOLATED_HALF = 2 };
+enum { BF_WALL_HALFWAY = 0, BF_WALL_INTERPOLATED_SDF = 1, BF_WALL_INTERPOLATED_HALF = 2,
+       BF_WALL_INTERPOLATED_LINKS = 3 };
 
 /* Population storage: FP32, or FP16 / BF16 offsets f - w_q with FP32
  * arithmetic. */
@@ -87,7 +88,8 @@ enum { BF_STORAGE_FP32 = 0, BF_STORAGE_FP16_OFFSET = 1, BF_STORAGE_BF16_OFFSET =
 enum { BF_ARITHMETIC_DEFAULT = 0, BF_ARITHMETIC_STRICT = 1, BF_ARITHMETIC_FUSED = 2 };
 
 /* Where each step's geometry comes from. */
-enum { BF_GEOMETRY_STATIC = 0, BF_GEOMETRY_MESH = 1, BF_GEOMETRY_SPHERE = 2, BF_GEOMETRY_BIRD = 3 };
+enum { BF_GEOMETRY_STATIC = 0, BF_GEOMETRY_MESH = 1, BF_GEOMETRY_SPHERE = 2, BF_GEOMETRY_BIRD = 3,
+       BF_GEOMETRY_WING = 4 };
 
 /* Solver configuration in lattice units (the host converts physical units
  * with the reference's LatticeScaling, SimulationConfiguration.swift:137-183).
@@ -236,6 +238,38 @@ bf_status bf_set_bird_keyframes(bf_context* ctx, const float* keyframes, uint32_
  * velocity + geometry radius, then left and right wing root, chord (w: tip
  * twist), span, normal, angular velocity (w: tip twist rate)). */
 bf_status bf_read_bird_frame(bf_context* ctx, float frame[56]);
+
+/* Prescribed beta-planform flapping wing (GPUFlappingWingParameters,
+ * GPUData.swift:383-388; BirdFlow.metal:940-1297): root (grid world units),
+ * mean chord, span radius, thickness, beta exponent and normalization, cycle
+ * (time units of the grid's time_step), stroke amplitude (rad), stroke and
+ * pitch transition durations (fractions of a cycle), pitch at mid-stroke
+ * (downstroke, upstroke; rad), maximum stroke rate (rad per cycle) and the
+ * reference speed, in that order. */
+typedef struct bf_flapping_wing {
+    float root[3], chord_mean;
+    float radius, thickness, beta_exponent, beta_normalization;
+    float cycle, stroke_amplitude, stroke_duration, pitch_duration;
+    float pitch_low, pitch_high, stroke_rate_max, reference_speed;
+} bf_flapping_wing;
+
+/* preparePrescribedFlappingWing + buildPrescribedFlappingWing +
+ * the link-fraction pass (BirdFlow.metal:1040-1297, encoded by
+ * MetalFlappingWingValidation.swift): each step's frame at t = step *
+ * time_step, mask 1 on the wing, wall = (omega x r * velocity_to_lattice,
+ * -1 on the wing, +1 near it), and the bisected fluid-side link fraction of
+ * every fluid cell whose source is on the wing, read by the wall rule
+ * BF_WALL_INTERPOLATED_LINKS.  Builds the current step and selects
+ * BF_GEOMETRY_WING. */
+bf_status bf_set_flapping_wing(bf_context* ctx, const bf_flapping_wing* wing);
+/* Prepared frame of the current step (GPUPreparedFlappingWing, 6 float4:
+ * root (w: mean chord), chord, span, normal, angular velocity, state =
+ * stroke, pitch and their rates per time unit). */
+bf_status bf_read_wing_frame(bf_context* ctx, float frame[24]);
+/* The current step's link table as links[q * N + cell] (count = 19 N): the
+ * fraction for a fluid cell whose source cell - c_q is on the wing, 0
+ * elsewhere. */
+bf_status bf_read_wing_links(bf_context* ctx, float* links, uint64_t count);
 
 /* ---- momentum ledger (FP64 on the device) ----
  * The device sums of BirdFlowSimulation.advanceWithCoupledMomentumLedger

@@ -32,9 +32,9 @@ ERROR_STATE_INVALIDATED = 10
 
 BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
 COLLISION_TRT, COLLISION_TRT_SYMMETRIC_LIMITER, COLLISION_REGULARIZED, COLLISION_RR3 = 0, 1, 2, 3
-WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF = 0, 1, 2
+WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF, WALL_INTERPOLATED_LINKS = 0, 1, 2, 3
 STORAGE_FP32, STORAGE_FP16_OFFSET, STORAGE_BF16_OFFSET = 0, 1, 2
-GEOMETRY_STATIC, GEOMETRY_MESH, GEOMETRY_SPHERE, GEOMETRY_BIRD = 0, 1, 2, 3
+GEOMETRY_STATIC, GEOMETRY_MESH, GEOMETRY_SPHERE, GEOMETRY_BIRD, GEOMETRY_WING = 0, 1, 2, 3, 4
 ARITHMETIC_DEFAULT, ARITHMETIC_STRICT, ARITHMETIC_FUSED = 0, 1, 2
 
 EXPORTED_FUNCTIONS = (
@@ -48,6 +48,7 @@ EXPORTED_FUNCTIONS = (
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
     "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity", "bf_mesh_freeze_masks",
     "bf_set_bird", "bf_set_bird_keyframes", "bf_read_bird_frame", "bf_ledger_fluid", "bf_ledger_step_sources",
+    "bf_set_flapping_wing", "bf_read_wing_frame", "bf_read_wing_links",
 )
 
 
@@ -120,6 +121,16 @@ class BirdGeometry(ctypes.Structure):
     ]
 
 
+WING_FIELDS = ("root_x", "root_y", "root_z", "chord_mean", "radius", "thickness", "beta_exponent",
+               "beta_normalization", "cycle", "stroke_amplitude", "stroke_duration", "pitch_duration", "pitch_low",
+               "pitch_high", "stroke_rate_max", "reference_speed")
+
+
+class FlappingWingC(ctypes.Structure):
+    """bf_flapping_wing (grid world units, time units, radians)."""
+    _fields_ = [(name, ctypes.c_float) for name in WING_FIELDS]
+
+
 _lib = None
 
 
@@ -185,6 +196,9 @@ def _declare(l):
         "bf_set_bird": (st, [vp, ctypes.POINTER(BirdGeometry)]),
         "bf_read_bird_frame": (st, [vp, vp]),
         "bf_set_bird_keyframes": (st, [vp, vp, u32]),
+        "bf_set_flapping_wing": (st, [vp, ctypes.POINTER(FlappingWingC)]),
+        "bf_read_wing_frame": (st, [vp, vp]),
+        "bf_read_wing_links": (st, [vp, vp, ctypes.c_uint64]),
         "bf_ledger_fluid": (st, [vp, vp]),
         "bf_ledger_step_sources": (st, [vp, vp, vp]),
     }

@@ -172,6 +172,52 @@ class BodyParameters:
 
 
 @dataclass
+class FlappingWing:
+    """GPUFlappingWingParameters (GPUData.swift:383-388) in grid world units
+    and time units: root, mean chord, span radius, thickness, beta exponent
+    (alpha - 1) and normalization, cycle, stroke amplitude, stroke and pitch
+    transition durations (cycle fractions), mid-stroke pitch (downstroke,
+    upstroke), maximum stroke rate (rad per cycle), reference speed."""
+    root: tuple
+    chord_mean: float
+    radius: float
+    thickness: float
+    beta_exponent: float
+    beta_normalization: float
+    cycle: float
+    stroke_amplitude: float
+    stroke_duration: float
+    pitch_duration: float
+    pitch_low: float
+    pitch_high: float
+    stroke_rate_max: float
+    reference_speed: float = 0.0
+
+    @classmethod
+    def li_nabawy(cls, chord_cells: float, root, cycle_steps: float, reference_speed: float = 0.0):
+        """The Li & Nabawy (2022) Re 100 case of MetalFlappingWingValidation.swift:528-569:
+        aspect ratio 3, thickness 0.05 c, beta shape alpha 1.48915 with the
+        normalization that makes the mean chord c, stroke half amplitude 80
+        deg, pitch 45 / 135 deg, acceleration and pitch durations of a
+        quarter cycle; the stroke rate's plateau solves stroke continuity."""
+        half = math.radians(80.0)
+        return cls(root=tuple(float(v) for v in root), chord_mean=chord_cells, radius=3.0 * chord_cells,
+                   thickness=0.05 * chord_cells, beta_exponent=1.4891506583557845 - 1.0,
+                   beta_normalization=2.497931737168364, cycle=cycle_steps, stroke_amplitude=half,
+                   stroke_duration=0.25, pitch_duration=0.25, pitch_low=math.radians(45.0),
+                   pitch_high=math.radians(135.0), stroke_rate_max=2.0 * half / (0.25 + 0.5 / math.pi),
+                   reference_speed=reference_speed)
+
+    def values(self):
+        return [*self.root, self.chord_mean, self.radius, self.thickness, self.beta_exponent,
+                self.beta_normalization, self.cycle, self.stroke_amplitude, self.stroke_duration,
+                self.pitch_duration, self.pitch_low, self.pitch_high, self.stroke_rate_max, self.reference_speed]
+
+    def to_c(self) -> _abi.FlappingWingC:
+        return _abi.FlappingWingC(*[float(v) for v in self.values()])
+
+
+@dataclass
 class BirdGeometry:
     """BirdParameters geometry and analytic wing kinematics
     (BirdModel.swift:129-217): metres, radians, Hz."""
@@ -394,6 +440,27 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_set_bird(self._ctx, ctypes.byref(cb)))
         self.geometry_source = {"kind": "bird", "bird": dataclasses.asdict(bird),
                                 "keyframes": None if kf is None else kf.copy()}
+
+    def set_flapping_wing(self, wing: "FlappingWing"):
+        """Prescribed beta-planform wing (BirdFlow.metal:940-1297) with its
+        link table; builds the current step.  Pair it with wall_rule
+        WALL_INTERPOLATED_LINKS."""
+        cw = wing.to_c()
+        self._check(self._lib.bf_set_flapping_wing(self._ctx, ctypes.byref(cw)))
+        self.geometry_source = {"kind": "wing", "wing": dataclasses.asdict(wing)}
+
+    def wing_frame(self) -> np.ndarray:
+        """Prepared frame of the current step, [6, 4] (GPUPreparedFlappingWing rows)."""
+        out = np.zeros((6, 4), np.float32)
+        self._check(self._lib.bf_read_wing_frame(self._ctx, _ptr(out)))
+        return out
+
+    def wing_links(self) -> np.ndarray:
+        """The current step's link table, [19, N]: the fluid-side fraction of
+        each fluid cell whose source is on the wing, 0 elsewhere."""
+        out = np.zeros(19 * self.n, np.float32)
+        self._check(self._lib.bf_read_wing_links(self._ctx, _ptr(out), out.size))
+        return out.reshape(19, -1)
 
     def bird_frame(self) -> np.ndarray:
         """Prepared frame of the current step, [14, 4] (GPUPreparedBirdGeometry rows)."""

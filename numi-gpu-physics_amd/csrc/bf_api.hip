@@ -135,6 +135,11 @@ struct bf_context {
     uint64_t f_writes = 0;                  // population writes other than steps
     uint64_t ledger_epoch = ~0ull;          // f_writes when the last step ran
     int64_t ledger_step = -1;               // and its index
+    // prescribed flapping wing: shape, prepared frames, link-fraction table
+    WingShape wing{};
+    float4* wing_frame = nullptr;         // [2][WING_FRAME_ROWS]
+    DeviceAlloc wing_alloc;
+    float* wing_links[2] = {nullptr, nullptr};   // [step parity][tiles][19][256]
     // analytic bird: shape and the prepared frame of each step parity
     BirdShape bird{};
     float4* bird_frame = nullptr;         // [2][BIRD_FRAME_ROWS]
@@ -443,6 +448,31 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         bp.velocity_scale = c->cfg.velocity_to_lattice;
         bp.tile_pass_blocks = 4u * c->tile_pass_blocks;
         e = launch_bird_geometry(c->bird, b, bp, c->sb);
+    } else if (c->source == BF_GEOMETRY_WING) {
+        WingBuffers b{};
+        b.frame = c->wing_frame + WING_FRAME_ROWS * par2(s);
+        b.mask_prev = fresh ? nullptr : c->mask[mod3(s - 1)];
+        b.dyn_prev = fresh ? nullptr : c->dyn[mod3(s - 1)];
+        b.tiles_new = c->tflags[mod4(s)];
+        b.tiles_mask_old = fresh ? c->no_tiles : c->tflags[mod4(s - 3)];
+        b.tiles_wall_old = fresh ? c->no_tiles : c->tflags[mod4(s - 2)];
+        b.list = c->geo_list;
+        b.list_count = &c->counters[2];
+        b.mask = c->mask[mod3(s)];
+        b.wall = c->wall[par2(s)];
+        b.dyn = c->dyn[mod3(s)];
+        b.links = c->wing_links[par2(s)];
+        BirdParams bp{};
+        bp.g = c->g;
+        for (int a = 0; a < 3; ++a) {
+            bp.origin[a] = c->cfg.origin[a];
+            bp.periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
+        }
+        bp.dx = c->cfg.cell_size;
+        bp.time = time_of(c, s);
+        bp.velocity_scale = c->cfg.velocity_to_lattice;
+        bp.tile_pass_blocks = 4u * c->tile_pass_blocks;
+        e = launch_wing_geometry(c->wing, b, bp, c->sb);
     } else if (c->source == BF_GEOMETRY_SPHERE) {
         int periodic[3];
         for (int a = 0; a < 3; ++a) periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
@@ -578,7 +608,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     const uint64_t n64 = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || n64 >= (1ull << 32)) return BF_ERROR_INVALID_CONFIGURATION;
     if (cfg->storage > BF_STORAGE_BF16_OFFSET || cfg->collision > BF_COLLISION_RR3 ||
-        cfg->wall_rule > BF_WALL_INTERPOLATED_HALF || cfg->force_mode > 6 || cfg->load_component > 2 ||
+        cfg->wall_rule > BF_WALL_INTERPOLATED_LINKS || cfg->force_mode > 6 || cfg->load_component > 2 ||
         cfg->arithmetic > BF_ARITHMETIC_FUSED)
         return BF_ERROR_INVALID_CONFIGURATION;
     if (cfg->arithmetic == BF_ARITHMETIC_STRICT && cfg->storage != BF_STORAGE_FP32)
@@ -746,6 +776,7 @@ void bf_destroy(bf_context* c) {
     c->mesh_alloc.release();
     c->ledger_alloc.release();
     c->keyframe_alloc.release();
+    c->wing_alloc.release();
     c->adj.alloc.release();
     c->adj.tangent_alloc.release();
     c->frozen_alloc.release();
@@ -761,6 +792,7 @@ uint64_t bf_step_index(const bf_context* c) { return c ? (uint64_t)c->step : 0; 
 static bf_status rebuild_mesh_geometry(bf_context* c);
 static bf_status rebuild_sphere_geometry(bf_context* c);
 static bf_status rebuild_bird_geometry(bf_context* c);
+static bf_status rebuild_wing_geometry(bf_context* c);
 
 bf_status bf_read_restart_state(bf_context* c, uint64_t* step, float* mesh_time_origin) {
     BF_ENTER(c);
@@ -790,6 +822,7 @@ bf_status bf_set_restart_state(bf_context* c, uint64_t step, float mesh_time_ori
     if (c->source == BF_GEOMETRY_MESH) r = rebuild_mesh_geometry(c);
     if (c->source == BF_GEOMETRY_SPHERE) r = rebuild_sphere_geometry(c);
     if (c->source == BF_GEOMETRY_BIRD) r = rebuild_bird_geometry(c);
+    if (c->source == BF_GEOMETRY_WING) r = rebuild_wing_geometry(c);
     if (r != BF_OK || !solid_mask) return r;
     // The saved mask is the previous-step mask of the next step.  It equals
     // the rebuilt one unless the geometry followed a free-flying body whose
@@ -1140,6 +1173,96 @@ bf_status bf_set_bird(bf_context* c, const bf_bird_geometry* bird) {
     return rebuild_bird_geometry(c);
 }
 
+// Wing geometry of the current step from cleared buffers, no previous
+// geometry; the previous step's mask starts equal to it.
+static bf_status rebuild_wing_geometry(bf_context* c) {
+    hipStream_t s = c->sb;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->mask[i], 0, c->g.n, s);
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_fill_u32((uint32_t*)c->wall[i], 0u, 4u * c->g.n, s);
+    for (int i = 0; i < 3 && e == hipSuccess; ++i) e = launch_fill_u8(c->dyn[i], 0, c->g.tiles, s);
+    for (int i = 0; i < 4 && e == hipSuccess; ++i) e = launch_fill_u8(c->tflags[i], 0, c->g.tiles, s);
+    if (e != hipSuccess) return hip_fail(c, e, "wing geometry clear");
+    const int64_t st = c->step;
+    if (bf_status r = enqueue_geometry(c, st, true, true)) return r;
+    const int k = mod3(st), kp = mod3(st - 1);
+    e = hipMemcpyAsync(c->mask[kp], c->mask[k], c->g.n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->dyn[kp], c->dyn[k], c->g.tiles, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->tflags[mod4(st - 1)], c->tflags[mod4(st)], c->g.tiles, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(c, e, "initial wing geometry");
+    c->geometry_ready = st;
+    return BF_OK;
+}
+
+bf_status bf_set_flapping_wing(bf_context* c, const bf_flapping_wing* wing) {
+    BF_ENTER(c);
+    if (!wing) return fail(c, BF_ERROR_INVALID_ARGUMENT, "wing parameters must not be NULL");
+    static_assert(sizeof(bf_flapping_wing) == sizeof(WingShape), "wing layouts differ");
+    const float* v = (const float*)wing;
+    for (size_t i = 0; i < sizeof(bf_flapping_wing) / sizeof(float); ++i)
+        if (!std::isfinite(v[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "wing parameters must be finite");
+    const bf_flapping_wing& w = *wing;
+    if (!(w.radius > 0.0f && w.chord_mean > 0.0f && w.thickness >= 0.0f && w.cycle > 0.0f &&
+          w.beta_normalization > 0.0f))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "wing span, mean chord, cycle and normalization must be positive");
+    if (!(w.stroke_duration > 0.0f && w.stroke_duration <= 0.5f && w.pitch_duration > 0.0f &&
+          w.pitch_duration <= 0.5f))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "stroke and pitch durations must lie in (0, 1/2] of a cycle");
+    if (!c->wing_links[0]) {
+        hipError_t e = hipSuccess;
+        const size_t entries = (size_t)c->g.tiles * 19 * TILE_CELLS;
+        if (entries > 0xFFFFFF00ull)
+            return fail(c, BF_ERROR_NOT_SUPPORTED, "the wing link table holds at most 2^32 - 256 entries");
+        c->wing_frame = (float4*)c->wing_alloc.take(e, 2ull * WING_FRAME_ROWS * sizeof(float4));
+        for (int k = 0; k < 2; ++k) c->wing_links[k] = (float*)c->wing_alloc.take(e, entries * sizeof(float));
+        for (int k = 0; k < 2 && e == hipSuccess; ++k)
+            e = launch_fill_u32((uint32_t*)c->wing_links[k], 0u, (uint32_t)entries, c->sb);
+        if (e != hipSuccess) {
+            c->wing_alloc.release();
+            c->wing_frame = nullptr;
+            c->wing_links[0] = c->wing_links[1] = nullptr;
+            return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("wing link table: ") + hipGetErrorString(e));
+        }
+    }
+    std::memcpy(&c->wing, wing, sizeof(WingShape));
+    c->source = BF_GEOMETRY_WING;
+    c->static_prev_differs = false;
+    c->time_start = 0.0f;
+    return rebuild_wing_geometry(c);
+}
+
+bf_status bf_read_wing_frame(bf_context* c, float frame[24]) {
+    BF_ENTER(c);
+    if (!frame) return fail(c, BF_ERROR_INVALID_ARGUMENT, "frame must not be NULL");
+    if (c->source != BF_GEOMETRY_WING) return fail(c, BF_ERROR_INVALID_ARGUMENT, "no flapping wing set");
+    hipError_t e = hipMemcpy(frame, c->wing_frame + WING_FRAME_ROWS * par2(c->step),
+                             WING_FRAME_ROWS * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read wing frame");
+    return BF_OK;
+}
+
+bf_status bf_read_wing_links(bf_context* c, float* links, uint64_t count) {
+    BF_ENTER(c);
+    if (!links) return fail(c, BF_ERROR_INVALID_ARGUMENT, "links must not be NULL");
+    if (c->source != BF_GEOMETRY_WING) return fail(c, BF_ERROR_INVALID_ARGUMENT, "no flapping wing set");
+    if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "link count must be 19 * N");
+    std::vector<float> t((size_t)c->g.tiles * 19 * TILE_CELLS);
+    hipError_t e = hipMemcpy(t.data(), c->wing_links[par2(c->step)], t.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read wing links");
+    const Grid& g = c->g;
+    for (int z = 0; z < g.nz; ++z)
+        for (int y = 0; y < g.ny; ++y)
+            for (int x = 0; x < g.nx; ++x) {
+                const size_t tile = ((size_t)z * g.tiles_y + (size_t)(y / TILE_Y)) * g.tiles_x + (size_t)(x / TILE_X);
+                const size_t lane = (size_t)(x % TILE_X) + (size_t)TILE_X * (size_t)(y % TILE_Y);
+                const size_t gid = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * (size_t)z);
+                for (int q = 0; q < 19; ++q) links[(size_t)q * g.n + gid] = t[(tile * 19 + q) * TILE_CELLS + lane];
+            }
+    return BF_OK;
+}
+
 bf_status bf_set_bird_keyframes(bf_context* c, const float* keyframes, uint32_t count) {
     BF_ENTER(c);
     if (count == 0) {
@@ -1261,6 +1384,8 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
     if (c->invalidated) return BF_ERROR_STATE_INVALIDATED;
     (void)hipSetDevice(c->device);
     if (steps == 0) return BF_OK;
+    if (c->cfg.wall_rule == BF_WALL_INTERPOLATED_LINKS && c->source != BF_GEOMETRY_WING)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "the link-table wall rule needs the prescribed flapping wing");
     const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision, arith = arithmetic_of(c->cfg);
     hipError_t e = hipSuccess;
     // Prologue: geometry (unless already built) and tile list of the first
@@ -1288,7 +1413,9 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
             }
             pe = &c->events[(size_t)c->profiled * PROFILE_EVENTS];
         }
-        const StepParams p = make_step_params(c);
+        StepParams p = make_step_params(c);
+        // the link table of step s; geometry(s + 1) fills the other slot meanwhile
+        p.links = c->wing_links[par2(s)];
         const FluidBuffers fb = fluid_buffers(c, s);
         c->ledger_prev = fb.solid_prev;
         if (c->source == BF_GEOMETRY_STATIC) c->static_prev_differs = false;
@@ -1522,7 +1649,8 @@ bf_status adjoint_supported(bf_context* c) {
             if (g.boundary[a][sd] == BF_BC_OUTLET)
                 return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers periodic and far-field faces");
     if (c->free_flight) return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint holds the body fixed (no free flight)");
-    if (c->source == BF_GEOMETRY_SPHERE || c->source == BF_GEOMETRY_BIRD)
+    if (c->source == BF_GEOMETRY_SPHERE || c->source == BF_GEOMETRY_BIRD || c->source == BF_GEOMETRY_WING ||
+        g.wall_rule == BF_WALL_INTERPOLATED_LINKS)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers mesh and static geometry");
     if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "step once after bf_set_geometry with a different previous mask");

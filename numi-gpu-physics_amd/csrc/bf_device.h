@@ -52,7 +52,7 @@ __host__ __device__ inline R equilibrium(int q, R rho, R ux, R uy, R uz) {
 // Boundary treatment per face.
 enum : int { BC_PERIODIC = 0, BC_FARFIELD = 1, BC_OUTLET = 2 };
 enum : int { COLL_TRT = 0, COLL_TRT_SYMMETRIC_LIMITER = 1, COLL_REGULARIZED = 2, COLL_RR3 = 3 };
-enum : int { WALL_HALFWAY = 0, WALL_INTERPOLATED_SDF = 1, WALL_INTERPOLATED_HALF = 2 };
+enum : int { WALL_HALFWAY = 0, WALL_INTERPOLATED_SDF = 1, WALL_INTERPOLATED_HALF = 2, WALL_INTERPOLATED_LINKS = 3 };
 enum : int { STORE_FP32 = 0, STORE_FP16_OFFSET = 1, STORE_BF16_OFFSET = 2 };
 enum : int { ARITH_FUSED = 0, ARITH_STRICT = 1 };
 
@@ -87,6 +87,7 @@ struct StepParams {
     float origin[3], dx;
     float force_scale;
     float far_eq[19];          // equilibrium(q, far_rho, far_u), FP32
+    const float* links;        // WALL_INTERPOLATED_LINKS: fractions [tile][19][256] of fluid cells' links
 };
 
 // XCD-aware block order: the hardware deals workgroups round-robin over the

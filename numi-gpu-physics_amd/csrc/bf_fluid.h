@@ -907,6 +907,11 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
                             float df = fmaxf(wg.w, 0.0f);
                             float dsv = fminf(ws.w, 0.0f);
                             lf = fminf(fmaxf(df / fmaxf(df - dsv, 1.0e-6f), 1.0e-4f), 1.0f);
+                        } else if (p.wall_rule == WALL_INTERPOLATED_LINKS) {
+                            // BirdFlow.metal:2738-2745: the stored link fraction
+                            const float v = p.links[((size_t)tile_of(g, x, y, z) * 19u + (size_t)q) * TILE_CELLS
+                                                    + threadIdx.x];
+                            lf = fminf(fmaxf(v, 1.0e-4f), 1.0f);
                         }
                         wvx = wg.x + (ws.x - wg.x) * lf;
                         wvy = wg.y + (ws.y - wg.y) * lf;

@@ -960,6 +960,297 @@ hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, co
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Prescribed beta-planform flapping wing.  preparePrescribedFlappingWing
+// (BirdFlow.metal:1040-1082) in one lane; buildPrescribedFlappingWing
+// (:1178-1245) on the tiles within the guard sphere (and near last step's
+// solid cells); then the link fractions (:1146-1174, :1271-1296) of every
+// fluid cell with a solid source, into a table indexed by tile, direction
+// and lane (the reference keeps them in the solid cells' dormant population
+// slots, which the next step's geometry may not touch here while the fluid
+// step runs).  Conventions shared with the oracle: fmod(a, 1) = a - trunc(a),
+// pow(x > 0, y) = exp2(y log2 x) with the log2 / exp2 below, pow(0, y) = 0,
+// mix(a, b, t) = a + (b - a) t, sin / cos by bird_sincos.
+namespace {
+
+__device__ inline float wing_log2(float x) {
+    int k;
+    float m = frexpf(x, &k);
+    if (m < 0.70710678f) {
+        m = m * 2.0f;
+        k = k - 1;
+    }
+    const float t = (m - 1.0f) / (m + 1.0f);
+    const float t2 = t * t;
+    float s = __builtin_fmaf(t2, 1.0f / 11.0f, 1.0f / 9.0f);
+    s = __builtin_fmaf(s, t2, 1.0f / 7.0f);
+    s = __builtin_fmaf(s, t2, 1.0f / 5.0f);
+    s = __builtin_fmaf(s, t2, 1.0f / 3.0f);
+    s = __builtin_fmaf(s, t2, 1.0f);
+    const float ln_m = 2.0f * t * s;
+    return (float)k + ln_m * 1.44269504f;
+}
+
+__device__ inline float wing_exp2(float v) {
+    const float n = __builtin_rintf(v);
+    const float f = (v - n) * 0.693147181f;
+    float e = __builtin_fmaf(f, 1.0f / 40320.0f, 1.0f / 5040.0f);
+    e = __builtin_fmaf(e, f, 1.0f / 720.0f);
+    e = __builtin_fmaf(e, f, 1.0f / 120.0f);
+    e = __builtin_fmaf(e, f, 1.0f / 24.0f);
+    e = __builtin_fmaf(e, f, 1.0f / 6.0f);
+    e = __builtin_fmaf(e, f, 0.5f);
+    e = __builtin_fmaf(e, f, 1.0f);
+    e = __builtin_fmaf(e, f, 1.0f);
+    return ldexpf(e, (int)n);
+}
+
+__device__ inline float wing_pow(float x, float y) {
+    if (!(x > 0.0f)) return 0.0f;
+    return wing_exp2(y * wing_log2(x));
+}
+
+__device__ inline void wing_stroke(float phase, const WingShape& w, float& angle, float& rate) {
+    const float pi = 3.14159265358979323846f;
+    const float amplitude = w.stroke_amplitude, duration = w.stroke_duration;
+    const float half = 0.5f * duration;
+    const float rmax = w.stroke_rate_max;
+    float s, c;
+    if (phase < half) {
+        bird_sincos(pi * (phase + half) / duration, s, c);
+        angle = amplitude + rmax * duration / pi * (s - 1.0f);
+        rate = rmax * c;
+    } else if (phase < 0.5f - half) {
+        const float end = amplitude - rmax * duration / pi;
+        angle = end - rmax * (phase - half);
+        rate = -rmax;
+    } else if (phase < 0.5f + half) {
+        const float start = 0.5f - half;
+        const float at = -amplitude + rmax * duration / pi;
+        bird_sincos(pi * (phase - start) / duration, s, c);
+        angle = at - rmax * duration / pi * s;
+        rate = -rmax * c;
+    } else if (phase < 1.0f - half) {
+        const float end = -amplitude + rmax * duration / pi;
+        angle = end + rmax * (phase - (0.5f + half));
+        rate = rmax;
+    } else {
+        const float start = 1.0f - half;
+        const float at = amplitude - rmax * duration / pi;
+        bird_sincos(pi * (phase - start) / duration, s, c);
+        angle = at + rmax * duration / pi * s;
+        rate = rmax * c;
+    }
+}
+
+__device__ inline void wing_pitch(float phase, const WingShape& w, float& angle, float& rate) {
+    const float pi = 3.14159265358979323846f;
+    const float duration = w.pitch_duration;
+    const float half = 0.5f * duration;
+    const float low = w.pitch_low, high = w.pitch_high;
+    const float delta = high - low;
+    float x, start, change;
+    if (phase < half || phase >= 1.0f - half) {
+        const float wrapped = phase < half ? phase + 1.0f : phase;
+        x = (wrapped - (1.0f - half)) / duration;
+        start = high;
+        change = -delta;
+    } else if (phase >= 0.5f - half && phase < 0.5f + half) {
+        x = (phase - (0.5f - half)) / duration;
+        start = low;
+        change = delta;
+    } else {
+        angle = phase < 0.5f ? low : high;
+        rate = 0.0f;
+        return;
+    }
+    float s, c;
+    bird_sincos(2.0f * pi * x, s, c);
+    const float blend = x - s / (2.0f * pi);
+    rate = change / duration * (1.0f - c);
+    angle = start + change * blend;
+}
+
+// largest violation of the planform's bounds (<= 0 inside)
+__device__ inline float wing_violation(const WingShape& w, const float4* f, float dx, V3 world) {
+    const V3 rel = vsub(world, row3(f[0]));
+    const float radial = vdot(rel, row3(f[2]));
+    const float normal = vdot(rel, row3(f[3]));
+    const float chord = vdot(rel, row3(f[1]));
+    const float radius = w.radius;
+    const float chord_mean = f[0].w;
+    const float half = 0.5f * smax(w.thickness, dx);
+    const float rf = clamp01(radial / radius);
+    const float base = smax(rf * (1.0f - rf), 0.0f);
+    const float local = chord_mean * w.beta_normalization * wing_pow(base, w.beta_exponent);
+    const float lead = -0.25f * chord_mean;
+    const float v_root = -radial, v_tip = radial - radius;
+    const float v_normal = fabsf(normal) - half;
+    const float v_lead = lead - chord, v_trail = chord - (lead + local);
+    return smax(smax(v_root, v_tip), smax(v_normal, smax(v_lead, v_trail)));
+}
+
+}  // namespace
+
+__global__ void wing_prepare(WingShape w, float time, float4* __restrict__ frame) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const float a = time / w.cycle;
+    float phase = a - truncf(a);
+    phase = phase < 0.0f ? phase + 1.0f : phase;
+    float sa, sr, pa, pr;
+    wing_stroke(phase, w, sa, sr);
+    wing_pitch(phase, w, pa, pr);
+    float ss, sc;
+    bird_sincos(sa, ss, sc);
+    const V3 axis = v3(0.0f, 0.0f, 1.0f);
+    const V3 span = v3(sc, ss, 0.0f);
+    const V3 tangent = vcross(axis, span);
+    const V3 chord = rotate_axis(tangent, span, -pa);
+    const V3 normal = vnormalize(vcross(span, chord));
+    const float inv = 1.0f / w.cycle;
+    const V3 omega = vsub(vmul(axis, sr * inv), vmul(span, pr * inv));
+    frame[0] = make_float4(w.root[0], w.root[1], w.root[2], w.chord_mean);
+    frame[1] = row4(vnormalize(chord), 0.0f);
+    frame[2] = row4(vnormalize(span), 0.0f);
+    frame[3] = row4(normal, 0.0f);
+    frame[4] = row4(omega, 0.0f);
+    frame[5] = make_float4(sa, pa, sr * inv, pr * inv);
+}
+
+// Tiles evaluated this step: within two cells of the guard sphere (radius
+// span + 1.5 mean chords + 2 cells), or near last step's solid cells.
+__global__ __launch_bounds__(256) void wing_mark_tiles(WingShape w, const float4* __restrict__ frame,
+                                                       const uint8_t* __restrict__ dyn_prev,
+                                                       uint8_t* __restrict__ tiles_new, BirdParams p) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= p.g.tiles) return;
+    int x0, y0, z;
+    tile_origin(p.g, t, x0, y0, z);
+    const int x1 = min(x0 + TILE_X, p.g.nx) - 1, y1 = min(y0 + TILE_Y, p.g.ny) - 1;
+    const float lx = p.origin[0] + ((float)x0 + 0.5f) * p.dx, hx = p.origin[0] + ((float)x1 + 0.5f) * p.dx;
+    const float ly = p.origin[1] + ((float)y0 + 0.5f) * p.dx, hy = p.origin[1] + ((float)y1 + 0.5f) * p.dx;
+    const float zc = p.origin[2] + ((float)z + 0.5f) * p.dx;
+    const float4 c = frame[0];
+    const float r = w.radius + 1.5f * c.w + 2.0f * p.dx + 2.0f * p.dx;
+    const bool hit = tile_box_dist2(lx, hx, ly, hy, zc, c.x, c.y, c.z) <= r * r;
+    tiles_new[t] = (hit || (dyn_prev && dyn_prev[t])) ? 1 : 0;
+}
+
+// buildPrescribedFlappingWing on the marked tiles (every cell the
+// reference's value); other listed tiles cleared to the reference's far
+// value (mask 0, wall (0, 0, 0, radius)).
+__global__ __launch_bounds__(256) void wing_tile_pass(WingShape w, const float4* __restrict__ frame,
+                                                      const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count,
+                                                      const uint8_t* __restrict__ tiles_new,
+                                                      const uint8_t* __restrict__ mask_prev,
+                                                      uint8_t* __restrict__ mask, float4* __restrict__ wall,
+                                                      uint8_t* __restrict__ dyn, BirdParams p) {
+    __shared__ float4 fr[WING_FRAME_ROWS];
+    if (threadIdx.x < WING_FRAME_ROWS) fr[threadIdx.x] = frame[threadIdx.x];
+    __syncthreads();
+    const V3 root = row3(fr[0]);
+    const V3 omega = row3(fr[4]);
+    const float chord_mean = fr[0].w;
+    const float radius = w.radius + 1.5f * chord_mean + 2.0f * p.dx;
+    const uint32_t n_tiles = *count;
+    for (uint32_t i = blockIdx.x; i < n_tiles; i += gridDim.x) {
+        const uint32_t tile = list[i];
+        int x0, y0, z;
+        tile_origin(p.g, tile, x0, y0, z);
+        const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+        const int y = y0 + (int)(threadIdx.x / TILE_X);
+        if (x >= p.g.nx || y >= p.g.ny) continue;
+        const uint32_t gid = (uint32_t)x + (uint32_t)p.g.nx * ((uint32_t)y + (uint32_t)p.g.ny * (uint32_t)z);
+        const V3 world = v3(p.origin[0] + ((float)x + 0.5f) * p.dx, p.origin[1] + ((float)y + 0.5f) * p.dx,
+                            p.origin[2] + ((float)z + 0.5f) * p.dx);
+        const V3 rel = vsub(world, root);
+        const bool was_solid = mask_prev && mask_prev[gid] != 0;
+        if (!tiles_new[tile] || (!was_solid && vdot(rel, rel) > radius * radius)) {
+            mask[gid] = 0;
+            wall[gid] = make_float4(0.0f, 0.0f, 0.0f, radius);
+            continue;
+        }
+        const float cc = vdot(rel, row3(fr[1]));
+        const float rc = vdot(rel, row3(fr[2]));
+        const float nc = vdot(rel, row3(fr[3]));
+        const float half = 0.5f * smax(w.thickness, p.dx);
+        const bool in_radial = rc >= 0.0f && rc <= w.radius;
+        const bool in_normal = fabsf(nc) <= half;
+        bool is_wing = false;
+        if (in_radial && in_normal) {
+            const float rf = clamp01(rc / w.radius);
+            const float base = smax(rf * (1.0f - rf), 0.0f);
+            const float local = chord_mean * w.beta_normalization * wing_pow(base, w.beta_exponent);
+            const float lead = -0.25f * chord_mean;
+            const float trail = lead + local;
+            is_wing = cc >= lead && cc <= trail;
+        }
+        const V3 v = vcross(omega, rel);
+        mask[gid] = is_wing ? 1 : 0;
+        wall[gid] = make_float4(v.x * p.velocity_scale, v.y * p.velocity_scale, v.z * p.velocity_scale,
+                                is_wing ? -1.0f : 1.0f);
+        if (is_wing) mark_tiles(p.g, p.periodic, dyn, x, y, z);
+    }
+}
+
+// Link fractions of the fluid cells of the tiles near the wing: for each
+// direction whose (in-domain) source is solid, the bisection from the
+// source's centre plus c_q dx (the reference's fluid point) to the source.
+__global__ __launch_bounds__(256) void wing_links(WingShape w, const float4* __restrict__ frame,
+                                                  const uint8_t* __restrict__ dyn, const uint8_t* __restrict__ mask,
+                                                  float* __restrict__ links, BirdParams p) {
+    __shared__ float4 fr[WING_FRAME_ROWS];
+    if (threadIdx.x < WING_FRAME_ROWS) fr[threadIdx.x] = frame[threadIdx.x];
+    __syncthreads();
+    for (uint32_t tile = blockIdx.x; tile < p.g.tiles; tile += gridDim.x) {
+        if (!dyn[tile]) continue;
+        int x0, y0, z;
+        tile_origin(p.g, tile, x0, y0, z);
+        const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
+        const int y = y0 + (int)(threadIdx.x / TILE_X);
+        if (x >= p.g.nx || y >= p.g.ny) continue;
+        const uint32_t gid = (uint32_t)x + (uint32_t)p.g.nx * ((uint32_t)y + (uint32_t)p.g.ny * (uint32_t)z);
+        if (mask[gid] != 0) continue;
+        float* out = links + (size_t)tile * 19 * TILE_CELLS + threadIdx.x;
+        for (int q = 1; q < 19; ++q) {
+            const int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
+            if (sx < 0 || sy < 0 || sz < 0 || sx >= p.g.nx || sy >= p.g.ny || sz >= p.g.nz) continue;
+            const uint32_t src = (uint32_t)sx + (uint32_t)p.g.nx * ((uint32_t)sy + (uint32_t)p.g.ny * (uint32_t)sz);
+            if (mask[src] == 0) continue;
+            const V3 solid = v3(p.origin[0] + ((float)sx + 0.5f) * p.dx, p.origin[1] + ((float)sy + 0.5f) * p.dx,
+                                p.origin[2] + ((float)sz + 0.5f) * p.dx);
+            const V3 fluid = v3(solid.x + (float)cx(q) * p.dx, solid.y + (float)cy(q) * p.dx,
+                                solid.z + (float)cz(q) * p.dx);
+            float lower = 0.0f, upper = 1.0f;
+            for (int it = 0; it < 10; ++it) {
+                const float mid = 0.5f * (lower + upper);
+                const V3 sample = v3(fluid.x + (solid.x - fluid.x) * mid, fluid.y + (solid.y - fluid.y) * mid,
+                                     fluid.z + (solid.z - fluid.z) * mid);
+                if (wing_violation(w, fr, p.dx, sample) > 0.0f) lower = mid;
+                else upper = mid;
+            }
+            out[(size_t)q * TILE_CELLS] = smin(smax(0.5f * (lower + upper), 1.0e-4f), 1.0f);
+        }
+    }
+}
+
+hipError_t launch_wing_geometry(const WingShape& shape, const WingBuffers& b, const BirdParams& p, hipStream_t s) {
+    const unsigned tile_blocks = (p.g.tiles + 255) / 256;
+    hipLaunchKernelGGL(wing_prepare, dim3(1), dim3(64), 0, s, shape, p.time, b.frame);
+    hipLaunchKernelGGL(wing_mark_tiles, dim3(tile_blocks), dim3(256), 0, s, shape, (const float4*)b.frame, b.dyn_prev,
+                       b.tiles_new, p);
+    hipLaunchKernelGGL(fill_u8, dim3(tile_blocks), dim3(256), 0, s, b.dyn, (uint8_t)0, p.g.tiles);
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
+    hipLaunchKernelGGL(geometry_tile_list, dim3(tile_blocks), dim3(256), 0, s, b.tiles_new, b.tiles_mask_old,
+                       b.tiles_wall_old, p.g.tiles, b.list, b.list_count);
+    hipLaunchKernelGGL(wing_tile_pass, dim3(p.tile_pass_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
+                       b.list, b.list_count, (const uint8_t*)b.tiles_new, b.mask_prev, b.mask, b.wall, b.dyn, p);
+    hipLaunchKernelGGL(wing_links, dim3(p.tile_pass_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
+                       (const uint8_t*)b.dyn, (const uint8_t*)b.mask, b.links, p);
+    return hipGetLastError();
+}
+
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
     if (m.g.nx > 65535 || m.g.ny > 65535) return hipErrorInvalidValue;   // raster packs x | y << 16
     if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1 || m.raster_split < 1 || m.raster_split > 64)

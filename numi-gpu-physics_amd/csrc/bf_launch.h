@@ -90,6 +90,31 @@ struct BirdBuffers {
     uint8_t* dyn;
 };
 
+// Prescribed beta-planform flapping wing (BirdFlow.metal:940-1297), as
+// GPUFlappingWingParameters holds it; same field order as bf_flapping_wing.
+struct WingShape {
+    float root[3], chord_mean;
+    float radius, thickness, beta_exponent, beta_normalization;
+    float cycle, stroke_amplitude, stroke_duration, pitch_duration;
+    float pitch_low, pitch_high, stroke_rate_max, reference_speed;
+};
+constexpr int WING_FRAME_ROWS = 6;   // GPUPreparedFlappingWing: 6 float4
+
+struct WingBuffers {
+    float4* frame;                    // prepared frame of this step (written)
+    const uint8_t* mask_prev;         // previous step's mask, or null: no previous geometry
+    const uint8_t* dyn_prev;
+    uint8_t* tiles_new;
+    const uint8_t* tiles_mask_old;
+    const uint8_t* tiles_wall_old;
+    uint32_t* list;
+    uint32_t* list_count;
+    uint8_t* mask;
+    float4* wall;
+    uint8_t* dyn;
+    float* links;                     // [tile][19][256] link fractions of fluid cells with a solid source
+};
+
 struct MeshBuffers {
     const double* body;                 // body state, read when MeshParams::move_with_body
     const float* points;
@@ -211,6 +236,7 @@ struct LedgerBuffers {
 };
 hipError_t launch_ledger(int storage, const LedgerBuffers& b, const StepParams& p, bool sources, hipStream_t s);
 hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s);
+hipError_t launch_wing_geometry(const WingShape& shape, const WingBuffers& b, const BirdParams& p, hipStream_t s);
 hipError_t launch_sphere_geometry(uint8_t* mask, float4* wall, uint8_t* dyn, const Grid& g, float4 cr, float3 gv,
                                   float4 wv, float t, const int periodic[3], hipStream_t s);
 hipError_t launch_mark_tiles(const uint8_t* mask, uint8_t* dyn, const Grid& g, const int periodic[3], bool clear,

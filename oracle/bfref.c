@@ -33,6 +33,10 @@ const double bfo_w[19] = {
     1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
 };
 
+/* link fractions of BFO_WALL_INTERPOLATED_LINKS steps ([19][N], fluid side) */
+static const float* bfo_link_table = NULL;
+void bfo_set_link_table(const float* links) { bfo_link_table = links; }
+
 #define R float
 #define WT float
 #define FN(name) name##_f32
@@ -502,3 +506,4 @@ void bfo_place_vertices(const bfo_body* b, int vertex_count, float velocity_scal
 }
 
 #include "bfref_bird.inc"
+#include "bfref_wing.inc"

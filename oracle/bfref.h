@@ -25,7 +25,8 @@
  *   - translating-sphere topology canonical  BirdFlow.metal:183-195,2263-2350
  *   - six-degree-of-freedom body update  BirdFlow.metal:6574-6627,
  *   - runtime safety monitor             BirdFlow.metal:6629-6707,
- *   - analytic signed-distance bird      BirdFlow.metal:486-533,611-938
+ *   - analytic signed-distance bird      BirdFlow.metal:486-533,550-938
+ *   - prescribed beta-planform wing      BirdFlow.metal:940-1297
  *       Sources/BirdFlowCore/RigidBody.swift:85-127
  *
  * Fluid functions exist in two precisions: *_f32 (same operation order as the
@@ -62,7 +63,8 @@ enum {
 enum {
     BFO_WALL_HALFWAY = 0,            /* halfway bounce-back, wall velocity of the solid source */
     BFO_WALL_INTERPOLATED_SDF = 1,   /* Bouzidi with q from the signed-distance ratio */
-    BFO_WALL_INTERPOLATED_HALF = 2   /* Bouzidi path with q fixed at 1/2 */
+    BFO_WALL_INTERPOLATED_HALF = 2,  /* Bouzidi path with q fixed at 1/2 */
+    BFO_WALL_INTERPOLATED_LINKS = 3  /* Bouzidi with q from a link table (bfo_set_link_table) */
 };
 
 typedef struct bfo_params {
@@ -110,6 +112,10 @@ void bfo_step_f64w(const bfo_params* p, const double* fin, double* fout,
                    const uint8_t* solid_prev, const uint8_t* solid_cur,
                    const double* wall4, double* density, double* velocity4,
                    double* loads);
+
+/* Link fractions read by BFO_WALL_INTERPOLATED_LINKS steps: [19][N], entry
+ * q * N + x for fluid x whose source x - c_q is solid (NULL: none). */
+void bfo_set_link_table(const float* links);
 
 /* Equilibrium fill: solid cells at their wall velocity, fluid at far_u. */
 void bfo_init_equilibrium_f32(const bfo_params* p, float* f, const uint8_t* solid, const float* wall4);
@@ -222,6 +228,42 @@ void bfo_bird_prepare(const bfo_bird* b, const float body_position[3], const flo
  * velocity_to_lattice, 0).  solid_prev NULL: no previous geometry. */
 void bfo_bird_build(const bfo_params* p, const bfo_bird* b, const bfo_bird_frame* f, const uint8_t* solid_prev,
                     float velocity_to_lattice, uint8_t* mask, float* wall4);
+
+/* ---- prescribed beta-planform flapping wing (FP32) ---- */
+/* GPUFlappingWingParameters (GPUData.swift:383-388): root position and mean
+ * chord; span (radius), thickness, beta exponent (shape - 1) and chord
+ * normalization; cycle length, stroke half amplitude, stroke acceleration
+ * and pitch durations (cycle fractions); pitch low / high angles, maximum
+ * stroke rate (per cycle), reference speed.  World units and time units as
+ * the grid's (the reference runs it in cells and steps). */
+typedef struct bfo_wing {
+    float root[3], chord_mean;
+    float radius, thickness, beta_exponent, beta_normalization;
+    float cycle, stroke_amplitude, stroke_duration, pitch_duration;
+    float pitch_low, pitch_high, stroke_rate_max, reference_speed;
+} bfo_wing;
+/* GPUPreparedFlappingWing: root (w: mean chord), chord, span, normal,
+ * angular velocity, state (stroke, pitch, their rates per time unit). */
+typedef struct bfo_wing_frame {
+    float root[4], chord[4], span[4], normal[4], angular[4], state[4];
+} bfo_wing_frame;
+
+float bfo_wing_pow(float x, float y);
+/* preparePrescribedFlappingWing (BirdFlow.metal:1040-1082) at `time`. */
+void bfo_wing_prepare(const bfo_wing* w, float time, bfo_wing_frame* out);
+/* prescribedWingLinkFraction (BirdFlow.metal:1146-1174): bisection from the
+ * fluid to the solid point, fraction measured from the fluid side. */
+float bfo_wing_link_fraction(const bfo_wing* w, const bfo_wing_frame* f, float dx, const float fluid[3],
+                             const float solid[3]);
+/* buildPrescribedFlappingWing (BirdFlow.metal:1178-1245): mask 1 on the
+ * wing, wall = (omega x r * velocity_to_lattice, -1 on the wing, +1 near
+ * it); (0, 0, 0, radius) beyond the guard sphere.  solid_prev NULL: none. */
+void bfo_wing_build(const bfo_params* p, const bfo_wing* w, const bfo_wing_frame* f, const uint8_t* solid_prev,
+                    float velocity_to_lattice, uint8_t* mask, float* wall4);
+/* The link table of the step (BirdFlow.metal:1271-1296), fluid side:
+ * links[q * N + x] for fluid x whose source x - c_q is solid; 0 elsewhere. */
+void bfo_wing_links(const bfo_params* p, const bfo_wing* w, const bfo_wing_frame* f, const uint8_t* mask,
+                    float* links);
 
 #ifdef __cplusplus
 }

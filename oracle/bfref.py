@@ -20,7 +20,7 @@ _LIB_PATH = os.path.join(_HERE, "libbfref.so")
 
 BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
 COLLISION_TRT, COLLISION_TRT_SYMMETRIC_LIMITER, COLLISION_REGULARIZED, COLLISION_RR3 = 0, 1, 2, 3
-WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF = 0, 1, 2
+WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF, WALL_INTERPOLATED_LINKS = 0, 1, 2, 3
 
 C = np.array(
     [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
@@ -71,6 +71,17 @@ class Bird(ctypes.Structure):
         ("frequency", ctypes.c_float), ("stroke_amplitude", ctypes.c_float), ("stroke_bias", ctypes.c_float),
         ("pitch_mean", ctypes.c_float), ("pitch_amplitude", ctypes.c_float), ("pitch_phase", ctypes.c_float),
     ]
+
+
+class Wing(ctypes.Structure):
+    """bfo_wing: GPUFlappingWingParameters."""
+    _fields_ = [(name, ctypes.c_float) for name in (
+        "root_x", "root_y", "root_z", "chord_mean", "radius", "thickness", "beta_exponent", "beta_normalization",
+        "cycle", "stroke_amplitude", "stroke_duration", "pitch_duration", "pitch_low", "pitch_high",
+        "stroke_rate_max", "reference_speed")]
+
+
+WING_FRAME_FIELDS = ("root", "chord", "span", "normal", "angular", "state")
 
 
 BIRD_FRAME_FIELDS = ("body_position", "orientation", "linear_velocity", "omega_world_radius",
@@ -136,6 +147,14 @@ def _declare(l):
     l.bfo_body_safety.restype = ctypes.c_uint32
     l.bfo_place_vertices.argtypes = [ctypes.POINTER(Body), ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
     l.bfo_sincos_n.argtypes = [vp, ctypes.c_int, vp, vp]
+    l.bfo_set_link_table.argtypes = [vp]
+    l.bfo_wing_pow.argtypes = [ctypes.c_float, ctypes.c_float]
+    l.bfo_wing_pow.restype = ctypes.c_float
+    l.bfo_wing_prepare.argtypes = [ctypes.POINTER(Wing), ctypes.c_float, vp]
+    l.bfo_wing_link_fraction.argtypes = [ctypes.POINTER(Wing), vp, ctypes.c_float, vp, vp]
+    l.bfo_wing_link_fraction.restype = ctypes.c_float
+    l.bfo_wing_build.argtypes = [P, ctypes.POINTER(Wing), vp, vp, ctypes.c_float, vp, vp]
+    l.bfo_wing_links.argtypes = [P, ctypes.POINTER(Wing), vp, vp, vp]
     l.bfo_bird_prepare.argtypes = [ctypes.POINTER(Bird), vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp,
                                    ctypes.c_int, vp]
     l.bfo_bird_build.argtypes = [P, ctypes.POINTER(Bird), vp, vp, ctypes.c_float, vp, vp]
@@ -574,3 +593,75 @@ def ledger_sources(p: Params, f_in, f_out, solid_prev, solid_cur, wall4):
 def bfref_far_eq(p: Params, q):
     """FP32 far-field equilibrium of direction q (the value the step uses)."""
     return float(equilibrium_f32(q, float(np.float32(p.far_rho)), tuple(float(v) for v in p.far_u)))
+
+
+# ---------------------------------------------------------------------------
+# prescribed beta-planform flapping wing
+# ---------------------------------------------------------------------------
+def make_wing(**kw) -> Wing:
+    w = Wing()
+    for k, v in kw.items():
+        if k == "root":
+            w.root_x, w.root_y, w.root_z = (float(c) for c in v)
+        else:
+            setattr(w, k, float(v))
+    return w
+
+
+def li_nabawy_wing(chord_cells, root, cycle_steps, reference_speed=0.0):
+    """The Li & Nabawy (2022) Re = 100 beta planform of the reference's
+    prescribed-wing case (MetalFlappingWingValidation.swift:528-569,
+    3449-3470): AR 3, thickness 0.05 c, beta shape 1.48915, stroke half
+    amplitude 80 deg, pitch 45 / 135 deg, acceleration and pitch durations
+    0.25 of a cycle, in cells and steps."""
+    import math as _m
+    half = 80.0 * _m.pi / 180.0
+    rmax = 2 * half / (0.25 + 2 * 0.25 / _m.pi)
+    return make_wing(root=root, chord_mean=chord_cells, radius=3.0 * chord_cells, thickness=0.05 * chord_cells,
+                     beta_exponent=1.4891506583557845 - 1.0, beta_normalization=2.497931737168364,
+                     cycle=cycle_steps, stroke_amplitude=half, stroke_duration=0.25, pitch_duration=0.25,
+                     pitch_low=45.0 * _m.pi / 180.0, pitch_high=135.0 * _m.pi / 180.0, stroke_rate_max=rmax,
+                     reference_speed=reference_speed)
+
+
+def wing_prepare(wing: Wing, time) -> np.ndarray:
+    out = np.zeros((6, 4), np.float32)
+    lib().bfo_wing_prepare(ctypes.byref(wing), np.float32(time), _ptr(out))
+    return out
+
+
+def wing_link_fraction(wing: Wing, frame, dx, fluid, solid):
+    fr = np.ascontiguousarray(frame, np.float32)
+    a = np.ascontiguousarray(fluid, np.float32)
+    b = np.ascontiguousarray(solid, np.float32)
+    return float(lib().bfo_wing_link_fraction(ctypes.byref(wing), _ptr(fr), np.float32(dx), _ptr(a), _ptr(b)))
+
+
+def wing_build(p: Params, wing: Wing, frame, solid_prev=None, velocity_to_lattice=1.0):
+    n = cells(p)
+    mask = np.zeros(n, np.uint8)
+    wall4 = np.zeros((n, 4), np.float32)
+    fr = np.ascontiguousarray(frame, np.float32)
+    prev = None if solid_prev is None else np.ascontiguousarray(solid_prev, np.uint8)
+    lib().bfo_wing_build(ctypes.byref(p), ctypes.byref(wing), _ptr(fr), _ptr(prev), np.float32(velocity_to_lattice),
+                         _ptr(mask), _ptr(wall4))
+    return mask, wall4
+
+
+def wing_links(p: Params, wing: Wing, frame, mask):
+    n = cells(p)
+    links = np.zeros(19 * n, np.float32)
+    fr = np.ascontiguousarray(frame, np.float32)
+    m = np.ascontiguousarray(mask, np.uint8)
+    lib().bfo_wing_links(ctypes.byref(p), ctypes.byref(wing), _ptr(fr), _ptr(m), _ptr(links))
+    return links
+
+
+def step_with_links(p: Params, fin, solid_prev, solid_cur, wall4, links, capture=False):
+    """One step with the BFO_WALL_INTERPOLATED_LINKS rule and this link table."""
+    tbl = np.ascontiguousarray(links, np.float32)
+    lib().bfo_set_link_table(_ptr(tbl))
+    try:
+        return step(p, fin, solid_prev, solid_cur, wall4, capture)
+    finally:
+        lib().bfo_set_link_table(None)

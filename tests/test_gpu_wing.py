@@ -1,0 +1,149 @@
+"""GPU parity of the prescribed beta-planform flapping wing
+(BirdFlow.metal:940-1297, driven by MetalFlappingWingValidation.swift) and
+of the link-table wall rule (BirdFlow.metal:2738-2745) against the FP32
+oracle, on the Li & Nabawy planform (chord 8 cells, span 24 cells, cycle
+200 time units, 2 time units per step, so the tip moves about 1.6 cells a
+step and cells change sides every step):
+
+* every step's prepared frame and mask are bit-identical to the oracle's,
+  and so is the wall wherever the oracle evaluates the wing (inside its
+  guard sphere; beyond it no step reads the wall);
+* the link table holds the oracle's bisected fraction, bit for bit, for
+  every fluid cell whose source is on the wing;
+* fluid steps with the link rule: populations bit-identical, loads to 1e-12
+  relative (FP64 sums of the same FP32 terms in another order);
+* the link rule without the wing, and the adjoint of a wing, are refused.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import bfref
+import birdflow_hip as bh
+import cases
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (64, 64, 32)
+ROOT = (32.0, 32.0, 16.0)
+CYCLE = 200.0
+TIME_STEP = 2.0
+VELOCITY_TO_LATTICE = 0.05
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def geometry_time(s):
+    return np.float32(np.float64(s) * np.float64(np.float32(TIME_STEP)))
+
+
+def wing_case():
+    p = bfref.make_params(SHAPE, omega_plus=1.6, omega_minus=1.0 / (0.1875 / (1.0 / 1.6 - 0.5) + 0.5),
+                          bc=((bfref.BC_FARFIELD, bfref.BC_FARFIELD),) * 3, collision=bfref.COLLISION_TRT,
+                          wall_rule=bfref.WALL_INTERPOLATED_LINKS, force_mode=6, accumulate_loads=True,
+                          sponge_strength=0.04, sponge_width=4.0, far_u=(0.01, 0.0, 0.0), origin=(0.0, 0.0, 0.0),
+                          dx=1.0, force_scale=1.0, body_position=ROOT)
+    cfg = cases.solver_config(p, time_step=TIME_STEP, velocity_to_lattice=VELOCITY_TO_LATTICE)
+    return p, cfg, bh.FlappingWing.li_nabawy(8.0, ROOT, CYCLE), bfref.li_nabawy_wing(8.0, ROOT, CYCLE)
+
+
+def oracle_geometry(p, ow, s, prev):
+    fr = bfref.wing_prepare(ow, geometry_time(s))
+    m, w = bfref.wing_build(p, ow, fr, prev, VELOCITY_TO_LATTICE)
+    return fr, m, w
+
+
+def source_is_solid(mask, q):
+    """Fluid cells whose in-domain source cell - c_q is solid, [N] bool."""
+    nx, ny, nz = SHAPE
+    m3 = mask.reshape(nz, ny, nx)
+    cx, cy, cz = bfref.C[q]
+    src = np.zeros_like(m3)
+    src[max(cz, 0):nz + min(cz, 0), max(cy, 0):ny + min(cy, 0), max(cx, 0):nx + min(cx, 0)] = \
+        m3[max(-cz, 0):nz + min(-cz, 0), max(-cy, 0):ny + min(-cy, 0), max(-cx, 0):nx + min(-cx, 0)]
+    return (src.ravel() != 0) & (mask == 0)
+
+
+def test_wing_geometry_and_links_match_oracle():
+    p, cfg, wing, ow = wing_case()
+    steps = 6
+    say(f"case: Li & Nabawy wing, {SHAPE}, cycle {CYCLE}, {TIME_STEP} per step, geometry and links, steps 0-{steps}")
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.set_flapping_wing(wing)
+        sim.init_equilibrium()
+        prev = None
+        changed = 0
+        for s in range(steps + 1):
+            if s > 0:
+                sim.advance(1)
+            fr_o, m_o, w_o = oracle_geometry(p, ow, s, prev)
+            fr = sim.wing_frame()
+            mask, wall = sim.read_geometry()
+            links = sim.wing_links()
+            links_o = bfref.wing_links(p, ow, fr_o, m_o).reshape(19, -1)
+            guard = w_o[:, 3] != w_o[:, 3].max()
+            n_links = 0
+            link_ok = True
+            for q in range(1, 19):
+                want = source_is_solid(m_o, q)
+                n_links += int(want.sum())
+                link_ok &= bool(np.array_equal(links[q][want].view(np.uint32), links_o[q][want].view(np.uint32)))
+            if prev is not None:
+                changed += int((prev != m_o).sum())
+            say(f"  step {s}: wing cells {int((m_o != 0).sum())}, links {n_links}, mask equal "
+                f"{np.array_equal(mask, m_o)}, wall equal in guard {np.array_equal(wall[guard].view(np.uint32), w_o[guard].view(np.uint32))}, "
+                f"links equal {link_ok}")
+            assert np.array_equal(fr.view(np.uint32), fr_o.view(np.uint32))
+            assert np.array_equal(mask, m_o)
+            assert np.array_equal(wall[guard].view(np.uint32), w_o[guard].view(np.uint32))
+            assert link_ok and n_links > 500
+            prev = m_o
+        assert changed > 0
+
+
+def test_wing_fluid_steps_match_oracle_bit_for_bit():
+    p, cfg, wing, ow = wing_case()
+    steps = 8
+    say(f"case: Li & Nabawy wing fluid steps, {SHAPE}, TRT, link rule, mode 6, strict FP32, {steps} steps")
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.set_flapping_wing(wing)
+        sim.init_equilibrium()
+        sim.advance(steps)
+        g = sim.read_populations()
+        loads = sim.load_history()
+    _, m, w = oracle_geometry(p, ow, 0, None)
+    f = bfref.init_equilibrium(p, m, w)
+    prev = m
+    loads_o = []
+    for s in range(1, steps + 1):
+        fr, m, w = oracle_geometry(p, ow, s, prev)
+        links = bfref.wing_links(p, ow, fr, m)
+        f, ld, _, _ = bfref.step_with_links(p, f, prev, m, w, links)
+        loads_o.append(ld)
+        prev = m
+    loads_o = np.array(loads_o)
+    fe = np.abs(loads[:, :3] - loads_o[:, :3]).max() / max(np.abs(loads_o[:, :3]).max(), 1e-30)
+    te = np.abs(loads[:, 3:6] - loads_o[:, 3:6]).max() / max(np.abs(loads_o[:, 3:6]).max(), 1e-30)
+    say(f"  populations differing {int((g != f).sum())}; load rel err force {fe:.2e} torque {te:.2e}; "
+        f"last force {loads[-1][:3]}")
+    assert np.array_equal(g.view(np.uint32), f.view(np.uint32))
+    assert fe <= 1e-12 and te <= 1e-12 and np.array_equal(loads[:, 6:], loads_o[:, 6:])
+    assert np.abs(loads_o[:, :3]).max() > 0
+
+
+def test_link_rule_needs_the_wing_and_the_adjoint_refuses_it():
+    p, cfg, wing, _ = wing_case()
+    say("case: link rule without the wing; adjoint of a wing")
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.init_equilibrium()
+        with pytest.raises(bh.BirdFlowError):
+            sim.advance(1)
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.set_flapping_wing(wing)
+        sim.init_equilibrium()
+        with pytest.raises(bh.BirdFlowError):
+            sim.adjoint_gradient(2, np.zeros((2, 6)))
