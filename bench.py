@@ -163,6 +163,53 @@ def run_sdf_bird(shape, steps, warmup):
                         "halfway walls, dx 4 mm, Re 500, FP32"}
 
 
+def run_wing(shape, steps, warmup):
+    """The prescribed Li & Nabawy beta-planform wing (BirdFlow.metal:940-1297)
+    with the link-table wall rule: 32-cell mean chord, span 96 cells rooted
+    at the grid centre, cycle 8192 steps (tip speed at most 0.08 lattice),
+    far field at rest, sponge 0.04 over 8 cells, TRT at tau+ 0.577 (Re 100
+    on the chord at 0.08), force mode 6, FP32.  MLUPS of whole steps
+    (frame, mask, wall, link bisection, fluid, loads)."""
+    import birdflow_hip as bh
+    root = (0.5 * shape[0], 0.5 * shape[1], 0.5 * shape[2])
+    tau = 3.0 * 0.08 * 32.0 / 100.0 + 0.5
+    omega_plus = 1.0 / tau
+    omega_minus = 1.0 / (0.1875 / (tau - 0.5) + 0.5)
+    cfg = bh.SolverConfig(shape=shape, omega_plus=omega_plus, omega_minus=omega_minus, storage=bh.STORAGE_FP32,
+                          collision=bh.COLLISION_TRT, wall_rule=bh.WALL_INTERPOLATED_LINKS, force_mode=6,
+                          accumulate_loads=True, boundary=((bh.BC_FARFIELD, bh.BC_FARFIELD),) * 3, cell_size=1.0,
+                          time_step=1.0, velocity_to_lattice=1.0, sponge_strength=0.04, sponge_width=8.0,
+                          body_position=root)
+    sim = bh.BirdFlowSimulation(cfg)
+    try:
+        sim.set_flapping_wing(bh.FlappingWing.li_nabawy(32.0, root, 8192.0))
+        sim.init_equilibrium()
+        sim.advance(warmup)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.advance(steps, synchronize=False)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        loads = sim.load_history(steps)
+        mask, _ = sim.read_geometry()
+        sim.set_profiling(True, level=2)
+        sim.advance(BREAKDOWN_STEPS)
+        brk = sim.read_profile()
+        sim.set_profiling(False)
+    finally:
+        sim.close()
+    n = cfg.cells
+    k = max(brk["steps"], 1)
+    return {"value": n * steps / (t1 - t0) / 1e6, "unit": "MLUPS", "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "steps": steps, "solid_cells": int((mask != 0).sum()), "general_tiles": int(brk["general_tiles"]),
+            "tiles": int(brk["tiles"]),
+            "step_breakdown_ms": {"geometry": brk["geometry_ms"] / k, "fluid_fast": brk["fast_ms"] / k,
+                                  "fluid_general": brk["general_ms"] / k},
+            "finite": bool(np.isfinite(loads).all()),
+            "workload": f"Li & Nabawy beta-planform wing {shape[0]}x{shape[1]}x{shape[2]}, chord 32 cells, "
+                        "link-table walls, TRT, FP32"}
+
+
 def omp_threads(n):
     """Sets the oracle's OpenMP thread count (libgomp, loaded with it)."""
     import ctypes
@@ -277,6 +324,10 @@ def main():
     ap.add_argument("--adjoint-interval", type=int, default=100, help="checkpoint interval of that sweep")
     ap.add_argument("--sdf-bird-steps", type=int, default=100,
                     help="timed steps of the analytic demonstration bird on the same grid (0: skip)")
+    ap.add_argument("--wing-steps", type=int, default=100,
+                    help="timed steps of the prescribed Li & Nabawy wing on a 384x384x192 grid (0: skip)")
+    ap.add_argument("--wing-only", action="store_true",
+                    help="run only the prescribed wing and print its result line (profiling)")
     ap.add_argument("--sdf-bird-only", action="store_true",
                     help="run only the analytic demonstration bird and print its result line (profiling)")
     args = ap.parse_args()
@@ -285,6 +336,11 @@ def main():
                           "requested_gpus": args.gpus}), flush=True)
         sys.exit(2)
     shape = tuple(args.shape)
+    if args.wing_only:
+        wr = run_wing((384, 384, 192), args.wing_steps, args.warmup)
+        print(json.dumps(dict(wr, metric="prescribed flapping wing step", n_gpus=1, warmup=args.warmup,
+                              dtype="f32", data="synthetic")), flush=True)
+        return
     if args.sdf_bird_only:
         sdf = run_sdf_bird(shape, args.sdf_bird_steps, args.warmup)
         print(json.dumps(dict(sdf, metric="analytic demonstration bird step", n_gpus=1, warmup=args.warmup,
@@ -309,6 +365,10 @@ def main():
         sdf = run_sdf_bird(shape, args.sdf_bird_steps, args.warmup)
         print(f"# analytic bird: {sdf}", file=sys.stderr, flush=True)
         extra["analytic_bird"] = sdf
+    if args.wing_steps > 0:
+        wr = run_wing((384, 384, 192), args.wing_steps, args.warmup)
+        print(f"# prescribed wing: {wr}", file=sys.stderr, flush=True)
+        extra["prescribed_wing"] = wr
     if args.adjoint_steps > 0:
         adj = run_adjoint((256, 128, 128), args.adjoint_steps, args.adjoint_interval)
         print(f"# adjoint: {adj}", file=sys.stderr, flush=True)

@@ -18,8 +18,8 @@ Where this archive differs from the reference's, and why:
   BF16 bits of the offsets f_q - w_q) in direction-major order, so a restart
   is bit-exact for every storage; the reference copies its buffer bytes in
   its own layout.
-* The geometry source travels in the archive (mesh arrays, sphere or
-  analytic-bird parameters, or the static wall field); the reference's
+* The geometry source travels in the archive (mesh arrays, sphere,
+  analytic-bird or flapping-wing parameters, or the static wall field); the reference's
   archive holds the bird parameters and its prepared frame, which a restore
   here prepares again from the body pose, and a mesh surface is caller
   data.
@@ -46,7 +46,7 @@ from typing import Dict, Optional
 import numpy as np
 
 from . import _abi
-from .simulation import BirdFlowSimulation, BirdGeometry, BodyParameters, SolverConfig
+from .simulation import BirdFlowSimulation, BirdGeometry, BodyParameters, FlappingWing, SolverConfig
 
 SCHEMA_VERSION = 1
 MANIFEST_NAME = "manifest.json"
@@ -61,6 +61,7 @@ _GEOMETRY_ARRAYS = {
     "static": (("wall", np.float32), ("mask_prev", np.uint8)),
     "sphere": (),
     "bird": (("keyframes", np.float32),),
+    "wing": (),
 }
 
 
@@ -186,7 +187,7 @@ def save_checkpoint(sim: BirdFlowSimulation, destination: str, compression: str 
     origin, the configuration and the geometry source.  Returns the
     manifest."""
     if sim.geometry_source is None:
-        raise CheckpointError("a checkpoint needs a geometry source (mesh, sphere or static geometry)")
+        raise CheckpointError("a checkpoint needs a geometry source (mesh, sphere, bird, wing or static geometry)")
     cfg = sim.config
     step, origin = sim.restart_state()
     populations = sim.read_population_storage()
@@ -206,6 +207,9 @@ def save_checkpoint(sim: BirdFlowSimulation, destination: str, compression: str 
     elif kind == "sphere":
         for name in ("center_radius", "geometry_velocity", "wall_velocity"):
             geometry[name] = _plain(src[name])
+    elif kind == "wing":
+        # prescribed: the parameters fix every step's frame, mask and links
+        geometry["wing"] = {k: _plain(v) for k, v in src["wing"].items()}
     elif kind == "bird":
         # the shape; a restore prepares the frame again from the body pose
         # and keeps the archive's mask as the next step's previous mask
@@ -298,6 +302,8 @@ def load_checkpoint(source: str) -> BirdFlowSimulation:
                             translation=a["translation"])
         elif g["kind"] == "sphere":
             sim.set_translating_sphere(g["center_radius"], g["geometry_velocity"], g["wall_velocity"])
+        elif g["kind"] == "wing":
+            sim.set_flapping_wing(FlappingWing(**g["wing"]))
         elif g["kind"] == "bird":
             sim.set_bird(BirdGeometry(**g["bird"]), keyframes=ck.geometry.get("keyframes"))
         else:

@@ -1131,8 +1131,33 @@ __global__ __launch_bounds__(256) void wing_mark_tiles(WingShape w, const float4
     const float ly = p.origin[1] + ((float)y0 + 0.5f) * p.dx, hy = p.origin[1] + ((float)y1 + 0.5f) * p.dx;
     const float zc = p.origin[2] + ((float)z + 0.5f) * p.dx;
     const float4 c = frame[0];
-    const float r = w.radius + 1.5f * c.w + 2.0f * p.dx + 2.0f * p.dx;
-    const bool hit = tile_box_dist2(lx, hx, ly, hy, zc, c.x, c.y, c.z) <= r * r;
+    bool hit;
+    if (w.beta_exponent > 0.0f) {
+        // The wing lies in the box chord [lead, lead + c_max], span [0, R],
+        // normal [-h, h] of its frame (c_max at mid-span, where r(1 - r) is
+        // largest).  A tile (the segment of cell centres [lx, hx] x [ly, hy]
+        // at z) is marked unless one of the frame's axes separates it from
+        // that box grown by two cells: every wing cell, and every cell that
+        // links to one, lies in a marked tile.
+        const float m = 2.0f * p.dx;
+        const float lead = -0.25f * c.w;
+        const float cmax = 1.01f * c.w * w.beta_normalization * wing_pow(0.25f, w.beta_exponent);
+        const float h = 0.5f * smax(w.thickness, p.dx);
+        const V3 mid = v3(0.5f * (lx + hx) - c.x, 0.5f * (ly + hy) - c.y, zc - c.z);
+        const float ex = 0.5f * (hx - lx), ey = 0.5f * (hy - ly);
+        const float lo[3] = {lead - m, -m, -h - m};
+        const float hi[3] = {lead + cmax + m, w.radius + m, h + m};
+        hit = true;
+        for (int a = 0; a < 3; ++a) {
+            const V3 axis = row3(frame[1 + a]);
+            const float centre = vdot(mid, axis);
+            const float reach = fabsf(axis.x) * ex + fabsf(axis.y) * ey;
+            hit = hit && centre + reach >= lo[a] && centre - reach <= hi[a];
+        }
+    } else {
+        const float r = w.radius + 1.5f * c.w + 2.0f * p.dx + 2.0f * p.dx;
+        hit = tile_box_dist2(lx, hx, ly, hy, zc, c.x, c.y, c.z) <= r * r;
+    }
     tiles_new[t] = (hit || (dyn_prev && dyn_prev[t])) ? 1 : 0;
 }
 
@@ -1198,12 +1223,18 @@ __global__ __launch_bounds__(256) void wing_tile_pass(WingShape w, const float4*
 // direction whose (in-domain) source is solid, the bisection from the
 // source's centre plus c_q dx (the reference's fluid point) to the source.
 __global__ __launch_bounds__(256) void wing_links(WingShape w, const float4* __restrict__ frame,
+                                                  const uint32_t* __restrict__ list,
+                                                  const uint32_t* __restrict__ count,
                                                   const uint8_t* __restrict__ dyn, const uint8_t* __restrict__ mask,
                                                   float* __restrict__ links, BirdParams p) {
     __shared__ float4 fr[WING_FRAME_ROWS];
     if (threadIdx.x < WING_FRAME_ROWS) fr[threadIdx.x] = frame[threadIdx.x];
     __syncthreads();
-    for (uint32_t tile = blockIdx.x; tile < p.g.tiles; tile += gridDim.x) {
+    // Tiles with wing cells or their neighbours (dyn) are marked tiles, so
+    // they are on the step's geometry list.
+    const uint32_t n_tiles = *count;
+    for (uint32_t i = blockIdx.x; i < n_tiles; i += gridDim.x) {
+        const uint32_t tile = list[i];
         if (!dyn[tile]) continue;
         int x0, y0, z;
         tile_origin(p.g, tile, x0, y0, z);
@@ -1247,7 +1278,8 @@ hipError_t launch_wing_geometry(const WingShape& shape, const WingBuffers& b, co
     hipLaunchKernelGGL(wing_tile_pass, dim3(p.tile_pass_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
                        b.list, b.list_count, (const uint8_t*)b.tiles_new, b.mask_prev, b.mask, b.wall, b.dyn, p);
     hipLaunchKernelGGL(wing_links, dim3(p.tile_pass_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
-                       (const uint8_t*)b.dyn, (const uint8_t*)b.mask, b.links, p);
+                       (const uint32_t*)b.list, (const uint32_t*)b.list_count, (const uint8_t*)b.dyn,
+                       (const uint8_t*)b.mask, b.links, p);
     return hipGetLastError();
 }
 

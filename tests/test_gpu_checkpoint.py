@@ -7,7 +7,8 @@
   populations, loads, masks and body state bit-identical after more steps
   (the reference allows 1e-6 / 1e-5; here the restart is exact, for FP32,
   FP16 and BF16 storage, a flapping-plate mesh, a translating sphere, a
-  free-flying body carrying its mesh and a free-flying analytic bird);
+  free-flying body carrying its mesh, a free-flying analytic bird and
+  the prescribed flapping wing with its link table);
 * a corrupted populations file is refused with CheckpointError.
 """
 import os
@@ -126,6 +127,15 @@ def test_restart_of_free_flying_analytic_bird_is_exact(tmp_path):
         sim.set_body(state0, bh.BodyParameters(mass=0.025, inertia=(4e-6, 7e-6, 8e-6), gravity=(0.0, 0.0, -9.81),
                                                substeps=2))
         sim.set_bird(bird, keyframes=cases.compact_bird_keyframes())
+        sim.init_equilibrium()
+        check_restart(sim, tmp_path, 5, 4)
+
+
+def test_restart_of_prescribed_wing_is_exact(tmp_path):
+    say("case: checkpoint of the Li & Nabawy wing 64x64x32, link rule, 5 + 4 steps")
+    _, cfg, wing, _ = cases.wing_case()
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.set_flapping_wing(wing)
         sim.init_equilibrium()
         check_restart(sim, tmp_path, 5, 4)
 
