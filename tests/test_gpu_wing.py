@@ -13,6 +13,9 @@ step and cells change sides every step):
   every fluid cell whose source is on the wing;
 * fluid steps with the link rule: populations bit-identical, loads to 1e-12
   relative (FP64 sums of the same FP32 terms in another order);
+* the fused-FP32 and FP16 / BF16 offset forms of the link rule read the
+  same table and track strict FP32 (populations 2e-4, force history 2e-3
+  of the largest force; measured 4e-5 and 2e-4 for BF16);
 * the link rule without the wing, and the adjoint of a wing, are refused.
 """
 import sys
@@ -135,3 +138,30 @@ def test_link_rule_needs_the_wing_and_the_adjoint_refuses_it():
         sim.init_equilibrium()
         with pytest.raises(bh.BirdFlowError):
             sim.adjoint_gradient(2, np.zeros((2, 6)))
+
+
+@pytest.mark.parametrize("storage,arithmetic", [(bh.STORAGE_FP32, bh.ARITHMETIC_FUSED),
+                                                (bh.STORAGE_FP16_OFFSET, bh.ARITHMETIC_DEFAULT),
+                                                (bh.STORAGE_BF16_OFFSET, bh.ARITHMETIC_DEFAULT)])
+def test_wing_link_rule_in_the_other_forms_tracks_strict_fp32(storage, arithmetic):
+    """The link rule in the fused-FP32 and FP16 / BF16 offset instantiations:
+    the same link table (bit-identical: the geometry does not depend on the
+    storage) and populations and loads close to the strict FP32 run."""
+    p, cfg, wing, _ = cases.wing_case()
+    steps = 8
+    say(f"case: wing link rule, storage {storage}, arithmetic {arithmetic}, {steps} steps vs strict FP32")
+    out = []
+    for st, ar in ((bh.STORAGE_FP32, bh.ARITHMETIC_STRICT), (storage, arithmetic)):
+        cfg.storage, cfg.arithmetic = st, ar
+        with bh.BirdFlowSimulation(cfg) as sim:
+            sim.set_flapping_wing(wing)
+            sim.init_equilibrium()
+            sim.advance(steps)
+            out.append((sim.read_populations(), sim.load_history(), sim.wing_links()))
+    (f32, l32, k32), (f, l, k) = out
+    pop = float(np.abs(f - f32).max())
+    force = float(np.abs(l[:, :3] - l32[:, :3]).max() / np.abs(l32[:, :3]).max())
+    say(f"  max |population diff| {pop:.2e}; force history max diff / max force {force:.2e}")
+    assert np.array_equal(k.view(np.uint32), k32.view(np.uint32))
+    assert np.all(np.isfinite(f)) and np.all(np.isfinite(l))
+    assert pop <= 2e-4 and force <= 2e-3
