@@ -274,3 +274,72 @@ def wing_case():
     cfg = solver_config(p, time_step=WING_TIME_STEP, velocity_to_lattice=WING_VELOCITY_TO_LATTICE)
     return (p, cfg, bh.FlappingWing.li_nabawy(8.0, WING_ROOT, WING_CYCLE),
             bfref.li_nabawy_wing(8.0, WING_ROOT, WING_CYCLE))
+
+
+# ---------------------------------------------------------------------------
+# The reference's indexed-dove production coupling replay
+# (MetalIndexedBirdSurfaceValidation.swift:19284-19600, archived as
+# ValidationArtifacts/deetjen-dove-indexed-production-coupling.json):
+# 59x53x50 periodic, TRT, signed-distance link fractions, force mode 6,
+# geometry of step s at time s * dt from mesh frames 0 and 1.
+# ---------------------------------------------------------------------------
+class DeetjenCoupling:
+    def __init__(self, golden_dir):
+        import json
+        import os
+        d = np.load(os.path.join(golden_dir, "deetjen_subset.npz"))
+        self.ref = json.load(open(os.path.join(golden_dir, "deetjen_coupling.json")))
+        f32 = np.float32
+        self.dx = f32(d["dx"])
+        self.dt = f32(self.ref["timeStepSeconds"])
+        # LatticeScaling (SimulationConfiguration.swift:137-183) in FP32 with the
+        # replay's reference length 8 cells, Re 1000, lattice speed 0.04, magic 3/16
+        nu = f32(f32(0.04) * f32(8)) / f32(1000)
+        tau_p = f32(0.5) + f32(3) * nu
+        tau_m = f32(0.5) + f32(f32(3.0) / f32(16.0)) / f32(tau_p - f32(0.5))
+        self.omega_plus, self.omega_minus = f32(1) / tau_p, f32(1) / tau_m
+        self.velocity_to_lattice = f32(self.dt / self.dx)
+        self.force_to_physical = f32(f32(self.dx ** f32(4)) / f32(self.dt ** f32(2)))
+        pts = deetjen_points(d)
+        self.points = np.stack([pts[0], pts[1]]).astype(np.float32)
+        self.times = d["times"][:2].astype(np.float32)
+        self.triangles = d["triangles"]
+        self.parts = d["triangle_parts"]
+        self.half_thickness = f32(d["half_thickness_cells"]) * self.dx
+        self.origin = tuple(float(v) for v in d["origin"])
+        self.grid = tuple(int(v) for v in d["grid"])
+        self.params = bfref.make_params(
+            self.grid, omega_plus=float(self.omega_plus), omega_minus=float(self.omega_minus),
+            collision=bfref.COLLISION_TRT, wall_rule=bfref.WALL_INTERPOLATED_SDF, force_mode=6,
+            accumulate_loads=True, origin=self.origin, dx=float(self.dx),
+            force_scale=float(self.force_to_physical))
+
+    def time(self, step):
+        """Mesh time of step s: initial time + Float(s) * dt (FP32)."""
+        return float(np.float32(self.times[0]) + np.float32(step) * self.dt)
+
+    def geometry(self, step):
+        t = self.time(step)
+        pos, vel = bfref.prepare_vertices(self.points, self.times, 0, t,
+                                          velocity_scale=float(self.velocity_to_lattice))
+        mask, wall, _ = bfref.raster_resolve(self.params, pos, vel, self.triangles, self.parts,
+                                             self.half_thickness, key_bits=64)
+        return mask, wall
+
+    def run_oracle(self, steps=8):
+        """Per step: covered, uncovered, persistent links, load, masks."""
+        p = self.params
+        mask0, wall0 = self.geometry(0)
+        f = bfref.init_equilibrium(p, mask0, wall0, np.float32)
+        prev = mask0
+        rows = []
+        for s in range(1, steps + 1):
+            mask, wall = self.geometry(s)
+            fo, ld, _, _ = bfref.step(p, f, prev, mask, wall)
+            _, counts = bfref.ledger_sources(p, f, fo, prev, mask, wall)
+            cov = int(((prev == 0) & (mask != 0)).sum())
+            unc = int(((prev != 0) & (mask == 0)).sum())
+            rows.append(dict(covered=cov, uncovered=unc, persistent=int(counts[2]), transitions=int(counts[3]),
+                             load=ld, mask=mask, populations=fo))
+            f, prev = fo, mask
+        return rows

@@ -16,15 +16,25 @@ Sources:
     occupancy SHA-256 and per-part solid-cell counts.
   * ValidationArtifacts/translating-body-topology-summary.json — cover /
     uncover events and mean conservative force of the translating sphere.
+  * ValidationArtifacts/deetjen-dove-indexed-production-coupling.json — per
+    step newly covered / uncovered cells, persistent wall links and impulses
+    of the 8-step indexed-dove replay (frames 0 and 1 of the mesh).
+
+Usage: make_golden.py [deetjen] [coupling] [translating] [shear]  (default: all)
 """
 from __future__ import annotations
 
-import hashlib
-import json
-import os
 import sys
 
-import numpy as np
+# Importing the reference's Python program must leave nothing behind in the
+# read-only checkout: no __pycache__ is written (or read) for it.
+sys.dont_write_bytecode = True
+
+import hashlib  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 REF = "/root/reference"
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -95,7 +105,7 @@ def deetjen():
     hi = (pts.reshape(-1, 3).max(axis=0) + pad).astype(np.float32)
     grid = [max(16, int(np.ceil(np.float32(e / dx))) + 1) for e in (hi - lo).astype(np.float32)]
     assert grid == [art["gridX"], art["gridY"], art["gridZ"]], grid
-    keep = sorted(set(list(range(0, frames, 12)) + [33, 89, 126, 134, 142, 143]))
+    keep = sorted(set(list(range(0, frames, 12)) + [1, 33, 89, 126, 134, 142, 143]))
     np.savez_compressed(
         os.path.join(OUT, "deetjen_subset.npz"),
         frames=np.asarray(keep, np.int32), points=pts[keep], times=times, triangles=tri.astype(np.uint32),
@@ -119,7 +129,28 @@ def translating_body():
         json.dump({k: s[k] for k in keys}, fh, indent=1)
 
 
+def coupling():
+    s = json.load(open(os.path.join(REF, "ValidationArtifacts", "deetjen-dove-indexed-production-coupling.json")))
+    keys = ["gridX", "gridY", "gridZ", "cellSizeMeters", "timeStepSeconds", "steps", "periodicBoundaries",
+            "spongeStrength", "forceEstimator", "fluidKernel", "linkKernel", "maximumWallSpeedLattice",
+            "newlyCoveredCellEvents", "newlyUncoveredCellEvents", "persistentBoundaryLinkEvents",
+            "componentSolidCellCounts", "relativeRMSBoundaryClosureResidual", "deviceName"]
+    sample_keys = ["step", "sourceTimeSeconds", "newlyCoveredCellCount", "newlyUncoveredCellCount",
+                   "persistentBoundaryLinkCount", "sourceLedgerTransitionCellCount", "aerodynamicImpulse",
+                   "fluidMomentumBefore", "fluidMomentumAfter", "diagnosticPersistentLinkImpulseToFluid"]
+    out = {k: s[k] for k in keys}
+    out["samples"] = [{k: smp[k] for k in sample_keys} for smp in s["samples"]]
+    with open(os.path.join(OUT, "deetjen_coupling.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 if __name__ == "__main__":
-    deetjen()
-    translating_body()
-    shear_wave()
+    parts = sys.argv[1:] or ["deetjen", "coupling", "translating", "shear"]
+    if "deetjen" in parts:
+        deetjen()
+    if "coupling" in parts:
+        coupling()
+    if "translating" in parts:
+        translating_body()
+    if "shear" in parts:
+        shear_wave()
