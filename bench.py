@@ -295,14 +295,30 @@ def run_adjoint(shape, steps, interval):
 
 
 def measured_traffic(storage, arithmetic, cells):
-    """HBM bytes per fast-kernel launch from the committed rocprofv3 PMC passes
-    (FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel) for this storage
-    and arithmetic at this launch size; None when no such measurement exists."""
+    """HBM bytes of one fast-kernel launch over `cells` cells, from the
+    committed rocprofv3 PMC passes (FETCH_SIZE and WRITE_SIZE per dispatch,
+    calibrated on a copy kernel) for this storage and arithmetic: the
+    measured bytes per cell times the cells of this launch (the number of
+    wall-free tiles moves by a few per step as the bird flaps).  None when no
+    such measurement exists."""
     try:
         rec = json.load(open(TRAFFIC_FILE))[f"{storage}/{arithmetic}"]
-    except (OSError, KeyError, ValueError):
+        per_cell = rec["bytes_per_launch"] / rec["cells_per_launch"]
+    except (OSError, KeyError, ValueError, ZeroDivisionError):
         return None
-    return rec["bytes_per_launch"] if rec.get("cells_per_launch") == cells else None
+    return per_cell * cells
+
+
+def cpu_model():
+    """The host CPU's model name (the CPU baseline's hardware)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -378,7 +394,8 @@ def main():
         mlups, threads, elapsed = run_cpu_baseline(shape, args.cpu_steps)
         print(f"# cpu: {mlups:.2f} MLUPS over {args.cpu_steps} steps ({elapsed:.1f} s, {threads} threads)",
               file=sys.stderr, flush=True)
-        cpu = {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": "port",
+        cpu = {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+               "host_cpus": os.cpu_count(),
                "sample": f"oracle (C, OpenMP over z) on the same bird workload, {args.cpu_steps} full steps "
                          f"(geometry + fluid) at {shape[0]}x{shape[1]}x{shape[2]}"}
         if args.cpu_single_steps > 0:
