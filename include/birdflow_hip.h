@@ -233,6 +233,27 @@ bf_status bf_set_bird(bf_context* ctx, const bf_bird_geometry* bird);
  * count 0 returns to the analytic sinusoid.  With the bird set, the current
  * step's geometry is rebuilt. */
 bf_status bf_set_bird_keyframes(bf_context* ctx, const float* keyframes, uint32_t count);
+/* Prescribed-wing dynamics of the analytic bird (BirdParameters.
+ * prescribedWingDynamics, BirdModel.swift:98-128, as GPUBirdParameters packs
+ * it, GPUData.swift:150-176): per wing, mass (kg), centre of mass from the
+ * hinge along the wing's chord, span and normal axes (m), principal inertia
+ * about those axes through the centre of mass (kg m^2). */
+typedef struct bf_wing_dynamics {
+    float mass;
+    float center_of_mass[3];
+    float inertia[3];
+} bf_wing_dynamics;
+/* Sets (both non-NULL) or clears (both NULL) the wings' dynamics: each step,
+ * after the frame is prepared, updateWingInertialReaction
+ * (BirdFlow.metal:6493-6572) forms each wing's hinge reaction -(momentum -
+ * previous momentum) / time_step, and in free flight integrateBirdBody
+ * (:6574-6627) adds both to the fluid force and torque.  With the bird set,
+ * the current step's frame is rebuilt and its momenta start the history
+ * (initializeOnly: no reaction on it). */
+bf_status bf_set_bird_wing_dynamics(bf_context* ctx, const bf_wing_dynamics* left, const bf_wing_dynamics* right);
+/* The current step's hinge reactions (GPUWingInertialReaction, 4 float4: left
+ * force, left torque, right force, right torque; world units, w = 0). */
+bf_status bf_read_wing_reaction(bf_context* ctx, float reaction[16]);
 /* Prepared frame of the current step (GPUPreparedBirdGeometry, 14 float4:
  * body position, orientation x y z w, linear velocity, world angular
  * velocity + geometry radius, then left and right wing root, chord (w: tip
@@ -362,10 +383,22 @@ bf_status bf_body_get(bf_context* ctx, bf_body_state* state);
 bf_status bf_read_body_history(bf_context* ctx, bf_body_state* out, uint32_t max_records, uint32_t* records);
 /* Safety record: metrics = max Mach, min clearance, Mach and clearance at
  * the first event; flags = 1 Mach above the limit, 2 clearance below zero,
- * 4 non-finite state (GPURuntimeSafetyRecord, GPUData.swift:234-245).
- * bf_synchronize returns BF_ERROR_RUNTIME_SAFETY_VIOLATION once when a new
- * event is recorded; the context stays usable. */
+ * 4 non-finite state (GPURuntimeSafetyRecord, GPUData.swift:234-245), and 8
+ * non-finite populations: every fluid kernel checks that each cell's stored
+ * populations sum to a finite density and records the first step that fails
+ * (event_step when no body event came first; reset by a new population
+ * state).  bf_synchronize returns BF_ERROR_RUNTIME_SAFETY_VIOLATION once when
+ * a new event is recorded; the context stays usable. */
 bf_status bf_read_safety(bf_context* ctx, double metrics[4], uint64_t* event_step, uint32_t* flags);
+/* reducePopulationMinimum (BirdFlow.metal:6404-6459, read back by
+ * MetalIndexedBirdSurfaceValidation.swift:20067-20087): the smallest current
+ * population (FP16 / BF16 offsets decoded), a non-finite value comparing as
+ * -infinity, ties to the smallest direction-major index q N + cell; the
+ * value, that index and whether it is non-finite. */
+bf_status bf_population_minimum(bf_context* ctx, float* value, uint64_t* index, uint32_t* non_finite);
+/* Fault injection for the safety checks: sets population `index`
+ * (direction-major, q N + cell) to `value` in the stored form. */
+bf_status bf_inject_population(bf_context* ctx, uint64_t index, float value);
 
 /* ---- Adjoint (reverse-mode gradient of a load functional) ----
  * J = sum over the next `steps` steps s of sum_i w[s][i] L_s[i], with L_s the
@@ -376,8 +409,8 @@ bf_status bf_read_safety(bf_context* ctx, double metrics[4], uint64_t* event_ste
  * geometry.  Each step's link set (masks) is held fixed, as it is
  * discontinuous in theta; link fractions and wall velocities are
  * differentiated.  No counterpart exists in the reference.  Supported: FP32
- * storage, TRT / regularized / RR3, periodic and far-field faces, force
- * mode 6, mesh or static geometry, no free flight.
+ * storage, TRT / regularized / RR3, periodic and far-field faces and outlets
+ * on the x faces, force mode 6, mesh or static geometry, no free flight.
  * Populations are snapshotted every checkpoint_interval steps in the forward
  * pass and recomputed segment by segment in the reverse sweep (memory:
  * about (segments + interval) x 76 B and interval x 21 B per cell). */

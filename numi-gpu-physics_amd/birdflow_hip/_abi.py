@@ -46,8 +46,10 @@ EXPORTED_FUNCTIONS = (
     "bf_mesh_upload", "bf_mesh_build", "bf_set_translating_sphere", "bf_step", "bf_synchronize",
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
+    "bf_population_minimum", "bf_inject_population",
     "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity", "bf_mesh_freeze_masks",
-    "bf_set_bird", "bf_set_bird_keyframes", "bf_read_bird_frame", "bf_ledger_fluid", "bf_ledger_step_sources",
+    "bf_set_bird", "bf_set_bird_keyframes", "bf_read_bird_frame", "bf_set_bird_wing_dynamics",
+    "bf_read_wing_reaction", "bf_ledger_fluid", "bf_ledger_step_sources",
     "bf_set_flapping_wing", "bf_read_wing_frame", "bf_read_wing_links",
 )
 
@@ -119,6 +121,11 @@ class BirdGeometry(ctypes.Structure):
         ("frequency", ctypes.c_float), ("stroke_amplitude", ctypes.c_float), ("stroke_bias", ctypes.c_float),
         ("pitch_mean", ctypes.c_float), ("pitch_amplitude", ctypes.c_float), ("pitch_phase", ctypes.c_float),
     ]
+
+
+class WingDynamicsC(ctypes.Structure):
+    """bf_wing_dynamics (kg, metres, kg m^2)."""
+    _fields_ = [("mass", ctypes.c_float), ("center_of_mass", ctypes.c_float * 3), ("inertia", ctypes.c_float * 3)]
 
 
 WING_FIELDS = ("root_x", "root_y", "root_z", "chord_mean", "radius", "thickness", "beta_exponent",
@@ -195,6 +202,10 @@ def _declare(l):
         "bf_mesh_freeze_masks": (st, [vp, vp, u32]),
         "bf_set_bird": (st, [vp, ctypes.POINTER(BirdGeometry)]),
         "bf_read_bird_frame": (st, [vp, vp]),
+        "bf_set_bird_wing_dynamics": (st, [vp, ctypes.POINTER(WingDynamicsC), ctypes.POINTER(WingDynamicsC)]),
+        "bf_read_wing_reaction": (st, [vp, vp]),
+        "bf_population_minimum": (st, [vp, vp, vp, vp]),
+        "bf_inject_population": (st, [vp, ctypes.c_uint64, ctypes.c_float]),
         "bf_set_bird_keyframes": (st, [vp, vp, u32]),
         "bf_set_flapping_wing": (st, [vp, ctypes.POINTER(FlappingWingC)]),
         "bf_read_wing_frame": (st, [vp, vp]),

@@ -2,15 +2,21 @@
 
 Follows the reference's CheckpointArchive
 (Sources/BirdFlowMetal/CheckpointArchive.swift): a directory holding
-manifest.json (schema 1, sorted keys) and the numerical state as compressed
+manifest.json (sorted keys) and the numerical state as compressed
 files, written into a temporary sibling directory that is then moved into
-place (:43-79), and read back with the schema, the byte counts and the
+place (:43-79), and read back with the format, the byte counts and the
 SHA-256 of every file's contents checked (:81-118).  save_checkpoint and
 load_checkpoint mirror BirdFlowSimulation.saveCheckpoint
 (BirdFlowSimulation.swift:1214-1298) and init(checkpointURL:) with
 restoreCheckpointState (:661-680, :1920-1990).
 
 Where this archive differs from the reference's, and why:
+
+* The manifest names its own format ("format": "birdflow-hip", "schema"
+  counting that format's versions).  Its fields are not those of the
+  reference's schema-1 BirdFlowCheckpointManifest (populationBytes,
+  populationSHA256, ...; CheckpointArchive.swift:23-37), so a reader of
+  either format refuses the other's archive instead of misreading it.
 
 * zlib (Python's standard library) compresses instead of LZFSE, an Apple
   library.
@@ -48,6 +54,7 @@ import numpy as np
 from . import _abi
 from .simulation import BirdFlowSimulation, BirdGeometry, BodyParameters, FlappingWing, SolverConfig
 
+FORMAT_NAME = "birdflow-hip"
 SCHEMA_VERSION = 1
 MANIFEST_NAME = "manifest.json"
 POPULATIONS_NAME = "populations.bin"
@@ -131,7 +138,7 @@ def write_archive(destination: str, manifest: dict, files: Dict[str, bytes], com
             with open(os.path.join(temporary, stored), "wb") as fh:
                 fh.write(_compress(data, compression))
             table[name] = {"file": stored, "bytes": len(data), "sha256": sha256(data)}
-        full = dict(manifest, schema=SCHEMA_VERSION, compression=compression, files=table)
+        full = dict(manifest, format=FORMAT_NAME, schema=SCHEMA_VERSION, compression=compression, files=table)
         with open(os.path.join(temporary, MANIFEST_NAME), "w") as fh:
             json.dump(full, fh, indent=2, sort_keys=True)
             fh.write("\n")
@@ -147,14 +154,16 @@ def write_archive(destination: str, manifest: dict, files: Dict[str, bytes], com
 
 def read_archive(source: str):
     """CheckpointArchive.read (CheckpointArchive.swift:81-118): the manifest
-    and every listed file, with the schema, byte counts and checksums
+    and every listed file, with the format, schema, byte counts and checksums
     checked.  Returns (manifest, {name: bytes})."""
     try:
         with open(os.path.join(source, MANIFEST_NAME)) as fh:
             manifest = json.load(fh)
     except (OSError, ValueError) as exc:
         raise CheckpointError(f"invalid archive: manifest unreadable ({exc})") from None
-    if not isinstance(manifest, dict) or manifest.get("schema") != SCHEMA_VERSION:
+    if not isinstance(manifest, dict) or manifest.get("format") != FORMAT_NAME:
+        raise CheckpointError(f"invalid archive: not a {FORMAT_NAME} archive")
+    if manifest.get("schema") != SCHEMA_VERSION:
         raise CheckpointError("invalid archive: unsupported schema")
     compression = manifest.get("compression")
     table = manifest.get("files")

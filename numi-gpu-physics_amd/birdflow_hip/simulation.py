@@ -462,6 +462,45 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_read_wing_links(self._ctx, _ptr(out), out.size))
         return out.reshape(19, -1)
 
+    def population_minimum(self):
+        """(value, direction-major index, non_finite) of the smallest current
+        population, a non-finite value ranking first (reducePopulationMinimum)."""
+        v = ctypes.c_float()
+        i = ctypes.c_uint64()
+        nf = ctypes.c_uint32()
+        self._check(self._lib.bf_population_minimum(self._ctx, ctypes.byref(v), ctypes.byref(i), ctypes.byref(nf)))
+        return float(v.value), int(i.value), bool(nf.value)
+
+    def inject_population(self, index: int, value: float):
+        """Fault injection: population `index` (q N + cell) set to `value`."""
+        self._check(self._lib.bf_inject_population(self._ctx, int(index), float(value)))
+
+    def set_wing_dynamics(self, left=None, right=None):
+        """Prescribed-wing dynamics of the analytic bird: each wing a
+        (mass, centre of mass from the hinge along chord / span / normal,
+        principal inertia) triple, or both None to clear."""
+        if (left is None) != (right is None):
+            raise ValueError("give both wings' dynamics, or neither")
+
+        def c(w):
+            d = _abi.WingDynamicsC()
+            d.mass = float(w[0])
+            d.center_of_mass[:] = [float(v) for v in w[1]]
+            d.inertia[:] = [float(v) for v in w[2]]
+            return d
+        if left is None:
+            self._check(self._lib.bf_set_bird_wing_dynamics(self._ctx, None, None))
+        else:
+            cl, cr = c(left), c(right)
+            self._check(self._lib.bf_set_bird_wing_dynamics(self._ctx, ctypes.byref(cl), ctypes.byref(cr)))
+
+    def wing_reaction(self) -> np.ndarray:
+        """The current step's hinge reactions [4, 4]: left force, left torque,
+        right force, right torque."""
+        out = np.zeros((4, 4), np.float32)
+        self._check(self._lib.bf_read_wing_reaction(self._ctx, _ptr(out)))
+        return out
+
     def bird_frame(self) -> np.ndarray:
         """Prepared frame of the current step, [14, 4] (GPUPreparedBirdGeometry rows)."""
         out = np.zeros((14, 4), np.float32)

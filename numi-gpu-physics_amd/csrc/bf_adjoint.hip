@@ -22,8 +22,16 @@
 //        interpolated links and of uncover impulses.
 // Targets of p2a are the cell itself; a target of p2b has one contributor.
 //
+// Outlet (zero-gradient) faces on x: a cell of the face plane pulls the
+// populations that would cross the face from its own plane (the clamp), so
+// those targets have a second puller beside the regular one.  Its share goes
+// to a face buffer indexed by the puller (one writer), and adjoint_outlet
+// adds it to the target after the single-writer passes; the populations that
+// leave through the face are pulled by nobody and are zeroed by their owner,
+// as behind a far-field face.
+//
 // Supported: FP32 storage, TRT / regularized / RR3, periodic and far-field
-// faces, sponge, every wall rule, force mode 6.
+// faces and outlets on the x faces, sponge, every wall rule, force mode 6.
 #include "bf_adjoint_math.h"
 #include "bf_fluid.h"
 
@@ -50,6 +58,39 @@ template <int COLL>
 __device__ __forceinline__ void collision_adjoint(const float (&g)[19], float op, float om, const float (&lp)[19],
                                                   float (&mu)[19]) {
     adj::reverse<COLL, float>(g, op, om, lp, mu);
+}
+
+// Outlet x faces (BC_OUTLET): the pull of direction q from cell x is
+// clamped into x's own plane, or (x, q) leaves through the face.
+template <int q>
+__device__ __forceinline__ int clamped_face(const StepParams& p, int x) {
+    if (cx(q) < 0 && x == p.g.nx - 1 && p.bc[0][1] == BC_OUTLET) return 1;
+    if (cx(q) > 0 && x == 0 && p.bc[0][0] == BC_OUTLET) return 0;
+    return -1;
+}
+template <int q>
+__device__ __forceinline__ bool leaves_outlet(const StepParams& p, int x) {
+    return (cx(q) > 0 && x == p.g.nx - 1 && p.bc[0][1] == BC_OUTLET) ||
+           (cx(q) < 0 && x == 0 && p.bc[0][0] == BC_OUTLET);
+}
+__device__ __forceinline__ size_t outlet_slot(const Grid& g, int face, int q, int y, int z) {
+    return ((size_t)face * 19 + (size_t)q) * ((size_t)g.ny * (size_t)g.nz) + (size_t)y + (size_t)g.ny * (size_t)z;
+}
+// lambda_{s-1,q}(source) = v from the (only regular) puller x, or into the
+// outlet face buffer when x's pull along q is clamped; nothing for a
+// far-field source.
+template <int q>
+__device__ __forceinline__ void write_pull(const AdjointBuffers& b, const StepParams& p, int x, int y, int z,
+                                           uint32_t src, bool far, float v) {
+    if (far) return;
+    const int face = clamped_face<q>(p, x);
+    if (face >= 0) b.outlet_extra[outlet_slot(p.g, face, q, y, z)] = v;
+    else b.lam_out[(size_t)q * p.g.n + src] = v;
+}
+// (x, q) pulled by nobody: it leaves through a far-field or outlet face
+template <int q>
+__device__ __forceinline__ bool unpulled(const Sources& s, const StepParams& p, int x) {
+    return orphan<q>(s) || leaves_outlet<q>(p, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -86,8 +127,8 @@ __global__ __launch_bounds__(256) void adjoint_fast(AdjointBuffers b, StepParams
     collision_adjoint<COLL>(g, p.omega_plus, p.omega_minus, lp, mu);
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if (!far[q]) b.lam_out[(size_t)q * n + src[q]] = mu[q];
-        if (orphan<q>(s)) b.lam_out[(size_t)q * n + gid] = 0.0f;
+        write_pull<q>(b, p, x, y, z, src[q], far[q], mu[q]);
+        if (unpulled<q>(s, p, x)) b.lam_out[(size_t)q * n + gid] = 0.0f;
     });
 }
 
@@ -113,12 +154,12 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
         src[q] = source_of<q>(s, g, x, y, z, far[q]);
-        if (orphan<q>(s)) b.lam_out[(size_t)q * n + gid] = 0.0f;
+        if (unpulled<q>(s, p, x)) b.lam_out[(size_t)q * n + gid] = 0.0f;
     });
     if (is_solid) {
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            if (!far[q]) b.lam_out[(size_t)q * n + src[q]] = 0.0f;
+            write_pull<q>(b, p, x, y, z, src[q], far[q], 0.0f);
             nu_cell[q * 256] = 0.0f;
         });
         return;
@@ -139,7 +180,7 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
         collision_adjoint<COLL>(gq, p.omega_plus, p.omega_minus, lp, mu);
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            if (!far[q]) b.lam_out[(size_t)q * n + src[q]] = 0.0f;
+            write_pull<q>(b, p, x, y, z, src[q], far[q], 0.0f);
             nu_cell[q * 256] = mu[q];
         });
         return;
@@ -194,7 +235,7 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
     collision_adjoint<COLL>(gq, p.omega_plus, p.omega_minus, lp, mu);
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if (!far[q]) b.lam_out[(size_t)q * n + src[q]] = link[q] ? 0.0f : mu[q];
+        write_pull<q>(b, p, x, y, z, src[q], far[q], link[q] ? 0.0f : mu[q]);
         nu_cell[q * 256] = link[q] ? mu[q] + load_dv[q] : 0.0f;
     });
 }
@@ -211,6 +252,29 @@ __global__ __launch_bounds__(256) void adjoint_general_p1(AdjointBuffers b, Step
         if (x < p.g.nx && y < p.g.ny)
             general_p1_cell<COLL>(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x);
     }
+}
+
+// Outlet x faces: adds each face cell's clamped pull (held in the face
+// buffer by the puller) to its target in the same plane, after the regular
+// pullers have set the targets.  One thread per (face cell, direction).
+__global__ __launch_bounds__(256) void adjoint_outlet(AdjointBuffers b, StepParams p) {
+    const Grid& g = p.g;
+    const uint32_t plane = (uint32_t)g.ny * (uint32_t)g.nz;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2u * 19u * plane) return;
+    const int face = (int)(i / (19u * plane));
+    const uint32_t r = i - (uint32_t)face * 19u * plane;
+    const int q = (int)(r / plane);
+    const uint32_t yz = r - (uint32_t)q * plane;
+    const int y = (int)(yz % (uint32_t)g.ny), z = (int)(yz / (uint32_t)g.ny);
+    const int x = face == 1 ? g.nx - 1 : 0;
+    if (p.bc[0][face] != BC_OUTLET || cx(q) != (face == 1 ? -1 : 1)) return;
+    bool far = false;
+    const int ys = shift_axis(y, cy(q), g.ny, p.bc[1][0], p.bc[1][1], far);
+    const int zs = shift_axis(z, cz(q), g.nz, p.bc[2][0], p.bc[2][1], far);
+    if (far) return;   // the pull was replaced by the far-field equilibrium
+    const uint32_t target = (uint32_t)x + (uint32_t)g.nx * ((uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs);
+    b.lam_out[(size_t)q * g.n + target] += b.outlet_extra[outlet_slot(g, face, q, y, z)];
 }
 
 // Sum of 8 FP64 lanes over the workgroup (fixed tree).
@@ -681,6 +745,11 @@ static hipError_t adjoint_step_t(const AdjointBuffers& b, const StepParams& p, c
                                  int general_blocks, hipStream_t s) {
     hipLaunchKernelGGL(adjoint_fast<COLL>, dim3(p.g.tiles), dim3(256), 0, s, b, p);
     hipLaunchKernelGGL(adjoint_general_p1<COLL>, dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    if (p.bc[0][0] == BC_OUTLET || p.bc[0][1] == BC_OUTLET) {
+        if (!b.outlet_extra) return hipErrorInvalidValue;
+        const uint32_t threads = 2u * 19u * (uint32_t)p.g.ny * (uint32_t)p.g.nz;
+        hipLaunchKernelGGL(adjoint_outlet, dim3((threads + 255) / 256), dim3(256), 0, s, b, p);
+    }
     hipLaunchKernelGGL(adjoint_general_p2a, dim3(general_blocks), dim3(256), 0, s, b, p, w);
     hipLaunchKernelGGL(adjoint_general_p2b, dim3(general_blocks), dim3(256), 0, s, b, p, w);
     hipLaunchKernelGGL(reduce_gradient, dim3(1), dim3(256), 0, s, b.grad_partials, b.general_count, b.grad_total);

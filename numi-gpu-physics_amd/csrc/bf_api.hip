@@ -123,6 +123,18 @@ struct bf_context {
     uint32_t m_frames = 0, m_vertices = 0, m_triangles = 0;
     float m_half = 0.0f;
     float m_translation[3] = {0, 0, 0};
+    // static triangles (the same vertex positions in every frame): their keys
+    // within the stepping band are cached once as (cell, key) pairs and seeded
+    // each step, and the raster walks the moving triangles only
+    uint32_t* m_dyn_list = nullptr;        // ascending ids of the moving triangles
+    uint32_t* m_static_list = nullptr;     // ascending ids of the static ones
+    uint32_t m_dyn_count = 0, m_static_count = 0;
+    bool static_keys_enabled = true;       // BIRDFLOW_STATIC_KEYS=0 rasters every triangle every step
+    bool static_ready = false;
+    DeviceAlloc static_alloc;
+    uint32_t* static_cells = nullptr;
+    unsigned long long* static_keys = nullptr;
+    uint32_t static_key_count = 0;
     // momentum ledger: block partials and results (allocated on first use);
     // the last step's inputs are intact while no population writer ran
     DeviceAlloc ledger_alloc;
@@ -146,6 +158,17 @@ struct bf_context {
     DeviceAlloc keyframe_alloc;           // measured wing keyframes, 17 floats each
     float* bird_keyframes = nullptr;
     uint32_t bird_keyframe_count = 0;
+    // prescribed-wing dynamics of the analytic bird: hinge reaction added to
+    // the body's load in free flight
+    bool wing_dynamics = false;
+    WingDynamics wing_dyn{};
+    // fluid health: the non-finite population guard of the fluid kernels
+    // ([0] flags, [1] first step) and the population-minimum reduction
+    unsigned long long* health = nullptr;
+    bool health_reported = false;
+    PopulationMinimum* popmin = nullptr;  // [POPULATION_MINIMUM_BLOCKS + 1]
+    float* wing_momentum = nullptr;       // [16] previous frame's wing momenta
+    float* wing_reaction = nullptr;       // [2][16] reaction of step s in slot s & 1
     // sphere
     float4 sphere_cr{};
     float3 sphere_gv{};
@@ -194,6 +217,7 @@ struct bf_context {
         hipStream_t sc = nullptr;                     // wall tangents (third stream, default priority)
         hipEvent_t ev_tan[2] = {nullptr, nullptr};    // dW[j] written (stream C)
         hipEvent_t ev_used[2] = {nullptr, nullptr};   // dW[j] read (stream B)
+        float* outlet_extra = nullptr;    // [2][19][ny nz]
         float4* dpos = nullptr;           // [8][vertices]
         float4* dvel = nullptr;
         double* grad_partials = nullptr;  // [tiles][8]
@@ -220,6 +244,7 @@ struct bf_context {
     uint32_t raster_blocks = 1u << 30;
     uint32_t raster_batch = 0;            // triangles per raster wave; 0: from the mesh size
     int fast_priority = 0, general_priority = 0;   // s_setprio of the fluid kernels' wavefronts
+    bool pair_cells = true;                        // 2-byte storage: fast tiles two cells per lane
     bool geometry_after_general = true;            // stream B starts the next geometry after general(s)
     uint32_t raster_batch_cap = 32;                // largest automatic raster batch
     uint32_t compute_units = 1;
@@ -266,6 +291,7 @@ StepParams make_step_params(const bf_context* c) {
     p.g = c->g;
     p.fast_priority = c->fast_priority;
     p.general_priority = c->general_priority;
+    p.pair_cells = c->pair_cells ? 1 : 0;
     for (int a = 0; a < 3; ++a)
         for (int s = 0; s < 2; ++s) p.bc[a][s] = (int)c->cfg.boundary[a][s];
     p.wall_rule = (int)c->cfg.wall_rule;
@@ -323,6 +349,19 @@ StepGeometry geometry_of(const bf_context* c, int64_t s) {
     return {c->mask[mod3(s)], c->mask[mod3(s - 1)], c->wall[par2(s)], c->dyn[mod3(s)], c->dyn[mod3(s - 1)]};
 }
 
+// Raster launch shape for `count` triangles: enough batches for two waves
+// per CU, at most raster_batch_cap triangles each (one gather round trip
+// serves many triangles of a large mesh, while a small mesh of large
+// triangles spreads one triangle per wave); a mesh of fewer triangles than
+// that splits each triangle's box among several wavefronts.
+void set_raster_shape(const bf_context* c, MeshParams& m, uint32_t count) {
+    m.raster_count = count;
+    m.raster_batch = c->raster_batch ? c->raster_batch
+                                     : std::min(c->raster_batch_cap, std::max(1u, count / (2u * c->compute_units)));
+    const uint32_t batches = (count + m.raster_batch - 1) / std::max(1u, m.raster_batch);
+    m.raster_split = std::min(64u, std::max(1u, (2u * c->compute_units) / std::max(1u, batches)));
+}
+
 MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     MeshParams m{};
     m.g = c->g;
@@ -338,24 +377,14 @@ MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     m.frame_count = c->m_frames;
     m.vertex_count = c->m_vertices;
     m.triangle_count = c->m_triangles;
+    m.raster_count = c->m_triangles;
     // Stepping keys only the band that the fluid step reads: solid cells
     // (distance <= h), fluid cells one link away from them (<= h + sqrt(2)),
     // and cells uncovered this step (<= h + one step's wall travel).
     m.cull_cells = exact ? INFINITY : c->m_half / c->cfg.cell_size + 1.5f;
     m.move_with_body = c->move_mesh ? 1 : 0;
     m.raster_blocks = c->raster_blocks;
-    // enough batches for two waves per CU, at most raster_batch_cap triangles
-    // each: one gather round trip serves many triangles of a large mesh, while
-    // a small mesh of large triangles spreads one triangle per wave
-    m.raster_batch = c->raster_batch ? c->raster_batch
-                                     : std::min(c->raster_batch_cap,
-                                                std::max(1u, c->m_triangles / (2u * c->compute_units)));
-    // a mesh of fewer triangles than that splits each triangle's box among
-    // several wavefronts
-    {
-        const uint32_t batches = (c->m_triangles + m.raster_batch - 1) / std::max(1u, m.raster_batch);
-        m.raster_split = std::min(64u, std::max(1u, (2u * c->compute_units) / std::max(1u, batches)));
-    }
+    set_raster_shape(c, m, c->m_triangles);
     m.tile_pass_blocks = c->tile_pass_blocks;
     // frame interval, MeasuredBirdSurfaceSequence.swift:174-204
     const std::vector<float>& ts = c->host_times;
@@ -396,6 +425,61 @@ hipError_t replay_mask(bf_context* c, int64_t s, uint32_t k, int slot) {
     return e;
 }
 
+bool use_static_keys(const bf_context* c) {
+    return c->static_keys_enabled && c->m_static_count > 0 && !c->move_mesh;
+}
+
+// The cached keys of the static triangles within the stepping band: the
+// static triangles rastered alone into the (empty) key grid, then collected
+// into (cell, key) pairs, which empties the grid again.  Once per mesh
+// upload; synchronizes stream B.
+bf_status ensure_static_keys(bf_context* c) {
+    if (c->static_ready) return BF_OK;
+    hipStream_t st = c->sb;
+    DeviceAlloc& A = c->static_alloc;
+    A.release();
+    c->static_cells = nullptr;
+    c->static_keys = nullptr;
+    hipError_t e = hipSuccess;
+    uint8_t* tiles = (uint8_t*)A.take(e, c->g.tiles);
+    unsigned long long* counter = (unsigned long long*)A.take(e, 8);
+    if (e != hipSuccess) {
+        A.release();
+        return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("static key cache: ") + hipGetErrorString(e));
+    }
+    MeshParams m = mesh_params(c, time_of(c, c->step), false);
+    set_raster_shape(c, m, c->m_static_count);
+    MeshBuffers b{};
+    b.body = c->body;
+    b.points = c->m_points;
+    b.times = c->m_times;
+    b.tri = c->m_tri;
+    b.tri_part = c->m_part;
+    b.pos = c->m_pos;
+    b.vel = c->m_vel;
+    b.keys = c->keys;
+    b.tiles_new = tiles;
+    b.raster_list = c->m_static_list;
+    e = launch_mesh_static_keys(b, m, counter, st);
+    unsigned long long n = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, counter, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(c, e, "static key raster");
+    if (n > c->g.n) return fail(c, BF_ERROR_LAUNCH_FAILED, "static key count exceeds the grid");
+    c->static_cells = (uint32_t*)A.take(e, 4 * std::max<unsigned long long>(n, 1));
+    c->static_keys = (unsigned long long*)A.take(e, 8 * std::max<unsigned long long>(n, 1));
+    if (e != hipSuccess) {
+        A.release();
+        return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("static key cache: ") + hipGetErrorString(e));
+    }
+    e = launch_collect_keys(c->keys, c->g.n, c->static_cells, c->static_keys, (uint32_t)n, counter, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(c, e, "static key collection");
+    c->static_key_count = (uint32_t)n;
+    c->static_ready = true;
+    return BF_OK;
+}
+
 bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
     hipError_t e = hipSuccess;
     if (c->source == BF_GEOMETRY_MESH) {
@@ -416,7 +500,16 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         b.mask = c->mask[mod3(s)];
         b.wall = c->wall[par2(s)];
         b.dyn = c->dyn[mod3(s)];
-        e = launch_mesh_geometry(b, mesh_params(c, time_of(c, s), exact), c->sb);
+        MeshParams m = mesh_params(c, time_of(c, s), exact);
+        if (!exact && use_static_keys(c)) {
+            if (bf_status r = ensure_static_keys(c)) return r;
+            set_raster_shape(c, m, c->m_dyn_count);
+            b.raster_list = c->m_dyn_list;
+            b.seed_cells = c->static_cells;
+            b.seed_keys = c->static_keys;
+            b.seed_count = c->static_key_count;
+        }
+        e = launch_mesh_geometry(b, m, c->sb);
         if (e == hipSuccess && c->frozen_count && s >= c->frozen_first && s < c->frozen_first + c->frozen_count)
             e = replay_mask(c, s, (uint32_t)(s - c->frozen_first), mod3(s));
     } else if (c->source == BF_GEOMETRY_BIRD) {
@@ -447,7 +540,14 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         bp.time = time_of(c, s);
         bp.velocity_scale = c->cfg.velocity_to_lattice;
         bp.tile_pass_blocks = 4u * c->tile_pass_blocks;
-        e = launch_bird_geometry(c->bird, b, bp, c->sb);
+        WingDynamics wd = c->wing_dyn;
+        if (c->wing_dynamics) {
+            b.wing_momentum = c->wing_momentum;
+            b.wing_reaction = c->wing_reaction + 16 * par2(s);
+            wd.dt = c->cfg.time_step;
+            wd.initialize = fresh ? 1 : 0;
+        }
+        e = launch_bird_geometry(c->bird, b, bp, c->sb, c->wing_dynamics ? &wd : nullptr);
     } else if (c->source == BF_GEOMETRY_WING) {
         WingBuffers b{};
         b.frame = c->wing_frame + WING_FRAME_ROWS * par2(s);
@@ -467,6 +567,8 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         for (int a = 0; a < 3; ++a) {
             bp.origin[a] = c->cfg.origin[a];
             bp.periodic[a] = periodic_axis(c->cfg, a) ? 1 : 0;
+            bp.bc[a][0] = (int)c->cfg.boundary[a][0];
+            bp.bc[a][1] = (int)c->cfg.boundary[a][1];
         }
         bp.dx = c->cfg.cell_size;
         bp.time = time_of(c, s);
@@ -600,7 +702,11 @@ bf_status bf_device_info(char* name, size_t cap, uint64_t* total) {
     return BF_OK;
 }
 
-uint64_t bf_allocated_bytes(const bf_context* c) { return c ? c->alloc.bytes + c->mesh_alloc.bytes : 0; }
+uint64_t bf_allocated_bytes(const bf_context* c) {
+    if (!c) return 0;
+    return c->alloc.bytes + c->mesh_alloc.bytes + c->static_alloc.bytes + c->ledger_alloc.bytes + c->wing_alloc.bytes +
+           c->keyframe_alloc.bytes + c->adj.alloc.bytes + c->adj.tangent_alloc.bytes + c->frozen_alloc.bytes;
+}
 
 bf_status bf_create(const bf_config* cfg, bf_context** out) {
     if (!cfg || !out) return BF_ERROR_INVALID_ARGUMENT;
@@ -667,6 +773,10 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     c->geometry_after_general = fp32;
     c->raster_batch_cap = fp32 ? 32u : 16u;
     if (const char* v = std::getenv("BIRDFLOW_FAST_PRIORITY")) c->fast_priority = std::min(3, std::max(0, std::atoi(v)));
+    // FP16 / BF16: fast tiles two cells per lane (0 selects one cell per lane)
+    if (const char* v = std::getenv("BIRDFLOW_PAIR_CELLS")) c->pair_cells = std::atoi(v) != 0;
+    // meshes: cached keys of static triangles (0 rasters every triangle every step)
+    if (const char* v = std::getenv("BIRDFLOW_STATIC_KEYS")) c->static_keys_enabled = std::atoi(v) != 0;
     if (const char* v = std::getenv("BIRDFLOW_GENERAL_PRIORITY"))
         c->general_priority = std::min(3, std::max(0, std::atoi(v)));
     if (const char* v = std::getenv("BIRDFLOW_GEOMETRY_AFTER_GENERAL")) c->geometry_after_general = std::atoi(v) != 0;
@@ -713,6 +823,10 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     for (int i = 0; i < 2; ++i) c->general_list[i] = (uint32_t*)A.take(e, 4ull * g.tiles);
     c->geo_list = (uint32_t*)A.take(e, 4ull * g.tiles);
     c->bird_frame = (float4*)A.take(e, 2ull * BIRD_FRAME_ROWS * sizeof(float4));
+    c->wing_momentum = (float*)A.take(e, 16 * sizeof(float));
+    c->health = (unsigned long long*)A.take(e, 2 * sizeof(unsigned long long));
+    c->popmin = (PopulationMinimum*)A.take(e, (POPULATION_MINIMUM_BLOCKS + 1) * sizeof(PopulationMinimum));
+    c->wing_reaction = (float*)A.take(e, 32 * sizeof(float));
     c->counters = (uint32_t*)A.take(e, 64);
     c->partials = (double*)A.take(e, 128ull * g.tiles);   // two steps' per-tile load partials
     c->block_out = (double*)A.take(e, 64ull * ((g.tiles + 1023) / 1024 + 1));
@@ -740,6 +854,10 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     if (e == hipSuccess) e = launch_fill_u32(c->counters, 0u, 16u, s);
     if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->last_load, 0, 64, s);
     if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->partials, 0, (uint32_t)(128ull * g.tiles), s);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->wing_momentum, 0, 16 * sizeof(float), s);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->health, 0, 8, s);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->health + 8, 0xFF, 8, s);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->wing_reaction, 0, 32 * sizeof(float), s);
     // body at rest at the torque reference point, identity orientation
     const double body0[16] = {cfg->body_position[0], cfg->body_position[1], cfg->body_position[2], 0.0, 0.0, 0.0,
                               1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -774,6 +892,7 @@ void bf_destroy(bf_context* c) {
         if (c->adj.ev_used[i]) (void)hipEventDestroy(c->adj.ev_used[i]);
     }
     c->mesh_alloc.release();
+    c->static_alloc.release();
     c->ledger_alloc.release();
     c->keyframe_alloc.release();
     c->wing_alloc.release();
@@ -846,9 +965,18 @@ bf_status bf_set_restart_state(bf_context* c, uint64_t step, float mesh_time_ori
     return BF_OK;
 }
 
+// A new population state clears the non-finite population record.
+static hipError_t reset_health(bf_context* c) {
+    c->health_reported = false;
+    hipError_t e = launch_fill_u8((uint8_t*)c->health, 0, 8, c->sb);
+    if (e == hipSuccess) e = launch_fill_u8((uint8_t*)c->health + 8, 0xFF, 8, c->sb);
+    return e;
+}
+
 bf_status bf_init_equilibrium(bf_context* c) {
     BF_ENTER(c);
     ++c->f_writes;
+    if (hipError_t e0 = reset_health(c)) return hip_fail(c, e0, "reset health");
     const StepGeometry sg = geometry_of(c, c->step);
     hipError_t e = launch_init_equilibrium((int)c->cfg.storage, c->f[c->cur], sg.mask_cur, sg.wall,
                                            c->cfg.far_velocity, c->g, c->sb);
@@ -860,6 +988,7 @@ bf_status bf_init_equilibrium(bf_context* c) {
 bf_status bf_init_shear_wave(bf_context* c, float amplitude) {
     BF_ENTER(c);
     ++c->f_writes;
+    if (hipError_t e0 = reset_health(c)) return hip_fail(c, e0, "reset health");
     hipError_t e = launch_init_shear_wave((int)c->cfg.storage, c->f[c->cur], amplitude, c->g, c->sb);
     if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "initialize shear wave");
@@ -882,6 +1011,7 @@ static bf_status ensure_staging(bf_context* c) {
 bf_status bf_write_populations(bf_context* c, const float* host, uint64_t count) {
     BF_ENTER(c);
     ++c->f_writes;
+    if (hipError_t e0 = reset_health(c)) return hip_fail(c, e0, "reset health");
     if (check_host_array(c, host, "populations")) return BF_ERROR_INVALID_ARGUMENT;
     if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population count must be 19 * N");
     // through the staging buffer: the device layout differs from the host's
@@ -914,6 +1044,7 @@ static uint64_t stored_bytes(const bf_context* c) {
 bf_status bf_write_population_storage(bf_context* c, const void* host, uint64_t bytes) {
     BF_ENTER(c);
     ++c->f_writes;
+    if (hipError_t e0 = reset_health(c)) return hip_fail(c, e0, "reset health");
     if (check_host_array(c, host, "population storage")) return BF_ERROR_INVALID_ARGUMENT;
     if (bytes != stored_bytes(c))
         return fail(c, BF_ERROR_INVALID_ARGUMENT, "population storage size must be 19 * N * element size");
@@ -1043,7 +1174,29 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     }
     if (!(half_thickness > 0.0f)) return fail(c, BF_ERROR_INVALID_ARGUMENT, "half thickness must be positive");
 
+    // Static triangles: every vertex at the same position in every frame
+    // (their keys are the same every step unless a moving body carries the
+    // mesh); ids ascending, moving ones first in their own list.
+    std::vector<uint8_t> vertex_static(vertex_count, 1);
+    for (uint32_t f = 1; f < frame_count; ++f)
+        for (uint32_t v = 0; v < vertex_count; ++v) {
+            const float* p0 = points + 3ull * v;
+            const float* pf = points + 3ull * ((uint64_t)f * vertex_count + v);
+            if (std::memcmp(p0, pf, 12) != 0) vertex_static[v] = 0;
+        }
+    std::vector<uint32_t> dyn_list, static_list;
+    for (uint32_t t = 0; t < triangle_count; ++t) {
+        const uint32_t* v = triangles + 3ull * t;
+        if (vertex_static[v[0]] && vertex_static[v[1]] && vertex_static[v[2]]) static_list.push_back(t);
+        else dyn_list.push_back(t);
+    }
+
     c->mesh_alloc.release();
+    c->static_alloc.release();
+    c->static_ready = false;
+    c->static_cells = nullptr;
+    c->static_keys = nullptr;
+    c->static_key_count = 0;
     c->adj.tangent_alloc.release();   // tangents belong to the previous mesh
     c->adj.K = 0;
     c->adj.dpoints = nullptr;
@@ -1057,6 +1210,8 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     c->m_pos = (float4*)A.take(e, 16ull * vertex_count);
     c->m_vel = (float4*)A.take(e, 16ull * vertex_count);
     c->keys = (unsigned long long*)A.take(e, 8ull * c->g.n);
+    c->m_dyn_list = (uint32_t*)A.take(e, 4ull * std::max<size_t>(dyn_list.size(), 1));
+    c->m_static_list = (uint32_t*)A.take(e, 4ull * std::max<size_t>(static_list.size(), 1));
     if (e != hipSuccess) {
         A.release();
         c->source = BF_GEOMETRY_STATIC;
@@ -1067,7 +1222,15 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     if (e == hipSuccess) e = hipMemcpyAsync(c->m_times, frame_times, 4ull * frame_count, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->m_tri, triangles, 12ull * triangle_count, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->m_part, triangle_parts, triangle_count, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !dyn_list.empty())
+        e = hipMemcpyAsync(c->m_dyn_list, dyn_list.data(), 4 * dyn_list.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !static_list.empty())
+        e = hipMemcpyAsync(c->m_static_list, static_list.data(), 4 * static_list.size(), hipMemcpyHostToDevice, s);
+    // the host lists are read by the copies: finish them before they go
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(c, e, "mesh upload");
+    c->m_dyn_count = (uint32_t)dyn_list.size();
+    c->m_static_count = (uint32_t)static_list.size();
     c->host_times.assign(frame_times, frame_times + frame_count);
     c->m_frames = frame_count;
     c->m_vertices = vertex_count;
@@ -1302,6 +1465,45 @@ bf_status bf_set_bird_keyframes(bf_context* c, const float* keyframes, uint32_t 
     return c->source == BF_GEOMETRY_BIRD ? rebuild_bird_geometry(c) : BF_OK;
 }
 
+bf_status bf_set_bird_wing_dynamics(bf_context* c, const bf_wing_dynamics* left, const bf_wing_dynamics* right) {
+    BF_ENTER(c);
+    if ((left == nullptr) != (right == nullptr))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "give both wings' dynamics, or neither");
+    if (!left) {
+        c->wing_dynamics = false;
+        return BF_OK;
+    }
+    const bf_wing_dynamics* w[2] = {left, right};
+    for (int k = 0; k < 2; ++k) {
+        bool ok = std::isfinite(w[k]->mass) && w[k]->mass >= 0.0f;
+        for (int i = 0; i < 3; ++i)
+            ok = ok && std::isfinite(w[k]->center_of_mass[i]) && std::isfinite(w[k]->inertia[i]) &&
+                 w[k]->inertia[i] >= 0.0f;
+        if (!ok)
+            return fail(c, BF_ERROR_INVALID_ARGUMENT,
+                        "wing mass and inertia must be finite and non-negative, the centre of mass finite");
+        c->wing_dyn.mass[k] = w[k]->mass;
+        for (int i = 0; i < 3; ++i) {
+            c->wing_dyn.center_of_mass[k][i] = w[k]->center_of_mass[i];
+            c->wing_dyn.inertia[k][i] = w[k]->inertia[i];
+        }
+    }
+    c->wing_dynamics = true;
+    // the current frame's momenta start the history (no reaction on it)
+    return c->source == BF_GEOMETRY_BIRD ? rebuild_bird_geometry(c) : BF_OK;
+}
+
+bf_status bf_read_wing_reaction(bf_context* c, float reaction[16]) {
+    BF_ENTER(c);
+    if (!reaction) return fail(c, BF_ERROR_INVALID_ARGUMENT, "reaction must not be NULL");
+    if (!c->wing_dynamics || c->source != BF_GEOMETRY_BIRD)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "no analytic bird with wing dynamics set");
+    hipError_t e = hipMemcpy(reaction, c->wing_reaction + 16 * par2(c->step), 16 * sizeof(float),
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read wing reaction");
+    return BF_OK;
+}
+
 bf_status bf_read_bird_frame(bf_context* c, float frame[56]) {
     BF_ENTER(c);
     if (!frame) return fail(c, BF_ERROR_INVALID_ARGUMENT, "frame must not be NULL");
@@ -1416,6 +1618,8 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         StepParams p = make_step_params(c);
         // the link table of step s; geometry(s + 1) fills the other slot meanwhile
         p.links = c->wing_links[par2(s)];
+        p.health = c->health;
+        p.step_index = (unsigned long long)s;
         const FluidBuffers fb = fluid_buffers(c, s);
         c->ledger_prev = fb.solid_prev;
         if (c->source == BF_GEOMETRY_STATIC) c->static_prev_differs = false;
@@ -1475,7 +1679,9 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         if (e == hipSuccess) e = launch_load_reduction(fb, p, (uint32_t)(s % HISTORY_CAPACITY), c->sb);
         if (e == hipSuccess && c->free_flight)
             e = launch_body_step(c->body, c->last_load, c->body_history, (uint32_t)(s % HISTORY_CAPACITY),
-                                 c->safety_metrics, c->safety_event, (uint64_t)s, c->body_params, c->sb);
+                                 c->safety_metrics, c->safety_event, (uint64_t)s, c->body_params, c->sb,
+                                 c->wing_dynamics && c->source == BF_GEOMETRY_BIRD ? c->wing_reaction + 16 * par2(s)
+                                                                                   : nullptr);
         if (e != hipSuccess) return hip_fail(c, e, "fluid step launch");
         if (coupled) {
             if (bf_status r = next_geometry()) return r;
@@ -1504,6 +1710,21 @@ bf_status bf_synchronize(bf_context* c) {
     (void)hipSetDevice(c->device);
     hipError_t e = drain(c);
     if (e != hipSuccess) return hip_fail(c, e, "synchronize");
+    if (!c->health_reported) {
+        // a non-finite population is reported once, with the first step
+        // that stored one
+        unsigned long long h[2];
+        e = hipMemcpy(h, c->health, sizeof h, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(c, e, "read fluid health");
+        if (h[0] != 0ull) {
+            c->health_reported = true;
+            char msg[160];
+            std::snprintf(msg, sizeof msg, "runtime safety violation: non-finite populations from step %llu",
+                          (unsigned long long)h[1]);
+            c->error = msg;
+            return BF_ERROR_RUNTIME_SAFETY_VIOLATION;
+        }
+    }
     if (c->free_flight && c->body_params.monitor) {
         // a new runtime-safety event is reported once (the reference throws
         // runtimeSafetyViolation after the batch, BirdFlowSimulation.swift:735-741)
@@ -1628,6 +1849,41 @@ bf_status bf_read_safety(bf_context* c, double metrics[4], uint64_t* event_step,
     if (e != hipSuccess) return hip_fail(c, e, "read safety record");
     *event_step = (uint64_t)ev[0] | ((uint64_t)ev[1] << 32);
     *flags = ev[2];
+    unsigned long long h[2];
+    e = hipMemcpy(h, c->health, sizeof h, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read fluid health");
+    if (h[0] != 0ull) {
+        if (*flags == 0u) *event_step = (uint64_t)h[1];
+        *flags |= 8u;
+    }
+    return BF_OK;
+}
+
+bf_status bf_population_minimum(bf_context* c, float* value, uint64_t* index, uint32_t* non_finite) {
+    BF_ENTER(c);
+    if (!value || !index || !non_finite) return fail(c, BF_ERROR_INVALID_ARGUMENT, "outputs must not be NULL");
+    const uint64_t total = 19ull * c->g.n;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(POPULATION_MINIMUM_BLOCKS, (total + 255) / 256);
+    hipError_t e = launch_population_minimum((int)c->cfg.storage, c->f[c->cur], c->g, c->popmin, blocks,
+                                             c->popmin + POPULATION_MINIMUM_BLOCKS, c->sb);
+    PopulationMinimum r{};
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&r, c->popmin + POPULATION_MINIMUM_BLOCKS, sizeof r, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "population minimum");
+    *value = r.value;
+    *index = r.index;
+    *non_finite = r.non_finite;
+    return BF_OK;
+}
+
+bf_status bf_inject_population(bf_context* c, uint64_t index, float value) {
+    BF_ENTER(c);
+    if (index >= 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "population index beyond 19 * N");
+    ++c->f_writes;
+    hipError_t e = launch_poke_population((int)c->cfg.storage, c->f[c->cur], c->g, index, value, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "inject population");
     return BF_OK;
 }
 
@@ -1644,10 +1900,13 @@ bf_status adjoint_supported(bf_context* c) {
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers TRT, regularized and RR3 collisions");
     if (g.force_mode != 6) return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates force mode 6");
     if (!g.accumulate_loads) return fail(c, BF_ERROR_INVALID_ARGUMENT, "the objective needs accumulate_loads");
-    for (int a = 0; a < 3; ++a)
+    for (int a = 1; a < 3; ++a)
         for (int sd = 0; sd < 2; ++sd)
             if (g.boundary[a][sd] == BF_BC_OUTLET)
-                return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers periodic and far-field faces");
+                return fail(c, BF_ERROR_NOT_SUPPORTED,
+                            "the adjoint covers periodic and far-field faces, and outlets on the x faces");
+    if ((g.boundary[0][0] == BF_BC_OUTLET || g.boundary[0][1] == BF_BC_OUTLET) && g.nx < 2)
+        return fail(c, BF_ERROR_NOT_SUPPORTED, "an outlet face needs at least two cells along x");
     if (c->free_flight) return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint holds the body fixed (no free flight)");
     if (c->source == BF_GEOMETRY_SPHERE || c->source == BF_GEOMETRY_BIRD || c->source == BF_GEOMETRY_WING ||
         g.wall_rule == BF_WALL_INTERPOLATED_LINKS)
@@ -1665,7 +1924,7 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     const uint64_t verts = std::max<uint32_t>(c->m_vertices, 1u);
     const uint64_t need = pop * segments + pop * interval + 2 * (uint64_t)(interval + 1) * (21 * n + 7 * tiles + 4) +
                           2 * pop + tiles * 19 * 256 * 4 + 2 * 8 * 16 * n + 2 * 8 * 16 * verts + 2 * 16 * verts +
-                          tiles * 64 + 4 * tiles + 1024;
+                          tiles * 64 + 4 * tiles + 2ull * 19 * c->g.ny * c->g.nz * 4 + 1024;
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     if (need > (uint64_t)free_b) {
@@ -1709,6 +1968,8 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     a.dvel = (float4*)A.take(e, 8 * 16 * verts);
     a.grad_partials = (double*)A.take(e, tiles * 64);
     a.grad_total = (double*)A.take(e, 64);
+    // clamped pulls of the outlet x faces: [face][19][ny nz]
+    a.outlet_extra = (float*)A.take(e, 2ull * 19 * c->g.ny * c->g.nz * 4);
     if (e != hipSuccess) {
         A.release();
         a.checkpoints = nullptr;
@@ -1984,6 +2245,7 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
             ab.grad_partials = a.grad_partials;
             ab.grad_total = a.grad_total;
             ab.body = c->body;
+            ab.outlet_extra = a.outlet_extra;
             AdjointParams w{};
             const double* lw = load_weights + 6 * (size_t)(s - s0 - 1);
             for (int i = 0; i < 3; ++i) {

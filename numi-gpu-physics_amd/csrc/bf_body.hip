@@ -38,7 +38,7 @@ __device__ void qrotate_d(const double q[4], const double v[3], double o[3]) {
 // state: position[3], orientation[4], linear_velocity[3], angular_velocity_body[3]
 __global__ void body_step(double* __restrict__ state, const double* __restrict__ load, double* __restrict__ history,
                           uint32_t history_slot, double* __restrict__ metrics, uint32_t* __restrict__ event,
-                          uint64_t step, BodyParams bp) {
+                          uint64_t step, BodyParams bp, const float* __restrict__ wing_reaction) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double x[3] = {state[0], state[1], state[2]};
     double q[4] = {state[3], state[4], state[5], state[6]};
@@ -48,6 +48,11 @@ __global__ void body_step(double* __restrict__ state, const double* __restrict__
     for (int i = 0; i < 3; ++i) {
         force[i] = load[i] + bp.external_force[i];
         torque[i] = load[3 + i] + bp.external_torque[i];
+        if (wing_reaction) {
+            // the prescribed wings' hinge reactions (BirdFlow.metal:6586-6591)
+            force[i] = force[i] + (double)wing_reaction[i] + (double)wing_reaction[8 + i];
+            torque[i] = torque[i] + (double)wing_reaction[4 + i] + (double)wing_reaction[12 + i];
+        }
     }
     const int substeps = bp.substeps < 1 ? 1 : bp.substeps;
     const double dt = bp.dt / (double)substeps;
@@ -116,9 +121,10 @@ __global__ void body_step(double* __restrict__ state, const double* __restrict__
 }
 
 hipError_t launch_body_step(double* state, const double* load, double* history, uint32_t history_slot,
-                            double* metrics, uint32_t* event, uint64_t step, const BodyParams& bp, hipStream_t s) {
+                            double* metrics, uint32_t* event, uint64_t step, const BodyParams& bp, hipStream_t s,
+                            const float* wing_reaction) {
     hipLaunchKernelGGL(body_step, dim3(1), dim3(64), 0, s, state, load, history, history_slot, metrics, event, step,
-                       bp);
+                       bp, wing_reaction);
     return hipGetLastError();
 }
 

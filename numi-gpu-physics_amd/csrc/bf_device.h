@@ -80,6 +80,7 @@ struct StepParams {
     int accumulate_loads;
     int capture_fields;
     int fast_priority;         // s_setprio level of the fast kernel's wavefronts (0..3)
+    int pair_cells;            // 2-byte storage: fast tiles two cells per lane (fluid_fast_pair)
     int general_priority;      // and of the general kernel's
     float omega_plus, omega_minus;
     float sponge_strength, sponge_width;
@@ -88,19 +89,34 @@ struct StepParams {
     float force_scale;
     float far_eq[19];          // equilibrium(q, far_rho, far_u), FP32
     const float* links;        // WALL_INTERPOLATED_LINKS: fractions [tile][19][256] of fluid cells' links
+    // non-finite population guard (optional): [0] flags (1 = a cell's
+    // post-collision populations did not sum to a finite density), [1] the
+    // first step that saw one; step_index = this step
+    unsigned long long* health;
+    unsigned long long step_index;
 };
+
+// A cell whose stored populations do not sum to a finite density (NaN or
+// infinity among them) flags the step: sticky, the earliest step kept.
+// Per-lane atomics on the rare bad lane only.
+__device__ __forceinline__ void guard_finite(const StepParams& p, float density) {
+    if (p.health && !isfinite(density)) {
+        atomicOr(&p.health[0], 1ull);
+        atomicMin(&p.health[1], p.step_index);
+    }
+}
 
 // XCD-aware block order: the hardware deals workgroups round-robin over the
 // 8 XCDs; remap so each XCD walks one contiguous run of tiles (its L2 then
 // holds the cache lines that neighbouring tiles share).
-__device__ inline uint32_t xcd_remap(uint32_t block, uint32_t blocks) {
+__host__ __device__ inline uint32_t xcd_remap(uint32_t block, uint32_t blocks) {
     constexpr uint32_t XCDS = 8;
     uint32_t xcd = block % XCDS, local = block / XCDS;
     uint32_t per = blocks / XCDS, rem = blocks % XCDS;
     return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
-__device__ inline void tile_origin(const Grid& g, uint32_t tile, int& x0, int& y0, int& z) {
+__host__ __device__ inline void tile_origin(const Grid& g, uint32_t tile, int& x0, int& y0, int& z) {
     uint32_t per_plane = (uint32_t)g.tiles_x * (uint32_t)g.tiles_y;
     z = (int)(tile / per_plane);
     uint32_t r = tile - (uint32_t)z * per_plane;
@@ -110,7 +126,7 @@ __device__ inline void tile_origin(const Grid& g, uint32_t tile, int& x0, int& y
     y0 = ty * TILE_Y;
 }
 
-__device__ inline uint32_t tile_of(const Grid& g, int x, int y, int z) {
+__host__ __device__ inline uint32_t tile_of(const Grid& g, int x, int y, int z) {
     return ((uint32_t)z * (uint32_t)g.tiles_y + (uint32_t)(y / TILE_Y)) * (uint32_t)g.tiles_x
         + (uint32_t)(x / TILE_X);
 }
@@ -139,7 +155,7 @@ struct Store;
 // (FP32) and 9 % (FP16) faster than with direction-major planes.  The
 // host-facing layout is direction-major (the pack / unpack kernels convert).
 template <int S>
-__device__ __forceinline__ size_t pop_index(const Grid& g, int q, int x, uint32_t row) {
+__host__ __device__ __forceinline__ size_t pop_index(const Grid& g, int q, int x, uint32_t row) {
     return ((size_t)row * 19u + (size_t)q) * (size_t)g.nx + (size_t)x;
 }
 
@@ -167,13 +183,8 @@ template <>
 struct Store<STORE_BF16_OFFSET> {
     using T = uint16_t;  // bfloat16 bits
     __device__ static inline float to_float(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-    __device__ static inline uint16_t from_float(float f) {
-        // round to nearest even; NaN stays NaN
-        uint32_t u = __float_as_uint(f);
-        if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return (uint16_t)(u >> 16);
-    }
+    // round to nearest even in hardware (v_cvt_pk_bf16_f32 on gfx950)
+    __device__ static inline uint16_t from_float(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
     __device__ static inline float load(const T* __restrict__ p, size_t i, int q) {
         return to_float(p[i]) + weight<float>(q);
     }

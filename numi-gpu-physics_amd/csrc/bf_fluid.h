@@ -453,7 +453,7 @@ __device__ __forceinline__ void collide_fused(const float (&f)[19], float op, fl
 
 // Pull-source coordinate along one axis with the face treatment; sets far
 // when the far-field equilibrium replaces the population.
-__device__ __forceinline__ int shift_axis(int v, int d, int n, int bc_lo, int bc_hi, bool& far) {
+__host__ __device__ __forceinline__ int shift_axis(int v, int d, int n, int bc_lo, int bc_hi, bool& far) {
     int s = v - d;
     if (s < 0) {
         if (bc_lo == BC_PERIODIC) s = n - 1;
@@ -565,9 +565,12 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
     const bool general_tile = (static_flags[tile] | dyn_prev[tile] | dyn_cur[tile]) != 0;
     int x0, y0, z;
     tile_origin(p.g, tile, x0, y0, z);
-    const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
-    const int y = y0 + (int)(threadIdx.x / TILE_X);
     const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
+    // Lanes of a partial tile beyond the grid load from the last column / row
+    // (their tile is a general tile: nothing they load is used), so no load
+    // leaves the lattice; in whole tiles the clamp changes nothing.
+    const int x = min(x0 + (int)(threadIdx.x & (TILE_X - 1)), nx - 1);
+    const int y = min(y0 + (int)(threadIdx.x / TILE_X), ny - 1);
 
     bool xm_far = false, xp_far = false, ym_far = false, yp_far = false, zm_far = false, zp_far = false;
     // source coordinates for c = +1 (s = v - 1) and c = -1 (s = v + 1)
@@ -626,6 +629,7 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
         write_post<S>(fout, x, row_self, post, sponge_factor(p, x, y, z), p, crho, cmx, cmy, cmz);
     else
         write_post<S>(fout, x, row_self, post, 0.0f, p, crho, cmx, cmy, cmz);
+    guard_finite(p, crho);
     if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
     if constexpr (COLL != COLL_TRT) {
         // Limiter activations of fast cells: an exact count and the maximum
@@ -641,6 +645,230 @@ __global__ __launch_bounds__(256) void fluid_fast(const typename Store<S>::T* __
                 if ((threadIdx.x & 63) == 0) {
                     atomicAdd(&activation[0], (uint32_t)__popcll(hit));
                     atomicMax(&activation[1], __float_as_uint(r));
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fast tiles in 2-byte storage (FP16 / BF16 offsets): two cells per lane.
+//
+// One 256-lane workgroup per pair of x-adjacent tiles (128 x 4 cells); each
+// wavefront streams one 128-cell row, lane l the cells X0 + 2l and
+// X0 + 2l + 1, so every population load and store moves a 4-byte pair (256
+// bytes per wavefront instruction instead of 128) and a wavefront has twice
+// the bytes in flight per load.  Loads stay aligned: a direction with
+// c_x = +1 needs the pair (X0 + 2l - 1, X0 + 2l), whose first half is the
+// second half of the lane below's aligned pair (a DPP wavefront shift) and,
+// for lane 0, the element at X0 - 1, a wave-uniform address (one scalar
+// load); c_x = -1 mirrors this with the lane above and X0 + 128.  y / z
+// sources and their face treatment are wave-uniform.  Each cell's arithmetic
+// is exactly fluid_fast's, so a cell gets bit-identical results from either
+// kernel.  Needs nx a multiple of 128 (whole tile pairs, rows starting at
+// even elements; the launcher keeps fluid_fast otherwise); lanes of a tile
+// on the general path load but do not store, and the rows of a partial tile
+// beyond ny load from the last row.
+// ---------------------------------------------------------------------------
+template <int S>
+struct PairStore;
+
+template <>
+struct PairStore<STORE_FP16_OFFSET> {
+    __device__ static inline float lo(uint32_t u, int q) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu)) + weight<float>(q);
+    }
+    __device__ static inline float hi(uint32_t u, int q) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)(u >> 16)) + weight<float>(q);
+    }
+    __device__ static inline uint32_t pack(float a, float b, int q) {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const h2 h = {(_Float16)(a - weight<float>(q)), (_Float16)(b - weight<float>(q))};
+        return __builtin_bit_cast(uint32_t, h);
+    }
+};
+
+template <>
+struct PairStore<STORE_BF16_OFFSET> {
+    __device__ static inline float lo(uint32_t u, int q) { return __uint_as_float(u << 16) + weight<float>(q); }
+    __device__ static inline float hi(uint32_t u, int q) {
+        return __uint_as_float(u & 0xffff0000u) + weight<float>(q);
+    }
+    __device__ static inline uint32_t pack(float a, float b, int q) {
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        const b2 h = {(__bf16)(a - weight<float>(q)), (__bf16)(b - weight<float>(q))};
+        return __builtin_bit_cast(uint32_t, h);
+    }
+};
+
+template <int ARITH, int COLL, int S>
+__global__ __launch_bounds__(256) void fluid_fast_pair(const typename Store<S>::T* __restrict__ fin_t,
+                                                       typename Store<S>::T* __restrict__ fout_t,
+                                                       const uint8_t* __restrict__ static_flags,
+                                                       const uint8_t* __restrict__ dyn_prev,
+                                                       const uint8_t* __restrict__ dyn_cur,
+                                                       float* __restrict__ density, float4* __restrict__ velocity,
+                                                       uint32_t* __restrict__ activation, StepParams p) {
+    static_assert(S == STORE_FP16_OFFSET || S == STORE_BF16_OFFSET, "two-byte storage only");
+    set_priority(p.fast_priority);
+    const uint32_t* __restrict__ fin = reinterpret_cast<const uint32_t*>(fin_t);
+    uint32_t* __restrict__ fout = reinterpret_cast<uint32_t*>(fout_t);
+    const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
+    const uint32_t pairs_x = (uint32_t)p.g.tiles_x / 2u;
+    const uint32_t pair = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t per_plane = pairs_x * (uint32_t)p.g.tiles_y;
+    const int z = (int)(pair / per_plane);
+    const uint32_t r = pair - (uint32_t)z * per_plane;
+    const uint32_t ty = r / pairs_x, px = r - ty * pairs_x;
+    const uint32_t tile_a = ((uint32_t)z * (uint32_t)p.g.tiles_y + ty) * (uint32_t)p.g.tiles_x + 2u * px;
+    // the six flag bytes load with the populations and are tested after them
+    uint32_t flags_a = (uint32_t)static_flags[tile_a], flags_a1 = (uint32_t)dyn_prev[tile_a],
+             flags_a2 = (uint32_t)dyn_cur[tile_a];
+    uint32_t flags_b = (uint32_t)static_flags[tile_a + 1], flags_b1 = (uint32_t)dyn_prev[tile_a + 1],
+             flags_b2 = (uint32_t)dyn_cur[tile_a + 1];
+    const int lane = (int)(threadIdx.x & 63u);
+    const int wave = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int X0 = (int)px * 2 * TILE_X;
+    const int xa = X0 + 2 * lane;
+    const int y = min((int)ty * TILE_Y + wave, ny - 1);
+
+    // wave-uniform sources along y and z; the halo elements along x
+    bool ym_far = false, yp_far = false, zm_far = false, zp_far = false, xlo_far = false, xhi_far = false;
+    const int ys_p = shift_axis(y, 1, ny, p.bc[1][0], p.bc[1][1], ym_far);
+    const int ys_m = shift_axis(y, -1, ny, p.bc[1][0], p.bc[1][1], yp_far);
+    const int zs_p = shift_axis(z, 1, nz, p.bc[2][0], p.bc[2][1], zm_far);
+    const int zs_m = shift_axis(z, -1, nz, p.bc[2][0], p.bc[2][1], zp_far);
+    int xh_lo = shift_axis(X0, 1, nx, p.bc[0][0], p.bc[0][1], xlo_far);           // source of X0 along +x
+    int xh_hi = shift_axis(X0 + 2 * TILE_X - 1, -1, nx, p.bc[0][0], p.bc[0][1], xhi_far);   // of X0 + 127 along -x
+    if (xlo_far) xh_lo = X0;   // in-domain stand-in; the value is replaced
+    if (xhi_far) xh_hi = X0;
+    const uint32_t row_self = (uint32_t)y + (uint32_t)ny * (uint32_t)z;
+
+    uint32_t v[19];
+    uint32_t halo[19];         // the 4-byte word holding each x-direction's halo element
+    bool far_yz[19];
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const int ys = cy(q) > 0 ? ys_p : (cy(q) < 0 ? ys_m : y);
+        const int zs = cz(q) > 0 ? zs_p : (cz(q) < 0 ? zs_m : z);
+        const bool far = (cy(q) > 0 && ym_far) || (cy(q) < 0 && yp_far) || (cz(q) > 0 && zm_far) ||
+                         (cz(q) < 0 && zp_far);
+        const uint32_t row_src = far ? row_self : (uint32_t)ys + (uint32_t)ny * (uint32_t)zs;
+        const size_t base = pop_index<S>(p.g, q, 0, row_src);          // even: nx is even
+        v[q] = fin[(base + (size_t)X0) / 2 + (size_t)lane];
+        if constexpr (cx(q) != 0) halo[q] = fin[(base + (size_t)(cx(q) > 0 ? xh_lo : xh_hi)) / 2];
+        far_yz[q] = far;
+    });
+    // Every population, halo and flag load issues before the first wait:
+    // the empty asm statements consume all of them at once (otherwise the
+    // compiler extracts each halo half right after its scalar load, waiting
+    // on it, and tests the flags before the population loads).
+    asm volatile("" : BF_ASM19(v), "+s"(halo[1]), "+s"(halo[2]), "+s"(halo[7]), "+s"(halo[8]), "+s"(halo[9]),
+                 "+s"(halo[10]), "+s"(halo[11]), "+s"(halo[12]), "+s"(halo[13]), "+s"(halo[14]));
+    asm volatile("" : "+v"(flags_a), "+v"(flags_a1), "+v"(flags_a2), "+v"(flags_b), "+v"(flags_b1), "+v"(flags_b2));
+    const bool general_a = (flags_a | flags_a1 | flags_a2) != 0;
+    const bool general_b = (flags_b | flags_b1 | flags_b2) != 0;
+    if (general_a && general_b) return;
+    // the halo half within its word (population rows start at even elements)
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (cx(q) != 0) {
+            const int xh = cx(q) > 0 ? xh_lo : xh_hi;
+            halo[q] = (xh & 1) ? (halo[q] >> 16) : (halo[q] & 0xffffu);
+        }
+    });
+
+    // shift the x-neighbour halves into place
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (cx(q) > 0) {
+            const uint32_t below = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[q], 0x138, 0xf, 0xf, false);
+            const uint32_t first = lane == 0 ? halo[q] : (below >> 16);
+            v[q] = first | (v[q] << 16);
+        } else if constexpr (cx(q) < 0) {
+            const uint32_t above = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[q], 0x130, 0xf, 0xf, false);
+            const uint32_t second = lane == 63 ? halo[q] : (above & 0xffffu);
+            v[q] = (v[q] >> 16) | (second << 16);
+        }
+    });
+    const bool store = lane < 32 ? !general_a : !general_b;
+    const bool sponge_on = p.sponge_strength > 0.0f;
+    float act_n = 0.0f, restr_max = 0.0f;
+    uint32_t out[19];
+    float crho[2], cmx[2], cmy[2], cmz[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int x = xa + c;
+        float f[19];
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = c == 0 ? PairStore<S>::lo(v[q], q) : PairStore<S>::hi(v[q], q);
+            bool far = far_yz[q];
+            if constexpr (cx(q) > 0) far = far || (xlo_far && c == 0 && lane == 0);
+            if constexpr (cx(q) < 0) far = far || (xhi_far && c == 1 && lane == 63);
+            if (far) f[q] = p.far_eq[q];
+        });
+        float post[19];
+        float act, restr;
+        if constexpr (ARITH == ARITH_FUSED) {
+            collide_fused<COLL>(f, p.omega_plus, p.omega_minus, post, act, restr);
+        } else {
+            float rho, mx, my, mz;
+            moments(f, rho, mx, my, mz);
+            rho = fmaxf(rho, 1.0e-8f);
+            collide<COLL>(f, rho, mx / rho, my / rho, mz / rho, p.omega_plus, p.omega_minus, post, act, restr);
+        }
+        act_n += act;
+        restr_max = fmaxf(restr_max, restr);
+        const float sponge = sponge_on ? sponge_factor(p, x, y, z) : 0.0f;
+        crho[c] = 0.0f;
+        cmx[c] = 0.0f;
+        cmy[c] = 0.0f;
+        cmz[c] = 0.0f;
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            float val = post[q];
+            if (sponge > 0.0f) val = val + (p.far_eq[q] - val) * sponge;
+            if (c == 0) out[q] = __float_as_uint(val);
+            else out[q] = PairStore<S>::pack(__uint_as_float(out[q]), val, q);
+            crho[c] = crho[c] + val;
+            if constexpr (cx(q) > 0) cmx[c] = cmx[c] + val;
+            if constexpr (cx(q) < 0) cmx[c] = cmx[c] - val;
+            if constexpr (cy(q) > 0) cmy[c] = cmy[c] + val;
+            if constexpr (cy(q) < 0) cmy[c] = cmy[c] - val;
+            if constexpr (cz(q) > 0) cmz[c] = cmz[c] + val;
+            if constexpr (cz(q) < 0) cmz[c] = cmz[c] - val;
+        });
+    }
+    if (store) {
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            __builtin_nontemporal_store(out[q], fout + (pop_index<S>(p.g, q, xa, row_self) >> 1));
+        });
+        guard_finite(p, crho[0] + crho[1]);
+        if (p.capture_fields) {
+            const uint32_t gid = (uint32_t)xa + (uint32_t)nx * row_self;
+            capture(density, velocity, gid, crho[0], cmx[0], cmy[0], cmz[0]);
+            capture(density, velocity, gid + 1, crho[1], cmx[1], cmy[1], cmz[1]);
+        }
+    }
+    if constexpr (COLL != COLL_TRT) {
+        // limiter activations of the stored cells: an exact count and the
+        // maximum restriction (order-independent integer and max atomics)
+        if (p.accumulate_loads) {
+            const float hits = store ? act_n : 0.0f;
+            const unsigned long long any = __ballot(hits != 0.0f);
+            if (any != 0) {
+                uint32_t count = (uint32_t)hits;
+                float rm = store ? restr_max : 0.0f;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    count += (uint32_t)__shfl_xor((int)count, off, 64);
+                    rm = fmaxf(rm, __shfl_xor(rm, off, 64));
+                }
+                if (lane == 0) {
+                    atomicAdd(&activation[0], count);
+                    atomicMax(&activation[1], __float_as_uint(rm));
                 }
             }
         }
@@ -1010,6 +1238,7 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
     }
     float crho, cmx, cmy, cmz;
     write_post<S>(fout, x, row, post, sponge, p, crho, cmx, cmy, cmz);
+    guard_finite(p, crho);
     if (p.capture_fields) capture(density, velocity, gid, crho, cmx, cmy, cmz);
     if (p.accumulate_loads) {
         L.fx = flx * fscale;
@@ -1093,6 +1322,15 @@ __global__ __launch_bounds__(256) void fluid_general(const typename Store<S>::T*
 template <int ARITH, int COLL, int S>
 static hipError_t launch_fast_t(const FluidBuffers& b, const StepParams& p, hipStream_t s) {
     using T = typename Store<S>::T;
+    if constexpr (S != STORE_FP32) {
+        // two cells per lane over tile pairs (whole pairs along x)
+        if (p.g.nx % (2 * TILE_X) == 0 && p.pair_cells) {
+            hipLaunchKernelGGL((fluid_fast_pair<ARITH, COLL, S>), dim3(p.g.tiles / 2), dim3(256), 0, s,
+                               (const T*)b.fin, (T*)b.fout, b.static_flags, b.dyn_prev, b.dyn_cur, b.density,
+                               b.velocity, b.activation, p);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((fluid_fast<ARITH, COLL, S>), dim3(p.g.tiles), dim3(256), 0, s, (const T*)b.fin,
                        (T*)b.fout, b.static_flags, b.dyn_prev, b.dyn_cur, b.density, b.velocity, b.activation, p);
     return hipGetLastError();

@@ -176,6 +176,119 @@ __global__ void permute_populations(const T* __restrict__ src, T* __restrict__ d
         dst[i] = src[d];
 }
 
+// reducePopulationMinimum (BirdFlow.metal:6404-6459) over the stored lattice
+// in the host's direction-major numbering (i = q N + cell): the smallest
+// value, a non-finite value comparing as -infinity, ties to the smallest
+// index.  That order is total, so the result does not depend on how the
+// elements are split among lanes and blocks: each lane keeps the best of its
+// grid-stride elements, a workgroup tree keeps the best of the lanes, one
+// block the best of the workgroups.
+__device__ __forceinline__ bool pop_better(float ca, unsigned long long ia, float cb, unsigned long long ib) {
+    return ca < cb || (ca == cb && ia < ib);
+}
+
+__device__ void pop_min_block(PopulationMinimum best, PopulationMinimum* __restrict__ out) {
+    __shared__ float cmp[256], raw[256];
+    __shared__ unsigned long long idx[256];
+    __shared__ uint32_t bad[256];
+    const uint32_t t = threadIdx.x;
+    cmp[t] = best.comparison;
+    raw[t] = best.value;
+    idx[t] = best.index;
+    bad[t] = best.non_finite;
+    __syncthreads();
+    for (uint32_t stride = 128; stride > 0; stride >>= 1) {
+        if (t < stride && pop_better(cmp[t + stride], idx[t + stride], cmp[t], idx[t])) {
+            cmp[t] = cmp[t + stride];
+            raw[t] = raw[t + stride];
+            idx[t] = idx[t + stride];
+            bad[t] = bad[t + stride];
+        }
+        __syncthreads();
+    }
+    if (t == 0) *out = PopulationMinimum{cmp[0], raw[0], idx[0], bad[0], 0u};
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void population_minimum(const typename Store<S>::T* __restrict__ f, Grid g,
+                                                          PopulationMinimum* __restrict__ partials) {
+    const size_t total = 19ull * g.n;
+    PopulationMinimum best{INFINITY, INFINITY, ~0ull, 0u, 0u};
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        // device position e = (row 19 + q) nx + x
+        const size_t r = e / (size_t)g.nx;
+        const int x = (int)(e - r * (size_t)g.nx);
+        const int q = (int)(r % 19u);
+        const size_t row = r / 19u;
+        const float v = Store<S>::load(f, e, q);
+        const bool finite = isfinite(v);
+        const float c = finite ? v : -INFINITY;
+        const unsigned long long i = (unsigned long long)q * g.n + (unsigned long long)x + (unsigned long long)g.nx * row;
+        if (pop_better(c, i, best.comparison, best.index)) best = PopulationMinimum{c, v, i, finite ? 0u : 1u, 0u};
+    }
+    pop_min_block(best, partials + blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void population_minimum_final(const PopulationMinimum* __restrict__ partials,
+                                                                uint32_t count, PopulationMinimum* __restrict__ out) {
+    PopulationMinimum best{INFINITY, INFINITY, ~0ull, 0u, 0u};
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x)
+        if (pop_better(partials[i].comparison, partials[i].index, best.comparison, best.index)) best = partials[i];
+    pop_min_block(best, out);
+}
+
+// One stored population set to `value` (the host's direction-major index):
+// fault injection for the safety checks.
+template <int S>
+__global__ void poke_population(typename Store<S>::T* __restrict__ f, Grid g, unsigned long long index, float value) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int q = (int)(index / g.n);
+    const uint32_t gid = (uint32_t)(index - (unsigned long long)q * g.n);
+    const uint32_t row = gid / (uint32_t)g.nx;
+    Store<S>::store(f, pop_index<S>(g, q, (int)(gid - row * (uint32_t)g.nx), row), q, value);
+}
+
+template <int S>
+static hipError_t pop_min_s(const void* f, const Grid& g, PopulationMinimum* partials, uint32_t blocks,
+                            PopulationMinimum* out, hipStream_t s) {
+    hipLaunchKernelGGL(population_minimum<S>, dim3(blocks), dim3(256), 0, s, (const typename Store<S>::T*)f, g,
+                       partials);
+    hipLaunchKernelGGL(population_minimum_final, dim3(1), dim3(256), 0, s, (const PopulationMinimum*)partials, blocks,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_population_minimum(int storage, const void* f, const Grid& g, PopulationMinimum* partials,
+                                     uint32_t blocks, PopulationMinimum* out, hipStream_t s) {
+    if (blocks < 1 || blocks > POPULATION_MINIMUM_BLOCKS) return hipErrorInvalidValue;
+    switch (storage) {
+        case STORE_FP32: return pop_min_s<STORE_FP32>(f, g, partials, blocks, out, s);
+        case STORE_FP16_OFFSET: return pop_min_s<STORE_FP16_OFFSET>(f, g, partials, blocks, out, s);
+        case STORE_BF16_OFFSET: return pop_min_s<STORE_BF16_OFFSET>(f, g, partials, blocks, out, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_poke_population(int storage, void* f, const Grid& g, unsigned long long index, float value,
+                                  hipStream_t s) {
+    if (index >= 19ull * g.n) return hipErrorInvalidValue;
+    switch (storage) {
+        case STORE_FP32:
+            hipLaunchKernelGGL(poke_population<STORE_FP32>, dim3(1), dim3(64), 0, s, (float*)f, g, index, value);
+            break;
+        case STORE_FP16_OFFSET:
+            hipLaunchKernelGGL(poke_population<STORE_FP16_OFFSET>, dim3(1), dim3(64), 0, s, (_Float16*)f, g, index,
+                               value);
+            break;
+        case STORE_BF16_OFFSET:
+            hipLaunchKernelGGL(poke_population<STORE_BF16_OFFSET>, dim3(1), dim3(64), 0, s, (uint16_t*)f, g, index,
+                               value);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_load_reduction(const FluidBuffers& b, const StepParams& p, uint32_t history_slot, hipStream_t s) {
     if (!p.accumulate_loads) return hipSuccess;
     const uint32_t blocks = (p.g.tiles + 1023) / 1024;

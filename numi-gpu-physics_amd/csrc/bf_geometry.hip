@@ -173,8 +173,9 @@ __global__ void fill_wall(float4* __restrict__ p, uint32_t n) {
 //     for this step's tile pass (a crowded table sends a key straight to the
 //     grid).  A cell within the band always ends with a key, so marking on
 //     every attempt marks the same tiles as marking on a win.
-// Keys are a minimum over (cell, triangle) pairs: the result does not depend
-// on the order of the atomics.  With m.cull_cells = +inf (the reference's
+// With a triangle list the batches walk the listed ids (the keys of the other
+// triangles come from seed_keys, below).  Keys are a minimum over (cell,
+// triangle) pairs: the result does not depend on the order of the atomics.  With m.cull_cells = +inf (the reference's
 // full AABB) the filter passes every cell of the AABB.
 // LDS per single-wavefront workgroup: candidates 2 KB + key table 8 KB.  At
 // 10 KB the compiler targets 4 waves per SIMD and keeps the raster under 100
@@ -196,6 +197,7 @@ __device__ __forceinline__ float bcast(float v, uint32_t j) {
 __device__ __forceinline__ int bcast(int v, uint32_t j) { return __builtin_amdgcn_readlane(v, (int)j); }
 
 __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
+                                                  const uint32_t* __restrict__ list,
                                                   unsigned long long* __restrict__ keys,
                                                   uint8_t* __restrict__ tiles_new, MeshParams m) {
     __shared__ uint2 cand[RASTER_CAP];
@@ -216,10 +218,10 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
     const bool filter = isfinite(m.cull_cells);
     const float reach = (filter ? m.cull_cells : 0.0f) * dx + 1.0e-3f * dx;
     const uint32_t bsz = m.raster_batch;  // triangles per batch, 1..64
-    for (uint32_t batch = blockIdx.x * bsz; batch < m.triangle_count; batch += gridDim.x * bsz) {
+    for (uint32_t batch = blockIdx.x * bsz; batch < m.raster_count; batch += gridDim.x * bsz) {
         // 0. per-lane set-up of triangle batch + lane
-        const uint32_t tl = batch + lane;
-        const bool mine = lane < bsz && tl < m.triangle_count;
+        const bool mine = lane < bsz && batch + lane < m.raster_count;
+        const uint32_t tl = !mine ? 0u : (list ? list[batch + lane] : batch + lane);
         float3 a = make_float3(0.0f, 0.0f, 0.0f), b = a, c = a;
         if (mine) {
             const float4 a4 = pos[tri[3 * (size_t)tl + 0]];
@@ -269,9 +271,9 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
             lo[i] = l;
             ext[i] = mine ? max(h - l + 1, 0) : 0;
         }
-        const uint32_t nb = min(bsz, m.triangle_count - batch);
+        const uint32_t nb = min(bsz, m.raster_count - batch);
         for (uint32_t j = 0; j < nb; ++j) {
-            const uint32_t t = batch + j;
+            const uint32_t t = (uint32_t)bcast((int)tl, j);
             const float3 ta = make_float3(bcast(a.x, j), bcast(a.y, j), bcast(a.z, j));
             const float3 tb = make_float3(bcast(b.x, j), bcast(b.y, j), bcast(b.z, j));
             const float3 tc = make_float3(bcast(c.x, j), bcast(c.y, j), bcast(c.z, j));
@@ -387,6 +389,45 @@ __global__ __launch_bounds__(64) void mesh_raster(const float4* __restrict__ pos
             }
         }
         __syncthreads();
+    }
+}
+
+// Seeds the keys of the triangles the raster skips (static triangles: the
+// same vertex positions in every frame, so the same keys every step) from
+// their cached (cell, key) pairs, and marks the cells' tiles.  Runs before the
+// raster of the moving triangles, when every key is empty (each tile pass
+// empties the keys it resolves), so plain stores suffice; the raster's
+// atomicMin then takes the minimum over both sets.
+__global__ void seed_keys(const uint32_t* __restrict__ cells, const unsigned long long* __restrict__ cached,
+                          uint32_t count, unsigned long long* __restrict__ keys, uint8_t* __restrict__ tiles_new,
+                          Grid g) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t gid = cells[i];
+    keys[gid] = cached[i];
+    const uint32_t row = gid / (uint32_t)g.nx;
+    tiles_new[tile_of(g, (int)(gid - row * (uint32_t)g.nx), (int)(row % (uint32_t)g.ny), (int)(row / (uint32_t)g.ny))] = 1;
+}
+
+// Number of keyed cells (capacity 0), or: moves every key into the next
+// (cell, key) slot and empties the cell.  One atomic per wavefront.
+__global__ void collect_keys(unsigned long long* __restrict__ keys, uint32_t n, uint32_t* __restrict__ cells,
+                             unsigned long long* __restrict__ out, uint32_t capacity,
+                             unsigned long long* __restrict__ counter) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long key = gid < n ? keys[gid] : EMPTY_KEY;
+    const bool keyed = key != EMPTY_KEY;
+    const unsigned long long b = __ballot(keyed);
+    if (b == 0) return;
+    unsigned long long base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, (unsigned long long)__popcll(b));
+    base = __shfl(base, 0, 64);
+    if (!keyed || capacity == 0) return;
+    const unsigned long long slot = base + lane_rank(b);
+    if (slot < capacity) {
+        cells[slot] = gid;
+        out[slot] = key;
+        keys[gid] = EMPTY_KEY;
     }
 }
 
@@ -945,10 +986,66 @@ __global__ __launch_bounds__(256) void bird_tile_pass(BirdShape b, const float4*
     }
 }
 
-hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s) {
+// Momentum of one prescribed rigid wing about the body position from its
+// prepared frame rows (prescribedWingMomentum, BirdFlow.metal:6461-6490):
+// centre offset c = chord m_c + span m_s + normal m_n from the hinge,
+// P = m (w x c), L = (root + c - body) x P + sum over the axes a of
+// a (I_a (w . a)).  Same operations in the same order as the oracle.
+__device__ void wing_momentum(const float4 root, const float4 chord, const float4 span, const float4 normal,
+                              const float4 omega, float mass, const float com[3], const float inertia[3],
+                              const float4 body, float linear[3], float angular[3]) {
+    const float ch[3] = {chord.x, chord.y, chord.z}, sp[3] = {span.x, span.y, span.z};
+    const float nm[3] = {normal.x, normal.y, normal.z}, rt[3] = {root.x, root.y, root.z};
+    const float bd[3] = {body.x, body.y, body.z};
+    float off[3], arm[3], spin[3];
+    for (int i = 0; i < 3; ++i) off[i] = ch[i] * com[0] + sp[i] * com[1] + nm[i] * com[2];
+    const float cv[3] = {omega.y * off[2] - omega.z * off[1], omega.z * off[0] - omega.x * off[2],
+                         omega.x * off[1] - omega.y * off[0]};
+    for (int i = 0; i < 3; ++i) linear[i] = mass * cv[i];
+    const float wc = omega.x * ch[0] + omega.y * ch[1] + omega.z * ch[2];
+    const float ws = omega.x * sp[0] + omega.y * sp[1] + omega.z * sp[2];
+    const float wn = omega.x * nm[0] + omega.y * nm[1] + omega.z * nm[2];
+    for (int i = 0; i < 3; ++i) {
+        spin[i] = ch[i] * (inertia[0] * wc) + sp[i] * (inertia[1] * ws) + nm[i] * (inertia[2] * wn);
+        arm[i] = rt[i] + off[i] - bd[i];
+    }
+    const float ax[3] = {arm[1] * linear[2] - arm[2] * linear[1], arm[2] * linear[0] - arm[0] * linear[2],
+                         arm[0] * linear[1] - arm[1] * linear[0]};
+    for (int i = 0; i < 3; ++i) angular[i] = ax[i] + spin[i];
+}
+
+// updateWingInertialReaction (BirdFlow.metal:6493-6572), one lane: the hinge
+// reaction -(momentum - previous) / dt of each wing from the step's prepared
+// frame (0 on the first frame), and the momenta kept for the next step.
+__global__ void wing_inertial_reaction(const float4* __restrict__ frame, WingDynamics d,
+                                       float* __restrict__ previous, float* __restrict__ reaction) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float m[4][3];
+    // frame rows: 0 body position, 4..8 left root / chord / span / normal /
+    // angular velocity, 9..13 the right wing's
+    wing_momentum(frame[4], frame[5], frame[6], frame[7], frame[8], d.mass[0], d.center_of_mass[0], d.inertia[0],
+                  frame[0], m[0], m[1]);
+    wing_momentum(frame[9], frame[10], frame[11], frame[12], frame[13], d.mass[1], d.center_of_mass[1], d.inertia[1],
+                  frame[0], m[2], m[3]);
+    const float inverse_dt = 1.0f / d.dt;
+    for (int r = 0; r < 4; ++r) {
+        for (int i = 0; i < 3; ++i) {
+            reaction[4 * r + i] = d.initialize ? 0.0f : -(m[r][i] - previous[4 * r + i]) * inverse_dt;
+            previous[4 * r + i] = m[r][i];
+        }
+        reaction[4 * r + 3] = 0.0f;
+        previous[4 * r + 3] = 0.0f;
+    }
+}
+
+hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s,
+                                const WingDynamics* wing_dynamics) {
     const unsigned tile_blocks = (p.g.tiles + 255) / 256;
     hipLaunchKernelGGL(bird_prepare, dim3(1), dim3(64), 0, s, shape, b.body, b.keyframes, (int)b.keyframe_count,
                        p.time, p.dx, b.frame);
+    if (wing_dynamics && b.wing_momentum && b.wing_reaction)
+        hipLaunchKernelGGL(wing_inertial_reaction, dim3(1), dim3(64), 0, s, (const float4*)b.frame, *wing_dynamics,
+                           b.wing_momentum, b.wing_reaction);
     hipLaunchKernelGGL(bird_mark_tiles, dim3(tile_blocks), dim3(256), 0, s, shape, (const float4*)b.frame,
                        b.dyn_prev, b.tiles_new, p);
     hipLaunchKernelGGL(fill_u8, dim3(tile_blocks), dim3(256), 0, s, b.dyn, (uint8_t)0, p.g.tiles);
@@ -1222,6 +1319,21 @@ __global__ __launch_bounds__(256) void wing_tile_pass(WingShape w, const float4*
 // Link fractions of the fluid cells of the tiles near the wing: for each
 // direction whose (in-domain) source is solid, the bisection from the
 // source's centre plus c_q dx (the reference's fluid point) to the source.
+// Source coordinate of a link along one axis, as the fluid kernel's pull
+// treats the faces: periodic wraps, an outlet clamps into the domain, a far
+// field has no in-domain source (false).
+__device__ inline bool link_source(const int bc[2], int s, int n, int& out) {
+    if (s < 0) {
+        if (bc[0] == BC_FARFIELD) return false;
+        s = bc[0] == BC_PERIODIC ? n - 1 : 0;
+    } else if (s >= n) {
+        if (bc[1] == BC_FARFIELD) return false;
+        s = bc[1] == BC_PERIODIC ? 0 : n - 1;
+    }
+    out = s;
+    return true;
+}
+
 __global__ __launch_bounds__(256) void wing_links(WingShape w, const float4* __restrict__ frame,
                                                   const uint32_t* __restrict__ list,
                                                   const uint32_t* __restrict__ count,
@@ -1245,8 +1357,12 @@ __global__ __launch_bounds__(256) void wing_links(WingShape w, const float4* __r
         if (mask[gid] != 0) continue;
         float* out = links + (size_t)tile * 19 * TILE_CELLS + threadIdx.x;
         for (int q = 1; q < 19; ++q) {
-            const int sx = x - cx(q), sy = y - cy(q), sz = z - cz(q);
-            if (sx < 0 || sy < 0 || sz < 0 || sx >= p.g.nx || sy >= p.g.ny || sz >= p.g.nz) continue;
+            // the source the fluid kernel pulls from: wrapped across a
+            // periodic face, clamped at an outlet, none beyond a far field
+            int sx, sy, sz;
+            if (!link_source(p.bc[0], x - cx(q), p.g.nx, sx) || !link_source(p.bc[1], y - cy(q), p.g.ny, sy) ||
+                !link_source(p.bc[2], z - cz(q), p.g.nz, sz))
+                continue;
             const uint32_t src = (uint32_t)sx + (uint32_t)p.g.nx * ((uint32_t)sy + (uint32_t)p.g.ny * (uint32_t)sz);
             if (mask[src] == 0) continue;
             const V3 solid = v3(p.origin[0] + ((float)sx + 0.5f) * p.dx, p.origin[1] + ((float)sy + 0.5f) * p.dx,
@@ -1283,6 +1399,30 @@ hipError_t launch_wing_geometry(const WingShape& shape, const WingBuffers& b, co
     return hipGetLastError();
 }
 
+hipError_t launch_mesh_static_keys(const MeshBuffers& b, const MeshParams& m, unsigned long long* count,
+                                   hipStream_t s) {
+    if (m.g.nx > 65535 || m.g.ny > 65535) return hipErrorInvalidValue;   // raster packs x | y << 16
+    if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1 || m.raster_split < 1 || m.raster_split > 64)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
+                       b.vel, b.body, m);
+    hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new, (uint8_t)0, m.g.tiles);
+    if (m.raster_count > 0)
+        hipLaunchKernelGGL(mesh_raster, dim3(std::min((m.raster_count + m.raster_batch - 1) / m.raster_batch, m.raster_blocks), m.raster_split), dim3(64), 0, s, b.pos,
+                           b.tri, b.raster_list, b.keys, b.tiles_new, m);
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)count, (uint64_t)0, 1u);
+    hipLaunchKernelGGL(collect_keys, dim3((m.g.n + 255) / 256), dim3(256), 0, s, b.keys, m.g.n, (uint32_t*)nullptr,
+                       (unsigned long long*)nullptr, 0u, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_collect_keys(unsigned long long* keys, uint32_t n, uint32_t* cells, unsigned long long* out,
+                               uint32_t capacity, unsigned long long* counter, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, s, (uint64_t*)counter, (uint64_t)0, 1u);
+    hipLaunchKernelGGL(collect_keys, dim3((n + 255) / 256), dim3(256), 0, s, keys, n, cells, out, capacity, counter);
+    return hipGetLastError();
+}
+
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
     if (m.g.nx > 65535 || m.g.ny > 65535) return hipErrorInvalidValue;   // raster packs x | y << 16
     if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1 || m.raster_split < 1 || m.raster_split > 64)
@@ -1292,9 +1432,12 @@ hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipSt
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.dyn, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);
-    if (m.triangle_count > 0)
-        hipLaunchKernelGGL(mesh_raster, dim3(std::min((m.triangle_count + m.raster_batch - 1) / m.raster_batch, m.raster_blocks), m.raster_split), dim3(64), 0, s, b.pos,
-                           b.tri, b.keys, b.tiles_new, m);
+    if (b.seed_count > 0)
+        hipLaunchKernelGGL(seed_keys, dim3((b.seed_count + 255) / 256), dim3(256), 0, s, b.seed_cells, b.seed_keys,
+                           b.seed_count, b.keys, b.tiles_new, m.g);
+    if (m.raster_count > 0)
+        hipLaunchKernelGGL(mesh_raster, dim3(std::min((m.raster_count + m.raster_batch - 1) / m.raster_batch, m.raster_blocks), m.raster_split), dim3(64), 0, s, b.pos,
+                           b.tri, b.raster_list, b.keys, b.tiles_new, m);
     hipLaunchKernelGGL(geometry_tile_list, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new,
                        b.tiles_mask_old, b.tiles_wall_old, m.g.tiles, b.list, b.list_count);
     hipLaunchKernelGGL(geometry_tile_pass, dim3(m.tile_pass_blocks), dim3(256), 0, s, b.pos, b.vel, b.tri, b.tri_part, b.keys, b.list,

@@ -42,6 +42,7 @@ struct MeshParams {
     uint32_t frame_count;
     uint32_t vertex_count;
     uint32_t triangle_count;
+    uint32_t raster_count;     // triangles the raster walks: MeshBuffers::raster_list, or 0..raster_count-1
     int periodic[3];
     float cull_cells;          // key only cells at most this far (cells) from a triangle; +inf = whole AABB
     int move_with_body;        // frames are body-frame; place them with the device body pose
@@ -70,7 +71,19 @@ struct BirdParams {
     float time;                // geometry time of the step (s)
     float velocity_scale;      // metres / second -> lattice
     int periodic[3];
+    int bc[3][2];              // face types (BC_*), [axis][low, high]: the wing's link sources
     uint32_t tile_pass_blocks;
+};
+
+// Prescribed-wing dynamics of the analytic bird (GPUData.swift:150-176):
+// per wing (0 left, 1 right) mass, centre of mass from the hinge along chord /
+// span / normal, principal inertia about those axes; the step's time (s).
+struct WingDynamics {
+    float mass[2];
+    float center_of_mass[2][3];
+    float inertia[2][3];
+    float dt;
+    int initialize;            // the first frame: momentum only, reaction 0
 };
 
 struct BirdBuffers {
@@ -88,6 +101,10 @@ struct BirdBuffers {
     uint8_t* mask;
     float4* wall;
     uint8_t* dyn;
+    // optional hinge reaction of the prescribed wings: the previous frame's
+    // wing momenta (16 floats, updated) and this step's reaction (16 floats)
+    float* wing_momentum;
+    float* wing_reaction;
 };
 
 // Prescribed beta-planform flapping wing (BirdFlow.metal:940-1297), as
@@ -133,6 +150,13 @@ struct MeshBuffers {
     float4* wall;
     uint8_t* dyn;
     uint32_t* tri_out;                  // optional: nearest triangle of each resolved cell
+    // Optional: the raster walks only the listed triangles (ascending ids),
+    // and the keys of the others, constant from step to step, are seeded
+    // from a cache of (cell, key) pairs before it runs.
+    const uint32_t* raster_list;
+    const uint32_t* seed_cells;
+    const unsigned long long* seed_keys;
+    uint32_t seed_count;
 };
 
 // Adjoint of one step (bf_adjoint.hip).  lam_in = dJ/df_s, lam_out =
@@ -155,6 +179,7 @@ struct AdjointBuffers {
     double* grad_partials;        // [listed tile][8]
     double* grad_total;           // [8], accumulated over steps
     const double* body;
+    float* outlet_extra;          // [x face 0 / 1][19][ny * nz]: clamped pulls at outlet x faces
 };
 struct AdjointParams {
     float cF[3], cT[3];           // load weights of the step: J_s = cF . F + cT . T
@@ -185,8 +210,12 @@ struct BodyParams {
     double far_velocity_world[3];
     double velocity_to_lattice;
 };
+// wing_reaction (optional): the step's prescribed-wing hinge reaction, 16
+// floats (left force, left torque, right force, right torque; xyz + 0),
+// added to the fluid load.
 hipError_t launch_body_step(double* state, const double* load, double* history, uint32_t history_slot,
-                            double* metrics, uint32_t* event, uint64_t step, const BodyParams& bp, hipStream_t s);
+                            double* metrics, uint32_t* event, uint64_t step, const BodyParams& bp, hipStream_t s,
+                            const float* wing_reaction = nullptr);
 
 hipError_t launch_fluid_fast(int arith, int storage, int coll, const FluidBuffers& b, const StepParams& p,
                              hipStream_t s);
@@ -213,9 +242,33 @@ hipError_t launch_extract_fields(int storage, const void* f, float* rho, float* 
 // device row-interleaved order, unchanged.
 hipError_t launch_permute_populations(int storage, bool to_device, const void* src, void* dst, size_t count,
                                       const Grid& g, hipStream_t s);
+// The population minimum (bf_fluid.hip): comparison value (a non-finite
+// value compares as -infinity), the value itself, its direction-major index
+// and whether it is non-finite.
+struct PopulationMinimum {
+    float comparison;
+    float value;
+    unsigned long long index;
+    uint32_t non_finite;
+    uint32_t pad;
+};
+constexpr uint32_t POPULATION_MINIMUM_BLOCKS = 4096;
+hipError_t launch_population_minimum(int storage, const void* f, const Grid& g, PopulationMinimum* partials,
+                                     uint32_t blocks, PopulationMinimum* out, hipStream_t s);
+hipError_t launch_poke_population(int storage, void* f, const Grid& g, unsigned long long index, float value,
+                                  hipStream_t s);
 hipError_t launch_convert_populations(int storage, bool pack, const void* src, void* dst, size_t count,
                                       const Grid& g, hipStream_t s);
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
+// Keys of the listed (static) triangles alone: vertex prepare, then the
+// raster of b.raster_list into b.keys (tiles into b.tiles_new), then the
+// number of keyed cells into *count.
+hipError_t launch_mesh_static_keys(const MeshBuffers& b, const MeshParams& m, unsigned long long* count,
+                                   hipStream_t s);
+// Moves every key of `keys` (n cells) into (cells, out) pairs, at most
+// `capacity`, and empties it; *counter ends at the number of keys.
+hipError_t launch_collect_keys(unsigned long long* keys, uint32_t n, uint32_t* cells, unsigned long long* out,
+                               uint32_t capacity, unsigned long long* counter, hipStream_t s);
 hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
 
 // Momentum ledger (bf_ledger.hip).  sources: the completed step's change,
@@ -235,7 +288,8 @@ struct LedgerBuffers {
     unsigned long long* out_n;
 };
 hipError_t launch_ledger(int storage, const LedgerBuffers& b, const StepParams& p, bool sources, hipStream_t s);
-hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s);
+hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s,
+                                const WingDynamics* wing_dynamics = nullptr);
 hipError_t launch_wing_geometry(const WingShape& shape, const WingBuffers& b, const BirdParams& p, hipStream_t s);
 hipError_t launch_sphere_geometry(uint8_t* mask, float4* wall, uint8_t* dyn, const Grid& g, float4 cr, float3 gv,
                                   float4 wv, float t, const int periodic[3], hipStream_t s);

@@ -507,3 +507,181 @@ void bfo_place_vertices(const bfo_body* b, int vertex_count, float velocity_scal
 
 #include "bfref_bird.inc"
 #include "bfref_wing.inc"
+
+/* ---- exact link fractions against a meshed shell ---- */
+static float rdot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void rcross(const float a[3], const float b[3], float o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static void rsub(const float a[3], const float b[3], float o[3])
+{
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+/* point P on the plane side of edge (U, V) of a triangle with normal n */
+static int redge_in(const float U[3], const float V[3], const float P[3], const float n[3])
+{
+    float e[3], m[3], c[3];
+    rsub(V, U, e);
+    rsub(P, U, m);
+    rcross(e, m, c);
+    return rdot(c, n) >= 0.0f;
+}
+
+float bfo_ray_shell(const float F[3], const float D[3], const float A[3], const float B[3], const float C[3],
+                    float h)
+{
+    /* F within the shell (inside a vertex sphere, an edge cylinder over the
+     * edge, or the slab over the face) enters at 0 */
+    float best = 2.0f;
+    float e1[3], e2[3], n[3];
+    rsub(B, A, e1);
+    rsub(C, A, e2);
+    rcross(e1, e2, n);
+    const float nl = sqrtf(rdot(n, n));
+    if (nl > 0.0f) {
+        for (int i = 0; i < 3; ++i) n[i] = n[i] / nl;
+        float fa[3];
+        rsub(F, A, fa);
+        const float fn = rdot(fa, n);
+        if (fn <= h && fn >= -h && redge_in(A, B, F, n) && redge_in(B, C, F, n) && redge_in(C, A, F, n))
+            return 0.0f;
+        const float dn = rdot(D, n);
+        if (dn != 0.0f) {
+            for (int side = 0; side < 2; ++side) {
+                const float off = side == 0 ? h : -h;
+                const float t = (off - fn) / dn;
+                if (t >= 0.0f && t < best) {
+                    const float P[3] = {F[0] + D[0] * t, F[1] + D[1] * t, F[2] + D[2] * t};
+                    if (redge_in(A, B, P, n) && redge_in(B, C, P, n) && redge_in(C, A, P, n)) best = t;
+                }
+            }
+        }
+    }
+    const float* ends[3][2] = {{A, B}, {B, C}, {C, A}};
+    const float dd = rdot(D, D);
+    for (int k = 0; k < 3; ++k) {
+        float e[3], m[3];
+        rsub(ends[k][1], ends[k][0], e);
+        rsub(F, ends[k][0], m);
+        const float ee = rdot(e, e), de = rdot(D, e), me = rdot(m, e), md = rdot(m, D), mm = rdot(m, m);
+        const float a = ee * dd - de * de;
+        const float hb = ee * md - me * de;
+        const float c = (ee * mm - me * me) - (h * h) * ee;
+        if (c <= 0.0f && me >= 0.0f && me <= ee) return 0.0f;
+        if (a > 0.0f) {
+            const float disc = hb * hb - a * c;
+            if (disc >= 0.0f) {
+                const float t = (-hb - sqrtf(disc)) / a;
+                if (t >= 0.0f && t < best) {
+                    const float u = (me + t * de) / ee;
+                    if (u >= 0.0f && u <= 1.0f) best = t;
+                }
+            }
+        }
+    }
+    const float* verts[3] = {A, B, C};
+    for (int k = 0; k < 3; ++k) {
+        float m[3];
+        rsub(F, verts[k], m);
+        const float md = rdot(m, D);
+        const float c = rdot(m, m) - h * h;
+        if (c <= 0.0f) return 0.0f;
+        const float disc = md * md - dd * c;
+        if (disc >= 0.0f) {
+            const float t = (-md - sqrtf(disc)) / dd;
+            if (t >= 0.0f && t < best) best = t;
+        }
+    }
+    return best;
+}
+
+/* the source coordinate of a link along one axis (the step's face
+ * treatment); 0: beyond a far-field face */
+static int ray_source(const int32_t bc[2], int s, int n, int* out)
+{
+    if (s < 0) {
+        if (bc[0] == BFO_BC_FARFIELD) return 0;
+        s = bc[0] == BFO_BC_PERIODIC ? n - 1 : 0;
+    } else if (s >= n) {
+        if (bc[1] == BFO_BC_FARFIELD) return 0;
+        s = bc[1] == BFO_BC_PERIODIC ? 0 : n - 1;
+    }
+    *out = s;
+    return 1;
+}
+
+void bfo_ray_links(const bfo_params* p, const float* position3, const uint32_t* tri, int triangle_count,
+                   float half_thickness, const uint8_t* mask, const float* wall4, float* links)
+{
+    const size_t n = (size_t)p->nx * p->ny * p->nz;
+    const float dx = p->dx;
+    /* per-triangle boxes of the shell, for a quick rejection */
+    float* box = (float*)malloc(sizeof(float) * 6 * (size_t)(triangle_count > 0 ? triangle_count : 1));
+    for (int t = 0; t < triangle_count; ++t)
+        for (int a = 0; a < 3; ++a) {
+            float lo = position3[3 * tri[3 * t] + a], hi = lo;
+            for (int k = 1; k < 3; ++k) {
+                const float v = position3[3 * tri[3 * t + k] + a];
+                lo = v < lo ? v : lo;
+                hi = v > hi ? v : hi;
+            }
+            box[6 * t + a] = lo - half_thickness - 0.01f * dx;
+            box[6 * t + 3 + a] = hi + half_thickness + 0.01f * dx;
+        }
+    for (size_t i = 0; i < 19 * n; ++i) links[i] = 0.0f;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int z = 0; z < p->nz; ++z)
+        for (int y = 0; y < p->ny; ++y)
+            for (int x = 0; x < p->nx; ++x) {
+                const size_t gid = (size_t)x + (size_t)p->nx * ((size_t)y + (size_t)p->ny * (size_t)z);
+                if (mask[gid] != 0) continue;
+                for (int q = 1; q < 19; ++q) {
+                    int sx, sy, sz;
+                    if (!ray_source(p->bc[0], x - bfo_c[q][0], p->nx, &sx) ||
+                        !ray_source(p->bc[1], y - bfo_c[q][1], p->ny, &sy) ||
+                        !ray_source(p->bc[2], z - bfo_c[q][2], p->nz, &sz))
+                        continue;
+                    const size_t src = (size_t)sx + (size_t)p->nx * ((size_t)sy + (size_t)p->ny * (size_t)sz);
+                    if (mask[src] == 0) continue;
+                    const float S[3] = {p->origin[0] + ((float)sx + 0.5f) * dx, p->origin[1] + ((float)sy + 0.5f) * dx,
+                                        p->origin[2] + ((float)sz + 0.5f) * dx};
+                    const float F[3] = {S[0] + (float)bfo_c[q][0] * dx, S[1] + (float)bfo_c[q][1] * dx,
+                                        S[2] + (float)bfo_c[q][2] * dx};
+                    const float D[3] = {(float)(-bfo_c[q][0]) * dx, (float)(-bfo_c[q][1]) * dx,
+                                        (float)(-bfo_c[q][2]) * dx};
+                    float lo[3], hi[3];
+                    for (int a = 0; a < 3; ++a) {
+                        lo[a] = F[a] < S[a] ? F[a] : S[a];
+                        hi[a] = F[a] < S[a] ? S[a] : F[a];
+                    }
+                    float best = 2.0f;
+                    for (int t = 0; t < triangle_count; ++t) {
+                        const float* bx = box + 6 * t;
+                        if (hi[0] < bx[0] || lo[0] > bx[3] || hi[1] < bx[1] || lo[1] > bx[4] || hi[2] < bx[2] ||
+                            lo[2] > bx[5])
+                            continue;
+                        const float r = bfo_ray_shell(F, D, position3 + 3 * tri[3 * t], position3 + 3 * tri[3 * t + 1],
+                                                      position3 + 3 * tri[3 * t + 2], half_thickness);
+                        best = r < best ? r : best;
+                    }
+                    float qf;
+                    if (best <= 1.0f) {
+                        qf = best;
+                    } else {
+                        const float df = wall4[4 * gid + 3] > 0.0f ? wall4[4 * gid + 3] : 0.0f;
+                        const float ds = wall4[4 * src + 3] < 0.0f ? wall4[4 * src + 3] : 0.0f;
+                        const float den = df - ds > 1.0e-6f ? df - ds : 1.0e-6f;
+                        qf = df / den;
+                    }
+                    qf = qf < 1.0e-4f ? 1.0e-4f : qf;
+                    qf = qf > 1.0f ? 1.0f : qf;
+                    links[(size_t)q * n + gid] = qf;
+                }
+            }
+    free(box);
+}

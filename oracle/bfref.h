@@ -156,6 +156,29 @@ void bfo_resolve(const bfo_params* p, const float* position3, const float* veloc
                  uint8_t* mask, float* wall4);
 void bfo_clear_geometry(const bfo_params* p, uint8_t* mask, float* wall4);
 
+/* Exact link fractions of a meshed shell (the ray-root wall rule): for every
+ * fluid cell whose pull source along q (face treatment as the step's: wrapped
+ * across a periodic face, clamped at an outlet, none beyond a far field) is
+ * solid, the fraction q of the link from the fluid node toward the solid node
+ * at which the segment first comes within half_thickness of the mesh (the
+ * union of every triangle's shell: the offset surface the reference's
+ * link-ray-root audit bisects, MetalIndexedBirdSurfaceValidation.swift:
+ * 10317-10438), clamped to [1e-4, 1]; links[q * N + cell], 0 elsewhere.  The
+ * first entry into each triangle's shell (the triangle swept by a ball of
+ * radius half_thickness) is the smallest root over its two offset faces
+ * (inside the triangle), three edge cylinders (within the edge) and three
+ * vertex spheres; the root is the minimum over all triangles.  A link whose
+ * segment enters no shell (a solid source only by the nearest-key
+ * quantization) takes the signed-distance ratio of BirdFlow.metal:2044-2076
+ * from wall4.  FP32, no contraction. */
+void bfo_ray_links(const bfo_params* p, const float* position3, const uint32_t* tri, int triangle_count,
+                   float half_thickness, const uint8_t* mask, const float* wall4, float* links);
+/* The first entry t in [0, 1] of the segment F + t D into the shell of radius
+ * h around triangle ABC: 0 when F lies in it, 2 when the segment does not
+ * enter it. */
+float bfo_ray_shell(const float F[3], const float D[3], const float A[3], const float B[3], const float C[3],
+                    float h);
+
 /* Translating sphere geometry at time t (geometry center = c0 + v_geom * t). */
 void bfo_translating_sphere(const bfo_params* p, const float center_radius[4],
                             const float geometry_velocity[3], const float wall_velocity[4],
@@ -228,6 +251,27 @@ void bfo_bird_prepare(const bfo_bird* b, const float body_position[3], const flo
  * velocity_to_lattice, 0).  solid_prev NULL: no previous geometry. */
 void bfo_bird_build(const bfo_params* p, const bfo_bird* b, const bfo_bird_frame* f, const uint8_t* solid_prev,
                     float velocity_to_lattice, uint8_t* mask, float* wall4);
+
+/* Rigid-wing inertial properties of the prescribed-wing dynamics
+ * (PrescribedWingDynamics, BirdModel.swift:98-128, packed as
+ * GPUData.swift:150-176): mass (kg); centre of mass from the hinge along the
+ * wing's chord, span and normal axes (m); principal inertia about those axes
+ * through the centre of mass (kg m^2). */
+typedef struct bfo_wing_dynamics {
+    float mass;
+    float center_of_mass[3];
+    float inertia[3];
+} bfo_wing_dynamics;
+
+/* updateWingInertialReaction (BirdFlow.metal:6493-6572) with
+ * prescribedWingMomentum (:6461-6490): each wing's linear and angular
+ * momentum about the body position from the prepared frame; reaction[16] =
+ * left force, left torque, right force, right torque (float4 rows, w = 0) =
+ * -(momentum - previous) / dt, or 0 when initialize; previous[16] (the same
+ * rows: left linear, left angular, right linear, right angular) becomes this
+ * frame's momentum.  FP32, no contraction. */
+void bfo_wing_reaction(const bfo_bird_frame* f, const bfo_wing_dynamics* left, const bfo_wing_dynamics* right,
+                       float dt, int initialize, float previous[16], float reaction[16]);
 
 /* ---- prescribed beta-planform flapping wing (FP32) ---- */
 /* GPUFlappingWingParameters (GPUData.swift:383-388): root position and mean

@@ -60,6 +60,20 @@ class Body(ctypes.Structure):
     ]
 
 
+class WingDynamics(ctypes.Structure):
+    """bfo_wing_dynamics: mass, centre of mass from the hinge along chord /
+    span / normal, principal inertia about those axes."""
+    _fields_ = [("mass", ctypes.c_float), ("center_of_mass", ctypes.c_float * 3), ("inertia", ctypes.c_float * 3)]
+
+
+def make_wing_dynamics(mass, center_of_mass, inertia) -> WingDynamics:
+    d = WingDynamics()
+    d.mass = mass
+    d.center_of_mass[:] = list(center_of_mass)
+    d.inertia[:] = list(inertia)
+    return d
+
+
 class Bird(ctypes.Structure):
     """bfo_bird: GPUBirdParameters geometry and analytic kinematics."""
     _fields_ = [
@@ -158,6 +172,11 @@ def _declare(l):
     l.bfo_bird_prepare.argtypes = [ctypes.POINTER(Bird), vp, vp, vp, vp, ctypes.c_float, ctypes.c_float, vp,
                                    ctypes.c_int, vp]
     l.bfo_bird_build.argtypes = [P, ctypes.POINTER(Bird), vp, vp, ctypes.c_float, vp, vp]
+    l.bfo_ray_links.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_float, vp, vp, vp]
+    l.bfo_ray_shell.argtypes = [vp, vp, vp, vp, vp, ctypes.c_float]
+    l.bfo_ray_shell.restype = ctypes.c_float
+    l.bfo_wing_reaction.argtypes = [vp, ctypes.POINTER(WingDynamics), ctypes.POINTER(WingDynamics), ctypes.c_float,
+                                    ctypes.c_int, vp, vp]
 
 
 def make_params(shape, *, omega_plus, omega_minus, bc=None, collision=COLLISION_TRT,
@@ -483,6 +502,17 @@ def bird_prepare(bird: Bird, position, orientation=(0, 0, 0, 1), linear_velocity
     return out
 
 
+def wing_reaction(frame, left: WingDynamics, right: WingDynamics, dt, previous, initialize=False):
+    """updateWingInertialReaction: (reaction [4, 4], new previous [4, 4]);
+    rows left force, left torque, right force, right torque / the momenta."""
+    fr = np.ascontiguousarray(frame, np.float32)
+    prev = np.array(previous, np.float32).reshape(16).copy()
+    out = np.zeros(16, np.float32)
+    lib().bfo_wing_reaction(_ptr(fr), ctypes.byref(left), ctypes.byref(right), np.float32(dt), int(initialize),
+                            _ptr(prev), _ptr(out))
+    return out.reshape(4, 4), prev.reshape(4, 4)
+
+
 def bird_build(p: Params, bird: Bird, frame, solid_prev=None, velocity_to_lattice=1.0):
     n = cells(p)
     mask = np.zeros(n, np.uint8)
@@ -655,6 +685,26 @@ def wing_links(p: Params, wing: Wing, frame, mask):
     m = np.ascontiguousarray(mask, np.uint8)
     lib().bfo_wing_links(ctypes.byref(p), ctypes.byref(wing), _ptr(fr), _ptr(m), _ptr(links))
     return links
+
+
+def ray_links(p: Params, pos, tri, half_thickness, mask, wall4):
+    """Exact link fractions of the meshed shell (bfo_ray_links): [19 * N],
+    q * N + cell."""
+    n = cells(p)
+    links = np.zeros(19 * n, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    tri = np.ascontiguousarray(tri, np.uint32)
+    lib().bfo_ray_links(ctypes.byref(p), _ptr(pos), _ptr(tri), int(tri.shape[0]), np.float32(half_thickness),
+                        _ptr(np.ascontiguousarray(mask, np.uint8)), _ptr(np.ascontiguousarray(wall4, np.float32)),
+                        _ptr(links))
+    return links
+
+
+def ray_shell(F, D, A, B, C, h):
+    """First entry in [0, 1] of F + t D into the shell of radius h around
+    triangle ABC, or 2."""
+    arr = [np.ascontiguousarray(v, np.float32) for v in (F, D, A, B, C)]
+    return float(lib().bfo_ray_shell(*[_ptr(a) for a in arr], np.float32(h)))
 
 
 def step_with_links(p: Params, fin, solid_prev, solid_cur, wall4, links, capture=False):

@@ -259,21 +259,23 @@ WING_TIME_STEP = 2.0
 WING_VELOCITY_TO_LATTICE = 0.05
 
 
-def wing_case():
+def wing_case(root=WING_ROOT, periodic=False):
     """(oracle params, SolverConfig, FlappingWing, oracle Wing): the Li &
-    Nabawy planform with an 8-cell mean chord rooted at the centre of a
-    64 x 64 x 32 box in cells (dx 1), cycle 200 time units at 2 per step,
-    far-field faces with a weak stream, sponge 0.04 over 4 cells, TRT,
-    the link-table wall rule, force mode 6."""
+    Nabawy planform with an 8-cell mean chord rooted at `root` (default the
+    centre) of a 64 x 64 x 32 box in cells (dx 1), cycle 200 time units at 2
+    per step, far-field faces with a weak stream and a sponge 0.04 over 4
+    cells (or, periodic, every face periodic and no sponge), TRT, the
+    link-table wall rule, force mode 6."""
     import birdflow_hip as bh
+    bc = ((bfref.BC_PERIODIC, bfref.BC_PERIODIC) if periodic else (bfref.BC_FARFIELD, bfref.BC_FARFIELD),) * 3
     p = bfref.make_params(WING_SHAPE, omega_plus=1.6, omega_minus=1.0 / (0.1875 / (1.0 / 1.6 - 0.5) + 0.5),
-                          bc=((bfref.BC_FARFIELD, bfref.BC_FARFIELD),) * 3, collision=bfref.COLLISION_TRT,
+                          bc=bc, collision=bfref.COLLISION_TRT,
                           wall_rule=bfref.WALL_INTERPOLATED_LINKS, force_mode=6, accumulate_loads=True,
-                          sponge_strength=0.04, sponge_width=4.0, far_u=(0.01, 0.0, 0.0), origin=(0.0, 0.0, 0.0),
-                          dx=1.0, force_scale=1.0, body_position=WING_ROOT)
+                          sponge_strength=0.0 if periodic else 0.04, sponge_width=4.0, far_u=(0.01, 0.0, 0.0),
+                          origin=(0.0, 0.0, 0.0), dx=1.0, force_scale=1.0, body_position=root)
     cfg = solver_config(p, time_step=WING_TIME_STEP, velocity_to_lattice=WING_VELOCITY_TO_LATTICE)
-    return (p, cfg, bh.FlappingWing.li_nabawy(8.0, WING_ROOT, WING_CYCLE),
-            bfref.li_nabawy_wing(8.0, WING_ROOT, WING_CYCLE))
+    return (p, cfg, bh.FlappingWing.li_nabawy(8.0, root, WING_CYCLE),
+            bfref.li_nabawy_wing(8.0, root, WING_CYCLE))
 
 
 # ---------------------------------------------------------------------------

@@ -19,8 +19,11 @@ Sources:
   * ValidationArtifacts/deetjen-dove-indexed-production-coupling.json — per
     step newly covered / uncovered cells, persistent wall links and impulses
     of the 8-step indexed-dove replay (frames 0 and 1 of the mesh).
+  * ValidationArtifacts/deetjen-dove-moving-wall-link-ray-root.json and
+    deetjen-dove-moving-wall-link-intersection.json — the 15 links of the
+    exact link-ray-root audit (cells, directions, exact fractions).
 
-Usage: make_golden.py [deetjen] [coupling] [translating] [shear]  (default: all)
+Usage: make_golden.py [deetjen] [coupling] [translating] [shear] [rayroots]  (default: all)
 """
 from __future__ import annotations
 
@@ -105,7 +108,8 @@ def deetjen():
     hi = (pts.reshape(-1, 3).max(axis=0) + pad).astype(np.float32)
     grid = [max(16, int(np.ceil(np.float32(e / dx))) + 1) for e in (hi - lo).astype(np.float32)]
     assert grid == [art["gridX"], art["gridY"], art["gridZ"]], grid
-    keep = sorted(set(list(range(0, frames, 12)) + [1, 33, 89, 126, 134, 142, 143]))
+    # 26 and 27 bracket the ray-root audit's 26.5 ms phase
+    keep = sorted(set(list(range(0, frames, 12)) + [1, 26, 27, 33, 89, 126, 134, 142, 143]))
     np.savez_compressed(
         os.path.join(OUT, "deetjen_subset.npz"),
         frames=np.asarray(keep, np.int32), points=pts[keep], times=times, triangles=tri.astype(np.uint32),
@@ -144,8 +148,60 @@ def coupling():
         json.dump(out, fh, indent=1)
 
 
+def ray_roots():
+    """The 15 links of the reference's exact link-ray-root audit
+    (deetjen-dove-moving-wall-link-ray-root.json, with the solid / fluid
+    cells and production fractions of deetjen-dove-moving-wall-link-
+    intersection.json) on its D12 and D16 grids at 26.5 ms: each link's
+    solid cell, direction toward the fluid cell, and the fluid-to-
+    intersection fraction of the global union's first root (256-interval
+    scan + 48 bisections in the reference).  The grids' origin and spacing
+    are recovered from the archived intersection points (least squares over
+    every record; residual printed)."""
+    inter = json.load(open(os.path.join(REF, "ValidationArtifacts", "deetjen-dove-moving-wall-link-intersection.json")))
+    roots = json.load(open(os.path.join(REF, "ValidationArtifacts", "deetjen-dove-moving-wall-link-ray-root.json")))
+    c = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0),
+         (1, -1, 0), (-1, 1, 0), (1, 0, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1), (0, 1, 1), (0, -1, -1), (0, 1, -1),
+         (0, -1, 1)]
+    out = {"timeSeconds": None, "cases": {}}
+    for case in ("d12", "d16"):
+        outliers = inter[case]["outliers"]
+        rows, rhs = [], []
+        for o in outliers:
+            cell = np.asarray(o["cellCoordinate"], float)
+            d = np.asarray(c[o["directionIndex"]], float)
+            for a in range(3):
+                r = np.zeros(4)
+                r[a] = 1.0
+                r[3] = cell[a] + 0.5 + d[a] * (1.0 - o["fluidToIntersectionFraction"])
+                rows.append(r)
+                rhs.append(o["intersectionMeters"][a])
+        sol = np.linalg.lstsq(np.asarray(rows), np.asarray(rhs), rcond=None)[0]
+        residual = float(np.abs(np.asarray(rows) @ sol - np.asarray(rhs)).max())
+        print(f"{case}: origin {sol[:3]}, dx {sol[3]}, max residual {residual:.2e} m")
+        links = []
+        for smp in roots[case]["samples"]:
+            o = outliers[smp["sourceOutlierIndex"]]
+            assert o["cellCoordinate"] == smp["cellCoordinate"] and o["directionIndex"] == smp["directionIndex"]
+            links.append({"solidCell": o["cellCoordinate"], "fluidCell": o["neighborCellCoordinate"],
+                          "directionIndex": o["directionIndex"],
+                          "productionFluidToIntersectionFraction": o["fluidToIntersectionFraction"],
+                          "exactGlobalFluidToIntersectionFraction": smp["exactGlobalFluidToIntersectionFraction"],
+                          "exactGlobalRootTriangleIndex": smp["exactGlobalRootTriangleIndex"],
+                          "exactGlobalRootClosureResidualCells": smp["exactGlobalRootClosureResidualCells"]})
+        out["cases"][case] = {"grid": [inter[case]["gridX"], inter[case]["gridY"], inter[case]["gridZ"]],
+                              "origin": [float(v) for v in sol[:3]], "cellSize": float(sol[3]),
+                              "originResidualMeters": residual, "links": links}
+        out["timeSeconds"] = inter[case]["frozenSourceTimeSeconds"]
+    out["halfThicknessMeters"] = 0.0075
+    with open(os.path.join(OUT, "deetjen_ray_roots.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 if __name__ == "__main__":
-    parts = sys.argv[1:] or ["deetjen", "coupling", "translating", "shear"]
+    parts = sys.argv[1:] or ["deetjen", "coupling", "translating", "shear", "rayroots"]
+    if "rayroots" in parts:
+        ray_roots()
     if "deetjen" in parts:
         deetjen()
     if "coupling" in parts:

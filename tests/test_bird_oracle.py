@@ -323,3 +323,115 @@ def test_measured_wing_angular_velocity_matches_frame_differences():
             # tip twist rate against differences of the tip twist
             dtw = (row(fp, f"{name}_chord")[3] - row(fm, f"{name}_chord")[3]) / (2 * h)
             assert abs(dtw - row(f, f"{name}_angular")[3]) <= 2e-3 * abs(dtw) + 1e-3
+
+
+# ---------------------------------------------------------------------------
+# Prescribed-wing hinge reaction (updateWingInertialReaction,
+# BirdFlow.metal:6493-6572): closed forms on hand-built frames.
+# ---------------------------------------------------------------------------
+def wing_frame(theta, omega, root=(0.0, 0.0, 0.0), body=(0.0, 0.0, 0.0)):
+    """A frame whose two wings turn about the z axis: span (cos t, sin t, 0),
+    chord (-sin t, cos t, 0), normal z, angular velocity (0, 0, omega); the
+    right wing mirrors the left across the x-z plane (span (cos t, -sin t, 0),
+    angular velocity (0, 0, -omega))."""
+    f = np.zeros((14, 4), np.float32)
+    f[0, :3] = body
+    c, s = math.cos(theta), math.sin(theta)
+    f[4, :3], f[5, :3], f[6, :3], f[7, :3], f[8, :3] = root, (-s, c, 0.0), (c, s, 0.0), (0, 0, 1), (0, 0, omega)
+    f[9, :3], f[10, :3], f[11, :3], f[12, :3], f[13, :3] = ((root[0], -root[1], root[2]), (-s, -c, 0.0), (c, -s, 0.0),
+                                                            (0, 0, -1), (0, 0, -omega))
+    return f
+
+
+def test_wing_reaction_of_a_centred_isotropic_wing_is_the_spin_change():
+    # centre of mass on the hinge: no linear momentum; isotropic inertia I:
+    # angular momentum I w whatever the axes, so the reaction torque is
+    # -I (w2 - w1) / dt and the force zero
+    d = bfref.make_wing_dynamics(0.03, (0.0, 0.0, 0.0), (2e-4, 2e-4, 2e-4))
+    dt = 0.002
+    f1, f2 = wing_frame(0.3, 5.0), wing_frame(0.31, 7.0)
+    _, prev = bfref.wing_reaction(f1, d, d, dt, np.zeros(16), initialize=True)
+    r, _ = bfref.wing_reaction(f2, d, d, dt, prev)
+    assert np.abs(r[0, :3]).max() == 0.0 and np.abs(r[2, :3]).max() == 0.0
+    assert r[1, 2] == pytest.approx(-2e-4 * (7.0 - 5.0) / dt, rel=1e-5)
+    assert r[3, 2] == pytest.approx(+2e-4 * (7.0 - 5.0) / dt, rel=1e-5)
+    assert abs(r[1, 0]) + abs(r[1, 1]) <= 1e-9
+
+
+@pytest.mark.parametrize("omega,dt", [(20.0, 0.001), (3.0, 0.004)])
+def test_wing_reaction_of_a_steady_stroke_is_the_centrifugal_pull(omega, dt):
+    # constant rate w about z, centre of mass r along the span, hinge at the
+    # body position: P = m w r chord turns with the wing, so the reaction
+    # -dP/dt = m w r (2 sin(w dt / 2) / dt) along the mean span (outward: the
+    # wing pulls the hinge), and L = m w r^2 z + I_n w z is constant (zero
+    # torque); the mirrored right wing gives the mirrored force
+    m, r = 0.04, 0.12
+    d = bfref.make_wing_dynamics(m, (0.0, r, 0.0), (1e-4, 2e-5, 3e-4))
+    t1 = 0.4
+    t2 = t1 + omega * dt
+    f1, f2 = wing_frame(t1, omega), wing_frame(t2, omega)
+    _, prev = bfref.wing_reaction(f1, d, d, dt, np.zeros(16), initialize=True)
+    rxn, _ = bfref.wing_reaction(f2, d, d, dt, prev)
+    mid = 0.5 * (t1 + t2)
+    span_mid = np.array([math.cos(mid), math.sin(mid), 0.0])
+    magnitude = m * omega * r * 2.0 * math.sin(omega * dt / 2.0) / dt
+    force = rxn[0, :3].astype(np.float64)
+    assert np.linalg.norm(force) == pytest.approx(magnitude, rel=2e-4)
+    assert float(np.dot(force, span_mid)) == pytest.approx(magnitude, rel=2e-4)
+    assert np.abs(rxn[1, :3]).max() <= 1e-4 * magnitude * r
+    right = rxn[2, :3].astype(np.float64)
+    assert right == pytest.approx(force * np.array([1.0, -1.0, 1.0]), rel=1e-4, abs=1e-6 * magnitude)
+
+
+def test_wing_reaction_torque_of_an_offset_hinge():
+    # hinge at distance a along x from the body position, centre of mass on
+    # the hinge, no inertia, the wing translating with the body frame... the
+    # momentum is zero; then the centre of mass r along the span with the
+    # hinge offset: L = (a x + r span) x P adds a x P to the centred case,
+    # so the torque reaction gains a x (force reaction)
+    m, r, a = 0.02, 0.1, 0.05
+    d = bfref.make_wing_dynamics(m, (0.0, r, 0.0), (0.0, 0.0, 0.0))
+    omega, dt = 10.0, 0.002
+    f1, f2 = wing_frame(0.2, omega, root=(a, 0.0, 0.0)), wing_frame(0.2 + omega * dt, omega, root=(a, 0.0, 0.0))
+    g1, g2 = wing_frame(0.2, omega), wing_frame(0.2 + omega * dt, omega)
+    _, p1 = bfref.wing_reaction(f1, d, d, dt, np.zeros(16), initialize=True)
+    off, _ = bfref.wing_reaction(f2, d, d, dt, p1)
+    _, q1 = bfref.wing_reaction(g1, d, d, dt, np.zeros(16), initialize=True)
+    cen, _ = bfref.wing_reaction(g2, d, d, dt, q1)
+    assert off[0, :3] == pytest.approx(cen[0, :3], rel=1e-5)
+    extra = np.cross(np.array([a, 0.0, 0.0]), off[0, :3].astype(np.float64))
+    assert (off[1, :3] - cen[1, :3]).astype(np.float64) == pytest.approx(extra, rel=1e-3, abs=1e-7)
+
+
+def test_wing_reaction_first_frame_is_zero_and_of_the_demonstration_bird_is_mirrored():
+    bird = bfref.make_bird(**DEMO)
+    d = bfref.make_wing_dynamics(0.02, (0.03, 0.09, 0.0), (4e-5, 1e-5, 5e-5))
+    dt = 0.001
+    f0 = bfref.bird_prepare(bird, (0.5, 0.4, 0.3), time=0.010, dx=0.004)
+    f1 = bfref.bird_prepare(bird, (0.5, 0.4, 0.3), time=0.011, dx=0.004)
+    r0, prev = bfref.wing_reaction(f0, d, d, dt, np.full(16, 7.0), initialize=True)
+    assert np.abs(r0).max() == 0.0
+    r1, _ = bfref.wing_reaction(f1, d, d, dt, prev)
+    assert np.abs(r1[0, :3]).max() > 0.0
+    # the symmetric bird: the right wing's reaction mirrors the left one's
+    # across the body x-z plane (force y flips; torque x and z flip)
+    assert r1[2, :3] == pytest.approx(r1[0, :3] * np.array([1, -1, 1]), rel=1e-5, abs=1e-9)
+    assert r1[3, :3] == pytest.approx(r1[1, :3] * np.array([-1, 1, -1]), rel=1e-5, abs=1e-9)
+
+
+def test_wing_reaction_uses_the_inertia_about_each_axis():
+    # anisotropic inertia (I_chord, I_span, I_normal): a rate change about
+    # the normal alone is resisted by I_normal, one about the span (a wing
+    # pitching about its own span) by I_span
+    d = bfref.make_wing_dynamics(0.0, (0.0, 0.0, 0.0), (1e-4, 2e-4, 3e-4))
+    dt = 0.001
+    f1, f2 = wing_frame(0.0, 4.0), wing_frame(0.0, 9.0)
+    _, prev = bfref.wing_reaction(f1, d, d, dt, np.zeros(16), initialize=True)
+    r, _ = bfref.wing_reaction(f2, d, d, dt, prev)
+    assert r[1, 2] == pytest.approx(-3e-4 * 5.0 / dt, rel=1e-5)
+    g1, g2 = wing_frame(0.0, 0.0), wing_frame(0.0, 0.0)
+    g1[8, :3] = (2.0, 0.0, 0.0)          # theta 0: the span is x
+    g2[8, :3] = (6.0, 0.0, 0.0)
+    _, prev = bfref.wing_reaction(g1, d, d, dt, np.zeros(16), initialize=True)
+    r, _ = bfref.wing_reaction(g2, d, d, dt, prev)
+    assert r[1, 0] == pytest.approx(-2e-4 * 4.0 / dt, rel=1e-5)

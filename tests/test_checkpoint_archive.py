@@ -112,6 +112,15 @@ def test_manifest_checks(tmp_path):
     rewrite(schema=2)
     with pytest.raises(checkpoint.CheckpointError, match="schema"):
         checkpoint.read_archive(d)
+    # the reference's own schema-1 manifest (no format field) is refused,
+    # not misread
+    reference_like = {k: v for k, v in good.items() if k != "format"}
+    json.dump(dict(reference_like, populationBytes=0, populationSHA256="0" * 64), open(mpath, "w"))
+    with pytest.raises(checkpoint.CheckpointError, match="not a birdflow-hip archive"):
+        checkpoint.read_archive(d)
+    rewrite(format="birdflow-metal")
+    with pytest.raises(checkpoint.CheckpointError, match="not a birdflow-hip archive"):
+        checkpoint.read_archive(d)
     files = json.loads(json.dumps(good["files"]))
     files["populations.bin"]["bytes"] += 1
     rewrite(files=files)

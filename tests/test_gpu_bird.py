@@ -186,6 +186,82 @@ def test_bird_free_flight_matches_oracle():
     assert np.array_equal(g.view(np.uint32), f.view(np.uint32))
 
 
+WING_DYNAMICS = ((0.004, (0.004, 0.012, 0.0), (3e-8, 2e-8, 5e-8)),
+                 (0.004, (0.004, 0.012, 0.0), (3e-8, 2e-8, 5e-8)))
+
+
+def test_bird_free_flight_with_wing_dynamics_matches_oracle():
+    """The prescribed-wing hinge reaction (updateWingInertialReaction,
+    BirdFlow.metal:6493-6572) in free flight: each step's reaction equals the
+    oracle's bit for bit (zero on the first frame), the body history follows
+    the oracle integrating loads plus reactions to 1e-12, and the reaction
+    moves the body measurably against the same run without it."""
+    p, cfg, bird, obird, sc = cases.compact_bird_case(free_flight=True)
+    cfg.time_step = 0.002
+    steps = 10
+    params = dict(mass=0.025, inertia=(4e-6, 7e-6, 8e-6), gravity=(0.0, 0.0, -9.81), external_force=(0.0, 0.0, 0.0),
+                  external_torque=(0.0, 0.0, 0.0), substeps=2)
+    state0 = np.array(list(p.body_position) + [0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    left, right = WING_DYNAMICS
+    odl, odr = bfref.make_wing_dynamics(*left), bfref.make_wing_dynamics(*right)
+    say(f"case: free-flying compact bird with wing dynamics {WING_DYNAMICS}, {steps} steps, strict FP32, FP64 body")
+    hist, reactions = {}, []
+    for dynamics in (True, False):
+        with bh.BirdFlowSimulation(cfg) as sim:
+            sim.set_body(state0, bh.BodyParameters(mass=params["mass"], inertia=params["inertia"],
+                                                   gravity=params["gravity"], substeps=2))
+            sim.set_bird(bird)
+            if dynamics:
+                sim.set_wing_dynamics(left, right)
+                r0 = sim.wing_reaction()
+            sim.init_equilibrium()
+            for _ in range(steps):
+                sim.advance(1)
+                if dynamics:
+                    reactions.append(sim.wing_reaction())
+            hist[dynamics] = sim.body_history()
+    assert np.abs(r0).max() == 0.0
+    body = bfref.make_body(position=state0[:3], linear_velocity=state0[7:10])
+
+    def pose():
+        return bfref.body_array(body)
+
+    dt = float(np.float32(cfg.time_step))
+    f_o = cases.bird_frame_oracle(obird, pose(), geometry_time(cfg, 0), p.dx)
+    _, prev_m = bfref.wing_reaction(f_o, odl, odr, np.float32(cfg.time_step), np.zeros(16), initialize=True)
+    m, w = bfref.bird_build(p, obird, f_o, None, cfg.velocity_to_lattice)
+    f = bfref.init_equilibrium(p, m, w)
+    prev = m
+    states_o, reactions_o = [], []
+    for s in range(1, steps + 1):
+        fr = cases.bird_frame_oracle(obird, pose(), geometry_time(cfg, s), p.dx)
+        rxn, prev_m = bfref.wing_reaction(fr, odl, odr, np.float32(cfg.time_step), prev_m)
+        reactions_o.append(rxn)
+        m, w = bfref.bird_build(p, obird, fr, prev, cfg.velocity_to_lattice)
+        for i in range(3):
+            p.body_position[i] = body.position[i]
+        f, ld, _, _ = bfref.step(p, f, prev, m, w)
+        extra = dict(params)
+        rx = rxn.astype(np.float64)
+        # the device adds (load + external) + left + right in FP64
+        extra_load = np.array(ld, np.float64).copy()
+        for i in range(3):
+            extra_load[i] = ((ld[i] + 0.0) + rx[0, i]) + rx[2, i]
+            extra_load[3 + i] = ((ld[3 + i] + 0.0) + rx[1, i]) + rx[3, i]
+        cases.integrate_body_oracle(body, extra_load, extra, dt)
+        states_o.append(pose())
+        prev = m
+    reactions, reactions_o, states_o = np.array(reactions), np.array(reactions_o), np.array(states_o)
+    rerr = int((reactions.view(np.uint32) != reactions_o.view(np.uint32)).sum())
+    berr = float(np.abs(hist[True] - states_o).max())
+    effect = float(np.abs(hist[True][:, 7:13] - hist[False][:, 7:13]).max())
+    say(f"  reactions differing {rerr} of {reactions.size}; max |reaction force| {np.abs(reactions[:, 0, :3]).max():.3e} N; "
+        f"body history max |diff| vs oracle {berr:.3e}; velocity change from the reaction {effect:.3e}")
+    assert rerr == 0
+    assert berr <= 1e-12
+    assert effect > 1e-6
+
+
 def test_bird_batches_are_invariant():
     # MetalExecutionTests.swift:472-591: one call of 16 steps against 16
     # calls of one step (loads, fields, body state identical).

@@ -1,0 +1,214 @@
+"""The workloads at their stated sizes.
+
+* Flapping plate (256 x 128 x 128, RR3, plate 48 x 16 x 2 cells as 12
+  triangles at x = 64, heave 8 sin(2 pi t / 800), pitch 30 deg sin(2 pi t /
+  800 + pi / 2), inflow u_x = 0.05 through a far-field inlet at -x, zero-
+  gradient outlet at +x, periodic y / z, tau+ 0.6, interpolated moving walls,
+  FP64 loads):
+    - the first 200 steps against the FP32 oracle: populations (relative L2
+      bar 1e-5) and force history (relative bar 1e-4); strict FP32 makes both
+      exact;
+    - all 4,000 steps: finite loads, fluid mass within 1e-3 of its start, and
+      the periodic state: the lift of the fifth period (steps 3,200-4,000)
+      repeats the fourth's.
+* Synthetic bird (512 x 256 x 256): all 2,400 steps, loads finite, mass
+  within 1e-3, the limiter's restriction below 1 (tests/test_gpu_bird_mesh.py
+  compares its first steps with the oracle).
+* Precision form (the bird scaled to 1024 x 512 x 512, 268M cells): FP16 and
+  BF16 offset storage against the FP32 path at full size over 200 steps
+  (force history, velocity L2, mass drift), and the 128 x 64 x 64 twin of the
+  same bird in FP32, FP16 and BF16 against an FP64 oracle run of the same
+  inputs (velocity L2, mass drift, force history).  Bars: FP16 within 0.5 %
+  of the reference's force history, BF16 within 5 % (8 mantissa bits).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import bfref
+import birdflow_hip as bh
+import cases
+from birdflow_hip import workloads
+
+pytestmark = pytest.mark.gpu
+
+PLATE_SHAPE = (256, 128, 128)
+PLATE_TAU = 0.6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def plate_case(steps):
+    p = cases.plate_params(PLATE_SHAPE, collision=bfref.COLLISION_RR3, omega_plus=1.0 / PLATE_TAU, inflow=0.05)
+    seq = workloads.flapping_plate(steps, center=(64.0, 64.0, 64.0), span=48.0, chord=16.0, thickness=2.0,
+                                   heave=8.0, pitch_deg=30.0, period=800.0, half_thickness=0.5)
+    return p, seq
+
+
+def test_plate_first_200_steps_match_oracle():
+    steps = 200
+    p, seq = plate_case(steps)
+    say(f"case: flapping plate {PLATE_SHAPE}, {len(seq.triangles)} triangles, RR3, strict FP32, {steps} steps vs oracle")
+    f_o, loads_o, _ = cases.run_mesh_oracle(p, seq, steps)
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        sim.advance(steps)
+        g = sim.read_populations()
+        loads = sim.load_history(steps)
+    rel_l2 = float(np.linalg.norm((g - f_o).astype(np.float64)) / np.linalg.norm(f_o.astype(np.float64)))
+    force = float(np.abs(loads[:, :3] - loads_o[:, :3]).max() / np.abs(loads_o[:, :3]).max())
+    differing = int((g.view(np.uint32) != f_o.view(np.uint32)).sum())
+    say(f"  populations rel L2 {rel_l2:.3e} ({differing} differing), force history max rel err {force:.3e}, "
+        f"last force {loads[-1, :3]}")
+    assert rel_l2 <= 1e-5 and force <= 1e-4
+    assert differing == 0 and force <= 1e-12
+
+
+def test_plate_4000_steps_properties():
+    steps = 4000
+    period = 800
+    p, seq = plate_case(steps)
+    say(f"case: flapping plate {PLATE_SHAPE}, RR3, {steps} steps: finite, mass, periodic lift")
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        m0 = sim.fluid_momentum()[0]
+        sim.advance(steps)
+        loads = sim.load_history(steps)
+        m1 = sim.fluid_momentum()[0]
+    drift = abs(m1 - m0) / m0
+    lift = loads[:, 2]
+    p4, p5 = lift[3 * period:4 * period], lift[4 * period:5 * period]
+    amp = float(np.abs(p5).max())
+    repeat = float(np.sqrt(np.mean((p5 - p4) ** 2)) / max(np.sqrt(np.mean(p5 ** 2)), 1e-30))
+    say(f"  loads finite {bool(np.isfinite(loads).all())}; fluid mass {m0:.6e} -> {m1:.6e} (drift {drift:.2e}); "
+        f"lift period 4 mean {p4.mean():.4e}, period 5 mean {p5.mean():.4e}, max |lift| {amp:.4e}, "
+        f"period-to-period rms difference / rms {repeat:.3e}; limiter activations {loads[:, 6].sum():.0f}")
+    assert np.isfinite(loads).all()
+    assert drift <= 1e-3
+    assert repeat <= 0.1
+
+
+def bird_params(shape, storage=bh.STORAGE_FP32, tau=0.52):
+    s = shape[0] / 512.0
+    center = (160.0 * s, 128.0 * s, 128.0 * s)
+    p = bfref.make_params(shape, omega_plus=1.0 / tau, omega_minus=1.0 / (0.5 + (3.0 / 16.0) / (tau - 0.5)),
+                          bc=((bfref.BC_FARFIELD, bfref.BC_OUTLET), (bfref.BC_PERIODIC, bfref.BC_PERIODIC),
+                              (bfref.BC_PERIODIC, bfref.BC_PERIODIC)),
+                          collision=bfref.COLLISION_RR3, wall_rule=bfref.WALL_INTERPOLATED_SDF, force_mode=6,
+                          accumulate_loads=True, far_u=(0.04, 0.0, 0.0), body_position=center)
+    return p, cases.solver_config(p, storage=storage)
+
+
+def bird_sequence(shape, steps):
+    s = shape[0] / 512.0
+    center = (160.0 * s, 128.0 * s, 128.0 * s)
+    if s == 1.0:
+        return workloads.synthetic_bird(steps, seed=7, center=center)
+    return workloads.synthetic_bird(steps, seed=7, center=center, body_radii=(40.0 * s, 14.0 * s, 14.0 * s),
+                                    half_span=100.0 * s, root_chord=40.0 * s, tip_chord=12.0 * s,
+                                    tail=(30.0 * s, 20.0 * s), period=1200.0 * s)
+
+
+def test_bird_2400_steps_properties():
+    steps = 2400
+    shape = (512, 256, 256)
+    p, cfg = bird_params(shape)
+    seq = bird_sequence(shape, steps)
+    say(f"case: synthetic bird {shape}, {len(seq.triangles)} triangles, RR3, {steps} steps: finite, mass, limiter")
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        m0 = sim.fluid_momentum()[0]
+        sim.advance(steps)
+        loads = sim.load_history(steps)
+        m1 = sim.fluid_momentum()[0]
+    drift = abs(m1 - m0) / m0
+    say(f"  loads finite {bool(np.isfinite(loads).all())}; mass drift {drift:.2e}; force at the end {loads[-1, :3]}; "
+        f"mean drag {loads[1200:, 0].mean():.4e}, mean lift {loads[1200:, 2].mean():.4e}; "
+        f"limiter activations {loads[:, 6].sum():.0f}, max restriction {loads[:, 7].max():.4f}")
+    assert np.isfinite(loads).all()
+    assert drift <= 1e-3
+    assert loads[:, 7].max() < 1.0
+
+
+def run_storage(shape, steps, storage, seq):
+    p, cfg = bird_params(shape, storage)
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        m0 = sim.fluid_momentum()[0]
+        sim.advance(steps)
+        loads = sim.load_history(steps)
+        m1 = sim.fluid_momentum()[0]
+        _, u = sim.copy_macroscopic_fields()
+    return loads, (m1 - m0) / m0, u
+
+
+def velocity_rel_l2(u, ref):
+    num = 0.0
+    den = 0.0
+    for k in range(0, u.shape[0], 1 << 24):   # chunks: 268M cells in float64 would need 6 GB at once
+        d = (u[k:k + (1 << 24)] - ref[k:k + (1 << 24)]).astype(np.float64)
+        r = ref[k:k + (1 << 24)].astype(np.float64)
+        num += float((d * d).sum())
+        den += float((r * r).sum())
+    return math.sqrt(num / den)
+
+
+BARS = {bh.STORAGE_FP16_OFFSET: 5e-3, bh.STORAGE_BF16_OFFSET: 5e-2}
+NAMES = {bh.STORAGE_FP32: "FP32", bh.STORAGE_FP16_OFFSET: "FP16", bh.STORAGE_BF16_OFFSET: "BF16"}
+
+
+def test_precision_form_at_full_size_against_fp32():
+    steps = 200
+    shape = (1024, 512, 512)
+    seq = bird_sequence(shape, steps)
+    say(f"case: synthetic bird {shape} (268M cells), {steps} steps, FP16 / BF16 offsets vs FP32 at full size")
+    loads32, drift32, u32 = run_storage(shape, steps, bh.STORAGE_FP32, seq)
+    fmax = float(np.abs(loads32[:, :3]).max())
+    say(f"  FP32: mass drift {drift32:.2e}, max |force| {fmax:.4e}")
+    for storage in (bh.STORAGE_FP16_OFFSET, bh.STORAGE_BF16_OFFSET):
+        loads, drift, u = run_storage(shape, steps, storage, seq)
+        force = float(np.abs(loads[:, :3] - loads32[:, :3]).max() / fmax)
+        vel = velocity_rel_l2(u, u32)
+        del u
+        say(f"  {NAMES[storage]}: force history max diff / max force {force:.3e}, velocity rel L2 {vel:.3e}, "
+            f"mass drift {drift:.2e} (FP32 {drift32:.2e})")
+        assert np.isfinite(loads).all()
+        assert force <= BARS[storage]
+        assert vel <= 10 * BARS[storage]
+        assert abs(drift - drift32) <= 1e-4
+
+
+def test_precision_form_twin_against_fp64_oracle():
+    steps = 200
+    shape = (128, 64, 64)
+    seq = bird_sequence(shape, steps)
+    p, _ = bird_params(shape)
+    say(f"case: synthetic bird twin {shape}, {steps} steps, FP32 / FP16 / BF16 vs the FP64 oracle")
+    f64, loads64, _ = cases.run_mesh_oracle(p, seq, steps, dtype=np.float64)
+    rho64, u64 = bfref.moments(p, f64)
+    mask0, _ = cases.mesh_geometry_oracle(p, seq, float(seq.times[0]))
+    mask_end, _ = cases.mesh_geometry_oracle(p, seq, float(seq.times[0]) + steps)
+    fluid = mask_end == 0
+    mass0 = float((mask0 == 0).sum())            # rho = 1 in every fluid cell at the start
+    drift64 = (float(rho64[fluid].sum()) - mass0) / mass0
+    fmax = float(np.abs(loads64[:, :3]).max())
+    for storage in (bh.STORAGE_FP32, bh.STORAGE_FP16_OFFSET, bh.STORAGE_BF16_OFFSET):
+        loads, drift, u = run_storage(shape, steps, storage, seq)
+        force = float(np.abs(loads[:, :3] - loads64[:, :3]).max() / fmax)
+        mean_force = float(np.abs(loads[:, :3].mean(0) - loads64[:, :3].mean(0)).max() / fmax)
+        vel = velocity_rel_l2(u[fluid], u64[fluid])
+        say(f"  {NAMES[storage]}: force history max diff / max force {force:.3e} (mean {mean_force:.3e}), "
+            f"velocity rel L2 {vel:.3e}, mass drift {drift:.3e} (FP64 {drift64:.3e})")
+        assert np.isfinite(loads).all()
+        assert force <= BARS.get(storage, 1e-3)
+        assert vel <= 10 * BARS.get(storage, 1e-3)
+        assert abs(drift - drift64) <= 1e-4

@@ -50,26 +50,39 @@ def oracle_geometry(p, ow, s, prev):
     return fr, m, w
 
 
-def source_is_solid(mask, q):
-    """Fluid cells whose in-domain source cell - c_q is solid, [N] bool."""
+def source_is_solid(mask, q, periodic=False):
+    """Fluid cells whose source cell - c_q is solid, [N] bool: in-domain
+    sources, or wrapped across the faces when periodic."""
     nx, ny, nz = SHAPE
     m3 = mask.reshape(nz, ny, nx)
     cx, cy, cz = bfref.C[q]
-    src = np.zeros_like(m3)
-    src[max(cz, 0):nz + min(cz, 0), max(cy, 0):ny + min(cy, 0), max(cx, 0):nx + min(cx, 0)] = \
-        m3[max(-cz, 0):nz + min(-cz, 0), max(-cy, 0):ny + min(-cy, 0), max(-cx, 0):nx + min(-cx, 0)]
+    if periodic:
+        src = np.roll(m3, (cz, cy, cx), axis=(0, 1, 2))
+    else:
+        src = np.zeros_like(m3)
+        src[max(cz, 0):nz + min(cz, 0), max(cy, 0):ny + min(cy, 0), max(cx, 0):nx + min(cx, 0)] = \
+            m3[max(-cz, 0):nz + min(-cz, 0), max(-cy, 0):ny + min(-cy, 0), max(-cx, 0):nx + min(-cx, 0)]
     return (src.ravel() != 0) & (mask == 0)
 
 
-def test_wing_geometry_and_links_match_oracle():
-    p, cfg, wing, ow = cases.wing_case()
+# the wing rooted 3 cells inside the +x face with every face periodic: at
+# the stroke reversal its tip reaches the last x plane, so cells of the first
+# plane pull from wing cells across the face
+CROSSING_ROOT = (61.0, 32.0, 16.0)
+
+
+@pytest.mark.parametrize("root,periodic", [(ROOT, False), (CROSSING_ROOT, True)])
+def test_wing_geometry_and_links_match_oracle(root, periodic):
+    p, cfg, wing, ow = cases.wing_case(root, periodic)
     steps = 6
-    say(f"case: Li & Nabawy wing, {SHAPE}, cycle {CYCLE}, {TIME_STEP} per step, geometry and links, steps 0-{steps}")
+    say(f"case: Li & Nabawy wing, {SHAPE}, root {root}, periodic {periodic}, cycle {CYCLE}, {TIME_STEP} per step, "
+        f"geometry and links, steps 0-{steps}")
     with bh.BirdFlowSimulation(cfg) as sim:
         sim.set_flapping_wing(wing)
         sim.init_equilibrium()
         prev = None
         changed = 0
+        total_wrapped = 0
         for s in range(steps + 1):
             if s > 0:
                 sim.advance(1)
@@ -80,14 +93,19 @@ def test_wing_geometry_and_links_match_oracle():
             links_o = bfref.wing_links(p, ow, fr_o, m_o).reshape(19, -1)
             guard = (m_o != 0) if prev is None else ((m_o != 0) | (prev != 0))
             n_links = 0
+            wrapped = 0
             link_ok = True
             for q in range(1, 19):
-                want = source_is_solid(m_o, q)
+                want = source_is_solid(m_o, q, periodic)
                 n_links += int(want.sum())
+                wrapped += int((want & ~source_is_solid(m_o, q)).sum())
                 link_ok &= bool(np.array_equal(links[q][want].view(np.uint32), links_o[q][want].view(np.uint32)))
+                # wrapped sources carry a bisected fraction, not the 1e-4 floor
+                link_ok &= bool((links_o[q][want] > 1e-4).mean() > 0.5) if want.any() else True
             if prev is not None:
                 changed += int((prev != m_o).sum())
-            say(f"  step {s}: wing cells {int((m_o != 0).sum())}, links {n_links}, mask equal "
+            total_wrapped += wrapped
+            say(f"  step {s}: wing cells {int((m_o != 0).sum())}, links {n_links} ({wrapped} across a face), mask equal "
                 f"{np.array_equal(mask, m_o)}, wall equal where read {np.array_equal(wall[guard].view(np.uint32), w_o[guard].view(np.uint32))}, "
                 f"links equal {link_ok}")
             assert np.array_equal(fr.view(np.uint32), fr_o.view(np.uint32))
@@ -96,12 +114,15 @@ def test_wing_geometry_and_links_match_oracle():
             assert link_ok and n_links > 500
             prev = m_o
         assert changed > 0
+        assert (total_wrapped > 0) == periodic
 
 
-def test_wing_fluid_steps_match_oracle_bit_for_bit():
-    p, cfg, wing, ow = cases.wing_case()
+@pytest.mark.parametrize("root,periodic", [(ROOT, False), (CROSSING_ROOT, True)])
+def test_wing_fluid_steps_match_oracle_bit_for_bit(root, periodic):
+    p, cfg, wing, ow = cases.wing_case(root, periodic)
     steps = 8
-    say(f"case: Li & Nabawy wing fluid steps, {SHAPE}, TRT, link rule, mode 6, strict FP32, {steps} steps")
+    say(f"case: Li & Nabawy wing fluid steps, {SHAPE}, root {root}, periodic {periodic}, TRT, link rule, mode 6, "
+        f"strict FP32, {steps} steps")
     with bh.BirdFlowSimulation(cfg) as sim:
         sim.set_flapping_wing(wing)
         sim.init_equilibrium()

@@ -129,23 +129,51 @@ def test_planform_mean_chord():
     assert float(np.mean(norm * (r * (1 - r)) ** (a - 1))) == pytest.approx(1.0, abs=1e-3)
 
 
-def test_link_table_covers_exactly_the_solid_sources():
-    w = wing(chord=8.0, root=(20.5, 20.5, 20.5))
-    p = bfref.make_params((48, 40, 40), omega_plus=1.0, omega_minus=1.0)
-    f = bfref.wing_prepare(w, 0.3 * CYCLE)
+def solid_sources(mask, shape, q, periodic):
+    """Fluid cells whose pull source cell - c_q is solid, [N] bool: wrapped
+    across the faces when periodic, else only in-domain sources."""
+    nx, ny, nz = shape
+    m3 = mask.reshape(nz, ny, nx)
+    cx, cy, cz = bfref.C[q]
+    if periodic:
+        src = np.roll(m3, (cz, cy, cx), axis=(0, 1, 2))
+    else:
+        src = np.zeros_like(m3)
+        src[max(cz, 0):nz + min(cz, 0), max(cy, 0):ny + min(cy, 0), max(cx, 0):nx + min(cx, 0)] = \
+            m3[max(-cz, 0):nz + min(-cz, 0), max(-cy, 0):ny + min(-cy, 0), max(-cx, 0):nx + min(-cx, 0)]
+    return (src.ravel() != 0) & (mask == 0)
+
+
+@pytest.mark.parametrize("root,phase,periodic", [((20.5, 20.5, 20.5), 0.3, True), ((20.5, 20.5, 20.5), 0.3, False),
+                                                 ((44.5, 20.5, 20.5), 0.0, True), ((44.5, 20.5, 20.5), 0.0, False)])
+def test_link_table_covers_exactly_the_solid_sources(root, phase, periodic):
+    """Every fluid cell whose pull source is on the wing has a fraction, and
+    no other cell: with periodic faces the source wraps (a wing rooted 3.5
+    cells from the x = 48 face at the stroke reversal reaches its last plane, so cells of the first
+    plane pull from it), behind a far-field face there is no source."""
+    shape = (48, 40, 40)
+    w = wing(chord=8.0, root=root)
+    bc = ((bfref.BC_PERIODIC, bfref.BC_PERIODIC) if periodic else (bfref.BC_FARFIELD, bfref.BC_FARFIELD),) * 3
+    p = bfref.make_params(shape, omega_plus=1.0, omega_minus=1.0, bc=bc)
+    f = bfref.wing_prepare(w, phase * CYCLE)
     mask, wall = bfref.wing_build(p, w, f)
     assert (mask != 0).sum() > 100
     links = bfref.wing_links(p, w, f, mask).reshape(19, -1)
-    m3 = mask.reshape(40, 40, 48)
+    wrapped = 0
     for q in range(1, 19):
-        cx, cy, cz = bfref.C[q]
-        src = np.zeros_like(m3)
-        src[max(cz, 0):40 + min(cz, 0), max(cy, 0):40 + min(cy, 0), max(cx, 0):48 + min(cx, 0)] = \
-            m3[max(-cz, 0):40 + min(-cz, 0), max(-cy, 0):40 + min(-cy, 0), max(-cx, 0):48 + min(-cx, 0)]
-        want = (src.ravel() != 0) & (mask == 0)
+        want = solid_sources(mask, shape, q, periodic)
+        wrapped += int((want & ~solid_sources(mask, shape, q, False)).sum())
         have = links[q] != 0
         assert np.array_equal(want, have)
         assert (links[q][have] >= 1e-4).all() and (links[q][have] <= 1.0).all()
+    if periodic and root[0] > 40:
+        # the wing crosses the face; its wrapped links are bisected, not
+        # left at the minimum fraction
+        assert wrapped > 20
+        m3 = mask.reshape(40, 40, 48)
+        assert m3[:, :, -1].any()
+    else:
+        assert wrapped == 0
     # on-wing cells carry -1, their guard neighbourhood +1, and the rigid rotation
     assert (wall[mask != 0, 3] == -1).all()
     near = (mask == 0) & (wall[:, 3] == 1.0)
