@@ -307,6 +307,18 @@ bf_status bf_read_wing_links(bf_context* ctx, float* links, uint64_t count);
  * cells whose occupancy changed.  Otherwise BF_ERROR_INVALID_ARGUMENT. */
 bf_status bf_ledger_fluid(bf_context* ctx, double out[4]);
 bf_status bf_ledger_step_sources(bf_context* ctx, double sums[16], uint64_t counts[4]);
+/* bf_ledger_step_sources without allocating.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same kernels and block count, so the same sums bit for bit; the block
+ * partials (at most 4,096 blocks of 16 + 4 values) and the sums live in a
+ * file-scope device array of 655,616 bytes that each GPU holds until the process
+ * ends, instead of buffers allocated on the first ledger call.  Every grid
+ * fits (the kernels stride over cells); a block count above the scratch is
+ * BF_ERROR_INVALID_ARGUMENT.  All contexts on one GPU share the array: calls
+ * that overlap on one GPU race, and a call overwrites the previous call's
+ * contents. */
+bf_status bf_ledger_step_sources_noalloc(bf_context* ctx, double sums[16], uint64_t counts[4]);
 
 /* ---- stepping ---- */
 /* advance(steps:) (BirdFlowSimulation.swift:687-799): per step, geometry at

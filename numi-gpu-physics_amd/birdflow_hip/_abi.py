@@ -50,7 +50,7 @@ EXPORTED_FUNCTIONS = (
     "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity", "bf_mesh_freeze_masks",
     "bf_set_bird", "bf_set_bird_keyframes", "bf_read_bird_frame", "bf_set_bird_wing_dynamics",
     "bf_read_wing_reaction", "bf_ledger_fluid", "bf_ledger_step_sources",
-    "bf_set_flapping_wing", "bf_read_wing_frame", "bf_read_wing_links",
+    "bf_ledger_step_sources_noalloc", "bf_set_flapping_wing", "bf_read_wing_frame", "bf_read_wing_links",
 )
 
 
@@ -212,6 +212,7 @@ def _declare(l):
         "bf_read_wing_links": (st, [vp, vp, ctypes.c_uint64]),
         "bf_ledger_fluid": (st, [vp, vp]),
         "bf_ledger_step_sources": (st, [vp, vp, vp]),
+        "bf_ledger_step_sources_noalloc": (st, [vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(l, name)

@@ -514,13 +514,16 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_ledger_fluid(self._ctx, _ptr(out)))
         return out
 
-    def step_momentum_sources(self) -> dict:
+    def step_momentum_sources(self, allocate: bool = True) -> dict:
         """Momentum sums of the step just completed (lattice units, FP64):
         fluid change, open-boundary, sponge and persistent-link sources as
-        (mass, px, py, pz), and the link / cell counts."""
+        (mass, px, py, pz), and the link / cell counts.  allocate=False
+        goes through bf_ledger_step_sources_noalloc (the same sums from the
+        library's fixed device scratch, no device allocation)."""
         sums = np.zeros(16, np.float64)
         counts = np.zeros(4, np.uint64)
-        self._check(self._lib.bf_ledger_step_sources(self._ctx, _ptr(sums), _ptr(counts)))
+        fn = self._lib.bf_ledger_step_sources if allocate else self._lib.bf_ledger_step_sources_noalloc
+        self._check(fn(self._ctx, _ptr(sums), _ptr(counts)))
         return {"change": sums[0:4].copy(), "open_boundary": sums[4:8].copy(), "sponge": sums[8:12].copy(),
                 "persistent_link": sums[12:16].copy(), "open_boundary_links": int(counts[0]),
                 "sponge_cells": int(counts[1]), "persistent_links": int(counts[2]),

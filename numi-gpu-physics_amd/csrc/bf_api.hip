@@ -796,6 +796,12 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
         delete c;
         return BF_ERROR_WORKING_SET_EXCEEDS_MEMORY;
     }
+    // The no-allocation ledger's device scratch: its address on this device
+    // (looked up once per device, with the device current).
+    if (ledger_scratch_setup(dev) != hipSuccess) {
+        delete c;
+        return BF_ERROR_DEVICE_UNAVAILABLE;
+    }
     // Both streams at the default priority.
     if (hipStreamCreateWithFlags(&c->sa, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->sb, hipStreamNonBlocking) != hipSuccess) {
@@ -1514,10 +1520,14 @@ bf_status bf_read_bird_frame(bf_context* c, float frame[56]) {
     return BF_OK;
 }
 
+static uint32_t ledger_blocks(const bf_context* c) {
+    return std::min<uint32_t>(LEDGER_SCRATCH_BLOCKS, (c->g.n + 255u) / 256u);
+}
+
 static bf_status ensure_ledger(bf_context* c) {
     if (c->ledger_pv) return BF_OK;
     hipError_t e = hipSuccess;
-    const uint32_t blocks = std::min<uint32_t>(4096u, (c->g.n + 255u) / 256u);
+    const uint32_t blocks = ledger_blocks(c);
     DeviceAlloc& A = c->ledger_alloc;
     double* pv = (double*)A.take(e, 16ull * 8 * blocks);
     unsigned long long* pn = (unsigned long long*)A.take(e, 4ull * 8 * blocks);
@@ -1554,31 +1564,56 @@ bf_status bf_ledger_fluid(bf_context* c, double out[4]) {
     return BF_OK;
 }
 
-bf_status bf_ledger_step_sources(bf_context* c, double sums[16], uint64_t counts[4]) {
-    BF_ENTER(c);
-    if (!sums || !counts) return fail(c, BF_ERROR_INVALID_ARGUMENT, "outputs must not be NULL");
-    if (c->ledger_step != c->step || c->ledger_epoch != c->f_writes || c->step < 1)
-        return fail(c, BF_ERROR_INVALID_ARGUMENT,
-                    "the ledger needs the inputs of the last step: call it right after bf_step");
-    if (bf_status r = ensure_ledger(c)) return r;
+// The step-source sums of bf_ledger_step_sources into `sums` / `counts`
+// through the partial and output buffers of `b`.
+static bf_status ledger_step_sources_into(bf_context* c, LedgerBuffers b, double sums[16], uint64_t counts[4]) {
     const StepGeometry sg = geometry_of(c, c->step);
-    LedgerBuffers b{};
     b.f_in = c->f[1 - c->cur];
     b.f_out = c->f[c->cur];
     b.solid_prev = c->ledger_prev;
     b.solid_cur = sg.mask_cur;
     b.wall = sg.wall;
+    hipError_t e = launch_ledger((int)c->cfg.storage, b, make_step_params(c), true, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(sums, b.out_v, 16 * 8, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, b.out_n, 4 * 8, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "step momentum sources");
+    return BF_OK;
+}
+
+static bf_status ledger_sources_ready(bf_context* c, const double* sums, const uint64_t* counts) {
+    if (!sums || !counts) return fail(c, BF_ERROR_INVALID_ARGUMENT, "outputs must not be NULL");
+    if (c->ledger_step != c->step || c->ledger_epoch != c->f_writes || c->step < 1)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT,
+                    "the ledger needs the inputs of the last step: call it right after bf_step");
+    return BF_OK;
+}
+
+bf_status bf_ledger_step_sources(bf_context* c, double sums[16], uint64_t counts[4]) {
+    BF_ENTER(c);
+    if (bf_status r = ledger_sources_ready(c, sums, counts)) return r;
+    if (bf_status r = ensure_ledger(c)) return r;
+    LedgerBuffers b{};
     b.blocks = c->ledger_blocks;
     b.partial_v = c->ledger_pv;
     b.partial_n = c->ledger_pn;
     b.out_v = c->ledger_out;
     b.out_n = c->ledger_out_n;
-    hipError_t e = launch_ledger((int)c->cfg.storage, b, make_step_params(c), true, c->sb);
-    if (e == hipSuccess) e = hipMemcpyAsync(sums, c->ledger_out, 16 * 8, hipMemcpyDeviceToHost, c->sb);
-    if (e == hipSuccess) e = hipMemcpyAsync(counts, c->ledger_out_n, 4 * 8, hipMemcpyDeviceToHost, c->sb);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
-    if (e != hipSuccess) return hip_fail(c, e, "step momentum sources");
-    return BF_OK;
+    return ledger_step_sources_into(c, b, sums, counts);
+}
+
+bf_status bf_ledger_step_sources_noalloc(bf_context* c, double sums[16], uint64_t counts[4]) {
+    BF_ENTER(c);
+    if (bf_status r = ledger_sources_ready(c, sums, counts)) return r;
+    const uint32_t blocks = ledger_blocks(c);
+    if (blocks > LEDGER_SCRATCH_BLOCKS)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT,
+                    "the ledger needs " + std::to_string(blocks) + " block partials; the no-allocation scratch holds " +
+                        std::to_string(LEDGER_SCRATCH_BLOCKS));
+    LedgerBuffers b{};
+    hipError_t e = ledger_scratch_buffers(c->device, blocks, b);
+    if (e != hipSuccess) return hip_fail(c, e, "no-allocation ledger scratch", false);
+    return ledger_step_sources_into(c, b, sums, counts);
 }
 
 bf_status bf_step(bf_context* c, uint32_t steps) {

@@ -288,6 +288,15 @@ struct LedgerBuffers {
     unsigned long long* out_n;
 };
 hipError_t launch_ledger(int storage, const LedgerBuffers& b, const StepParams& p, bool sources, hipStream_t s);
+// Block count of the ledger kernels (grid-stride over cells): at most this.
+constexpr uint32_t LEDGER_SCRATCH_BLOCKS = 4096;
+// The no-allocation ledger's file-scope device scratch (bf_ledger.hip):
+// setup looks up its address on `device` (the current device) once;
+// buffers fills b's partials and outputs from it for `blocks` blocks
+// (hipErrorInvalidValue beyond LEDGER_SCRATCH_BLOCKS, hipErrorNotInitialized
+// before setup on that device).
+hipError_t ledger_scratch_setup(int device);
+hipError_t ledger_scratch_buffers(int device, uint32_t blocks, LedgerBuffers& b);
 hipError_t launch_bird_geometry(const BirdShape& shape, const BirdBuffers& b, const BirdParams& p, hipStream_t s,
                                 const WingDynamics* wing_dynamics = nullptr);
 hipError_t launch_wing_geometry(const WingShape& shape, const WingBuffers& b, const BirdParams& p, hipStream_t s);

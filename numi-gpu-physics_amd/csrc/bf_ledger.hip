@@ -23,6 +23,8 @@
 //        density far_rho, 2 w_q rho (c.u_wall) / cs^2.
 // Counts: open-boundary links, sponge cells, persistent links, cells whose
 // occupancy changed.
+#include <mutex>
+
 #include "bf_fluid.h"
 
 namespace bf {
@@ -207,6 +209,62 @@ static hipError_t ledger_s(const LedgerBuffers& b, const StepParams& p, bool sou
                            (const unsigned long long*)nullptr, b.blocks, 4, 0, b.out_v, b.out_n);
     }
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation ledger entry point
+// (bf_ledger_step_sources_noalloc): the block partials and the reduced sums
+// live in a file-scope device array instead of buffers the context allocates
+// on its first ledger call.  Every GPU holds its own copy of the array until
+// the process ends, at its own address, looked up once per device
+// (hipGetSymbolAddress, with that device current) and kept per device; host
+// code uses only that address.  All contexts on one GPU share the array, so
+// calls that overlap on one GPU race.  Nothing in it needs clearing: each
+// launch writes every partial it later reads, and the finishing kernel
+// writes the outputs.
+// ---------------------------------------------------------------------------
+constexpr size_t SCRATCH_V = (size_t)LEDGER_SCRATCH_BLOCKS * LEDGER_SUMS * sizeof(double);
+constexpr size_t SCRATCH_N = (size_t)LEDGER_SCRATCH_BLOCKS * LEDGER_COUNTS * sizeof(unsigned long long);
+constexpr size_t SCRATCH_OUT_V = SCRATCH_V + SCRATCH_N;
+constexpr size_t SCRATCH_OUT_N = SCRATCH_OUT_V + LEDGER_SUMS * sizeof(double);
+constexpr size_t SCRATCH_BYTES = (SCRATCH_OUT_N + LEDGER_COUNTS * sizeof(unsigned long long) + 255) / 256 * 256;
+static_assert(SCRATCH_BYTES <= 1u << 20, "the ledger scratch stays under 1 MiB per GPU");
+
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mutex;
+unsigned char* scratch_address[SCRATCH_DEVICES] = {};
+}  // namespace
+
+hipError_t ledger_scratch_setup(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (scratch_address[device]) return hipSuccess;
+    unsigned char* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    scratch_address[device] = ptr;
+    return hipSuccess;
+}
+
+hipError_t ledger_scratch_buffers(int device, uint32_t blocks, LedgerBuffers& b) {
+    if (blocks == 0 || blocks > LEDGER_SCRATCH_BLOCKS) return hipErrorInvalidValue;
+    if (device < 0 || device >= SCRATCH_DEVICES) return hipErrorInvalidDevice;
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mutex);
+        base = scratch_address[device];
+    }
+    if (!base) return hipErrorNotInitialized;
+    b.blocks = blocks;
+    b.partial_v = (double*)base;
+    b.partial_n = (unsigned long long*)(base + SCRATCH_V);
+    b.out_v = (double*)(base + SCRATCH_OUT_V);
+    b.out_n = (unsigned long long*)(base + SCRATCH_OUT_N);
+    return hipSuccess;
 }
 
 hipError_t launch_ledger(int storage, const LedgerBuffers& b, const StepParams& p, bool sources, hipStream_t s) {
