@@ -69,9 +69,17 @@ enum { BF_COLLISION_TRT = 0, BF_COLLISION_TRT_SYMMETRIC_LIMITER = 1,
 /* Wall rule: halfway bounce-back, or Bouzidi interpolated bounce-back with the
  * link fraction from the signed-distance ratio (BirdFlow.metal:2044-2076,
  * 2729-2806), or the interpolated path at q = 1/2 (translating-sphere
- * canonical, BirdFlow.metal:2340-2345). */
+ * canonical, BirdFlow.metal:2340-2345), or a link table (the prescribed
+ * wing's bisected fractions, BirdFlow.metal:2738-2745).
+ * BF_WALL_INTERPOLATED_RAY (meshes): the exact fraction of each link, the
+ * first entry of the link from the fluid node toward its solid source into
+ * the meshed shell {dist(p, triangle) <= half thickness}, in closed form
+ * (offset faces, edge cylinders, vertex spheres); the reference computes
+ * these roots only in its CPU audits (MetalIndexedBirdSurfaceValidation.swift:
+ * 14245-14444).  A link with no entry within it keeps the signed-distance
+ * ratio.  Masks, wall velocities and link counts are those of the SDF rule. */
 enum { BF_WALL_HALFWAY = 0, BF_WALL_INTERPOLATED_SDF = 1, BF_WALL_INTERPOLATED_HALF = 2,
-       BF_WALL_INTERPOLATED_LINKS = 3 };
+       BF_WALL_INTERPOLATED_LINKS = 3, BF_WALL_INTERPOLATED_RAY = 4 };
 
 /* Population storage: FP32, or FP16 / BF16 offsets f - w_q with FP32
  * arithmetic. */
@@ -122,7 +130,9 @@ typedef struct bf_config {
 typedef struct bf_context bf_context;
 
 /* MetalBackend.init + BirdFlowSimulation.init buffer plan
- * (MetalBackend.swift:14-46, 77-106; BirdFlowSimulation.swift:480-625). */
+ * (MetalBackend.swift:14-46, 77-106; BirdFlowSimulation.swift:480-625).
+ * The populations start at the far-field equilibrium of the empty geometry;
+ * bf_init_equilibrium re-initializes them over the geometry set since. */
 bf_status bf_create(const bf_config* config, bf_context** out);
 void bf_destroy(bf_context* ctx);
 const char* bf_last_error(const bf_context* ctx);
@@ -291,6 +301,12 @@ bf_status bf_read_wing_frame(bf_context* ctx, float frame[24]);
  * fraction for a fluid cell whose source cell - c_q is on the wing, 0
  * elsewhere. */
 bf_status bf_read_wing_links(bf_context* ctx, float* links, uint64_t count);
+/* The current step's exact-root table (BF_WALL_INTERPOLATED_RAY with a mesh)
+ * as links[q * N + cell] (count = 19 N): the clamped fraction of every fluid
+ * cell's link whose source is solid, 0 elsewhere.  Replaces the per-link
+ * roots of the reference's link-ray-root audit
+ * (MetalIndexedBirdSurfaceValidation.swift:14245-14444). */
+bf_status bf_read_ray_links(bf_context* ctx, float* links, uint64_t count);
 
 /* ---- momentum ledger (FP64 on the device) ----
  * The device sums of BirdFlowSimulation.advanceWithCoupledMomentumLedger

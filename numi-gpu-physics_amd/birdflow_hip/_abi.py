@@ -32,7 +32,7 @@ ERROR_STATE_INVALIDATED = 10
 
 BC_PERIODIC, BC_FARFIELD, BC_OUTLET = 0, 1, 2
 COLLISION_TRT, COLLISION_TRT_SYMMETRIC_LIMITER, COLLISION_REGULARIZED, COLLISION_RR3 = 0, 1, 2, 3
-WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF, WALL_INTERPOLATED_LINKS = 0, 1, 2, 3
+WALL_HALFWAY, WALL_INTERPOLATED_SDF, WALL_INTERPOLATED_HALF, WALL_INTERPOLATED_LINKS, WALL_INTERPOLATED_RAY = 0, 1, 2, 3, 4
 STORAGE_FP32, STORAGE_FP16_OFFSET, STORAGE_BF16_OFFSET = 0, 1, 2
 GEOMETRY_STATIC, GEOMETRY_MESH, GEOMETRY_SPHERE, GEOMETRY_BIRD, GEOMETRY_WING = 0, 1, 2, 3, 4
 ARITHMETIC_DEFAULT, ARITHMETIC_STRICT, ARITHMETIC_FUSED = 0, 1, 2
@@ -51,6 +51,7 @@ EXPORTED_FUNCTIONS = (
     "bf_set_bird", "bf_set_bird_keyframes", "bf_read_bird_frame", "bf_set_bird_wing_dynamics",
     "bf_read_wing_reaction", "bf_ledger_fluid", "bf_ledger_step_sources",
     "bf_ledger_step_sources_noalloc", "bf_set_flapping_wing", "bf_read_wing_frame", "bf_read_wing_links",
+    "bf_read_ray_links",
 )
 
 
@@ -210,6 +211,7 @@ def _declare(l):
         "bf_set_flapping_wing": (st, [vp, ctypes.POINTER(FlappingWingC)]),
         "bf_read_wing_frame": (st, [vp, vp]),
         "bf_read_wing_links": (st, [vp, vp, ctypes.c_uint64]),
+        "bf_read_ray_links": (st, [vp, vp, ctypes.c_uint64]),
         "bf_ledger_fluid": (st, [vp, vp]),
         "bf_ledger_step_sources": (st, [vp, vp, vp]),
         "bf_ledger_step_sources_noalloc": (st, [vp, vp, vp]),

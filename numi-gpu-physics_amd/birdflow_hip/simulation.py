@@ -462,6 +462,14 @@ class BirdFlowSimulation:
         self._check(self._lib.bf_read_wing_links(self._ctx, _ptr(out), out.size))
         return out.reshape(19, -1)
 
+    def ray_links(self) -> np.ndarray:
+        """The current step's exact-root link table (wall rule
+        WALL_INTERPOLATED_RAY with a mesh), [19, N]: the clamped fraction of
+        each fluid cell's link whose source is solid, 0 elsewhere."""
+        out = np.zeros(19 * self.n, np.float32)
+        self._check(self._lib.bf_read_ray_links(self._ctx, _ptr(out), out.size))
+        return out.reshape(19, -1)
+
     def population_minimum(self):
         """(value, direction-major index, non_finite) of the smallest current
         population, a non-finite value ranking first (reducePopulationMinimum)."""

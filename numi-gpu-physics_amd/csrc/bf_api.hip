@@ -152,6 +152,12 @@ struct bf_context {
     float4* wing_frame = nullptr;         // [2][WING_FRAME_ROWS]
     DeviceAlloc wing_alloc;
     float* wing_links[2] = {nullptr, nullptr};   // [step parity][tiles][19][256]
+    // exact link fractions of a mesh (BF_WALL_INTERPOLATED_RAY): the table
+    // of each step parity and the scratch list of the tiles it covers
+    DeviceAlloc ray_alloc;
+    float* ray_links[2] = {nullptr, nullptr};    // [step parity][tiles][19][256]
+    uint32_t* ray_list = nullptr;
+    uint32_t* ray_count = nullptr;
     // analytic bird: shape and the prepared frame of each step parity
     BirdShape bird{};
     float4* bird_frame = nullptr;         // [2][BIRD_FRAME_ROWS]
@@ -294,7 +300,8 @@ StepParams make_step_params(const bf_context* c) {
     p.pair_cells = c->pair_cells ? 1 : 0;
     for (int a = 0; a < 3; ++a)
         for (int s = 0; s < 2; ++s) p.bc[a][s] = (int)c->cfg.boundary[a][s];
-    p.wall_rule = (int)c->cfg.wall_rule;
+    // the exact-root rule reads its per-step table like the wing's
+    p.wall_rule = c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY ? WALL_INTERPOLATED_LINKS : (int)c->cfg.wall_rule;
     p.force_mode = (int)c->cfg.force_mode;
     p.load_component = (int)c->cfg.load_component;
     p.selected_part = (int)c->cfg.selected_part;
@@ -425,6 +432,26 @@ hipError_t replay_mask(bf_context* c, int64_t s, uint32_t k, int slot) {
     return e;
 }
 
+RayParams ray_params(const bf_context* c) {
+    RayParams r{};
+    r.g = c->g;
+    for (int a = 0; a < 3; ++a) {
+        r.origin[a] = c->cfg.origin[a];
+        r.bc[a][0] = (int)c->cfg.boundary[a][0];
+        r.bc[a][1] = (int)c->cfg.boundary[a][1];
+    }
+    r.dx = c->cfg.cell_size;
+    r.half_thickness = c->m_half;
+    r.triangle_count = c->m_triangles;
+    // four triangles per block; a mesh of fewer triangles than two
+    // wavefronts per CU splits each triangle's cells among several
+    const uint32_t waves = 2u * c->compute_units;
+    r.triangle_blocks = std::max(1u, std::min((c->m_triangles + 3u) / 4u, 4u * c->compute_units));
+    r.split = std::min(64u, std::max(1u, waves / std::max(1u, c->m_triangles)));
+    r.tile_pass_blocks = c->tile_pass_blocks;
+    return r;
+}
+
 bool use_static_keys(const bf_context* c) {
     return c->static_keys_enabled && c->m_static_count > 0 && !c->move_mesh;
 }
@@ -512,6 +539,18 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         e = launch_mesh_geometry(b, m, c->sb);
         if (e == hipSuccess && c->frozen_count && s >= c->frozen_first && s < c->frozen_first + c->frozen_count)
             e = replay_mask(c, s, (uint32_t)(s - c->frozen_first), mod3(s));
+        if (e == hipSuccess && c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY) {
+            RayBuffers rb{};
+            rb.pos = c->m_pos;
+            rb.tri = c->m_tri;
+            rb.mask = b.mask;
+            rb.wall = b.wall;
+            rb.dyn = b.dyn;
+            rb.list = c->ray_list;
+            rb.list_count = c->ray_count;
+            rb.links = c->ray_links[par2(s)];
+            e = launch_ray_links(rb, ray_params(c), c->sb);
+        }
     } else if (c->source == BF_GEOMETRY_BIRD) {
         // fresh: built from cleared buffers with no previous geometry
         // (the reference's initialization build)
@@ -705,7 +744,8 @@ bf_status bf_device_info(char* name, size_t cap, uint64_t* total) {
 uint64_t bf_allocated_bytes(const bf_context* c) {
     if (!c) return 0;
     return c->alloc.bytes + c->mesh_alloc.bytes + c->static_alloc.bytes + c->ledger_alloc.bytes + c->wing_alloc.bytes +
-           c->keyframe_alloc.bytes + c->adj.alloc.bytes + c->adj.tangent_alloc.bytes + c->frozen_alloc.bytes;
+           c->ray_alloc.bytes + c->keyframe_alloc.bytes + c->adj.alloc.bytes + c->adj.tangent_alloc.bytes +
+           c->frozen_alloc.bytes;
 }
 
 bf_status bf_create(const bf_config* cfg, bf_context** out) {
@@ -714,7 +754,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     const uint64_t n64 = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || n64 >= (1ull << 32)) return BF_ERROR_INVALID_CONFIGURATION;
     if (cfg->storage > BF_STORAGE_BF16_OFFSET || cfg->collision > BF_COLLISION_RR3 ||
-        cfg->wall_rule > BF_WALL_INTERPOLATED_LINKS || cfg->force_mode > 6 || cfg->load_component > 2 ||
+        cfg->wall_rule > BF_WALL_INTERPOLATED_RAY || cfg->force_mode > 6 || cfg->load_component > 2 ||
         cfg->arithmetic > BF_ARITHMETIC_FUSED)
         return BF_ERROR_INVALID_CONFIGURATION;
     if (cfg->arithmetic == BF_ARITHMETIC_STRICT && cfg->storage != BF_STORAGE_FP32)
@@ -872,6 +912,10 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     if (e == hipSuccess) e = hipMemcpyAsync(c->body, body0, sizeof body0, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->safety_metrics, safety0, sizeof safety0, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->safety_event, event0, sizeof event0, hipMemcpyHostToDevice, s);
+    // populations defined from the start: the far-field equilibrium of the
+    // empty geometry (initializePopulations before any body is placed)
+    if (e == hipSuccess)
+        e = launch_init_equilibrium((int)cfg->storage, c->f[c->cur], c->mask[0], c->wall[0], cfg->far_velocity, g, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         bf_destroy(c);
@@ -902,6 +946,7 @@ void bf_destroy(bf_context* c) {
     c->ledger_alloc.release();
     c->keyframe_alloc.release();
     c->wing_alloc.release();
+    c->ray_alloc.release();
     c->adj.alloc.release();
     c->adj.tangent_alloc.release();
     c->frozen_alloc.release();
@@ -1179,6 +1224,22 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
             return fail(c, BF_ERROR_INVALID_ARGUMENT, "degenerate (zero-area) triangle in the first frame");
     }
     if (!(half_thickness > 0.0f)) return fail(c, BF_ERROR_INVALID_ARGUMENT, "half thickness must be positive");
+    if (c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY && !c->ray_links[0]) {
+        // the exact-root tables of both step parities (every tile's 19 x 256
+        // entries; the passes write the tiles next to solid cells) and the
+        // tile list
+        hipError_t e = hipSuccess;
+        const size_t entries = (size_t)c->g.tiles * 19 * TILE_CELLS;
+        for (int k = 0; k < 2; ++k) c->ray_links[k] = (float*)c->ray_alloc.take(e, entries * sizeof(float));
+        c->ray_list = (uint32_t*)c->ray_alloc.take(e, 4ull * c->g.tiles);
+        c->ray_count = (uint32_t*)c->ray_alloc.take(e, 256);
+        if (e != hipSuccess) {
+            c->ray_alloc.release();
+            c->ray_links[0] = c->ray_links[1] = nullptr;
+            c->ray_list = c->ray_count = nullptr;
+            return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("exact-root link tables: ") + hipGetErrorString(e));
+        }
+    }
 
     // Static triangles: every vertex at the same position in every frame
     // (their keys are the same every step unless a moving body carries the
@@ -1432,6 +1493,36 @@ bf_status bf_read_wing_links(bf_context* c, float* links, uint64_t count) {
     return BF_OK;
 }
 
+bf_status bf_read_ray_links(bf_context* c, float* links, uint64_t count) {
+    BF_ENTER(c);
+    if (!links) return fail(c, BF_ERROR_INVALID_ARGUMENT, "links must not be NULL");
+    if (count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "links needs 19 * N floats");
+    if (c->cfg.wall_rule != BF_WALL_INTERPOLATED_RAY || c->source != BF_GEOMETRY_MESH || !c->ray_links[0])
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "no exact-root table: the wall rule is not BF_WALL_INTERPOLATED_RAY "
+                                                  "or no mesh is uploaded");
+    hipError_t e = hipDeviceSynchronize();
+    std::vector<float> t((size_t)c->g.tiles * 19 * TILE_CELLS);
+    std::vector<uint8_t> dyn(c->g.tiles);
+    if (e == hipSuccess)
+        e = hipMemcpy(t.data(), c->ray_links[par2(c->step)], t.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(dyn.data(), c->dyn[mod3(c->step)], dyn.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(c, e, "read exact-root links");
+    const Grid& g = c->g;
+    std::memset(links, 0, sizeof(float) * 19ull * g.n);
+    for (uint32_t tile = 0; tile < g.tiles; ++tile) {
+        if (!dyn[tile]) continue;   // the table's other tiles hold no entry of this step
+        int x0, y0, z;
+        tile_origin(g, tile, x0, y0, z);
+        for (uint32_t lane = 0; lane < TILE_CELLS; ++lane) {
+            const int x = x0 + (int)(lane & (TILE_X - 1)), y = y0 + (int)(lane / TILE_X);
+            if (x >= g.nx || y >= g.ny) continue;
+            const uint64_t gid = (uint64_t)x + (uint64_t)g.nx * ((uint64_t)y + (uint64_t)g.ny * (uint64_t)z);
+            for (int q = 0; q < 19; ++q) links[(uint64_t)q * g.n + gid] = t[((size_t)tile * 19 + q) * TILE_CELLS + lane];
+        }
+    }
+    return BF_OK;
+}
+
 bf_status bf_set_bird_keyframes(bf_context* c, const float* keyframes, uint32_t count) {
     BF_ENTER(c);
     if (count == 0) {
@@ -1623,6 +1714,8 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
     if (steps == 0) return BF_OK;
     if (c->cfg.wall_rule == BF_WALL_INTERPOLATED_LINKS && c->source != BF_GEOMETRY_WING)
         return fail(c, BF_ERROR_INVALID_ARGUMENT, "the link-table wall rule needs the prescribed flapping wing");
+    if (c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY && (c->source != BF_GEOMETRY_MESH || !c->ray_links[0]))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "the exact-root wall rule needs an uploaded mesh");
     const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision, arith = arithmetic_of(c->cfg);
     hipError_t e = hipSuccess;
     // Prologue: geometry (unless already built) and tile list of the first
@@ -1652,7 +1745,7 @@ bf_status bf_step(bf_context* c, uint32_t steps) {
         }
         StepParams p = make_step_params(c);
         // the link table of step s; geometry(s + 1) fills the other slot meanwhile
-        p.links = c->wing_links[par2(s)];
+        p.links = (c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY ? c->ray_links : c->wing_links)[par2(s)];
         p.health = c->health;
         p.step_index = (unsigned long long)s;
         const FluidBuffers fb = fluid_buffers(c, s);
@@ -1946,6 +2039,8 @@ bf_status adjoint_supported(bf_context* c) {
     if (c->source == BF_GEOMETRY_SPHERE || c->source == BF_GEOMETRY_BIRD || c->source == BF_GEOMETRY_WING ||
         g.wall_rule == BF_WALL_INTERPOLATED_LINKS)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers mesh and static geometry");
+    if (g.wall_rule == BF_WALL_INTERPOLATED_RAY)
+        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates the signed-distance link fractions");
     if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "step once after bf_set_geometry with a different previous mask");
     return BF_OK;

@@ -159,6 +159,31 @@ struct MeshBuffers {
     uint32_t seed_count;
 };
 
+// Exact link fractions against the meshed shell (bf_rays.hip): the table
+// [tile][19][256] of the step, read by the step as a link table.
+struct RayParams {
+    Grid g;
+    float origin[3];
+    float dx;
+    float half_thickness;
+    int bc[3][2];
+    uint32_t triangle_count;
+    uint32_t triangle_blocks;  // triangle pass grid (four triangles per block, grid-stride)
+    uint32_t split;            // wavefronts per triangle (its cells split among them), 1..64
+    uint32_t tile_pass_blocks; // init / finalize grid (grid-stride over listed tiles)
+};
+
+struct RayBuffers {
+    const float4* pos;       // prepared vertices of the step
+    const uint32_t* tri;
+    const uint8_t* mask;     // the step's part mask
+    const float4* wall;      // the step's wall field (signed distance in .w)
+    const uint8_t* dyn;      // tiles whose cells pull from a solid cell
+    uint32_t* list;          // scratch: listed tiles
+    uint32_t* list_count;
+    float* links;            // the step's table
+};
+
 // Adjoint of one step (bf_adjoint.hip).  lam_in = dJ/df_s, lam_out =
 // dJ/df_{s-1}; f_prev = f_{s-1}; masks and wall of step s-1 / s; nu:
 // per listed tile 19 x 256 floats; dW: [K][N] wall tangents of step s.
@@ -260,6 +285,7 @@ hipError_t launch_poke_population(int storage, void* f, const Grid& g, unsigned 
 hipError_t launch_convert_populations(int storage, bool pack, const void* src, void* dst, size_t count,
                                       const Grid& g, hipStream_t s);
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s);
+hipError_t launch_ray_links(const RayBuffers& b, const RayParams& p, hipStream_t s);
 // Keys of the listed (static) triangles alone: vertex prepare, then the
 // raster of b.raster_list into b.keys (tiles into b.tiles_new), then the
 // number of keyed cells into *count.

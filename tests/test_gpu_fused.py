@@ -262,6 +262,9 @@ def bird_fp64_loads():
     loads = np.zeros((BIRD_STEPS, 8))
     with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
         sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        # the device populations are stepped too (the fluid kernels' guard
+        # rejects a state never initialized)
+        sim.init_equilibrium()
         prev, wall = sim.read_geometry()
         f = bfref.init_equilibrium(p, prev, wall, np.float64)
         for s in range(BIRD_STEPS):
