@@ -6,9 +6,11 @@ Workload (default): the synthetic articulated bird, 512 x 256 x 256 cells
 with signed-distance link fractions, conservative momentum-exchange loads
 reduced in FP64 every step, link rebuild every step, inflow u_x = 0.04 at -x,
 zero-gradient outlet at +x, periodic y/z, tau+ = 0.52.  Synthetic data: the
-mesh comes from birdflow_hip.workloads.synthetic_bird (seed 7).
+mesh comes from birdflow_hip.workloads (seed 7), uploaded as an articulated
+mesh whose 3-harmonic Fourier wing kinematics the device evaluates every
+step (--mesh frames uploads one recorded frame per step instead).
 
-A step is one full pass of the path: mesh prepare, raster, resolve, fluid
+A step is one full pass of the path: vertex prepare, raster, resolve, fluid
 (fast + general tiles), load reduction.  value = cells x steps / wall time of
 K steps bracketed by stream synchronizations, in MLUPS; the roofline entry is
 the fast fluid kernel (populations only: 152 B per cell update) timed with HIP
@@ -48,27 +50,39 @@ def bird_config(shape, storage, arithmetic="default"):
                                                      128.0 * shape[0] / 512.0))
 
 
-def bird_mesh(steps, shape):
-    """The synthetic bird placed in `shape`; for other than 512 cells along x
+def bird_mesh(steps, shape, form="articulated"):
+    """The synthetic bird placed in `shape`, as an articulated mesh (on-device
+    kinematics) or as one frame per step; for other than 512 cells along x
     the bird (and its stroke period) scale with the domain, so that
     1024x512x512 is the 512x256x256 case scaled 2x per axis."""
     from birdflow_hip import workloads
     s = shape[0] / 512.0
-    if s == 1.0:
-        return workloads.synthetic_bird(steps, seed=7, center=(160.0, 128.0, 128.0))
-    return workloads.synthetic_bird(steps, seed=7, center=(160.0 * s, 128.0 * s, 128.0 * s),
-                                    body_radii=(40.0 * s, 14.0 * s, 14.0 * s), half_span=100.0 * s,
-                                    root_chord=40.0 * s, tip_chord=12.0 * s, tail=(30.0 * s, 20.0 * s),
-                                    period=1200.0 * s)
+    geometry = {} if s == 1.0 else dict(body_radii=(40.0 * s, 14.0 * s, 14.0 * s), half_span=100.0 * s,
+                                        root_chord=40.0 * s, tip_chord=12.0 * s, tail=(30.0 * s, 20.0 * s),
+                                        period=1200.0 * s)
+    center = (160.0 * s, 128.0 * s, 128.0 * s)
+    if form == "articulated":
+        return workloads.synthetic_bird_articulated(seed=7, center=center, **geometry)
+    return workloads.synthetic_bird(steps, seed=7, center=center, **geometry)
 
 
-def run_gpu(shape, steps, warmup, storage, arithmetic="default"):
+def upload_bird(sim, mesh):
+    if hasattr(mesh, "joints"):
+        sim.upload_articulated(mesh)
+    else:
+        sim.upload_mesh(mesh.points, mesh.times, mesh.triangles, mesh.parts, mesh.half_thickness)
+
+
+def run_gpu(shape, steps, warmup, storage, arithmetic="default", form="articulated", wall_rule=None):
     import birdflow_hip as bh
-    seq = bird_mesh(warmup + steps, shape)
+    import dataclasses
+    mesh = bird_mesh(warmup + steps + BREAKDOWN_STEPS, shape, form)
     cfg = bird_config(shape, storage, arithmetic)
+    if wall_rule is not None:
+        cfg = dataclasses.replace(cfg, wall_rule=wall_rule)
     sim = bh.BirdFlowSimulation(cfg)
     try:
-        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        upload_bird(sim, mesh)
         sim.init_equilibrium()
         sim.advance(warmup)
         sim.synchronize()
@@ -216,24 +230,36 @@ def omp_threads(n):
     ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
 
 
-def run_cpu_baseline(shape, steps, threads=None):
+def run_cpu_baseline(shape, steps, threads=None, form="articulated"):
     """The oracle (C, OpenMP over z) on a bounded sample of the same workload,
     on `threads` OpenMP threads (default: OMP_NUM_THREADS or all host cores)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import bfref
     all_threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     omp_threads(threads or all_threads)
-    seq = bird_mesh(steps, shape)
+    mesh = bird_mesh(steps, shape, form)
     cfg = bird_config(shape, "fp32")
     p = bfref.make_params(shape, omega_plus=cfg.omega_plus, omega_minus=cfg.omega_minus,
                           bc=((bfref.BC_FARFIELD, bfref.BC_OUTLET), (0, 0), (0, 0)), collision=bfref.COLLISION_RR3,
                           wall_rule=bfref.WALL_INTERPOLATED_SDF, force_mode=6, accumulate_loads=True,
                           far_u=(0.04, 0.0, 0.0), body_position=cfg.body_position)
 
+    if hasattr(mesh, "joints"):
+        k = bfref.make_kinematics(mesh.period, mesh.amplitude_scale, mesh.center,
+                                  [dict(side=j.side, root=j.root, stroke=j.stroke, deviation=j.deviation,
+                                        pitch=j.pitch, twist=j.twist) for j in mesh.joints],
+                                  harmonics=mesh.harmonics)
+
+        def vertices(t):
+            return bfref.articulated_vertices(mesh.rest, mesh.vertex_joint, mesh.span_fraction, k, t)
+    else:
+        def vertices(t):
+            first, _, _ = bfref.frame_interval(mesh.times, t)
+            return bfref.prepare_vertices(mesh.points, mesh.times, first, t)
+
     def geometry(t):
-        first, _, _ = bfref.frame_interval(seq.times, t)
-        pos, vel = bfref.prepare_vertices(seq.points, seq.times, first, t)
-        mask, wall, _ = bfref.raster_resolve(p, pos, vel, seq.triangles, seq.parts, seq.half_thickness, 64)
+        pos, vel = vertices(t)
+        mask, wall, _ = bfref.raster_resolve(p, pos, vel, mesh.triangles, mesh.parts, mesh.half_thickness, 64)
         return mask, wall
 
     mask0, wall0 = geometry(0.0)
@@ -330,6 +356,10 @@ def main():
     ap.add_argument("--arithmetic", choices=list(ARITHMETIC), default="default",
                     help="collision arithmetic; default: strict for fp32, fused for fp16/bf16")
     ap.add_argument("--shape", type=int, nargs=3, default=(512, 256, 256))
+    ap.add_argument("--mesh", choices=("articulated", "frames"), default="articulated",
+                    help="the bird's upload form: on-device kinematics, or one recorded frame per step")
+    ap.add_argument("--ray-steps", type=int, default=30,
+                    help="timed steps of the same bird with exact ray-root link fractions (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
     ap.add_argument("--cpu-single-steps", type=int, default=1,
                     help="oracle steps timed on one thread beside it (0: skip)")
@@ -367,16 +397,24 @@ def main():
     import birdflow_hip as bh
     copy_gbs = bh.copy_bandwidth(1 << 30, 20)
     print(f"# copy kernel: {copy_gbs:.1f} GB/s (1 GiB buffers)", file=sys.stderr, flush=True)
-    r = run_gpu(shape, args.steps, args.warmup, args.storage, args.arithmetic)
+    r = run_gpu(shape, args.steps, args.warmup, args.storage, args.arithmetic, args.mesh)
     print(f"# gpu: {r}", file=sys.stderr, flush=True)
     extra = {"copy_kernel_gbs": copy_gbs}
     if not r["finite"]:
         extra["error"] = "non-finite loads: the run diverged"
     if args.storage == "fp32" and not args.skip_fp16:
-        r16 = run_gpu(shape, args.steps, args.warmup, "fp16", args.arithmetic)
+        r16 = run_gpu(shape, args.steps, args.warmup, "fp16", args.arithmetic, args.mesh)
         print(f"# gpu fp16: {r16}", file=sys.stderr, flush=True)
         extra["fp16_store"] = {"value": r16["mlups"], "unit": "MLUPS", "ratio_vs_fp32": r16["mlups"] / r["mlups"],
                                "fast_kernel_ms": r16["fast_kernel_ms"], "achieved_gbs": r16["achieved_gbs"]}
+    if args.ray_steps > 0:
+        rr = run_gpu(shape, args.ray_steps, args.warmup, args.storage, args.arithmetic, args.mesh,
+                     wall_rule=bh.WALL_INTERPOLATED_RAY)
+        print(f"# gpu, exact ray roots: {rr}", file=sys.stderr, flush=True)
+        extra["exact_ray_roots"] = {"value": rr["mlups"], "unit": "MLUPS", "ms_per_step": rr["ms_per_step"],
+                                    "geometry_ms": rr["geometry_ms"], "fast_kernel_ms": rr["fast_kernel_ms"],
+                                    "steps": args.ray_steps,
+                                    "workload": "the same bird with BF_WALL_INTERPOLATED_RAY link fractions"}
     if args.sdf_bird_steps > 0:
         sdf = run_sdf_bird(shape, args.sdf_bird_steps, args.warmup)
         print(f"# analytic bird: {sdf}", file=sys.stderr, flush=True)
@@ -391,7 +429,7 @@ def main():
         extra["adjoint"] = adj
     cpu = None
     if args.cpu_steps > 0:
-        mlups, threads, elapsed = run_cpu_baseline(shape, args.cpu_steps)
+        mlups, threads, elapsed = run_cpu_baseline(shape, args.cpu_steps, form=args.mesh)
         print(f"# cpu: {mlups:.2f} MLUPS over {args.cpu_steps} steps ({elapsed:.1f} s, {threads} threads)",
               file=sys.stderr, flush=True)
         cpu = {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
@@ -399,7 +437,7 @@ def main():
                "sample": f"oracle (C, OpenMP over z) on the same bird workload, {args.cpu_steps} full steps "
                          f"(geometry + fluid) at {shape[0]}x{shape[1]}x{shape[2]}"}
         if args.cpu_single_steps > 0:
-            m1, _, e1 = run_cpu_baseline(shape, args.cpu_single_steps, threads=1)
+            m1, _, e1 = run_cpu_baseline(shape, args.cpu_single_steps, threads=1, form=args.mesh)
             print(f"# cpu, 1 thread: {m1:.2f} MLUPS over {args.cpu_single_steps} steps ({e1:.1f} s)",
                   file=sys.stderr, flush=True)
             cpu["single_thread"] = {"value": m1, "unit": "MLUPS", "cores": 1,
@@ -421,7 +459,9 @@ def main():
                                "articulated mesh, RR3, interpolated moving bounce-back, FP64 loads, "
                                "link rebuild every step",
                    "cells": shape[0] * shape[1] * shape[2], "storage": args.storage,
-                   "arithmetic": args.arithmetic},
+                   "arithmetic": args.arithmetic,
+                   "kinematics": "on-device Fourier (3 harmonics)" if args.mesh == "articulated"
+                   else "one uploaded frame per step"},
         "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r["achieved_gbs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.storage, args.arithmetic, r["fast_cells"]),

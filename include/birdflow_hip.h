@@ -207,6 +207,58 @@ bf_status bf_mesh_upload(bf_context* ctx, const float* points, const float* fram
  * MetalIndexedBirdSurfaceReplay.snapshot, :17916-17992). */
 bf_status bf_mesh_build(bf_context* ctx, float time);
 
+/* On-device kinematics (bf_set_kinematics of the boundary plan): an
+ * articulated mesh whose vertices ride on rigid joints driven by Fourier
+ * series, evaluated on the device each step instead of uploading one frame
+ * per step.  The reference drives its bird's wings the same way, by stroke,
+ * deviation, pitch and tip twist angles about a hinge (makeWingFrame,
+ * prepareBirdGeometry, BirdFlow.metal:611-814); its indexed meshes carry
+ * only recorded frames.  A vertex of joint 0 is rigid: position = center +
+ * rest.  A vertex of joint j >= 1 (side s = joints[j-1].side, +1 or -1):
+ *   l   = (rest.x, s rest.y, rest.z)
+ *   a   = pitch + twist * span_fraction  (radians)
+ *   u3  = Ry(a) l,  Ry(a) (x, y, z) = (cos a x + sin a z, y, -sin a x + cos a z)
+ *   u2  = Rz(s deviation) u3,  u1 = Rx(s stroke) u2  (right-handed rotations)
+ *   pos = center + (root.x, s root.y, root.z) + u1
+ * with each angle = amplitude_scale * sum_h (c[h][0] sin(h phi) + c[h][1]
+ * cos(h phi)), phi = 2 pi (t mod period) / period, h = 1..harmonics; the
+ * velocity is the time derivative of pos (rates from the same series) times
+ * velocity_to_lattice.  FP32 with this library's own sine and cosine.
+ * Triangles whose three vertices are rigid keep their raster keys from step
+ * to step.  Builds the geometry at time_start like bf_mesh_upload and selects
+ * BF_GEOMETRY_MESH (the adjoint, which differentiates frame tangents, refuses
+ * an articulated mesh). */
+enum { BF_MAX_JOINTS = 8, BF_MAX_HARMONICS = 4 };
+typedef struct bf_joint {
+    float side;                          /* +1 or -1 */
+    float root[3];                       /* hinge of the +y side (world units) */
+    float stroke[BF_MAX_HARMONICS][2];   /* [harmonic 1..4][sin, cos] radians */
+    float deviation[BF_MAX_HARMONICS][2];
+    float pitch[BF_MAX_HARMONICS][2];
+    float twist[BF_MAX_HARMONICS][2];
+} bf_joint;
+typedef struct bf_kinematics {
+    float period;                        /* time units, > 0 */
+    float amplitude_scale;               /* multiplies every angle */
+    uint32_t harmonics;                  /* 1..BF_MAX_HARMONICS */
+    uint32_t joint_count;                /* 0..BF_MAX_JOINTS */
+    float center[3];                     /* world position of the rest frame's origin */
+    float reserved;
+    bf_joint joints[BF_MAX_JOINTS];
+} bf_kinematics;
+/* rest: [vertex_count][3] (world units, relative to center); vertex_joint:
+ * [vertex_count] in 0..joint_count; span_fraction: [vertex_count] (the
+ * twist's weight); triangles and parts as in bf_mesh_upload. */
+bf_status bf_mesh_upload_articulated(bf_context* ctx, const float* rest, const uint8_t* vertex_joint,
+                                     const float* span_fraction, uint32_t vertex_count, const uint32_t* triangles,
+                                     const uint8_t* triangle_parts, uint32_t triangle_count, float half_thickness,
+                                     const bf_kinematics* kinematics, float time_start);
+/* Positions and lattice velocities [vertex_count][3] of the uploaded mesh
+ * (either form) prepared at `time` exactly as a step at that time prepares
+ * them (the vertex-prepare pass, BirdFlow.metal:1589-1622). */
+bf_status bf_mesh_vertices_at(bf_context* ctx, float time, float* position3, float* velocity3,
+                              uint32_t vertex_count);
+
 /* Translating-sphere canonical (BirdFlow.metal:2263-2350): center(t) =
  * center_radius.xyz + geometry_velocity * t, wall_velocity.w selects uniform
  * (0), tangential (1) or normal (2) wall velocity.  Builds the geometry at

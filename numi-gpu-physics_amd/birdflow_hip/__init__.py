@@ -8,11 +8,11 @@ from ._abi import (ARITHMETIC_DEFAULT, ARITHMETIC_FUSED, ARITHMETIC_STRICT, BC_F
                    STORAGE_FP32, WALL_HALFWAY, WALL_INTERPOLATED_HALF, WALL_INTERPOLATED_LINKS, WALL_INTERPOLATED_RAY,
                    WALL_INTERPOLATED_SDF,
                    LibraryMissing)
-from .simulation import (BirdFlowError, BirdFlowSimulation, BirdGeometry, BodyParameters, FlappingWing, LatticeScaling, SolverConfig, copy_bandwidth,
-                         device_info)
+from .simulation import (ArticulatedMesh, BirdFlowError, BirdFlowSimulation, BirdGeometry, BodyParameters, FlappingWing,
+                         Joint, LatticeScaling, SolverConfig, copy_bandwidth, device_info)
 
 __all__ = [
-    "BirdFlowError", "BirdFlowSimulation", "BirdGeometry", "FlappingWing", "BodyParameters", "LatticeScaling", "SolverConfig", "device_info", "copy_bandwidth", "LibraryMissing",
+    "ArticulatedMesh", "Joint", "BirdFlowError", "BirdFlowSimulation", "BirdGeometry", "FlappingWing", "BodyParameters", "LatticeScaling", "SolverConfig", "device_info", "copy_bandwidth", "LibraryMissing",
     "LIBRARY_PATH", "BC_PERIODIC", "BC_FARFIELD", "BC_OUTLET", "COLLISION_TRT", "COLLISION_TRT_SYMMETRIC_LIMITER",
     "COLLISION_REGULARIZED", "COLLISION_RR3", "WALL_HALFWAY", "WALL_INTERPOLATED_SDF", "WALL_INTERPOLATED_HALF", "WALL_INTERPOLATED_LINKS",
     "WALL_INTERPOLATED_RAY",

@@ -43,7 +43,8 @@ EXPORTED_FUNCTIONS = (
     "bf_read_populations", "bf_write_population_storage", "bf_read_population_storage", "bf_read_fields",
     "bf_step_index", "bf_read_restart_state", "bf_set_restart_state",
     "bf_set_geometry", "bf_read_geometry",
-    "bf_mesh_upload", "bf_mesh_build", "bf_set_translating_sphere", "bf_step", "bf_synchronize",
+    "bf_mesh_upload", "bf_mesh_build", "bf_mesh_upload_articulated", "bf_mesh_vertices_at",
+    "bf_set_translating_sphere", "bf_step", "bf_synchronize",
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
     "bf_population_minimum", "bf_inject_population",
@@ -53,6 +54,19 @@ EXPORTED_FUNCTIONS = (
     "bf_ledger_step_sources_noalloc", "bf_set_flapping_wing", "bf_read_wing_frame", "bf_read_wing_links",
     "bf_read_ray_links",
 )
+
+
+class Joint(ctypes.Structure):
+    """bf_joint: side, hinge, Fourier series [harmonic][sin, cos] per angle."""
+    _fields_ = [("side", ctypes.c_float), ("root", ctypes.c_float * 3)] + [
+        (name, (ctypes.c_float * 2) * 4) for name in ("stroke", "deviation", "pitch", "twist")]
+
+
+class Kinematics(ctypes.Structure):
+    """bf_kinematics: period, amplitude scale, harmonics, centre, joints."""
+    _fields_ = [("period", ctypes.c_float), ("amplitude_scale", ctypes.c_float), ("harmonics", ctypes.c_uint32),
+                ("joint_count", ctypes.c_uint32), ("center", ctypes.c_float * 3), ("reserved", ctypes.c_float),
+                ("joints", Joint * 8)]
 
 
 class Config(ctypes.Structure):
@@ -184,6 +198,8 @@ def _declare(l):
         "bf_read_geometry": (st, [vp, vp, vp]),
         "bf_mesh_upload": (st, [vp, vp, vp, u32, u32, vp, vp, u32, f32, vp, f32]),
         "bf_mesh_build": (st, [vp, f32]),
+        "bf_mesh_upload_articulated": (st, [vp, vp, vp, vp, u32, vp, vp, u32, f32, vp, f32]),
+        "bf_mesh_vertices_at": (st, [vp, f32, vp, vp, u32]),
         "bf_set_translating_sphere": (st, [vp, vp, vp, vp]),
         "bf_step": (st, [vp, u32]),
         "bf_synchronize": (st, [vp]),

@@ -52,7 +52,7 @@ from typing import Dict, Optional
 import numpy as np
 
 from . import _abi
-from .simulation import BirdFlowSimulation, BirdGeometry, BodyParameters, FlappingWing, SolverConfig
+from .simulation import ArticulatedMesh, BirdFlowSimulation, BirdGeometry, BodyParameters, FlappingWing, SolverConfig
 
 FORMAT_NAME = "birdflow-hip"
 SCHEMA_VERSION = 1
@@ -65,6 +65,8 @@ _STORAGE_ELEMENTS = {_abi.STORAGE_FP32: "float32", _abi.STORAGE_FP16_OFFSET: "fl
 _GEOMETRY_ARRAYS = {
     "mesh": (("points", np.float32), ("frame_times", np.float32), ("triangles", np.uint32),
              ("triangle_parts", np.uint8), ("translation", np.float32)),
+    "articulated": (("rest", np.float32), ("vertex_joint", np.uint8), ("span_fraction", np.float32),
+                    ("triangles", np.uint32), ("triangle_parts", np.uint8)),
     "static": (("wall", np.float32), ("mask_prev", np.uint8)),
     "sphere": (),
     "bird": (("keyframes", np.float32),),
@@ -196,7 +198,7 @@ def save_checkpoint(sim: BirdFlowSimulation, destination: str, compression: str 
     origin, the configuration and the geometry source.  Returns the
     manifest."""
     if sim.geometry_source is None:
-        raise CheckpointError("a checkpoint needs a geometry source (mesh, sphere, bird, wing or static geometry)")
+        raise CheckpointError("a checkpoint needs a geometry source (mesh, articulated mesh, sphere, bird, wing or static geometry)")
     cfg = sim.config
     step, origin = sim.restart_state()
     populations = sim.read_population_storage()
@@ -211,6 +213,12 @@ def save_checkpoint(sim: BirdFlowSimulation, destination: str, compression: str 
     if kind == "mesh":
         geometry["half_thickness"] = float(src["half_thickness"])
         for name, _ in _GEOMETRY_ARRAYS["mesh"]:
+            arrays[name] = src[name]
+        geometry["shapes"] = {k: list(v.shape) for k, v in arrays.items()}
+    elif kind == "articulated":
+        # rest pose, joints and the kinematics: every step's vertices follow
+        geometry["kinematics"] = src["kinematics"]
+        for name, _ in _GEOMETRY_ARRAYS["articulated"]:
             arrays[name] = src[name]
         geometry["shapes"] = {k: list(v.shape) for k, v in arrays.items()}
     elif kind == "sphere":
@@ -309,6 +317,8 @@ def load_checkpoint(source: str) -> BirdFlowSimulation:
             a = ck.geometry
             sim.upload_mesh(a["points"], a["frame_times"], a["triangles"], a["triangle_parts"], g["half_thickness"],
                             translation=a["translation"])
+        elif g["kind"] == "articulated":
+            sim.upload_articulated(ArticulatedMesh.from_arrays(ck.geometry, g["kinematics"]))
         elif g["kind"] == "sphere":
             sim.set_translating_sphere(g["center_radius"], g["geometry_velocity"], g["wall_velocity"])
         elif g["kind"] == "wing":

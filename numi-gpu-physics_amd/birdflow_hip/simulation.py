@@ -259,6 +259,80 @@ class BirdGeometry:
         return b
 
 
+@dataclass
+class Joint:
+    """One articulated joint: side (+1 / -1 mirrors y), hinge of the +y side
+    and the Fourier series [harmonic][sin, cos] (radians) of its stroke,
+    deviation, pitch and tip twist."""
+
+    side: float
+    root: Sequence[float]
+    stroke: np.ndarray
+    deviation: np.ndarray
+    pitch: np.ndarray
+    twist: np.ndarray
+
+    ANGLES = ("stroke", "deviation", "pitch", "twist")
+
+
+@dataclass
+class ArticulatedMesh:
+    """An articulated mesh for bf_mesh_upload_articulated: rest positions
+    relative to `center`, each vertex's joint (0 rigid, k = joints[k - 1]),
+    the twist weight of each vertex, the triangles and their part ids, and
+    the Fourier kinematics of the joints (period in time units, one amplitude
+    scale for every angle)."""
+
+    rest: np.ndarray
+    vertex_joint: np.ndarray
+    span_fraction: np.ndarray
+    triangles: np.ndarray
+    parts: np.ndarray
+    half_thickness: float
+    period: float
+    amplitude_scale: float
+    center: Sequence[float]
+    joints: list
+    harmonics: int = 3
+
+    def to_c(self) -> _abi.Kinematics:
+        k = _abi.Kinematics()
+        k.period, k.amplitude_scale = float(self.period), float(self.amplitude_scale)
+        k.harmonics, k.joint_count = int(self.harmonics), len(self.joints)
+        k.center[:] = [float(v) for v in self.center]
+        for i, j in enumerate(self.joints):
+            J = k.joints[i]
+            J.side = float(j.side)
+            J.root[:] = [float(v) for v in j.root]
+            for name in Joint.ANGLES:
+                c = np.asarray(getattr(j, name), np.float32).reshape(-1, 2)
+                arr = getattr(J, name)
+                for h in range(c.shape[0]):
+                    arr[h][0], arr[h][1] = float(c[h, 0]), float(c[h, 1])
+        return k
+
+    def kinematics_dict(self) -> dict:
+        """The kinematics as plain values (checkpoint manifests), each rounded
+        to the FP32 value the library uses, so that a restored mesh writes
+        the same manifest again."""
+        f32 = lambda v: float(np.float32(v))
+        return {"period": f32(self.period), "amplitude_scale": f32(self.amplitude_scale),
+                "harmonics": int(self.harmonics), "center": [f32(v) for v in self.center],
+                "half_thickness": f32(self.half_thickness),
+                "joints": [{"side": f32(j.side), "root": [f32(v) for v in j.root],
+                            **{n: np.asarray(getattr(j, n), np.float32).astype(np.float64).tolist()
+                               for n in Joint.ANGLES}}
+                           for j in self.joints]}
+
+    @classmethod
+    def from_arrays(cls, arrays: dict, kin: dict) -> "ArticulatedMesh":
+        joints = [Joint(j["side"], j["root"], *(np.asarray(j[n], np.float32) for n in Joint.ANGLES))
+                  for j in kin["joints"]]
+        return cls(arrays["rest"], arrays["vertex_joint"], arrays["span_fraction"], arrays["triangles"],
+                   arrays["triangle_parts"], kin["half_thickness"], kin["period"], kin["amplitude_scale"],
+                   kin["center"], joints, kin["harmonics"])
+
+
 def _body_to_c(state) -> _abi.BodyState:
     """13-vector (position, orientation x y z w, linear velocity, body angular velocity)."""
     v = np.asarray(state, np.float64).ravel()
@@ -419,6 +493,31 @@ class BirdFlowSimulation:
 
     def build_mesh_geometry(self, time: float):
         self._check(self._lib.bf_mesh_build(self._ctx, float(time)))
+
+    def upload_articulated(self, mesh: ArticulatedMesh, time_start: float = 0.0):
+        """Articulated mesh with on-device Fourier kinematics
+        (bf_mesh_upload_articulated): no frames; each step prepares the
+        vertices at its own time."""
+        rest = np.ascontiguousarray(mesh.rest, np.float32).reshape(-1, 3)
+        joint = np.ascontiguousarray(mesh.vertex_joint, np.uint8)
+        span = np.ascontiguousarray(mesh.span_fraction, np.float32)
+        tri = np.ascontiguousarray(mesh.triangles, np.uint32).reshape(-1, 3)
+        parts = np.ascontiguousarray(mesh.parts, np.uint8)
+        k = mesh.to_c()
+        self._check(self._lib.bf_mesh_upload_articulated(self._ctx, _ptr(rest), _ptr(joint), _ptr(span), rest.shape[0],
+                                                         _ptr(tri), _ptr(parts), tri.shape[0],
+                                                         float(mesh.half_thickness), ctypes.byref(k),
+                                                         float(time_start)))
+        self.geometry_source = {"kind": "articulated", "rest": rest.copy(), "vertex_joint": joint.copy(),
+                                "span_fraction": span.copy(), "triangles": tri.copy(),
+                                "triangle_parts": parts.copy(), "kinematics": mesh.kinematics_dict()}
+
+    def mesh_vertices_at(self, time: float, vertex_count: int):
+        """Prepared vertex positions and lattice velocities [V][3] at `time`."""
+        pos = np.zeros((vertex_count, 3), np.float32)
+        vel = np.zeros((vertex_count, 3), np.float32)
+        self._check(self._lib.bf_mesh_vertices_at(self._ctx, float(time), _ptr(pos), _ptr(vel), vertex_count))
+        return pos, vel
 
     def set_translating_sphere(self, center_radius, geometry_velocity, wall_velocity):
         cr = np.ascontiguousarray(center_radius, np.float32)

@@ -231,6 +231,22 @@ class SyntheticBird:
         parts = [self.body, self.wing(0, 1.0, t), self.wing(1, -1.0, t), self.tail_local]
         return np.concatenate(parts) + self.center
 
+    def articulated(self, half_thickness: float = 1.0):
+        """The same bird as an articulated mesh (on-device kinematics): the
+        body and tail rigid (joint 0), the wings on joints 1 (+y) and 2 (-y)
+        with this bird's Fourier series; positions agree with frame(t) to
+        FP32 rounding."""
+        from .simulation import ArticulatedMesh, Joint
+        nb, nw, nt = len(self.body), len(self.wing_local), len(self.tail_local)
+        rest = np.concatenate([self.body, self.wing_local, self.wing_local, self.tail_local]).astype(np.float32)
+        joint = np.concatenate([np.zeros(nb), np.ones(nw), np.full(nw, 2), np.zeros(nt)]).astype(np.uint8)
+        span = np.concatenate([np.zeros(nb), self.span_fraction, self.span_fraction, np.zeros(nt)]).astype(np.float32)
+        k = self.kin
+        joints = [Joint(side, self.root.tolist(), k.stroke[w], k.deviation[w], k.pitch[w], k.twist[w])
+                  for w, side in ((0, 1.0), (1, -1.0))]
+        return ArticulatedMesh(rest, joint, span, self.triangles, self.parts, float(half_thickness), float(k.period),
+                               float(self.amplitude_scale), self.center.tolist(), joints, harmonics=3)
+
     def max_wing_speed(self, samples=240):
         """Largest wing-vertex speed over one period (central differences)."""
         period = self.kin.period
@@ -269,3 +285,13 @@ def synthetic_bird(steps: int, seed: int = 7, center=(160.0, 128.0, 128.0), infl
     return MeshSequence(pts, times.astype(np.float32), bird.triangles, bird.parts, float(half_thickness),
                         meta={"amplitude_scale": scale, "max_wing_speed": bird.max_wing_speed(),
                               "triangles": int(len(bird.triangles)), "vertices": int(pts.shape[1])})
+
+
+def synthetic_bird_articulated(seed: int = 7, center=(160.0, 128.0, 128.0), inflow: float = 0.04, mach: float = 0.15,
+                               half_thickness: float = 1.0, **geometry):
+    """The synthetic bird of synthetic_bird() as an articulated mesh: the
+    same rest geometry, Fourier series and Mach-limited amplitude scale, with
+    the kinematics evaluated on the device at every step's time."""
+    bird = SyntheticBird(seed=seed, center=center, **geometry)
+    bird.limit_mach(abs(inflow), mach)
+    return bird.articulated(half_thickness)

@@ -36,6 +36,10 @@ using namespace bf;
 
 static_assert(BF_STORAGE_FP32 == STORE_FP32 && BF_STORAGE_FP16_OFFSET == STORE_FP16_OFFSET &&
                   BF_STORAGE_BF16_OFFSET == STORE_BF16_OFFSET, "storage ids");
+static_assert(sizeof(bf_kinematics) == sizeof(Kinematics) && sizeof(bf_joint) == sizeof(KinJoint) &&
+                  offsetof(bf_kinematics, joints) == offsetof(Kinematics, joints) && BF_MAX_JOINTS == 8 &&
+                  BF_MAX_HARMONICS == 4,
+              "kinematics layouts differ");
 static_assert(BF_COLLISION_RR3 == COLL_RR3 && BF_WALL_INTERPOLATED_HALF == WALL_INTERPOLATED_HALF, "operator ids");
 
 namespace {
@@ -114,6 +118,13 @@ struct bf_context {
     // mesh
     float* m_points = nullptr;
     float* m_times = nullptr;
+    // articulated mesh (instead of frames): rest positions, joints, twist
+    // weights and the kinematics on the device
+    bool m_articulated = false;
+    float* m_rest = nullptr;
+    uint8_t* m_joint = nullptr;
+    float* m_span = nullptr;
+    Kinematics* m_kin = nullptr;
     uint32_t* m_tri = nullptr;
     uint8_t* m_part = nullptr;
     float4* m_pos = nullptr;
@@ -158,6 +169,7 @@ struct bf_context {
     float* ray_links[2] = {nullptr, nullptr};    // [step parity][tiles][19][256]
     uint32_t* ray_list = nullptr;
     uint32_t* ray_count = nullptr;
+    uint32_t* ray_dirs = nullptr;                // per cell: link directions pulling from it
     // analytic bird: shape and the prepared frame of each step parity
     BirdShape bird{};
     float4* bird_frame = nullptr;         // [2][BIRD_FRAME_ROWS]
@@ -255,6 +267,8 @@ struct bf_context {
     uint32_t raster_batch_cap = 32;                // largest automatic raster batch
     uint32_t compute_units = 1;
     uint32_t tile_pass_blocks = 512;
+    uint32_t ray_blocks_per_cu = 64;        // triangle pass: single-wavefront blocks per CU
+    uint32_t ray_tile_blocks_per_cu = 8;    // exact-root init / finalize: 256-lane blocks per CU
 };
 
 namespace {
@@ -393,6 +407,7 @@ MeshParams mesh_params(const bf_context* c, float t, bool exact) {
     m.raster_blocks = c->raster_blocks;
     set_raster_shape(c, m, c->m_triangles);
     m.tile_pass_blocks = c->tile_pass_blocks;
+    if (c->m_articulated) return m;   // no frames: the kinematics give every time
     // frame interval, MeasuredBirdSurfaceSequence.swift:174-204
     const std::vector<float>& ts = c->host_times;
     const int last = (int)c->m_frames - 1;
@@ -414,6 +429,23 @@ MeshParams mesh_params(const bf_context* c, float t, bool exact) {
 
 float time_of(const bf_context* c, int64_t s) {
     return c->time_start + (float)((double)s * c->cfg.time_step);
+}
+
+// The mesh arrays of either upload form.
+MeshBuffers mesh_buffers(const bf_context* c) {
+    MeshBuffers b{};
+    b.body = c->body;
+    b.points = c->m_points;
+    b.times = c->m_times;
+    b.rest = c->m_rest;
+    b.vertex_joint = c->m_joint;
+    b.span_fraction = c->m_span;
+    b.kinematics = c->m_articulated ? c->m_kin : nullptr;
+    b.tri = c->m_tri;
+    b.tri_part = c->m_part;
+    b.pos = c->m_pos;
+    b.vel = c->m_vel;
+    return b;
 }
 
 // Enqueue on stream B the geometry of step s (dynamic sources).  fresh: the
@@ -443,12 +475,14 @@ RayParams ray_params(const bf_context* c) {
     r.dx = c->cfg.cell_size;
     r.half_thickness = c->m_half;
     r.triangle_count = c->m_triangles;
-    // four triangles per block; a mesh of fewer triangles than two
-    // wavefronts per CU splits each triangle's cells among several
+    // one triangle per single-wavefront block (the pass waits on loads of
+    // the link directions: many wavefronts per CU hide that); a mesh of fewer
+    // triangles than two wavefronts per CU splits each triangle's cells among
+    // several
     const uint32_t waves = 2u * c->compute_units;
-    r.triangle_blocks = std::max(1u, std::min((c->m_triangles + 3u) / 4u, 4u * c->compute_units));
+    r.triangle_blocks = std::max(1u, std::min(c->m_triangles, c->ray_blocks_per_cu * c->compute_units));
     r.split = std::min(64u, std::max(1u, waves / std::max(1u, c->m_triangles)));
-    r.tile_pass_blocks = c->tile_pass_blocks;
+    r.tile_pass_blocks = c->ray_tile_blocks_per_cu * c->compute_units;
     return r;
 }
 
@@ -476,14 +510,7 @@ bf_status ensure_static_keys(bf_context* c) {
     }
     MeshParams m = mesh_params(c, time_of(c, c->step), false);
     set_raster_shape(c, m, c->m_static_count);
-    MeshBuffers b{};
-    b.body = c->body;
-    b.points = c->m_points;
-    b.times = c->m_times;
-    b.tri = c->m_tri;
-    b.tri_part = c->m_part;
-    b.pos = c->m_pos;
-    b.vel = c->m_vel;
+    MeshBuffers b = mesh_buffers(c);
     b.keys = c->keys;
     b.tiles_new = tiles;
     b.raster_list = c->m_static_list;
@@ -510,14 +537,7 @@ bf_status ensure_static_keys(bf_context* c) {
 bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
     hipError_t e = hipSuccess;
     if (c->source == BF_GEOMETRY_MESH) {
-        MeshBuffers b{};
-        b.body = c->body;
-        b.points = c->m_points;
-        b.times = c->m_times;
-        b.tri = c->m_tri;
-        b.tri_part = c->m_part;
-        b.pos = c->m_pos;
-        b.vel = c->m_vel;
+        MeshBuffers b = mesh_buffers(c);
         b.keys = c->keys;
         b.tiles_new = c->tflags[mod4(s)];
         b.tiles_mask_old = fresh ? c->no_tiles : c->tflags[mod4(s - 3)];   // mask[s%3] last written at s-3
@@ -548,6 +568,7 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
             rb.dyn = b.dyn;
             rb.list = c->ray_list;
             rb.list_count = c->ray_count;
+            rb.dirs = c->ray_dirs;
             rb.links = c->ray_links[par2(s)];
             e = launch_ray_links(rb, ray_params(c), c->sb);
         }
@@ -822,6 +843,10 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     if (const char* v = std::getenv("BIRDFLOW_GEOMETRY_AFTER_GENERAL")) c->geometry_after_general = std::atoi(v) != 0;
     if (const char* v = std::getenv("BIRDFLOW_RASTER_BATCH"))
         c->raster_batch = (uint32_t)std::min(64, std::max(1, std::atoi(v)));
+    if (const char* v = std::getenv("BIRDFLOW_RAY_BLOCKS_PER_CU"))
+        c->ray_blocks_per_cu = (uint32_t)std::min(256, std::max(1, std::atoi(v)));
+    if (const char* v = std::getenv("BIRDFLOW_RAY_TILE_BLOCKS_PER_CU"))
+        c->ray_tile_blocks_per_cu = (uint32_t)std::min(64, std::max(1, std::atoi(v)));
     if (const char* v = std::getenv("BIRDFLOW_TILE_PASS_BLOCKS"))
         c->tile_pass_blocks = (uint32_t)std::max(1, std::atoi(v));
 
@@ -1194,63 +1219,65 @@ static bf_status rebuild_mesh_geometry(bf_context* c) {
     return BF_OK;
 }
 
-bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_times, uint32_t frame_count,
-                         uint32_t vertex_count, const uint32_t* triangles, const uint8_t* triangle_parts,
-                         uint32_t triangle_count, float half_thickness, const float translation[3], float time_start) {
-    BF_ENTER(c);
-    if (!points || !frame_times || !triangles || !triangle_parts)
-        return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh arrays must not be NULL");
-    if (frame_count < 2 || vertex_count == 0 || triangle_count == 0)
-        return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh needs >= 2 frames, >= 1 vertex and >= 1 triangle");
-    for (uint32_t i = 0; i + 1 < frame_count; ++i)
-        if (!(frame_times[i] < frame_times[i + 1]) || !std::isfinite(frame_times[i]) ||
-            !std::isfinite(frame_times[i + 1]))
-            return fail(c, BF_ERROR_INVALID_ARGUMENT, "frame times must be finite and strictly increasing");
-    for (uint64_t i = 0; i < 3ull * frame_count * vertex_count; ++i)
-        if (!std::isfinite(points[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "vertex positions must be finite");
+// Host arrays of one mesh upload: recorded frames, or an articulated mesh.
+struct MeshSource {
+    const float* points = nullptr;        // [frames][V][3]
+    const float* frame_times = nullptr;
+    uint32_t frame_count = 0;
+    const float* rest = nullptr;          // [V][3]
+    const uint8_t* vertex_joint = nullptr;
+    const float* span_fraction = nullptr;
+    const bf_kinematics* kinematics = nullptr;
+};
+
+// Triangle checks shared by both upload forms; `pose` gives vertex
+// positions [V][3] of one pose.
+static bf_status check_triangles(bf_context* c, const float* pose, uint32_t vertex_count, const uint32_t* triangles,
+                                 const uint8_t* triangle_parts, uint32_t triangle_count, const char* which) {
     for (uint32_t t = 0; t < triangle_count; ++t) {
         const uint32_t* v = triangles + 3ull * t;
         if (v[0] >= vertex_count || v[1] >= vertex_count || v[2] >= vertex_count)
             return fail(c, BF_ERROR_INVALID_ARGUMENT, "triangle index out of range");
         if (triangle_parts[t] == 0) return fail(c, BF_ERROR_INVALID_ARGUMENT, "triangle part ids must be 1..255");
         // A zero-area triangle makes the closest-point branches divide 0 by 0.
-        const float* a = points + 3ull * v[0];
-        const float* b = points + 3ull * v[1];
-        const float* d = points + 3ull * v[2];
+        const float* a = pose + 3ull * v[0];
+        const float* b = pose + 3ull * v[1];
+        const float* d = pose + 3ull * v[2];
         const double ux = (double)b[0] - a[0], uy = (double)b[1] - a[1], uz = (double)b[2] - a[2];
         const double wx = (double)d[0] - a[0], wy = (double)d[1] - a[1], wz = (double)d[2] - a[2];
         const double nx = uy * wz - uz * wy, ny = uz * wx - ux * wz, nz = ux * wy - uy * wx;
         if (!(nx * nx + ny * ny + nz * nz > 0.0))
-            return fail(c, BF_ERROR_INVALID_ARGUMENT, "degenerate (zero-area) triangle in the first frame");
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, std::string("degenerate (zero-area) triangle in ") + which);
     }
-    if (!(half_thickness > 0.0f)) return fail(c, BF_ERROR_INVALID_ARGUMENT, "half thickness must be positive");
+    return BF_OK;
+}
+
+// Installs a validated mesh: buffers, static / moving triangle lists (a
+// triangle is static when its three vertices are), the exact-root tables
+// when that wall rule is configured, and the geometry at time_start.
+static bf_status install_mesh(bf_context* c, const MeshSource& src, uint32_t vertex_count, const uint32_t* triangles,
+                              const uint8_t* triangle_parts, uint32_t triangle_count, float half_thickness,
+                              const float translation[3], float time_start,
+                              const std::vector<uint8_t>& vertex_static) {
     if (c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY && !c->ray_links[0]) {
         // the exact-root tables of both step parities (every tile's 19 x 256
-        // entries; the passes write the tiles next to solid cells) and the
-        // tile list
+        // entries; the passes write the tiles next to solid cells), the tile
+        // list and the per-cell link directions (zero between steps)
         hipError_t e = hipSuccess;
         const size_t entries = (size_t)c->g.tiles * 19 * TILE_CELLS;
         for (int k = 0; k < 2; ++k) c->ray_links[k] = (float*)c->ray_alloc.take(e, entries * sizeof(float));
         c->ray_list = (uint32_t*)c->ray_alloc.take(e, 4ull * c->g.tiles);
         c->ray_count = (uint32_t*)c->ray_alloc.take(e, 256);
+        c->ray_dirs = (uint32_t*)c->ray_alloc.take(e, 4ull * c->g.n);
+        if (e == hipSuccess) e = launch_fill_u32(c->ray_dirs, 0u, c->g.n, c->sb);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
         if (e != hipSuccess) {
             c->ray_alloc.release();
             c->ray_links[0] = c->ray_links[1] = nullptr;
-            c->ray_list = c->ray_count = nullptr;
+            c->ray_list = c->ray_count = c->ray_dirs = nullptr;
             return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("exact-root link tables: ") + hipGetErrorString(e));
         }
     }
-
-    // Static triangles: every vertex at the same position in every frame
-    // (their keys are the same every step unless a moving body carries the
-    // mesh); ids ascending, moving ones first in their own list.
-    std::vector<uint8_t> vertex_static(vertex_count, 1);
-    for (uint32_t f = 1; f < frame_count; ++f)
-        for (uint32_t v = 0; v < vertex_count; ++v) {
-            const float* p0 = points + 3ull * v;
-            const float* pf = points + 3ull * ((uint64_t)f * vertex_count + v);
-            if (std::memcmp(p0, pf, 12) != 0) vertex_static[v] = 0;
-        }
     std::vector<uint32_t> dyn_list, static_list;
     for (uint32_t t = 0; t < triangle_count; ++t) {
         const uint32_t* v = triangles + 3ull * t;
@@ -1267,11 +1294,22 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     c->adj.tangent_alloc.release();   // tangents belong to the previous mesh
     c->adj.K = 0;
     c->adj.dpoints = nullptr;
+    c->m_points = c->m_times = c->m_rest = c->m_span = nullptr;
+    c->m_joint = nullptr;
+    c->m_kin = nullptr;
     DeviceAlloc& A = c->mesh_alloc;
     hipError_t e = hipSuccess;
-    const uint64_t pts = 3ull * frame_count * vertex_count;
-    c->m_points = (float*)A.take(e, 4 * pts);
-    c->m_times = (float*)A.take(e, 4ull * frame_count);
+    const bool articulated = src.kinematics != nullptr;
+    const uint64_t pts = 3ull * src.frame_count * vertex_count;
+    if (articulated) {
+        c->m_rest = (float*)A.take(e, 12ull * vertex_count);
+        c->m_joint = (uint8_t*)A.take(e, vertex_count);
+        c->m_span = (float*)A.take(e, 4ull * vertex_count);
+        c->m_kin = (Kinematics*)A.take(e, sizeof(Kinematics));
+    } else {
+        c->m_points = (float*)A.take(e, 4 * pts);
+        c->m_times = (float*)A.take(e, 4ull * src.frame_count);
+    }
     c->m_tri = (uint32_t*)A.take(e, 12ull * triangle_count);
     c->m_part = (uint8_t*)A.take(e, triangle_count);
     c->m_pos = (float4*)A.take(e, 16ull * vertex_count);
@@ -1285,8 +1323,16 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
         return fail(c, BF_ERROR_ALLOCATION_FAILED, std::string("mesh buffers: ") + hipGetErrorString(e));
     }
     hipStream_t s = c->sb;
-    e = hipMemcpyAsync(c->m_points, points, 4 * pts, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->m_times, frame_times, 4ull * frame_count, hipMemcpyHostToDevice, s);
+    if (articulated) {
+        e = hipMemcpyAsync(c->m_rest, src.rest, 12ull * vertex_count, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->m_joint, src.vertex_joint, vertex_count, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->m_span, src.span_fraction, 4ull * vertex_count, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->m_kin, src.kinematics, sizeof(Kinematics), hipMemcpyHostToDevice, s);
+    } else {
+        e = hipMemcpyAsync(c->m_points, src.points, 4 * pts, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->m_times, src.frame_times, 4ull * src.frame_count, hipMemcpyHostToDevice, s);
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(c->m_tri, triangles, 12ull * triangle_count, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(c->m_part, triangle_parts, triangle_count, hipMemcpyHostToDevice, s);
     if (e == hipSuccess && !dyn_list.empty())
@@ -1298,8 +1344,10 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     if (e != hipSuccess) return hip_fail(c, e, "mesh upload");
     c->m_dyn_count = (uint32_t)dyn_list.size();
     c->m_static_count = (uint32_t)static_list.size();
-    c->host_times.assign(frame_times, frame_times + frame_count);
-    c->m_frames = frame_count;
+    if (articulated) c->host_times.clear();
+    else c->host_times.assign(src.frame_times, src.frame_times + src.frame_count);
+    c->m_articulated = articulated;
+    c->m_frames = src.frame_count;
     c->m_vertices = vertex_count;
     c->m_triangles = triangle_count;
     c->m_half = half_thickness;
@@ -1308,6 +1356,122 @@ bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_
     c->static_prev_differs = false;
     c->time_start = time_start - (float)((double)c->step * c->cfg.time_step);
     return rebuild_mesh_geometry(c);
+}
+
+bf_status bf_mesh_upload(bf_context* c, const float* points, const float* frame_times, uint32_t frame_count,
+                         uint32_t vertex_count, const uint32_t* triangles, const uint8_t* triangle_parts,
+                         uint32_t triangle_count, float half_thickness, const float translation[3], float time_start) {
+    BF_ENTER(c);
+    if (!points || !frame_times || !triangles || !triangle_parts)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh arrays must not be NULL");
+    if (frame_count < 2 || vertex_count == 0 || triangle_count == 0)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh needs >= 2 frames, >= 1 vertex and >= 1 triangle");
+    for (uint32_t i = 0; i + 1 < frame_count; ++i)
+        if (!(frame_times[i] < frame_times[i + 1]) || !std::isfinite(frame_times[i]) ||
+            !std::isfinite(frame_times[i + 1]))
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "frame times must be finite and strictly increasing");
+    for (uint64_t i = 0; i < 3ull * frame_count * vertex_count; ++i)
+        if (!std::isfinite(points[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "vertex positions must be finite");
+    if (bf_status r = check_triangles(c, points, vertex_count, triangles, triangle_parts, triangle_count,
+                                      "the first frame"))
+        return r;
+    if (!(half_thickness > 0.0f)) return fail(c, BF_ERROR_INVALID_ARGUMENT, "half thickness must be positive");
+    // Static triangles: every vertex at the same position in every frame
+    // (their keys are the same every step unless a moving body carries the
+    // mesh).
+    std::vector<uint8_t> vertex_static(vertex_count, 1);
+    for (uint32_t f = 1; f < frame_count; ++f)
+        for (uint32_t v = 0; v < vertex_count; ++v) {
+            const float* p0 = points + 3ull * v;
+            const float* pf = points + 3ull * ((uint64_t)f * vertex_count + v);
+            if (std::memcmp(p0, pf, 12) != 0) vertex_static[v] = 0;
+        }
+    MeshSource src;
+    src.points = points;
+    src.frame_times = frame_times;
+    src.frame_count = frame_count;
+    return install_mesh(c, src, vertex_count, triangles, triangle_parts, triangle_count, half_thickness, translation,
+                        time_start, vertex_static);
+}
+
+bf_status bf_mesh_upload_articulated(bf_context* c, const float* rest, const uint8_t* vertex_joint,
+                                     const float* span_fraction, uint32_t vertex_count, const uint32_t* triangles,
+                                     const uint8_t* triangle_parts, uint32_t triangle_count, float half_thickness,
+                                     const bf_kinematics* kinematics, float time_start) {
+    BF_ENTER(c);
+    if (!rest || !vertex_joint || !span_fraction || !triangles || !triangle_parts || !kinematics)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh arrays and kinematics must not be NULL");
+    if (vertex_count == 0 || triangle_count == 0)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "mesh needs >= 1 vertex and >= 1 triangle");
+    const bf_kinematics& k = *kinematics;
+    if (!(k.period > 0.0f) || !std::isfinite(k.period) || !std::isfinite(k.amplitude_scale))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "the kinematics period must be positive and the scale finite");
+    if (k.harmonics < 1 || k.harmonics > BF_MAX_HARMONICS || k.joint_count > BF_MAX_JOINTS)
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "1..4 harmonics and at most 8 joints");
+    for (int i = 0; i < 3; ++i)
+        if (!std::isfinite(k.center[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "the centre must be finite");
+    for (uint32_t j = 0; j < k.joint_count; ++j) {
+        const bf_joint& J = k.joints[j];
+        if (J.side != 1.0f && J.side != -1.0f) return fail(c, BF_ERROR_INVALID_ARGUMENT, "a joint's side is +1 or -1");
+        const float* f = &J.root[0];
+        for (size_t i = 0; i < 3 + 4 * 2 * BF_MAX_HARMONICS; ++i)
+            if (!std::isfinite(f[i])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "joint parameters must be finite");
+    }
+    // the rest pose with each joint's side applied (the degenerate-triangle
+    // check), and the static vertices: those of joint 0
+    std::vector<float> pose(3ull * vertex_count);
+    std::vector<uint8_t> vertex_static(vertex_count, 0);
+    for (uint32_t v = 0; v < vertex_count; ++v) {
+        const uint32_t j = vertex_joint[v];
+        if (j > k.joint_count) return fail(c, BF_ERROR_INVALID_ARGUMENT, "vertex joint out of range");
+        for (int i = 0; i < 3; ++i)
+            if (!std::isfinite(rest[3ull * v + i]))
+                return fail(c, BF_ERROR_INVALID_ARGUMENT, "rest positions must be finite");
+        if (!std::isfinite(span_fraction[v])) return fail(c, BF_ERROR_INVALID_ARGUMENT, "span fractions must be finite");
+        const float side = j == 0 ? 1.0f : k.joints[j - 1].side;
+        pose[3ull * v] = rest[3ull * v];
+        pose[3ull * v + 1] = side * rest[3ull * v + 1];
+        pose[3ull * v + 2] = rest[3ull * v + 2];
+        vertex_static[v] = j == 0 ? 1 : 0;
+    }
+    if (bf_status r = check_triangles(c, pose.data(), vertex_count, triangles, triangle_parts, triangle_count,
+                                      "the rest pose"))
+        return r;
+    if (!(half_thickness > 0.0f)) return fail(c, BF_ERROR_INVALID_ARGUMENT, "half thickness must be positive");
+    MeshSource src;
+    src.rest = rest;
+    src.vertex_joint = vertex_joint;
+    src.span_fraction = span_fraction;
+    src.kinematics = kinematics;
+    return install_mesh(c, src, vertex_count, triangles, triangle_parts, triangle_count, half_thickness, nullptr,
+                        time_start, vertex_static);
+}
+
+bf_status bf_mesh_vertices_at(bf_context* c, float time, float* position3, float* velocity3, uint32_t vertex_count) {
+    BF_ENTER(c);
+    if (c->source != BF_GEOMETRY_MESH) return fail(c, BF_ERROR_INVALID_ARGUMENT, "no mesh uploaded");
+    if (!position3 || !velocity3) return fail(c, BF_ERROR_INVALID_ARGUMENT, "outputs must not be NULL");
+    if (vertex_count != c->m_vertices) return fail(c, BF_ERROR_INVALID_ARGUMENT, "vertex count differs from the mesh");
+    // the prepare pass alone, on the geometry stream after any prefetched
+    // geometry (which has consumed the prepared vertices by then; every
+    // geometry build prepares them again)
+    MeshBuffers b = mesh_buffers(c);
+    const MeshParams m = mesh_params(c, time, true);
+    hipError_t e = launch_mesh_prepare(b, m, c->sb);
+    std::vector<float4> p(vertex_count), v(vertex_count);
+    if (e == hipSuccess) e = hipMemcpyAsync(p.data(), c->m_pos, 16ull * vertex_count, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(v.data(), c->m_vel, 16ull * vertex_count, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
+    if (e != hipSuccess) return hip_fail(c, e, "mesh vertices");
+    for (uint32_t i = 0; i < vertex_count; ++i) {
+        position3[3ull * i] = p[i].x;
+        position3[3ull * i + 1] = p[i].y;
+        position3[3ull * i + 2] = p[i].z;
+        velocity3[3ull * i] = v[i].x;
+        velocity3[3ull * i + 1] = v[i].y;
+        velocity3[3ull * i + 2] = v[i].z;
+    }
+    return BF_OK;
 }
 
 bf_status bf_mesh_build(bf_context* c, float t) {
@@ -2041,6 +2205,8 @@ bf_status adjoint_supported(bf_context* c) {
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers mesh and static geometry");
     if (g.wall_rule == BF_WALL_INTERPOLATED_RAY)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates the signed-distance link fractions");
+    if (c->source == BF_GEOMETRY_MESH && c->m_articulated)
+        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates frame tangents; upload the mesh as frames");
     if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "step once after bf_set_geometry with a different previous mask");
     return BF_OK;
@@ -2225,8 +2391,8 @@ bf_status bf_adjoint_set_tangents(bf_context* c, uint32_t parameter_count, const
     BF_ENTER(c);
     if (parameter_count > 8) return fail(c, BF_ERROR_INVALID_ARGUMENT, "at most 8 parameters");
     if (parameter_count > 0 && !point_tangents) return fail(c, BF_ERROR_INVALID_ARGUMENT, "tangents are NULL");
-    if (parameter_count > 0 && c->source != BF_GEOMETRY_MESH)
-        return fail(c, BF_ERROR_INVALID_ARGUMENT, "tangents describe mesh frames: upload a mesh first");
+    if (parameter_count > 0 && (c->source != BF_GEOMETRY_MESH || c->m_articulated))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "tangents describe mesh frames: upload a mesh as frames first");
     c->adj.tangent_alloc.release();
     c->adj.dpoints = nullptr;
     c->adj.K = 0;

@@ -76,31 +76,17 @@ __device__ __forceinline__ void qrotate_f(const float q[4], const float v[3], fl
     for (int i = 0; i < 3; ++i) o[i] = v[i] + q[3] * t[i] + c[i];
 }
 
-// Vertex positions and lattice velocities at the query time
-// (BirdFlow.metal:1589-1622).  With move_with_body the frames are in the
-// body frame and the device body pose places them:
-// p = x_b + R p_local, v = s (v_b + R (w_b x p_local) + R v_local).
-__global__ void mesh_prepare(const float* __restrict__ points, const float* __restrict__ times,
-                             float4* __restrict__ pos, float4* __restrict__ vel, const double* __restrict__ body,
-                             MeshParams m) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= m.vertex_count) return;
-    const uint32_t first = min(m.first, m.frame_count - 2u);
-    const uint32_t second = first + 1u;
-    const float duration = times[second] - times[first];
-    const float blend = fminf(fmaxf((m.query_time - times[first]) / duration, 0.0f), 1.0f);
-    const float* a = points + ((size_t)first * m.vertex_count + v) * 3;
-    const float* b = points + ((size_t)second * m.vertex_count + v) * 3;
-    const float dxv = b[0] - a[0], dyv = b[1] - a[1], dzv = b[2] - a[2];
-    const float lp[3] = {m.translation[0] + (a[0] + blend * dxv), m.translation[1] + (a[1] + blend * dyv),
-                         m.translation[2] + (a[2] + blend * dzv)};
+// Places vertex v from its local position lp and local velocity lv (per
+// time unit).  With move_with_body the local frame is the body frame and the
+// device body pose places it: p = x_b + R lp, v = s (v_b + R (w_b x lp) + R lv).
+__device__ __forceinline__ void place_vertex(uint32_t v, const float lp[3], const float lv[3],
+                                             const double* __restrict__ body, const MeshParams& m,
+                                             float4* __restrict__ pos, float4* __restrict__ vel) {
     if (!m.move_with_body) {
         pos[v] = make_float4(lp[0], lp[1], lp[2], 0.0f);
-        vel[v] = make_float4(dxv / duration * m.velocity_scale, dyv / duration * m.velocity_scale,
-                             dzv / duration * m.velocity_scale, 0.0f);
+        vel[v] = make_float4(lv[0] * m.velocity_scale, lv[1] * m.velocity_scale, lv[2] * m.velocity_scale, 0.0f);
         return;
     }
-    const float lv[3] = {dxv / duration, dyv / duration, dzv / duration};
     // normalised pose in FP64, then rounded once to FP32
     double qd[4] = {body[3], body[4], body[5], body[6]};
     const double n2 = qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2] + qd[3] * qd[3];
@@ -126,6 +112,34 @@ __global__ void mesh_prepare(const float* __restrict__ points, const float* __re
     pos[v] = make_float4(po[0], po[1], po[2], 0.0f);
     vel[v] = make_float4(ve[0], ve[1], ve[2], 0.0f);
 }
+
+// Vertex positions and lattice velocities at the query time
+// (BirdFlow.metal:1589-1622): frames interpolated, velocity from the frame
+// difference.
+__global__ void mesh_prepare(const float* __restrict__ points, const float* __restrict__ times,
+                             float4* __restrict__ pos, float4* __restrict__ vel, const double* __restrict__ body,
+                             MeshParams m) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m.vertex_count) return;
+    const uint32_t first = min(m.first, m.frame_count - 2u);
+    const uint32_t second = first + 1u;
+    const float duration = times[second] - times[first];
+    const float blend = fminf(fmaxf((m.query_time - times[first]) / duration, 0.0f), 1.0f);
+    const float* a = points + ((size_t)first * m.vertex_count + v) * 3;
+    const float* b = points + ((size_t)second * m.vertex_count + v) * 3;
+    const float dxv = b[0] - a[0], dyv = b[1] - a[1], dzv = b[2] - a[2];
+    const float lp[3] = {m.translation[0] + (a[0] + blend * dxv), m.translation[1] + (a[1] + blend * dyv),
+                         m.translation[2] + (a[2] + blend * dzv)};
+    const float lv[3] = {dxv / duration, dyv / duration, dzv / duration};
+    place_vertex(v, lp, lv, body, m, pos, vel);
+}
+
+// Articulated meshes (kinematics evaluated per vertex; defined below).
+struct Kinematics;
+__global__ void articulated_prepare(const float* __restrict__ rest, const uint8_t* __restrict__ vertex_joint,
+                                    const float* __restrict__ span_fraction, const Kinematics* __restrict__ kin,
+                                    float4* __restrict__ pos, float4* __restrict__ vel,
+                                    const double* __restrict__ body, MeshParams m);
 
 __global__ void fill_u8(uint8_t* __restrict__ p, uint8_t v, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -626,8 +640,12 @@ __global__ void link_census(const uint8_t* __restrict__ prev, const uint8_t* __r
 // ---------------------------------------------------------------------------
 
 hipError_t launch_mesh_prepare(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
-    hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
-                       b.vel, b.body, m);
+    if (b.kinematics)
+        hipLaunchKernelGGL(articulated_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.rest,
+                           b.vertex_joint, b.span_fraction, b.kinematics, b.pos, b.vel, b.body, m);
+    else
+        hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times,
+                           b.pos, b.vel, b.body, m);
     return hipGetLastError();
 }
 
@@ -1404,8 +1422,12 @@ hipError_t launch_mesh_static_keys(const MeshBuffers& b, const MeshParams& m, un
     if (m.g.nx > 65535 || m.g.ny > 65535) return hipErrorInvalidValue;   // raster packs x | y << 16
     if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1 || m.raster_split < 1 || m.raster_split > 64)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
-                       b.vel, b.body, m);
+    if (b.kinematics)
+        hipLaunchKernelGGL(articulated_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.rest,
+                           b.vertex_joint, b.span_fraction, b.kinematics, b.pos, b.vel, b.body, m);
+    else
+        hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times,
+                           b.pos, b.vel, b.body, m);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new, (uint8_t)0, m.g.tiles);
     if (m.raster_count > 0)
         hipLaunchKernelGGL(mesh_raster, dim3(std::min((m.raster_count + m.raster_batch - 1) / m.raster_batch, m.raster_blocks), m.raster_split), dim3(64), 0, s, b.pos,
@@ -1423,12 +1445,94 @@ hipError_t launch_collect_keys(unsigned long long* keys, uint32_t n, uint32_t* c
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Articulated mesh: vertices on rigid joints driven by Fourier series
+// (bf_mesh_upload_articulated).  Per joint: stroke, deviation, pitch and
+// twist angles and their rates at the query time; per vertex: the rest
+// position mirrored by the joint's side, pitch + span-weighted twist about
+// y, deviation about z, stroke about x, the hinge and the mesh centre; the
+// velocity is the time derivative.  FP32 with bird_sincos, in the oracle's
+// order (oracle/bfref_kin.inc): bit-identical positions and velocities.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void kin_series(const float (&c)[4][2], uint32_t harmonics, const float* sh,
+                                           const float* ch, float omega, float scale, float& angle, float& rate) {
+    float a = 0.0f, r = 0.0f;
+    for (uint32_t h = 0; h < harmonics; ++h) {
+        a = a + (c[h][0] * sh[h] + c[h][1] * ch[h]);
+        r = r + (float)(h + 1) * (c[h][0] * ch[h] - c[h][1] * sh[h]);
+    }
+    angle = a * scale;
+    rate = (r * omega) * scale;
+}
+
+__global__ void articulated_prepare(const float* __restrict__ rest, const uint8_t* __restrict__ vertex_joint,
+                                    const float* __restrict__ span_fraction, const Kinematics* __restrict__ kin,
+                                    float4* __restrict__ pos, float4* __restrict__ vel,
+                                    const double* __restrict__ body, MeshParams m) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m.vertex_count) return;
+    const Kinematics& k = *kin;
+    const float* r = rest + 3 * (size_t)v;
+    const uint32_t jv = vertex_joint[v];
+    float lp[3], lv[3];
+    if (jv == 0 || jv > k.joint_count) {
+        for (int i = 0; i < 3; ++i) {
+            lp[i] = k.center[i] + r[i];
+            lv[i] = 0.0f;
+        }
+    } else {
+        const KinJoint& J = k.joints[jv - 1];
+        const uint32_t harmonics = min(k.harmonics, 4u);
+        const float t = m.query_time;
+        const float period = k.period;
+        const float cycles = floorf(t / period);
+        const float tm = t - cycles * period;
+        const float omega = 6.28318548f / period;
+        const float phi = omega * tm;
+        float sh[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ch[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t h = 0; h < harmonics; ++h) bird_sincos((float)(h + 1) * phi, sh[h], ch[h]);
+        float ang[4], rate[4];
+        kin_series(J.stroke, harmonics, sh, ch, omega, k.amplitude_scale, ang[0], rate[0]);
+        kin_series(J.deviation, harmonics, sh, ch, omega, k.amplitude_scale, ang[1], rate[1]);
+        kin_series(J.pitch, harmonics, sh, ch, omega, k.amplitude_scale, ang[2], rate[2]);
+        kin_series(J.twist, harmonics, sh, ch, omega, k.amplitude_scale, ang[3], rate[3]);
+        const float side = J.side;
+        float sa, ca, sb, cb;
+        bird_sincos(side * ang[0], sa, ca);
+        bird_sincos(side * ang[1], sb, cb);
+        const float l[3] = {r[0], side * r[1], r[2]};
+        const float span = span_fraction[v];
+        const float a = ang[2] + ang[3] * span;
+        const float adot = rate[2] + rate[3] * span;
+        float sp, cp;
+        bird_sincos(a, sp, cp);
+        const float u3[3] = {cp * l[0] + sp * l[2], l[1], -sp * l[0] + cp * l[2]};
+        const float u2[3] = {cb * u3[0] - sb * u3[1], sb * u3[0] + cb * u3[1], u3[2]};
+        const float u1[3] = {u2[0], ca * u2[1] - sa * u2[2], sa * u2[1] + ca * u2[2]};
+        const float root[3] = {J.root[0], side * J.root[1], J.root[2]};
+        for (int i = 0; i < 3; ++i) lp[i] = (k.center[i] + root[i]) + u1[i];
+        const float ad = side * rate[0], bd = side * rate[1];
+        const float t1[3] = {0.0f, -(ad * u1[2]), ad * u1[1]};
+        const float w2[3] = {-(bd * u2[1]), bd * u2[0], 0.0f};
+        const float t2[3] = {w2[0], ca * w2[1] - sa * w2[2], sa * w2[1] + ca * w2[2]};
+        const float w3[3] = {adot * u3[2], 0.0f, -(adot * u3[0])};
+        const float z3[3] = {cb * w3[0] - sb * w3[1], sb * w3[0] + cb * w3[1], w3[2]};
+        const float t3[3] = {z3[0], ca * z3[1] - sa * z3[2], sa * z3[1] + ca * z3[2]};
+        for (int i = 0; i < 3; ++i) lv[i] = (t1[i] + t2[i]) + t3[i];
+    }
+    place_vertex(v, lp, lv, body, m, pos, vel);
+}
+
 hipError_t launch_mesh_geometry(const MeshBuffers& b, const MeshParams& m, hipStream_t s) {
     if (m.g.nx > 65535 || m.g.ny > 65535) return hipErrorInvalidValue;   // raster packs x | y << 16
     if (m.raster_batch < 1 || m.raster_batch > 64 || m.raster_blocks < 1 || m.raster_split < 1 || m.raster_split > 64)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times, b.pos,
-                       b.vel, b.body, m);
+    if (b.kinematics)
+        hipLaunchKernelGGL(articulated_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.rest,
+                           b.vertex_joint, b.span_fraction, b.kinematics, b.pos, b.vel, b.body, m);
+    else
+        hipLaunchKernelGGL(mesh_prepare, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, b.points, b.times,
+                           b.pos, b.vel, b.body, m);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.tiles_new, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u8, dim3((m.g.tiles + 255) / 256), dim3(256), 0, s, b.dyn, (uint8_t)0, m.g.tiles);
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, b.list_count, 0u, 1u);

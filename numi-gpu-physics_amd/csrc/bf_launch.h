@@ -132,10 +132,35 @@ struct WingBuffers {
     float* links;                     // [tile][19][256] link fractions of fluid cells with a solid source
 };
 
+// Fourier joint kinematics of an articulated mesh (layout of bf_kinematics).
+struct KinJoint {
+    float side;
+    float root[3];
+    float stroke[4][2];
+    float deviation[4][2];
+    float pitch[4][2];
+    float twist[4][2];
+};
+struct Kinematics {
+    float period;
+    float amplitude_scale;
+    uint32_t harmonics;
+    uint32_t joint_count;
+    float center[3];
+    float reserved;
+    KinJoint joints[8];
+};
+
 struct MeshBuffers {
     const double* body;                 // body state, read when MeshParams::move_with_body
     const float* points;
     const float* times;
+    // articulated mesh (instead of frames): rest positions [V][3], joint of
+    // each vertex, twist weight, kinematics (device copy)
+    const float* rest;
+    const uint8_t* vertex_joint;
+    const float* span_fraction;
+    const Kinematics* kinematics;
     const uint32_t* tri;
     const uint8_t* tri_part;
     float4* pos;
@@ -168,7 +193,7 @@ struct RayParams {
     float half_thickness;
     int bc[3][2];
     uint32_t triangle_count;
-    uint32_t triangle_blocks;  // triangle pass grid (four triangles per block, grid-stride)
+    uint32_t triangle_blocks;  // triangle pass grid (one triangle per single-wavefront block, grid-stride)
     uint32_t split;            // wavefronts per triangle (its cells split among them), 1..64
     uint32_t tile_pass_blocks; // init / finalize grid (grid-stride over listed tiles)
 };
@@ -181,6 +206,7 @@ struct RayBuffers {
     const uint8_t* dyn;      // tiles whose cells pull from a solid cell
     uint32_t* list;          // scratch: listed tiles
     uint32_t* list_count;
+    uint32_t* dirs;          // per cell: link directions pulling from it (zero between steps)
     float* links;            // the step's table
 };
 

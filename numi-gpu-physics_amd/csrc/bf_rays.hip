@@ -27,19 +27,22 @@
 // solid-proximity tile flags of the step are final):
 //  1. init: every cell of a tile flagged as pulling from a solid cell writes
 //     its 19 table entries: 2 (no entry yet) for a fluid cell's link whose
-//     source is solid, 0 otherwise;
+//     source is solid, 0 otherwise, and sets bit q of the source's link
+//     directions;
 //  2. triangles: one wavefront per triangle (grid-stride; a mesh of few
 //     large triangles splits each triangle's cells over several
-//     wavefronts).  Lanes take the solid cells S of the triangle's shell box
-//     grown by one cell; for each direction whose segment box meets the
-//     shell box (the oracle's rejection test, the same floats) the entry t
-//     is computed and, when t <= 1, merged into every fluid cell that pulls
-//     from S along that direction (periodic wrap and outlet clamps included)
-//     with a 32-bit atomicMin on the entry's bits (t >= 0, so the integer
-//     order is the float order; -0 is stored as +0).  The minimum over the
-//     mesh does not depend on the order of the atomics;
+//     wavefronts).  Lanes take the cells S of the triangle's shell box grown
+//     by one cell; each link direction of S whose segment box meets the
+//     shell box (the oracle's rejection test, the same floats) is queued in
+//     LDS, and the queue is spread over the lanes: the entry t is computed
+//     and, when t <= 1, merged into every fluid cell that pulls from S along
+//     that direction (periodic wrap and outlet clamps included) with a
+//     32-bit atomicMin on the bits of max(t, 1e-4) (non-negative floats order
+//     as their bits).  The minimum over the mesh does not depend on the
+//     order of the atomics;
 //  3. finalize: each link entry becomes the fraction (the entry when <= 1,
-//     else the signed-distance ratio), clamped.
+//     else the signed-distance ratio), clamped; the link directions are
+//     cleared for the next step.
 // Rejection by the shell boxes never drops a triangle whose entry lies in
 // the link: an entry point lies inside its triangle's shell box, so the
 // segment box and the shell box share it.
@@ -171,6 +174,10 @@ __device__ __forceinline__ int pullers(const int bc[2], int s, int c, int n, int
     return k;
 }
 
+__device__ __forceinline__ uint32_t lane_rank64(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 __device__ __forceinline__ size_t table_slot(const Grid& g, int x, int y, int z, int q) {
     const uint32_t lane = (uint32_t)(x & (TILE_X - 1)) + (uint32_t)(y & (TILE_Y - 1)) * (uint32_t)TILE_X;
     return ((size_t)tile_of(g, x, y, z) * 19u + (size_t)q) * TILE_CELLS + lane;
@@ -198,11 +205,24 @@ __global__ __launch_bounds__(256) void ray_tile_list(const uint8_t* __restrict__
     if (flagged) list[base + rank] = t;
 }
 
+// Source of direction q of cell (x, y, z) under the face treatment; false
+// beyond a far-field face.
+__device__ __forceinline__ bool ray_source_cell(const RayParams& p, int x, int y, int z, int q, uint32_t& src) {
+    int sx, sy, sz;
+    if (!ray_source(p.bc[0], x - cx(q), p.g.nx, sx) || !ray_source(p.bc[1], y - cy(q), p.g.ny, sy) ||
+        !ray_source(p.bc[2], z - cz(q), p.g.nz, sz))
+        return false;
+    src = (uint32_t)sx + (uint32_t)p.g.nx * ((uint32_t)sy + (uint32_t)p.g.ny * (uint32_t)sz);
+    return true;
+}
+
 // 1. the table entries of the listed tiles: 2 on a fluid cell's links whose
-// source is solid, 0 elsewhere.
+// source is solid, 0 elsewhere; each link sets bit q of its source's link
+// directions (`dirs`), which the triangle pass reads to skip solid cells no
+// fluid cell pulls from.  The 18 sources' part ids are loaded together.
 __global__ __launch_bounds__(256) void ray_init(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
                                                 const uint8_t* __restrict__ mask, float* __restrict__ links,
-                                                RayParams p) {
+                                                uint32_t* __restrict__ dirs, RayParams p) {
     const uint32_t n_tiles = *count;
     for (uint32_t i = blockIdx.x; i < n_tiles; i += gridDim.x) {
         const uint32_t tile = list[i];
@@ -212,34 +232,49 @@ __global__ __launch_bounds__(256) void ray_init(const uint32_t* __restrict__ lis
         const int y = y0 + (int)(threadIdx.x / TILE_X);
         if (x >= p.g.nx || y >= p.g.ny) continue;
         const uint32_t gid = (uint32_t)x + (uint32_t)p.g.nx * ((uint32_t)y + (uint32_t)p.g.ny * (uint32_t)z);
-        const bool fluid = mask[gid] == 0;
+        uint32_t src[19];
+        uint32_t okbits = 0;
+#pragma unroll
+        for (int q = 1; q < 19; ++q) {
+            uint32_t sidx = gid;
+            if (ray_source_cell(p, x, y, z, q, sidx)) okbits |= 1u << q;
+            src[q] = sidx;
+        }
+        const uint8_t own = mask[gid];
+        uint8_t part[19];
+#pragma unroll
+        for (int q = 1; q < 19; ++q) part[q] = mask[src[q]];
         uint32_t* out = (uint32_t*)links + (size_t)tile * 19 * TILE_CELLS + threadIdx.x;
         out[0] = 0u;
+#pragma unroll
         for (int q = 1; q < 19; ++q) {
-            uint32_t v = 0u;
-            int sx, sy, sz;
-            if (fluid && ray_source(p.bc[0], x - cx(q), p.g.nx, sx) && ray_source(p.bc[1], y - cy(q), p.g.ny, sy) &&
-                ray_source(p.bc[2], z - cz(q), p.g.nz, sz)) {
-                const uint32_t src = (uint32_t)sx + (uint32_t)p.g.nx * ((uint32_t)sy + (uint32_t)p.g.ny * (uint32_t)sz);
-                if (mask[src] != 0) v = NO_ENTRY;
-            }
-            out[(size_t)q * TILE_CELLS] = v;
+            const bool link = own == 0 && ((okbits >> q) & 1u) && part[q] != 0;
+            out[(size_t)q * TILE_CELLS] = link ? NO_ENTRY : 0u;
+            if (link) (void)atomicOr(&dirs[src[q]], 1u << q);
         }
     }
 }
 
-// 2. entries of every triangle's shell, merged by atomicMin.  Block: four
-// wavefronts, one triangle each; grid y splits a triangle's cells.
-__global__ __launch_bounds__(256) void ray_triangles(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
-                                                     const uint8_t* __restrict__ mask, float* __restrict__ links,
-                                                     RayParams p) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x >> 6;
+// 2. entries of every triangle's shell, merged by atomicMin.  One
+// single-wavefront workgroup per triangle (grid-stride); grid y splits a
+// large triangle's cells.  Lanes take the cells of the triangle's shell box
+// grown by one cell; a solid cell with link directions queues each
+// direction whose segment box meets the shell box as a (cell, q) pair in
+// LDS; the queued pairs are then spread over the lanes for the entry
+// computation, so that every lane has work.
+constexpr uint32_t RAY_CAP = 256;
+
+__global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
+                                                    const uint8_t* __restrict__ mask,
+                                                    const uint32_t* __restrict__ dirs, float* __restrict__ links,
+                                                    RayParams p) {
+    __shared__ uint2 cand[RAY_CAP];
+    const uint32_t lane = threadIdx.x;
     const float dx = p.dx;
     const float h = p.half_thickness;
     const int n[3] = {p.g.nx, p.g.ny, p.g.nz};
     uint32_t* table = (uint32_t*)links;
-    for (uint32_t t = blockIdx.x * 4u + wave; t < p.triangle_count; t += gridDim.x * 4u) {
+    for (uint32_t t = blockIdx.x; t < p.triangle_count; t += gridDim.x) {
         const float4 a4 = pos[tri[3 * (size_t)t]], b4 = pos[tri[3 * (size_t)t + 1]], c4 = pos[tri[3 * (size_t)t + 2]];
         const float A[3] = {a4.x, a4.y, a4.z}, B[3] = {b4.x, b4.y, b4.z}, C[3] = {c4.x, c4.y, c4.z};
         // the shell's box (the oracle's rejection box)
@@ -254,72 +289,106 @@ __global__ __launch_bounds__(256) void ray_triangles(const float4* __restrict__ 
             bl[a] = l - h - 0.01f * dx;
             bh[a] = u + h + 0.01f * dx;
             // source cells whose segment box (within one cell of the source
-            // centre) can meet it: centres in [bl - dx, bh + dx], one more
-            // cell each side for rounding
+            // centre) can meet it: centres in [bl - dx, bh + dx], widened by
+            // 0.01 cell for the rounding of the centres
             const float sl = ((bl[a] - dx) - p.origin[a]) / dx - 0.5f;
             const float su = ((bh[a] + dx) - p.origin[a]) / dx - 0.5f;
             const float top = (float)n[a] + 2.0f;
-            const int il = (int)floorf(fminf(fmaxf(sl, -2.0f), top)) - 1;
-            const int iu = (int)ceilf(fminf(fmaxf(su, -2.0f), top)) + 1;
+            const int il = (int)floorf(fminf(fmaxf(sl - 0.01f, -2.0f), top));
+            const int iu = (int)ceilf(fminf(fmaxf(su + 0.01f, -2.0f), top));
             lo[a] = max(il, 0);
             ext[a] = max(min(iu, n[a] - 1) - lo[a] + 1, 0);
         }
         const uint32_t cells = (uint32_t)ext[0] * (uint32_t)ext[1] * (uint32_t)ext[2];
-        for (uint32_t k = blockIdx.y * 64u + lane; k < cells; k += gridDim.y * 64u) {
-            const int sx = lo[0] + (int)(k % (uint32_t)ext[0]);
-            const uint32_t r = k / (uint32_t)ext[0];
-            const int sy = lo[1] + (int)(r % (uint32_t)ext[1]);
-            const int sz = lo[2] + (int)(r / (uint32_t)ext[1]);
-            const uint32_t src = (uint32_t)sx + (uint32_t)n[0] * ((uint32_t)sy + (uint32_t)n[1] * (uint32_t)sz);
-            if (mask[src] == 0) continue;
-            const float S[3] = {p.origin[0] + ((float)sx + 0.5f) * dx, p.origin[1] + ((float)sy + 0.5f) * dx,
-                                p.origin[2] + ((float)sz + 0.5f) * dx};
-            const int sc[3] = {sx, sy, sz};
-            for (int q = 1; q < 19; ++q) {
+        uint32_t count = 0;   // wave-uniform
+        auto flush = [&]() {
+            __syncthreads();
+            for (uint32_t k = lane; k < count; k += 64) {
+                const uint2 e = cand[k];
+                const int sx = (int)(e.x & 0xFFFFu), sy = (int)(e.x >> 16), sz = (int)(e.y >> 5);
+                const int q = (int)(e.y & 31u);
                 const int c[3] = {cx(q), cy(q), cz(q)};
+                const float S[3] = {p.origin[0] + ((float)sx + 0.5f) * dx, p.origin[1] + ((float)sy + 0.5f) * dx,
+                                    p.origin[2] + ((float)sz + 0.5f) * dx};
                 const float F[3] = {S[0] + (float)c[0] * dx, S[1] + (float)c[1] * dx, S[2] + (float)c[2] * dx};
-                bool meets = true;
-                for (int a = 0; a < 3; ++a) {
-                    const float l = F[a] < S[a] ? F[a] : S[a];
-                    const float u = F[a] < S[a] ? S[a] : F[a];
-                    meets = meets && !(u < bl[a] || l > bh[a]);
-                }
-                if (!meets) continue;
+                const float D[3] = {(float)(-c[0]) * dx, (float)(-c[1]) * dx, (float)(-c[2]) * dx};
+                const float v = ray_shell(F, D, A, B, C, h);
+                if (!(v <= 1.0f)) continue;
+                // entries below the table's floor 1e-4 are stored as the
+                // floor (the minimum of the floored entries is the floored
+                // minimum; a link entry is then never 0, the init value of
+                // the entries that are not links)
+                const uint32_t bits = __float_as_uint(fmaxf(v, 1.0e-4f));
                 // the fluid cells pulling from S along c (up to 8 with wrap
-                // and clamps); none: no link to serve
+                // and clamps)
                 int fx[2], fy[2], fz[2];
-                const int kx = pullers(p.bc[0], sc[0], c[0], n[0], fx);
-                const int ky = pullers(p.bc[1], sc[1], c[1], n[1], fy);
-                const int kz = pullers(p.bc[2], sc[2], c[2], n[2], fz);
-                uint32_t fluid = 0u;
+                const int kx = pullers(p.bc[0], sx, c[0], n[0], fx);
+                const int ky = pullers(p.bc[1], sy, c[1], n[1], fy);
+                const int kz = pullers(p.bc[2], sz, c[2], n[2], fz);
                 for (int iz = 0; iz < kz; ++iz)
                     for (int iy = 0; iy < ky; ++iy)
                         for (int ix = 0; ix < kx; ++ix) {
                             const uint32_t gid =
                                 (uint32_t)fx[ix] + (uint32_t)n[0] * ((uint32_t)fy[iy] + (uint32_t)n[1] * (uint32_t)fz[iz]);
-                            if (mask[gid] == 0) fluid |= 1u << ((iz * 2 + iy) * 2 + ix);
-                        }
-                if (fluid == 0u) continue;
-                const float D[3] = {(float)(-c[0]) * dx, (float)(-c[1]) * dx, (float)(-c[2]) * dx};
-                const float v = ray_shell(F, D, A, B, C, h);
-                if (!(v <= 1.0f)) continue;
-                const uint32_t bits = __float_as_uint(v == 0.0f ? 0.0f : v);
-                for (int iz = 0; iz < kz; ++iz)
-                    for (int iy = 0; iy < ky; ++iy)
-                        for (int ix = 0; ix < kx; ++ix)
-                            if ((fluid >> ((iz * 2 + iy) * 2 + ix)) & 1u)
+                            if (mask[gid] == 0)
                                 (void)atomicMin(&table[table_slot(p.g, fx[ix], fy[iy], fz[iz], q)], bits);
+                        }
+            }
+            __syncthreads();
+            count = 0;
+        };
+        for (uint32_t k0 = blockIdx.y * 64u; k0 < cells; k0 += gridDim.y * 64u) {
+            const uint32_t k = k0 + lane;
+            uint32_t bits = 0u;
+            int sx = 0, sy = 0, sz = 0;
+            if (k < cells) {
+                sx = lo[0] + (int)(k % (uint32_t)ext[0]);
+                const uint32_t r = k / (uint32_t)ext[0];
+                sy = lo[1] + (int)(r % (uint32_t)ext[1]);
+                sz = lo[2] + (int)(r / (uint32_t)ext[1]);
+                bits = dirs[(uint32_t)sx + (uint32_t)n[0] * ((uint32_t)sy + (uint32_t)n[1] * (uint32_t)sz)];
+            }
+            float S[3] = {0.0f, 0.0f, 0.0f};
+            if (bits) {
+                S[0] = p.origin[0] + ((float)sx + 0.5f) * dx;
+                S[1] = p.origin[1] + ((float)sy + 0.5f) * dx;
+                S[2] = p.origin[2] + ((float)sz + 0.5f) * dx;
+            }
+            // one direction per lane per round, lowest bit first
+            while (__ballot(bits != 0u)) {
+                bool pass = false;
+                int q = 0;
+                if (bits) {
+                    q = __ffs(bits) - 1;
+                    bits &= bits - 1u;
+                    const float F[3] = {S[0] + (float)cx(q) * dx, S[1] + (float)cy(q) * dx, S[2] + (float)cz(q) * dx};
+                    pass = true;
+                    for (int a = 0; a < 3; ++a) {
+                        const float l = F[a] < S[a] ? F[a] : S[a];
+                        const float u = F[a] < S[a] ? S[a] : F[a];
+                        pass = pass && !(u < bl[a] || l > bh[a]);
+                    }
+                }
+                const unsigned long long ballot = __ballot(pass);
+                if (pass)
+                    cand[count + lane_rank64(ballot)] =
+                        make_uint2((uint32_t)sx | ((uint32_t)sy << 16), ((uint32_t)sz << 5) | (uint32_t)q);
+                count += (uint32_t)__popcll(ballot);
+                if (count > RAY_CAP - 64) flush();
             }
         }
+        if (count > 0) flush();
     }
 }
 
 // 3. fractions: the entry when it lies in the link, else the signed-distance
-// ratio; clamped to [1e-4, 1].
+// ratio; clamped to [1e-4, 1].  Clears the cell's link directions for the
+// next step.
 __global__ __launch_bounds__(256) void ray_finalize(const uint32_t* __restrict__ list,
                                                     const uint32_t* __restrict__ count,
                                                     const uint8_t* __restrict__ mask, const float4* __restrict__ wall,
-                                                    float* __restrict__ links, RayParams p) {
+                                                    float* __restrict__ links, uint32_t* __restrict__ dirs,
+                                                    RayParams p) {
     const uint32_t n_tiles = *count;
     for (uint32_t i = blockIdx.x; i < n_tiles; i += gridDim.x) {
         const uint32_t tile = list[i];
@@ -329,21 +398,23 @@ __global__ __launch_bounds__(256) void ray_finalize(const uint32_t* __restrict__
         const int y = y0 + (int)(threadIdx.x / TILE_X);
         if (x >= p.g.nx || y >= p.g.ny) continue;
         const uint32_t gid = (uint32_t)x + (uint32_t)p.g.nx * ((uint32_t)y + (uint32_t)p.g.ny * (uint32_t)z);
+        dirs[gid] = 0u;
         if (mask[gid] != 0) continue;
-        const float wf = wall[gid].w;
         float* out = links + (size_t)tile * 19 * TILE_CELLS + threadIdx.x;
+        float best[19];
+#pragma unroll
+        for (int q = 1; q < 19; ++q) best[q] = out[(size_t)q * TILE_CELLS];
+        const float wf = wall[gid].w;
+#pragma unroll
         for (int q = 1; q < 19; ++q) {
-            int sx, sy, sz;
-            if (!ray_source(p.bc[0], x - cx(q), p.g.nx, sx) || !ray_source(p.bc[1], y - cy(q), p.g.ny, sy) ||
-                !ray_source(p.bc[2], z - cz(q), p.g.nz, sz))
-                continue;
-            const uint32_t src = (uint32_t)sx + (uint32_t)p.g.nx * ((uint32_t)sy + (uint32_t)p.g.ny * (uint32_t)sz);
-            if (mask[src] == 0) continue;
-            const float best = out[(size_t)q * TILE_CELLS];
+            // init left 0 on every entry that is not a link
+            if (best[q] == 0.0f) continue;
             float qf;
-            if (best <= 1.0f) {
-                qf = best;
+            if (best[q] <= 1.0f) {
+                qf = best[q];
             } else {
+                uint32_t src = gid;
+                (void)ray_source_cell(p, x, y, z, q, src);
                 const float ws = wall[src].w;
                 const float df = wf > 0.0f ? wf : 0.0f;
                 const float ds = ws < 0.0f ? ws : 0.0f;
@@ -359,17 +430,19 @@ __global__ __launch_bounds__(256) void ray_finalize(const uint32_t* __restrict__
 
 hipError_t launch_ray_links(const RayBuffers& b, const RayParams& p, hipStream_t s) {
     if (p.split < 1 || p.split > 64 || p.triangle_blocks < 1 || p.tile_pass_blocks < 1) return hipErrorInvalidValue;
+    // queued pairs pack x | y << 16 and z << 5 | q
+    if (p.g.nx > 65535 || p.g.ny > 65535 || p.g.nz >= (1 << 27)) return hipErrorInvalidValue;
     const unsigned tile_blocks = (p.g.tiles + 255) / 256;
     hipLaunchKernelGGL(ray_clear_count, dim3(1), dim3(64), 0, s, b.list_count);
     hipLaunchKernelGGL(ray_tile_list, dim3(tile_blocks), dim3(256), 0, s, (const uint8_t*)b.dyn, p.g.tiles, b.list,
                        b.list_count);
     hipLaunchKernelGGL(ray_init, dim3(p.tile_pass_blocks), dim3(256), 0, s, (const uint32_t*)b.list,
-                       (const uint32_t*)b.list_count, b.mask, b.links, p);
+                       (const uint32_t*)b.list_count, b.mask, b.links, b.dirs, p);
     if (p.triangle_count > 0)
-        hipLaunchKernelGGL(ray_triangles, dim3(p.triangle_blocks, p.split), dim3(256), 0, s, b.pos, b.tri, b.mask,
-                           b.links, p);
+        hipLaunchKernelGGL(ray_triangles, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, b.pos, b.tri, b.mask,
+                           (const uint32_t*)b.dirs, b.links, p);
     hipLaunchKernelGGL(ray_finalize, dim3(p.tile_pass_blocks), dim3(256), 0, s, (const uint32_t*)b.list,
-                       (const uint32_t*)b.list_count, b.mask, b.wall, b.links, p);
+                       (const uint32_t*)b.list_count, b.mask, b.wall, b.links, b.dirs, p);
     return hipGetLastError();
 }
 

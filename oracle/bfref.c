@@ -507,6 +507,7 @@ void bfo_place_vertices(const bfo_body* b, int vertex_count, float velocity_scal
 
 #include "bfref_bird.inc"
 #include "bfref_wing.inc"
+#include "bfref_kin.inc"
 
 /* ---- exact link fractions against a meshed shell ---- */
 static float rdot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }

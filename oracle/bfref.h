@@ -237,6 +237,33 @@ typedef struct bfo_bird_frame {
 } bfo_bird_frame;
 
 /* sin / cos of the geometry (quadrant reduction + Taylor polynomials). */
+/* Articulated mesh with Fourier joint kinematics (this build's on-device
+ * kinematics; the reference drives its bird's wings by stroke, deviation,
+ * pitch and tip-twist angles about a hinge, BirdFlow.metal:611-814).  Same
+ * layout as bf_kinematics in include/birdflow_hip.h. */
+typedef struct {
+    float side;
+    float root[3];
+    float stroke[4][2];
+    float deviation[4][2];
+    float pitch[4][2];
+    float twist[4][2];
+} bfo_joint;
+typedef struct {
+    float period;
+    float amplitude_scale;
+    uint32_t harmonics;
+    uint32_t joint_count;
+    float center[3];
+    float reserved;
+    bfo_joint joints[8];
+} bfo_kinematics;
+/* Positions and velocities (velocity_scale x d pos / dt) of an articulated
+ * mesh at time t (the formula documented with bf_mesh_upload_articulated). */
+void bfo_articulated_vertices(const float* rest, const uint8_t* vertex_joint, const float* span_fraction,
+                              int vertex_count, const bfo_kinematics* k, float t, float velocity_scale,
+                              float* position3, float* velocity3);
+
 void bfo_sincos(float x, float* s, float* c);
 void bfo_sincos_n(const float* x, int n, float* s, float* c);
 /* prepareBirdGeometry (BirdFlow.metal:686-814) at `time` for a body pose;

@@ -95,6 +95,53 @@ class Wing(ctypes.Structure):
         "stroke_rate_max", "reference_speed")]
 
 
+class Joint(ctypes.Structure):
+    """bfo_joint: one articulated joint (side, hinge, Fourier series)."""
+    _fields_ = [("side", ctypes.c_float), ("root", ctypes.c_float * 3)] + [
+        (name, (ctypes.c_float * 2) * 4) for name in ("stroke", "deviation", "pitch", "twist")]
+
+
+class Kinematics(ctypes.Structure):
+    """bfo_kinematics: period, amplitude scale, harmonics, joints."""
+    _fields_ = [("period", ctypes.c_float), ("amplitude_scale", ctypes.c_float), ("harmonics", ctypes.c_uint32),
+                ("joint_count", ctypes.c_uint32), ("center", ctypes.c_float * 3), ("reserved", ctypes.c_float),
+                ("joints", Joint * 8)]
+
+
+def make_kinematics(period, amplitude_scale, center, joints, harmonics=None) -> Kinematics:
+    """joints: list of dicts with side, root[3] and stroke / deviation /
+    pitch / twist arrays [harmonics][2] (sin, cos coefficients, radians)."""
+    k = Kinematics()
+    k.period, k.amplitude_scale = period, amplitude_scale
+    k.center[:] = [float(v) for v in center]
+    k.joint_count = len(joints)
+    h_max = 1
+    for i, j in enumerate(joints):
+        J = k.joints[i]
+        J.side = float(j["side"])
+        J.root[:] = [float(v) for v in j["root"]]
+        for name in ("stroke", "deviation", "pitch", "twist"):
+            c = np.asarray(j[name], np.float32).reshape(-1, 2)
+            h_max = max(h_max, c.shape[0])
+            arr = getattr(J, name)
+            for h in range(c.shape[0]):
+                arr[h][0], arr[h][1] = float(c[h, 0]), float(c[h, 1])
+    k.harmonics = harmonics if harmonics is not None else h_max
+    return k
+
+
+def articulated_vertices(rest, vertex_joint, span_fraction, k: Kinematics, t, velocity_scale=1.0):
+    """Positions and velocities [V][3] of an articulated mesh at time t."""
+    rest = np.ascontiguousarray(rest, np.float32).reshape(-1, 3)
+    vj = np.ascontiguousarray(vertex_joint, np.uint8)
+    sf = np.ascontiguousarray(span_fraction, np.float32)
+    pos = np.empty_like(rest)
+    vel = np.empty_like(rest)
+    lib().bfo_articulated_vertices(_ptr(rest), _ptr(vj), _ptr(sf), rest.shape[0], ctypes.byref(k), np.float32(t),
+                                   np.float32(velocity_scale), _ptr(pos), _ptr(vel))
+    return pos, vel
+
+
 WING_FRAME_FIELDS = ("root", "chord", "span", "normal", "angular", "state")
 
 
@@ -160,6 +207,8 @@ def _declare(l):
                                   vp, vp, vp, ctypes.c_double, ctypes.c_uint64, vp, vp]
     l.bfo_body_safety.restype = ctypes.c_uint32
     l.bfo_place_vertices.argtypes = [ctypes.POINTER(Body), ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
+    l.bfo_articulated_vertices.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.POINTER(Kinematics), ctypes.c_float,
+                                           ctypes.c_float, vp, vp]
     l.bfo_sincos_n.argtypes = [vp, ctypes.c_int, vp, vp]
     l.bfo_set_link_table.argtypes = [vp]
     l.bfo_wing_pow.argtypes = [ctypes.c_float, ctypes.c_float]

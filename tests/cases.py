@@ -345,3 +345,55 @@ class DeetjenCoupling:
                              load=ld, mask=mask, populations=fo))
             f, prev = fo, mask
         return rows
+
+
+# ---------------------------------------------------------------------------
+# The reference's production fixed-wing diagnostic
+# (MetalWingValidator.runSingleCase(resolution: 80),
+# MetalWingValidation.swift:102-147, 295-436; locked in
+# MetalExecutionTests.swift:2204-2224): the Taira & Colonius AR = 2 flat
+# plate at Re 100 and 30 degrees, a one-cell voxel plate
+# (initializeFixedWingCase, BirdFlow.metal:2357-2396) in uniform flow at
+# 0.08 inclined by the angle of attack, 80 x 80 x 48 cells (8 per chord),
+# far-field faces, a 4-cell sponge of strength 0.04, TRT with magic 3/16,
+# halfway bounce-back, force mode 6; the force of step 1,300 (U t / c = 13).
+# ---------------------------------------------------------------------------
+FIXED_WING_LOCKED = {"lift": 0.5426947226244379, "drag": 0.672869301552491}
+
+
+class FixedWing:
+    def __init__(self, resolution=80):
+        f32 = np.float32
+        self.chord = resolution // 10
+        span = 2 * self.chord
+        shape = (resolution, self.chord * 10, self.chord * 6)
+        self.shape = shape
+        speed = f32(0.08)
+        # LatticeScaling (SimulationConfiguration.swift:137-183) in FP32:
+        # length = chord cells, reference speed = lattice speed, Re 100
+        nu = f32(speed * f32(self.chord)) / f32(100)
+        tau_p = f32(f32(0.5) + f32(3) * nu)
+        tau_m = f32(f32(0.5) + f32(f32(3.0) / f32(16.0)) / f32(tau_p - f32(0.5)))
+        self.alpha = 30.0 * math.pi / 180.0
+        a32 = f32(self.alpha)
+        far = (float(f32(speed * f32(math.cos(a32)))), float(f32(speed * f32(math.sin(a32)))), 0.0)
+        center = (f32(0.3) * f32(resolution), f32(0.5) * f32(shape[1]) + f32(0.5), f32(0.5) * f32(shape[2]))
+        z, y, x = np.meshgrid(np.arange(shape[2]), np.arange(shape[1]), np.arange(shape[0]), indexing="ij")
+        rx = (x.astype(f32) + f32(0.5)) - center[0]
+        ry = (y.astype(f32) + f32(0.5)) - center[1]
+        rz = (z.astype(f32) + f32(0.5)) - center[2]
+        plate = (np.abs(rx) <= f32(0.5 * self.chord)) & (np.abs(rz) <= f32(0.5 * span)) & (np.abs(ry) <= f32(0.5))
+        self.mask = plate.astype(np.uint8).ravel()
+        self.wall = np.zeros((self.mask.size, 4), np.float32)
+        self.params = bfref.make_params(
+            shape, omega_plus=float(f32(1) / tau_p), omega_minus=float(f32(1) / tau_m),
+            bc=((bfref.BC_FARFIELD, bfref.BC_FARFIELD),) * 3, collision=bfref.COLLISION_TRT,
+            wall_rule=bfref.WALL_HALFWAY, force_mode=6, accumulate_loads=True, sponge_strength=0.04,
+            sponge_width=float(max(4, self.chord // 2)), far_u=far, body_position=tuple(float(c) for c in center))
+        self.steps = int(round(13.0 * self.chord / 0.08))
+
+    def coefficients(self, load):
+        """(lift, drag) of a load (forceCoefficients, MetalWingValidation.swift:421-436)."""
+        q = 0.5 * 0.08 * 0.08 * (2.0 * self.chord * self.chord)
+        s, c = math.sin(self.alpha), math.cos(self.alpha)
+        return (-load[0] * s + load[1] * c) / q, (load[0] * c + load[1] * s) / q

@@ -6,7 +6,8 @@
 * the restored context continues exactly as the saved one: stored
   populations, loads, masks and body state bit-identical after more steps
   (the reference allows 1e-6 / 1e-5; here the restart is exact, for FP32,
-  FP16 and BF16 storage, a flapping-plate mesh, a translating sphere, a
+  FP16 and BF16 storage, a flapping-plate mesh, an articulated mesh with
+  on-device kinematics, a translating sphere, a
   free-flying body carrying its mesh, a free-flying analytic bird and
   the prescribed flapping wing with its link table);
 * a corrupted populations file is refused with CheckpointError.
@@ -84,6 +85,19 @@ def test_restart_of_flapping_plate_is_exact(storage, tmp_path):
     seq = plate_sequence(12)
     with bh.BirdFlowSimulation(cases.solver_config(p, storage=storage)) as sim:
         sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        check_restart(sim, tmp_path, 5, 4)
+
+
+def test_restart_of_articulated_bird_is_exact(tmp_path):
+    say("case: checkpoint of the articulated synthetic bird (on-device kinematics) 128x64x64, 5 + 4 steps")
+    p = cases.plate_params(shape=(128, 64, 64))
+    mesh = workloads.synthetic_bird_articulated(seed=7, center=(40.0, 32.0, 32.0), body_radii=(10.0, 3.5, 3.5),
+                                                half_span=25.0, root_chord=10.0, tip_chord=3.0, tail=(7.5, 5.0),
+                                                period=300.0, body_segments=(24, 16), wing_segments=(40, 10),
+                                                tail_segments=(12, 6))
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.upload_articulated(mesh)
         sim.init_equilibrium()
         check_restart(sim, tmp_path, 5, 4)
 
