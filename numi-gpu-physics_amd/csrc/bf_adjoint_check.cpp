@@ -4,10 +4,12 @@
 // population-layout index functions, so that tests can check every index
 // they produce against its allocation without a GPU.  Test support; not
 // linked into libbirdflow_hip.so.
+#include <algorithm>
 #include <vector>
 
 #include "bf_adjoint_math.h"
 #include "bf_fluid.h"
+#include "bf_rays.h"
 
 using namespace bf;
 
@@ -173,6 +175,55 @@ int bfa_layout_check(uint32_t nx, uint32_t ny, uint32_t nz, const int* bc, uint6
                 pop(pop_index<STORE_FP32>(g, q, xs, (uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs), 2);
             }
         }
+    }
+    // exact-root pass (bf_rays.h): every puller of a source along q is a
+    // grid cell that pulls from that source, with its table slot inside the
+    // table; and every cell with a source is among that source's pullers
+    {
+        const int bcs[3][2] = {{bc[0], bc[1]}, {bc[2], bc[3]}, {bc[4], bc[5]}};
+        const int dims[3] = {g.nx, g.ny, g.nz};
+        auto cells = [](int n) {
+            std::vector<int> v;
+            for (int c : {0, 1, n / 2, n - 2, n - 1})
+                if (c >= 0 && c < n && std::find(v.begin(), v.end(), c) == v.end()) v.push_back(c);
+            return v;
+        };
+        const std::vector<int> xs = cells(g.nx), ys = cells(g.ny), zs = cells(g.nz);
+        for (int x : xs)
+            for (int y : ys)
+                for (int z : zs)
+                    for (int q = 1; q < 19; ++q) {
+                        const int c[3] = {cx(q), cy(q), cz(q)};
+                        const int at[3] = {x, y, z};
+                        int pl[3][2], k[3];
+                        for (int a = 0; a < 3; ++a) {
+                            k[a] = pullers(bcs[a], at[a], c[a], dims[a], pl[a]);
+                            for (int i = 0; i < k[a]; ++i) {
+                                int src = -1;
+                                if (pl[a][i] < 0 || pl[a][i] >= dims[a] ||
+                                    !ray_source(bcs[a], pl[a][i] - c[a], dims[a], src) || src != at[a]) {
+                                    if (!fail) fail = 7;
+                                }
+                            }
+                        }
+                        for (int iz = 0; iz < k[2]; ++iz)
+                            for (int iy = 0; iy < k[1]; ++iy)
+                                for (int ix = 0; ix < k[0]; ++ix) {
+                                    const size_t li = table_slot(g, pl[0][ix], pl[1][iy], pl[2][iz], q);
+                                    if (li > max_link) max_link = li;
+                                    if (li >= table && !fail) fail = 7;
+                                }
+                        // the cell as a puller: its source lists it
+                        for (int a = 0; a < 3; ++a) {
+                            int src;
+                            if (!ray_source(bcs[a], at[a] - c[a], dims[a], src)) continue;
+                            int back[2];
+                            const int kb = pullers(bcs[a], src, c[a], dims[a], back);
+                            bool found = false;
+                            for (int i = 0; i < kb; ++i) found = found || back[i] == at[a];
+                            if (!found && !fail) fail = 8;
+                        }
+                    }
     }
     // fluid_fast_pair (2-byte storage, nx a multiple of 128): block -> tile
     // pair -> the 128-cell row of each wavefront -> the aligned 4-byte word

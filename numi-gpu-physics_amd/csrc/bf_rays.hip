@@ -48,6 +48,7 @@
 // segment box and the shell box share it.
 #include "bf_device.h"
 #include "bf_launch.h"
+#include "bf_rays.h"
 
 namespace bf {
 
@@ -142,45 +143,8 @@ __device__ float ray_shell(const float F[3], const float D[3], const float A[3],
     return best;
 }
 
-// The source coordinate the fluid step pulls from along one axis: wrapped
-// across a periodic face, clamped at an outlet, none beyond a far field.
-__device__ __forceinline__ bool ray_source(const int bc[2], int s, int n, int& out) {
-    if (s < 0) {
-        if (bc[0] == BC_FARFIELD) return false;
-        s = bc[0] == BC_PERIODIC ? n - 1 : 0;
-    } else if (s >= n) {
-        if (bc[1] == BC_FARFIELD) return false;
-        s = bc[1] == BC_PERIODIC ? 0 : n - 1;
-    }
-    out = s;
-    return true;
-}
-
-// Coordinates x along one axis whose pull along c lands on source coordinate
-// s (at most two: the neighbour s + c, wrapped across a periodic face, and
-// at an outlet the face cell itself, whose pull is clamped onto itself).
-__device__ __forceinline__ int pullers(const int bc[2], int s, int c, int n, int out[2]) {
-    int k = 0;
-    const int cand[4] = {s + c, s + c - n, s + c + n, s};
-    for (int i = 0; i < 4; ++i) {
-        const int x = cand[i];
-        if (x < 0 || x >= n) continue;
-        int src;
-        if (!ray_source(bc, x - c, n, src) || src != s) continue;
-        bool seen = false;
-        for (int j = 0; j < k; ++j) seen = seen || out[j] == x;
-        if (!seen && k < 2) out[k++] = x;
-    }
-    return k;
-}
-
 __device__ __forceinline__ uint32_t lane_rank64(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-__device__ __forceinline__ size_t table_slot(const Grid& g, int x, int y, int z, int q) {
-    const uint32_t lane = (uint32_t)(x & (TILE_X - 1)) + (uint32_t)(y & (TILE_Y - 1)) * (uint32_t)TILE_X;
-    return ((size_t)tile_of(g, x, y, z) * 19u + (size_t)q) * TILE_CELLS + lane;
 }
 
 constexpr uint32_t NO_ENTRY = 0x40000000u;   // 2.0f

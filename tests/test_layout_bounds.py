@@ -2,8 +2,12 @@
 instantiation of the functions every fluid and conversion kernel indexes
 with (pop_index, tile_origin, xcd_remap, shift_axis), walked along each
 kernel's index path, must keep every population index below 19 N (the
-elements of one lattice in any storage) and every wing link-table index
-below tiles x 19 x 256, and the block -> tile remap must be a bijection.
+elements of one lattice in any storage) and every link-table index (the
+wing's and the exact-root pass's) below tiles x 19 x 256, and the block ->
+tile remap must be a bijection.  The exact-root pass's puller enumeration
+(bf_rays.h) is walked both ways: each puller of a source pulls from it
+(periodic wrap, outlet clamps), and each cell is among its source's
+pullers.
 
 Shapes: the four workloads (64^3 periodic shear wave, the 256x128x128
 plate with an inlet / outlet pair, the 512x256x256 bird, the 1024x512x512
