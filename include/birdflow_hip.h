@@ -490,13 +490,33 @@ bf_status bf_inject_population(bf_context* ctx, uint64_t index, float value);
  * discontinuous in theta; link fractions and wall velocities are
  * differentiated.  No counterpart exists in the reference.  Supported: FP32
  * storage, TRT / regularized / RR3, periodic and far-field faces and outlets
- * on the x faces, force mode 6, mesh or static geometry, no free flight.
+ * on the x faces, force mode 6, mesh (frames or articulated) or static
+ * geometry, a fixed body pose (move_mesh placements included), no free
+ * flight.
  * Populations are snapshotted every checkpoint_interval steps in the forward
  * pass and recomputed segment by segment in the reverse sweep (memory:
  * about (segments + interval) x 76 B and interval x 21 B per cell). */
 /* point_tangents: [parameter_count][frame_count][vertex_count][3]; 0 clears.
  * A new bf_mesh_upload clears the tangents. */
 bf_status bf_adjoint_set_tangents(bf_context* ctx, uint32_t parameter_count, const float* point_tangents);
+/* The parameters of an articulated mesh (bf_mesh_upload_articulated) are
+ * its Fourier coefficients: parameter k is an offset added to coefficient
+ * [harmonic][component] (component 0 multiplies sin((h+1) phi), 1
+ * cos((h+1) phi)) of series `channel` (0 stroke, 1 deviation, 2 pitch,
+ * 3 twist) of every joint whose bit is set in joint_mask (bit j: joints[j];
+ * one parameter can move both wings: its derivative is the sum of the
+ * joints' derivatives).  The device differentiates the kinematics of each
+ * step's time
+ * (forward-mode dual numbers), no tangents are uploaded.  Replaces frame
+ * tangents; 0 clears.  A new mesh upload clears the parameters. */
+typedef struct bf_kinematic_parameter {
+    uint32_t joint_mask;
+    uint32_t channel;
+    uint32_t harmonic;
+    uint32_t component;
+} bf_kinematic_parameter;
+bf_status bf_adjoint_set_kinematic_parameters(bf_context* ctx, uint32_t parameter_count,
+                                              const bf_kinematic_parameter* parameters);
 /* Runs `steps` forward steps from the current state (the context advances
  * as with bf_step), then the reverse sweep.  load_weights: [steps][6].
  * objective <- J; gradient[parameter_count] <- dJ/dtheta (FP64 sums);

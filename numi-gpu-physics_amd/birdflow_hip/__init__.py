@@ -9,10 +9,10 @@ from ._abi import (ARITHMETIC_DEFAULT, ARITHMETIC_FUSED, ARITHMETIC_STRICT, BC_F
                    WALL_INTERPOLATED_SDF,
                    LibraryMissing)
 from .simulation import (ArticulatedMesh, BirdFlowError, BirdFlowSimulation, BirdGeometry, BodyParameters, FlappingWing,
-                         Joint, LatticeScaling, SolverConfig, copy_bandwidth, device_info)
+                         Joint, KinematicParameter, LatticeScaling, SolverConfig, copy_bandwidth, device_info)
 
 __all__ = [
-    "ArticulatedMesh", "Joint", "BirdFlowError", "BirdFlowSimulation", "BirdGeometry", "FlappingWing", "BodyParameters", "LatticeScaling", "SolverConfig", "device_info", "copy_bandwidth", "LibraryMissing",
+    "ArticulatedMesh", "Joint", "KinematicParameter", "BirdFlowError", "BirdFlowSimulation", "BirdGeometry", "FlappingWing", "BodyParameters", "LatticeScaling", "SolverConfig", "device_info", "copy_bandwidth", "LibraryMissing",
     "LIBRARY_PATH", "BC_PERIODIC", "BC_FARFIELD", "BC_OUTLET", "COLLISION_TRT", "COLLISION_TRT_SYMMETRIC_LIMITER",
     "COLLISION_REGULARIZED", "COLLISION_RR3", "WALL_HALFWAY", "WALL_INTERPOLATED_SDF", "WALL_INTERPOLATED_HALF", "WALL_INTERPOLATED_LINKS",
     "WALL_INTERPOLATED_RAY",

@@ -48,7 +48,7 @@ EXPORTED_FUNCTIONS = (
     "bf_read_loads", "bf_read_load_history", "bf_link_census", "bf_set_profiling", "bf_read_profile",
     "bf_copy_bandwidth", "bf_body_set", "bf_body_get", "bf_read_body_history", "bf_read_safety",
     "bf_population_minimum", "bf_inject_population",
-    "bf_adjoint_set_tangents", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity", "bf_mesh_freeze_masks",
+    "bf_adjoint_set_tangents", "bf_adjoint_set_kinematic_parameters", "bf_adjoint_gradient", "bf_adjoint_read_sensitivity", "bf_mesh_freeze_masks",
     "bf_set_bird", "bf_set_bird_keyframes", "bf_read_bird_frame", "bf_set_bird_wing_dynamics",
     "bf_read_wing_reaction", "bf_ledger_fluid", "bf_ledger_step_sources",
     "bf_ledger_step_sources_noalloc", "bf_set_flapping_wing", "bf_read_wing_frame", "bf_read_wing_links",
@@ -67,6 +67,13 @@ class Kinematics(ctypes.Structure):
     _fields_ = [("period", ctypes.c_float), ("amplitude_scale", ctypes.c_float), ("harmonics", ctypes.c_uint32),
                 ("joint_count", ctypes.c_uint32), ("center", ctypes.c_float * 3), ("reserved", ctypes.c_float),
                 ("joints", Joint * 8)]
+
+
+class KinematicParameter(ctypes.Structure):
+    """bf_kinematic_parameter: joint bitmask, series (0 stroke .. 3 twist),
+    harmonic, component (0 sin, 1 cos)."""
+    _fields_ = [("joint_mask", ctypes.c_uint32), ("channel", ctypes.c_uint32), ("harmonic", ctypes.c_uint32),
+                ("component", ctypes.c_uint32)]
 
 
 class Config(ctypes.Structure):
@@ -214,6 +221,7 @@ def _declare(l):
         "bf_read_body_history": (st, [vp, vp, u32, ctypes.POINTER(u32)]),
         "bf_read_safety": (st, [vp, vp, ctypes.POINTER(u64), ctypes.POINTER(u32)]),
         "bf_adjoint_set_tangents": (st, [vp, u32, vp]),
+        "bf_adjoint_set_kinematic_parameters": (st, [vp, u32, vp]),
         "bf_adjoint_gradient": (st, [vp, u32, vp, u32, ctypes.POINTER(ctypes.c_double), vp, vp]),
         "bf_adjoint_read_sensitivity": (st, [vp, vp, u64]),
         "bf_mesh_freeze_masks": (st, [vp, vp, u32]),

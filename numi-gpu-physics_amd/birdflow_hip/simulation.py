@@ -275,6 +275,32 @@ class Joint:
     ANGLES = ("stroke", "deviation", "pitch", "twist")
 
 
+@dataclass(frozen=True)
+class KinematicParameter:
+    """One Fourier coefficient of an articulated mesh's kinematics as an
+    adjoint parameter: an offset added to coefficient [harmonic][0 for "sin",
+    1 for "cos"] of series `channel` ("stroke", "deviation", "pitch",
+    "twist") of every joint in `joints` (0-based indices into
+    ArticulatedMesh.joints; one parameter can move both wings, its
+    derivative sums over them)."""
+
+    joints: tuple
+    channel: str
+    harmonic: int = 0
+    term: str = "sin"
+
+    def to_c(self) -> _abi.KinematicParameter:
+        if self.channel not in Joint.ANGLES:
+            raise ValueError(f"channel is one of {Joint.ANGLES}")
+        if self.term not in ("sin", "cos"):
+            raise ValueError("term is 'sin' or 'cos'")
+        mask = 0
+        for j in self.joints:
+            mask |= 1 << int(j)
+        return _abi.KinematicParameter(mask, Joint.ANGLES.index(self.channel), int(self.harmonic),
+                                       0 if self.term == "sin" else 1)
+
+
 @dataclass
 class ArticulatedMesh:
     """An articulated mesh for bf_mesh_upload_articulated: rest positions
@@ -323,6 +349,17 @@ class ArticulatedMesh:
                             **{n: np.asarray(getattr(j, n), np.float32).astype(np.float64).tolist()
                                for n in Joint.ANGLES}}
                            for j in self.joints]}
+
+    def perturbed(self, parameters, offsets) -> "ArticulatedMesh":
+        """A copy with offsets[k] added to KinematicParameter k's coefficient
+        in every joint it names (kept in FP64 until the upload rounds them):
+        the function whose derivative at offsets = 0 the adjoint returns."""
+        joints = [dataclasses.replace(j, **{n: np.array(getattr(j, n), np.float64).reshape(-1, 2)
+                                            for n in Joint.ANGLES}) for j in self.joints]
+        for q, d in zip(parameters, np.asarray(offsets, np.float64)):
+            for j in q.joints:
+                getattr(joints[j], q.channel)[q.harmonic, 0 if q.term == "sin" else 1] += d
+        return dataclasses.replace(self, joints=joints)
 
     @classmethod
     def from_arrays(cls, arrays: dict, kin: dict) -> "ArticulatedMesh":
@@ -487,6 +524,7 @@ class BirdFlowSimulation:
         t0 = float(times[0]) if time_start is None else float(time_start)
         self._check(self._lib.bf_mesh_upload(self._ctx, _ptr(pts), _ptr(times), frames, verts, _ptr(tri), _ptr(parts),
                                              tri.shape[0], float(half_thickness), _ptr(tr), t0))
+        self._parameters = 0          # the upload clears the adjoint's tangents and parameters
         self.geometry_source = {"kind": "mesh", "points": pts.copy(), "frame_times": times.copy(),
                                 "triangles": tri.copy(), "triangle_parts": parts.copy(),
                                 "half_thickness": float(half_thickness), "translation": tr.copy()}
@@ -508,6 +546,7 @@ class BirdFlowSimulation:
                                                          _ptr(tri), _ptr(parts), tri.shape[0],
                                                          float(mesh.half_thickness), ctypes.byref(k),
                                                          float(time_start)))
+        self._parameters = 0
         self.geometry_source = {"kind": "articulated", "rest": rest.copy(), "vertex_joint": joint.copy(),
                                 "span_fraction": span.copy(), "triangles": tri.copy(),
                                 "triangle_parts": parts.copy(), "kinematics": mesh.kinematics_dict()}
@@ -695,6 +734,15 @@ class BirdFlowSimulation:
         k = 0 if t.size == 0 else t.shape[0]
         self._check(self._lib.bf_adjoint_set_tangents(self._ctx, k, _ptr(t) if k else None))
         self._parameters = k
+
+    def adjoint_set_kinematic_parameters(self, parameters):
+        """Fourier coefficients of the uploaded articulated mesh as the
+        adjoint's parameters (a sequence of KinematicParameter, at most 8; the
+        device differentiates the kinematics, no tangents are uploaded)."""
+        ps = list(parameters)
+        arr = (_abi.KinematicParameter * max(len(ps), 1))(*[q.to_c() for q in ps])
+        self._check(self._lib.bf_adjoint_set_kinematic_parameters(self._ctx, len(ps), arr if ps else None))
+        self._parameters = len(ps)
 
     def adjoint_gradient(self, steps: int, load_weights, checkpoint_interval: int = 100):
         """Runs `steps` steps and the reverse sweep for J = sum_s w[s] . L_s

@@ -76,16 +76,21 @@ __device__ __forceinline__ bool leaves_outlet(const StepParams& p, int x) {
 __device__ __forceinline__ size_t outlet_slot(const Grid& g, int face, int q, int y, int z) {
     return ((size_t)face * 19 + (size_t)q) * ((size_t)g.ny * (size_t)g.nz) + (size_t)y + (size_t)g.ny * (size_t)z;
 }
+// Adjoint populations use the forward lattice's row-interleaved layout:
+// lambda_q of cell (x, row = y + ny z) at pop_index(q, x, row).
+__device__ __forceinline__ size_t lam_at(const Grid& g, int q, int x, uint32_t row) {
+    return pop_index<STORE_FP32>(g, q, x, row);
+}
 // lambda_{s-1,q}(source) = v from the (only regular) puller x, or into the
 // outlet face buffer when x's pull along q is clamped; nothing for a
-// far-field source.
+// far-field source.  (xs, rows): the source's x and row.
 template <int q>
 __device__ __forceinline__ void write_pull(const AdjointBuffers& b, const StepParams& p, int x, int y, int z,
-                                           uint32_t src, bool far, float v) {
+                                           int xs, uint32_t rows, bool far, float v) {
     if (far) return;
     const int face = clamped_face<q>(p, x);
     if (face >= 0) b.outlet_extra[outlet_slot(p.g, face, q, y, z)] = v;
-    else b.lam_out[(size_t)q * p.g.n + src] = v;
+    else b.lam_out[lam_at(p.g, q, xs, rows)] = v;
 }
 // (x, q) pulled by nobody: it leaves through a far-field or outlet face
 template <int q>
@@ -96,39 +101,50 @@ __device__ __forceinline__ bool unpulled(const Sources& s, const StepParams& p, 
 // ---------------------------------------------------------------------------
 // Wall-free tiles.
 // ---------------------------------------------------------------------------
+// At least 4 wavefronts per SIMD (128 VGPRs): the RR3 instance spills 22
+// registers to scratch there and still runs 3 % faster than at its natural
+// 140 VGPRs and 3 wavefronts (193 against 200 us at 256 x 128 x 128).
 template <int COLL>
-__global__ __launch_bounds__(256) void adjoint_fast(AdjointBuffers b, StepParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void adjoint_fast(AdjointBuffers b,
+                                                                                           StepParams p) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     if ((b.static_flags[tile] | b.dyn_prev[tile] | b.dyn_cur[tile]) != 0) return;
     int x0, y0, z;
     tile_origin(p.g, tile, x0, y0, z);
     const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
     const int y = y0 + (int)(threadIdx.x / TILE_X);
-    const size_t n = p.g.n;
     const uint32_t row = (uint32_t)y + (uint32_t)p.g.ny * (uint32_t)z;
-    const uint32_t gid = (uint32_t)x + (uint32_t)p.g.nx * row;
     const Sources s = sources(p, x, y, z);
     // the sponge blend f_out = post + (f_far - post) s scales the adjoint by 1 - s
     const float keep = 1.0f - (p.sponge_strength > 0.0f ? sponge_factor(p, x, y, z) : 0.0f);
     float g[19], lp[19], mu[19];
-    uint32_t src[19];
-    bool far[19];
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
         int xs;
         uint32_t rows;
-        source_xr<q>(s, p.g, x, y, z, xs, rows, far[q]);
-        src[q] = (uint32_t)xs + (uint32_t)p.g.nx * rows;
+        bool far;
+        source_xr<q>(s, p.g, x, y, z, xs, rows, far);
         // unconditional load (a far-field source reads this cell)
-        const float v = b.f_prev[far[q] ? pop_index<STORE_FP32>(p.g, q, x, row) : pop_index<STORE_FP32>(p.g, q, xs, rows)];
-        g[q] = far[q] ? p.far_eq[q] : v;
-        lp[q] = b.lam_in[(size_t)q * n + gid] * keep;
+        const float v = b.f_prev[far ? pop_index<STORE_FP32>(p.g, q, x, row) : pop_index<STORE_FP32>(p.g, q, xs, rows)];
+        g[q] = far ? p.far_eq[q] : v;
+        lp[q] = b.lam_in[lam_at(p.g, q, x, row)] * keep;
     });
     collision_adjoint<COLL>(g, p.omega_plus, p.omega_minus, lp, mu);
+    // The sources again, from coordinates the compiler cannot match with the
+    // ones above: otherwise it keeps the 19 load addresses (the store
+    // addresses are the same numbers) live across the collision, 38 VGPRs.
+    int xr = x, yr = y, zr = z;
+    asm volatile("" : "+v"(xr), "+v"(yr), "+v"(zr));
+    const Sources sr = sources(p, xr, yr, zr);
+    const uint32_t rowr = (uint32_t)yr + (uint32_t)p.g.ny * (uint32_t)zr;
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        write_pull<q>(b, p, x, y, z, src[q], far[q], mu[q]);
-        if (unpulled<q>(s, p, x)) b.lam_out[(size_t)q * n + gid] = 0.0f;
+        int xs;
+        uint32_t rows;
+        bool far;
+        source_xr<q>(sr, p.g, xr, yr, zr, xs, rows, far);
+        write_pull<q>(b, p, xr, yr, zr, xs, rows, far, mu[q]);
+        if (unpulled<q>(sr, p, xr)) b.lam_out[lam_at(p.g, q, xr, rowr)] = 0.0f;
     });
 }
 
@@ -143,23 +159,25 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
                                                 const AdjointParams& w, int x, int y, int z, float bx, float by,
                                                 float bz, float* __restrict__ nu_cell) {
     const Grid& g = p.g;
-    const size_t n = g.n;
     const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
     const uint32_t gid = (uint32_t)x + (uint32_t)g.nx * row;
     const bool is_solid = b.solid_cur[gid] != 0;
     const bool was_solid = b.solid_prev[gid] != 0;
     const Sources s = sources(p, x, y, z);
     uint32_t src[19];
+    int sx[19];
+    uint32_t srow[19];
     bool far[19];
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        src[q] = source_of<q>(s, g, x, y, z, far[q]);
-        if (unpulled<q>(s, p, x)) b.lam_out[(size_t)q * n + gid] = 0.0f;
+        source_xr<q>(s, g, x, y, z, sx[q], srow[q], far[q]);
+        src[q] = (uint32_t)sx[q] + (uint32_t)g.nx * srow[q];
+        if (unpulled<q>(s, p, x)) b.lam_out[lam_at(g, q, x, row)] = 0.0f;
     });
     if (is_solid) {
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            write_pull<q>(b, p, x, y, z, src[q], far[q], 0.0f);
+            write_pull<q>(b, p, x, y, z, sx[q], srow[q], far[q], 0.0f);
             nu_cell[q * 256] = 0.0f;
         });
         return;
@@ -168,7 +186,7 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
     float lp[19], gq[19], mu[19];
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        lp[q] = b.lam_in[(size_t)q * n + gid] * (1.0f - sponge);
+        lp[q] = b.lam_in[lam_at(g, q, x, row)] * (1.0f - sponge);
     });
     if (was_solid) {
         // refilled with feq(1, u_wall): no dependence on the pulled populations
@@ -180,7 +198,7 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
         collision_adjoint<COLL>(gq, p.omega_plus, p.omega_minus, lp, mu);
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            write_pull<q>(b, p, x, y, z, src[q], far[q], 0.0f);
+            write_pull<q>(b, p, x, y, z, sx[q], srow[q], far[q], 0.0f);
             nu_cell[q * 256] = mu[q];
         });
         return;
@@ -235,7 +253,7 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
     collision_adjoint<COLL>(gq, p.omega_plus, p.omega_minus, lp, mu);
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        write_pull<q>(b, p, x, y, z, src[q], far[q], link[q] ? 0.0f : mu[q]);
+        write_pull<q>(b, p, x, y, z, sx[q], srow[q], far[q], link[q] ? 0.0f : mu[q]);
         nu_cell[q * 256] = link[q] ? mu[q] + load_dv[q] : 0.0f;
     });
 }
@@ -273,8 +291,8 @@ __global__ __launch_bounds__(256) void adjoint_outlet(AdjointBuffers b, StepPara
     const int ys = shift_axis(y, cy(q), g.ny, p.bc[1][0], p.bc[1][1], far);
     const int zs = shift_axis(z, cz(q), g.nz, p.bc[2][0], p.bc[2][1], far);
     if (far) return;   // the pull was replaced by the far-field equilibrium
-    const uint32_t target = (uint32_t)x + (uint32_t)g.nx * ((uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs);
-    b.lam_out[(size_t)q * g.n + target] += b.outlet_extra[outlet_slot(g, face, q, y, z)];
+    b.lam_out[lam_at(g, q, x, (uint32_t)ys + (uint32_t)g.ny * (uint32_t)zs)] +=
+        b.outlet_extra[outlet_slot(g, face, q, y, z)];
 }
 
 // Sum of 8 FP64 lanes over the workgroup (fixed tree).
@@ -334,7 +352,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
         float l[19];
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            l[q] = b.lam_in[(size_t)q * n + gid];
+            l[q] = b.lam_in[lam_at(g, q, x, row)];
         });
         float gx, gy, gz;
         adj::feq_velocity_adjoint<float>(l, wv.x, wv.y, wv.z, gx, gy, gz);
@@ -343,7 +361,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
             // fresh cover: F += sum_q f_q c_q (the preserved momentum)
             for_q(AllQ{}, [&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                if constexpr (q != 0) b.lam_out[(size_t)q * n + gid] += fs * load_weight<q>(w, arm);
+                if constexpr (q != 0) b.lam_out[lam_at(g, q, x, row)] += fs * load_weight<q>(w, arm);
             });
         }
         return;
@@ -372,7 +390,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
                     if (inside(g, fx, fy, fz)) {
                         const uint32_t nb = flat(g, fx, fy, fz);
                         if (b.solid_prev[nb] == 0 && b.solid_cur[nb] == 0)
-                            b.lam_out[(size_t)q * n + gid] += -fs * load_weight<q>(w, arm);
+                            b.lam_out[lam_at(g, q, x, row)] += -fs * load_weight<q>(w, arm);
                     }
                 }
             });
@@ -412,7 +430,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
         part[q] = b.solid_cur[src_of(qc)];
         fpart[q] = b.solid_cur[fid_of(qc)];
         fown[q] = b.f_prev[pop_index<STORE_FP32>(g, q, x, row)];
-        lam[q] = b.lam_out[(size_t)q * n + gid];
+        lam[q] = b.lam_out[lam_at(g, q, x, row)];
         nuv[q] = nu_cell[q * 256];
     });
     asm volatile("" : BF_ASM19(part), BF_ASM19(fpart), BF_ASM19(fown));
@@ -499,7 +517,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
     link_group(QSeq<16, 17, 18>{}, std::integral_constant<int, 16>{}, 0x70000u);
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        b.lam_out[(size_t)q * n + gid] = lam[q];
+        b.lam_out[lam_at(g, q, x, row)] = lam[q];
     });
     contract(acc, w.K, b.dW, n, gid, gxw[0], gxw[1], gxw[2], gxw[3]);
 }
@@ -526,7 +544,6 @@ __global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, Ste
     const uint32_t count = *b.general_count;
     const float bx = (float)b.body[0], by = (float)b.body[1], bz = (float)b.body[2];
     const Grid& g = p.g;
-    const size_t n = g.n;
     for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
         int x0, y0, z;
         tile_origin(g, b.general_list[i], x0, y0, z);
@@ -549,7 +566,8 @@ __global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, Ste
                     if (inside(g, fx, fy, fz)) {
                         const uint32_t nb = flat(g, fx, fy, fz);
                         if (b.solid_prev[nb] == 0 && b.solid_cur[nb] == 0)
-                            b.lam_out[(size_t)opp(q) * n + nb] += -p.force_scale * load_weight<q>(w, arm);
+                            b.lam_out[lam_at(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)] +=
+                                -p.force_scale * load_weight<q>(w, arm);
                     }
                 }
             });
@@ -575,7 +593,9 @@ __global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, Ste
                 b.f_prev[pop_index<STORE_FP32>(g, opp(q), x, row)],
                 b.f_prev[pop_index<STORE_FP32>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)],
                 b.f_prev[pop_index<STORE_FP32>(g, q, x, row)], wgv, wsv);
-            if (d.uses_farther) b.lam_out[(size_t)opp(q) * n + fid] += nu_cell[q * 256] * d.d_farther;
+            if (d.uses_farther)
+                b.lam_out[lam_at(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)] +=
+                    nu_cell[q * 256] * d.d_farther;
         });
     }
 }
@@ -601,8 +621,47 @@ __global__ __launch_bounds__(256) void reduce_gradient(const double* __restrict_
 // vertex positions and velocities at the query time are the frame
 // interpolation (prepare), so their tangents interpolate the same way.
 // ---------------------------------------------------------------------------
+// Tangents of a body-frame vertex placed by the body pose (place_vertex in
+// bf_geometry.hip: p = x_b + R lp, v = s (v_b + R (w_b x lp) + R lv)): the
+// pose does not depend on theta, so dp = R dlp, dv = s (R (w_b x dlp) + R dlv).
+__device__ __forceinline__ void rotate_tangent(const double* __restrict__ body, const float v[3], float o[3]) {
+    double qd[4] = {body[3], body[4], body[5], body[6]};
+    const double n2 = qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2] + qd[3] * qd[3];
+    if (n2 > 1.0e-20) {
+        const double sc = 1.0 / sqrt(n2);
+        for (int i = 0; i < 4; ++i) qd[i] *= sc;
+    } else {
+        qd[0] = qd[1] = qd[2] = 0.0;
+        qd[3] = 1.0;
+    }
+    const float q[4] = {(float)qd[0], (float)qd[1], (float)qd[2], (float)qd[3]};
+    const float t[3] = {2.0f * (q[1] * v[2] - q[2] * v[1]), 2.0f * (q[2] * v[0] - q[0] * v[2]),
+                        2.0f * (q[0] * v[1] - q[1] * v[0])};
+    const float c[3] = {q[1] * t[2] - q[2] * t[1], q[2] * t[0] - q[0] * t[2], q[0] * t[1] - q[1] * t[0]};
+    for (int i = 0; i < 3; ++i) o[i] = v[i] + q[3] * t[i] + c[i];
+}
+
+__device__ __forceinline__ void place_tangent(const float dlp[3], const float dlv[3], const double* __restrict__ body,
+                                              const MeshParams& m, float4& dp, float4& dv) {
+    if (!m.move_with_body) {
+        dp = make_float4(dlp[0], dlp[1], dlp[2], 0.0f);
+        dv = make_float4(dlv[0] * m.velocity_scale, dlv[1] * m.velocity_scale, dlv[2] * m.velocity_scale, 0.0f);
+        return;
+    }
+    const float w[3] = {(float)body[10], (float)body[11], (float)body[12]};
+    const float wxp[3] = {w[1] * dlp[2] - w[2] * dlp[1], w[2] * dlp[0] - w[0] * dlp[2], w[0] * dlp[1] - w[1] * dlp[0]};
+    float r[3], rw[3], ru[3];
+    rotate_tangent(body, dlp, r);
+    rotate_tangent(body, wxp, rw);
+    rotate_tangent(body, dlv, ru);
+    dp = make_float4(r[0], r[1], r[2], 0.0f);
+    dv = make_float4((rw[0] + ru[0]) * m.velocity_scale, (rw[1] + ru[1]) * m.velocity_scale,
+                     (rw[2] + ru[2]) * m.velocity_scale, 0.0f);
+}
+
 __global__ void tangent_vertices(const float* __restrict__ dpoints, const float* __restrict__ times,
-                                 float4* __restrict__ dpos, float4* __restrict__ dvel, MeshParams m, int K) {
+                                 const double* __restrict__ body, float4* __restrict__ dpos,
+                                 float4* __restrict__ dvel, MeshParams m, int K) {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= m.vertex_count) return;
     const uint32_t first = min(m.first, m.frame_count - 2u);
@@ -612,11 +671,10 @@ __global__ void tangent_vertices(const float* __restrict__ dpoints, const float*
     for (int k = 0; k < K; ++k) {
         const float* a = dpoints + (((size_t)k * m.frame_count + first) * m.vertex_count + v) * 3;
         const float* b = dpoints + (((size_t)k * m.frame_count + second) * m.vertex_count + v) * 3;
-        const float d0 = b[0] - a[0], d1 = b[1] - a[1], d2 = b[2] - a[2];
-        dpos[(size_t)k * m.vertex_count + v] = make_float4(a[0] + blend * d0, a[1] + blend * d1, a[2] + blend * d2, 0.0f);
-        dvel[(size_t)k * m.vertex_count + v] =
-            make_float4(d0 / duration * m.velocity_scale, d1 / duration * m.velocity_scale,
-                        d2 / duration * m.velocity_scale, 0.0f);
+        const float d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const float dlp[3] = {a[0] + blend * d[0], a[1] + blend * d[1], a[2] + blend * d[2]};
+        const float dlv[3] = {d[0] / duration, d[1] / duration, d[2] / duration};
+        place_tangent(dlp, dlv, body, m, dpos[(size_t)k * m.vertex_count + v], dvel[(size_t)k * m.vertex_count + v]);
     }
 }
 
@@ -686,6 +744,98 @@ __device__ void closest_point_dual(const float P[3], const D a[3], const D b[3],
     D p[3];
     for (int i = 0; i < 3; ++i) p[i] = a[i] + ab[i] * v + ac[i] * w;
     return set(p, dconst(1) - v - w, v, w);
+}
+
+// Tangents of an articulated mesh (articulated_prepare in bf_geometry.hip)
+// with respect to Fourier coefficients of its joints, by forward-mode dual
+// numbers through the same kinematics: the angle of series c of a joint is
+//   A_c(t) = s_a sum_h (a_h sin((h+1) phi) + b_h cos((h+1) phi)),  phi = omega (t mod T),
+// its rate A'_c = s_a omega sum_h (h+1) (a_h cos - b_h sin), so
+// dA_c/da_h = s_a sin((h+1) phi), dA'_c/da_h = s_a omega (h+1) cos((h+1) phi),
+// dA_c/db_h = s_a cos((h+1) phi), dA'_c/db_h = -s_a omega (h+1) sin((h+1) phi);
+// the rotations u3 = Ry(pitch + twist span) l, u2 = Rz(side dev) u3,
+// u1 = Rx(side stroke) u2 and the analytic velocity carry the derivative.
+__device__ __forceinline__ D dsin_(D a, float s, float c) { return {s, c * a.d}; }
+__device__ __forceinline__ D dcos_(D a, float s, float c) { return {c, -(s * a.d)}; }
+__device__ __forceinline__ void dsincos(D a, D& sn, D& cs) {
+    float s, c;
+    sincosf(a.v, &s, &c);
+    sn = dsin_(a, s, c);
+    cs = dcos_(a, s, c);
+}
+
+__global__ void articulated_tangents(const float* __restrict__ rest, const uint8_t* __restrict__ vertex_joint,
+                                     const float* __restrict__ span_fraction, const Kinematics* __restrict__ kin,
+                                     KinParams kp, const double* __restrict__ body, float4* __restrict__ dpos,
+                                     float4* __restrict__ dvel, MeshParams m, int K) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m.vertex_count) return;
+    const Kinematics& k = *kin;
+    const uint32_t jv = vertex_joint[v];
+    const bool articulated = jv >= 1 && jv <= k.joint_count;
+    const uint32_t j = articulated ? jv - 1 : 0;
+    const KinJoint& J = k.joints[j];
+    const uint32_t harmonics = min(k.harmonics, 4u);
+    const float t = m.query_time;
+    const float period = k.period;
+    const float tm = t - floorf(t / period) * period;
+    const float omega = 6.28318548f / period;
+    const float phi = omega * tm;
+    float sh[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ch[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t h = 0; h < harmonics; ++h) sincosf((float)(h + 1) * phi, &sh[h], &ch[h]);
+    const float(*series[4])[2] = {J.stroke, J.deviation, J.pitch, J.twist};
+    float ang[4], rate[4];
+    for (int c = 0; c < 4; ++c) {
+        float a = 0.0f, r = 0.0f;
+        for (uint32_t h = 0; h < harmonics; ++h) {
+            a += series[c][h][0] * sh[h] + series[c][h][1] * ch[h];
+            r += (float)(h + 1) * (series[c][h][0] * ch[h] - series[c][h][1] * sh[h]);
+        }
+        ang[c] = a * k.amplitude_scale;
+        rate[c] = r * omega * k.amplitude_scale;
+    }
+    const float* rp = rest + 3 * (size_t)v;
+    const float side = J.side;
+    const float span = span_fraction[v];
+    for (int p = 0; p < K; ++p) {
+        const KinParam P = kp.p[p];
+        float dlp[3] = {0.0f, 0.0f, 0.0f}, dlv[3] = {0.0f, 0.0f, 0.0f};
+        if (articulated && ((P.joint_mask >> j) & 1u) && P.harmonic < harmonics) {
+            const uint32_t h = P.harmonic;
+            const float hs = (float)(h + 1);
+            const float da = k.amplitude_scale * (P.component == 0 ? sh[h] : ch[h]);
+            const float dr = k.amplitude_scale * omega * hs * (P.component == 0 ? ch[h] : -sh[h]);
+            D A[4], R[4];
+            for (int c = 0; c < 4; ++c) {
+                A[c] = {ang[c], c == (int)P.channel ? da : 0.0f};
+                R[c] = {rate[c], c == (int)P.channel ? dr : 0.0f};
+            }
+            const D sd = dconst(side);
+            D sa, ca, sb, cb, sp, cp;
+            dsincos(sd * A[0], sa, ca);
+            dsincos(sd * A[1], sb, cb);
+            const D l[3] = {dconst(rp[0]), dconst(side * rp[1]), dconst(rp[2])};
+            const D a = A[2] + A[3] * dconst(span);
+            const D adot = R[2] + R[3] * dconst(span);
+            dsincos(a, sp, cp);
+            const D u3[3] = {cp * l[0] + sp * l[2], l[1], cp * l[2] - sp * l[0]};
+            const D u2[3] = {cb * u3[0] - sb * u3[1], sb * u3[0] + cb * u3[1], u3[2]};
+            const D u1[3] = {u2[0], ca * u2[1] - sa * u2[2], sa * u2[1] + ca * u2[2]};
+            const D ad = sd * R[0], bd = sd * R[1];
+            const D zero = dconst(0.0f);
+            const D t1[3] = {zero, zero - ad * u1[2], ad * u1[1]};
+            const D w2[3] = {zero - bd * u2[1], bd * u2[0], zero};
+            const D t2[3] = {w2[0], ca * w2[1] - sa * w2[2], sa * w2[1] + ca * w2[2]};
+            const D w3[3] = {adot * u3[2], zero, zero - adot * u3[0]};
+            const D z3[3] = {cb * w3[0] - sb * w3[1], sb * w3[0] + cb * w3[1], w3[2]};
+            const D t3[3] = {z3[0], ca * z3[1] - sa * z3[2], sa * z3[1] + ca * z3[2]};
+            for (int i = 0; i < 3; ++i) {
+                dlp[i] = u1[i].d;
+                dlv[i] = (t1[i] + t2[i] + t3[i]).d;
+            }
+        }
+        place_tangent(dlp, dlv, body, m, dpos[(size_t)p * m.vertex_count + v], dvel[(size_t)p * m.vertex_count + v]);
+    }
 }
 
 // d wall / d theta_k of every keyed cell: wall = (sum_i b_i v_i, (|P - cp| - h) / dx)
@@ -767,11 +917,16 @@ hipError_t launch_adjoint_step(int coll, const AdjointBuffers& b, const StepPara
 }
 
 hipError_t launch_wall_tangent(const uint32_t* list, const uint32_t* count, int blocks, const uint32_t* tri_of_cell,
-                               const uint32_t* tri, const float4* pos, const float4* vel, const float* dpoints,
-                               const float* times, float4* dpos, float4* dvel, float4* dW, const MeshParams& m, int K,
-                               hipStream_t s) {
-    hipLaunchKernelGGL(tangent_vertices, dim3((m.vertex_count + 255) / 256), dim3(256), 0, s, dpoints, times, dpos,
-                       dvel, m, K);
+                               const uint32_t* tri, const float4* pos, const float4* vel, const MeshBuffers& mb,
+                               const float* dpoints, const KinParams& kp, float4* dpos, float4* dvel, float4* dW,
+                               const MeshParams& m, int K, hipStream_t s) {
+    if (K < 1 || K > 8) return hipErrorInvalidValue;
+    const dim3 grid((m.vertex_count + 255) / 256);
+    if (mb.kinematics)
+        hipLaunchKernelGGL(articulated_tangents, grid, dim3(256), 0, s, mb.rest, mb.vertex_joint, mb.span_fraction,
+                           mb.kinematics, kp, mb.body, dpos, dvel, m, K);
+    else
+        hipLaunchKernelGGL(tangent_vertices, grid, dim3(256), 0, s, dpoints, mb.times, mb.body, dpos, dvel, m, K);
     hipLaunchKernelGGL(wall_tangent, dim3(blocks), dim3(256), 0, s, list, count, tri_of_cell, tri, pos, vel, dpos,
                        dvel, dW, m, K);
     return hipGetLastError();

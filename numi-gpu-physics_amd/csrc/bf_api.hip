@@ -40,6 +40,7 @@ static_assert(sizeof(bf_kinematics) == sizeof(Kinematics) && sizeof(bf_joint) ==
                   offsetof(bf_kinematics, joints) == offsetof(Kinematics, joints) && BF_MAX_JOINTS == 8 &&
                   BF_MAX_HARMONICS == 4,
               "kinematics layouts differ");
+static_assert(sizeof(bf_kinematic_parameter) == sizeof(KinParam), "kinematic parameter layouts differ");
 static_assert(BF_COLLISION_RR3 == COLL_RR3 && BF_WALL_INTERPOLATED_HALF == WALL_INTERPOLATED_HALF, "operator ids");
 
 namespace {
@@ -125,6 +126,7 @@ struct bf_context {
     uint8_t* m_joint = nullptr;
     float* m_span = nullptr;
     Kinematics* m_kin = nullptr;
+    uint32_t m_joint_count = 0, m_harmonics = 0;
     uint32_t* m_tri = nullptr;
     uint8_t* m_part = nullptr;
     float4* m_pos = nullptr;
@@ -209,7 +211,10 @@ struct bf_context {
         DeviceAlloc tangent_alloc;
         uint32_t K = 0;
         float* dpoints = nullptr;         // [K][frames][vertices][3]
+        bool kinematic = false;           // K kinematic parameters of an articulated mesh instead
+        KinParams kin{};
         uint32_t interval = 0, segments = 0;
+        uint64_t verts = 0;               // vertex capacity of dpos, dvel, gpos, gvel
         float* checkpoints = nullptr;     // [segments][19N]
         float* seg_f = nullptr;           // [interval][19N]: f_a .. f_{b-1}
         // geometry of steps a .. b of a segment, two sets: the reverse sweep
@@ -1291,9 +1296,10 @@ static bf_status install_mesh(bf_context* c, const MeshSource& src, uint32_t ver
     c->static_cells = nullptr;
     c->static_keys = nullptr;
     c->static_key_count = 0;
-    c->adj.tangent_alloc.release();   // tangents belong to the previous mesh
+    c->adj.tangent_alloc.release();   // tangents and parameters belong to the previous mesh
     c->adj.K = 0;
     c->adj.dpoints = nullptr;
+    c->adj.kinematic = false;
     c->m_points = c->m_times = c->m_rest = c->m_span = nullptr;
     c->m_joint = nullptr;
     c->m_kin = nullptr;
@@ -1347,6 +1353,8 @@ static bf_status install_mesh(bf_context* c, const MeshSource& src, uint32_t ver
     if (articulated) c->host_times.clear();
     else c->host_times.assign(src.frame_times, src.frame_times + src.frame_count);
     c->m_articulated = articulated;
+    c->m_joint_count = articulated ? src.kinematics->joint_count : 0;
+    c->m_harmonics = articulated ? src.kinematics->harmonics : 0;
     c->m_frames = src.frame_count;
     c->m_vertices = vertex_count;
     c->m_triangles = triangle_count;
@@ -2205,8 +2213,6 @@ bf_status adjoint_supported(bf_context* c) {
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers mesh and static geometry");
     if (g.wall_rule == BF_WALL_INTERPOLATED_RAY)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates the signed-distance link fractions");
-    if (c->source == BF_GEOMETRY_MESH && c->m_articulated)
-        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates frame tangents; upload the mesh as frames");
     if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "step once after bf_set_geometry with a different previous mask");
     return BF_OK;
@@ -2214,7 +2220,9 @@ bf_status adjoint_supported(bf_context* c) {
 
 bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     auto& a = c->adj;
-    if (a.interval == interval && a.segments >= segments && a.checkpoints) return BF_OK;
+    // the vertex buffers are sized by the mesh of the allocation: a larger
+    // mesh uploaded since needs new ones
+    if (a.interval == interval && a.segments >= segments && a.checkpoints && a.verts >= c->m_vertices) return BF_OK;
     a.alloc.release();
     const uint64_t n = c->g.n, tiles = c->g.tiles, pop = 19ull * n * 4;
     const uint64_t verts = std::max<uint32_t>(c->m_vertices, 1u);
@@ -2289,6 +2297,7 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     }
     a.interval = interval;
     a.segments = segments;
+    a.verts = verts;
     return BF_OK;
 }
 
@@ -2328,12 +2337,7 @@ bf_status adjoint_geometry(bf_context* c, int64_t s, uint32_t k, int set, hipStr
     uint8_t* touched_new = a.slot_tiles[set] + ((uint64_t)a.gen[set] * slots + k) * tiles;
     uint8_t* touched_old = a.slot_tiles[set] + ((uint64_t)(1 - a.gen[set]) * slots + k) * tiles;
     hipError_t e = hipSuccess;
-    MeshBuffers b{};
-    b.body = c->body;
-    b.points = c->m_points;
-    b.times = c->m_times;
-    b.tri = c->m_tri;
-    b.tri_part = c->m_part;
+    MeshBuffers b = mesh_buffers(c);   // frames or articulated arrays
     b.pos = a.gpos;
     b.vel = a.gvel;
     b.keys = c->keys;
@@ -2396,6 +2400,7 @@ bf_status bf_adjoint_set_tangents(bf_context* c, uint32_t parameter_count, const
     c->adj.tangent_alloc.release();
     c->adj.dpoints = nullptr;
     c->adj.K = 0;
+    c->adj.kinematic = false;
     if (parameter_count == 0) return BF_OK;
     const uint64_t count = 3ull * parameter_count * c->m_frames * c->m_vertices;
     for (uint64_t i = 0; i < count; ++i)
@@ -2405,6 +2410,31 @@ bf_status bf_adjoint_set_tangents(bf_context* c, uint32_t parameter_count, const
     if (e == hipSuccess) e = hipMemcpy(c->adj.dpoints, point_tangents, 4 * count, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(c, e, "tangent upload", false);
     c->adj.K = parameter_count;
+    return BF_OK;
+}
+
+bf_status bf_adjoint_set_kinematic_parameters(bf_context* c, uint32_t parameter_count,
+                                              const bf_kinematic_parameter* parameters) {
+    BF_ENTER(c);
+    if (parameter_count > 8) return fail(c, BF_ERROR_INVALID_ARGUMENT, "at most 8 parameters");
+    if (parameter_count > 0 && !parameters) return fail(c, BF_ERROR_INVALID_ARGUMENT, "parameters are NULL");
+    if (parameter_count > 0 && (c->source != BF_GEOMETRY_MESH || !c->m_articulated))
+        return fail(c, BF_ERROR_INVALID_ARGUMENT, "kinematic parameters need an articulated mesh");
+    for (uint32_t k = 0; k < parameter_count; ++k) {
+        const bf_kinematic_parameter& q = parameters[k];
+        if (q.joint_mask == 0 || (c->m_joint_count < 32 && (q.joint_mask >> c->m_joint_count) != 0))
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "joint_mask names no joint or a joint the mesh lacks");
+        if (q.channel > 3 || q.component > 1 || q.harmonic >= c->m_harmonics)
+            return fail(c, BF_ERROR_INVALID_ARGUMENT, "channel < 4, component < 2 and harmonic < harmonics");
+    }
+    c->adj.tangent_alloc.release();
+    c->adj.dpoints = nullptr;
+    c->adj.K = parameter_count;
+    c->adj.kinematic = parameter_count > 0;
+    c->adj.kin = KinParams{};
+    for (uint32_t k = 0; k < parameter_count; ++k)
+        c->adj.kin.p[k] = {parameters[k].joint_mask, parameters[k].channel, parameters[k].harmonic,
+                           parameters[k].component};
     return BF_OK;
 }
 
@@ -2507,18 +2537,13 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
                 // once the step that last read it (s + 2) is done.
                 if (s == sb_) e = hipStreamWaitEvent(a.sc, a.ev_geo[set], 0);
                 if (e == hipSuccess && tangent_steps >= 2) e = hipStreamWaitEvent(a.sc, a.ev_used[j], 0);
-                MeshBuffers mb{};
-                mb.body = c->body;
-                mb.points = c->m_points;
-                mb.times = c->m_times;
-                mb.pos = c->m_pos;
-                mb.vel = c->m_vel;
+                const MeshBuffers mb = mesh_buffers(c);
                 const MeshParams mp = mesh_params(c, time_of(c, s), false);
                 if (e == hipSuccess) e = launch_mesh_prepare(mb, mp, a.sc);
                 if (e == hipSuccess)
                     e = launch_wall_tangent(a.slot_list[set] + (uint64_t)k * c->g.tiles, a.slot_count[set] + k,
-                                            c->general_blocks, gc.tri, c->m_tri, c->m_pos, c->m_vel, a.dpoints,
-                                            c->m_times, a.dpos, a.dvel, a.dW[j], mp, (int)a.K, a.sc);
+                                            c->general_blocks, gc.tri, c->m_tri, c->m_pos, c->m_vel, mb, a.dpoints,
+                                            a.kin, a.dpos, a.dvel, a.dW[j], mp, (int)a.K, a.sc);
                 if (e == hipSuccess) e = hipEventRecord(a.ev_tan[j], a.sc);
                 if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, a.ev_tan[j], 0);
                 if (e != hipSuccess) return hip_fail(c, e, "wall tangents");
@@ -2606,7 +2631,13 @@ bf_status bf_adjoint_read_sensitivity(bf_context* c, float* out, uint64_t count)
     BF_ENTER(c);
     if (!out || count != 19ull * c->g.n) return fail(c, BF_ERROR_INVALID_ARGUMENT, "count must be 19 * cells");
     if (!c->adj.have_sensitivity) return fail(c, BF_ERROR_INVALID_ARGUMENT, "no adjoint sweep has run");
-    hipError_t e = hipMemcpy(out, c->adj.lam[c->adj.lam_cur], 4 * count, hipMemcpyDeviceToHost);
+    // lambda lives in the row-interleaved layout of an FP32 lattice: unpack
+    // it to the host's direction-major order through the staging buffer
+    if (bf_status st = ensure_staging(c)) return st;
+    hipError_t e = launch_convert_populations(STORE_FP32, false, c->adj.lam[c->adj.lam_cur], c->staging, count, c->g,
+                                              c->sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, c->staging, 4 * count, hipMemcpyDeviceToHost, c->sb);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->sb);
     if (e != hipSuccess) return hip_fail(c, e, "read sensitivity");
     return BF_OK;
 }

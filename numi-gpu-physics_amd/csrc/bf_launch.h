@@ -238,10 +238,23 @@ struct AdjointParams {
 };
 hipError_t launch_adjoint_step(int coll, const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
                                int general_blocks, hipStream_t s);
+// Kinematic parameters of an articulated mesh (layout of
+// bf_kinematic_parameter): coefficient [harmonic][component] of series
+// `channel` (stroke, deviation, pitch, twist) of every joint in joint_mask.
+struct KinParam {
+    uint32_t joint_mask, channel, harmonic, component;
+};
+struct KinParams {
+    KinParam p[8];
+};
+// Vertex tangents of step time m.query_time (from the frame tangents
+// dpoints, or for an articulated mesh (mb.kinematics) from kp), placed by
+// the body pose when m.move_with_body, then the wall-field tangents dW of
+// the listed tiles' keyed cells.
 hipError_t launch_wall_tangent(const uint32_t* list, const uint32_t* count, int blocks, const uint32_t* tri_of_cell,
-                               const uint32_t* tri, const float4* pos, const float4* vel, const float* dpoints,
-                               const float* times, float4* dpos, float4* dvel, float4* dW, const MeshParams& m, int K,
-                               hipStream_t s);
+                               const uint32_t* tri, const float4* pos, const float4* vel, const MeshBuffers& mb,
+                               const float* dpoints, const KinParams& kp, float4* dpos, float4* dvel, float4* dW,
+                               const MeshParams& m, int K, hipStream_t s);
 
 // Rigid body (FP64): integration after a step's loads and the safety monitor.
 struct BodyParams {

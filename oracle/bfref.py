@@ -142,6 +142,57 @@ def articulated_vertices(rest, vertex_joint, span_fraction, k: Kinematics, t, ve
     return pos, vel
 
 
+def articulated_vertices_f64(rest, vertex_joint, span_fraction, kin: dict, t, velocity_scale=1.0):
+    """bfo_articulated_vertices (bfref_kin.inc) in FP64 with FP64
+    coefficients: positions and velocities [V][3] at time t, the reference
+    for finite differences with respect to the Fourier coefficients.  kin:
+    period, amplitude_scale, harmonics, center and joints (dicts with side,
+    root and stroke / deviation / pitch / twist [harmonic][sin, cos])."""
+    rest = np.asarray(rest, np.float64).reshape(-1, 3)
+    vj = np.asarray(vertex_joint).astype(np.int64)
+    sf = np.asarray(span_fraction, np.float64)
+    period = float(kin["period"])
+    scale = float(kin["amplitude_scale"])
+    H = int(kin["harmonics"])
+    omega = 2.0 * np.pi / period
+    phi = omega * (t - np.floor(t / period) * period)
+    h = np.arange(1, H + 1, dtype=np.float64)
+    sh, ch = np.sin(h * phi), np.cos(h * phi)
+    center = np.asarray(kin["center"], np.float64)
+    pos = center + rest
+    vel = np.zeros_like(rest)
+    for j, J in enumerate(kin["joints"]):
+        sel = vj == j + 1
+        if not sel.any():
+            continue
+        ang, rate = [], []
+        for name in ("stroke", "deviation", "pitch", "twist"):
+            c = np.asarray(J[name], np.float64).reshape(-1, 2)[:H]
+            ang.append(scale * float(c[:, 0] @ sh + c[:, 1] @ ch))
+            rate.append(scale * omega * float((h * (c[:, 0] * ch - c[:, 1] * sh)).sum()))
+        s = float(J["side"])
+        sa, ca = np.sin(s * ang[0]), np.cos(s * ang[0])
+        sb, cb = np.sin(s * ang[1]), np.cos(s * ang[1])
+        l = rest[sel] * np.array([1.0, s, 1.0])
+        a = ang[2] + ang[3] * sf[sel]
+        adot = rate[2] + rate[3] * sf[sel]
+        sp, cp = np.sin(a), np.cos(a)
+        u3 = np.stack([cp * l[:, 0] + sp * l[:, 2], l[:, 1], -sp * l[:, 0] + cp * l[:, 2]], 1)
+        u2 = np.stack([cb * u3[:, 0] - sb * u3[:, 1], sb * u3[:, 0] + cb * u3[:, 1], u3[:, 2]], 1)
+        u1 = np.stack([u2[:, 0], ca * u2[:, 1] - sa * u2[:, 2], sa * u2[:, 1] + ca * u2[:, 2]], 1)
+        root = np.asarray(J["root"], np.float64) * np.array([1.0, s, 1.0])
+        pos[sel] = center + root + u1
+        ad, bd = s * rate[0], s * rate[1]
+        t1 = np.stack([np.zeros(len(u1)), -ad * u1[:, 2], ad * u1[:, 1]], 1)
+        w2 = np.stack([-bd * u2[:, 1], bd * u2[:, 0], np.zeros(len(u2))], 1)
+        t2 = np.stack([w2[:, 0], ca * w2[:, 1] - sa * w2[:, 2], sa * w2[:, 1] + ca * w2[:, 2]], 1)
+        w3 = np.stack([adot * u3[:, 2], np.zeros(len(u3)), -adot * u3[:, 0]], 1)
+        z3 = np.stack([cb * w3[:, 0] - sb * w3[:, 1], sb * w3[:, 0] + cb * w3[:, 1], w3[:, 2]], 1)
+        t3 = np.stack([z3[:, 0], ca * z3[:, 1] - sa * z3[:, 2], sa * z3[:, 1] + ca * z3[:, 2]], 1)
+        vel[sel] = t1 + t2 + t3
+    return pos, vel * velocity_scale
+
+
 WING_FRAME_FIELDS = ("root", "chord", "span", "normal", "angular", "state")
 
 

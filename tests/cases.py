@@ -397,3 +397,21 @@ class FixedWing:
         q = 0.5 * 0.08 * 0.08 * (2.0 * self.chord * self.chord)
         s, c = math.sin(self.alpha), math.cos(self.alpha)
         return (-load[0] * s + load[1] * c) / q, (load[0] * c + load[1] * s) / q
+
+
+def kinematics_f64(mesh):
+    """An ArticulatedMesh's kinematics as the dict bfref.articulated_vertices_f64
+    takes, coefficients kept in FP64."""
+    return {"period": float(mesh.period), "amplitude_scale": float(mesh.amplitude_scale),
+            "harmonics": int(mesh.harmonics), "center": [float(v) for v in mesh.center],
+            "joints": [{"side": float(j.side), "root": [float(v) for v in j.root],
+                        **{n: np.asarray(getattr(j, n), np.float64) for n in ("stroke", "deviation", "pitch", "twist")}}
+                       for j in mesh.joints]}
+
+
+def oracle_kinematics(mesh):
+    """An ArticulatedMesh's kinematics for the FP32 oracle (bfref.Kinematics)."""
+    return bfref.make_kinematics(mesh.period, mesh.amplitude_scale, mesh.center,
+                                 [dict(side=j.side, root=j.root, stroke=j.stroke, deviation=j.deviation,
+                                       pitch=j.pitch, twist=j.twist) for j in mesh.joints],
+                                 harmonics=mesh.harmonics)

@@ -15,7 +15,14 @@
 * the differentiation workload's inlet / outlet plate on its 64 x 32 x 32
   twin over one full flapping period against the FP64 oracle's central
   differences, bar 1e-3.
-* the torch.autograd wrapper returns the adjoint gradient.
+* an articulated bird (on-device Fourier kinematics; 96 x 48 x 48, RR3,
+  inlet / outlet) with 6 of its Fourier coefficients as parameters, placed
+  in the world and by a rotated, spinning body pose (move_mesh): dJ/dtheta
+  against central differences of the FP64 oracle objective, the vertices
+  from the FP64 kinematics (articulated_vertices_f64), on the frozen link
+  sets.  Bar: 1e-3 relative.
+* the torch.autograd wrapper returns the adjoint gradient (frame tangents
+  and kinematic parameters).
 """
 import sys
 
@@ -318,3 +325,147 @@ def test_torch_autograd_returns_adjoint_gradient():
     say(f"case: torch autograd, J {J.item():.6e}, grad {theta.grad.numpy()}")
     assert np.allclose(theta.grad.numpy(), fn.last["gradient"], rtol=0, atol=0)
     assert J.item() == fn.last["objective"]
+
+
+def test_torch_autograd_over_fourier_coefficients():
+    torch = pytest.importorskip("torch")
+    from birdflow_hip.autograd import KinematicsFunctional, load_functional
+    steps = 8
+    p, mesh = articulated_bird_case(None)
+    weights = np.zeros((steps, 6))
+    weights[:, 2] = 1.0 / steps
+    fn = KinematicsFunctional(cases.solver_config(p), mesh, BIRD_PARAMETERS, weights, checkpoint_interval=4)
+    theta = torch.zeros(len(BIRD_PARAMETERS), dtype=torch.float64, requires_grad=True)
+    J = load_functional(theta, fn)
+    (2.0 * J).backward()
+    say(f"case: torch autograd over Fourier coefficients, J {J.item():.6e}, grad {theta.grad.numpy()}")
+    assert np.array_equal(theta.grad.numpy(), 2.0 * fn.last["gradient"])
+    assert np.abs(fn.last["gradient"]).max() > 0.0
+
+
+BIRD_PARAMETERS = [bh.KinematicParameter((0, 1), "stroke", 0, "sin"), bh.KinematicParameter((0, 1), "stroke", 0, "cos"),
+                   bh.KinematicParameter((0, 1), "pitch", 0, "sin"), bh.KinematicParameter((0,), "pitch", 0, "cos"),
+                   bh.KinematicParameter((0, 1), "twist", 1, "sin"), bh.KinematicParameter((1,), "deviation", 0, "cos")]
+
+
+def articulated_bird_case(pose):
+    """The synthetic bird at 96 / 512 scale (1,168 triangles), Mach-limited
+    kinematics, RR3 at tau+ 0.6, far-field inlet and outlet on x, periodic
+    y / z.  pose: None (mesh centred in the world) or a body state (mesh in
+    the body frame, placed by it)."""
+    shape = (96, 48, 48)
+    s = shape[0] / 512.0
+    center = (30.0, 24.0, 24.0)
+    bird = workloads.SyntheticBird(seed=7, center=(0.0, 0.0, 0.0) if pose is not None else center,
+                                   body_radii=(40 * s, 14 * s, 14 * s), half_span=100 * s, root_chord=40 * s,
+                                   tip_chord=12 * s, tail=(30 * s, 20 * s), period=1200 * s,
+                                   body_segments=(24, 12), wing_segments=(24, 6), tail_segments=(8, 4))
+    bird.limit_mach(0.04, 0.15)
+    tau = 0.6
+    p = bfref.make_params(shape, omega_plus=1.0 / tau, omega_minus=1.0 / (0.5 + (3.0 / 16.0) / (tau - 0.5)),
+                          bc=((bfref.BC_FARFIELD, bfref.BC_OUTLET), (bfref.BC_PERIODIC, bfref.BC_PERIODIC),
+                              (bfref.BC_PERIODIC, bfref.BC_PERIODIC)),
+                          collision=bfref.COLLISION_RR3, wall_rule=bfref.WALL_INTERPOLATED_SDF, force_mode=6,
+                          accumulate_loads=True, far_u=(0.04, 0.0, 0.0), body_position=center)
+    return p, bird.articulated(half_thickness=1.0)
+
+
+def place_f64(pose, pos, vel):
+    """The body pose placing body-frame vertices (place_vertex) in FP64."""
+    if pose is None:
+        return pos, vel
+    x, y, z, w = np.asarray(pose[3:7], np.float64) / np.linalg.norm(pose[3:7])
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    om = np.asarray(pose[10:13], np.float64)
+    return pose[:3] + pos @ R.T, pose[7:10] + np.cross(om, pos) @ R.T + vel @ R.T
+
+
+@pytest.mark.parametrize("posed", [False, True], ids=["world", "body pose"])
+def test_adjoint_of_articulated_bird_matches_fp64_reference(posed):
+    steps, interval = 40, 20
+    pose = None
+    if posed:
+        # 12 degrees about (1, 2, 3), spinning slowly; free flight off, so the
+        # pose stays put and only places the mesh
+        ax = np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0)
+        half = np.radians(12.0) / 2
+        pose = np.array([30.0, 24.0, 24.0, *(ax * np.sin(half)), np.cos(half), 0.001, 0.0, 0.0, 0.002, -0.001, 0.0015])
+    p, mesh = articulated_bird_case(pose)
+    weights = np.zeros((steps, 6))
+    weights[steps // 2:, 2] = 1.0 / (steps - steps // 2)
+    say(f"case: adjoint of mean lift, articulated bird {p.nx}x{p.ny}x{p.nz}, {len(mesh.triangles)} triangles, RR3, "
+        f"{steps} steps, 6 Fourier coefficients, {'body pose' if posed else 'world'}")
+    cfg = cases.solver_config(p)
+    with bh.BirdFlowSimulation(cfg) as sim:
+        if posed:
+            sim.set_body(pose, bh.BodyParameters(mass=1e6, inertia=(1e7, 1e7, 1e7), free_flight=False,
+                                                 move_mesh=True))
+        sim.upload_articulated(mesh, 0.0)
+        sim.adjoint_set_kinematic_parameters(BIRD_PARAMETERS)
+        sim.init_equilibrium()
+        f0 = sim.read_populations()
+        J, grad, timings = sim.adjoint_gradient(steps, weights, checkpoint_interval=interval)
+    # frozen link sets from the FP32 oracle geometry (the GPU's, bit for bit)
+    k32 = cases.oracle_kinematics(mesh)
+    body = None if pose is None else bfref.make_body(position=pose[:3], orientation=pose[3:7],
+                                                     linear_velocity=pose[7:10], angular_velocity_body=pose[10:13])
+    masks, tris = [], []
+    for s in range(steps + 1):
+        pos, vel = bfref.articulated_vertices(mesh.rest, mesh.vertex_joint, mesh.span_fraction, k32, float(s))
+        if body is not None:
+            pos, vel = bfref.place_vertices(body, pos, vel)
+        mask, _, keys = bfref.raster_resolve(p, pos, vel, mesh.triangles, mesh.parts, mesh.half_thickness, 64)
+        masks.append(mask)
+        tris.append(np.where(keys != np.iinfo(np.uint64).max, (keys & 0xFFFFFFFF).astype(np.int64), -1))
+    theta0 = np.zeros(len(BIRD_PARAMETERS))       # offsets of the coefficients
+
+    def J64(theta):
+        k64 = cases.kinematics_f64(mesh.perturbed(BIRD_PARAMETERS, theta))
+
+        def geo(s):
+            pos, vel = bfref.articulated_vertices_f64(mesh.rest, mesh.vertex_joint, mesh.span_fraction, k64, float(s))
+            pos, vel = place_f64(pose, pos, vel)
+            return masks[s - 1], masks[s], bfref.wall_field_f64(p, pos, vel, mesh.triangles, tris[s],
+                                                                 mesh.half_thickness)
+        return objective_oracle(p, f0, steps, weights, geo)
+
+    J0 = J64(theta0)
+    ref = np.zeros(len(theta0))
+    for k in range(len(theta0)):
+        h = 1e-5
+        e = np.zeros(len(theta0))
+        e[k] = h
+        ref[k] = (J64(theta0 + e) - J64(theta0 - e)) / (2 * h)
+    err = np.abs(grad - ref).max() / np.abs(ref).max()
+    say(f"  J gpu {J:.9e} vs FP64 reference {J0:.9e}")
+    for k, q in enumerate(BIRD_PARAMETERS):
+        say(f"  {q.channel:9s} h{q.harmonic} {q.term} joints {q.joints}: adjoint {grad[k]: .6e}  "
+            f"FP64 reference {ref[k]: .6e}")
+    say(f"  max |adjoint - reference| / max |reference| = {err:.2e}; forward {timings['forward_ms']:.1f} ms, "
+        f"reverse {timings['reverse_ms']:.1f} ms")
+    assert abs(J - J0) <= 1e-4 * abs(J0)
+    assert err <= 1e-3
+
+
+def test_kinematic_parameter_refusals():
+    p, mesh = articulated_bird_case(None)
+    seq, _, _ = plate_frames(THETA0, 4)
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        with pytest.raises(bh.BirdFlowError):          # frames, not an articulated mesh
+            sim.adjoint_set_kinematic_parameters(BIRD_PARAMETERS[:1])
+        sim.upload_articulated(mesh, 0.0)
+        for bad in (bh.KinematicParameter((2,), "stroke", 0, "sin"),      # the bird has joints 0 and 1
+                    bh.KinematicParameter((), "stroke", 0, "sin"),
+                    bh.KinematicParameter((0,), "stroke", 3, "sin")):     # 3 harmonics
+            with pytest.raises(bh.BirdFlowError):
+                sim.adjoint_set_kinematic_parameters([bad])
+        with pytest.raises(bh.BirdFlowError):          # frame tangents need frames
+            sim.adjoint_set_tangents(np.zeros((1, 2, len(mesh.rest), 3), np.float32))
+        sim.adjoint_set_kinematic_parameters(BIRD_PARAMETERS)
+        sim.upload_articulated(mesh, 0.0)              # a new upload clears them
+        sim.init_equilibrium()
+        J, grad, _ = sim.adjoint_gradient(4, np.ones((4, 6)), checkpoint_interval=2)
+        assert grad.size == 0 and np.isfinite(J)

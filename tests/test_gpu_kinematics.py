@@ -10,7 +10,8 @@
   not 2,400 frames;
 * (tests/test_gpu_checkpoint.py: an archive of an articulated mesh
   restores and continues bit for bit);
-* the adjoint refuses an articulated mesh (it differentiates frame tangents).
+* the adjoint's forward pass over an articulated mesh is the ordinary step
+  (its gradients against FP64 differences: tests/test_gpu_adjoint.py).
 """
 import sys
 
@@ -118,14 +119,30 @@ def test_articulated_bird_steps_match_oracle():
     assert mesh_bytes <= 8 * p.nx * p.ny * p.nz + 64 * V + 20 * len(mesh.triangles) + (1 << 20)
 
 
-def test_adjoint_refuses_articulated_mesh():
+def test_adjoint_forward_pass_over_articulated_mesh_is_the_ordinary_step():
+    # the adjoint's forward pass steps the articulated mesh as bf_step does:
+    # its objective equals the weighted loads of a plain run, bit for bit
+    # (its gradients: tests/test_gpu_adjoint.py)
     shape = (64, 32, 32)
+    steps = 6
     p, center = bird_params(shape)
     mesh = workloads.synthetic_bird_articulated(seed=7, center=center, body_radii=(5.0, 2.0, 2.0), half_span=12.0,
                                                 root_chord=5.0, tip_chord=2.0, tail=(4.0, 3.0), period=100.0,
                                                 body_segments=(12, 8), wing_segments=(12, 4), tail_segments=(6, 3))
+    weights = np.random.default_rng(3).uniform(-1.0, 1.0, (steps, 6))
     with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
         sim.upload_articulated(mesh)
         sim.init_equilibrium()
-        with pytest.raises(bh.BirdFlowError):
-            sim.adjoint_gradient(4, np.zeros((4, 6)), checkpoint_interval=2)
+        sim.advance(steps)
+        loads = sim.load_history()
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.upload_articulated(mesh)
+        sim.adjoint_set_kinematic_parameters([bh.KinematicParameter((0, 1), "stroke", 0, "sin")])
+        sim.init_equilibrium()
+        J, grad, _ = sim.adjoint_gradient(steps, weights, checkpoint_interval=4)
+    want = 0.0                      # the library's summation order
+    for i in range(steps):
+        for k in range(6):
+            want += float(weights[i, k]) * float(loads[i, k])
+    say(f"case: adjoint forward pass over an articulated bird {shape}: J {J!r}, plain run {want!r}, grad {grad}")
+    assert J == want and grad.size == 1 and np.isfinite(grad).all()

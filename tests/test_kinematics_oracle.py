@@ -10,7 +10,13 @@ its own formula:
   circle (0, s y cos(s a), s y sin(s a)); a pure pitch turns a chord-axis
   vertex in the x-z plane; twist grows linearly with the span fraction;
 * the rigid vertices sit at centre + rest exactly, and t and t + period give
-  the same pose.
+  the same pose;
+* the FP64 form (articulated_vertices_f64, the finite-difference reference
+  of the articulated adjoint) against the same FP64 frames to 5e-6 cells
+  (the FP32 rounding of the mesh's rest positions), its
+  velocities against their central differences, and its response to a
+  perturbed Fourier coefficient (ArticulatedMesh.perturbed) against the
+  frames rebuilt with the same coefficient perturbed.
 """
 import math
 
@@ -19,7 +25,9 @@ import pytest
 
 import bfref
 from birdflow_hip import workloads
-from birdflow_hip.simulation import ArticulatedMesh, Joint
+from birdflow_hip.simulation import ArticulatedMesh, Joint, KinematicParameter
+
+import cases
 
 
 def oracle_kinematics(mesh: ArticulatedMesh):
@@ -131,3 +139,61 @@ def test_rigid_vertices_and_periodicity():
     left, right = pm[mesh.vertex_joint == 1], pm[mesh.vertex_joint == 2]
     assert np.abs(left[:, [0, 2]] - right[:, [0, 2]]).max() <= 1e-4
     assert np.abs((left[:, 1] - center[1]) + (right[:, 1] - center[1])).max() <= 1e-4
+
+
+def test_fp64_kinematics_match_the_fp64_frames():
+    bird = small_bird()
+    mesh = bird.articulated()
+    k64 = cases.kinematics_f64(mesh)
+    worst = 0.0
+    for t in (0.0, 17.0, 123.0, 299.0, 451.5):
+        pos, vel = bfref.articulated_vertices_f64(mesh.rest, mesh.vertex_joint, mesh.span_fraction, k64, t)
+        worst = max(worst, float(np.abs(pos - bird.frame(t)).max()))
+        eps = 1e-3
+        fd = (bird.frame(t + eps) - bird.frame(t - eps)) / (2 * eps)
+        assert np.abs(vel - fd).max() <= 1e-6 * np.abs(fd).max()
+        # and the FP32 oracle within its rounding
+        p32, v32 = bfref.articulated_vertices(mesh.rest, mesh.vertex_joint, mesh.span_fraction,
+                                              oracle_kinematics(mesh), t)
+        assert np.abs(p32 - pos).max() <= 2e-4
+        assert np.abs(v32 - vel).max() <= 2e-3 * np.abs(vel).max()
+    # the mesh holds its rest positions and span fractions in FP32 (up to
+    # 40 cells: 2.4e-6 per coordinate), the frames in FP64
+    print(f"max |FP64 kinematics - FP64 frames| = {worst:.2e} cells")
+    assert worst <= 5e-6
+
+
+@pytest.mark.parametrize("channel,harmonic,term,joints", [("stroke", 0, "sin", (0, 1)), ("pitch", 1, "cos", (1,)),
+                                                          ("twist", 2, "sin", (0,)), ("deviation", 0, "cos", (0, 1))])
+def test_perturbed_coefficient_moves_the_mesh_like_the_frames(channel, harmonic, term, joints):
+    # d(vertex)/d(coefficient) by central differences of the FP64 kinematics
+    # (the adjoint's reference) and of the FP64 frames with the bird's own
+    # coefficient perturbed in every named wing
+    bird = small_bird()
+    mesh = bird.articulated()
+    q = KinematicParameter(joints, channel, harmonic, term)
+    h = 1e-6
+    t = 77.0
+    series = getattr(bird.kin, channel)
+    comp = 0 if term == "sin" else 1
+    d_mesh, d_frame = [], []
+    for sign in (1.0, -1.0):
+        m = mesh.perturbed([q], [sign * h])
+        pos, _ = bfref.articulated_vertices_f64(m.rest, m.vertex_joint, m.span_fraction, cases.kinematics_f64(m), t)
+        d_mesh.append(pos)
+        saved = series.copy()
+        for w in joints:
+            series[w, harmonic, comp] += sign * h
+        d_frame.append(bird.frame(t))
+        series[...] = saved
+    dm = (d_mesh[0] - d_mesh[1]) / (2 * h)
+    df = (d_frame[0] - d_frame[1]) / (2 * h)
+    moved = np.abs(df).max()
+    print(f"{channel} h{harmonic} {term} joints {joints}: max |d vertex / d c| = {moved:.3e}, "
+          f"difference {np.abs(dm - df).max():.2e}")
+    assert moved > 0.1
+    assert np.abs(dm - df).max() <= 1e-6 * moved
+    # the joints not named stay put
+    other = [j for j in (0, 1) if j not in joints]
+    for j in other:
+        assert not dm[mesh.vertex_joint == j + 1].any()
