@@ -489,13 +489,17 @@ bf_status bf_inject_population(bf_context* ctx, uint64_t index, float value);
  * geometry.  Each step's link set (masks) is held fixed, as it is
  * discontinuous in theta; link fractions and wall velocities are
  * differentiated.  No counterpart exists in the reference.  Supported: FP32
- * storage, TRT / regularized / RR3, periodic and far-field faces and outlets
- * on the x faces, force mode 6, mesh (frames or articulated) or static
- * geometry, a fixed body pose (move_mesh placements included), no free
- * flight.
+ * and FP16 / BF16 offset storage (populations decoded on load, the storage
+ * rounding taken as the identity; dJ/df stays FP32), every collision
+ * (the limiters differentiated through the direction that sets their
+ * scale), periodic and far-field faces and outlets on the x faces, force
+ * mode 6, mesh (frames or articulated) or static geometry with the
+ * signed-distance, half-way or interpolated-half wall rules, a fixed body
+ * pose (move_mesh placements included), no free flight.
  * Populations are snapshotted every checkpoint_interval steps in the forward
  * pass and recomputed segment by segment in the reverse sweep (memory:
- * about (segments + interval) x 76 B and interval x 21 B per cell). */
+ * about (segments + interval) x 19 x the storage's bytes and interval x 21 B
+ * per cell). */
 /* point_tangents: [parameter_count][frame_count][vertex_count][3]; 0 clears.
  * A new bf_mesh_upload clears the tangents. */
 bf_status bf_adjoint_set_tangents(bf_context* ctx, uint32_t parameter_count, const float* point_tangents);

@@ -30,8 +30,10 @@
 // leave through the face are pulled by nobody and are zeroed by their owner,
 // as behind a far-field face.
 //
-// Supported: FP32 storage, TRT / regularized / RR3, periodic and far-field
-// faces and outlets on the x faces, sponge, every wall rule, force mode 6.
+// Supported: FP32 storage and FP16 / BF16 offsets (decoded on load; lambda
+// stays FP32), TRT (with or without the symmetric limiter) / regularized /
+// RR3, periodic and far-field faces and outlets on the x faces, sponge, every
+// wall rule, force mode 6.
 #include "bf_adjoint_math.h"
 #include "bf_fluid.h"
 
@@ -81,6 +83,14 @@ __device__ __forceinline__ size_t outlet_slot(const Grid& g, int face, int q, in
 __device__ __forceinline__ size_t lam_at(const Grid& g, int q, int x, uint32_t row) {
     return pop_index<STORE_FP32>(g, q, x, row);
 }
+// A population of the step's input lattice (index i, direction q), decoded
+// from its stored form: FP32, or FP16 / BF16 offsets from w_q.  The adjoint
+// differentiates the FP32 arithmetic and takes the storage rounding as the
+// identity.
+template <int S>
+__device__ __forceinline__ float fprev(const AdjointBuffers& b, size_t i, int q) {
+    return Store<S>::load((const typename Store<S>::T*)b.f_prev, i, q);
+}
 // lambda_{s-1,q}(source) = v from the (only regular) puller x, or into the
 // outlet face buffer when x's pull along q is clamped; nothing for a
 // far-field source.  (xs, rows): the source's x and row.
@@ -104,7 +114,7 @@ __device__ __forceinline__ bool unpulled(const Sources& s, const StepParams& p, 
 // At least 4 wavefronts per SIMD (128 VGPRs): the RR3 instance spills 22
 // registers to scratch there and still runs 3 % faster than at its natural
 // 140 VGPRs and 3 wavefronts (193 against 200 us at 256 x 128 x 128).
-template <int COLL>
+template <int COLL, int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void adjoint_fast(AdjointBuffers b,
                                                                                            StepParams p) {
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -125,7 +135,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void a
         bool far;
         source_xr<q>(s, p.g, x, y, z, xs, rows, far);
         // unconditional load (a far-field source reads this cell)
-        const float v = b.f_prev[far ? pop_index<STORE_FP32>(p.g, q, x, row) : pop_index<STORE_FP32>(p.g, q, xs, rows)];
+        const float v = fprev<S>(b, far ? pop_index<S>(p.g, q, x, row) : pop_index<S>(p.g, q, xs, rows), q);
         g[q] = far ? p.far_eq[q] : v;
         lp[q] = b.lam_in[lam_at(p.g, q, x, row)] * keep;
     });
@@ -154,7 +164,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void a
 // sensitivity of J to each wall-link value (collision part plus the link's
 // own load), for a refilled cell mu.
 // ---------------------------------------------------------------------------
-template <int COLL>
+template <int COLL, int S>
 __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const StepParams& p,
                                                 const AdjointParams& w, int x, int y, int z, float bx, float by,
                                                 float bz, float* __restrict__ nu_cell) {
@@ -224,7 +234,7 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
             uint32_t rows;
             bool fq;
             source_xr<q>(s, g, x, y, z, xs, rows, fq);
-            gq[q] = b.f_prev[pop_index<STORE_FP32>(g, q, xs, rows)];
+            gq[q] = fprev<S>(b, pop_index<S>(g, q, xs, rows), q);
             return;
         }
         link[q] = true;
@@ -234,11 +244,11 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
         const bool fin_ok = inside(g, fx, fy, fz);
         const uint32_t fid = fin_ok ? flat(g, fx, fy, fz) : gid;
         const bool farther_ok = fin_ok && b.solid_cur[fid] == 0;
-        const float refl = b.f_prev[pop_index<STORE_FP32>(g, opp(q), x, row)];
+        const float refl = fprev<S>(b, pop_index<S>(g, opp(q), x, row), opp(q));
         const float farther =
-            farther_ok ? b.f_prev[pop_index<STORE_FP32>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)]
+            farther_ok ? fprev<S>(b, pop_index<S>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz), opp(q))
                        : 0.0f;
-        const float pin = b.f_prev[pop_index<STORE_FP32>(g, q, x, row)];
+        const float pin = fprev<S>(b, pop_index<S>(g, q, x, row), q);
         const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
                                                            p.wall_rule, farther_ok, refl, farther, pin, wgv, wsv);
         gq[q] = d.value;
@@ -258,7 +268,7 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
     });
 }
 
-template <int COLL>
+template <int COLL, int S>
 __global__ __launch_bounds__(256) void adjoint_general_p1(AdjointBuffers b, StepParams p, AdjointParams w) {
     const uint32_t count = *b.general_count;
     const float bx = (float)b.body[0], by = (float)b.body[1], bz = (float)b.body[2];
@@ -268,7 +278,7 @@ __global__ __launch_bounds__(256) void adjoint_general_p1(AdjointBuffers b, Step
         const int x = x0 + (int)(threadIdx.x & (TILE_X - 1));
         const int y = y0 + (int)(threadIdx.x / TILE_X);
         if (x < p.g.nx && y < p.g.ny)
-            general_p1_cell<COLL>(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x);
+            general_p1_cell<COLL, S>(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x);
     }
 }
 
@@ -331,6 +341,7 @@ __device__ __forceinline__ void contract(double (&acc)[8], int K, const float4* 
 // ---------------------------------------------------------------------------
 // Wall tiles, pass 2a: the cell's own populations and the wall gradient.
 // ---------------------------------------------------------------------------
+template <int S>
 __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const StepParams& p,
                                                  const AdjointParams& w, int x, int y, int z, float bx, float by,
                                                  float bz, const float* __restrict__ nu_cell, double (&acc)[8]) {
@@ -429,7 +440,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
         okbits |= (uint32_t)inside(g, x + cx(q), y + cy(q), z + cz(q)) << q;
         part[q] = b.solid_cur[src_of(qc)];
         fpart[q] = b.solid_cur[fid_of(qc)];
-        fown[q] = b.f_prev[pop_index<STORE_FP32>(g, q, x, row)];
+        fown[q] = fprev<S>(b, pop_index<S>(g, q, x, row), q);
         lam[q] = b.lam_out[lam_at(g, q, x, row)];
         nuv[q] = nu_cell[q * 256];
     });
@@ -468,9 +479,9 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
             srcs[q - q0] = src;
             ws[q - q0] = b.wall[src];
             const int fx = x + cx(q), fy = y + cy(q), fz = z + cz(q);
-            fo[q - q0] = b.f_prev[lk && inside(g, fx, fy, fz)
-                                      ? pop_index<STORE_FP32>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)
-                                      : pop_index<STORE_FP32>(g, opp(q), x, row)];
+            fo[q - q0] = fprev<S>(b, lk && inside(g, fx, fy, fz)
+                                      ? pop_index<S>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)
+                                      : pop_index<S>(g, opp(q), x, row), opp(q));
         });
         asm volatile("" : "+v"(ws[0].x), "+v"(ws[0].y), "+v"(ws[0].z), "+v"(ws[0].w), "+v"(ws[1].x),
                      "+v"(ws[1].y), "+v"(ws[1].z), "+v"(ws[1].w), "+v"(ws[2].x), "+v"(ws[2].y), "+v"(ws[2].z),
@@ -522,6 +533,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
     contract(acc, w.K, b.dW, n, gid, gxw[0], gxw[1], gxw[2], gxw[3]);
 }
 
+template <int S>
 __global__ __launch_bounds__(256) void adjoint_general_p2a(AdjointBuffers b, StepParams p, AdjointParams w) {
     const uint32_t count = *b.general_count;
     const float bx = (float)b.body[0], by = (float)b.body[1], bz = (float)b.body[2];
@@ -532,7 +544,7 @@ __global__ __launch_bounds__(256) void adjoint_general_p2a(AdjointBuffers b, Ste
         const int y = y0 + (int)(threadIdx.x / TILE_X);
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (x < p.g.nx && y < p.g.ny)
-            general_p2a_cell(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x, acc);
+            general_p2a_cell<S>(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x, acc);
         block_sum8(acc, b.grad_partials + (size_t)i * 8);
     }
 }
@@ -540,6 +552,7 @@ __global__ __launch_bounds__(256) void adjoint_general_p2a(AdjointBuffers b, Ste
 // ---------------------------------------------------------------------------
 // Wall tiles, pass 2b: populations one link beyond the cell.
 // ---------------------------------------------------------------------------
+template <int S>
 __global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, StepParams p, AdjointParams w) {
     const uint32_t count = *b.general_count;
     const float bx = (float)b.body[0], by = (float)b.body[1], bz = (float)b.body[2];
@@ -590,9 +603,9 @@ __global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, Ste
             const float wsv[4] = {ws4.x, ws4.y, ws4.z, ws4.w};
             const adj::LinkD<float> d = adj::link_value<float>(
                 cx(q), cy(q), cz(q), weight<float>(q), p.far_rho, p.wall_rule, true,
-                b.f_prev[pop_index<STORE_FP32>(g, opp(q), x, row)],
-                b.f_prev[pop_index<STORE_FP32>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)],
-                b.f_prev[pop_index<STORE_FP32>(g, q, x, row)], wgv, wsv);
+                fprev<S>(b, pop_index<S>(g, opp(q), x, row), opp(q)),
+                fprev<S>(b, pop_index<S>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz), opp(q)),
+                fprev<S>(b, pop_index<S>(g, q, x, row), q), wgv, wsv);
             if (d.uses_farther)
                 b.lam_out[lam_at(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)] +=
                     nu_cell[q * 256] * d.d_farther;
@@ -890,28 +903,40 @@ __global__ __launch_bounds__(256) void wall_tangent(const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-template <int COLL>
+template <int COLL, int S>
 static hipError_t adjoint_step_t(const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
                                  int general_blocks, hipStream_t s) {
-    hipLaunchKernelGGL(adjoint_fast<COLL>, dim3(p.g.tiles), dim3(256), 0, s, b, p);
-    hipLaunchKernelGGL(adjoint_general_p1<COLL>, dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    hipLaunchKernelGGL((adjoint_fast<COLL, S>), dim3(p.g.tiles), dim3(256), 0, s, b, p);
+    hipLaunchKernelGGL((adjoint_general_p1<COLL, S>), dim3(general_blocks), dim3(256), 0, s, b, p, w);
     if (p.bc[0][0] == BC_OUTLET || p.bc[0][1] == BC_OUTLET) {
         if (!b.outlet_extra) return hipErrorInvalidValue;
         const uint32_t threads = 2u * 19u * (uint32_t)p.g.ny * (uint32_t)p.g.nz;
         hipLaunchKernelGGL(adjoint_outlet, dim3((threads + 255) / 256), dim3(256), 0, s, b, p);
     }
-    hipLaunchKernelGGL(adjoint_general_p2a, dim3(general_blocks), dim3(256), 0, s, b, p, w);
-    hipLaunchKernelGGL(adjoint_general_p2b, dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    hipLaunchKernelGGL(adjoint_general_p2a<S>, dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    hipLaunchKernelGGL(adjoint_general_p2b<S>, dim3(general_blocks), dim3(256), 0, s, b, p, w);
     hipLaunchKernelGGL(reduce_gradient, dim3(1), dim3(256), 0, s, b.grad_partials, b.general_count, b.grad_total);
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint_step(int coll, const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
-                               int general_blocks, hipStream_t s) {
+template <int S>
+static hipError_t adjoint_step_s(int coll, const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
+                                 int general_blocks, hipStream_t s) {
     switch (coll) {
-        case COLL_TRT: return adjoint_step_t<COLL_TRT>(b, p, w, general_blocks, s);
-        case COLL_REGULARIZED: return adjoint_step_t<COLL_REGULARIZED>(b, p, w, general_blocks, s);
-        case COLL_RR3: return adjoint_step_t<COLL_RR3>(b, p, w, general_blocks, s);
+        case COLL_TRT: return adjoint_step_t<COLL_TRT, S>(b, p, w, general_blocks, s);
+        case COLL_TRT_SYMMETRIC_LIMITER: return adjoint_step_t<COLL_TRT_SYMMETRIC_LIMITER, S>(b, p, w, general_blocks, s);
+        case COLL_REGULARIZED: return adjoint_step_t<COLL_REGULARIZED, S>(b, p, w, general_blocks, s);
+        case COLL_RR3: return adjoint_step_t<COLL_RR3, S>(b, p, w, general_blocks, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_adjoint_step(int coll, int storage, const AdjointBuffers& b, const StepParams& p,
+                               const AdjointParams& w, int general_blocks, hipStream_t s) {
+    switch (storage) {
+        case STORE_FP32: return adjoint_step_s<STORE_FP32>(coll, b, p, w, general_blocks, s);
+        case STORE_FP16_OFFSET: return adjoint_step_s<STORE_FP16_OFFSET>(coll, b, p, w, general_blocks, s);
+        case STORE_BF16_OFFSET: return adjoint_step_s<STORE_BF16_OFFSET>(coll, b, p, w, general_blocks, s);
         default: return hipErrorInvalidValue;
     }
 }

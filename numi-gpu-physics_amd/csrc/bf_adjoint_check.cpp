@@ -42,6 +42,7 @@ int bfa_collide_f64(int coll, const double* f, double op, double om, double* pos
     adj::Fwd<double> t;
     switch (coll) {
         case COLL_TRT: adj::forward<COLL_TRT>(fi, op, om, po, t); break;
+        case COLL_TRT_SYMMETRIC_LIMITER: adj::forward<COLL_TRT_SYMMETRIC_LIMITER>(fi, op, om, po, t); break;
         case COLL_REGULARIZED: adj::forward<COLL_REGULARIZED>(fi, op, om, po, t); break;
         case COLL_RR3: adj::forward<COLL_RR3>(fi, op, om, po, t); break;
         default: return 1;
@@ -54,6 +55,7 @@ int bfa_collide_f64(int coll, const double* f, double op, double om, double* pos
 int bfa_collide_adjoint_f64(int coll, const double* f, double op, double om, const double* lp, double* lam) {
     switch (coll) {
         case COLL_TRT: rev<COLL_TRT>(f, op, om, lp, lam); return 0;
+        case COLL_TRT_SYMMETRIC_LIMITER: rev<COLL_TRT_SYMMETRIC_LIMITER>(f, op, om, lp, lam); return 0;
         case COLL_REGULARIZED: rev<COLL_REGULARIZED>(f, op, om, lp, lam); return 0;
         case COLL_RR3: rev<COLL_RR3>(f, op, om, lp, lam); return 0;
         default: return 1;

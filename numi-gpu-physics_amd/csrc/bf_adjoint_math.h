@@ -106,15 +106,62 @@ __host__ __device__ __forceinline__ void forward(const R (&f)[19], R op, R om, R
     t.limited = false;
     t.scale = R(1);
     t.qstar = -1;
-    if constexpr (COLL == COLL_TRT) {
+    if constexpr (COLL == COLL_TRT || COLL == COLL_TRT_SYMMETRIC_LIMITER) {
         post[0] = fma_(-op, f[0] - t.feq0, f[0]);
+        R hs[9], ha[9];
         each(Pairs{}, [&](auto kc) {
             constexpr int k = decltype(kc)::value, q = 2 * k + 1;
-            const R h = fma_(R(0.5), t.sp[k], -t.A[k]);
-            const R hd = fma_(R(0.5), t.dp[k], -t.B[k]);
-            post[q] = fma_(-om, hd, fma_(-op, h, f[q]));
-            post[q + 1] = fma_(om, hd, fma_(-op, h, f[q + 1]));
+            hs[k] = fma_(R(0.5), t.sp[k], -t.A[k]);
+            ha[k] = fma_(R(0.5), t.dp[k], -t.B[k]);
+            post[q] = fma_(-om, ha[k], fma_(-op, hs[k], f[q]));
+            post[q + 1] = fma_(om, ha[k], fma_(-op, hs[k], f[q + 1]));
         });
+        if constexpr (COLL == COLL_TRT_SYMMETRIC_LIMITER) {
+            // the symmetric increment si = -w+ (f^s - feq^s) scaled by the
+            // largest s <= 1 keeping every population at or above its floor
+            // (collide_fused in bf_fluid.h)
+            R lo = fma_(R(-1.0e-6), abs_(t.feq0), post[0]);
+            each(Pairs{}, [&](auto kc) {
+                constexpr int k = decltype(kc)::value, q = 2 * k + 1;
+                lo = min_(lo, min_(fma_(R(-1.0e-6), abs_(t.A[k] + t.B[k]), post[q]),
+                                   fma_(R(-1.0e-6), abs_(t.A[k] - t.B[k]), post[q + 1])));
+            });
+            if (lo < R(2.0e-12)) {
+                R s = R(1);
+                int qs = -1;
+                auto limit = [&](int q, R fq, R feq, R si, R ai) {
+                    if (si < R(0)) {
+                        const R b = fq + ai;
+                        const R floorv = max_(R(1.0e-12), R(1.0e-6) * max_(feq, R(0)));
+                        if (b + si < floorv) {
+                            const R adm = (b - floorv) / max_(-si, R(1.0e-30));
+                            const R c = min_(max_(adm, R(0)), R(1));
+                            if (c < s) {
+                                s = c;
+                                qs = adm > R(0) && adm < R(1) ? q : -1;
+                            }
+                        }
+                    }
+                };
+                limit(0, f[0], t.feq0, -op * (f[0] - t.feq0), R(0));
+                each(Pairs{}, [&](auto kc) {
+                    constexpr int k = decltype(kc)::value, q = 2 * k + 1;
+                    limit(q, f[q], t.A[k] + t.B[k], -op * hs[k], -om * ha[k]);
+                    limit(q + 1, f[q + 1], t.A[k] - t.B[k], -op * hs[k], om * ha[k]);
+                });
+                if (s < R(1)) {
+                    t.limited = true;
+                    t.scale = s;
+                    t.qstar = qs;
+                    post[0] = fma_(s, -op * (f[0] - t.feq0), f[0]);
+                    each(Pairs{}, [&](auto kc) {
+                        constexpr int k = decltype(kc)::value, q = 2 * k + 1;
+                        post[q] = fma_(s, -op * hs[k], f[q]) - om * ha[k];
+                        post[q + 1] = fma_(s, -op * hs[k], f[q + 1]) + om * ha[k];
+                    });
+                }
+            }
+        }
     } else {
         constexpr bool RR3 = COLL == COLL_RR3;
         const R c2 = R(1) / R(3);
@@ -256,6 +303,77 @@ __host__ __device__ __forceinline__ void reverse(const R (&f)[19], R op, R om, c
             lA[k] = -lh;
             ldp[k] = R(0.5) * lhd;
             lB[k] = -lhd;
+        });
+    } else if constexpr (COLL == COLL_TRT_SYMMETRIC_LIMITER) {
+        // post_q = f_q + s si_q + ai_q with si = -w+ h (h: symmetric
+        // non-equilibrium), ai = -w- hd; s = 1 unless limited, then
+        // s = (f* + ai* - floor*) / (-si*) of the direction q* that sets it
+        const R s = t.scale;
+        R lh[9], lhd[9];
+        R lsi0 = s * lp[0];
+        R ls = R(0);
+        each(Pairs{}, [&](auto kc) {
+            constexpr int k = decltype(kc)::value, q = 2 * k + 1;
+            lh[k] = -op * s * (lp[q] + lp[q + 1]);
+            lhd[k] = -om * (lp[q] - lp[q + 1]);
+        });
+        lf0 = lp[0];
+        if (t.limited) {
+            const R si0 = -op * (f[0] - t.feq0);
+            ls = si0 * lp[0];
+            each(Pairs{}, [&](auto kc) {
+                constexpr int k = decltype(kc)::value, q = 2 * k + 1;
+                ls = ls + (-op * fma_(R(0.5), t.sp[k], -t.A[k])) * (lp[q] + lp[q + 1]);
+            });
+        }
+        R lfeq_star = R(0);
+        if (t.limited && t.qstar >= 0) {
+            // the binding direction's population, increments and equilibrium
+            const int qs = t.qstar;
+            const int k = qs == 0 ? -1 : (qs - 1) / 2;
+            const R sgn = (qs != 0 && (qs % 2) == 0) ? R(-1) : R(1);      // q + 1 of a pair: -B, +w- hd
+            const R fq = f[qs];
+            const R feq = qs == 0 ? t.feq0 : t.A[k] + sgn * t.B[k];
+            const R hsk = qs == 0 ? R(0) : fma_(R(0.5), t.sp[k], -t.A[k]);
+            const R hak = qs == 0 ? R(0) : fma_(R(0.5), t.dp[k], -t.B[k]);
+            const R si = qs == 0 ? -op * (f[0] - t.feq0) : -op * hsk;
+            const R ai = qs == 0 ? R(0) : -sgn * om * hak;
+            const bool lin = R(1.0e-6) * max_(feq, R(0)) > R(1.0e-12);
+            const R fl = lin ? R(1.0e-6) * feq : R(1.0e-12);
+            const R num = fq + ai - fl;
+            const R ds_dnum = R(1) / (-si);
+            const R ds_dsi = num / (si * si);
+            const R lnum = ls * ds_dnum;
+            const R lsi = ls * ds_dsi;
+            // num = f* + ai* - floor(feq*)
+            if (qs == 0) lf0 = lf0 + lnum;
+            else direct[qs] = direct[qs] + lnum;
+            if (qs != 0) lhd[k] = lhd[k] + (-sgn * om) * lnum;
+            lfeq_star = lin ? -R(1.0e-6) * lnum : R(0);
+            // si* = -w+ (f0 - feq0) or -w+ hs_k
+            if (qs == 0) {
+                lf0 = lf0 - op * lsi;
+                lfeq0 = lfeq0 + op * lsi;
+            } else {
+                lh[k] = lh[k] - op * lsi;
+            }
+            if (qs == 0) lfeq0 = lfeq0 + lfeq_star;
+            else {
+                lA[k] = lA[k] + lfeq_star;
+                lB[k] = lB[k] + sgn * lfeq_star;
+            }
+        }
+        // post_0 = f0 + s (-w+)(f0 - feq0)
+        lf0 = lf0 - op * lsi0;
+        lfeq0 = lfeq0 + op * lsi0;
+        each(Pairs{}, [&](auto kc) {
+            constexpr int k = decltype(kc)::value, q = 2 * k + 1;
+            direct[q] = direct[q] + lp[q];
+            direct[q + 1] = direct[q + 1] + lp[q + 1];
+            lsp[k] = R(0.5) * lh[k];
+            lA[k] = lA[k] - lh[k];
+            ldp[k] = R(0.5) * lhd[k];
+            lB[k] = lB[k] - lhd[k];
         });
     } else {
         constexpr bool RR3 = COLL == COLL_RR3;

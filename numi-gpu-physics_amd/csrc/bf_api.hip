@@ -215,8 +215,8 @@ struct bf_context {
         KinParams kin{};
         uint32_t interval = 0, segments = 0;
         uint64_t verts = 0;               // vertex capacity of dpos, dvel, gpos, gvel
-        float* checkpoints = nullptr;     // [segments][19N]
-        float* seg_f = nullptr;           // [interval][19N]: f_a .. f_{b-1}
+        uint8_t* checkpoints = nullptr;   // [segments][19N], stored form (pop_elem bytes each)
+        uint8_t* seg_f = nullptr;         // [interval][19N]: f_a .. f_{b-1}, stored form
         // geometry of steps a .. b of a segment, two sets: the reverse sweep
         // of one segment (stream B) reads one while the geometry of the next
         // segment is built into the other (stream A)
@@ -2194,10 +2194,6 @@ namespace {
 
 bf_status adjoint_supported(bf_context* c) {
     const bf_config& g = c->cfg;
-    if (g.storage != BF_STORAGE_FP32)
-        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint runs on FP32 population storage");
-    if (g.collision == BF_COLLISION_TRT_SYMMETRIC_LIMITER)
-        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers TRT, regularized and RR3 collisions");
     if (g.force_mode != 6) return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates force mode 6");
     if (!g.accumulate_loads) return fail(c, BF_ERROR_INVALID_ARGUMENT, "the objective needs accumulate_loads");
     for (int a = 1; a < 3; ++a)
@@ -2224,9 +2220,10 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     // mesh uploaded since needs new ones
     if (a.interval == interval && a.segments >= segments && a.checkpoints && a.verts >= c->m_vertices) return BF_OK;
     a.alloc.release();
-    const uint64_t n = c->g.n, tiles = c->g.tiles, pop = 19ull * n * 4;
+    // pop: one FP32 lattice (lambda); popS: one lattice in the storage's form
+    const uint64_t n = c->g.n, tiles = c->g.tiles, pop = 19ull * n * 4, popS = 19ull * n * c->pop_elem;
     const uint64_t verts = std::max<uint32_t>(c->m_vertices, 1u);
-    const uint64_t need = pop * segments + pop * interval + 2 * (uint64_t)(interval + 1) * (21 * n + 7 * tiles + 4) +
+    const uint64_t need = popS * segments + popS * interval + 2 * (uint64_t)(interval + 1) * (21 * n + 7 * tiles + 4) +
                           2 * pop + tiles * 19 * 256 * 4 + 2 * 8 * 16 * n + 2 * 8 * 16 * verts + 2 * 16 * verts +
                           tiles * 64 + 4 * tiles + 2ull * 19 * c->g.ny * c->g.nz * 4 + 1024;
     size_t free_b = 0, total_b = 0;
@@ -2239,8 +2236,8 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     }
     hipError_t e = hipSuccess;
     DeviceAlloc& A = a.alloc;
-    a.checkpoints = (float*)A.take(e, pop * segments);
-    a.seg_f = (float*)A.take(e, pop * interval);
+    a.checkpoints = (uint8_t*)A.take(e, popS * segments);
+    a.seg_f = (uint8_t*)A.take(e, popS * interval);
     for (int i = 0; i < 2; ++i) {
         a.seg_mask[i] = (uint8_t*)A.take(e, (interval + 1) * n);
         a.seg_wall[i] = (float4*)A.take(e, (interval + 1) * 16 * n);
@@ -2451,7 +2448,7 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
     const uint32_t interval = std::min(checkpoint_interval, steps);
     const uint32_t segments = (steps + interval - 1) / interval;
     if (bf_status r = adjoint_alloc(c, interval, segments)) return r;
-    const uint64_t n = c->g.n, pop = 19ull * n;
+    const uint64_t n = c->g.n, pop = 19ull * n, popS = pop * c->pop_elem;   // elements; snapshot bytes
     const int64_t s0 = c->step;
     hipEvent_t t0, t1, t2;
     (void)hipEventCreate(&t0);
@@ -2462,7 +2459,7 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
     // forward pass: the ordinary pipelined steps, a snapshot every interval
     hipError_t e = hipEventRecord(t0, c->sb);
     for (uint32_t seg = 0; seg < segments && e == hipSuccess; ++seg) {
-        e = hipMemcpyAsync(a.checkpoints + seg * pop, c->f[c->cur], 4 * pop, hipMemcpyDeviceToDevice, c->sb);
+        e = hipMemcpyAsync(a.checkpoints + seg * popS, c->f[c->cur], popS, hipMemcpyDeviceToDevice, c->sb);
         if (e != hipSuccess) break;
         const uint32_t todo = std::min<uint32_t>(interval, steps - seg * interval);
         if (bf_status r = bf_step(c, todo)) return r;
@@ -2503,13 +2500,13 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
             if (bf_status r = segment_geometry(c, s0, steps, interval, seg - 1)) return r;
         }
         if (e == hipSuccess)
-            e = hipMemcpyAsync(a.seg_f, a.checkpoints + seg * pop, 4 * pop, hipMemcpyDeviceToDevice, c->sb);
+            e = hipMemcpyAsync(a.seg_f, a.checkpoints + seg * popS, popS, hipMemcpyDeviceToDevice, c->sb);
         for (int64_t s = sa_ + 1; s < sb_ && e == hipSuccess; ++s) {
             const SlotGeometry gp = slot_geometry(c, (uint32_t)(s - 1 - sa_), set),
                                gc = slot_geometry(c, (uint32_t)(s - sa_), set);
             FluidBuffers fb{};
-            fb.fin = a.seg_f + (s - 1 - sa_) * pop;
-            fb.fout = a.seg_f + (s - sa_) * pop;
+            fb.fin = a.seg_f + (s - 1 - sa_) * popS;
+            fb.fout = a.seg_f + (s - sa_) * popS;
             fb.solid_prev = gp.mask;
             fb.solid_cur = gc.mask;
             fb.wall = gc.wall;
@@ -2550,7 +2547,7 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
                 ++tangent_steps;
             }
             AdjointBuffers ab{};
-            ab.f_prev = a.seg_f + (k - 1) * pop;
+            ab.f_prev = a.seg_f + (k - 1) * popS;
             ab.lam_in = a.lam[a.lam_cur];
             ab.lam_out = a.lam[1 - a.lam_cur];
             ab.solid_prev = gp.mask;
@@ -2574,7 +2571,7 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
                 w.cT[i] = (float)lw[3 + i];
             }
             w.K = (int)a.K;
-            e = launch_adjoint_step(coll, ab, pa, w, c->general_blocks, c->sb);
+            e = launch_adjoint_step(coll, storage, ab, pa, w, c->general_blocks, c->sb);
             if (e == hipSuccess && a.K > 0) e = hipEventRecord(a.ev_used[j], c->sb);
             if (e != hipSuccess) return hip_fail(c, e, "adjoint step");
             a.lam_cur = 1 - a.lam_cur;

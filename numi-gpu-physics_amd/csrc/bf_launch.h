@@ -214,7 +214,7 @@ struct RayBuffers {
 // dJ/df_{s-1}; f_prev = f_{s-1}; masks and wall of step s-1 / s; nu:
 // per listed tile 19 x 256 floats; dW: [K][N] wall tangents of step s.
 struct AdjointBuffers {
-    const float* f_prev;
+    const void* f_prev;           // the step's input populations, stored form of the storage
     const float* lam_in;
     float* lam_out;
     const uint8_t* solid_prev;
@@ -236,8 +236,8 @@ struct AdjointParams {
     float cF[3], cT[3];           // load weights of the step: J_s = cF . F + cT . T
     int K;                        // parameters (<= 8)
 };
-hipError_t launch_adjoint_step(int coll, const AdjointBuffers& b, const StepParams& p, const AdjointParams& w,
-                               int general_blocks, hipStream_t s);
+hipError_t launch_adjoint_step(int coll, int storage, const AdjointBuffers& b, const StepParams& p,
+                               const AdjointParams& w, int general_blocks, hipStream_t s);
 // Kinematic parameters of an articulated mesh (layout of
 // bf_kinematic_parameter): coefficient [harmonic][component] of series
 // `channel` (stroke, deviation, pitch, twist) of every joint in joint_mask.

@@ -1,8 +1,9 @@
 """CPU checks of the hand-written adjoint math (FP64 instantiations of the
 device code) against central finite differences of its forward twin:
-collision (TRT, regularized, RR3 with the positivity limiter), the
-moving-wall bounce-back value in every branch, and the equilibrium's
-velocity derivative."""
+collision (TRT, TRT with the symmetric limiter, regularized, RR3 with the
+positivity limiter), the moving-wall bounce-back value in every branch, and
+the equilibrium's velocity derivative.  The forward twin of the symmetric
+limiter is itself checked against the FP64 oracle's collision."""
 import ctypes
 import os
 
@@ -12,7 +13,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "numi-gpu-physics_amd", "birdflow_hip", "libbirdflow_adjoint_check.so")
 
-TRT, REG, RR3 = 0, 2, 3
+TRT, SYM, REG, RR3 = 0, 1, 2, 3
 CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0])
 CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1])
 CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1])
@@ -49,7 +50,7 @@ def equilibrium(rho, u):
     return W * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * np.dot(u, u))
 
 
-@pytest.mark.parametrize("coll", [TRT, REG, RR3])
+@pytest.mark.parametrize("coll", [TRT, SYM, REG, RR3])
 @pytest.mark.parametrize("strong", [False, True])
 def test_collision_adjoint_matches_finite_differences(lib, coll, strong):
     rng = np.random.default_rng(17 + coll + 10 * strong)
@@ -77,6 +78,28 @@ def test_collision_adjoint_matches_finite_differences(lib, coll, strong):
     assert checked >= 30
     if strong and coll != TRT:
         assert limited_seen >= 3
+
+
+def test_symmetric_limiter_twin_matches_the_oracle(lib):
+    # the adjoint's forward twin of the symmetric-limiter TRT against the
+    # FP64 oracle's collision of one cell (a 1 x 1 x 1 periodic step), both
+    # when the limiter binds and when it does not
+    import bfref
+    rng = np.random.default_rng(5)
+    op, om = 1.95, 1.2
+    limited = 0
+    for _ in range(200):
+        rho, u = 1.0 + 0.05 * rng.standard_normal(), 0.1 * rng.standard_normal(3)
+        f = np.abs(equilibrium(rho, u) * (1.0 + 0.8 * rng.standard_normal(19)))
+        post, lim = collide(lib, SYM, f, op, om)
+        p = bfref.make_params((1, 1, 1), omega_plus=op, omega_minus=om,
+                              collision=bfref.COLLISION_TRT_SYMMETRIC_LIMITER, accumulate_loads=True)
+        want, loads, *_ = bfref.step(p, f.copy())
+        # the oracle keeps its rates in FP32: compare at their rounding
+        assert np.abs(post - want).max() <= 1e-6 * np.abs(want).max(), (post, want)
+        assert lim == int(loads[6] != 0.0)
+        limited += lim
+    assert limited >= 20
 
 
 WALL_HALFWAY, WALL_SDF, WALL_HALF = 0, 1, 2

@@ -2,7 +2,10 @@
 
 * dJ/df at the start state (the whole reverse sweep through stream, collide,
   wall links, sponge, far-field and outlet faces, checkpoint / recompute)
-  against central finite differences of the FP64 oracle.
+  against central finite differences of the FP64 oracle; also for TRT with
+  the symmetric limiter binding in many cells.
+* dJ/dtheta of the plate with FP16 / BF16 offset storage against the FP64
+  reference gradient (the storage's error stated).
 * dJ/dtheta of a flapping plate's time-averaged lift with respect to its 8
   kinematic parameters against an FP64 oracle reference gradient: central
   differences of the FP64 step with FP64 wall fields on the same frozen link
@@ -112,6 +115,49 @@ def test_adjoint_sensitivity_to_start_state_matches_fp64_differences(collision, 
         assert abs(ad - fd) <= 2e-4 * abs(fd)
 
 
+def test_adjoint_of_the_symmetric_limiter_where_it_binds():
+    # TRT with the symmetric limiter (far-field faces, static walls) from a
+    # start state whose 6 x 6 x 6 interior block is far from equilibrium, so
+    # that the limiter scales the symmetric increment of many cells in the
+    # first steps: dJ/df0 against FP64 central differences of the oracle.
+    say("case: adjoint dJ/df0, TRT with the symmetric limiter binding, static walls, far field, 4 steps")
+    rng = np.random.default_rng(77)
+    shape = (64, 16, 14)
+    n = shape[0] * shape[1] * shape[2]
+    p = bfref.make_params(shape, omega_plus=1.0 / 0.52, omega_minus=1.0 / 1.1, bc=FACES["far field"],
+                          collision=bfref.COLLISION_TRT_SYMMETRIC_LIMITER, wall_rule=bfref.WALL_INTERPOLATED_SDF,
+                          force_mode=6, accumulate_loads=True, far_u=(0.05, 0.01, 0.0), body_position=(32.0, 8.0, 7.0))
+    solid = np.zeros((shape[2], shape[1], shape[0]), np.uint8)
+    solid[5:9, 5:11, 40:46] = 2
+    wall = np.zeros((n, 4), np.float32)
+    wall[:, :3] = rng.uniform(-0.04, 0.04, (n, 3))
+    wall[:, 3] = rng.uniform(-1.0, 2.0, n)
+    mult = rng.uniform(0.97, 1.03, (19, shape[2], shape[1], shape[0]))
+    mult[:, 4:10, 5:11, 12:18] = rng.uniform(0.02, 1.98, (19, 6, 6, 6))
+    f0 = (bfref.init_equilibrium(p, dtype=np.float32).reshape(19, n) * mult.reshape(19, n).astype(np.float32)).ravel()
+    steps = 4
+    weights = rng.normal(size=(steps, 6))
+    with bh.BirdFlowSimulation(cases.solver_config(p)) as sim:
+        sim.set_geometry(solid.ravel(), wall)
+        sim.write_populations(f0)
+        J, grad, timings = sim.adjoint_gradient(steps, weights, checkpoint_interval=2)
+        lam = sim.adjoint_sensitivity().astype(np.float64)
+    geo = lambda s: (solid.ravel(), solid.ravel(), wall)
+    J32 = objective_oracle(p, f0, steps, weights, geo, dtype=np.float32)
+    _, loads, _, _ = bfref.step(p, np.asarray(f0, np.float64), solid.ravel(), solid.ravel(), wall)
+    say(f"  J gpu {J:.9e} vs FP32 oracle {J32:.9e}; cells limited in step 1: {int(loads[6])}")
+    assert loads[6] >= 30                # 44 cells limited in step 1, 79 over the 4 steps
+    assert abs(J - J32) <= 1e-9 * max(1.0, abs(J32))
+    for trial in range(2):
+        delta = rng.normal(size=19 * n) * f0
+        h = 1e-7
+        fd = (objective_oracle(p, f0 + h * delta, steps, weights, geo)
+              - objective_oracle(p, f0 - h * delta, steps, weights, geo)) / (2 * h)
+        ad = float(np.dot(lam, delta))
+        say(f"  direction {trial}: adjoint {ad:.9e}, FP64 central difference {fd:.9e}, rel {abs(ad - fd) / abs(fd):.2e}")
+        assert abs(ad - fd) <= 2e-4 * abs(fd)
+
+
 def plate_case(steps):
     p = cases.plate_params(shape=(96, 48, 48))
     p.bc[0][1] = bfref.BC_FARFIELD          # far field on both x faces (the inlet / outlet plate is below)
@@ -169,6 +215,68 @@ def test_adjoint_gradient_of_plate_lift_matches_fp64_reference():
     say(f"  max |adjoint - reference| / max |reference| = {err:.2e}; timings {timings}")
     assert abs(J - J0) <= 1e-4 * abs(J0)
     assert err <= 1e-3
+
+
+# bars for (|J - J64| / |J64|, gradient error): measured 1.7e-5 / 3.9e-5 (FP16)
+# and 7.9e-4 / 8.5e-4 (BF16), FP32 storage from the same start 9e-6
+OFFSET_BARS = {bh.STORAGE_FP16_OFFSET: (2e-4, 5e-4), bh.STORAGE_BF16_OFFSET: (3e-3, 5e-3)}
+
+
+@pytest.mark.parametrize("storage", [bh.STORAGE_FP16_OFFSET, bh.STORAGE_BF16_OFFSET], ids=["fp16", "bf16"])
+def test_adjoint_with_offset_storage_against_fp64_reference(storage):
+    # The same plate with populations stored as FP16 / BF16 offsets: the
+    # forward steps and the recompute run in that storage, lambda in FP32,
+    # the storage rounding taken as the identity.  Against the FP64
+    # reference gradient (no storage rounding) the error is the storage's;
+    # the FP32-storage gradient from the same start state is printed beside.
+    steps = 30
+    p, weights = plate_case(steps)
+    seq, _, tan = plate_frames(THETA0, steps)
+    name = {bh.STORAGE_FP16_OFFSET: "FP16", bh.STORAGE_BF16_OFFSET: "BF16"}[storage]
+    say(f"case: adjoint d(mean lift)/d theta, parametric plate {p.nx}x{p.ny}x{p.nz} RR3, {steps} steps, "
+        f"{name} offset storage")
+    grads = {}
+    for st in (storage, bh.STORAGE_FP32):
+        with bh.BirdFlowSimulation(cases.solver_config(p, storage=st)) as sim:
+            sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+            sim.adjoint_set_tangents(tan)
+            if st == storage:
+                sim.init_equilibrium()
+                f0 = sim.read_populations()          # decoded from the stored offsets
+            else:
+                sim.write_populations(f0)            # FP32 holds those values exactly
+            J, grad, _ = sim.adjoint_gradient(steps, weights, checkpoint_interval=10)
+            grads[st] = (J, grad)
+    masks, tris = frozen_links(p, seq, steps)
+
+    def J64(theta):
+        _, pts64, _ = plate_frames(theta, steps)
+
+        def geo(s):
+            pos, vel = bfref.interpolate_frames_f64(pts64, seq.times, float(seq.times[0]) + s)
+            return masks[s - 1], masks[s], bfref.wall_field_f64(p, pos, vel, seq.triangles, tris[s],
+                                                                 seq.half_thickness)
+        return objective_oracle(p, f0, steps, weights, geo)
+
+    J0 = J64(THETA0)
+    ref = np.zeros(8)
+    for k in range(8):
+        h = 1e-5
+        e = np.zeros(8)
+        e[k] = h
+        ref[k] = (J64(THETA0 + e) - J64(THETA0 - e)) / (2 * h)
+    J, grad = grads[storage]
+    err = np.abs(grad - ref).max() / np.abs(ref).max()
+    err32 = np.abs(grads[bh.STORAGE_FP32][1] - ref).max() / np.abs(ref).max()
+    say(f"  J {name} {J:.9e}, FP32 storage {grads[bh.STORAGE_FP32][0]:.9e}, FP64 reference {J0:.9e}")
+    for k, pname in enumerate(workloads.PLATE_PARAMETERS):
+        say(f"  {pname:14s} adjoint {name} {grad[k]: .6e}  FP32 storage {grads[bh.STORAGE_FP32][1][k]: .6e}  "
+            f"FP64 reference {ref[k]: .6e}")
+    say(f"  max |adjoint - reference| / max |reference|: {name} {err:.2e}, FP32 storage {err32:.2e}; "
+        f"|J - J64| / |J64| {abs(J - J0) / abs(J0):.2e}")
+    bar_J, bar_g = OFFSET_BARS[storage]
+    assert abs(J - J0) <= bar_J * abs(J0)
+    assert err <= bar_g and err32 <= 1e-4
 
 
 def frozen_links(p, seq, steps):
