@@ -189,9 +189,11 @@ def run_storage_articulated(shape, steps, storage, mesh):
     return loads, (m1 - m0) / m0, u
 
 
-# the whole 2,000 steps (the wing stroke's first 1.7 periods at 2x scale):
-# measured FP16 / BF16 force history against FP32 ... / ...
-BARS_2000 = {bh.STORAGE_FP16_OFFSET: 5e-2, bh.STORAGE_BF16_OFFSET: 2e-1}
+# the whole 2,000 steps (0.83 of the 2,400-step stroke period at 2x scale):
+# force history against FP32 within 0.5 % (FP16) / 5 % (BF16) of the largest
+# force, velocity L2 at the end within 5e-3 / 5e-2; measured 2.1e-4 / 5.1e-3
+# and 8.4e-4 / 1.1e-2
+BARS_2000 = {bh.STORAGE_FP16_OFFSET: 5e-3, bh.STORAGE_BF16_OFFSET: 5e-2}
 
 
 def test_precision_form_at_full_size_against_fp32():
@@ -217,6 +219,7 @@ def test_precision_form_at_full_size_against_fp32():
         assert np.isfinite(loads).all()
         assert force200 <= BARS[storage]
         assert force <= BARS_2000[storage]
+        assert vel <= BARS_2000[storage]
         assert abs(drift - drift32) <= 1e-3
 
 
