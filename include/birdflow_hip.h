@@ -494,8 +494,9 @@ bf_status bf_inject_population(bf_context* ctx, uint64_t index, float value);
  * (the limiters differentiated through the direction that sets their
  * scale), periodic and far-field faces and outlets on the x faces, force
  * mode 6, mesh (frames or articulated) or static geometry with the
- * signed-distance, half-way or interpolated-half wall rules, a fixed body
- * pose (move_mesh placements included), no free flight.
+ * signed-distance, half-way, interpolated-half or exact-root wall rule (each
+ * exact root differentiated through the triangle whose shell gives it), a
+ * fixed body pose (move_mesh placements included), no free flight.
  * Populations are snapshotted every checkpoint_interval steps in the forward
  * pass and recomputed segment by segment in the reverse sweep (memory:
  * about (segments + interval) x 19 x the storage's bytes and interval x 21 B

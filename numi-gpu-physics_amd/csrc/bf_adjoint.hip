@@ -36,6 +36,7 @@
 // wall rule, force mode 6.
 #include "bf_adjoint_math.h"
 #include "bf_fluid.h"
+#include "bf_rays.h"
 
 namespace bf {
 
@@ -90,6 +91,20 @@ __device__ __forceinline__ size_t lam_at(const Grid& g, int q, int x, uint32_t r
 template <int S>
 __device__ __forceinline__ float fprev(const AdjointBuffers& b, size_t i, int q) {
     return Store<S>::load((const typename Store<S>::T*)b.f_prev, i, q);
+}
+// Rule and table fraction of link (x, q) (its source solid): under the
+// exact-root rule the table's entry where a triangle's shell gave it, else
+// the signed-distance ratio (the table holds that ratio there, computed from
+// the same walls as link_value computes it).
+__device__ __forceinline__ int link_rule(const AdjointBuffers& b, const StepParams& p, int x, int y, int z, int q,
+                                         float& tlf, uint32_t& win) {
+    tlf = 0.0f;
+    win = NO_WINNER;
+    if (p.wall_rule != WALL_INTERPOLATED_LINKS) return p.wall_rule;
+    const size_t slot = table_slot(p.g, x, y, z, q);
+    win = b.winners[slot];
+    tlf = b.links[slot];
+    return win == NO_WINNER ? WALL_INTERPOLATED_SDF : WALL_INTERPOLATED_LINKS;
 }
 // lambda_{s-1,q}(source) = v from the (only regular) puller x, or into the
 // outlet face buffer when x's pull along q is clamped; nothing for a
@@ -249,8 +264,11 @@ __device__ __forceinline__ void general_p1_cell(const AdjointBuffers& b, const S
             farther_ok ? fprev<S>(b, pop_index<S>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz), opp(q))
                        : 0.0f;
         const float pin = fprev<S>(b, pop_index<S>(g, q, x, row), q);
-        const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
-                                                           p.wall_rule, farther_ok, refl, farther, pin, wgv, wsv);
+        float tlf;
+        uint32_t win;
+        const int rule = link_rule(b, p, x, y, z, q, tlf, win);
+        const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho, rule,
+                                                           farther_ok, refl, farther, pin, wgv, wsv, tlf);
         gq[q] = d.value;
         const int sel = p.selected_part;
         if (p.accumulate_loads && p.load_component != 2 && (sel == 0 || sel == (int)part)) {
@@ -337,6 +355,14 @@ __device__ __forceinline__ void contract(double (&acc)[8], int K, const float4* 
         acc[k] += (double)gx * t.x + (double)gy * t.y + (double)gz * t.z + (double)gw * t.w;
     }
 }
+
+// d lf / d theta_k of an exact-root link (x, q) whose entry triangle t lies
+// in the link (defined below, with the dual numbers).
+struct RayDQ {
+    float v[8];
+};
+__device__ __noinline__ RayDQ ray_link_dq(const AdjointBuffers& b, const StepParams& p, int x, int y, int z, int q,
+                                          uint32_t t, int K);
 
 // ---------------------------------------------------------------------------
 // Wall tiles, pass 2a: the cell's own populations and the wall gradient.
@@ -495,9 +521,19 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
             const float refl = fown[opp(q)];
             const float farther = farther_ok ? fo[q - q0] : 0.0f;
             const float pin = fown[q];
+            float tlf;
+            uint32_t win;
+            const int rule = link_rule(b, p, x, y, z, q, tlf, win);
             const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
-                                                               p.wall_rule, farther_ok, refl, farther, pin, wgv, wsv);
+                                                               rule, farther_ok, refl, farther, pin, wgv, wsv, tlf);
             const float nu = nuv[q];
+            // exact-root fraction of an unclamped entry: d lf / d theta from
+            // the entry triangle's shell
+            if (rule == WALL_INTERPOLATED_LINKS && w.K > 0 && tlf > 1.0e-4f && tlf < 1.0f && d.dv_dlf != 0.0f) {
+                const RayDQ r = ray_link_dq(b, p, x, y, z, q, win, w.K);
+                const double sc = (double)(nu * d.dv_dlf);
+                for (int k = 0; k < w.K; ++k) acc[k] += sc * (double)r.v[k];
+            }
             float refl_load = 0.0f;
             if (p.accumulate_loads && p.load_component != 2 && ((selbits >> q) & 1u)) {
                 const float larm[3] = {wx - d.lf * (float)cx(q) * p.dx - bx, wy - d.lf * (float)cy(q) * p.dx - by,
@@ -601,11 +637,14 @@ __global__ __launch_bounds__(256) void adjoint_general_p2b(AdjointBuffers b, Ste
             if (b.solid_cur[fid] != 0) return;
             const float4 ws4 = b.wall[src];
             const float wsv[4] = {ws4.x, ws4.y, ws4.z, ws4.w};
+            float tlf;
+            uint32_t win;
+            const int rule = link_rule(b, p, x, y, z, q, tlf, win);
             const adj::LinkD<float> d = adj::link_value<float>(
-                cx(q), cy(q), cz(q), weight<float>(q), p.far_rho, p.wall_rule, true,
+                cx(q), cy(q), cz(q), weight<float>(q), p.far_rho, rule, true,
                 fprev<S>(b, pop_index<S>(g, opp(q), x, row), opp(q)),
                 fprev<S>(b, pop_index<S>(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz), opp(q)),
-                fprev<S>(b, pop_index<S>(g, q, x, row), q), wgv, wsv);
+                fprev<S>(b, pop_index<S>(g, q, x, row), q), wgv, wsv, tlf);
             if (d.uses_farther)
                 b.lam_out[lam_at(g, opp(q), fx, (uint32_t)fy + (uint32_t)g.ny * (uint32_t)fz)] +=
                     nu_cell[q * 256] * d.d_farther;
@@ -849,6 +888,129 @@ __global__ void articulated_tangents(const float* __restrict__ rest, const uint8
         }
         place_tangent(dlp, dlv, body, m, dpos[(size_t)p * m.vertex_count + v], dvel[(size_t)p * m.vertex_count + v]);
     }
+}
+
+// The shell entry of ray_shell (bf_rays.hip) with the triangle's vertices
+// carrying derivatives along one parameter: the same operations on the
+// values (so the same branch: face, edge cylinder or vertex sphere) and
+// their derivatives; the ray F + t D itself does not move.
+__device__ __forceinline__ D dsqrt(D a) {
+    const float r = sqrtf(a.v);
+    return {r, r > 0.0f ? a.d / (2.0f * r) : 0.0f};
+}
+__device__ __forceinline__ void dsub3(const D a[3], const D b[3], D o[3]) {
+    for (int i = 0; i < 3; ++i) o[i] = a[i] - b[i];
+}
+__device__ __forceinline__ void dcross3(const D a[3], const D b[3], D o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ bool dedge_in(const D U[3], const D V[3], const D P[3], const D n[3]) {
+    D e[3], m[3], c[3];
+    dsub3(V, U, e);
+    dsub3(P, U, m);
+    dcross3(e, m, c);
+    return ddot(c, n).v >= 0.0f;
+}
+
+__device__ D ray_shell_dual(const float Ff[3], const float Df[3], const D A[3], const D B[3], const D C[3], float h) {
+    const D F[3] = {dconst(Ff[0]), dconst(Ff[1]), dconst(Ff[2])};
+    const D Dv[3] = {dconst(Df[0]), dconst(Df[1]), dconst(Df[2])};
+    D best = dconst(2.0f);
+    D e1[3], e2[3], n[3];
+    dsub3(B, A, e1);
+    dsub3(C, A, e2);
+    dcross3(e1, e2, n);
+    const D nl = dsqrt(ddot(n, n));
+    if (nl.v > 0.0f) {
+        for (int i = 0; i < 3; ++i) n[i] = n[i] / nl;
+        D fa[3];
+        dsub3(F, A, fa);
+        const D fn = ddot(fa, n);
+        if (fn.v <= h && fn.v >= -h && dedge_in(A, B, F, n) && dedge_in(B, C, F, n) && dedge_in(C, A, F, n))
+            return dconst(0.0f);
+        const D dn = ddot(Dv, n);
+        if (dn.v != 0.0f) {
+            for (int side = 0; side < 2; ++side) {
+                const float off = side == 0 ? h : -h;
+                const D t = (dconst(off) - fn) / dn;
+                if (t.v >= 0.0f && t.v < best.v) {
+                    const D P[3] = {F[0] + Dv[0] * t, F[1] + Dv[1] * t, F[2] + Dv[2] * t};
+                    if (dedge_in(A, B, P, n) && dedge_in(B, C, P, n) && dedge_in(C, A, P, n)) best = t;
+                }
+            }
+        }
+    }
+    const D dd = ddot(Dv, Dv);
+    for (int k = 0; k < 3; ++k) {
+        const D* u = k == 0 ? A : (k == 1 ? B : C);
+        const D* v = k == 0 ? B : (k == 1 ? C : A);
+        D e[3], m[3];
+        dsub3(v, u, e);
+        dsub3(F, u, m);
+        const D ee = ddot(e, e), de = ddot(Dv, e), me = ddot(m, e), md = ddot(m, Dv), mm = ddot(m, m);
+        const D a = ee * dd - de * de;
+        const D hb = ee * md - me * de;
+        const D c = (ee * mm - me * me) - dconst(h * h) * ee;
+        if (c.v <= 0.0f && me.v >= 0.0f && me.v <= ee.v) return dconst(0.0f);
+        if (a.v > 0.0f) {
+            const D disc = hb * hb - a * c;
+            if (disc.v >= 0.0f) {
+                const D t = (dconst(0.0f) - hb - dsqrt(disc)) / a;
+                if (t.v >= 0.0f && t.v < best.v) {
+                    const D uu = (me + t * de) / ee;
+                    if (uu.v >= 0.0f && uu.v <= 1.0f) best = t;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        const D* v = k == 0 ? A : (k == 1 ? B : C);
+        D m[3];
+        dsub3(F, v, m);
+        const D md = ddot(m, Dv);
+        const D c = ddot(m, m) - dconst(h * h);
+        if (c.v <= 0.0f) return dconst(0.0f);
+        const D disc = md * md - dd * c;
+        if (disc.v >= 0.0f) {
+            const D t = (dconst(0.0f) - md - dsqrt(disc)) / dd;
+            if (t.v >= 0.0f && t.v < best.v) best = t;
+        }
+    }
+    return best;
+}
+
+__device__ __noinline__ RayDQ ray_link_dq(const AdjointBuffers& b, const StepParams& p, int x, int y, int z, int q,
+                                          uint32_t t, int K) {
+    RayDQ r;
+    for (int k = 0; k < 8; ++k) r.v[k] = 0.0f;
+    // the link's segment as the ray pass forms it: from the fluid node of
+    // the source S along c toward S (S wrapped / clamped by the faces)
+    const int c[3] = {cx(q), cy(q), cz(q)};
+    const int xyz[3] = {x, y, z};
+    const int n[3] = {p.g.nx, p.g.ny, p.g.nz};
+    float F[3], Dr[3];
+    for (int a = 0; a < 3; ++a) {
+        int sa = xyz[a];
+        if (!ray_source(p.bc[a], xyz[a] - c[a], n[a], sa)) return r;
+        const float Sa = p.origin[a] + ((float)sa + 0.5f) * p.dx;
+        F[a] = Sa + (float)c[a] * p.dx;
+        Dr[a] = (float)(-c[a]) * p.dx;
+    }
+    if (t == NO_WINNER) return r;
+    const uint32_t iv[3] = {b.tri[3 * (size_t)t], b.tri[3 * (size_t)t + 1], b.tri[3 * (size_t)t + 2]};
+    const float4 P0 = b.vpos[iv[0]], P1 = b.vpos[iv[1]], P2 = b.vpos[iv[2]];
+    for (int k = 0; k < K; ++k) {
+        const float4 d0 = b.dvpos[(size_t)k * b.vertex_count + iv[0]];
+        const float4 d1 = b.dvpos[(size_t)k * b.vertex_count + iv[1]];
+        const float4 d2 = b.dvpos[(size_t)k * b.vertex_count + iv[2]];
+        const D A[3] = {{P0.x, d0.x}, {P0.y, d0.y}, {P0.z, d0.z}};
+        const D B[3] = {{P1.x, d1.x}, {P1.y, d1.y}, {P1.z, d1.z}};
+        const D C[3] = {{P2.x, d2.x}, {P2.y, d2.y}, {P2.z, d2.z}};
+        r.v[k] = ray_shell_dual(F, Dr, A, B, C, b.half_thickness).d;
+    }
+    return r;
 }
 
 // d wall / d theta_k of every keyed cell: wall = (sum_i b_i v_i, (|P - cp| - h) / dx)

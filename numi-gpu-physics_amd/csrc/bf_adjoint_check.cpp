@@ -80,6 +80,18 @@ void bfa_link_f64(int q, int rule, int farther_ok, double rho_wall, const double
     out[12] = d.lf;
 }
 
+// The link-table rule (lf from the table): out = value, d value / d lf, lf
+void bfa_link_table_f64(int q, int farther_ok, double rho_wall, const double* refl_farther_pin, const double* wg,
+                        const double* ws, double table_lf, double* out) {
+    const double g[4] = {wg[0], wg[1], wg[2], wg[3]}, s[4] = {ws[0], ws[1], ws[2], ws[3]};
+    const adj::LinkD<double> d = adj::link_value<double>(
+        cx(q), cy(q), cz(q), adj::wq<double>(q), rho_wall, WALL_INTERPOLATED_LINKS, farther_ok != 0,
+        refl_farther_pin[0], refl_farther_pin[1], refl_farther_pin[2], g, s, table_lf);
+    out[0] = d.value;
+    out[1] = d.dv_dlf;
+    out[2] = d.lf;
+}
+
 void bfa_feq_velocity_adjoint_f64(const double* l, const double* u, double* g) {
     double li[19];
     for (int q = 0; q < 19; ++q) li[q] = l[q];

@@ -586,13 +586,16 @@ struct LinkD {
     R d_refl, d_farther, d_pin;
     R d_wg[4], d_ws[4];   // d value / d (wall xyz, w) of the fluid cell and of the source
     R lf;
+    R dv_dlf;             // d value / d lf (the link-table rule differentiates lf itself)
     bool uses_farther, uses_pin;
 };
 
+// table_lf: the link's fraction under WALL_INTERPOLATED_LINKS (clamped like
+// the step); ignored by the other rules.
 template <typename R>
 __host__ __device__ __forceinline__ LinkD<R> link_value(int q_cx, int q_cy, int q_cz, R w_q, R rho_wall, int rule,
                                                         bool farther_ok, R reflected, R farther, R pin,
-                                                        const R (&wg)[4], const R (&ws)[4]) {
+                                                        const R (&wg)[4], const R (&ws)[4], R table_lf = R(0)) {
     LinkD<R> o;
     for (int i = 0; i < 4; ++i) o.d_wg[i] = o.d_ws[i] = R(0);
     o.d_refl = o.d_farther = o.d_pin = R(0);
@@ -609,6 +612,8 @@ __host__ __device__ __forceinline__ LinkD<R> link_value(int q_cx, int q_cy, int 
             dlf_dwgw = wg[3] > R(0) ? -ds * inv2 : R(0);
             dlf_dwsw = ws[3] < R(0) ? df * inv2 : R(0);
         }
+    } else if (interp && rule == WALL_INTERPOLATED_LINKS) {
+        lf = min_(max_(table_lf, R(1.0e-4)), R(1));
     }
     if (interp && lf <= R(0.5) && !farther_ok) {
         interp = false;   // halfway fallback
@@ -652,6 +657,7 @@ __host__ __device__ __forceinline__ LinkD<R> link_value(int q_cx, int q_cy, int 
     o.d_wg[3] = dv_dlf * dlf_dwgw;
     o.d_ws[3] = dv_dlf * dlf_dwsw;
     o.lf = lf;
+    o.dv_dlf = dv_dlf;
     return o;
 }
 

@@ -241,8 +241,23 @@ struct bf_context {
         hipEvent_t ev_tan[2] = {nullptr, nullptr};    // dW[j] written (stream C)
         hipEvent_t ev_used[2] = {nullptr, nullptr};   // dW[j] read (stream B)
         float* outlet_extra = nullptr;    // [2][19][ny nz]
-        float4* dpos = nullptr;           // [8][vertices]
-        float4* dvel = nullptr;
+        // by step parity (stream C writes step s while stream B reads step
+        // s + 1): vertex positions / velocities of the step and their
+        // tangents [8][vertices]
+        float4* vpos[2] = {nullptr, nullptr};
+        float4* vvel[2] = {nullptr, nullptr};
+        float4* dpos[2] = {nullptr, nullptr};
+        float4* dvel[2] = {nullptr, nullptr};
+        // exact-root wall rule: the recompute's vertices (stream B); per step
+        // parity the reverse step's link table and entry triangles, and
+        // stream C's own scratch of the ray pass
+        float4* rpos = nullptr;
+        float4* rvel = nullptr;
+        float* rtab[2] = {nullptr, nullptr};
+        uint32_t* rwin[2] = {nullptr, nullptr};
+        uint32_t* rdirs = nullptr;
+        uint32_t* rlist = nullptr;
+        uint32_t* rcount = nullptr;
         double* grad_partials = nullptr;  // [tiles][8]
         double* grad_total = nullptr;     // [8]
     } adj;
@@ -2207,8 +2222,8 @@ bf_status adjoint_supported(bf_context* c) {
     if (c->source == BF_GEOMETRY_SPHERE || c->source == BF_GEOMETRY_BIRD || c->source == BF_GEOMETRY_WING ||
         g.wall_rule == BF_WALL_INTERPOLATED_LINKS)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint covers mesh and static geometry");
-    if (g.wall_rule == BF_WALL_INTERPOLATED_RAY)
-        return fail(c, BF_ERROR_NOT_SUPPORTED, "the adjoint differentiates the signed-distance link fractions");
+    if (g.wall_rule == BF_WALL_INTERPOLATED_RAY && (c->source != BF_GEOMETRY_MESH || !c->ray_links[0]))
+        return fail(c, BF_ERROR_NOT_SUPPORTED, "the exact-root adjoint needs a mesh");
     if (c->source == BF_GEOMETRY_STATIC && c->static_prev_differs)
         return fail(c, BF_ERROR_NOT_SUPPORTED, "step once after bf_set_geometry with a different previous mask");
     return BF_OK;
@@ -2223,9 +2238,11 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
     // pop: one FP32 lattice (lambda); popS: one lattice in the storage's form
     const uint64_t n = c->g.n, tiles = c->g.tiles, pop = 19ull * n * 4, popS = 19ull * n * c->pop_elem;
     const uint64_t verts = std::max<uint32_t>(c->m_vertices, 1u);
+    const bool ray = c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY;
     const uint64_t need = popS * segments + popS * interval + 2 * (uint64_t)(interval + 1) * (21 * n + 7 * tiles + 4) +
-                          2 * pop + tiles * 19 * 256 * 4 + 2 * 8 * 16 * n + 2 * 8 * 16 * verts + 2 * 16 * verts +
-                          tiles * 64 + 4 * tiles + 2ull * 19 * c->g.ny * c->g.nz * 4 + 1024;
+                          2 * pop + tiles * 19 * 256 * 4 + 2 * 8 * 16 * n + 4 * 8 * 16 * verts + 6 * 16 * verts +
+                          tiles * 64 + 4 * tiles + 2ull * 19 * c->g.ny * c->g.nz * 4 + 1024 +
+                          (ray ? 2 * 16 * verts + 4 * tiles * 19 * 256 * 4 + 4 * n + 4 * tiles + 256 : 0);
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     if (need > (uint64_t)free_b) {
@@ -2265,8 +2282,29 @@ bf_status adjoint_alloc(bf_context* c, uint32_t interval, uint32_t segments) {
         if (!a.ev_tan[i]) e = hipEventCreateWithFlags(&a.ev_tan[i], hipEventDisableTiming);
         if (e == hipSuccess && !a.ev_used[i]) e = hipEventCreateWithFlags(&a.ev_used[i], hipEventDisableTiming);
     }
-    a.dpos = (float4*)A.take(e, 8 * 16 * verts);
-    a.dvel = (float4*)A.take(e, 8 * 16 * verts);
+    for (int i = 0; i < 2; ++i) {
+        a.vpos[i] = (float4*)A.take(e, 16 * verts);
+        a.vvel[i] = (float4*)A.take(e, 16 * verts);
+        a.dpos[i] = (float4*)A.take(e, 8 * 16 * verts);
+        a.dvel[i] = (float4*)A.take(e, 8 * 16 * verts);
+    }
+    a.rpos = a.rvel = nullptr;
+    a.rtab[0] = a.rtab[1] = nullptr;
+    a.rwin[0] = a.rwin[1] = nullptr;
+    a.rdirs = a.rlist = a.rcount = nullptr;
+    if (ray) {
+        a.rpos = (float4*)A.take(e, 16 * verts);
+        a.rvel = (float4*)A.take(e, 16 * verts);
+        for (int i = 0; i < 2; ++i) {
+            a.rtab[i] = (float*)A.take(e, tiles * 19 * 256 * 4);
+            a.rwin[i] = (uint32_t*)A.take(e, tiles * 19 * 256 * 4);
+        }
+        a.rdirs = (uint32_t*)A.take(e, 4 * n);
+        a.rlist = (uint32_t*)A.take(e, 4 * tiles);
+        a.rcount = (uint32_t*)A.take(e, 256);
+        // the ray pass leaves its link directions cleared; they start so
+        if (e == hipSuccess) e = launch_fill_u32(a.rdirs, 0u, (uint32_t)n, c->sb);
+    }
     a.grad_partials = (double*)A.take(e, tiles * 64);
     a.grad_total = (double*)A.take(e, 64);
     // clamped pulls of the outlet x faces: [face][19][ny nz]
@@ -2304,6 +2342,23 @@ struct SlotGeometry {
     const uint8_t* dyn;
     const uint32_t* tri;
 };
+
+// The ray pass's buffers for one step's geometry slot.
+RayBuffers ray_buffers(const bf_context* c, const SlotGeometry& gs, const float4* pos, uint32_t* list,
+                       uint32_t* count, uint32_t* dirs, float* links, uint32_t* winners) {
+    RayBuffers rb{};
+    rb.pos = pos;
+    rb.tri = c->m_tri;
+    rb.mask = gs.mask;
+    rb.wall = gs.wall;
+    rb.dyn = gs.dyn;
+    rb.list = list;
+    rb.list_count = count;
+    rb.dirs = dirs;
+    rb.links = links;
+    rb.winners = winners;
+    return rb;
+}
 
 // Geometry of step s into segment slot k: the same raster / resolve as the
 // forward pass (masks identical, walls identical wherever the step reads
@@ -2477,6 +2532,7 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
 
     // reverse sweep
     const int storage = (int)c->cfg.storage, coll = (int)c->cfg.collision, arith = arithmetic_of(c->cfg);
+    const bool ray = c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY;
     StepParams pf = make_step_params(c);
     pf.accumulate_loads = 0;     // the recompute needs populations only
     const StepParams pa = make_step_params(c);
@@ -2520,27 +2576,45 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
             fb.velocity = c->velocity;
             fb.activation = &c->counters[8];
             fb.body = c->body;
-            e = launch_fluid_fast(arith, storage, coll, fb, pf, c->sb);
-            if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, pf, c->general_blocks, c->sb);
+            StepParams ps = pf;
+            if (ray) {
+                // the exact-root table of step s, as the forward step built it
+                MeshBuffers mb = mesh_buffers(c);
+                mb.pos = a.rpos;
+                mb.vel = a.rvel;
+                e = launch_mesh_prepare(mb, mesh_params(c, time_of(c, s), false), c->sb);
+                if (e == hipSuccess) e = launch_ray_links(ray_buffers(c, gc, a.rpos, c->ray_list, c->ray_count,
+                                                                      c->ray_dirs, c->ray_links[s & 1], nullptr),
+                                                          ray_params(c), c->sb);
+                ps.links = c->ray_links[s & 1];
+            }
+            if (e == hipSuccess) e = launch_fluid_fast(arith, storage, coll, fb, ps, c->sb);
+            if (e == hipSuccess) e = launch_fluid_general(arith, storage, coll, fb, ps, c->general_blocks, c->sb);
         }
         if (e != hipSuccess) return hip_fail(c, e, "adjoint recompute");
         for (int64_t s = sb_; s > sa_; --s) {
             const uint32_t k = (uint32_t)(s - sa_);
             const SlotGeometry gp = slot_geometry(c, k - 1, set), gc = slot_geometry(c, k, set);
             const int j = (int)(s & 1);
-            if (a.K > 0) {
+            if (a.K > 0 || ray) {
                 // Stream C, up to two steps ahead: vertex positions and
-                // velocities of step s, then the wall tangents into dW[s & 1]
-                // once the step that last read it (s + 2) is done.
+                // velocities of step s, the exact-root table and its entry
+                // triangles, then the wall tangents into dW[s & 1], once the
+                // step that last read these parity-j buffers (s + 2) is done.
                 if (s == sb_) e = hipStreamWaitEvent(a.sc, a.ev_geo[set], 0);
                 if (e == hipSuccess && tangent_steps >= 2) e = hipStreamWaitEvent(a.sc, a.ev_used[j], 0);
-                const MeshBuffers mb = mesh_buffers(c);
+                MeshBuffers mb = mesh_buffers(c);
+                mb.pos = a.vpos[j];
+                mb.vel = a.vvel[j];
                 const MeshParams mp = mesh_params(c, time_of(c, s), false);
                 if (e == hipSuccess) e = launch_mesh_prepare(mb, mp, a.sc);
-                if (e == hipSuccess)
+                if (e == hipSuccess && ray)
+                    e = launch_ray_links(ray_buffers(c, gc, a.vpos[j], a.rlist, a.rcount, a.rdirs, a.rtab[j], a.rwin[j]),
+                                         ray_params(c), a.sc);
+                if (e == hipSuccess && a.K > 0)
                     e = launch_wall_tangent(a.slot_list[set] + (uint64_t)k * c->g.tiles, a.slot_count[set] + k,
-                                            c->general_blocks, gc.tri, c->m_tri, c->m_pos, c->m_vel, mb, a.dpoints,
-                                            a.kin, a.dpos, a.dvel, a.dW[j], mp, (int)a.K, a.sc);
+                                            c->general_blocks, gc.tri, c->m_tri, a.vpos[j], a.vvel[j], mb, a.dpoints,
+                                            a.kin, a.dpos[j], a.dvel[j], a.dW[j], mp, (int)a.K, a.sc);
                 if (e == hipSuccess) e = hipEventRecord(a.ev_tan[j], a.sc);
                 if (e == hipSuccess) e = hipStreamWaitEvent(c->sb, a.ev_tan[j], 0);
                 if (e != hipSuccess) return hip_fail(c, e, "wall tangents");
@@ -2564,6 +2638,13 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
             ab.grad_total = a.grad_total;
             ab.body = c->body;
             ab.outlet_extra = a.outlet_extra;
+            ab.links = a.rtab[j];
+            ab.winners = a.rwin[j];
+            ab.vpos = a.vpos[j];
+            ab.dvpos = a.dpos[j];
+            ab.tri = c->m_tri;
+            ab.vertex_count = c->m_vertices;
+            ab.half_thickness = c->m_half;
             AdjointParams w{};
             const double* lw = load_weights + 6 * (size_t)(s - s0 - 1);
             for (int i = 0; i < 3; ++i) {
@@ -2572,7 +2653,7 @@ bf_status bf_adjoint_gradient(bf_context* c, uint32_t steps, const double* load_
             }
             w.K = (int)a.K;
             e = launch_adjoint_step(coll, storage, ab, pa, w, c->general_blocks, c->sb);
-            if (e == hipSuccess && a.K > 0) e = hipEventRecord(a.ev_used[j], c->sb);
+            if (e == hipSuccess && (a.K > 0 || ray)) e = hipEventRecord(a.ev_used[j], c->sb);
             if (e != hipSuccess) return hip_fail(c, e, "adjoint step");
             a.lam_cur = 1 - a.lam_cur;
         }

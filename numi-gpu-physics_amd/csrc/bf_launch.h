@@ -208,7 +208,11 @@ struct RayBuffers {
     uint32_t* list_count;
     uint32_t* dirs;          // per cell: link directions pulling from it (zero between steps)
     float* links;            // the step's table
+    // optional (the adjoint): per table entry, the triangle whose shell
+    // gives the entry (NO_WINNER where none lies in the link); same layout
+    uint32_t* winners;
 };
+constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
 
 // Adjoint of one step (bf_adjoint.hip).  lam_in = dJ/df_s, lam_out =
 // dJ/df_{s-1}; f_prev = f_{s-1}; masks and wall of step s-1 / s; nu:
@@ -231,6 +235,18 @@ struct AdjointBuffers {
     double* grad_total;           // [8], accumulated over steps
     const double* body;
     float* outlet_extra;          // [x face 0 / 1][19][ny * nz]: clamped pulls at outlet x faces
+    // exact-root wall rule (StepParams::wall_rule is then
+    // WALL_INTERPOLATED_LINKS): the step's link table, the triangle giving
+    // each entry (NO_WINNER: the signed-distance ratio), the vertex
+    // positions of the step and their tangents [K][vertices], the triangles
+    // and the shell radius
+    const float* links;
+    const uint32_t* winners;
+    const float4* vpos;
+    const float4* dvpos;
+    const uint32_t* tri;
+    uint32_t vertex_count;
+    float half_thickness;
 };
 struct AdjointParams {
     float cF[3], cT[3];           // load weights of the step: J_s = cF . F + cT . T

@@ -43,6 +43,10 @@
 //  3. finalize: each link entry becomes the fraction (the entry when <= 1,
 //     else the signed-distance ratio), clamped; the link directions are
 //     cleared for the next step.
+// For the adjoint (RayBuffers::winners) init also clears a winner per entry
+// and a second triangles pass, before finalize, stores the triangle whose
+// entry equals the finished minimum: the adjoint differentiates that
+// triangle's root.
 // Rejection by the shell boxes never drops a triangle whose entry lies in
 // the link: an entry point lies inside its triangle's shell box, so the
 // segment box and the shell box share it.
@@ -186,7 +190,8 @@ __device__ __forceinline__ bool ray_source_cell(const RayParams& p, int x, int y
 // fluid cell pulls from.  The 18 sources' part ids are loaded together.
 __global__ __launch_bounds__(256) void ray_init(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
                                                 const uint8_t* __restrict__ mask, float* __restrict__ links,
-                                                uint32_t* __restrict__ dirs, RayParams p) {
+                                                uint32_t* __restrict__ dirs, uint32_t* __restrict__ winners,
+                                                RayParams p) {
     const uint32_t n_tiles = *count;
     for (uint32_t i = blockIdx.x; i < n_tiles; i += gridDim.x) {
         const uint32_t tile = list[i];
@@ -216,6 +221,11 @@ __global__ __launch_bounds__(256) void ray_init(const uint32_t* __restrict__ lis
             out[(size_t)q * TILE_CELLS] = link ? NO_ENTRY : 0u;
             if (link) (void)atomicOr(&dirs[src[q]], 1u << q);
         }
+        if (winners) {
+            uint32_t* w = winners + (size_t)tile * 19 * TILE_CELLS + threadIdx.x;
+#pragma unroll
+            for (int q = 0; q < 19; ++q) w[(size_t)q * TILE_CELLS] = NO_WINNER;
+        }
     }
 }
 
@@ -228,10 +238,14 @@ __global__ __launch_bounds__(256) void ray_init(const uint32_t* __restrict__ lis
 // computation, so that every lane has work.
 constexpr uint32_t RAY_CAP = 256;
 
+// WINNERS: a second pass over the finished minima that records, for each
+// entry, a triangle whose shell gives it (entries equal in bits; any of
+// several equal ones).
+template <bool WINNERS>
 __global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
                                                     const uint8_t* __restrict__ mask,
                                                     const uint32_t* __restrict__ dirs, float* __restrict__ links,
-                                                    RayParams p) {
+                                                    uint32_t* __restrict__ winners, RayParams p) {
     __shared__ uint2 cand[RAY_CAP];
     const uint32_t lane = threadIdx.x;
     const float dx = p.dx;
@@ -294,8 +308,14 @@ __global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ p
                         for (int ix = 0; ix < kx; ++ix) {
                             const uint32_t gid =
                                 (uint32_t)fx[ix] + (uint32_t)n[0] * ((uint32_t)fy[iy] + (uint32_t)n[1] * (uint32_t)fz[iz]);
-                            if (mask[gid] == 0)
-                                (void)atomicMin(&table[table_slot(p.g, fx[ix], fy[iy], fz[iz], q)], bits);
+                            if (mask[gid] == 0) {
+                                const size_t slot = table_slot(p.g, fx[ix], fy[iy], fz[iz], q);
+                                if constexpr (WINNERS) {
+                                    if (table[slot] == bits) winners[slot] = t;
+                                } else {
+                                    (void)atomicMin(&table[slot], bits);
+                                }
+                            }
                         }
             }
             __syncthreads();
@@ -401,10 +421,14 @@ hipError_t launch_ray_links(const RayBuffers& b, const RayParams& p, hipStream_t
     hipLaunchKernelGGL(ray_tile_list, dim3(tile_blocks), dim3(256), 0, s, (const uint8_t*)b.dyn, p.g.tiles, b.list,
                        b.list_count);
     hipLaunchKernelGGL(ray_init, dim3(p.tile_pass_blocks), dim3(256), 0, s, (const uint32_t*)b.list,
-                       (const uint32_t*)b.list_count, b.mask, b.links, b.dirs, p);
-    if (p.triangle_count > 0)
-        hipLaunchKernelGGL(ray_triangles, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, b.pos, b.tri, b.mask,
-                           (const uint32_t*)b.dirs, b.links, p);
+                       (const uint32_t*)b.list_count, b.mask, b.links, b.dirs, b.winners, p);
+    if (p.triangle_count > 0) {
+        hipLaunchKernelGGL(ray_triangles<false>, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, b.pos, b.tri,
+                           b.mask, (const uint32_t*)b.dirs, b.links, (uint32_t*)nullptr, p);
+        if (b.winners)
+            hipLaunchKernelGGL(ray_triangles<true>, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, b.pos, b.tri,
+                               b.mask, (const uint32_t*)b.dirs, b.links, b.winners, p);
+    }
     hipLaunchKernelGGL(ray_finalize, dim3(p.tile_pass_blocks), dim3(256), 0, s, (const uint32_t*)b.list,
                        (const uint32_t*)b.list_count, b.mask, b.wall, b.links, b.dirs, p);
     return hipGetLastError();

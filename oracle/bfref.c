@@ -33,9 +33,12 @@ const double bfo_w[19] = {
     1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
 };
 
-/* link fractions of BFO_WALL_INTERPOLATED_LINKS steps ([19][N], fluid side) */
+/* link fractions of BFO_WALL_INTERPOLATED_LINKS steps ([19][N], fluid side),
+ * FP32 or (finite differences of FP64 steps) FP64 */
 static const float* bfo_link_table = NULL;
+static const double* bfo_link_table64 = NULL;
 void bfo_set_link_table(const float* links) { bfo_link_table = links; }
+void bfo_set_link_table_f64(const double* links) { bfo_link_table64 = links; }
 
 #define R float
 #define WT float
@@ -601,6 +604,8 @@ float bfo_ray_shell(const float F[3], const float D[3], const float A[3], const 
     return best;
 }
 
+#include "bfref_ray64.inc"
+
 /* the source coordinate of a link along one axis (the step's face
  * treatment); 0: beyond a far-field face */
 static int ray_source(const int32_t bc[2], int s, int n, int* out)
@@ -616,8 +621,19 @@ static int ray_source(const int32_t bc[2], int s, int n, int* out)
     return 1;
 }
 
+void bfo_ray_links_winners(const bfo_params* p, const float* position3, const uint32_t* tri, int triangle_count,
+                           float half_thickness, const uint8_t* mask, const float* wall4, float* links,
+                           uint32_t* winners);
+
 void bfo_ray_links(const bfo_params* p, const float* position3, const uint32_t* tri, int triangle_count,
                    float half_thickness, const uint8_t* mask, const float* wall4, float* links)
+{
+    bfo_ray_links_winners(p, position3, tri, triangle_count, half_thickness, mask, wall4, links, NULL);
+}
+
+void bfo_ray_links_winners(const bfo_params* p, const float* position3, const uint32_t* tri, int triangle_count,
+                           float half_thickness, const uint8_t* mask, const float* wall4, float* links,
+                           uint32_t* winners)
 {
     const size_t n = (size_t)p->nx * p->ny * p->nz;
     const float dx = p->dx;
@@ -635,6 +651,8 @@ void bfo_ray_links(const bfo_params* p, const float* position3, const uint32_t* 
             box[6 * t + 3 + a] = hi + half_thickness + 0.01f * dx;
         }
     for (size_t i = 0; i < 19 * n; ++i) links[i] = 0.0f;
+    if (winners)
+        for (size_t i = 0; i < 19 * n; ++i) winners[i] = 0xFFFFFFFFu;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int z = 0; z < p->nz; ++z)
         for (int y = 0; y < p->ny; ++y)
@@ -661,6 +679,7 @@ void bfo_ray_links(const bfo_params* p, const float* position3, const uint32_t* 
                         hi[a] = F[a] < S[a] ? S[a] : F[a];
                     }
                     float best = 2.0f;
+                    uint32_t win = 0xFFFFFFFFu;   /* the first triangle with the least entry */
                     for (int t = 0; t < triangle_count; ++t) {
                         const float* bx = box + 6 * t;
                         if (hi[0] < bx[0] || lo[0] > bx[3] || hi[1] < bx[1] || lo[1] > bx[4] || hi[2] < bx[2] ||
@@ -668,8 +687,10 @@ void bfo_ray_links(const bfo_params* p, const float* position3, const uint32_t* 
                             continue;
                         const float r = bfo_ray_shell(F, D, position3 + 3 * tri[3 * t], position3 + 3 * tri[3 * t + 1],
                                                       position3 + 3 * tri[3 * t + 2], half_thickness);
+                        if (r < best) win = (uint32_t)t;
                         best = r < best ? r : best;
                     }
+                    if (winners && best <= 1.0f) winners[(size_t)q * n + gid] = win;
                     float qf;
                     if (best <= 1.0f) {
                         qf = best;

@@ -116,6 +116,8 @@ void bfo_step_f64w(const bfo_params* p, const double* fin, double* fout,
 /* Link fractions read by BFO_WALL_INTERPOLATED_LINKS steps: [19][N], entry
  * q * N + x for fluid x whose source x - c_q is solid (NULL: none). */
 void bfo_set_link_table(const float* links);
+/* an FP64 table instead (takes precedence while set; NULL clears) */
+void bfo_set_link_table_f64(const double* links);
 
 /* Equilibrium fill: solid cells at their wall velocity, fluid at far_u. */
 void bfo_init_equilibrium_f32(const bfo_params* p, float* f, const uint8_t* solid, const float* wall4);
@@ -178,6 +180,18 @@ void bfo_ray_links(const bfo_params* p, const float* position3, const uint32_t* 
  * enter it. */
 float bfo_ray_shell(const float F[3], const float D[3], const float A[3], const float B[3], const float C[3],
                     float h);
+/* bfo_ray_links that also reports, per link with an entry (t <= 1), the
+ * first triangle attaining the least entry (0xFFFFFFFF elsewhere, and where
+ * the signed-distance ratio applies). */
+void bfo_ray_links_winners(const bfo_params* p, const float* position3, const uint32_t* tri, int triangle_count,
+                           float half_thickness, const uint8_t* mask, const float* wall4, float* links,
+                           uint32_t* winners);
+/* bfo_ray_shell in FP64 (bfref_ray64.inc), and over count rays ([count][3]
+ * arrays). */
+double bfo_ray_shell_f64(const double F[3], const double D[3], const double A[3], const double B[3],
+                         const double C[3], double h);
+void bfo_ray_shell_f64_n(int count, const double* F, const double* D, const double* A, const double* B,
+                         const double* C, double h, double* out);
 
 /* Translating sphere geometry at time t (geometry center = c0 + v_geom * t). */
 void bfo_translating_sphere(const bfo_params* p, const float center_radius[4],

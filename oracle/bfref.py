@@ -275,6 +275,9 @@ def _declare(l):
     l.bfo_ray_links.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_float, vp, vp, vp]
     l.bfo_ray_shell.argtypes = [vp, vp, vp, vp, vp, ctypes.c_float]
     l.bfo_ray_shell.restype = ctypes.c_float
+    l.bfo_ray_links_winners.argtypes = [P, vp, vp, ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
+    l.bfo_ray_shell_f64_n.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_double, vp]
+    l.bfo_set_link_table_f64.argtypes = [vp]
     l.bfo_wing_reaction.argtypes = [vp, ctypes.POINTER(WingDynamics), ctypes.POINTER(WingDynamics), ctypes.c_float,
                                     ctypes.c_int, vp, vp]
 
@@ -787,17 +790,30 @@ def wing_links(p: Params, wing: Wing, frame, mask):
     return links
 
 
-def ray_links(p: Params, pos, tri, half_thickness, mask, wall4):
+def ray_links(p: Params, pos, tri, half_thickness, mask, wall4, winners=False):
     """Exact link fractions of the meshed shell (bfo_ray_links): [19 * N],
-    q * N + cell."""
+    q * N + cell.  winners: also the triangle giving each link's entry
+    (bfo_ray_links_winners; 0xFFFFFFFF where no entry lies in the link)."""
     n = cells(p)
     links = np.zeros(19 * n, np.float32)
     pos = np.ascontiguousarray(pos, np.float32)
     tri = np.ascontiguousarray(tri, np.uint32)
-    lib().bfo_ray_links(ctypes.byref(p), _ptr(pos), _ptr(tri), int(tri.shape[0]), np.float32(half_thickness),
-                        _ptr(np.ascontiguousarray(mask, np.uint8)), _ptr(np.ascontiguousarray(wall4, np.float32)),
-                        _ptr(links))
-    return links
+    args = (ctypes.byref(p), _ptr(pos), _ptr(tri), int(tri.shape[0]), np.float32(half_thickness),
+            _ptr(np.ascontiguousarray(mask, np.uint8)), _ptr(np.ascontiguousarray(wall4, np.float32)), _ptr(links))
+    if not winners:
+        lib().bfo_ray_links(*args)
+        return links
+    win = np.zeros(19 * n, np.uint32)
+    lib().bfo_ray_links_winners(*args, _ptr(win))
+    return links, win
+
+
+def ray_shell_f64(F, D, A, B, C, h):
+    """bfo_ray_shell in FP64 over rows of [n][3] arrays."""
+    arr = [np.ascontiguousarray(np.asarray(v, np.float64).reshape(-1, 3)) for v in (F, D, A, B, C)]
+    out = np.empty(arr[0].shape[0], np.float64)
+    lib().bfo_ray_shell_f64_n(int(out.size), *[_ptr(a) for a in arr], float(h), _ptr(out))
+    return out
 
 
 def ray_shell(F, D, A, B, C, h):
@@ -808,7 +824,15 @@ def ray_shell(F, D, A, B, C, h):
 
 
 def step_with_links(p: Params, fin, solid_prev, solid_cur, wall4, links, capture=False):
-    """One step with the BFO_WALL_INTERPOLATED_LINKS rule and this link table."""
+    """One step with the BFO_WALL_INTERPOLATED_LINKS rule and this link
+    table (FP32, or FP64 when given as float64)."""
+    if np.asarray(links).dtype == np.float64:
+        tbl = np.ascontiguousarray(links, np.float64)
+        lib().bfo_set_link_table_f64(_ptr(tbl))
+        try:
+            return step(p, fin, solid_prev, solid_cur, wall4, capture)
+        finally:
+            lib().bfo_set_link_table_f64(None)
     tbl = np.ascontiguousarray(links, np.float32)
     lib().bfo_set_link_table(_ptr(tbl))
     try:

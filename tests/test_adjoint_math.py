@@ -30,6 +30,7 @@ def lib():
     l.bfa_collide_adjoint_f64.argtypes = [ctypes.c_int, d, ctypes.c_double, ctypes.c_double, d, d]
     l.bfa_link_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, d, d, d, d]
     l.bfa_feq_velocity_adjoint_f64.argtypes = [d, d, d]
+    l.bfa_link_table_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, d, d, d, ctypes.c_double, d]
     return l
 
 
@@ -103,6 +104,36 @@ def test_symmetric_limiter_twin_matches_the_oracle(lib):
 
 
 WALL_HALFWAY, WALL_SDF, WALL_HALF = 0, 1, 2
+
+
+def test_link_table_fraction_derivative(lib):
+    # the link-table rule (the exact-root fractions): d value / d lf against
+    # central differences in lf, both Bouzidi branches, with and without the
+    # far side (halfway fallback below 1/2 without it: no dependence)
+    rng = np.random.default_rng(8)
+    seen = set()
+    for trial in range(200):
+        q = int(rng.integers(1, 19))
+        ok = int(rng.integers(0, 2))
+        x = np.ascontiguousarray(rng.uniform(0.01, 0.1, 3))
+        wg = np.ascontiguousarray(np.concatenate([rng.uniform(-0.05, 0.05, 3), [0.5]]))
+        ws = np.ascontiguousarray(np.concatenate([rng.uniform(-0.05, 0.05, 3), [-0.5]]))
+        lf = float(rng.uniform(0.02, 0.98))
+
+        def val(l):
+            o = np.zeros(3)
+            lib.bfa_link_table_f64(q, ok, 1.0, _p(x), _p(wg), _p(ws), l, _p(o))
+            return o
+
+        o = val(lf)
+        assert o[2] == (0.5 if (not ok and lf <= 0.5) else lf)      # halfway fallback reports 1/2
+        h = 1e-7
+        fd = (val(lf + h)[0] - val(lf - h)[0]) / (2 * h)
+        assert abs(fd - o[1]) <= 1e-6 * max(1.0, abs(fd)), (q, ok, lf, fd, o[1])
+        if not ok and lf <= 0.5:
+            assert o[1] == 0.0
+        seen.add((ok, lf <= 0.5))
+    assert len(seen) == 4
 
 
 @pytest.mark.parametrize("rule", [WALL_HALFWAY, WALL_SDF, WALL_HALF])

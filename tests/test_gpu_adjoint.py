@@ -6,6 +6,9 @@
   the symmetric limiter binding in many cells.
 * dJ/dtheta of the plate with FP16 / BF16 offset storage against the FP64
   reference gradient (the storage's error stated).
+* dJ/dtheta of the plate with exact-root link fractions (each fraction
+  differentiated through its entry triangle's shell root) against the FP64
+  oracle with the FP64 roots of the frozen entry triangles, bar 1e-3.
 * dJ/dtheta of a flapping plate's time-averaged lift with respect to its 8
   kinematic parameters against an FP64 oracle reference gradient: central
   differences of the FP64 step with FP64 wall fields on the same frozen link
@@ -27,6 +30,7 @@
 * the torch.autograd wrapper returns the adjoint gradient (frame tangents
   and kinematic parameters).
 """
+import dataclasses
 import sys
 
 import numpy as np
@@ -277,6 +281,124 @@ def test_adjoint_with_offset_storage_against_fp64_reference(storage):
     bar_J, bar_g = OFFSET_BARS[storage]
     assert abs(J - J0) <= bar_J * abs(J0)
     assert err <= bar_g and err32 <= 1e-4
+
+
+NO_WINNER = 0xFFFFFFFF
+
+
+def ray_table_f64(p, pos64, tri, h, wall64, links32, win):
+    """The exact-root link table in FP64 with the entry triangles frozen:
+    each link with an entry triangle takes that triangle's shell root along
+    the link, each other link the signed-distance ratio of the FP64 walls
+    (the rule's fallback).  Links as the table of the unperturbed run holds
+    them; an entry held at the table's floor 1e-4 (the fluid node in or at
+    the shell) keeps it: its fraction does not move with theta, which the
+    adjoint mirrors."""
+    nx, ny, nz = p.nx, p.ny, p.nz
+    n = nx * ny * nz
+    L = links32.reshape(19, n)
+    W = win.reshape(19, n)
+    out = np.zeros((19, n), np.float64)
+    q, g = np.nonzero(L != 0)
+    c = bfref.C[q]
+    xyz = np.stack([g % nx, (g // nx) % ny, g // (nx * ny)], 1)
+    S = xyz - c
+    for a, m in enumerate((nx, ny, nz)):
+        for side, sel in ((0, S[:, a] < 0), (1, S[:, a] >= m)):
+            bc = p.bc[a][side]
+            assert bc != bfref.BC_FARFIELD or not sel.any()
+            S[sel, a] = (m - 1 if side == 0 else 0) if bc == bfref.BC_PERIODIC else (0 if side == 0 else m - 1)
+    src = S[:, 0] + nx * (S[:, 1] + ny * S[:, 2])
+    origin = np.array([p.origin[0], p.origin[1], p.origin[2]], np.float64)
+    dx = float(p.dx)
+    F = origin + (S + 0.5) * dx + c * dx
+    D = (-c * dx).astype(np.float64)
+    hit = W[q, g] != NO_WINNER
+    floor = L[q, g] <= np.float32(1e-4)
+    root = hit & ~floor
+    t = np.asarray(tri, np.int64)[W[q[root], g[root]]]
+    r = bfref.ray_shell_f64(F[root], D[root], pos64[t[:, 0]], pos64[t[:, 1]], pos64[t[:, 2]], h)
+    val = np.empty(q.size)
+    val[hit & floor] = L[q, g][hit & floor]
+    val[root] = np.minimum(np.maximum(r, 1e-4), 1.0)
+    df = np.maximum(wall64[g, 3], 0.0)
+    ds = np.minimum(wall64[src, 3], 0.0)
+    sdf = np.minimum(np.maximum(df / np.maximum(df - ds, 1e-6), 1e-4), 1.0)
+    val[~hit] = sdf[~hit]
+    out[q, g] = val
+    return out.ravel(), int(hit.sum()), int((~hit).sum())
+
+
+def test_adjoint_of_exact_root_fractions_matches_fp64_reference():
+    # The plate with the exact-root wall rule: the link fractions are the
+    # roots of the links in the plate's shell; the adjoint differentiates
+    # each through its entry triangle (dual numbers through the shell's
+    # face / cylinder / sphere root).  Against central differences of the
+    # FP64 oracle on the frozen link sets (masks, nearest triangles, entry
+    # triangles of the unperturbed run).  Bar 1e-3.
+    steps = 30
+    p, weights = plate_case(steps)
+    # the plate off the half-cell grid: with its edges at half-integers the
+    # links through the cell centres graze its edge cylinders (distance h
+    # exactly), where the root has no derivative
+    plate = dict(PLATE, center=(32.0, 24.3, 24.15))
+    frames = lambda theta: workloads.parametric_plate(theta, steps, **plate)
+    seq, _, tan = frames(THETA0)
+    cfg = dataclasses.replace(cases.solver_config(p), wall_rule=bh.WALL_INTERPOLATED_RAY)
+    say(f"case: adjoint d(mean lift)/d theta, parametric plate {p.nx}x{p.ny}x{p.nz} RR3, {steps} steps, "
+        f"exact-root link fractions")
+    with bh.BirdFlowSimulation(cfg) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.adjoint_set_tangents(tan)
+        sim.init_equilibrium()
+        f0 = sim.read_populations()
+        J, grad, timings = sim.adjoint_gradient(steps, weights, checkpoint_interval=10)
+    masks, tris, tables, wins = [], [], [], []
+    for s in range(steps + 1):
+        t = float(seq.times[0]) + s
+        first, _, _ = bfref.frame_interval(seq.times, t)
+        pos, vel = bfref.prepare_vertices(seq.points, seq.times, first, t)
+        mask, wall, keys = bfref.raster_resolve(p, pos, vel, seq.triangles, seq.parts, seq.half_thickness, 64)
+        links, win = bfref.ray_links(p, pos, seq.triangles, seq.half_thickness, mask, wall, winners=True)
+        masks.append(mask)
+        tris.append(np.where(keys != np.iinfo(np.uint64).max, (keys & 0xFFFFFFFF).astype(np.int64), -1))
+        tables.append(links)
+        wins.append(win)
+    pl = type(p).from_buffer_copy(p)
+    pl.wall_rule = bfref.WALL_INTERPOLATED_LINKS
+    counts = []
+
+    def J64(theta):
+        _, pts64, _ = frames(theta)
+        f = np.asarray(f0, np.float64)
+        total = 0.0
+        for s in range(1, steps + 1):
+            pos, vel = bfref.interpolate_frames_f64(pts64, seq.times, float(seq.times[0]) + s)
+            wall = bfref.wall_field_f64(p, pos, vel, seq.triangles, tris[s], seq.half_thickness)
+            table, hits, falls = ray_table_f64(p, pos, seq.triangles, seq.half_thickness, wall, tables[s], wins[s])
+            counts.append((hits, falls))
+            f, loads, _, _ = bfref.step_with_links(pl, f, masks[s - 1], masks[s], wall, table)
+            total += float(np.dot(weights[s - 1], loads[:6]))
+        return total
+
+    J0 = J64(THETA0)
+    ref = np.zeros(8)
+    for k in range(8):
+        h = 1e-5
+        e = np.zeros(8)
+        e[k] = h
+        ref[k] = (J64(THETA0 + e) - J64(THETA0 - e)) / (2 * h)
+    err = np.abs(grad - ref).max() / np.abs(ref).max()
+    hits = sum(h for h, _ in counts[:steps])
+    falls = sum(f for _, f in counts[:steps])
+    say(f"  J gpu {J:.9e} vs FP64 reference {J0:.9e}; links over the run: {hits} with an entry triangle, "
+        f"{falls} signed-distance fallbacks")
+    for k, name in enumerate(workloads.PLATE_PARAMETERS):
+        say(f"  {name:14s} adjoint {grad[k]: .6e}  FP64 reference {ref[k]: .6e}")
+    say(f"  max |adjoint - reference| / max |reference| = {err:.2e}; timings {timings}")
+    assert hits > 1000
+    assert abs(J - J0) <= 1e-4 * abs(J0)
+    assert err <= 1e-3
 
 
 def frozen_links(p, seq, steps):

@@ -220,3 +220,54 @@ def test_the_reference_exact_roots(golden_dir):
             # the signed-distance ratio the production path uses is far off
             # for these links (they were selected as its outliers)
     assert worst > 0.0
+
+
+def test_fp64_shell_matches_closed_forms_and_the_fp32_shell():
+    # bfo_ray_shell_f64 (the reference for differences of the exact roots):
+    # the closed forms above, and the FP32 shell on random rays to its
+    # rounding (grazing rays, where the two may disagree on a hit, skipped)
+    f64 = lambda F, D, tri=TRI, h=0.5: float(bfref.ray_shell_f64(F, D, tri[0], tri[1], tri[2], h)[0])
+    assert f64((0.2, 0.2, 3.0), (0.0, 0.0, -4.0)) == pytest.approx((3.0 - 0.5) / 4.0, abs=1e-15)
+    assert f64((0.5, -3.0, 0.0), (0.0, 4.0, 0.0)) == pytest.approx((3.0 - 0.5) / 4.0, abs=1e-15)
+    assert f64((3.0, 0.0, 0.0), (-4.0, 0.0, 0.0)) == pytest.approx((3.0 - 1.5) / 4.0, abs=1e-15)
+    assert f64((-1.0, 0.2, 0.6), (3.0, 0.0, 0.0)) == 2.0
+    assert f64((0.2, 0.2, 0.3), (0.0, 0.0, 1.0)) == 0.0
+    rng = np.random.default_rng(19)
+    compared = 0
+    for _ in range(600):
+        tri = tuple(rng.uniform(-1.0, 1.0, 3).astype(np.float32) for _ in range(3))
+        h = np.float32(rng.uniform(0.1, 0.6))
+        F = rng.uniform(-2.0, 2.0, 3).astype(np.float32)
+        D = ((rng.uniform(-1.0, 1.0, 3) - F) * 1.2).astype(np.float32)
+        t32 = bfref.ray_shell(F, D, *tri, h)
+        t64 = f64(F.astype(np.float64), D.astype(np.float64), tuple(v.astype(np.float64) for v in tri), float(h))
+        if (t32 <= 1.0) != (t64 <= 1.0):
+            continue
+        if t32 <= 1.0:
+            assert abs(t32 - t64) <= 3e-5
+            compared += 1
+    assert compared > 150
+
+
+def test_entry_triangles_attain_the_table():
+    # bfo_ray_links_winners: for every link with an entry, its triangle's
+    # shell gives exactly the table's fraction; links without an entry (the
+    # signed-distance fallback) and non-links report none
+    p, seq, pos, mask, wall = box_case()
+    links, win = bfref.ray_links(p, pos, seq.triangles, seq.half_thickness, mask, wall, winners=True)
+    assert np.array_equal(links, bfref.ray_links(p, pos, seq.triangles, seq.half_thickness, mask, wall))
+    n = p.nx * p.ny * p.nz
+    have = np.nonzero(win != 0xFFFFFFFF)[0]
+    assert have.size > 100
+    assert (links[have] != 0).all()
+    assert (win[links == 0] == 0xFFFFFFFF).all()
+    for i in have[::7]:
+        q, g = divmod(int(i), n)
+        x, y, z = g % p.nx, (g // p.nx) % p.ny, g // (p.nx * p.ny)
+        c = np.asarray(bfref.C[q])
+        S = np.array([x, y, z]) - c                       # far-field faces: no wrap in this box case
+        F = ((S + 0.5) + c).astype(np.float32)
+        D = (-c).astype(np.float32)
+        a, b, cc = (pos[k] for k in seq.triangles[win[i]])
+        t = bfref.ray_shell(F, D, a, b, cc, np.float32(seq.half_thickness))
+        assert np.float32(min(max(t, 1e-4), 1.0)) == links[i]
