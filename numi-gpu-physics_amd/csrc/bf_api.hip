@@ -172,6 +172,14 @@ struct bf_context {
     uint32_t* ray_list = nullptr;
     uint32_t* ray_count = nullptr;
     uint32_t* ray_dirs = nullptr;                // per cell: link directions pulling from it
+    // the static triangles' entries per (source cell, q), built once per
+    // mesh when static triangles exist (BIRDFLOW_RAY_STATIC=0 walks every
+    // triangle every step)
+    DeviceAlloc ray_static_alloc;
+    uint4* ray_static = nullptr;
+    uint32_t ray_static_count = 0;
+    bool ray_static_ready = false;
+    bool ray_static_enabled = true;
     // analytic bird: shape and the prepared frame of each step parity
     BirdShape bird{};
     float4* bird_frame = nullptr;         // [2][BIRD_FRAME_ROWS]
@@ -510,6 +518,16 @@ bool use_static_keys(const bf_context* c) {
     return c->static_keys_enabled && c->m_static_count > 0 && !c->move_mesh;
 }
 
+// The static cache for a plain ray pass (the adjoint's pass with winners
+// walks every triangle).
+void set_ray_static(const bf_context* c, RayBuffers& rb) {
+    if (!c->ray_static_ready || c->ray_static_count == 0) return;
+    rb.static_entries = c->ray_static;
+    rb.static_count = c->ray_static_count;
+    rb.dynamic_list = c->m_dyn_list;
+    rb.dynamic_count = c->m_dyn_count;
+}
+
 // The cached keys of the static triangles within the stepping band: the
 // static triangles rastered alone into the (empty) key grid, then collected
 // into (cell, key) pairs, which empties the grid again.  Once per mesh
@@ -554,6 +572,65 @@ bf_status ensure_static_keys(bf_context* c) {
     return BF_OK;
 }
 
+// The exact-root rule's static cache: entries of the static triangles'
+// shells for every direction of every source cell near them, from the
+// prepared vertices in m_pos (static vertices do not move).  Needs a
+// scratch table of 76 B per cell while it builds; without the room, or
+// without static triangles, the pass walks every triangle.
+bf_status ensure_ray_static(bf_context* c) {
+    if (c->ray_static_ready) return BF_OK;
+    c->ray_static_ready = true;
+    c->ray_static_alloc.release();
+    c->ray_static = nullptr;
+    c->ray_static_count = 0;
+    if (!c->ray_static_enabled || !use_static_keys(c)) return BF_OK;
+    const uint64_t scratch_bytes = 19ull * c->g.n * 4;
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    if (scratch_bytes + (64ull << 20) > (uint64_t)free_b) return BF_OK;
+    hipStream_t st = c->sb;
+    DeviceAlloc scratch;                                  // released before every return below
+    hipError_t e = hipSuccess;
+    uint32_t* table = (uint32_t*)scratch.take(e, scratch_bytes);
+    uint32_t* counter = (uint32_t*)scratch.take(e, 256);
+    if (e != hipSuccess) {                                // no room: walk every triangle
+        scratch.release();
+        return BF_OK;
+    }
+    const RayParams rp = ray_params(c);
+    uint32_t n = 0;
+    // 19 N exceeds 32 bits at 1024 x 512 x 512: fill in chunks
+    for (uint64_t off = 0, total = 19ull * c->g.n; off < total && e == hipSuccess; off += 1ull << 30)
+        e = launch_fill_u32(table + off, 0x40000000u, (uint32_t)std::min<uint64_t>(1ull << 30, total - off), st);
+    if (e == hipSuccess)
+        e = launch_ray_static(c->m_pos, c->m_tri, c->m_static_list, c->m_static_count, table, nullptr, 0, counter, rp,
+                              st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, counter, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && n > 0) {
+        c->ray_static = (uint4*)c->ray_static_alloc.take(e, 16ull * n);
+        if (e == hipSuccess)
+            e = launch_ray_static(c->m_pos, c->m_tri, c->m_static_list, c->m_static_count, table, c->ray_static, n,
+                                  counter, rp, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipErrorOutOfMemory) {                   // no room for the list: walk every triangle
+            (void)hipGetLastError();
+            c->ray_static_alloc.release();
+            c->ray_static = nullptr;
+            scratch.release();
+            return BF_OK;
+        }
+    }
+    scratch.release();
+    if (e != hipSuccess) {
+        c->ray_static_alloc.release();
+        c->ray_static = nullptr;
+        return hip_fail(c, e, "exact-root static cache");
+    }
+    c->ray_static_count = n;
+    return BF_OK;
+}
+
 bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
     hipError_t e = hipSuccess;
     if (c->source == BF_GEOMETRY_MESH) {
@@ -580,6 +657,7 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
         if (e == hipSuccess && c->frozen_count && s >= c->frozen_first && s < c->frozen_first + c->frozen_count)
             e = replay_mask(c, s, (uint32_t)(s - c->frozen_first), mod3(s));
         if (e == hipSuccess && c->cfg.wall_rule == BF_WALL_INTERPOLATED_RAY) {
+            if (bf_status r = ensure_ray_static(c)) return r;
             RayBuffers rb{};
             rb.pos = c->m_pos;
             rb.tri = c->m_tri;
@@ -590,6 +668,7 @@ bf_status enqueue_geometry(bf_context* c, int64_t s, bool exact, bool fresh) {
             rb.list_count = c->ray_count;
             rb.dirs = c->ray_dirs;
             rb.links = c->ray_links[par2(s)];
+            set_ray_static(c, rb);
             e = launch_ray_links(rb, ray_params(c), c->sb);
         }
     } else if (c->source == BF_GEOMETRY_BIRD) {
@@ -785,8 +864,8 @@ bf_status bf_device_info(char* name, size_t cap, uint64_t* total) {
 uint64_t bf_allocated_bytes(const bf_context* c) {
     if (!c) return 0;
     return c->alloc.bytes + c->mesh_alloc.bytes + c->static_alloc.bytes + c->ledger_alloc.bytes + c->wing_alloc.bytes +
-           c->ray_alloc.bytes + c->keyframe_alloc.bytes + c->adj.alloc.bytes + c->adj.tangent_alloc.bytes +
-           c->frozen_alloc.bytes;
+           c->ray_alloc.bytes + c->ray_static_alloc.bytes + c->keyframe_alloc.bytes + c->adj.alloc.bytes +
+           c->adj.tangent_alloc.bytes + c->frozen_alloc.bytes;
 }
 
 bf_status bf_create(const bf_config* cfg, bf_context** out) {
@@ -858,6 +937,7 @@ bf_status bf_create(const bf_config* cfg, bf_context** out) {
     if (const char* v = std::getenv("BIRDFLOW_PAIR_CELLS")) c->pair_cells = std::atoi(v) != 0;
     // meshes: cached keys of static triangles (0 rasters every triangle every step)
     if (const char* v = std::getenv("BIRDFLOW_STATIC_KEYS")) c->static_keys_enabled = std::atoi(v) != 0;
+    if (const char* v = std::getenv("BIRDFLOW_RAY_STATIC")) c->ray_static_enabled = std::atoi(v) != 0;
     if (const char* v = std::getenv("BIRDFLOW_GENERAL_PRIORITY"))
         c->general_priority = std::min(3, std::max(0, std::atoi(v)));
     if (const char* v = std::getenv("BIRDFLOW_GEOMETRY_AFTER_GENERAL")) c->geometry_after_general = std::atoi(v) != 0;
@@ -992,6 +1072,7 @@ void bf_destroy(bf_context* c) {
     c->keyframe_alloc.release();
     c->wing_alloc.release();
     c->ray_alloc.release();
+    c->ray_static_alloc.release();
     c->adj.alloc.release();
     c->adj.tangent_alloc.release();
     c->frozen_alloc.release();
@@ -1311,6 +1392,10 @@ static bf_status install_mesh(bf_context* c, const MeshSource& src, uint32_t ver
     c->static_cells = nullptr;
     c->static_keys = nullptr;
     c->static_key_count = 0;
+    c->ray_static_alloc.release();
+    c->ray_static = nullptr;
+    c->ray_static_count = 0;
+    c->ray_static_ready = false;
     c->adj.tangent_alloc.release();   // tangents and parameters belong to the previous mesh
     c->adj.K = 0;
     c->adj.dpoints = nullptr;
@@ -2357,6 +2442,7 @@ RayBuffers ray_buffers(const bf_context* c, const SlotGeometry& gs, const float4
     rb.dirs = dirs;
     rb.links = links;
     rb.winners = winners;
+    if (!winners) set_ray_static(c, rb);
     return rb;
 }
 

@@ -211,8 +211,22 @@ struct RayBuffers {
     // optional (the adjoint): per table entry, the triangle whose shell
     // gives the entry (NO_WINNER where none lies in the link); same layout
     uint32_t* winners;
+    // optional static cache (launch_ray_static): the static triangles'
+    // entries per (source cell, q), merged each step, so that only the
+    // moving triangles are walked
+    const uint4* static_entries;
+    uint32_t static_count;
+    const uint32_t* dynamic_list;
+    uint32_t dynamic_count;
 };
 constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
+// Builds the static cache: scratch_table ([19][N] u32, filled with the bits
+// of 2.0f by the caller) takes the static triangles' entries per source
+// cell and direction, then its entries are listed in out (at most capacity;
+// *count receives their number, which may exceed capacity).
+hipError_t launch_ray_static(const float4* pos, const uint32_t* tri, const uint32_t* static_list,
+                             uint32_t static_count, uint32_t* scratch_table, uint4* out, uint32_t capacity,
+                             uint32_t* count, const RayParams& p, hipStream_t s);
 
 // Adjoint of one step (bf_adjoint.hip).  lam_in = dJ/df_s, lam_out =
 // dJ/df_{s-1}; f_prev = f_{s-1}; masks and wall of step s-1 / s; nu:

@@ -238,11 +238,16 @@ __global__ __launch_bounds__(256) void ray_init(const uint32_t* __restrict__ lis
 // computation, so that every lane has work.
 constexpr uint32_t RAY_CAP = 256;
 
-// WINNERS: a second pass over the finished minima that records, for each
-// entry, a triangle whose shell gives it (entries equal in bits; any of
-// several equal ones).
-template <bool WINNERS>
+// Modes: RAY_MIN merges entries into the step's table; RAY_WINNERS, a second
+// pass over the finished minima, records for each entry a triangle whose
+// shell gives it (entries equal in bits; any of several equal ones);
+// RAY_STATIC (once per mesh, over the static triangles) merges every
+// direction of every source cell, linked or not, into a per-source table
+// [19][N] (the static cache).  list: the triangles to walk (NULL: all).
+enum { RAY_MIN = 0, RAY_WINNERS = 1, RAY_STATIC = 2 };
+template <int MODE>
 __global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ pos, const uint32_t* __restrict__ tri,
+                                                    const uint32_t* __restrict__ list, uint32_t list_count,
                                                     const uint8_t* __restrict__ mask,
                                                     const uint32_t* __restrict__ dirs, float* __restrict__ links,
                                                     uint32_t* __restrict__ winners, RayParams p) {
@@ -252,7 +257,8 @@ __global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ p
     const float h = p.half_thickness;
     const int n[3] = {p.g.nx, p.g.ny, p.g.nz};
     uint32_t* table = (uint32_t*)links;
-    for (uint32_t t = blockIdx.x; t < p.triangle_count; t += gridDim.x) {
+    for (uint32_t i = blockIdx.x; i < list_count; i += gridDim.x) {
+        const uint32_t t = list ? list[i] : i;
         const float4 a4 = pos[tri[3 * (size_t)t]], b4 = pos[tri[3 * (size_t)t + 1]], c4 = pos[tri[3 * (size_t)t + 2]];
         const float A[3] = {a4.x, a4.y, a4.z}, B[3] = {b4.x, b4.y, b4.z}, C[3] = {c4.x, c4.y, c4.z};
         // the shell's box (the oracle's rejection box)
@@ -297,6 +303,12 @@ __global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ p
                 // minimum; a link entry is then never 0, the init value of
                 // the entries that are not links)
                 const uint32_t bits = __float_as_uint(fmaxf(v, 1.0e-4f));
+                if constexpr (MODE == RAY_STATIC) {
+                    const uint32_t src =
+                        (uint32_t)sx + (uint32_t)n[0] * ((uint32_t)sy + (uint32_t)n[1] * (uint32_t)sz);
+                    (void)atomicMin(&table[(size_t)q * p.g.n + src], bits);
+                    continue;
+                }
                 // the fluid cells pulling from S along c (up to 8 with wrap
                 // and clamps)
                 int fx[2], fy[2], fz[2];
@@ -310,7 +322,7 @@ __global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ p
                                 (uint32_t)fx[ix] + (uint32_t)n[0] * ((uint32_t)fy[iy] + (uint32_t)n[1] * (uint32_t)fz[iz]);
                             if (mask[gid] == 0) {
                                 const size_t slot = table_slot(p.g, fx[ix], fy[iy], fz[iz], q);
-                                if constexpr (WINNERS) {
+                                if constexpr (MODE == RAY_WINNERS) {
                                     if (table[slot] == bits) winners[slot] = t;
                                 } else {
                                     (void)atomicMin(&table[slot], bits);
@@ -330,7 +342,9 @@ __global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ p
                 const uint32_t r = k / (uint32_t)ext[0];
                 sy = lo[1] + (int)(r % (uint32_t)ext[1]);
                 sz = lo[2] + (int)(r / (uint32_t)ext[1]);
-                bits = dirs[(uint32_t)sx + (uint32_t)n[0] * ((uint32_t)sy + (uint32_t)n[1] * (uint32_t)sz)];
+                bits = MODE == RAY_STATIC
+                           ? 0x7FFFEu
+                           : dirs[(uint32_t)sx + (uint32_t)n[0] * ((uint32_t)sy + (uint32_t)n[1] * (uint32_t)sz)];
             }
             float S[3] = {0.0f, 0.0f, 0.0f};
             if (bits) {
@@ -363,6 +377,51 @@ __global__ __launch_bounds__(64) void ray_triangles(const float4* __restrict__ p
         }
         if (count > 0) flush();
     }
+}
+
+// Static cache, built once per mesh: the per-source table's entries (a
+// triangle's shell entered within the link) as (source cell, q, bits);
+// order of the list is free (the step merges them by minimum).
+__global__ __launch_bounds__(256) void ray_static_compact(const uint32_t* __restrict__ table, uint64_t n,
+                                                          uint4* __restrict__ out, uint32_t capacity,
+                                                          uint32_t* __restrict__ count) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 19ull * n) return;
+    const uint32_t v = table[i];
+    if (v == NO_ENTRY) return;
+    const uint32_t k = atomicAdd(count, 1u);
+    if (k < capacity) out[k] = make_uint4((uint32_t)(i % n), (uint32_t)(i / n), v, 0u);
+}
+
+// Each step: the static triangles' entries of the links this step has (the
+// source's link direction set by init), merged into every fluid puller's
+// entry like the triangles pass merges them.
+__global__ __launch_bounds__(256) void ray_static_seed(const uint4* __restrict__ entries, uint32_t count,
+                                                       const uint8_t* __restrict__ mask,
+                                                       const uint32_t* __restrict__ dirs, float* __restrict__ links,
+                                                       RayParams p) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint4 e = entries[i];
+    const uint32_t src = e.x;
+    const int q = (int)e.y;
+    if (!((dirs[src] >> q) & 1u)) return;
+    const int n[3] = {p.g.nx, p.g.ny, p.g.nz};
+    const int sx = (int)(src % (uint32_t)n[0]), sy = (int)((src / (uint32_t)n[0]) % (uint32_t)n[1]),
+              sz = (int)(src / ((uint32_t)n[0] * (uint32_t)n[1]));
+    const int c[3] = {cx(q), cy(q), cz(q)};
+    int fx[2], fy[2], fz[2];
+    const int kx = pullers(p.bc[0], sx, c[0], n[0], fx);
+    const int ky = pullers(p.bc[1], sy, c[1], n[1], fy);
+    const int kz = pullers(p.bc[2], sz, c[2], n[2], fz);
+    uint32_t* table = (uint32_t*)links;
+    for (int iz = 0; iz < kz; ++iz)
+        for (int iy = 0; iy < ky; ++iy)
+            for (int ix = 0; ix < kx; ++ix) {
+                const uint32_t gid =
+                    (uint32_t)fx[ix] + (uint32_t)n[0] * ((uint32_t)fy[iy] + (uint32_t)n[1] * (uint32_t)fz[iz]);
+                if (mask[gid] == 0) (void)atomicMin(&table[table_slot(p.g, fx[ix], fy[iy], fz[iz], q)], e.z);
+            }
 }
 
 // 3. fractions: the entry when it lies in the link, else the signed-distance
@@ -422,15 +481,43 @@ hipError_t launch_ray_links(const RayBuffers& b, const RayParams& p, hipStream_t
                        b.list_count);
     hipLaunchKernelGGL(ray_init, dim3(p.tile_pass_blocks), dim3(256), 0, s, (const uint32_t*)b.list,
                        (const uint32_t*)b.list_count, b.mask, b.links, b.dirs, b.winners, p);
-    if (p.triangle_count > 0) {
-        hipLaunchKernelGGL(ray_triangles<false>, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, b.pos, b.tri,
-                           b.mask, (const uint32_t*)b.dirs, b.links, (uint32_t*)nullptr, p);
+    // the static cache serves the plain pass; the adjoint's pass with
+    // winners walks every triangle
+    const bool cached = b.static_entries && !b.winners;
+    if (cached && b.static_count > 0)
+        hipLaunchKernelGGL(ray_static_seed, dim3((b.static_count + 255) / 256), dim3(256), 0, s, b.static_entries,
+                           b.static_count, b.mask, (const uint32_t*)b.dirs, b.links, p);
+    const uint32_t* walk = cached ? b.dynamic_list : nullptr;
+    const uint32_t walk_count = cached ? b.dynamic_count : p.triangle_count;
+    if (walk_count > 0) {
+        hipLaunchKernelGGL(ray_triangles<RAY_MIN>, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, b.pos, b.tri,
+                           walk, walk_count, b.mask, (const uint32_t*)b.dirs, b.links, (uint32_t*)nullptr, p);
         if (b.winners)
-            hipLaunchKernelGGL(ray_triangles<true>, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, b.pos, b.tri,
-                               b.mask, (const uint32_t*)b.dirs, b.links, b.winners, p);
+            hipLaunchKernelGGL(ray_triangles<RAY_WINNERS>, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, b.pos,
+                               b.tri, walk, walk_count, b.mask, (const uint32_t*)b.dirs, b.links, b.winners, p);
     }
     hipLaunchKernelGGL(ray_finalize, dim3(p.tile_pass_blocks), dim3(256), 0, s, (const uint32_t*)b.list,
                        (const uint32_t*)b.list_count, b.mask, b.wall, b.links, b.dirs, p);
+    return hipGetLastError();
+}
+
+}  // namespace bf
+
+namespace bf {
+
+hipError_t launch_ray_static(const float4* pos, const uint32_t* tri, const uint32_t* static_list,
+                             uint32_t static_count, uint32_t* scratch_table, uint4* out, uint32_t capacity,
+                             uint32_t* count, const RayParams& p, hipStream_t s) {
+    if (p.split < 1 || p.split > 64 || p.triangle_blocks < 1) return hipErrorInvalidValue;
+    if (p.g.nx > 65535 || p.g.ny > 65535 || p.g.nz >= (1 << 27)) return hipErrorInvalidValue;
+    const uint64_t entries = 19ull * p.g.n;
+    hipLaunchKernelGGL(ray_clear_count, dim3(1), dim3(64), 0, s, count);
+    if (static_count > 0)
+        hipLaunchKernelGGL(ray_triangles<RAY_STATIC>, dim3(p.triangle_blocks, p.split), dim3(64), 0, s, pos, tri,
+                           static_list, static_count, (const uint8_t*)nullptr, (const uint32_t*)nullptr,
+                           (float*)scratch_table, (uint32_t*)nullptr, p);
+    hipLaunchKernelGGL(ray_static_compact, dim3((unsigned)((entries + 255) / 256)), dim3(256), 0, s,
+                       (const uint32_t*)scratch_table, p.g.n, out, capacity, count);
     return hipGetLastError();
 }
 
