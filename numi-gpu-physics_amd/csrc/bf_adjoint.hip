@@ -439,7 +439,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
     const float4 wg4 = b.wall[gid];
     const float wgv[4] = {wg4.x, wg4.y, wg4.z, wg4.w};
     float gxw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    // Round 1, unconditional (a far-field source or a far side outside the
+    // Load pass 1, unconditional (a far-field source or a far side outside the
     // domain reads this cell and is ignored): the part ids of every source
     // and far side, the cell's previous populations, its adjoint slots (kept
     // in registers and stored once at the end: only this cell writes them
@@ -482,7 +482,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
         fartherbits |= (uint32_t)(((okbits >> q) & 1u) && fpart[q] == 0) << q;
         selbits |= (uint32_t)(sel == 0 || sel == (int)part[q]) << q;
     });
-    // Round 2, three directions at a time: each link's source wall entry
+    // Load pass 2, three directions at a time: each link's source wall entry
     // and far-side population loaded together; then the wall-field tangents
     // of the group's link sources.
     auto link_group = [&](auto seq, auto first_c, uint32_t gmask) {

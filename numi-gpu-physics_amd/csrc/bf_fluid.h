@@ -1081,7 +1081,7 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
             fallbits |= (uint32_t)(!((okbits >> q) & 1u) || fpart[q] != 0) << q;
             selbits |= (uint32_t)(sel == 0 || sel == (int)partq[q]) << q;
         });
-        // Round 3, six directions at a time: the wall entry of each link's
+        // Load pass 3, six directions at a time: the wall entry of each link's
         // source and the population on its far side, all loaded before any is
         // used (a direction without a link reads this cell's entries and
         // ignores them); then the values of the group's directions in
