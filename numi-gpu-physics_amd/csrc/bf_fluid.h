@@ -1022,7 +1022,7 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
         const float mwd = p.far_rho;
         const bool interpolated_rule = p.wall_rule != WALL_HALFWAY;
         const float4 wg = wall[gid];
-        // Loads in two rounds.  First, unconditionally and without a branch
+        // Loads in two passes.  First, unconditionally and without a branch
         // between them (a far-field source, or a far side outside the
         // domain, reads this cell and is ignored): every source's part id and
         // population, this cell's own populations (reflected and x-side
@@ -1081,7 +1081,7 @@ __device__ __forceinline__ void general_cell(const typename Store<S>::T* __restr
             fallbits |= (uint32_t)(!((okbits >> q) & 1u) || fpart[q] != 0) << q;
             selbits |= (uint32_t)(sel == 0 || sel == (int)partq[q]) << q;
         });
-        // Load pass 3, six directions at a time: the wall entry of each link's
+        // Second load pass, six directions at a time: the wall entry of each link's
         // source and the population on its far side, all loaded before any is
         // used (a direction without a link reads this cell's entries and
         // ignores them); then the values of the group's directions in
