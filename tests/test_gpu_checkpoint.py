@@ -13,6 +13,7 @@
 * a corrupted populations file is refused with CheckpointError.
 """
 import os
+import dataclasses
 import sys
 
 import numpy as np
@@ -85,6 +86,19 @@ def test_restart_of_flapping_plate_is_exact(storage, tmp_path):
     p = cases.plate_params()
     seq = plate_sequence(12)
     with bh.BirdFlowSimulation(cases.solver_config(p, storage=storage)) as sim:
+        sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+        sim.init_equilibrium()
+        check_restart(sim, tmp_path, 5, 4)
+
+
+def test_restart_under_exact_root_rule_is_exact(tmp_path):
+    # the exact-root wall rule is part of the saved configuration; its
+    # tables are rebuilt from the geometry at every step
+    say("case: checkpoint of a flapping plate 96x48x48, exact-root link fractions, 5 + 4 steps")
+    p = cases.plate_params()
+    seq = plate_sequence(12)
+    cfg = dataclasses.replace(cases.solver_config(p), wall_rule=bh.WALL_INTERPOLATED_RAY)
+    with bh.BirdFlowSimulation(cfg) as sim:
         sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
         sim.init_equilibrium()
         check_restart(sim, tmp_path, 5, 4)

@@ -12,7 +12,11 @@ oracle's bfo_ray_links / step with that table, bit for bit (strict FP32):
 * the flapping plate (96 x 48 x 48, RR3, inflow / outlet, periodic y / z):
   table every 5 steps and populations and loads after 20 steps;
 * the synthetic bird scaled to 256 x 128 x 128 (40,000 triangles, moving
-  wings): table, populations and loads over 3 steps.
+  wings): table, populations and loads over 3 steps;
+* the articulated synthetic bird (kinematics on the device; its static
+  body and tail entries cached once) at 128 x 64 x 64, 3 steps, bit for bit;
+* FP16 offset storage under the rule: identical tables, force history
+  within 0.5 % of FP32.
 """
 import dataclasses
 import json
@@ -178,6 +182,82 @@ def test_synthetic_bird_matches_oracle():
                           accumulate_loads=True, far_u=(0.04, 0.0, 0.0), body_position=center)
     say(f"case: synthetic bird {shape}, {len(seq.triangles)} triangles, RR3, {steps} steps, exact roots vs oracle")
     run_pair(p, seq, steps, check_every=1)
+
+
+def test_articulated_bird_matches_oracle():
+    # the exact-root rule on an articulated mesh (kinematics on the device):
+    # its rigid joint's triangles are static, so the pass merges their cached
+    # entries and walks only the wings; bit for bit against the oracle
+    steps = 3
+    shape = (128, 64, 64)
+    s = shape[0] / 512.0
+    center = (160.0 * s, 128.0 * s, 128.0 * s)
+    mesh = workloads.synthetic_bird_articulated(seed=7, center=center, body_radii=(40.0 * s, 14.0 * s, 14.0 * s),
+                                                half_span=100.0 * s, root_chord=40.0 * s, tip_chord=12.0 * s,
+                                                tail=(30.0 * s, 20.0 * s), period=1200.0 * s)
+    tau = 0.52
+    p = bfref.make_params(shape, omega_plus=1.0 / tau, omega_minus=1.0 / (0.5 + (3.0 / 16.0) / (tau - 0.5)),
+                          bc=((bfref.BC_FARFIELD, bfref.BC_OUTLET), (bfref.BC_PERIODIC, bfref.BC_PERIODIC),
+                              (bfref.BC_PERIODIC, bfref.BC_PERIODIC)),
+                          collision=bfref.COLLISION_RR3, wall_rule=bfref.WALL_INTERPOLATED_SDF, force_mode=6,
+                          accumulate_loads=True, far_u=(0.04, 0.0, 0.0), body_position=center)
+    k = cases.oracle_kinematics(mesh)
+    say(f"case: articulated bird {shape}, {len(mesh.triangles)} triangles "
+        f"({int((mesh.vertex_joint[mesh.triangles] == 0).all(axis=1).sum())} static), RR3, {steps} steps, exact roots")
+
+    def geometry(t):
+        pos, vel = bfref.articulated_vertices(mesh.rest, mesh.vertex_joint, mesh.span_fraction, k, t)
+        mask, wall, _ = bfref.raster_resolve(p, pos, vel, mesh.triangles, mesh.parts, mesh.half_thickness, 64)
+        return mask, wall, bfref.ray_links(p, pos, mesh.triangles, mesh.half_thickness, mask, wall)
+
+    pl = links_params(p)
+    mask0, wall0, _ = geometry(0.0)
+    f = bfref.init_equilibrium(p, mask0, wall0, np.float32)
+    prev = mask0
+    loads_o = []
+    with bh.BirdFlowSimulation(ray_config(p)) as sim:
+        sim.upload_articulated(mesh, 0.0)
+        sim.init_equilibrium()
+        for st in range(1, steps + 1):
+            sim.advance(1)
+            mask, wall, links = geometry(float(st))
+            f, ld, _, _ = bfref.step_with_links(pl, f, prev, mask, wall, links)
+            loads_o.append(ld)
+            prev = mask
+            assert np.array_equal(sim.read_geometry()[0], mask), st
+            compare_tables(f"step {st}", sim.ray_links(), links, mask)
+        g = sim.read_populations()
+        loads = sim.load_history(steps)
+    loads_o = np.array(loads_o)
+    differing = int((g.view(np.uint32) != f.view(np.uint32)).sum())
+    rel = float(np.abs(loads[:, :6] - loads_o[:, :6]).max() / np.abs(loads_o[:, :6]).max())
+    say(f"  after {steps} steps: populations differing {differing}, loads max rel err {rel:.2e}")
+    assert differing == 0 and rel <= 1e-12
+
+
+def test_offset_storage_with_exact_roots_tracks_fp32():
+    # FP16 offsets under the exact-root rule: the same tables (they depend
+    # on the geometry only) and a force history within 0.5 % of the FP32
+    # run's largest force over 20 steps of the flapping plate
+    steps = 20
+    p = cases.plate_params()
+    seq = workloads.flapping_plate(steps + 1, center=(24.0, 24.0, 24.0), span=20.0, chord=8.0, thickness=2.0,
+                                   heave=2.0, pitch_deg=20.0, period=200.0, half_thickness=0.5)
+    say(f"case: flapping plate, exact roots, FP16 offsets against FP32, {steps} steps")
+    out = {}
+    for storage in (bh.STORAGE_FP32, bh.STORAGE_FP16_OFFSET):
+        with bh.BirdFlowSimulation(ray_config(p, storage)) as sim:
+            sim.upload_mesh(seq.points, seq.times, seq.triangles, seq.parts, seq.half_thickness)
+            sim.init_equilibrium()
+            sim.advance(steps)
+            out[storage] = (sim.ray_links(), sim.load_history(steps))
+    t32, l32 = out[bh.STORAGE_FP32]
+    t16, l16 = out[bh.STORAGE_FP16_OFFSET]
+    force = float(np.abs(l16[:, :3] - l32[:, :3]).max() / np.abs(l32[:, :3]).max())
+    say(f"  tables identical: {np.array_equal(t16.view(np.uint32), t32.view(np.uint32))}; force history max diff / "
+        f"max force {force:.2e}")
+    assert np.array_equal(t16.view(np.uint32), t32.view(np.uint32))
+    assert force <= 5e-3
 
 
 def test_ray_rule_refusals():
