@@ -367,7 +367,9 @@ __device__ __noinline__ RayDQ ray_link_dq(const AdjointBuffers& b, const StepPar
 // ---------------------------------------------------------------------------
 // Wall tiles, pass 2a: the cell's own populations and the wall gradient.
 // ---------------------------------------------------------------------------
-template <int S>
+// RAY instantiates the exact-root contraction; without it the kernel holds no
+// call (a call site reserves a 504-byte stack per lane and halves occupancy).
+template <int S, bool RAY>
 __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const StepParams& p,
                                                  const AdjointParams& w, int x, int y, int z, float bx, float by,
                                                  float bz, const float* __restrict__ nu_cell, double (&acc)[8]) {
@@ -521,18 +523,24 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
             const float refl = fown[opp(q)];
             const float farther = farther_ok ? fo[q - q0] : 0.0f;
             const float pin = fown[q];
-            float tlf;
-            uint32_t win;
-            const int rule = link_rule(b, p, x, y, z, q, tlf, win);
+            float tlf = 0.0f;
+            uint32_t win = NO_WINNER;
+            int rule = p.wall_rule;
+            if constexpr (RAY)
+                rule = link_rule(b, p, x, y, z, q, tlf, win);
+            else
+                __builtin_assume(rule != WALL_INTERPOLATED_LINKS);
             const adj::LinkD<float> d = adj::link_value<float>(cx(q), cy(q), cz(q), weight<float>(q), p.far_rho,
                                                                rule, farther_ok, refl, farther, pin, wgv, wsv, tlf);
             const float nu = nuv[q];
             // exact-root fraction of an unclamped entry: d lf / d theta from
             // the entry triangle's shell
-            if (rule == WALL_INTERPOLATED_LINKS && w.K > 0 && tlf > 1.0e-4f && tlf < 1.0f && d.dv_dlf != 0.0f) {
-                const RayDQ r = ray_link_dq(b, p, x, y, z, q, win, w.K);
-                const double sc = (double)(nu * d.dv_dlf);
-                for (int k = 0; k < w.K; ++k) acc[k] += sc * (double)r.v[k];
+            if constexpr (RAY) {
+                if (rule == WALL_INTERPOLATED_LINKS && w.K > 0 && tlf > 1.0e-4f && tlf < 1.0f && d.dv_dlf != 0.0f) {
+                    const RayDQ r = ray_link_dq(b, p, x, y, z, q, win, w.K);
+                    const double sc = (double)(nu * d.dv_dlf);
+                    for (int k = 0; k < w.K; ++k) acc[k] += sc * (double)r.v[k];
+                }
             }
             float refl_load = 0.0f;
             if (p.accumulate_loads && p.load_component != 2 && ((selbits >> q) & 1u)) {
@@ -569,7 +577,7 @@ __device__ __forceinline__ void general_p2a_cell(const AdjointBuffers& b, const 
     contract(acc, w.K, b.dW, n, gid, gxw[0], gxw[1], gxw[2], gxw[3]);
 }
 
-template <int S>
+template <int S, bool RAY>
 __global__ __launch_bounds__(256) void adjoint_general_p2a(AdjointBuffers b, StepParams p, AdjointParams w) {
     const uint32_t count = *b.general_count;
     const float bx = (float)b.body[0], by = (float)b.body[1], bz = (float)b.body[2];
@@ -580,7 +588,7 @@ __global__ __launch_bounds__(256) void adjoint_general_p2a(AdjointBuffers b, Ste
         const int y = y0 + (int)(threadIdx.x / TILE_X);
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (x < p.g.nx && y < p.g.ny)
-            general_p2a_cell<S>(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x, acc);
+            general_p2a_cell<S, RAY>(b, p, w, x, y, z, bx, by, bz, b.nu + (size_t)i * 19 * 256 + threadIdx.x, acc);
         block_sum8(acc, b.grad_partials + (size_t)i * 8);
     }
 }
@@ -1075,7 +1083,10 @@ static hipError_t adjoint_step_t(const AdjointBuffers& b, const StepParams& p, c
         const uint32_t threads = 2u * 19u * (uint32_t)p.g.ny * (uint32_t)p.g.nz;
         hipLaunchKernelGGL(adjoint_outlet, dim3((threads + 255) / 256), dim3(256), 0, s, b, p);
     }
-    hipLaunchKernelGGL(adjoint_general_p2a<S>, dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    if (p.wall_rule == WALL_INTERPOLATED_LINKS)
+        hipLaunchKernelGGL((adjoint_general_p2a<S, true>), dim3(general_blocks), dim3(256), 0, s, b, p, w);
+    else
+        hipLaunchKernelGGL((adjoint_general_p2a<S, false>), dim3(general_blocks), dim3(256), 0, s, b, p, w);
     hipLaunchKernelGGL(adjoint_general_p2b<S>, dim3(general_blocks), dim3(256), 0, s, b, p, w);
     hipLaunchKernelGGL(reduce_gradient, dim3(1), dim3(256), 0, s, b.grad_partials, b.general_count, b.grad_total);
     return hipGetLastError();
