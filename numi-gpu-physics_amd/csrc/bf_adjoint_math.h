@@ -328,16 +328,27 @@ __host__ __device__ __forceinline__ void reverse(const R (&f)[19], R op, R om, c
         }
         R lfeq_star = R(0);
         if (t.limited && t.qstar >= 0) {
-            // the binding direction's population, increments and equilibrium
+            // the binding direction's population, increments and equilibrium,
+            // gathered by compile-time index (a run-time index into these
+            // register arrays would place them in scratch memory)
             const int qs = t.qstar;
-            const int k = qs == 0 ? -1 : (qs - 1) / 2;
-            const R sgn = (qs != 0 && (qs % 2) == 0) ? R(-1) : R(1);      // q + 1 of a pair: -B, +w- hd
-            const R fq = f[qs];
-            const R feq = qs == 0 ? t.feq0 : t.A[k] + sgn * t.B[k];
-            const R hsk = qs == 0 ? R(0) : fma_(R(0.5), t.sp[k], -t.A[k]);
-            const R hak = qs == 0 ? R(0) : fma_(R(0.5), t.dp[k], -t.B[k]);
-            const R si = qs == 0 ? -op * (f[0] - t.feq0) : -op * hsk;
-            const R ai = qs == 0 ? R(0) : -sgn * om * hak;
+            R fq = f[0], feq = t.feq0, si = -op * (f[0] - t.feq0), ai = R(0);
+            each(Pairs{}, [&](auto kc) {
+                constexpr int k = decltype(kc)::value, q = 2 * k + 1;
+                const R hsk = fma_(R(0.5), t.sp[k], -t.A[k]);
+                const R hak = fma_(R(0.5), t.dp[k], -t.B[k]);
+                if (qs == q) {              // +B, -w- hd
+                    fq = f[q];
+                    feq = t.A[k] + t.B[k];
+                    si = -op * hsk;
+                    ai = -om * hak;
+                } else if (qs == q + 1) {   // -B, +w- hd
+                    fq = f[q + 1];
+                    feq = t.A[k] - t.B[k];
+                    si = -op * hsk;
+                    ai = om * hak;
+                }
+            });
             const bool lin = R(1.0e-6) * max_(feq, R(0)) > R(1.0e-12);
             const R fl = lin ? R(1.0e-6) * feq : R(1.0e-12);
             const R num = fq + ai - fl;
@@ -345,23 +356,26 @@ __host__ __device__ __forceinline__ void reverse(const R (&f)[19], R op, R om, c
             const R ds_dsi = num / (si * si);
             const R lnum = ls * ds_dnum;
             const R lsi = ls * ds_dsi;
-            // num = f* + ai* - floor(feq*)
-            if (qs == 0) lf0 = lf0 + lnum;
-            else direct[qs] = direct[qs] + lnum;
-            if (qs != 0) lhd[k] = lhd[k] + (-sgn * om) * lnum;
             lfeq_star = lin ? -R(1.0e-6) * lnum : R(0);
-            // si* = -w+ (f0 - feq0) or -w+ hs_k
+            // num = f* + ai* - floor(feq*); si* = -w+ (f0 - feq0) or -w+ hs_k
             if (qs == 0) {
+                lf0 = lf0 + lnum;
                 lf0 = lf0 - op * lsi;
                 lfeq0 = lfeq0 + op * lsi;
-            } else {
-                lh[k] = lh[k] - op * lsi;
+                lfeq0 = lfeq0 + lfeq_star;
             }
-            if (qs == 0) lfeq0 = lfeq0 + lfeq_star;
-            else {
-                lA[k] = lA[k] + lfeq_star;
-                lB[k] = lB[k] + sgn * lfeq_star;
-            }
+            each(Pairs{}, [&](auto kc) {
+                constexpr int k = decltype(kc)::value, q = 2 * k + 1;
+                if (qs == q || qs == q + 1) {
+                    const R sgn = qs == q ? R(1) : R(-1);
+                    if (qs == q) direct[q] = direct[q] + lnum;
+                    else direct[q + 1] = direct[q + 1] + lnum;
+                    lhd[k] = lhd[k] + (-sgn * om) * lnum;
+                    lh[k] = lh[k] - op * lsi;
+                    lA[k] = lA[k] + lfeq_star;
+                    lB[k] = lB[k] + sgn * lfeq_star;
+                }
+            });
         }
         // post_0 = f0 + s (-w+)(f0 - feq0)
         lf0 = lf0 - op * lsi0;
